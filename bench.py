@@ -1,0 +1,176 @@
+#!/usr/bin/env python3
+"""Benchmark of the fractional-step core on one MI355X.
+
+Workload (one "step" = one Flow_Solver time step, solvers.c:313-759, -imp 0):
+seeded curvilinear channel 256x192x256 cells (12,582,912), periodic x/z,
+no-slip walls, LES (Cs = 0.1), central second-order convection, RK4 momentum,
+Poisson to a 1e-8 relative residual with FGMRES(20) + FP32 V-cycle, pressure
+update, projection and divergence check, dt = 0.3 dt_CFL1 held fixed.
+
+Prints one JSON line: Mcell-steps/s, the roofline entry of the dominant kernel
+(HIP-event timing of its launches inside the timed region) and the FP64 CPU
+oracle timed on a bounded sample of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import importlib
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+
+# algorithmic bytes per grid point of each timed kernel: every input read
+# once, every output written once (FP64 unless stated)
+ALGO_BYTES = {
+    # ucont 24 + ucat 24 + nu_t 8 + nvert 8 + 3 face metric families (9 + 1 doubles) 240; 18 fluxes out 144
+    "face_fluxes": 24 + 24 + 8 + 8 + 240 + 144,
+    # 18 fluxes 144 + csi/eta/zet 72 + iaj/jaj/kaj 24 + nvert 8 + dP 24 + pUcont 24; Ucont out 24
+    "cell_rhs": 144 + 72 + 24 + 8 + 24 + 24 + 24,
+    # FP32: 19 coefficients 76 + x 4 + b 4 + row mask 1; x out 4
+    "line_jacobi_fp32": 76 + 4 + 4 + 1 + 4,
+    # FP64: 19 coefficients 152 + x 8 + row mask 1; y out 8
+    "poisson_apply_fp64": 152 + 8 + 1 + 8,
+}
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(nx=64, ny=48, nz=64, steps=3):
+    """FP64 CPU oracle (single thread) on the 64x48x64 channel: Mcell-steps/s."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle as po
+
+    cases = importlib.import_module("vfs-wind_amd.cases")
+    c = cases.channel(nx, ny, nz)
+    p = po.make_params(*c.da, per=c.per, bctype=c.bctype, ren=c.ren, dt=1e-3, inlet_flux=c.inlet_flux)
+    o = po.Oracle(p, c.coor)
+    o.set("ucont", c.ucont)
+    o.contra2cart()
+    o.set("ucat_old", o["ucat"])
+    p.dt = 0.3 * o.min_dt()
+    o.mean_k_area()
+    o.fractional_step(tol=1e-8)  # warm-up (first solve starts from zero)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        o.fractional_step(tol=1e-8)
+    dt = time.perf_counter() - t0
+    cells = nx * ny * nz
+    return {"value": cells * steps / dt / 1e6, "unit": "Mcell-steps/s", "cores": 1, "kind": "port",
+            "sample": f"FP64 CPU oracle, channel {nx}x{ny}x{nz}, {steps} full steps (RK4 + FGMRES/MG 1e-8 + projection), "
+                      f"{dt:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--nx", type=int, default=256)
+    ap.add_argument("--ny", type=int, default=192)
+    ap.add_argument("--nz", type=int, default=256)
+    ap.add_argument("--mg-fp64", action="store_true", help="plain FP64 V-cycle instead of FP32")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for {args.gpus})", file=sys.stderr)
+        sys.exit(2)
+
+    vfs = importlib.import_module("vfs-wind_amd")
+    cases = importlib.import_module("vfs-wind_amd.cases")
+    log(f"building case {args.nx}x{args.ny}x{args.nz}")
+    c = cases.channel(args.nx, args.ny, args.nz)
+    params = cases.params_for(vfs, c, dt=1e-3)
+    ctx = vfs.Context(params)
+    ctx.upload("COOR", c.coor)
+    ctx.form_metrics()
+    ctx.mean_k_area()
+    ctx.upload("UCONT", c.ucont)
+    ctx.contra2cart()
+    ctx.upload("UCAT_OLD", ctx.download("UCAT"))
+    params.dt = 0.3 * ctx.min_dt()
+    ctx.set_params(params)
+    del c
+    opts = vfs.default_solve_opts(tol=1e-8, mg_fp32=0 if args.mg_fp64 else 1)
+    log(f"dt = {params.dt:.4e}; warmup {args.warmup} steps")
+    for s in range(args.warmup):
+        st = ctx.fractional_step(opts)
+        log(f"warmup step {s}: poisson its {st.poisson_iterations} relres {st.poisson_relres:.2e} "
+            f"maxdiv {st.maxdiv:.2e} ms {st.ms_total:.1f}")
+    ctx.sync()
+    ctx.profile(True)
+    its, worst = [], 0.0
+    t0 = time.perf_counter()
+    for s in range(args.steps):
+        st = ctx.fractional_step(opts)
+        its.append(st.poisson_iterations)
+        worst = max(worst, st.poisson_relres)
+        log(f"step {s}: poisson its {st.poisson_iterations} relres {st.poisson_relres:.2e} maxdiv {st.maxdiv:.2e} "
+            f"momentum {st.ms_momentum:.1f} ms poisson {st.ms_poisson:.1f} ms")
+    ctx.sync()
+    elapsed = time.perf_counter() - t0
+    cells = args.nx * args.ny * args.nz
+    ms_step = elapsed / args.steps * 1e3
+    value = cells * args.steps / elapsed / 1e6
+
+    # per-kernel timing inside the timed region
+    npts = params.mx * params.my * params.mz
+    kern = {}
+    for name in ALGO_BYTES:
+        ms, cnt = ctx.profile_read(name)
+        if cnt:
+            avg = ms / cnt
+            kern[name] = {"total_ms": ms, "launches": cnt, "avg_ms": avg,
+                          "GBs": ALGO_BYTES[name] * npts / (avg * 1e-3) / 1e9}
+    ctx.profile(False)
+    dom = max(kern, key=lambda k: kern[k]["total_ms"]) if kern else None
+    roofline = None
+    if dom:
+        a = kern[dom]["GBs"]
+        roofline = {"bound": "hbm", "kernel": dom, "achieved": round(a, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": round(a / HBM_PEAK_GBS, 4), "traffic": None,
+                    "bytes_per_point": ALGO_BYTES[dom], "avg_launch_ms": round(kern[dom]["avg_ms"], 4)}
+    log("kernels: " + json.dumps({k: {kk: round(vv, 3) for kk, vv in v.items()} for k, v in kern.items()}))
+    ctx.close()
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        log("cpu baseline (FP64 oracle, bounded sample)")
+        cpu = cpu_baseline()
+
+    out = {
+        "metric": "Mcell-steps/s at 1e-8 Poisson resid; ms/Poisson solve plain MG vs FP32+net",
+        "value": round(value, 3),
+        "unit": "Mcell-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"curvilinear channel {args.nx}x{args.ny}x{args.nz} cells, periodic x/z, no-slip walls, "
+                        f"LES Cs=0.1, central 2nd order, RK4 + Poisson FGMRES(20)/"
+                        f"{'FP64' if args.mg_fp64 else 'FP32'} V-cycle to 1e-8 + projection, dt=0.3 dt_CFL1",
+            "cells": cells,
+            "poisson_iterations_mean": sum(its) / len(its),
+            "poisson_relres_worst": worst,
+        },
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,169 @@
+/*
+ * vfs_wind.h -- C ABI of the MI355X (gfx950) fractional-step core.
+ *
+ * The library keeps one single-rank PETSc-DMDA block per context: every field
+ * is a device-resident "global" array of mz*my*mx points with i fastest
+ * (init.c:157-160), vector fields interleaved {x,y,z} (Cmpnts,
+ * variables.h:44-46).  Cells 1..m-2 are interior, 0 and m-1 ghost cells; in a
+ * DA-periodic direction (ii/jj/kk_periodic) a stencil index outside [0,m)
+ * wraps modulo m, as a one-rank periodic DA ghost exchange does.
+ *
+ * A context plays the role of the reference's UserCtx: the entry points take
+ * it where the reference takes UserCtx* and act on the arrays it owns, so a
+ * host driver uploads its DMDA arrays (VecGetArray) once, calls the same
+ * sequence Flow_Solver calls (solvers.c:313-759) and downloads the results.
+ * All entry points return 0 on success and a nonzero VFS_E* code on failure;
+ * vfs_last_error() then describes the failure.  A context is not thread safe;
+ * all work is ordered on the context's own HIP stream.
+ */
+#ifndef VFS_WIND_H
+#define VFS_WIND_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    VFS_OK = 0,
+    VFS_EINVAL = 1,   /* bad argument or unsupported configuration */
+    VFS_EHIP = 2,     /* HIP runtime error */
+    VFS_ENOMEM = 3,   /* device allocation failed */
+    VFS_ENOCONV = 4   /* iterative solver did not reach its tolerance */
+};
+
+/* Every option a hot-path kernel reads; the reference keeps these in
+ * globals (main.c:11-355, variables.h:908-1348) and in UserCtx. */
+typedef struct {
+    int mx, my, mz;            /* DA sizes: IM+1, JM+1, KM+1 (init.c:157)       */
+    int per_i, per_j, per_k;   /* -ii_periodic, -jj_periodic, -kk_periodic      */
+    int bctype[6];             /* bcs.dat codes: i-, i+, j-, j+, k-, k+         */
+    int les;                   /* -les: 0 or 1 (constant Smagorinsky)          */
+    int second_order;          /* -second_order                                */
+    int laplacian;             /* -laplacian                                   */
+    double ren, dt, st;        /* -ren, -dt, Strouhal (init.c:59-72)           */
+    double cs2;                /* value of lCs (Cs^2) used by les==1           */
+    double inlet_flux;         /* -flux; < 0 when unset                        */
+    double poisson_threshold;  /* poisson_threshold, main.c:124 (0.1)          */
+    double flux_in_sum;        /* FluxInSum of InflowFlux (outlet balance)     */
+} vfs_params;
+
+/* Device-resident arrays of a context. */
+typedef enum {
+    VFS_COOR = 0,   /* node coordinates (Cmpnts)                         */
+    VFS_NVERT,      /* blanking (0 = fluid)                              */
+    VFS_CSI, VFS_ETA, VFS_ZET, VFS_AJ,           /* cell metrics            */
+    VFS_ICSI, VFS_IETA, VFS_IZET, VFS_IAJ,       /* i-face metrics          */
+    VFS_JCSI, VFS_JETA, VFS_JZET, VFS_JAJ,       /* j-face metrics          */
+    VFS_KCSI, VFS_KETA, VFS_KZET, VFS_KAJ,       /* k-face metrics          */
+    VFS_CENT,       /* cell centres                                      */
+    VFS_UCONT,      /* contravariant face fluxes (lUcont)                */
+    VFS_UCAT,       /* Cartesian cell velocity (Ucat)                    */
+    VFS_UCAT_OLD,   /* lUcat_old, the initial field used by slip walls   */
+    VFS_NU_T,       /* eddy viscosity (lNu_t)                            */
+    VFS_P,          /* pressure                                          */
+    VFS_PHI,        /* pressure correction                               */
+    VFS_DP,         /* contravariant pressure gradient (dP)              */
+    VFS_RHS,        /* momentum right-hand side (Rhs)                    */
+    VFS_B,          /* Poisson right-hand side                           */
+    VFS_FEUL,       /* turbine body force on faces (lF_eul)              */
+    VFS_NFIELDS
+} vfs_field;
+
+typedef struct vfs_ctx vfs_ctx;
+
+/* ---- context ---------------------------------------------------------- */
+const char *vfs_last_error(void);
+int vfs_ctx_create(const vfs_params *params, vfs_ctx **out);
+int vfs_ctx_destroy(vfs_ctx *ctx);
+int vfs_set_params(vfs_ctx *ctx, const vfs_params *params);
+int vfs_get_params(const vfs_ctx *ctx, vfs_params *params);
+size_t vfs_field_size(const vfs_ctx *ctx, vfs_field f);      /* doubles */
+int vfs_upload(vfs_ctx *ctx, vfs_field f, const double *host);
+int vfs_download(vfs_ctx *ctx, vfs_field f, double *host);
+double *vfs_field_device_ptr(vfs_ctx *ctx, vfs_field f);
+int vfs_sync(vfs_ctx *ctx);
+
+/* ---- setup (metrics.c, bcs.c) ------------------------------------------ */
+/* FormMetrics (metrics.c:12-1355): all metric fields from VFS_COOR. */
+int vfs_form_metrics(vfs_ctx *ctx);
+/* Calc_Inlet_Area's mean_k_area (bcs.c:137-199); stored in the context. */
+int vfs_mean_k_area(vfs_ctx *ctx, double *area);
+
+/* ---- momentum (rhs.c, momentum.c, les.c) ------------------------------- */
+/* Contra2Cart (rhs.c:35 -> Contra2Cart_2, rhs.c:65-749). */
+int vfs_contra2cart(vfs_ctx *ctx);
+/* FormBCS (bcs.c:1684-2767) for codes 1, 10, 4 and periodic directions. */
+int vfs_form_bcs(vfs_ctx *ctx, int outflow_scale);
+/* Compute_eddy_viscosity_LES (les.c:1143-1361) with lCs = params.cs2. */
+int vfs_eddy_viscosity(vfs_ctx *ctx);
+/* Calc_k_Flux (rhs2.c:1549-1633): mean_k_flux, stored in the context. */
+int vfs_calc_k_flux(vfs_ctx *ctx, double *mean_k_flux);
+/* Pressure_Gradient (momentum.c:203-439): VFS_P -> VFS_DP. */
+int vfs_pressure_gradient(vfs_ctx *ctx);
+/* Formfunction_2 (momentum.c:454-2013): VFS_RHS += scale * F(U). */
+int vfs_formfunction2(vfs_ctx *ctx, double scale);
+/* RungeKutta (rhs.c:1586-1708), -imp 0. */
+int vfs_runge_kutta(vfs_ctx *ctx);
+/* Calc_Minimum_dt (momentum.c:110-201): the CFL-1 time step. */
+int vfs_min_dt(vfs_ctx *ctx, double *dt_cfl1);
+
+/* ---- pressure (poisson.c, poisson_hypre.c, main.c) --------------------- */
+typedef struct {
+    double tol;          /* relative residual ||b-A phi||/||b|| (1e-8)       */
+    int restart;         /* FGMRES restart length (20)                       */
+    int maxit;           /* total iteration cap                              */
+    int levels;          /* multigrid levels (4)                             */
+    int pre, post;       /* smoothing sweeps (2, 2)                          */
+    int coarse_sweeps;   /* sweeps on the coarsest level                     */
+    int mg_fp32;         /* 1: FP32 V-cycle inside the FP64 FGMRES           */
+    int use_guess;       /* 1: start from VFS_PHI; 0: from zero              */
+} vfs_solve_opts;
+
+typedef struct {
+    int iterations;      /* FGMRES iterations (= V-cycles)                   */
+    double relres;       /* final true relative residual                     */
+    double ms;           /* wall time of the solve                           */
+    int fallback;        /* learned stage rejected (see vfs_poisson_solve_learned) */
+} vfs_solve_stats;
+
+/* PoissonRHS2_hypre + Remove_Nullspace_Scale (poisson_hypre.c:118-407): VFS_UCONT -> VFS_B. */
+int vfs_poisson_rhs(vfs_ctx *ctx);
+/* y = A x with the PoissonLHSNew operator (poisson.c:977-1688); device arrays. */
+int vfs_poisson_apply(vfs_ctx *ctx, const double *x_dev, double *y_dev);
+/* Solve A phi = b (PoissonSolver_Hypre's role, poisson_hypre.c:506-636):
+ * VFS_B -> VFS_PHI, plain-mean null-space removal (poisson_hypre.c:608). */
+int vfs_poisson_solve(vfs_ctx *ctx, const vfs_solve_opts *opts, vfs_solve_stats *stats);
+/* UpdatePressure with Convert_Phi2_Phi's ghost handling (poisson.c:98-186, 3137-3297). */
+int vfs_update_pressure(vfs_ctx *ctx);
+/* Projection (poisson.c:2700-3066), including its closing Contra2Cart. */
+int vfs_projection(vfs_ctx *ctx);
+/* Divergence (main.c:936-1074): max |div U| * aj over fluid cells. */
+int vfs_divergence(vfs_ctx *ctx, double *maxdiv);
+
+/* ---- one step ------------------------------------------------------------ */
+typedef struct {
+    int poisson_iterations;
+    double poisson_relres;
+    double maxdiv;
+    double ms_momentum, ms_poisson, ms_total;
+} vfs_step_stats;
+
+/* Flow_Solver (solvers.c:313-759) for -imp 0: k-flux, LES, Pressure_Gradient,
+ * RungeKutta, Poisson, UpdatePressure, Projection, Divergence. */
+int vfs_fractional_step(vfs_ctx *ctx, const vfs_solve_opts *opts, vfs_step_stats *stats);
+
+/* ---- measurement (no counterpart in the reference) -------------------- */
+/* Enable (1) or disable (0) HIP-event timing of the hot kernels' launches on
+ * the context stream and clear earlier records.  For benchmarks only. */
+int vfs_profile(vfs_ctx *ctx, int enable);
+/* Total time and launch count recorded for kernel id 0..7: face_fluxes,
+ * cell_rhs, line_jacobi_fp32, line_jacobi_fp64, poisson_apply_fp64,
+ * contra2cart, pressure_gradient, eddy_viscosity. */
+int vfs_profile_read(vfs_ctx *ctx, int kernel, double *total_ms, long long *count, const char **name);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,150 @@
+/*
+ * vfs_oracle.h -- FP64 CPU restatement of the VFS-Wind fractional-step core.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the parity checker for the HIP
+ * path in vfs-wind_amd/.  Only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py may load it.  Nothing in the product links it.
+ *
+ * Data model (single rank of a PETSc DMDA):
+ *   - every field is a "global" array of mz*my*mx points, i fastest
+ *     (init.c:157-160); vector fields are interleaved {x,y,z} (variables.h:44);
+ *   - cells 1..m-2 are interior, 0 and m-1 are ghost cells (rhs.c:824-845);
+ *   - in a DA-periodic direction (ii/jj/kk_periodic, init.c:145-155) a local
+ *     index outside [0,m) wraps modulo m, which is what a one-rank periodic DA
+ *     ghost exchange delivers; ghost cells 0 and m-1 hold copies of m-2 and 1
+ *     after the reference's "periodic fix" loops (e.g. momentum.c:638-666).
+ */
+#ifndef VFS_ORACLE_H
+#define VFS_ORACLE_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct {
+    int mx, my, mz;                 /* DA sizes = cells + 2 (IM+1, JM+1, KM+1) */
+    int per_i, per_j, per_k;        /* ii_/jj_/kk_periodic (main.c:283-288)    */
+    int bctype[6];                  /* bcs.dat: i-, i+, j-, j+, k-, k+          */
+    int les;                        /* 0 or 1 (constant Smagorinsky)            */
+    int second_order;               /* main.c:1408                              */
+    int laplacian;                  /* main.c:277                               */
+    double ren, dt, st;             /* Reynolds number, time step, Strouhal     */
+    double cs2;                     /* value held in lCs (= Cs^2), les.c:82-85  */
+    double inlet_flux;              /* -flux (main.c:1729), <0 when unset       */
+    double mean_k_area;             /* Calc_Inlet_Area, bcs.c:196-199           */
+    double mean_k_flux;             /* Calc_k_Flux, rhs2.c:1549-1633            */
+    double poisson_threshold;       /* main.c:124, 0.1                          */
+    double flux_in_sum;             /* FluxInSum from InflowFlux (outlet scaling)*/
+} orc_params;
+
+/* Metric set as the momentum/Poisson kernels read it (lCsi ... lKAj after the
+ * periodic fix of metrics.c:1180-1300).  Vector arrays hold 3 doubles/point. */
+typedef struct {
+    double *csi, *eta, *zet, *aj;
+    double *icsi, *ieta, *izet, *iaj;
+    double *jcsi, *jeta, *jzet, *jaj;
+    double *kcsi, *keta, *kzet, *kaj;
+    double *cent;                   /* cell centres (metrics.c:79-100)         */
+    double *nvert;                  /* blanking field, 0 = fluid               */
+    const double *coor;             /* node coordinates the metrics came from;
+                                       the multigrid hierarchy coarsens them  */
+} orc_metrics;
+
+/* a1: FormMetrics (metrics.c:12-1355).  coor: DA node coordinates, 3 doubles
+ * per DA point, nodes 0..IM-1 (etc.) filled, the last DA plane unused. */
+void orc_form_metrics(const orc_params *p, const double *coor, orc_metrics *m);
+
+/* a2: Contra2Cart_2 (rhs.c:65-749), BC codes 1 and 10 plus periodic wrap.
+ * ucont is the local Ucont (periodic fix applied in place); ucat in/out. */
+void orc_contra2cart(const orc_params *p, const orc_metrics *m, double *ucont,
+                     double *ucat, const double *ucat_old);
+
+/* FormBCS subset (bcs.c:1684-2767): codes 1, 10, 4 (outlet), periodic. */
+void orc_form_bcs(const orc_params *p, const orc_metrics *m, double *ucont,
+                  double *ucat, const double *ucat_old, int outflow_scale);
+
+/* a4: Compute_eddy_viscosity_LES (les.c:1143-1361) with constant lCs. */
+void orc_eddy_viscosity(const orc_params *p, const orc_metrics *m,
+                        const double *ucat, double *nu_t);
+
+/* Calc_k_Flux (rhs2.c:1549-1633): mean streamwise flux per k plane. */
+double orc_calc_k_flux(const orc_params *p, const orc_metrics *m, const double *ucont);
+
+/* Calc_Inlet_Area mean_k_area part (bcs.c:137-199). */
+double orc_mean_k_area(const orc_params *p, const orc_metrics *m);
+
+/* a5: Pressure_Gradient (momentum.c:203-439).  pres: periodic fix in place. */
+void orc_pressure_gradient(const orc_params *p, const orc_metrics *m,
+                           double *pres, double *dp);
+
+/* a6: Formfunction_2 (momentum.c:454-2013).  rhs += scale * F(U). */
+void orc_formfunction2(const orc_params *p, const orc_metrics *m, double *ucont,
+                       const double *ucat, const double *nu_t, double *rhs,
+                       double scale);
+
+/* a7: RungeKutta (rhs.c:1586-1708) for -imp 0 with no inflow plane
+ * (InflowFlux only acts on bctype 5 faces).  ucont_o: start-of-step field,
+ * duold out (may be NULL). */
+void orc_runge_kutta(const orc_params *p, const orc_metrics *m, double *ucont,
+                     double *ucat, const double *ucat_old, const double *nu_t,
+                     const double *dp, double *duold);
+
+/* a8: Calc_Minimum_dt (momentum.c:110-201): CFL-1 time step. */
+double orc_min_dt(const orc_params *p, const orc_metrics *m, const double *ucont);
+
+/* a11: PoissonRHS2_hypre + Remove_Nullspace_Scale (poisson_hypre.c:118-407):
+ * b on the full DA grid (0 outside fluid interior cells). */
+void orc_poisson_rhs(const orc_params *p, const orc_metrics *m,
+                     const double *ucont, double *b);
+
+/* a10: the PoissonLHSNew operator (poisson.c:977-1688, poisson==1 rows)
+ * applied matrix-free: y = A*phi on fluid interior cells, 0 elsewhere.
+ * phi: full DA array; periodic ghost cells are read through the wrap. */
+void orc_poisson_apply(const orc_params *p, const orc_metrics *m,
+                       const double *phi, double *y);
+
+/* a12: FP64 FGMRES(restart) + geometric V-cycle solve of A phi = b to the
+ * relative residual tol; phi holds the initial guess on entry.  The solution
+ * has its plain mean removed (poisson_hypre.c:202-263).  Returns iterations,
+ * *relres gets the final true relative residual. */
+int orc_poisson_solve(const orc_params *p, const orc_metrics *m, const double *b,
+                      double *phi, double tol, int restart, int maxit,
+                      int levels, double *relres);
+
+/* Null-space removal of the solution (poisson_hypre.c:202-263). */
+void orc_remove_mean(const orc_params *p, const orc_metrics *m, double *phi);
+
+/* a13: Convert_Phi2_Phi ghost handling + UpdatePressure (poisson.c:98-186,
+ * 3137-3297).  phi: periodic ghosts filled in place; pres += phi. */
+void orc_update_pressure(const orc_params *p, const orc_metrics *m,
+                         double *pres, double *phi);
+
+/* a14: Projection (poisson.c:2700-3066) incl. the closing Contra2Cart. */
+void orc_projection(const orc_params *p, const orc_metrics *m, const double *phi,
+                    double *ucont, double *ucat, const double *ucat_old);
+
+/* a15: Divergence (main.c:936-1074): max |div U * aj| over fluid cells. */
+double orc_divergence(const orc_params *p, const orc_metrics *m, const double *ucont);
+
+/* One RK4 fractional step in Flow_Solver order (solvers.c:313-759), -imp 0,
+ * -poisson with the FP64 solver above.  Returns Poisson iterations. */
+int orc_fractional_step(orc_params *p, const orc_metrics *m, double *ucont,
+                        double *ucat, const double *ucat_old, double *nu_t,
+                        double *pres, double *phi, double *dp, double tol,
+                        double *maxdiv, double *relres);
+
+/* Periodic fix of a field with nc components (e.g. momentum.c:638-666). */
+void orc_periodic_fix(const orc_params *p, double *f, int nc);
+
+/* ALM delta kernels (rotor_model.c:5095-5393). */
+double orc_dfunc_2h(double r);
+double orc_dfunc_s3h(double r);
+double orc_dfunc_s4h(double r);
+double orc_dfunc_exp(double r, double n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,142 @@
+"""Parity of the HIP path against the FP64 CPU oracle, function by function,
+on the seeded channel case (periodic x/z, no-slip walls, curvilinear grid).
+
+Tolerances: stencil kernels 1e-12 normwise relative (max|gpu-cpu|/max|cpu|);
+the Poisson solution is judged by its own residual (<= 1e-8 relative, checked
+with the oracle's operator) and by closeness to the oracle's converged
+solution, which can differ by about cond(A) * 1e-8 since the two Krylov
+histories differ.
+"""
+import numpy as np
+import pytest
+
+from helpers import channel_pair, relerr, vfs
+
+pytestmark = pytest.mark.gpu
+
+RHS_TOL = 1e-12
+
+
+@pytest.fixture(scope="module", params=[(1, 1, 1), (1, 0, 0), (0, 1, 1)], ids=["les-2nd-lap", "les-4th", "nolesi-2nd"])
+def pair(request):
+    les, so, lap = request.param
+    c, o, ctx = channel_pair(24, 16, 20, les=les, second_order=so, laplacian=lap)
+    yield c, o, ctx
+    ctx.close()
+
+
+def test_metrics(pair):
+    c, o, ctx = pair
+    for name in vfs.METRIC_FIELDS:
+        g = ctx.download(name)
+        ref = o.m[name.lower()]
+        assert relerr(g, ref) <= 1e-13, name
+
+
+def test_contra2cart(pair):
+    c, o, ctx = pair
+    assert relerr(ctx.download("UCAT"), o["ucat"]) <= RHS_TOL
+    assert relerr(ctx.download("UCONT"), o["ucont"]) <= RHS_TOL
+
+
+def test_eddy_viscosity(pair):
+    c, o, ctx = pair
+    o.eddy_viscosity()
+    ctx.eddy_viscosity()
+    assert relerr(ctx.download("NU_T"), o["nu_t"]) <= RHS_TOL
+
+
+def test_pressure_gradient(pair):
+    c, o, ctx = pair
+    rng = np.random.default_rng(7)
+    p = rng.standard_normal(o.shape)
+    o.set("p", p)
+    ctx.upload("P", p)
+    o.calc_k_flux()
+    ctx.calc_k_flux()
+    o.pressure_gradient()
+    ctx.pressure_gradient()
+    assert relerr(ctx.download("DP"), o["dp"]) <= RHS_TOL
+    assert relerr(ctx.download("P"), o["p"]) == 0.0
+
+
+def test_formfunction2(pair):
+    c, o, ctx = pair
+    o.eddy_viscosity()
+    ctx.upload("NU_T", o["nu_t"])
+    rng = np.random.default_rng(11)
+    rhs0 = rng.standard_normal(o["rhs"].shape)
+    o.set("rhs", rhs0)
+    ctx.upload("RHS", rhs0)
+    o.formfunction2(0.7)
+    ctx.formfunction2(0.7)
+    assert relerr(ctx.download("RHS"), o["rhs"]) <= RHS_TOL
+
+
+def test_runge_kutta(pair):
+    c, o, ctx = pair
+    o.eddy_viscosity(); ctx.upload("NU_T", o["nu_t"])
+    o.calc_k_flux(); ctx.calc_k_flux()
+    o.pressure_gradient(); ctx.pressure_gradient()
+    o.runge_kutta()
+    ctx.runge_kutta()
+    assert relerr(ctx.download("UCONT"), o["ucont"]) <= RHS_TOL
+    assert relerr(ctx.download("UCAT"), o["ucat"]) <= RHS_TOL
+
+
+def test_poisson_rhs_apply(pair):
+    c, o, ctx = pair
+    o.poisson_rhs()
+    ctx.poisson_rhs()
+    assert relerr(ctx.download("B"), o["b"]) <= RHS_TOL
+    rng = np.random.default_rng(3)
+    x = rng.standard_normal(o.shape)
+    ctx.upload("PHI", x)
+    ctx.poisson_apply_dev(ctx.device_ptr("PHI"), ctx.device_ptr("RHS"))
+    # RHS is a vector field; the operator wrote the first third of it
+    y = ctx.download("RHS").reshape(-1)[: x.size].reshape(o.shape)
+    ref = o.poisson_apply(x)
+    assert relerr(y, ref) <= RHS_TOL
+
+
+@pytest.mark.parametrize("fp32", [1, 0])
+def test_poisson_solve(pair, fp32):
+    c, o, ctx = pair
+    o.poisson_rhs()
+    ctx.poisson_rhs()
+    ctx.upload("PHI", np.zeros(o.shape))
+    st = ctx.poisson_solve(vfs.default_solve_opts(tol=1e-8, mg_fp32=fp32, use_guess=0))
+    assert st.relres <= 1e-8
+    phi = ctx.download("PHI")
+    b = o["b"]
+    r = b - o.poisson_apply(phi)
+    assert np.linalg.norm(r) / np.linalg.norm(b) <= 1.01e-8
+    # both solvers converged hard: the solutions must then agree closely
+    ctx.upload("PHI", np.zeros(o.shape))
+    st = ctx.poisson_solve(vfs.default_solve_opts(tol=1e-11, mg_fp32=fp32, use_guess=0, maxit=400))
+    assert st.relres <= 1e-11
+    o.set("phi", np.zeros(o.shape))
+    it, rel = o.poisson_solve(tol=1e-11)
+    assert rel <= 1e-11
+    assert relerr(ctx.download("PHI"), o["phi"]) <= 1e-6
+
+
+def test_update_projection_divergence(pair):
+    c, o, ctx = pair
+    o.poisson_rhs()
+    o.set("phi", np.zeros(o.shape))
+    o.poisson_solve(tol=1e-10)
+    phi = o["phi"].copy()
+    rng = np.random.default_rng(5)
+    p = rng.standard_normal(o.shape)
+    o.set("p", p); ctx.upload("P", p)
+    ctx.upload("PHI", phi)
+    o.update_pressure(); ctx.update_pressure()
+    assert relerr(ctx.download("P"), o["p"]) <= RHS_TOL
+    assert relerr(ctx.download("PHI"), o["phi"]) == 0.0
+    o.projection(); ctx.projection()
+    assert relerr(ctx.download("UCONT"), o["ucont"]) <= RHS_TOL
+    assert relerr(ctx.download("UCAT"), o["ucat"]) <= RHS_TOL
+    d_cpu = o.divergence()
+    d_gpu = ctx.divergence()
+    assert abs(d_gpu - d_cpu) <= 1e-12 * max(1.0, abs(d_cpu))

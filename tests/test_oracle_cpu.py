@@ -1,0 +1,211 @@
+"""Pins of the FP64 CPU oracle by closed forms, invariants and library
+routines (no GPU).  Each check is chosen so that a plausible restatement
+mistake (a dropped term, a wrong sign or index, a transposed metric) fails.
+"""
+import numpy as np
+import pytest
+
+import pyoracle as po
+from helpers import cases, channel_pair
+
+
+def cartesian(nx, ny, nz, h=(0.1, 0.2, 0.05), per=(0, 0, 0), bctype=(1, 1, 1, 1, 1, 1), **kw):
+    mx, my, mz = nx + 2, ny + 2, nz + 2
+    coor = np.zeros((mz, my, mx, 3))
+    K, J, I = np.meshgrid(np.arange(nz + 1), np.arange(ny + 1), np.arange(nx + 1), indexing="ij")
+    coor[: nz + 1, : ny + 1, : nx + 1, 0] = I * h[0]
+    coor[: nz + 1, : ny + 1, : nx + 1, 1] = J * h[1]
+    coor[: nz + 1, : ny + 1, : nx + 1, 2] = K * h[2]
+    p = po.make_params(mx, my, mz, per=per, bctype=bctype, **kw)
+    return p, po.Oracle(p, coor)
+
+
+def test_metrics_closed_form_cartesian():
+    hx, hy, hz = 0.1, 0.2, 0.05
+    p, o = cartesian(6, 5, 7, (hx, hy, hz))
+    s = (slice(1, -1),) * 3
+    csi, eta, zet = o.m["csi"][s], o.m["eta"][s], o.m["zet"][s]
+    np.testing.assert_allclose(csi[..., 0], hy * hz, rtol=1e-14)
+    np.testing.assert_allclose(eta[..., 1], hx * hz, rtol=1e-14)
+    np.testing.assert_allclose(zet[..., 2], hx * hy, rtol=1e-14)
+    for v, d in ((csi, (1, 2)), (eta, (0, 2)), (zet, (0, 1))):
+        assert np.abs(v[..., list(d)]).max() < 1e-15
+    np.testing.assert_allclose(o.m["aj"][s], 1 / (hx * hy * hz), rtol=1e-12)
+    for f, ax, area in (("icsi", 0, hy * hz), ("jeta", 1, hx * hz), ("kzet", 2, hx * hy)):
+        np.testing.assert_allclose(o.m[f][s][..., ax], area, rtol=1e-12)
+        np.testing.assert_allclose(o.m[f[0] + "aj"][s], 1 / (hx * hy * hz), rtol=1e-12)
+
+
+def test_metrics_gcl_channel():
+    """Discrete geometric conservation: the node-plane area vectors of each
+    interior cell sum to zero (closed cell surface)."""
+    c = cases.channel(16, 12, 16)
+    p = po.make_params(*c.da, per=c.per, bctype=c.bctype)
+    o = po.Oracle(p, c.coor)
+    csi, eta, zet = o.m["csi"], o.m["eta"], o.m["zet"]
+    s = np.s_[2:-2, 2:-2, 2:-2]
+    sm = np.s_[2:-2, 2:-2, 1:-3]
+    net = (csi[s] - csi[sm]) + (eta[s] - eta[2:-2, 1:-3, 2:-2]) + (zet[s] - zet[1:-3, 2:-2, 2:-2])
+    assert np.abs(net).max() < 1e-15
+
+
+def test_contra2cart_recovers_uniform_flow():
+    """Ucont = u . (node-plane area vector) on any grid gives back u exactly."""
+    c = cases.channel(12, 10, 12)
+    p = po.make_params(*c.da, per=c.per, bctype=c.bctype)
+    o = po.Oracle(p, c.coor)
+    u = np.array([0.3, -0.2, 1.1])
+    ucont = np.zeros(o["ucont"].shape)
+    ucont[..., 0] = o.m["csi"] @ u
+    ucont[..., 1] = o.m["eta"] @ u
+    ucont[..., 2] = o.m["zet"] @ u
+    # the cell solve averages faces i-1 and i, which hold the same csi only on
+    # an affine grid; use the Cartesian box for an exact statement
+    p, o = cartesian(5, 4, 6, per=(1, 0, 1), bctype=(100, 100, 1, 1, 100, 100))
+    ucont = np.zeros(o["ucont"].shape)
+    ucont[..., 0] = o.m["csi"] @ u
+    ucont[..., 1] = o.m["eta"] @ u
+    ucont[..., 2] = o.m["zet"] @ u
+    o.set("ucont", ucont)
+    o.contra2cart()
+    np.testing.assert_allclose(o["ucat"][1:-1, 1:-1, 1:-1], np.broadcast_to(u, o["ucat"][1:-1, 1:-1, 1:-1].shape),
+                               rtol=1e-13, atol=1e-15)
+    # no-slip ghost rows mirror the first interior cell with a sign flip
+    np.testing.assert_allclose(o["ucat"][1:-1, 0, 1:-1], -o["ucat"][1:-1, 1, 1:-1], rtol=0, atol=0)
+
+
+def test_eddy_viscosity_linear_field():
+    """u = A x on a Cartesian grid: central differences are exact, so
+    nu_t = Cs^2 * V^(2/3) * sqrt(2 S:S) with S = sym(A)."""
+    hx, hy, hz = 0.1, 0.2, 0.05
+    p, o = cartesian(6, 6, 6, (hx, hy, hz), les=1, cs2=0.01)
+    A = np.array([[0.3, 0.1, -0.2], [0.05, -0.4, 0.2], [0.7, 0.0, 0.1]])
+    cent = o.m["cent"]
+    ucat = np.einsum("ab,kjib->kjia", A, cent)
+    # ghost cells: extrapolate the linear field so one-sided and central agree
+    X = np.zeros(cent.shape)
+    K, J, I = np.meshgrid(np.arange(o.shape[0]), np.arange(o.shape[1]), np.arange(o.shape[2]), indexing="ij")
+    X[..., 0], X[..., 1], X[..., 2] = (I - 0.5) * hx, (J - 0.5) * hy, (K - 0.5) * hz
+    ucat = np.einsum("ab,kjib->kjia", A, X)
+    o.set("ucat", ucat)
+    o.eddy_viscosity()
+    S = 0.5 * (A + A.T)
+    expect = 0.01 * (hx * hy * hz) ** (2 / 3) * np.sqrt(2 * np.sum(S * S))
+    np.testing.assert_allclose(o["nu_t"][1:-1, 1:-1, 1:-1], expect, rtol=1e-12)
+
+
+def test_formfunction_uniform_flow_is_steady():
+    """A uniform flow on the periodic Cartesian box has zero convective and
+    viscous divergence: Formfunction_2 returns zero (to round-off)."""
+    p, o = cartesian(6, 5, 6, per=(1, 1, 1), bctype=(100,) * 6, les=1, cs2=0.01)
+    u = np.array([0.3, -0.2, 1.1])
+    ucont = np.zeros(o["ucont"].shape)
+    ucont[..., 0] = o.m["csi"] @ u
+    ucont[..., 1] = o.m["eta"] @ u
+    ucont[..., 2] = o.m["zet"] @ u
+    o.set("ucont", ucont)
+    o.contra2cart()
+    o.eddy_viscosity()
+    for so in (1, 0):
+        p.second_order = so
+        o.set("rhs", np.zeros(o["rhs"].shape))
+        o.formfunction2(1.0)
+        assert np.abs(o["rhs"]).max() < 1e-14
+
+
+def test_formfunction_conservation_periodic_box():
+    """Flux form: on a fully periodic Cartesian box every face flux enters two
+    cells with opposite signs, so the RHS summed over all cells vanishes for
+    any velocity field (convective + viscous, both schemes)."""
+    p, o = cartesian(6, 5, 7, per=(1, 1, 1), bctype=(100,) * 6, les=1, cs2=0.01, ren=50.0)
+    rng = np.random.default_rng(1)
+    ucont = rng.standard_normal(o["ucont"].shape)
+    for so in (1, 0):
+        p.second_order = so
+        o.set("ucont", ucont)
+        o.contra2cart()
+        o.eddy_viscosity()
+        o.set("rhs", np.zeros(o["rhs"].shape))
+        o.formfunction2(1.0)
+        r = o["rhs"][1:-1, 1:-1, 1:-1]
+        for comp in range(3):
+            assert abs(r[..., comp].sum()) < 1e-12 * np.abs(r[..., comp]).sum()
+
+
+def test_poisson_operator_reduces_to_7_point():
+    """On a Cartesian grid the 19-point operator is the 7-point Laplacian
+    with face coefficients area/spacing, Neumann at walls."""
+    hx, hy, hz = 0.1, 0.2, 0.05
+    p, o = cartesian(5, 4, 6, (hx, hy, hz), per=(1, 0, 1), bctype=(100, 100, 1, 1, 100, 100))
+    rng = np.random.default_rng(0)
+    x = np.zeros(o.shape)
+    x[1:-1, 1:-1, 1:-1] = rng.standard_normal((6, 4, 5))
+    y = o.poisson_apply(x)
+    xi = x[1:-1, 1:-1, 1:-1]
+    gx, gy, gz = hy * hz / hx, hx * hz / hy, hx * hy / hz
+    lap = gx * (np.roll(xi, -1, 2) - 2 * xi + np.roll(xi, 1, 2)) + gz * (np.roll(xi, -1, 0) - 2 * xi + np.roll(xi, 1, 0))
+    yy = np.zeros_like(xi)
+    yy[:, 1:, :] += gy * (xi[:, :-1, :] - xi[:, 1:, :])
+    yy[:, :-1, :] += gy * (xi[:, 1:, :] - xi[:, :-1, :])
+    np.testing.assert_allclose(y[1:-1, 1:-1, 1:-1], lap + yy, rtol=1e-12, atol=1e-13)
+
+
+def test_poisson_constant_nullspace_and_solution():
+    """A.1 = 0 on the curvilinear channel, the RHS is made consistent (sum b = 0),
+    and on a tiny grid the solver agrees with a dense least-squares solve."""
+    c = cases.channel(6, 4, 6)
+    p = po.make_params(*c.da, per=c.per, bctype=c.bctype, dt=0.01)
+    o = po.Oracle(p, c.coor)
+    one = np.zeros(o.shape)
+    one[1:-1, 1:-1, 1:-1] = 1.0
+    assert np.abs(o.poisson_apply(one)).max() < 1e-13
+    o.set("ucont", c.ucont)
+    o.poisson_rhs()
+    b = o["b"]
+    assert abs(b.sum()) < 1e-12 * np.abs(b).sum()
+    # dense matrix by probing the operator with unit vectors
+    idx = [(k, j, i) for k in range(1, o.shape[0] - 1) for j in range(1, o.shape[1] - 1) for i in range(1, o.shape[2] - 1)]
+    n = len(idx)
+    A = np.zeros((n, n))
+    for col, (k, j, i) in enumerate(idx):
+        e = np.zeros(o.shape)
+        e[k, j, i] = 1.0
+        y = o.poisson_apply(e)
+        A[:, col] = [y[q] for q in idx]
+    bb = np.array([b[q] for q in idx])
+    xs = np.linalg.lstsq(A, bb, rcond=None)[0]
+    xs -= xs.mean()
+    o.set("phi", np.zeros(o.shape))
+    it, rel = o.poisson_solve(tol=1e-12, levels=1)
+    assert rel <= 1e-12
+    phi = np.array([o["phi"][q] for q in idx])
+    np.testing.assert_allclose(phi, xs, rtol=0, atol=1e-9 * np.abs(xs).max())
+
+
+def test_projection_divergence_matches_residual():
+    """After the solve and the projection, max|div U| * aj is the residual
+    of the Poisson equation times dt: it falls with the solver tolerance."""
+    c, o, _ = channel_pair(16, 12, 16, with_gpu=False)
+    o.calc_k_flux()
+    o.pressure_gradient()
+    o.runge_kutta()
+    d0 = o.divergence()
+    o.poisson_rhs()
+    o.set("phi", np.zeros(o.shape))
+    o.poisson_solve(tol=1e-12)
+    o.update_pressure()
+    o.projection()
+    d1 = o.divergence()
+    assert d1 < 1e-9 * d0 and d1 < 1e-10
+
+
+def test_dfuncs_partition_of_unity():
+    lib = po.load()
+    for f in (lib.orc_dfunc_2h, lib.orc_dfunc_s3h, lib.orc_dfunc_s4h):
+        for r in np.linspace(-0.5, 0.5, 11):
+            s = sum(f(r - j) for j in range(-4, 5))
+            assert abs(s - 1.0) < 5e-8
+    # Gaussian kernel integrates to one (trapezoid on a fine grid)
+    r = np.linspace(-20, 20, 40001)
+    v = np.array([lib.orc_dfunc_exp(x, 2.0) for x in r])
+    assert abs(np.trapezoid(v, r) - 1.0) < 1e-7

@@ -1,0 +1,292 @@
+"""vfs-wind_amd: MI355X (gfx950) fractional-step core of VFS-Wind.
+
+Python binding of the C ABI in include/vfs_wind.h (libvfswind.so, built by
+build_lib.py).  The binding only moves host arrays in and out of a device
+context and calls the ABI; every computation runs in the HIP library.  It
+raises at import time if the library is missing: there is no CPU fallback.
+
+The package directory name contains a hyphen, so import it with
+``importlib.import_module("vfs-wind_amd")``.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+import numpy as np
+
+PKG_DIR = Path(__file__).resolve().parent
+LIB_PATH = PKG_DIR / "libvfswind.so"
+
+VFS_OK, VFS_EINVAL, VFS_EHIP, VFS_ENOMEM, VFS_ENOCONV = 0, 1, 2, 3, 4
+
+# vfs_field enumeration (include/vfs_wind.h)
+FIELDS = [
+    "COOR", "NVERT", "CSI", "ETA", "ZET", "AJ", "ICSI", "IETA", "IZET", "IAJ",
+    "JCSI", "JETA", "JZET", "JAJ", "KCSI", "KETA", "KZET", "KAJ", "CENT",
+    "UCONT", "UCAT", "UCAT_OLD", "NU_T", "P", "PHI", "DP", "RHS", "B", "FEUL",
+]
+FIELD = {name: i for i, name in enumerate(FIELDS)}
+VECTOR_FIELDS = {"COOR", "CSI", "ETA", "ZET", "ICSI", "IETA", "IZET", "JCSI", "JETA", "JZET",
+                 "KCSI", "KETA", "KZET", "CENT", "UCONT", "UCAT", "UCAT_OLD", "DP", "RHS", "FEUL"}
+METRIC_FIELDS = ["CSI", "ETA", "ZET", "AJ", "ICSI", "IETA", "IZET", "IAJ", "JCSI", "JETA", "JZET",
+                 "JAJ", "KCSI", "KETA", "KZET", "KAJ", "CENT"]
+
+
+class Params(C.Structure):
+    """vfs_params: every option a hot-path kernel reads."""
+
+    _fields_ = [
+        ("mx", C.c_int), ("my", C.c_int), ("mz", C.c_int),
+        ("per_i", C.c_int), ("per_j", C.c_int), ("per_k", C.c_int),
+        ("bctype", C.c_int * 6),
+        ("les", C.c_int), ("second_order", C.c_int), ("laplacian", C.c_int),
+        ("ren", C.c_double), ("dt", C.c_double), ("st", C.c_double),
+        ("cs2", C.c_double), ("inlet_flux", C.c_double),
+        ("poisson_threshold", C.c_double), ("flux_in_sum", C.c_double),
+    ]
+
+
+class SolveOpts(C.Structure):
+    _fields_ = [
+        ("tol", C.c_double), ("restart", C.c_int), ("maxit", C.c_int), ("levels", C.c_int),
+        ("pre", C.c_int), ("post", C.c_int), ("coarse_sweeps", C.c_int),
+        ("mg_fp32", C.c_int), ("use_guess", C.c_int),
+    ]
+
+
+class SolveStats(C.Structure):
+    _fields_ = [("iterations", C.c_int), ("relres", C.c_double), ("ms", C.c_double), ("fallback", C.c_int)]
+
+
+class StepStats(C.Structure):
+    _fields_ = [
+        ("poisson_iterations", C.c_int), ("poisson_relres", C.c_double), ("maxdiv", C.c_double),
+        ("ms_momentum", C.c_double), ("ms_poisson", C.c_double), ("ms_total", C.c_double),
+    ]
+
+
+# entry points and their argument types
+_SIGS = {
+    "vfs_last_error": ([], C.c_char_p),
+    "vfs_ctx_create": ([C.POINTER(Params), C.POINTER(C.c_void_p)], C.c_int),
+    "vfs_ctx_destroy": ([C.c_void_p], C.c_int),
+    "vfs_set_params": ([C.c_void_p, C.POINTER(Params)], C.c_int),
+    "vfs_get_params": ([C.c_void_p, C.POINTER(Params)], C.c_int),
+    "vfs_field_size": ([C.c_void_p, C.c_int], C.c_size_t),
+    "vfs_upload": ([C.c_void_p, C.c_int, C.c_void_p], C.c_int),
+    "vfs_download": ([C.c_void_p, C.c_int, C.c_void_p], C.c_int),
+    "vfs_field_device_ptr": ([C.c_void_p, C.c_int], C.c_void_p),
+    "vfs_sync": ([C.c_void_p], C.c_int),
+    "vfs_form_metrics": ([C.c_void_p], C.c_int),
+    "vfs_mean_k_area": ([C.c_void_p, C.POINTER(C.c_double)], C.c_int),
+    "vfs_contra2cart": ([C.c_void_p], C.c_int),
+    "vfs_form_bcs": ([C.c_void_p, C.c_int], C.c_int),
+    "vfs_eddy_viscosity": ([C.c_void_p], C.c_int),
+    "vfs_calc_k_flux": ([C.c_void_p, C.POINTER(C.c_double)], C.c_int),
+    "vfs_pressure_gradient": ([C.c_void_p], C.c_int),
+    "vfs_formfunction2": ([C.c_void_p, C.c_double], C.c_int),
+    "vfs_runge_kutta": ([C.c_void_p], C.c_int),
+    "vfs_min_dt": ([C.c_void_p, C.POINTER(C.c_double)], C.c_int),
+    "vfs_poisson_rhs": ([C.c_void_p], C.c_int),
+    "vfs_poisson_apply": ([C.c_void_p, C.c_void_p, C.c_void_p], C.c_int),
+    "vfs_poisson_solve": ([C.c_void_p, C.POINTER(SolveOpts), C.POINTER(SolveStats)], C.c_int),
+    "vfs_update_pressure": ([C.c_void_p], C.c_int),
+    "vfs_projection": ([C.c_void_p], C.c_int),
+    "vfs_divergence": ([C.c_void_p, C.POINTER(C.c_double)], C.c_int),
+    "vfs_fractional_step": ([C.c_void_p, C.POINTER(SolveOpts), C.POINTER(StepStats)], C.c_int),
+    "vfs_profile": ([C.c_void_p, C.c_int], C.c_int),
+    "vfs_profile_read": ([C.c_void_p, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_longlong),
+                          C.POINTER(C.c_char_p)], C.c_int),
+}
+
+KERNEL_IDS = {"face_fluxes": 0, "cell_rhs": 1, "line_jacobi_fp32": 2, "line_jacobi_fp64": 3,
+              "poisson_apply_fp64": 4, "contra2cart": 5, "pressure_gradient": 6, "eddy_viscosity": 7}
+
+
+class VFSError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"vfs error {code}: {msg}")
+        self.code = code
+
+
+_lib = None
+
+
+def load_library(path: str | os.PathLike | None = None) -> C.CDLL:
+    """Load libvfswind.so; raises if it was not built."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = Path(path) if path else LIB_PATH
+    if not p.exists():
+        raise ImportError(f"{p} is missing: build it with vfs-wind_amd/build_lib.py (no CPU fallback exists)")
+    lib = C.CDLL(str(p))
+    for name, (args, res) in _SIGS.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = res
+    if path is None:
+        _lib = lib
+    return lib
+
+
+def exported_symbols() -> list[str]:
+    return list(_SIGS)
+
+
+def make_params(mx, my, mz, *, per=(1, 0, 1), bctype=(100, 100, 1, 1, 100, 100), les=1, second_order=1,
+                laplacian=1, ren=2800.0, dt=1e-3, st=1.0, cs2=0.01, inlet_flux=-1.0,
+                poisson_threshold=0.1, flux_in_sum=0.0) -> Params:
+    p = Params()
+    p.mx, p.my, p.mz = mx, my, mz
+    p.per_i, p.per_j, p.per_k = per
+    for q in range(6):
+        p.bctype[q] = int(bctype[q])
+    p.les, p.second_order, p.laplacian = les, second_order, laplacian
+    p.ren, p.dt, p.st, p.cs2 = ren, dt, st, cs2
+    p.inlet_flux, p.poisson_threshold, p.flux_in_sum = inlet_flux, poisson_threshold, flux_in_sum
+    return p
+
+
+def default_solve_opts(tol=1e-8, restart=20, maxit=200, levels=4, pre=2, post=2, coarse_sweeps=20, mg_fp32=1,
+                       use_guess=1) -> SolveOpts:
+    o = SolveOpts()
+    o.tol, o.restart, o.maxit, o.levels = tol, restart, maxit, levels
+    o.pre, o.post, o.coarse_sweeps, o.mg_fp32, o.use_guess = pre, post, coarse_sweeps, mg_fp32, use_guess
+    return o
+
+
+class Context:
+    """A device-resident DMDA block (the reference's UserCtx for the hot path)."""
+
+    def __init__(self, params: Params, lib: C.CDLL | None = None):
+        self.lib = lib or load_library()
+        self.params = params
+        h = C.c_void_p()
+        self._check(self.lib.vfs_ctx_create(C.byref(params), C.byref(h)))
+        self.h = h
+        self.shape = (params.mz, params.my, params.mx)
+
+    # -- plumbing --------------------------------------------------------------
+    def _check(self, rc: int):
+        if rc != VFS_OK:
+            raise VFSError(rc, self.lib.vfs_last_error().decode())
+        return rc
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.vfs_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def field_shape(self, name: str):
+        return self.shape + ((3,) if name in VECTOR_FIELDS else ())
+
+    def upload(self, name: str, arr: np.ndarray):
+        a = np.ascontiguousarray(arr, dtype=np.float64)
+        if a.shape != self.field_shape(name):
+            raise ValueError(f"{name}: expected shape {self.field_shape(name)}, got {a.shape}")
+        self._check(self.lib.vfs_upload(self.h, FIELD[name], a.ctypes.data))
+
+    def download(self, name: str) -> np.ndarray:
+        a = np.empty(self.field_shape(name), dtype=np.float64)
+        self._check(self.lib.vfs_download(self.h, FIELD[name], a.ctypes.data))
+        return a
+
+    def device_ptr(self, name: str) -> int:
+        return int(self.lib.vfs_field_device_ptr(self.h, FIELD[name]) or 0)
+
+    def set_params(self, params: Params):
+        self._check(self.lib.vfs_set_params(self.h, C.byref(params)))
+        self.params = params
+
+    def sync(self):
+        self._check(self.lib.vfs_sync(self.h))
+
+    # -- reference entry points ---------------------------------------------
+    def form_metrics(self):
+        self._check(self.lib.vfs_form_metrics(self.h))
+
+    def mean_k_area(self) -> float:
+        v = C.c_double()
+        self._check(self.lib.vfs_mean_k_area(self.h, C.byref(v)))
+        return v.value
+
+    def contra2cart(self):
+        self._check(self.lib.vfs_contra2cart(self.h))
+
+    def form_bcs(self, outflow_scale: int = 0):
+        self._check(self.lib.vfs_form_bcs(self.h, int(outflow_scale)))
+
+    def eddy_viscosity(self):
+        self._check(self.lib.vfs_eddy_viscosity(self.h))
+
+    def calc_k_flux(self) -> float:
+        v = C.c_double()
+        self._check(self.lib.vfs_calc_k_flux(self.h, C.byref(v)))
+        return v.value
+
+    def pressure_gradient(self):
+        self._check(self.lib.vfs_pressure_gradient(self.h))
+
+    def formfunction2(self, scale: float = 1.0):
+        self._check(self.lib.vfs_formfunction2(self.h, C.c_double(scale)))
+
+    def runge_kutta(self):
+        self._check(self.lib.vfs_runge_kutta(self.h))
+
+    def min_dt(self) -> float:
+        v = C.c_double()
+        self._check(self.lib.vfs_min_dt(self.h, C.byref(v)))
+        return v.value
+
+    def poisson_rhs(self):
+        self._check(self.lib.vfs_poisson_rhs(self.h))
+
+    def poisson_apply_dev(self, x_ptr: int, y_ptr: int):
+        self._check(self.lib.vfs_poisson_apply(self.h, C.c_void_p(x_ptr), C.c_void_p(y_ptr)))
+
+    def poisson_solve(self, opts: SolveOpts | None = None, check: bool = True) -> SolveStats:
+        st = SolveStats()
+        rc = self.lib.vfs_poisson_solve(self.h, C.byref(opts or default_solve_opts()), C.byref(st))
+        if check or rc != VFS_ENOCONV:
+            self._check(rc)
+        return st
+
+    def update_pressure(self):
+        self._check(self.lib.vfs_update_pressure(self.h))
+
+    def projection(self):
+        self._check(self.lib.vfs_projection(self.h))
+
+    def divergence(self) -> float:
+        v = C.c_double()
+        self._check(self.lib.vfs_divergence(self.h, C.byref(v)))
+        return v.value
+
+    def profile(self, enable: bool):
+        self._check(self.lib.vfs_profile(self.h, int(enable)))
+
+    def profile_read(self, kernel: str):
+        """(total ms, launch count) recorded for a kernel since profile(True)."""
+        ms, cnt, name = C.c_double(), C.c_longlong(), C.c_char_p()
+        self._check(self.lib.vfs_profile_read(self.h, KERNEL_IDS[kernel], C.byref(ms), C.byref(cnt), C.byref(name)))
+        return ms.value, cnt.value
+
+    def fractional_step(self, opts: SolveOpts | None = None) -> StepStats:
+        st = StepStats()
+        self._check(self.lib.vfs_fractional_step(self.h, C.byref(opts or default_solve_opts()), C.byref(st)))
+        return st

@@ -1,0 +1,128 @@
+"""Seeded synthetic inputs of the benchmark configurations.
+
+Channel (periodic x/z, no-slip walls in y): the reference's channel deck
+(Instructional_Cases/Test_10_ChannelFlow_Retau3000/control.dat: -ii_periodic 1
+-kk_periodic 1, bcs.dat "100 100 1 10 100 100") with both j faces no-slip,
+x in [0, 2pi] (i), y in [-1, 1] (j), z in [0, pi] (k, streamwise).  Wall
+stretching y = tanh(beta(2 eta - 1))/tanh(beta), beta = 2; every node moves by
+0.02 (1 - y^2) sin(2 pi x/Lx) sin(2 pi z/Lz) times the local spacing along
+each axis, so the metrics are non-orthogonal while the grid stays periodic.
+Velocity: laminar 1.5(1 - y^2) along z plus 10% seeded uniform noise.
+
+Arrays use the DMDA layout: (mz, my, mx[, 3]), i fastest; node coordinates
+occupy DA indices 0..N (the last DA plane is unused, init.c:300-394).
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+import numpy as np
+
+SEED = 20261202
+
+
+@dataclass
+class Case:
+    name: str
+    cells: tuple  # (nx, ny, nz)
+    per: tuple    # (per_i, per_j, per_k)
+    bctype: tuple
+    ren: float
+    coor: np.ndarray      # (mz, my, mx, 3) node coordinates
+    ucont: np.ndarray     # (mz, my, mx, 3) initial contravariant fluxes
+    inlet_flux: float
+
+    @property
+    def da(self):
+        nx, ny, nz = self.cells
+        return nx + 2, ny + 2, nz + 2
+
+
+def _face_areas(coor: np.ndarray):
+    """Area vectors of the i, j and k node planes (the cross products of the
+    face diagonals, as the node-plane metrics of metrics.c:116-185)."""
+    X = coor
+    # i faces between cells i and i+1 at node plane i: corners (k,j),(k-1,j),(k,j-1),(k-1,j-1)
+    a = X[1:, 1:, :, :]
+    b = X[:-1, 1:, :, :]
+    c = X[1:, :-1, :, :]
+    d = X[:-1, :-1, :, :]
+    de = 0.5 * (a + b - c - d)          # d/d eta
+    dz = 0.5 * (c + a - d - b)          # d/d zeta
+    ai = np.cross(de, dz)
+    # j faces at node plane j: corners (k,i),(k-1,i),(k,i-1),(k-1,i-1)
+    a = X[1:, :, 1:, :]
+    b = X[:-1, :, 1:, :]
+    c = X[1:, :, :-1, :]
+    d = X[:-1, :, :-1, :]
+    dc = 0.5 * (a + b - c - d)
+    dz = 0.5 * (a + c - b - d)
+    aj = np.cross(dz, dc)
+    # k faces at node plane k: corners (j,i),(j-1,i),(j,i-1),(j-1,i-1)
+    a = X[:, 1:, 1:, :]
+    b = X[:, :-1, 1:, :]
+    c = X[:, 1:, :-1, :]
+    d = X[:, :-1, :-1, :]
+    dc = 0.5 * (a + b - c - d)
+    de = 0.5 * (a + c - b - d)
+    ak = np.cross(dc, de)
+    return ai, aj, ak
+
+
+def channel(nx: int, ny: int, nz: int, seed: int = SEED, beta: float = 2.0, bump: float = 0.02,
+            noise: float = 0.1, ren: float = 2800.0) -> Case:
+    IM, JM, KM = nx + 1, ny + 1, nz + 1
+    mx, my, mz = nx + 2, ny + 2, nz + 2
+    Lx, Lz = 2 * np.pi, np.pi
+    xi = np.linspace(0.0, Lx, IM)
+    eta = np.arange(JM) / (JM - 1)
+    yj = np.tanh(beta * (2 * eta - 1)) / np.tanh(beta)
+    zk = np.linspace(0.0, Lz, KM)
+    dx, dz = Lx / nx, Lz / nz
+    dy = np.gradient(yj)
+    Z, Y, X = np.meshgrid(zk, yj, xi, indexing="ij")
+    DY = np.broadcast_to(dy[None, :, None], X.shape)
+    amp = bump * (1 - Y ** 2) * np.sin(2 * np.pi * X / Lx) * np.sin(2 * np.pi * Z / Lz)
+    coor = np.zeros((mz, my, mx, 3))
+    coor[:KM, :JM, :IM, 0] = X + amp * dx
+    coor[:KM, :JM, :IM, 1] = Y + amp * DY
+    coor[:KM, :JM, :IM, 2] = Z + amp * dz
+
+    # cell-centre velocity of interior cells
+    rng = np.random.Generator(np.random.PCG64(seed))
+    nodes = coor[:KM, :JM, :IM]
+    cent = 0.125 * (nodes[1:, 1:, 1:] + nodes[1:, 1:, :-1] + nodes[1:, :-1, 1:] + nodes[1:, :-1, :-1] +
+                    nodes[:-1, 1:, 1:] + nodes[:-1, 1:, :-1] + nodes[:-1, :-1, 1:] + nodes[:-1, :-1, :-1])
+    yc = cent[..., 1]
+    u = noise * rng.uniform(-1.0, 1.0, size=cent.shape)
+    u[..., 2] += 1.5 * (1 - yc ** 2)
+    ucat = np.zeros((mz, my, mx, 3))
+    ucat[1:-1, 1:-1, 1:-1] = u
+    # periodic images in i and k, zero-velocity walls in j
+    ucat[:, :, 0] = ucat[:, :, mx - 2]
+    ucat[:, :, mx - 1] = ucat[:, :, 1]
+    ucat[0] = ucat[mz - 2]
+    ucat[mz - 1] = ucat[1]
+
+    ai, aj, ak = _face_areas(coor[:KM, :JM, :IM])
+    ucont = np.zeros((mz, my, mx, 3))
+    # i faces i = 0..mx-2 between cells i and i+1 (node plane i), k, j interior
+    for_i = 0.5 * (ucat[1:-1, 1:-1, :-1] + ucat[1:-1, 1:-1, 1:])
+    ucont[1:-1, 1:-1, :-1, 0] = np.einsum("kjic,kjic->kji", for_i, ai[:, :, :IM])
+    # j faces j = 1..my-3 (walls at 0 and my-2 carry no flux)
+    for_j = 0.5 * (ucat[1:-1, 1:-2, 1:-1] + ucat[1:-1, 2:-1, 1:-1])
+    ucont[1:-1, 1:-2, 1:-1, 1] = np.einsum("kjic,kjic->kji", for_j, aj[:, 1:JM - 1, :])
+    # k faces k = 0..mz-2
+    for_k = 0.5 * (ucat[:-1, 1:-1, 1:-1] + ucat[1:, 1:-1, 1:-1])
+    ucont[:-1, 1:-1, 1:-1, 2] = np.einsum("kjic,kjic->kji", for_k, ak[:KM, :, :])
+    inlet_flux = float(ucont[1:mz - 1, 1:my - 1, 1:mx - 1, 2].sum() / (mz - 2))
+    return Case(name=f"channel{nx}x{ny}x{nz}", cells=(nx, ny, nz), per=(1, 0, 1),
+                bctype=(100, 100, 1, 1, 100, 100), ren=ren, coor=coor, ucont=ucont, inlet_flux=inlet_flux)
+
+
+def params_for(mod, case: Case, dt: float, les: int = 1, second_order: int = 1, laplacian: int = 1,
+               cs2: float = 0.01):
+    """vfs_params of a case (mod: the vfs-wind_amd module or the oracle binding)."""
+    mx, my, mz = case.da
+    return mod.make_params(mx, my, mz, per=case.per, bctype=case.bctype, les=les, second_order=second_order,
+                           laplacian=laplacian, ren=case.ren, dt=dt, st=1.0, cs2=cs2, inlet_flux=case.inlet_flux)

@@ -1,0 +1,417 @@
+// formfunction.hip -- Formfunction_2 (momentum.c:454-2013) for gfx950.
+//
+// Two passes over the DMDA block:
+//   face pass: one thread per point computes the convective and viscous
+//     fluxes of the i, j and k faces it owns (momentum.c:668-1451) and
+//     writes them as 18 structure-of-arrays planes (coalesced stores);
+//   cell pass: one thread per point forms the flux divergence Fp of its cell
+//     and of the three upper neighbours (momentum.c:1548-1678), projects
+//     them on the contravariant directions (momentum.c:1715-1841), applies
+//     the boundary zeroing (momentum.c:1833-1938) and either accumulates into
+//     Rhs or, inside RungeKutta, writes Ucont = a*(Rhs - dP) + pUcont
+//     directly (rhs.c:1647-1658), so Rhs never round-trips through HBM.
+// The arithmetic follows the reference's operation order, with FP
+// contraction disabled for this file, so results match the FP64 CPU path to
+// a few ulps.
+#include <hip/hip_runtime.h>
+
+#include "vfs_internal.h"
+
+#pragma clang fp contract(off)
+
+namespace vfs {
+
+constexpr int FF_TPB = 256;
+
+struct Metrics9 {
+    const double *csi, *eta, *zet, *aj;  // one face family (I, J or K)
+};
+
+struct FaceArgs {
+    Grid g;
+    const double *ucont, *ucat, *nut, *nvert;
+    Metrics9 fi, fj, fk;
+    double nu;
+    int les, second_order, laplacian;
+    double *out;  // 18 planes of n doubles: div1,div2,div3 (xyz each), visc1..3
+};
+
+__device__ inline double Uc(const FaceArgs &a, int c, int k, int j, int i) { return a.ucat[3 * gidx(a.g, k, j, i) + c]; }
+__device__ inline double NVd(const FaceArgs &a, int k, int j, int i) { return a.nvert[gidx(a.g, k, j, i)]; }
+
+// viscous face flux, momentum.c:682-696 and 863-902 (and the j/k twins)
+__device__ inline void visc_face(const double cs[3], const double et[3], const double ze[3], const double nrm[3],
+                                 double ajc, const double d[3][3], double nu, double nu_t, int les, int laplacian,
+                                 double out[3])
+{
+    double g11 = cs[0] * nrm[0] + cs[1] * nrm[1] + cs[2] * nrm[2];
+    double g21 = et[0] * nrm[0] + et[1] * nrm[1] + et[2] * nrm[2];
+    double g31 = ze[0] * nrm[0] + ze[1] * nrm[1] + ze[2] * nrm[2];
+    double r[3][3];
+    for (int c = 0; c < 3; c++)
+        for (int q = 0; q < 3; q++) r[c][q] = d[c][0] * cs[q] + d[c][1] * et[q] + d[c][2] * ze[q];
+    for (int c = 0; c < 3; c++) out[c] = 0;
+    if (les)
+        for (int c = 0; c < 3; c++)
+            out[c] = (g11 * d[c][0] + g21 * d[c][1] + g31 * d[c][2] + r[0][c] * nrm[0] + r[1][c] * nrm[1] +
+                      r[2][c] * nrm[2]) * ajc * (nu_t);
+    if (laplacian)
+        for (int c = 0; c < 3; c++)
+            for (int q = 0; q < 3; q++) r[c][q] = 0.;
+    for (int c = 0; c < 3; c++)
+        out[c] += (g11 * d[c][0] + g21 * d[c][1] + g31 * d[c][2] + r[0][c] * nrm[0] + r[1][c] * nrm[1] +
+                   r[2][c] * nrm[2]) * ajc * (nu);
+}
+
+__global__ void __launch_bounds__(FF_TPB) k_face_fluxes(FaceArgs a)
+{
+    const Grid &g = a.g;
+    const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (t >= g.n) return;
+    const int i = t % g.mx;
+    const long long rr = t / g.mx;
+    const int j = rr % g.my, k = rr / g.my;
+    const long long q = t;
+    const long long n = g.n;
+    const int mx = g.mx, my = g.my, mz = g.mz;
+    const double SOL = 0.1;  // k-omega.c:19
+
+    // ---- i face (momentum.c:668-928): i in [0,mx-2], j,k interior
+    {
+        double dv[3] = {0, 0, 0}, vs[3] = {0, 0, 0};
+        if (i < mx - 1 && j >= 1 && j < my - 1 && k >= 1 && k < mz - 1) {
+            const double cs[3] = {a.fi.csi[3 * q], a.fi.csi[3 * q + 1], a.fi.csi[3 * q + 2]};
+            const double et[3] = {a.fi.eta[3 * q], a.fi.eta[3 * q + 1], a.fi.eta[3 * q + 2]};
+            const double ze[3] = {a.fi.zet[3 * q], a.fi.zet[3 * q + 1], a.fi.zet[3 * q + 2]};
+            double d[3][3];
+            // Compute_du_i, k-omega.c:56-116
+            for (int c = 0; c < 3; c++) {
+                d[c][0] = Uc(a, c, k, j, i + 1) - Uc(a, c, k, j, i);
+                if (NVd(a, k, j + 1, i) > SOL || NVd(a, k, j + 1, i + 1) > SOL)
+                    d[c][1] = (Uc(a, c, k, j, i + 1) + Uc(a, c, k, j, i) - Uc(a, c, k, j - 1, i + 1) - Uc(a, c, k, j - 1, i)) * 0.5;
+                else if (NVd(a, k, j - 1, i) > SOL || NVd(a, k, j - 1, i + 1) > SOL)
+                    d[c][1] = (Uc(a, c, k, j + 1, i + 1) + Uc(a, c, k, j + 1, i) - Uc(a, c, k, j, i + 1) - Uc(a, c, k, j, i)) * 0.5;
+                else
+                    d[c][1] = (Uc(a, c, k, j + 1, i + 1) + Uc(a, c, k, j + 1, i) - Uc(a, c, k, j - 1, i + 1) - Uc(a, c, k, j - 1, i)) * 0.25;
+                if (NVd(a, k + 1, j, i) > SOL || NVd(a, k + 1, j, i + 1) > SOL)
+                    d[c][2] = (Uc(a, c, k, j, i + 1) + Uc(a, c, k, j, i) - Uc(a, c, k - 1, j, i + 1) - Uc(a, c, k - 1, j, i)) * 0.5;
+                else if (NVd(a, k - 1, j, i) > SOL || NVd(a, k - 1, j, i + 1) > SOL)
+                    d[c][2] = (Uc(a, c, k + 1, j, i + 1) + Uc(a, c, k + 1, j, i) - Uc(a, c, k, j, i + 1) - Uc(a, c, k, j, i)) * 0.5;
+                else
+                    d[c][2] = (Uc(a, c, k + 1, j, i + 1) + Uc(a, c, k + 1, j, i) - Uc(a, c, k - 1, j, i + 1) - Uc(a, c, k - 1, j, i)) * 0.25;
+            }
+            int iL = i - 1, iR = i + 2;
+            if (i == 0 || i == mx - 2) {
+                if (g.pi && i == mx - 2) iR = mx + 2;
+                else if (g.pi && i == 0) iL = -3;
+                else iL = i, iR = i + 1;
+            } else if (NVd(a, k, j, iL) + NVd(a, k, j, iR) > 0.1) iL = i, iR = i + 1;
+            if (a.second_order) iL = i, iR = i + 1;
+            const double uc = a.ucont[3 * q + 0];
+            for (int c = 0; c < 3; c++) {
+                if (a.second_order) dv[c] = -uc * 0.5 * (Uc(a, c, k, j, i) + Uc(a, c, k, j, i + 1));
+                else dv[c] = -uc * 0.0625 * (-Uc(a, c, k, j, iL) + 9. * Uc(a, c, k, j, i) + 9. * Uc(a, c, k, j, i + 1) - Uc(a, c, k, j, iR));
+            }
+            if (NVd(a, k, j, i) + NVd(a, k, j, i + 1) > 0.1) dv[0] = dv[1] = dv[2] = 0;
+            double nut = 0;
+            if (a.les) {
+                if ((i == 0 && !g.pi) || NVd(a, k, j, i) > 0.1) nut = a.nut[gidx(g, k, j, i + 1)];
+                else if ((i == mx - 2 && !g.pi) || NVd(a, k, j, i + 1) > 0.1) nut = a.nut[q];
+                else nut = 0.5 * (a.nut[q] + a.nut[gidx(g, k, j, i + 1)]);
+            }
+            visc_face(cs, et, ze, cs, a.fi.aj[q], d, a.nu, nut, a.les, a.laplacian, vs);
+        }
+        for (int c = 0; c < 3; c++) { a.out[(0 + c) * n + q] = dv[c]; a.out[(9 + c) * n + q] = vs[c]; }
+    }
+    // ---- j face (momentum.c:930-1211): j in [0,my-2], i,k interior
+    {
+        double dv[3] = {0, 0, 0}, vs[3] = {0, 0, 0};
+        if (i >= 1 && i < mx - 1 && j < my - 1 && k >= 1 && k < mz - 1) {
+            const double cs[3] = {a.fj.csi[3 * q], a.fj.csi[3 * q + 1], a.fj.csi[3 * q + 2]};
+            const double et[3] = {a.fj.eta[3 * q], a.fj.eta[3 * q + 1], a.fj.eta[3 * q + 2]};
+            const double ze[3] = {a.fj.zet[3 * q], a.fj.zet[3 * q + 1], a.fj.zet[3 * q + 2]};
+            double d[3][3];
+            // Compute_du_j, k-omega.c:141-200
+            for (int c = 0; c < 3; c++) {
+                if (NVd(a, k, j, i + 1) > SOL || NVd(a, k, j + 1, i + 1) > SOL)
+                    d[c][0] = (Uc(a, c, k, j + 1, i) + Uc(a, c, k, j, i) - Uc(a, c, k, j + 1, i - 1) - Uc(a, c, k, j, i - 1)) * 0.5;
+                else if (NVd(a, k, j, i - 1) > SOL || NVd(a, k, j + 1, i - 1) > SOL)
+                    d[c][0] = (Uc(a, c, k, j + 1, i + 1) + Uc(a, c, k, j, i + 1) - Uc(a, c, k, j + 1, i) - Uc(a, c, k, j, i)) * 0.5;
+                else
+                    d[c][0] = (Uc(a, c, k, j + 1, i + 1) + Uc(a, c, k, j, i + 1) - Uc(a, c, k, j + 1, i - 1) - Uc(a, c, k, j, i - 1)) * 0.25;
+                d[c][1] = Uc(a, c, k, j + 1, i) - Uc(a, c, k, j, i);
+                if (NVd(a, k + 1, j, i) > SOL || NVd(a, k + 1, j + 1, i) > SOL)
+                    d[c][2] = (Uc(a, c, k, j + 1, i) + Uc(a, c, k, j, i) - Uc(a, c, k - 1, j + 1, i) - Uc(a, c, k - 1, j, i)) * 0.5;
+                else if (NVd(a, k - 1, j, i) > SOL || NVd(a, k - 1, j + 1, i) > SOL)
+                    d[c][2] = (Uc(a, c, k + 1, j + 1, i) + Uc(a, c, k + 1, j, i) - Uc(a, c, k, j + 1, i) - Uc(a, c, k, j, i)) * 0.5;
+                else
+                    d[c][2] = (Uc(a, c, k + 1, j + 1, i) + Uc(a, c, k + 1, j, i) - Uc(a, c, k - 1, j + 1, i) - Uc(a, c, k - 1, j, i)) * 0.25;
+            }
+            int jL = j - 1, jR = j + 2;
+            if (j == 0 || j == my - 2) {
+                if (g.pj && j == my - 2) jR = my + 2;
+                else if (g.pj && j == 0) jL = -3;
+                else jL = j, jR = j + 1;
+            } else if (NVd(a, k, jL, i) + NVd(a, k, jR, i) > 0.1) jL = j, jR = j + 1;
+            if (a.second_order) jL = j, jR = j + 1;
+            const double uc = a.ucont[3 * q + 1];
+            for (int c = 0; c < 3; c++) {
+                if (a.second_order) dv[c] = -uc * 0.5 * (Uc(a, c, k, j, i) + Uc(a, c, k, j + 1, i));
+                else dv[c] = -uc * 0.0625 * (-Uc(a, c, k, jL, i) + 9. * Uc(a, c, k, j, i) + 9. * Uc(a, c, k, j + 1, i) - Uc(a, c, k, jR, i));
+            }
+            if (NVd(a, k, j, i) + NVd(a, k, j + 1, i) > 0.1) dv[0] = dv[1] = dv[2] = 0;
+            double nut = 0;
+            if (a.les) {
+                if ((j == 0 && !g.pj) || NVd(a, k, j, i) > 0.1) nut = a.nut[gidx(g, k, j + 1, i)];
+                else if ((j == my - 2 && !g.pj) || NVd(a, k, j + 1, i) > 0.1) nut = a.nut[q];
+                else nut = 0.5 * (a.nut[q] + a.nut[gidx(g, k, j + 1, i)]);
+            }
+            visc_face(cs, et, ze, et, a.fj.aj[q], d, a.nu, nut, a.les, a.laplacian, vs);
+        }
+        for (int c = 0; c < 3; c++) { a.out[(3 + c) * n + q] = dv[c]; a.out[(12 + c) * n + q] = vs[c]; }
+    }
+    // ---- k face (momentum.c:1212-1451): k in [0,mz-2], i,j interior
+    {
+        double dv[3] = {0, 0, 0}, vs[3] = {0, 0, 0};
+        if (i >= 1 && i < mx - 1 && j >= 1 && j < my - 1 && k < mz - 1) {
+            const double cs[3] = {a.fk.csi[3 * q], a.fk.csi[3 * q + 1], a.fk.csi[3 * q + 2]};
+            const double et[3] = {a.fk.eta[3 * q], a.fk.eta[3 * q + 1], a.fk.eta[3 * q + 2]};
+            const double ze[3] = {a.fk.zet[3 * q], a.fk.zet[3 * q + 1], a.fk.zet[3 * q + 2]};
+            double d[3][3];
+            // Compute_du_k, k-omega.c:227-288
+            for (int c = 0; c < 3; c++) {
+                if (NVd(a, k, j, i + 1) > SOL || NVd(a, k + 1, j, i + 1) > SOL)
+                    d[c][0] = (Uc(a, c, k + 1, j, i) + Uc(a, c, k, j, i) - Uc(a, c, k + 1, j, i - 1) - Uc(a, c, k, j, i - 1)) * 0.5;
+                else if (NVd(a, k, j, i - 1) > SOL || NVd(a, k + 1, j, i - 1) > SOL)
+                    d[c][0] = (Uc(a, c, k + 1, j, i + 1) + Uc(a, c, k, j, i + 1) - Uc(a, c, k + 1, j, i) - Uc(a, c, k, j, i)) * 0.5;
+                else
+                    d[c][0] = (Uc(a, c, k + 1, j, i + 1) + Uc(a, c, k, j, i + 1) - Uc(a, c, k + 1, j, i - 1) - Uc(a, c, k, j, i - 1)) * 0.25;
+                if (NVd(a, k, j + 1, i) > SOL || NVd(a, k + 1, j + 1, i) > SOL)
+                    d[c][1] = (Uc(a, c, k + 1, j, i) + Uc(a, c, k, j, i) - Uc(a, c, k + 1, j - 1, i) - Uc(a, c, k, j - 1, i)) * 0.5;
+                else if (NVd(a, k, j - 1, i) > SOL || NVd(a, k + 1, j - 1, i) > SOL)
+                    d[c][1] = (Uc(a, c, k + 1, j + 1, i) + Uc(a, c, k, j + 1, i) - Uc(a, c, k + 1, j, i) - Uc(a, c, k, j, i)) * 0.5;
+                else
+                    d[c][1] = (Uc(a, c, k + 1, j + 1, i) + Uc(a, c, k, j + 1, i) - Uc(a, c, k + 1, j - 1, i) - Uc(a, c, k, j - 1, i)) * 0.25;
+                d[c][2] = Uc(a, c, k + 1, j, i) - Uc(a, c, k, j, i);
+            }
+            int kL = k - 1, kR = k + 2;
+            if (k == 0 || k == mz - 2) {
+                if (g.pk && k == mz - 2) kR = mz + 2;
+                else if (g.pk && k == 0) kL = -3;
+                else kL = k, kR = k + 1;
+            } else if (NVd(a, kL, j, i) + NVd(a, kR, j, i) > 0.1) kL = k, kR = k + 1;
+            if (a.second_order) kL = k, kR = k + 1;
+            const double uc = a.ucont[3 * q + 2];
+            for (int c = 0; c < 3; c++) {
+                if (a.second_order) dv[c] = -uc * 0.5 * (Uc(a, c, k, j, i) + Uc(a, c, k + 1, j, i));
+                else dv[c] = -uc * 0.0625 * (-Uc(a, c, kL, j, i) + 9. * Uc(a, c, k, j, i) + 9. * Uc(a, c, k + 1, j, i) - Uc(a, c, kR, j, i));
+            }
+            if (NVd(a, k, j, i) + NVd(a, k + 1, j, i) > 0.1) dv[0] = dv[1] = dv[2] = 0;
+            double nut = 0;
+            if (a.les) {
+                if ((k == 0 && !g.pk) || NVd(a, k, j, i) > 0.1) nut = a.nut[gidx(g, k + 1, j, i)];
+                else if ((k == mz - 2 && !g.pk) || NVd(a, k + 1, j, i) > 0.1) nut = a.nut[q];
+                else nut = 0.5 * (a.nut[q] + a.nut[gidx(g, k + 1, j, i)]);
+            }
+            visc_face(cs, et, ze, ze, a.fk.aj[q], d, a.nu, nut, a.les, a.laplacian, vs);
+        }
+        for (int c = 0; c < 3; c++) { a.out[(6 + c) * n + q] = dv[c]; a.out[(15 + c) * n + q] = vs[c]; }
+    }
+}
+
+// periodic fix of the 18 flux planes (momentum.c:1504-1546)
+__global__ void k_periodic_soa(Grid g, double *__restrict__ f, int nc, int dir)
+{
+    long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    int i, j, k;
+    if (dir == 0) {
+        if (t >= 2LL * g.my * g.mz) return;
+        int side = t & 1; long long r = t >> 1;
+        j = r % g.my; k = r / g.my; i = side ? g.mx - 1 : 0;
+    } else if (dir == 1) {
+        if (t >= 2LL * g.mx * g.mz) return;
+        i = t % g.mx; long long r = t / g.mx;
+        int side = r & 1; k = r >> 1; j = side ? g.my - 1 : 0;
+    } else {
+        if (t >= 2LL * g.mx * g.my) return;
+        i = t % g.mx; long long r = t / g.mx;
+        j = r % g.my; int side = (int)(r / g.my); k = side ? g.mz - 1 : 0;
+    }
+    int a, b, c, fi, fj, fk;
+    if (!periodic_src(g, k, j, i, c, b, a, fi, fj, fk)) return;
+    long long d = gidx(g, k, j, i), s = gidx(g, c, b, a);
+    for (int q = 0; q < nc; q++) f[q * g.n + d] = f[q * g.n + s];
+}
+
+struct CellArgs {
+    Grid g;
+    const double *face;  // 18 planes
+    const double *csi, *eta, *zet, *iaj, *jaj, *kaj, *nvert;
+    double scale;
+    int second_order;
+    double *rhs;          // Rhs (accumulated) when pucont == nullptr
+    const double *dp;     // RK mode: dP
+    const double *pucont; // RK mode: pUcont
+    double alpha;         // RK mode: alfa * dt * st
+    double *ucont_out;    // RK mode: Ucont
+};
+
+__device__ inline double Fv(const CellArgs &a, int plane, int k, int j, int i)
+{
+    return a.face[plane * a.g.n + gidx(a.g, k, j, i)];
+}
+
+// Fp of an interior cell, momentum.c:1548-1678
+__device__ void fp_interior(const CellArgs &a, int k, int j, int i, double f[3])
+{
+    const Grid &g = a.g;
+    const int mx = g.mx, my = g.my, mz = g.mz;
+    double dv[3];
+    for (int c = 0; c < 3; c++)
+        dv[c] = (Fv(a, c, k, j, i) - Fv(a, c, k, j, i - 1) + Fv(a, 3 + c, k, j, i) - Fv(a, 3 + c, k, j - 1, i) +
+                 Fv(a, 6 + c, k, j, i) - Fv(a, 6 + c, k - 1, j, i));
+    if (a.second_order) {
+        for (int c = 0; c < 3; c++) f[c] = dv[c];
+    } else {
+        int iR = i + 1, iL = i - 2, jR = j + 1, jL = j - 2, kR = k + 1, kL = k - 2;
+        double di = 3., dj = 3., dk = 3.;
+        if (i == 1) { if (g.pi) iL = -3; else iR = i, iL = i - 1, di = 1.; }
+        else if (i == 2 || i == mx - 3) { if (!g.pi) iR = i, iL = i - 1, di = 1.; }
+        else if (i == mx - 2) { if (g.pi) iR = mx + 1; else iR = i, iL = i - 1, di = 1.; }
+        if (j == 1) { if (g.pj) jL = -3; else jR = j, jL = j - 1, dj = 1.; }
+        else if (j == 2 || j == my - 3) { if (!g.pj) jR = j, jL = j - 1, dj = 1.; }
+        else if (j == my - 2) { if (g.pj) jR = my + 1; else jR = j, jL = j - 1, dj = 1.; }
+        if (k == 1) { if (g.pk) kL = -3; else kR = k, kL = k - 1, dk = 1.; }
+        else if (k == 2 || k == mz - 3) { if (!g.pk) kR = k, kL = k - 1, dk = 1.; }
+        else if (k == mz - 2) { if (g.pk) kR = mz + 1; else kR = k, kL = k - 1, dk = 1.; }
+        const double *nv = a.nvert;
+        if (nv[gidx(g, k, j, i - 1)] + nv[gidx(g, k, j, i)] + nv[gidx(g, k, j, i + 1)] > 0.1) iR = i, iL = i - 1, di = 1.;
+        if (nv[gidx(g, k, j - 1, i)] + nv[gidx(g, k, j, i)] + nv[gidx(g, k, j + 1, i)] > 0.1) jR = j, jL = j - 1, dj = 1.;
+        if (nv[gidx(g, k - 1, j, i)] + nv[gidx(g, k, j, i)] + nv[gidx(g, k + 1, j, i)] > 0.1) kR = k, kL = k - 1, dk = 1.;
+        for (int c = 0; c < 3; c++) {
+            double d4 = 0;
+            d4 += (Fv(a, c, k, j, iR) - Fv(a, c, k, j, iL)) * (1. / di);
+            d4 += (Fv(a, 3 + c, k, jR, i) - Fv(a, 3 + c, k, jL, i)) * (1. / dj);
+            d4 += (Fv(a, 6 + c, kR, j, i) - Fv(a, 6 + c, kL, j, i)) * (1. / dk);
+            f[c] = 9. / 8. * dv[c] + -1. / 8. * d4;
+        }
+    }
+    for (int c = 0; c < 3; c++)
+        f[c] += (Fv(a, 9 + c, k, j, i) - Fv(a, 9 + c, k, j, i - 1) + Fv(a, 12 + c, k, j, i) - Fv(a, 12 + c, k, j - 1, i) +
+                 Fv(a, 15 + c, k, j, i) - Fv(a, 15 + c, k - 1, j, i)) / 1.0;
+}
+
+// Fp as the projection loop reads it: interior cells directly, periodic ghost
+// cells through the periodic fix (momentum.c:1685-1713), other ghosts 0.
+__device__ inline void fp_at(const CellArgs &a, int k, int j, int i, double f[3])
+{
+    const Grid &g = a.g;
+    int c_, b_, a_, fi, fj, fk;
+    if (periodic_src(g, k, j, i, c_, b_, a_, fi, fj, fk)) { k = c_; j = b_; i = a_; }
+    if (i < 1 || i > g.mx - 2 || j < 1 || j > g.my - 2 || k < 1 || k > g.mz - 2) {
+        f[0] = f[1] = f[2] = 0;
+        return;
+    }
+    fp_interior(a, k, j, i, f);
+}
+
+__global__ void __launch_bounds__(FF_TPB) k_cell_rhs(CellArgs a)
+{
+    const Grid &g = a.g;
+    const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (t >= g.n) return;
+    const int i = t % g.mx;
+    const long long rr = t / g.mx;
+    const int j = rr % g.my, k = rr / g.my;
+    const long long q = t;
+    const int mx = g.mx, my = g.my, mz = g.mz;
+    double r[3];
+    const bool interior = i >= 1 && i < mx - 1 && j >= 1 && j < my - 1 && k >= 1 && k < mz - 1;
+    bool zero[3] = {true, true, true};
+    if (interior) {
+        double f0[3], fi_[3], fj_[3], fk_[3];
+        fp_at(a, k, j, i, f0);
+        fp_at(a, k, j, i + 1, fi_);
+        fp_at(a, k, j + 1, i, fj_);
+        fp_at(a, k + 1, j, i, fk_);
+        const long long qi = gidx(g, k, j, i + 1), qj = gidx(g, k, j + 1, i), qk = gidx(g, k + 1, j, i);
+        const double *c0 = a.csi + 3 * q, *c1 = a.csi + 3 * qi;
+        const double *e0 = a.eta + 3 * q, *e1 = a.eta + 3 * qj;
+        const double *z0 = a.zet + 3 * q, *z1 = a.zet + 3 * qk;
+        r[0] = a.scale * (0.5 * (c0[0] * f0[0] + c0[1] * f0[1] + c0[2] * f0[2]) +
+                          0.5 * (c1[0] * fi_[0] + c1[1] * fi_[1] + c1[2] * fi_[2])) * a.iaj[q];
+        r[1] = a.scale * (0.5 * (e0[0] * f0[0] + e0[1] * f0[1] + e0[2] * f0[2]) +
+                          0.5 * (e1[0] * fj_[0] + e1[1] * fj_[1] + e1[2] * fj_[2])) * a.jaj[q];
+        r[2] = a.scale * (0.5 * (z0[0] * f0[0] + z0[1] * f0[1] + z0[2] * f0[2]) +
+                          0.5 * (z1[0] * fk_[0] + z1[1] * fk_[1] + z1[2] * fk_[2])) * a.kaj[q];
+        const double *nv = a.nvert;
+        zero[0] = nv[q] + nv[qi] > 0.1 || (!g.pi && i == mx - 2);
+        zero[1] = nv[q] + nv[qj] > 0.1 || (!g.pj && j == my - 2);
+        zero[2] = nv[q] + nv[qk] > 0.1 || (!g.pk && k == mz - 2);
+    }
+    if (a.pucont) {
+        // Rhs = 0 + F; Rhs += -1*dP; Ucont = alpha*Rhs + pUcont  (rhs.c:1647-1658)
+        for (int c = 0; c < 3; c++) {
+            double rv = zero[c] ? 0.0 : (0.0 + r[c]);
+            rv += -1 * a.dp[3 * q + c];
+            a.ucont_out[3 * q + c] = a.alpha * rv + a.pucont[3 * q + c];
+        }
+    } else {
+        for (int c = 0; c < 3; c++) a.rhs[3 * q + c] = zero[c] ? 0.0 : a.rhs[3 * q + c] + r[c];
+    }
+}
+
+int formfunction2(vfs_ctx *c, double scale, const double *pucont, double rk_alpha, double *ucont_out)
+{
+    const Grid &g = c->g;
+    hipStream_t st = c->stream;
+    int rc = periodic_fix(c, c->f[VFS_UCONT], 3, 0);  // momentum.c:638-666
+    if (rc) return rc;
+    FaceArgs fa;
+    fa.g = g;
+    fa.ucont = c->f[VFS_UCONT];
+    fa.ucat = c->f[VFS_UCAT];
+    fa.nut = c->f[VFS_NU_T];
+    fa.nvert = c->f[VFS_NVERT];
+    fa.fi = {c->f[VFS_ICSI], c->f[VFS_IETA], c->f[VFS_IZET], c->f[VFS_IAJ]};
+    fa.fj = {c->f[VFS_JCSI], c->f[VFS_JETA], c->f[VFS_JZET], c->f[VFS_JAJ]};
+    fa.fk = {c->f[VFS_KCSI], c->f[VFS_KETA], c->f[VFS_KZET], c->f[VFS_KAJ]};
+    fa.nu = 1. / c->p.ren;
+    fa.les = c->p.les;
+    fa.second_order = c->p.second_order;
+    fa.laplacian = c->p.laplacian;
+    fa.out = c->face;
+    prof_begin(c, K_FACE);
+    hipLaunchKernelGGL(k_face_fluxes, dim3(blocks_for(g.n, FF_TPB)), dim3(FF_TPB), 0, st, fa);
+    prof_end(c);
+    VFS_LAUNCH_CHECK("face fluxes");
+    const long long np[3] = {2LL * g.my * g.mz, 2LL * g.mx * g.mz, 2LL * g.mx * g.my};
+    const int per[3] = {g.pi, g.pj, g.pk};
+    for (int d = 0; d < 3; d++)
+        if (per[d]) hipLaunchKernelGGL(k_periodic_soa, dim3(blocks_for(np[d], FF_TPB)), dim3(FF_TPB), 0, st, g, c->face, 18, d);
+    VFS_LAUNCH_CHECK("face periodic");
+    CellArgs ca;
+    ca.g = g;
+    ca.face = c->face;
+    ca.csi = c->f[VFS_CSI];
+    ca.eta = c->f[VFS_ETA];
+    ca.zet = c->f[VFS_ZET];
+    ca.iaj = c->f[VFS_IAJ];
+    ca.jaj = c->f[VFS_JAJ];
+    ca.kaj = c->f[VFS_KAJ];
+    ca.nvert = c->f[VFS_NVERT];
+    ca.scale = scale;
+    ca.second_order = c->p.second_order;
+    ca.rhs = c->f[VFS_RHS];
+    ca.dp = c->f[VFS_DP];
+    ca.pucont = pucont;
+    ca.alpha = rk_alpha;
+    ca.ucont_out = ucont_out;
+    prof_begin(c, K_CELL);
+    hipLaunchKernelGGL(k_cell_rhs, dim3(blocks_for(g.n, FF_TPB)), dim3(FF_TPB), 0, st, ca);
+    prof_end(c);
+    VFS_LAUNCH_CHECK("cell rhs");
+    return VFS_OK;
+}
+
+}  // namespace vfs

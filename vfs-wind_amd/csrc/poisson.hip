@@ -1,0 +1,913 @@
+// poisson.hip -- pressure Poisson operator, multigrid and FGMRES for gfx950.
+//
+// Operator: the poisson==1 rows of PoissonLHSNew (poisson.c:977-1688): one
+// row per fluid interior cell, 19 coefficients assembled from the face metric
+// products g_ab = (n_a . n_b) * face aj with the reference's one-sided
+// cross-derivative switching at walls and blanked cells.  Coefficients are
+// stored as 19 structure-of-arrays planes (coalesced), in FP64 for the outer
+// residual and in FP32 for the mixed-precision preconditioner.
+//
+// Preconditioner: geometric V-cycle; coarse grids take every other node
+// (init.c:402-455) and are rediscretised through the same metric and
+// assembly kernels; restriction sums the eight child residuals (rows are flux
+// sums), prolongation is cell-centred trilinear; the smoother is damped
+// Jacobi over wall-normal (j) lines, each line solved exactly by the Thomas
+// algorithm, one thread per line with the sweep marching in j so that every
+// load coalesces across the i-consecutive lanes of a wavefront.
+//
+// Outer solver: FGMRES(m) in FP64, classical Gram-Schmidt with one selective
+// re-orthogonalisation; the FP32 V-cycle inside it acts as iterative
+// refinement, so the FP64 true residual reaches the requested tolerance.
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "vfs_internal.h"
+#include "vfs_reduce.h"
+
+#pragma clang fp contract(off)
+
+namespace vfs {
+
+constexpr int PTPB = 256;
+constexpr int NST = 19;
+enum { CP, EP, WP, NP, SP, TP, BP, NE, SE, NW, SW, TN, BN, TS, BS, TE, BE, TW, BW };
+__constant__ int c_off[NST][3] = {{0, 0, 0},  {1, 0, 0},   {-1, 0, 0},  {0, 1, 0},  {0, -1, 0},
+                                  {0, 0, 1},  {0, 0, -1},  {1, 1, 0},   {1, -1, 0}, {-1, 1, 0},
+                                  {-1, -1, 0}, {0, 1, 1},  {0, 1, -1},  {0, -1, 1}, {0, -1, -1},
+                                  {1, 0, 1},  {1, 0, -1},  {-1, 0, 1},  {-1, 0, -1}};
+
+int periodic_fix_grid(hipStream_t st, const Grid &g, double *f, int nc, int mode);
+
+// periodic image of a neighbour index (gid ghost mapping, poisson.c:284-309)
+__device__ inline int canon(int i, int m, int per)
+{
+    if (per) {
+        if (i <= 0) i += m - 2;
+        else if (i >= m - 1) i -= m - 2;
+    }
+    return i;
+}
+
+__device__ inline long long nbr_idx(const Grid &g, int k, int j, int i, int s)
+{
+    int ii = canon(i + c_off[s][0], g.mx, g.pi);
+    int jj = canon(j + c_off[s][1], g.my, g.pj);
+    int kk = canon(k + c_off[s][2], g.mz, g.pk);
+    return ((long long)kk * g.my + jj) * g.mx + ii;
+}
+
+struct FaceSet {
+    const double *icsi, *ieta, *izet, *iaj, *jcsi, *jeta, *jzet, *jaj, *kcsi, *keta, *kzet, *kaj;
+};
+
+__device__ inline void face_g(const Grid &g, const FaceSet &F, int k, int j, int i, double gg[9])
+{
+    long long q = gidx(g, k, j, i);
+    const double *ic = F.icsi + 3 * q, *ie = F.ieta + 3 * q, *iz = F.izet + 3 * q;
+    const double *jc = F.jcsi + 3 * q, *je = F.jeta + 3 * q, *jz = F.jzet + 3 * q;
+    const double *kc = F.kcsi + 3 * q, *ke = F.keta + 3 * q, *kz = F.kzet + 3 * q;
+    const double iaj = F.iaj[q], jaj = F.jaj[q], kaj = F.kaj[q];
+    gg[0] = (ic[0] * ic[0] + ic[1] * ic[1] + ic[2] * ic[2]) * iaj;
+    gg[1] = (ie[0] * ic[0] + ie[1] * ic[1] + ie[2] * ic[2]) * iaj;
+    gg[2] = (iz[0] * ic[0] + iz[1] * ic[1] + iz[2] * ic[2]) * iaj;
+    gg[3] = (jc[0] * je[0] + jc[1] * je[1] + jc[2] * je[2]) * jaj;
+    gg[4] = (je[0] * je[0] + je[1] * je[1] + je[2] * je[2]) * jaj;
+    gg[5] = (jz[0] * je[0] + jz[1] * je[1] + jz[2] * je[2]) * jaj;
+    gg[6] = (kc[0] * kz[0] + kc[1] * kz[1] + kc[2] * kz[2]) * kaj;
+    gg[7] = (ke[0] * kz[0] + ke[1] * kz[1] + ke[2] * kz[2]) * kaj;
+    gg[8] = (kz[0] * kz[0] + kz[1] * kz[1] + kz[2] * kz[2]) * kaj;
+}
+
+// one row of PoissonLHSNew (poisson.c:1159-1561), poisson==1 form
+__device__ void assemble_row(const Grid &g, const FaceSet &F, const double *nvert, double th, int k, int j, int i,
+                             double vol[NST])
+{
+    const int mx = g.mx, my = g.my, mz = g.mz, pi_ = g.pi, pj = g.pj, pk = g.pk;
+    const double r = 1.0;
+    for (int s = 0; s < NST; s++) vol[s] = 0.;
+#define NV(kk, jj, ii) nvert[gidx(g, (kk), (jj), (ii))]
+    double gg[9];
+    if (NV(k, j, i + 1) < th && (i != mx - 2 || pi_)) {
+        face_g(g, F, k, j, i, gg);
+        vol[CP] -= gg[0] / r; vol[EP] += gg[0] / r;
+        if ((j == my - 2 && !pj) || NV(k, j + 1, i) + NV(k, j + 1, i + 1) > th) {
+            if (NV(k, j - 1, i) + NV(k, j - 1, i + 1) < th && j != 1) {
+                vol[CP] += gg[1] * 0.5 / r; vol[EP] += gg[1] * 0.5 / r; vol[SP] -= gg[1] * 0.5 / r; vol[SE] -= gg[1] * 0.5 / r;
+            }
+        } else if ((j == 1 && !pj) || NV(k, j - 1, i) + NV(k, j - 1, i + 1) > th) {
+            if (NV(k, j + 1, i) + NV(k, j + 1, i + 1) < th) {
+                vol[NP] += gg[1] * 0.5 / r; vol[NE] += gg[1] * 0.5 / r; vol[CP] -= gg[1] * 0.5 / r; vol[EP] -= gg[1] * 0.5 / r;
+            }
+        } else {
+            vol[NP] += gg[1] * 0.25 / r; vol[NE] += gg[1] * 0.25 / r; vol[SP] -= gg[1] * 0.25 / r; vol[SE] -= gg[1] * 0.25 / r;
+        }
+        if ((k == mz - 2 && !pk) || NV(k + 1, j, i) + NV(k + 1, j, i + 1) > th) {
+            if (NV(k - 1, j, i) + NV(k - 1, j, i + 1) < th && k != 1) {
+                vol[CP] += gg[2] * 0.5 / r; vol[EP] += gg[2] * 0.5 / r; vol[BP] -= gg[2] * 0.5 / r; vol[BE] -= gg[2] * 0.5 / r;
+            }
+        } else if ((k == 1 && !pk) || NV(k - 1, j, i) + NV(k - 1, j, i + 1) > th) {
+            if (NV(k + 1, j, i) + NV(k + 1, j, i + 1) < th) {
+                vol[TP] += gg[2] * 0.5 / r; vol[TE] += gg[2] * 0.5 / r; vol[CP] -= gg[2] * 0.5 / r; vol[EP] -= gg[2] * 0.5 / r;
+            }
+        } else {
+            vol[TP] += gg[2] * 0.25 / r; vol[TE] += gg[2] * 0.25 / r; vol[BP] -= gg[2] * 0.25 / r; vol[BE] -= gg[2] * 0.25 / r;
+        }
+    }
+    if (NV(k, j, i - 1) < th && (i != 1 || pi_)) {
+        face_g(g, F, k, j, i - 1, gg);
+        vol[CP] -= gg[0] / r; vol[WP] += gg[0] / r;
+        if ((j == my - 2 && !pj) || NV(k, j + 1, i) + NV(k, j + 1, i - 1) > th) {
+            if (NV(k, j - 1, i) + NV(k, j - 1, i - 1) < th && j != 1) {
+                vol[CP] -= gg[1] * 0.5 / r; vol[WP] -= gg[1] * 0.5 / r; vol[SP] += gg[1] * 0.5 / r; vol[SW] += gg[1] * 0.5 / r;
+            }
+        } else if ((j == 1 && !pj) || NV(k, j - 1, i) + NV(k, j - 1, i - 1) > th) {
+            if (NV(k, j + 1, i) + NV(k, j + 1, i - 1) < th) {
+                vol[NP] -= gg[1] * 0.5 / r; vol[NW] -= gg[1] * 0.5 / r; vol[CP] += gg[1] * 0.5 / r; vol[WP] += gg[1] * 0.5 / r;
+            }
+        } else {
+            vol[NP] -= gg[1] * 0.25 / r; vol[NW] -= gg[1] * 0.25 / r; vol[SP] += gg[1] * 0.25 / r; vol[SW] += gg[1] * 0.25 / r;
+        }
+        if ((k == mz - 2 && !pk) || NV(k + 1, j, i) + NV(k + 1, j, i - 1) > th) {
+            if (NV(k - 1, j, i) + NV(k - 1, j, i - 1) < th && k != 1) {
+                vol[CP] -= gg[2] * 0.5 / r; vol[WP] -= gg[2] * 0.5 / r; vol[BP] += gg[2] * 0.5 / r; vol[BW] += gg[2] * 0.5 / r;
+            }
+        } else if ((k == 1 && !pk) || NV(k - 1, j, i) + NV(k - 1, j, i - 1) > th) {
+            if (NV(k + 1, j, i) + NV(k + 1, j, i - 1) < th) {
+                vol[TP] -= gg[2] * 0.5 / r; vol[TW] -= gg[2] * 0.5 / r; vol[CP] += gg[2] * 0.5 / r; vol[WP] += gg[2] * 0.5 / r;
+            }
+        } else {
+            vol[TP] -= gg[2] * 0.25 / r; vol[TW] -= gg[2] * 0.25 / r; vol[BP] += gg[2] * 0.25 / r; vol[BW] += gg[2] * 0.25 / r;
+        }
+    }
+    if (NV(k, j + 1, i) < th && (j != my - 2 || pj)) {
+        face_g(g, F, k, j, i, gg);
+        if ((i == mx - 2 && !pi_) || NV(k, j, i + 1) + NV(k, j + 1, i + 1) > th) {
+            if (NV(k, j, i - 1) + NV(k, j + 1, i - 1) < th && i != 1) {
+                vol[CP] += gg[3] * 0.5 / r; vol[NP] += gg[3] * 0.5 / r; vol[WP] -= gg[3] * 0.5 / r; vol[NW] -= gg[3] * 0.5 / r;
+            }
+        } else if ((i == 1 && !pi_) || NV(k, j, i - 1) + NV(k, j + 1, i - 1) > th) {
+            if (NV(k, j, i + 1) + NV(k, j + 1, i + 1) < th) {
+                vol[EP] += gg[3] * 0.5 / r; vol[NE] += gg[3] * 0.5 / r; vol[CP] -= gg[3] * 0.5 / r; vol[NP] -= gg[3] * 0.5 / r;
+            }
+        } else {
+            vol[EP] += gg[3] * 0.25 / r; vol[NE] += gg[3] * 0.25 / r; vol[WP] -= gg[3] * 0.25 / r; vol[NW] -= gg[3] * 0.25 / r;
+        }
+        vol[CP] -= gg[4] / r; vol[NP] += gg[4] / r;
+        if ((k == mz - 2 && !pk) || NV(k + 1, j, i) + NV(k + 1, j + 1, i) > th) {
+            if (NV(k - 1, j, i) + NV(k - 1, j + 1, i) < th && k != 1) {
+                vol[CP] += gg[5] * 0.5 / r; vol[NP] += gg[5] * 0.5 / r; vol[BP] -= gg[5] * 0.5 / r; vol[BN] -= gg[5] * 0.5 / r;
+            }
+        } else if ((k == 1 && !pk) || NV(k - 1, j, i) + NV(k - 1, j + 1, i) > th) {
+            if (NV(k + 1, j, i) + NV(k + 1, j + 1, i) < th) {
+                vol[TP] += gg[5] * 0.5 / r; vol[TN] += gg[5] * 0.5 / r; vol[CP] -= gg[5] * 0.5 / r; vol[NP] -= gg[5] * 0.5 / r;
+            }
+        } else {
+            vol[TP] += gg[5] * 0.25 / r; vol[TN] += gg[5] * 0.25 / r; vol[BP] -= gg[5] * 0.25 / r; vol[BN] -= gg[5] * 0.25 / r;
+        }
+    }
+    if (NV(k, j - 1, i) < th && (j != 1 || pj)) {
+        face_g(g, F, k, j - 1, i, gg);
+        if ((i == mx - 2 && !pi_) || NV(k, j, i + 1) + NV(k, j - 1, i + 1) > th) {
+            if (NV(k, j, i - 1) + NV(k, j - 1, i - 1) < th && i != 1) {
+                vol[CP] -= gg[3] * 0.5 / r; vol[SP] -= gg[3] * 0.5 / r; vol[WP] += gg[3] * 0.5 / r; vol[SW] += gg[3] * 0.5 / r;
+            }
+        } else if ((i == 1 && !pi_) || NV(k, j, i - 1) + NV(k, j - 1, i - 1) > th) {
+            if (NV(k, j, i + 1) + NV(k, j - 1, i + 1) < th) {
+                vol[EP] -= gg[3] * 0.5 / r; vol[SE] -= gg[3] * 0.5 / r; vol[CP] += gg[3] * 0.5 / r; vol[SP] += gg[3] * 0.5 / r;
+            }
+        } else {
+            vol[EP] -= gg[3] * 0.25 / r; vol[SE] -= gg[3] * 0.25 / r; vol[WP] += gg[3] * 0.25 / r; vol[SW] += gg[3] * 0.25 / r;
+        }
+        vol[CP] -= gg[4] / r; vol[SP] += gg[4] / r;
+        if ((k == mz - 2 && !pk) || NV(k + 1, j, i) + NV(k + 1, j - 1, i) > th) {
+            if (NV(k - 1, j, i) + NV(k - 1, j - 1, i) < th && k != 1) {
+                vol[CP] -= gg[5] * 0.5 / r; vol[SP] -= gg[5] * 0.5 / r; vol[BP] += gg[5] * 0.5 / r; vol[BS] += gg[5] * 0.5 / r;
+            }
+        } else if ((k == 1 && !pk) || NV(k - 1, j, i) + NV(k - 1, j - 1, i) > th) {
+            if (NV(k + 1, j, i) + NV(k + 1, j - 1, i) < th) {
+                vol[TP] -= gg[5] * 0.5 / r; vol[TS] -= gg[5] * 0.5 / r; vol[CP] += gg[5] * 0.5 / r; vol[SP] += gg[5] * 0.5 / r;
+            }
+        } else {
+            vol[TP] -= gg[5] * 0.25 / r; vol[TS] -= gg[5] * 0.25 / r; vol[BP] += gg[5] * 0.25 / r; vol[BS] += gg[5] * 0.25 / r;
+        }
+    }
+    if (NV(k + 1, j, i) < th && (k != mz - 2 || pk)) {
+        face_g(g, F, k, j, i, gg);
+        if ((i == mx - 2 && !pi_) || NV(k, j, i + 1) + NV(k + 1, j, i + 1) > th) {
+            if (NV(k, j, i - 1) + NV(k + 1, j, i - 1) < th && i != 1) {
+                vol[CP] += gg[6] * 0.5 / r; vol[TP] += gg[6] * 0.5 / r; vol[WP] -= gg[6] * 0.5 / r; vol[TW] -= gg[6] * 0.5 / r;
+            }
+        } else if ((i == 1 && !pi_) || NV(k, j, i - 1) + NV(k + 1, j, i - 1) > th) {
+            if (NV(k, j, i + 1) + NV(k + 1, j, i + 1) < th) {
+                vol[EP] += gg[6] * 0.5 / r; vol[TE] += gg[6] * 0.5 / r; vol[CP] -= gg[6] * 0.5 / r; vol[TP] -= gg[6] * 0.5 / r;
+            }
+        } else {
+            vol[EP] += gg[6] * 0.25 / r; vol[TE] += gg[6] * 0.25 / r; vol[WP] -= gg[6] * 0.25 / r; vol[TW] -= gg[6] * 0.25 / r;
+        }
+        if ((j == my - 2 && !pj) || NV(k, j + 1, i) + NV(k + 1, j + 1, i) > th) {
+            if (NV(k, j - 1, i) + NV(k + 1, j - 1, i) < th && j != 1) {
+                vol[CP] += gg[7] * 0.5 / r; vol[TP] += gg[7] * 0.5 / r; vol[SP] -= gg[7] * 0.5 / r; vol[TS] -= gg[7] * 0.5 / r;
+            }
+        } else if ((j == 1 && !pj) || NV(k, j - 1, i) + NV(k + 1, j - 1, i) > th) {
+            if (NV(k, j + 1, i) + NV(k + 1, j + 1, i) < th) {
+                vol[NP] += gg[7] * 0.5 / r; vol[TN] += gg[7] * 0.5 / r; vol[CP] -= gg[7] * 0.5 / r; vol[TP] -= gg[7] * 0.5 / r;
+            }
+        } else {
+            vol[NP] += gg[7] * 0.25 / r; vol[TN] += gg[7] * 0.25 / r; vol[SP] -= gg[7] * 0.25 / r; vol[TS] -= gg[7] * 0.25 / r;
+        }
+        vol[CP] -= gg[8] / r; vol[TP] += gg[8] / r;
+    }
+    if (NV(k - 1, j, i) < th && (k != 1 || pk)) {
+        face_g(g, F, k - 1, j, i, gg);
+        if ((i == mx - 2 && !pi_) || NV(k, j, i + 1) + NV(k - 1, j, i + 1) > th) {
+            if (NV(k, j, i - 1) + NV(k - 1, j, i - 1) < th && i != 1) {
+                vol[CP] -= gg[6] * 0.5 / r; vol[BP] -= gg[6] * 0.5 / r; vol[WP] += gg[6] * 0.5 / r; vol[BW] += gg[6] * 0.5 / r;
+            }
+        } else if ((i == 1 && !pi_) || NV(k, j, i - 1) + NV(k - 1, j, i - 1) > th) {
+            if (NV(k, j, i + 1) + NV(k - 1, j, i + 1) < th) {
+                vol[EP] -= gg[6] * 0.5 / r; vol[BE] -= gg[6] * 0.5 / r; vol[CP] += gg[6] * 0.5 / r; vol[BP] += gg[6] * 0.5 / r;
+            }
+        } else {
+            vol[EP] -= gg[6] * 0.25 / r; vol[BE] -= gg[6] * 0.25 / r; vol[WP] += gg[6] * 0.25 / r; vol[BW] += gg[6] * 0.25 / r;
+        }
+        if ((j == my - 2 && !pj) || NV(k, j + 1, i) + NV(k - 1, j + 1, i) > th) {
+            if (NV(k, j - 1, i) + NV(k - 1, j - 1, i) < th && j != 1) {
+                vol[CP] -= gg[7] * 0.5 / r; vol[BP] -= gg[7] * 0.5 / r; vol[SP] += gg[7] * 0.5 / r; vol[BS] += gg[7] * 0.5 / r;
+            }
+        } else if ((j == 1 && !pj) || NV(k, j - 1, i) + NV(k - 1, j - 1, i) > th) {
+            if (NV(k, j + 1, i) + NV(k - 1, j + 1, i) < th) {
+                vol[NP] -= gg[7] * 0.5 / r; vol[BN] -= gg[7] * 0.5 / r; vol[CP] += gg[7] * 0.5 / r; vol[BP] += gg[7] * 0.5 / r;
+            }
+        } else {
+            vol[NP] -= gg[7] * 0.25 / r; vol[BN] -= gg[7] * 0.25 / r; vol[SP] += gg[7] * 0.25 / r; vol[BS] += gg[7] * 0.25 / r;
+        }
+        vol[CP] -= gg[8] / r; vol[BP] += gg[8] / r;
+    }
+#undef NV
+}
+
+__device__ inline bool is_row(const Grid &g, const double *nvert, double th, int k, int j, int i)
+{
+    if (i <= 0 || i >= g.mx - 1 || j <= 0 || j >= g.my - 1 || k <= 0 || k >= g.mz - 1) return false;
+    return nvert[gidx(g, k, j, i)] < th;
+}
+
+__global__ void k_row_mask(Grid g, const double *__restrict__ nvert, double th, unsigned char *__restrict__ row)
+{
+    long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (t >= g.n) return;
+    int i = t % g.mx; long long r = t / g.mx; int j = r % g.my, k = r / g.my;
+    row[t] = is_row(g, nvert, th, k, j, i) ? 1 : 0;
+}
+
+// coefficients of every row; columns of non-rows are dropped like PETSc
+// drops the negative gid columns of PoissonLHSNew's MatSetValues
+__global__ void k_assemble(Grid g, FaceSet F, const double *__restrict__ nvert, double th,
+                           const unsigned char *__restrict__ row, double *__restrict__ c64, float *__restrict__ c32)
+{
+    long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (t >= g.n) return;
+    int i = t % g.mx; long long r = t / g.mx; int j = r % g.my, k = r / g.my;
+    double vol[NST];
+    if (row[t]) {
+        assemble_row(g, F, nvert, th, k, j, i, vol);
+        for (int s = 1; s < NST; s++)
+            if (!row[nbr_idx(g, k, j, i, s)]) vol[s] = 0.;
+    } else {
+        for (int s = 0; s < NST; s++) vol[s] = 0.;
+    }
+    for (int s = 0; s < NST; s++) {
+        if (c64) c64[s * g.n + t] = vol[s];
+        c32[s * g.n + t] = (float)vol[s];
+    }
+}
+
+// y = A x (rows only), or r = b - A x when b != nullptr
+template <typename T>
+__global__ void __launch_bounds__(PTPB) k_apply(Grid g, const T *__restrict__ coef, const unsigned char *__restrict__ row,
+                                                const T *__restrict__ x, const T *__restrict__ b, T *__restrict__ y)
+{
+    long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (t >= g.n) return;
+    if (!row[t]) { y[t] = 0; return; }
+    int i = t % g.mx; long long r = t / g.mx; int j = r % g.my, k = r / g.my;
+    T s = 0;
+#pragma unroll
+    for (int q = 0; q < NST; q++) s += coef[q * g.n + t] * x[nbr_idx(g, k, j, i, q)];
+    y[t] = b ? b[t] - s : s;
+}
+
+// Damped Jacobi over j lines: each thread owns the line (i,k), assembles the
+// off-line part of every row from xin, solves the tridiagonal (SP, CP, NP)
+// system by the Thomas algorithm and writes xout = xin + w*(xline - xin).
+template <typename T>
+__global__ void __launch_bounds__(PTPB) k_line_jacobi(Grid g, const T *__restrict__ coef, const unsigned char *__restrict__ row,
+                                                      const T *__restrict__ xin, const T *__restrict__ b, T *__restrict__ xout,
+                                                      T *__restrict__ cpw, T *__restrict__ dpw, T omega)
+{
+    const int ni = g.mx - 2;
+    long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (t >= (long long)ni * (g.mz - 2)) return;
+    const int i = 1 + t % ni, k = 1 + t / ni;
+    const long long n = g.n;
+    bool have_prev = false;
+    T cprev = 0, dprev = 0;
+    for (int j = 1; j < g.my - 1; j++) {
+        const long long q = ((long long)k * g.my + j) * g.mx + i;
+        if (!row[q]) { cpw[q] = 0; dpw[q] = 0; have_prev = false; continue; }
+        T rhs = b[q];
+#pragma unroll
+        for (int s = 0; s < NST; s++) {
+            if (s == CP || s == NP || s == SP) continue;
+            rhs -= coef[s * n + q] * xin[nbr_idx(g, k, j, i, s)];
+        }
+        const T lo = have_prev ? coef[SP * n + q] : T(0);
+        const T up = coef[NP * n + q];  // 0 when j+1 is not a row (assembly)
+        const T den = coef[CP * n + q] - lo * (have_prev ? cprev : T(0));
+        const T cc = up / den;
+        const T dd = (rhs - lo * (have_prev ? dprev : T(0))) / den;
+        cpw[q] = cc; dpw[q] = dd;
+        cprev = cc; dprev = dd;
+        have_prev = true;
+    }
+    T next = 0;
+    bool next_valid = false;
+    for (int j = g.my - 2; j >= 1; j--) {
+        const long long q = ((long long)k * g.my + j) * g.mx + i;
+        if (!row[q]) { xout[q] = 0; next_valid = false; continue; }
+        T xl = dpw[q] - (next_valid ? cpw[q] * next : T(0));
+        next = xl;
+        next_valid = true;
+        xout[q] = xin[q] + omega * (xl - xin[q]);
+    }
+}
+
+// coarse b = sum of the eight child residuals
+template <typename T>
+__global__ void k_restrict(Grid gc, Grid gf, const unsigned char *__restrict__ rowc, const unsigned char *__restrict__ rowf,
+                           const T *__restrict__ rf, T *__restrict__ bc)
+{
+    long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (t >= gc.n) return;
+    if (!rowc[t]) { bc[t] = 0; return; }
+    int i = t % gc.mx; long long r = t / gc.mx; int j = r % gc.my, k = r / gc.my;
+    T s = 0;
+    for (int dk = 0; dk < 2; dk++)
+        for (int dj = 0; dj < 2; dj++)
+            for (int di = 0; di < 2; di++) {
+                long long qf = gidx(gf, 2 * k - 1 + dk, 2 * j - 1 + dj, 2 * i - 1 + di);
+                if (rowf[qf]) s += rf[qf];
+            }
+    bc[t] = s;
+}
+
+__device__ inline int parent_nb(int I, int d, int mc, int per)
+{
+    int J = I + d;
+    if (J < 1) J = per ? mc - 2 : 1;
+    else if (J > mc - 2) J = per ? 1 : mc - 2;
+    return J;
+}
+
+// fine x += trilinear(coarse x)
+template <typename T>
+__global__ void k_prolong(Grid gc, Grid gf, const unsigned char *__restrict__ rowf, const T *__restrict__ xc,
+                          T *__restrict__ xf)
+{
+    long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (t >= gf.n) return;
+    if (!rowf[t]) return;
+    int i = t % gf.mx; long long r = t / gf.mx; int j = r % gf.my, k = r / gf.my;
+    int I = (i + 1) / 2, J = (j + 1) / 2, K = (k + 1) / 2;
+    int I2 = parent_nb(I, (i & 1) ? -1 : 1, gc.mx, gc.pi);
+    int J2 = parent_nb(J, (j & 1) ? -1 : 1, gc.my, gc.pj);
+    int K2 = parent_nb(K, (k & 1) ? -1 : 1, gc.mz, gc.pk);
+#define XC(kk, jj, ii) xc[((long long)(kk) * gc.my + (jj)) * gc.mx + (ii)]
+    T v = T(27) * XC(K, J, I) + T(9) * (XC(K, J, I2) + XC(K, J2, I) + XC(K2, J, I)) +
+          T(3) * (XC(K, J2, I2) + XC(K2, J, I2) + XC(K2, J2, I)) + XC(K2, J2, I2);
+#undef XC
+    xf[t] += v / T(64);
+}
+
+// coarse node coordinates: every other fine node (init.c:402-455)
+__global__ void k_coarse_coor(Grid gc, Grid gf, const double *__restrict__ cf, double *__restrict__ cc)
+{
+    long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (t >= gc.n) return;
+    int i = t % gc.mx; long long r = t / gc.mx; int j = r % gc.my, k = r / gc.my;
+    if (i >= gc.mx - 1 || j >= gc.my - 1 || k >= gc.mz - 1) {
+        cc[3 * t] = cc[3 * t + 1] = cc[3 * t + 2] = 0;
+        return;
+    }
+    long long s = ((long long)(2 * k) * gf.my + 2 * j) * gf.mx + 2 * i;
+    cc[3 * t] = cf[3 * s]; cc[3 * t + 1] = cf[3 * s + 1]; cc[3 * t + 2] = cf[3 * s + 2];
+}
+
+// a coarse cell is blanked only when all eight children are
+__global__ void k_coarse_nvert(Grid gc, Grid gf, const double *__restrict__ nf, double *__restrict__ nc)
+{
+    long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (t >= gc.n) return;
+    int i = t % gc.mx; long long r = t / gc.mx; int j = r % gc.my, k = r / gc.my;
+    if (i < 1 || i > gc.mx - 2 || j < 1 || j > gc.my - 2 || k < 1 || k > gc.mz - 2) { nc[t] = 0; return; }
+    double mn = 1e30;
+    for (int dk = 0; dk < 2; dk++)
+        for (int dj = 0; dj < 2; dj++)
+            for (int di = 0; di < 2; di++) mn = fmin(mn, nf[gidx(gf, 2 * k - 1 + dk, 2 * j - 1 + dj, 2 * i - 1 + di)]);
+    nc[t] = mn;
+}
+
+// ---- vector kernels --------------------------------------------------------
+template <typename A, typename B>
+__global__ void k_convert(long long n, const A *__restrict__ x, B *__restrict__ y)
+{
+    long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (t < n) y[t] = (B)x[t];
+}
+
+template <typename T>
+__global__ void k_zero(long long n, T *__restrict__ x)
+{
+    long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (t < n) x[t] = 0;
+}
+
+// y = a * x   (a read from device memory so no host round trip is needed)
+__global__ void k_scale_dev(long long n, const double *__restrict__ a, int inv, const double *__restrict__ x,
+                            double *__restrict__ y)
+{
+    long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    double s = inv ? 1.0 / sqrt(a[0]) : a[0];
+    if (t < n) y[t] = s * x[t];
+}
+
+constexpr int MAXV = 33;
+
+// dots[v] = <w, V_v> for v < nv and dots[nv] = <w, w>
+__global__ void __launch_bounds__(RED_THREADS) k_multidot(long long n, int nv, const double *__restrict__ V, long long ld,
+                                                          const double *__restrict__ w, double *__restrict__ part)
+{
+    double acc[MAXV];
+    for (int v = 0; v <= nv; v++) acc[v] = 0;
+    for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < n; t += (long long)gridDim.x * blockDim.x) {
+        double wt = w[t];
+        for (int v = 0; v < nv; v++) acc[v] += V[v * ld + t] * wt;
+        acc[nv] += wt * wt;
+    }
+    for (int v = 0; v <= nv; v++) {
+        double s = block_reduce<OpSum>(acc[v]);
+        if (threadIdx.x == 0) part[v * RED_BLOCKS + blockIdx.x] = s;
+    }
+}
+
+__global__ void k_finish_many(int nv, const double *__restrict__ part, double *__restrict__ out)
+{
+    // one block per vector
+    int v = blockIdx.x;
+    if (v > nv) return;
+    double s = 0;
+    for (int q = threadIdx.x; q < RED_BLOCKS; q += blockDim.x) s += part[v * RED_BLOCKS + q];
+    s = block_reduce<OpSum>(s);
+    if (threadIdx.x == 0) out[v] = s;
+}
+
+// w = w - sum_v h[v] V_v
+__global__ void k_multiaxpy(long long n, int nv, const double *__restrict__ h, const double *__restrict__ V, long long ld,
+                            double *__restrict__ w)
+{
+    long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    double s = w[t];
+    for (int v = 0; v < nv; v++) s -= h[v] * V[v * ld + t];
+    w[t] = s;
+}
+
+// x = x + sum_v y[v] Z_v
+__global__ void k_multiupdate(long long n, int nv, const double *__restrict__ y, const double *__restrict__ Z, long long ld,
+                              double *__restrict__ x)
+{
+    long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    double s = x[t];
+    for (int v = 0; v < nv; v++) s += y[v] * Z[v * ld + t];
+    x[t] = s;
+}
+
+// sum and count of the rows of x (plain mean, poisson_hypre.c:202-263)
+__global__ void k_row_sum(long long n, const unsigned char *__restrict__ row, const double *__restrict__ x,
+                          double *__restrict__ part)
+{
+    double s = 0, c = 0;
+    for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < n; t += (long long)gridDim.x * blockDim.x)
+        if (row[t]) { s += x[t]; c += 1; }
+    s = block_reduce<OpSum>(s);
+    c = block_reduce<OpSum>(c);
+    if (threadIdx.x == 0) { part[blockIdx.x] = s; part[RED_BLOCKS + blockIdx.x] = c; }
+}
+
+__global__ void k_sub_mean(long long n, const unsigned char *__restrict__ row, const double *__restrict__ sc,
+                           double *__restrict__ x)
+{
+    long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (t >= n || !row[t]) return;
+    double val = -sc[0] / sc[1];
+    x[t] += val;
+}
+
+}  // namespace vfs
+
+// ---------------------------------------------------------------------------
+// solver state
+// ---------------------------------------------------------------------------
+
+struct PLevel {
+    vfs::Grid g;
+    double *coor = nullptr;
+    double *met[17] = {nullptr};
+    double *nvert = nullptr;
+    unsigned char *row = nullptr;
+    double *c64 = nullptr;
+    float *c32 = nullptr;
+    // work vectors
+    float *x32 = nullptr, *y32 = nullptr, *b32 = nullptr, *r32 = nullptr, *cp32 = nullptr, *dp32 = nullptr;
+    double *x64 = nullptr, *y64 = nullptr, *b64 = nullptr, *r64 = nullptr, *cp64 = nullptr, *dp64 = nullptr;
+};
+
+struct vfs_poisson {
+    int nl = 0;
+    int restart = 0;
+    PLevel L[8];
+    double *V = nullptr, *Z = nullptr, *w = nullptr, *dev_scalars = nullptr;
+    std::vector<void *> allocs;
+};
+
+namespace vfs {
+
+static int palloc(vfs_poisson *ps, void **p, size_t bytes)
+{
+    hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess) return check_hip(e, "hipMalloc (poisson)");
+    ps->allocs.push_back(*p);
+    return VFS_OK;
+}
+
+void poisson_free(vfs_ctx *c)
+{
+    if (!c || !c->ps) return;
+    hipStreamSynchronize(c->stream);
+    for (void *p : c->ps->allocs) hipFree(p);
+    delete c->ps;
+    c->ps = nullptr;
+}
+
+static int level_alloc_work(vfs_poisson *ps, PLevel &L)
+{
+    size_t n = (size_t)L.g.n;
+    int rc = 0;
+    rc = rc ? rc : palloc(ps, (void **)&L.x32, n * 4);
+    rc = rc ? rc : palloc(ps, (void **)&L.y32, n * 4);
+    rc = rc ? rc : palloc(ps, (void **)&L.b32, n * 4);
+    rc = rc ? rc : palloc(ps, (void **)&L.r32, n * 4);
+    rc = rc ? rc : palloc(ps, (void **)&L.cp32, n * 4);
+    rc = rc ? rc : palloc(ps, (void **)&L.dp32, n * 4);
+    rc = rc ? rc : palloc(ps, (void **)&L.x64, n * 8);
+    rc = rc ? rc : palloc(ps, (void **)&L.y64, n * 8);
+    rc = rc ? rc : palloc(ps, (void **)&L.b64, n * 8);
+    rc = rc ? rc : palloc(ps, (void **)&L.r64, n * 8);
+    rc = rc ? rc : palloc(ps, (void **)&L.cp64, n * 8);
+    rc = rc ? rc : palloc(ps, (void **)&L.dp64, n * 8);
+    rc = rc ? rc : palloc(ps, (void **)&L.row, n);
+    rc = rc ? rc : palloc(ps, (void **)&L.c32, NST * n * 4);
+    rc = rc ? rc : palloc(ps, (void **)&L.c64, NST * n * 8);
+    return rc;
+}
+
+static int level_assemble(vfs_ctx *c, PLevel &L, double *const met[17], const double *nvert)
+{
+    hipStream_t st = c->stream;
+    FaceSet F = {met[4], met[5], met[6], met[7], met[8], met[9], met[10], met[11], met[12], met[13], met[14], met[15]};
+    const double th = c->p.poisson_threshold;
+    hipLaunchKernelGGL(k_row_mask, dim3(blocks_for(L.g.n, PTPB)), dim3(PTPB), 0, st, L.g, nvert, th, L.row);
+    hipLaunchKernelGGL(k_assemble, dim3(blocks_for(L.g.n, PTPB)), dim3(PTPB), 0, st, L.g, F, nvert, th, L.row, L.c64, L.c32);
+    VFS_LAUNCH_CHECK("assemble");
+    return VFS_OK;
+}
+
+static int poisson_setup(vfs_ctx *c, int levels)
+{
+    if (c->ps && c->ps->nl == levels) return VFS_OK;
+    poisson_free(c);
+    vfs_poisson *ps = new vfs_poisson();
+    c->ps = ps;
+    hipStream_t st = c->stream;
+    int rc = 0;
+    // how many levels fit: every coarse cell count must halve exactly
+    int nl = 1;
+    {
+        int cx = c->g.mx - 2, cy = c->g.my - 2, cz = c->g.mz - 2;
+        while (nl < levels && cx % 2 == 0 && cy % 2 == 0 && cz % 2 == 0 && cx >= 4 && cy >= 4 && cz >= 4) {
+            cx /= 2; cy /= 2; cz /= 2;
+            nl++;
+        }
+    }
+    ps->nl = nl;
+    ps->L[0].g = c->g;
+    rc = level_alloc_work(ps, ps->L[0]);
+    if (rc) return rc;
+    double *const m0[17] = {c->f[VFS_CSI], c->f[VFS_ETA], c->f[VFS_ZET], c->f[VFS_AJ], c->f[VFS_ICSI], c->f[VFS_IETA],
+                            c->f[VFS_IZET], c->f[VFS_IAJ], c->f[VFS_JCSI], c->f[VFS_JETA], c->f[VFS_JZET], c->f[VFS_JAJ],
+                            c->f[VFS_KCSI], c->f[VFS_KETA], c->f[VFS_KZET], c->f[VFS_KAJ], c->f[VFS_CENT]};
+    rc = level_assemble(c, ps->L[0], m0, c->f[VFS_NVERT]);
+    if (rc) return rc;
+    for (int l = 1; l < nl; l++) {
+        PLevel &Lf = ps->L[l - 1], &Lc = ps->L[l];
+        vfs_params pc = c->p;
+        pc.mx = (Lf.g.mx - 1 + 1) / 2 + 1;
+        pc.my = (Lf.g.my - 1 + 1) / 2 + 1;
+        pc.mz = (Lf.g.mz - 1 + 1) / 2 + 1;
+        Lc.g = make_grid(pc);
+        size_t n = (size_t)Lc.g.n;
+        rc = palloc(ps, (void **)&Lc.coor, 3 * n * 8);
+        for (int q = 0; q < 17 && !rc; q++) {
+            bool vec = !(q == 3 || q == 7 || q == 11 || q == 15);
+            rc = palloc(ps, (void **)&Lc.met[q], (vec ? 3 : 1) * n * 8);
+        }
+        if (!rc) rc = palloc(ps, (void **)&Lc.nvert, n * 8);
+        if (!rc) rc = level_alloc_work(ps, Lc);
+        if (rc) return rc;
+        const double *fcoor = l == 1 ? c->f[VFS_COOR] : Lf.coor;
+        const double *fnv = l == 1 ? c->f[VFS_NVERT] : Lf.nvert;
+        hipLaunchKernelGGL(k_coarse_coor, dim3(blocks_for(Lc.g.n, PTPB)), dim3(PTPB), 0, st, Lc.g, Lf.g, fcoor, Lc.coor);
+        hipLaunchKernelGGL(k_coarse_nvert, dim3(blocks_for(Lc.g.n, PTPB)), dim3(PTPB), 0, st, Lc.g, Lf.g, fnv, Lc.nvert);
+        VFS_LAUNCH_CHECK("coarse grid");
+        rc = launch_metrics(c, Lc.g, Lc.coor, Lc.met);
+        if (!rc) rc = level_assemble(c, Lc, Lc.met, Lc.nvert);
+        if (rc) return rc;
+    }
+    return VFS_OK;
+}
+
+static int ensure_krylov(vfs_ctx *c, int restart)
+{
+    vfs_poisson *ps = c->ps;
+    if (ps->restart >= restart) return VFS_OK;
+    size_t n = (size_t)c->g.n;
+    int rc = palloc(ps, (void **)&ps->V, (size_t)(restart + 1) * n * 8);
+    if (!rc) rc = palloc(ps, (void **)&ps->Z, (size_t)restart * n * 8);
+    if (!rc) rc = palloc(ps, (void **)&ps->w, n * 8);
+    if (!rc) rc = palloc(ps, (void **)&ps->dev_scalars, 4 * MAXV * 8);
+    if (!rc) ps->restart = restart;
+    return rc;
+}
+
+// ---- V-cycle ---------------------------------------------------------------
+template <typename T>
+struct LV;
+template <>
+struct LV<float> {
+    static float *x(PLevel &L) { return L.x32; }
+    static float *y(PLevel &L) { return L.y32; }
+    static float *b(PLevel &L) { return L.b32; }
+    static float *r(PLevel &L) { return L.r32; }
+    static float *cp(PLevel &L) { return L.cp32; }
+    static float *dp(PLevel &L) { return L.dp32; }
+    static const float *coef(PLevel &L) { return L.c32; }
+};
+template <>
+struct LV<double> {
+    static double *x(PLevel &L) { return L.x64; }
+    static double *y(PLevel &L) { return L.y64; }
+    static double *b(PLevel &L) { return L.b64; }
+    static double *r(PLevel &L) { return L.r64; }
+    static double *cp(PLevel &L) { return L.cp64; }
+    static double *dp(PLevel &L) { return L.dp64; }
+    static const double *coef(PLevel &L) { return L.c64; }
+};
+
+constexpr double LINE_OMEGA = 0.8;
+
+template <typename T>
+static void smooth(vfs_ctx *c, PLevel &L, int sweeps)
+{
+    hipStream_t st = c->stream;
+    const long long nl = (long long)(L.g.mx - 2) * (L.g.mz - 2);
+    const bool fine = L.g.n == c->g.n;
+    for (int s = 0; s < sweeps; s++) {
+        if (fine) prof_begin(c, sizeof(T) == 4 ? K_LINE32 : K_LINE64);
+        hipLaunchKernelGGL(k_line_jacobi<T>, dim3(blocks_for(nl, PTPB)), dim3(PTPB), 0, st, L.g, LV<T>::coef(L), L.row,
+                           LV<T>::x(L), LV<T>::b(L), LV<T>::y(L), LV<T>::cp(L), LV<T>::dp(L), T(LINE_OMEGA));
+        if (fine) prof_end(c);
+        // the sweep wrote y; swap the roles of x and y
+        if constexpr (sizeof(T) == 4) std::swap(L.x32, L.y32);
+        else std::swap(L.x64, L.y64);
+    }
+}
+
+template <typename T>
+static void vcycle(vfs_ctx *c, vfs_poisson *ps, int l, int pre, int post, int coarse)
+{
+    hipStream_t st = c->stream;
+    PLevel &L = ps->L[l];
+    const unsigned nb = blocks_for(L.g.n, PTPB);
+    if (l == ps->nl - 1) {
+        smooth<T>(c, L, coarse);
+        return;
+    }
+    smooth<T>(c, L, pre);
+    hipLaunchKernelGGL(k_apply<T>, dim3(nb), dim3(PTPB), 0, st, L.g, LV<T>::coef(L), L.row, LV<T>::x(L), LV<T>::b(L), LV<T>::r(L));
+    PLevel &C = ps->L[l + 1];
+    hipLaunchKernelGGL(k_restrict<T>, dim3(blocks_for(C.g.n, PTPB)), dim3(PTPB), 0, st, C.g, L.g, C.row, L.row, LV<T>::r(L), LV<T>::b(C));
+    hipLaunchKernelGGL(k_zero<T>, dim3(blocks_for(C.g.n, PTPB)), dim3(PTPB), 0, st, C.g.n, LV<T>::x(C));
+    vcycle<T>(c, ps, l + 1, pre, post, coarse);
+    hipLaunchKernelGGL(k_prolong<T>, dim3(nb), dim3(PTPB), 0, st, C.g, L.g, L.row, LV<T>::x(C), LV<T>::x(L));
+    smooth<T>(c, L, post);
+}
+
+// z = M^{-1} v (V-cycle from zero); FP32 or FP64 hierarchy
+static int precondition(vfs_ctx *c, const vfs_solve_opts *o, const double *v, double *z)
+{
+    vfs_poisson *ps = c->ps;
+    hipStream_t st = c->stream;
+    PLevel &L = ps->L[0];
+    const long long n = L.g.n;
+    const unsigned nb = blocks_for(n, PTPB);
+    if (o->mg_fp32) {
+        hipLaunchKernelGGL((k_convert<double, float>), dim3(nb), dim3(PTPB), 0, st, n, v, L.b32);
+        hipLaunchKernelGGL(k_zero<float>, dim3(nb), dim3(PTPB), 0, st, n, L.x32);
+        vcycle<float>(c, ps, 0, o->pre, o->post, o->coarse_sweeps);
+        hipLaunchKernelGGL((k_convert<float, double>), dim3(nb), dim3(PTPB), 0, st, n, L.x32, z);
+    } else {
+        hipMemcpyAsync(L.b64, v, n * 8, hipMemcpyDeviceToDevice, st);
+        hipLaunchKernelGGL(k_zero<double>, dim3(nb), dim3(PTPB), 0, st, n, L.x64);
+        vcycle<double>(c, ps, 0, o->pre, o->post, o->coarse_sweeps);
+        hipMemcpyAsync(z, L.x64, n * 8, hipMemcpyDeviceToDevice, st);
+    }
+    VFS_LAUNCH_CHECK("v-cycle");
+    return VFS_OK;
+}
+
+// dots of w with V_0..V_{nv-1} plus <w,w>, returned on the host
+static int multidot(vfs_ctx *c, int nv, const double *V, const double *w, double *out)
+{
+    hipStream_t st = c->stream;
+    const long long n = c->g.n;
+    hipLaunchKernelGGL(k_multidot, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, st, n, nv, V, n, w, c->red);
+    hipLaunchKernelGGL(k_finish_many, dim3(nv + 1), dim3(RED_THREADS), 0, st, nv, c->red, c->red + MAXV * RED_BLOCKS);
+    VFS_LAUNCH_CHECK("multidot");
+    VFS_HIP(hipMemcpyAsync(c->red_host, c->red + MAXV * RED_BLOCKS, (nv + 1) * sizeof(double), hipMemcpyDeviceToHost, st));
+    VFS_HIP(hipStreamSynchronize(st));
+    std::memcpy(out, c->red_host, (nv + 1) * sizeof(double));
+    return VFS_OK;
+}
+
+static int remove_mean(vfs_ctx *c, const unsigned char *row, double *x)
+{
+    hipStream_t st = c->stream;
+    const long long n = c->g.n;
+    hipLaunchKernelGGL(k_row_sum, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, st, n, row, x, c->red);
+    hipLaunchKernelGGL(finish_reduce<OpSum>, dim3(1), dim3(RED_THREADS), 0, st, c->red, RED_BLOCKS, c->red + 2 * RED_BLOCKS);
+    hipLaunchKernelGGL(finish_reduce<OpSum>, dim3(1), dim3(RED_THREADS), 0, st, c->red + RED_BLOCKS, RED_BLOCKS,
+                       c->red + 2 * RED_BLOCKS + 1);
+    hipLaunchKernelGGL(k_sub_mean, dim3(blocks_for(n, PTPB)), dim3(PTPB), 0, st, n, row, c->red + 2 * RED_BLOCKS, x);
+    VFS_LAUNCH_CHECK("remove mean");
+    return VFS_OK;
+}
+
+int poisson_apply(vfs_ctx *c, const double *x, double *y)
+{
+    int rc = poisson_setup(c, c->ps ? c->ps->nl : 4);
+    if (rc) return rc;
+    PLevel &L = c->ps->L[0];
+    hipLaunchKernelGGL(k_apply<double>, dim3(blocks_for(L.g.n, PTPB)), dim3(PTPB), 0, c->stream, L.g, L.c64, L.row, x,
+                       (const double *)nullptr, y);
+    VFS_LAUNCH_CHECK("poisson apply");
+    VFS_HIP(hipStreamSynchronize(c->stream));
+    return VFS_OK;
+}
+
+int poisson_solve(vfs_ctx *c, const vfs_solve_opts *o, vfs_solve_stats *stats)
+{
+    using clk = std::chrono::steady_clock;
+    auto t0 = clk::now();
+    const int m = o->restart > 0 ? (o->restart < MAXV - 1 ? o->restart : MAXV - 2) : 20;
+    int rc = poisson_setup(c, o->levels > 0 ? o->levels : 4);
+    if (!rc) rc = ensure_krylov(c, m);
+    if (rc) return rc;
+    vfs_poisson *ps = c->ps;
+    hipStream_t st = c->stream;
+    PLevel &L = ps->L[0];
+    const long long n = L.g.n;
+    const unsigned nb = blocks_for(n, PTPB);
+    double *x = c->f[VFS_PHI], *b = c->f[VFS_B];
+    double *V = ps->V, *Z = ps->Z, *w = ps->w;
+    if (!o->use_guess) hipLaunchKernelGGL(k_zero<double>, dim3(nb), dim3(PTPB), 0, st, n, x);
+    double d[MAXV + 1];
+    // ||b||: multidot with no basis vectors returns <b,b>
+    rc = multidot(c, 0, V, b, d);
+    if (rc) return rc;
+    double bnorm = std::sqrt(d[0]);
+    if (bnorm == 0) bnorm = 1;
+    std::vector<double> H((size_t)(m + 1) * m), cs(m), sn(m), gv(m + 1), y(m);
+    int it = 0;
+    double rel = 1;
+    const double tol = o->tol > 0 ? o->tol : 1e-8;
+    const int maxit = o->maxit > 0 ? o->maxit : 200;
+    bool converged = false;
+    while (it < maxit) {
+        // r = b - A x into V_0
+        hipLaunchKernelGGL(k_apply<double>, dim3(nb), dim3(PTPB), 0, st, L.g, L.c64, L.row, x, b, V);
+        rc = multidot(c, 0, V, V, d);
+        if (rc) return rc;
+        double beta = std::sqrt(d[0]);
+        rel = beta / bnorm;
+        if (rel <= tol) { converged = true; break; }
+        c->red_host[0] = 1.0 / beta;
+        VFS_HIP(hipMemcpyAsync(ps->dev_scalars, c->red_host, sizeof(double), hipMemcpyHostToDevice, st));
+        hipLaunchKernelGGL(k_scale_dev, dim3(nb), dim3(PTPB), 0, st, n, ps->dev_scalars, 0, V, V);
+        std::fill(gv.begin(), gv.end(), 0.0);
+        gv[0] = beta;
+        int jj;
+        for (jj = 0; jj < m && it < maxit; jj++, it++) {
+            double *vj = V + (size_t)jj * n, *zj = Z + (size_t)jj * n;
+            rc = precondition(c, o, vj, zj);
+            if (rc) return rc;
+            prof_begin(c, K_APPLY64);
+            hipLaunchKernelGGL(k_apply<double>, dim3(nb), dim3(PTPB), 0, st, L.g, L.c64, L.row, zj, (const double *)nullptr, w);
+            prof_end(c);
+            // classical Gram-Schmidt, re-orthogonalised when the norm drops
+            rc = multidot(c, jj + 1, V, w, d);
+            if (rc) return rc;
+            double h[MAXV];
+            for (int v = 0; v <= jj; v++) h[v] = d[v];
+            double ww0 = d[jj + 1];
+            std::memcpy(c->red_host, h, (jj + 1) * sizeof(double));
+            VFS_HIP(hipMemcpyAsync(ps->dev_scalars, c->red_host, (jj + 1) * sizeof(double), hipMemcpyHostToDevice, st));
+            hipLaunchKernelGGL(k_multiaxpy, dim3(nb), dim3(PTPB), 0, st, n, jj + 1, ps->dev_scalars, V, n, w);
+            rc = multidot(c, jj + 1, V, w, d);
+            if (rc) return rc;
+            double ww = d[jj + 1];
+            if (ww < 0.25 * ww0) {
+                for (int v = 0; v <= jj; v++) h[v] += d[v];
+                std::memcpy(c->red_host, d, (jj + 1) * sizeof(double));
+                VFS_HIP(hipMemcpyAsync(ps->dev_scalars, c->red_host, (jj + 1) * sizeof(double), hipMemcpyHostToDevice, st));
+                hipLaunchKernelGGL(k_multiaxpy, dim3(nb), dim3(PTPB), 0, st, n, jj + 1, ps->dev_scalars, V, n, w);
+                double dd2[MAXV + 1];
+                rc = multidot(c, 0, V, w, dd2);
+                if (rc) return rc;
+                ww = dd2[0];
+            }
+            double hn = std::sqrt(ww > 0 ? ww : 0);
+            for (int v = 0; v <= jj; v++) H[(size_t)v * m + jj] = h[v];
+            H[(size_t)(jj + 1) * m + jj] = hn;
+            c->red_host[0] = hn > 0 ? 1.0 / hn : 0.0;
+            VFS_HIP(hipMemcpyAsync(ps->dev_scalars, c->red_host, sizeof(double), hipMemcpyHostToDevice, st));
+            hipLaunchKernelGGL(k_scale_dev, dim3(nb), dim3(PTPB), 0, st, n, ps->dev_scalars, 0, w, V + (size_t)(jj + 1) * n);
+            for (int i = 0; i < jj; i++) {
+                double tmp = cs[i] * H[(size_t)i * m + jj] + sn[i] * H[(size_t)(i + 1) * m + jj];
+                H[(size_t)(i + 1) * m + jj] = -sn[i] * H[(size_t)i * m + jj] + cs[i] * H[(size_t)(i + 1) * m + jj];
+                H[(size_t)i * m + jj] = tmp;
+            }
+            double a = H[(size_t)jj * m + jj], bb = H[(size_t)(jj + 1) * m + jj];
+            double den = std::sqrt(a * a + bb * bb);
+            cs[jj] = a / den; sn[jj] = bb / den;
+            H[(size_t)jj * m + jj] = den;
+            H[(size_t)(jj + 1) * m + jj] = 0;
+            gv[jj + 1] = -sn[jj] * gv[jj];
+            gv[jj] = cs[jj] * gv[jj];
+            if (std::fabs(gv[jj + 1]) / bnorm <= tol) { jj++; it++; break; }
+        }
+        for (int i = jj - 1; i >= 0; i--) {
+            double s = gv[i];
+            for (int q = i + 1; q < jj; q++) s -= H[(size_t)i * m + q] * y[q];
+            y[i] = s / H[(size_t)i * m + i];
+        }
+        std::memcpy(c->red_host, y.data(), jj * sizeof(double));
+        VFS_HIP(hipMemcpyAsync(ps->dev_scalars, c->red_host, jj * sizeof(double), hipMemcpyHostToDevice, st));
+        hipLaunchKernelGGL(k_multiupdate, dim3(nb), dim3(PTPB), 0, st, n, jj, ps->dev_scalars, Z, n, x);
+        VFS_LAUNCH_CHECK("fgmres update");
+    }
+    if (!converged) {
+        hipLaunchKernelGGL(k_apply<double>, dim3(nb), dim3(PTPB), 0, st, L.g, L.c64, L.row, x, b, w);
+        rc = multidot(c, 0, V, w, d);
+        if (rc) return rc;
+        rel = std::sqrt(d[0]) / bnorm;
+        converged = rel <= tol;
+    }
+    rc = remove_mean(c, L.row, x);  // poisson_hypre.c:608
+    if (rc) return rc;
+    VFS_HIP(hipStreamSynchronize(st));
+    if (stats) {
+        stats->iterations = it;
+        stats->relres = rel;
+        stats->ms = std::chrono::duration<double, std::milli>(clk::now() - t0).count();
+        stats->fallback = 0;
+    }
+    if (!converged) return set_error(VFS_ENOCONV, "Poisson FGMRES did not reach the tolerance");
+    return VFS_OK;
+}
+
+}  // namespace vfs

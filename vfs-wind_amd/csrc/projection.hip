@@ -1,0 +1,296 @@
+// projection.hip -- Poisson right-hand side, pressure update, projection and
+// divergence check (poisson_hypre.c:118-407, poisson.c:98-186, 2700-3297,
+// main.c:936-1074) as HIP kernels for gfx950.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+
+#include "vfs_internal.h"
+#include "vfs_reduce.h"
+
+#pragma clang fp contract(off)
+
+namespace vfs {
+
+constexpr int QTPB = 256;
+int periodic_fix_grid(hipStream_t st, const Grid &g, double *f, int nc, int mode);
+
+// ---------------------------------------------------------------------------
+// a11: PoissonRHS2_hypre (poisson_hypre.c:265-407)
+// ---------------------------------------------------------------------------
+
+__global__ void k_poisson_rhs(Grid g, double fac, const double *__restrict__ ucont, const double *__restrict__ nvert,
+                              const double *__restrict__ aj, double *__restrict__ b, double *__restrict__ part)
+{
+    double sb = 0, sa = 0;
+    for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < g.n; t += (long long)gridDim.x * blockDim.x) {
+        const int i = t % g.mx;
+        const long long r = t / g.mx;
+        const int j = r % g.my, k = r / g.my;
+        double val = 0;
+        if (i >= 1 && i < g.mx - 1 && j >= 1 && j < g.my - 1 && k >= 1 && k < g.mz - 1 && nvert[t] < 0.1) {
+            val -= ucont[3 * t + 0];
+            if (i == 1 && g.pi) val += ucont[3 * gidx(g, k, j, -2) + 0];
+            else val += ucont[3 * gidx(g, k, j, i - 1) + 0];
+            val -= ucont[3 * t + 1];
+            if (j == 1 && g.pj) val += ucont[3 * gidx(g, k, -2, i) + 1];
+            else val += ucont[3 * gidx(g, k, j - 1, i) + 1];
+            val -= ucont[3 * t + 2];
+            if (k == 1 && g.pk) val += ucont[3 * gidx(g, -2, j, i) + 2];
+            else val += ucont[3 * gidx(g, k - 1, j, i) + 2];
+            val *= fac;
+            sb += val;
+            sa += aj[t];
+        }
+        b[t] = val;
+    }
+    sb = block_reduce<OpSum>(sb);
+    sa = block_reduce<OpSum>(sa);
+    if (threadIdx.x == 0) { part[blockIdx.x] = sb; part[RED_BLOCKS + blockIdx.x] = sa; }
+}
+
+// Remove_Nullspace_Scale, poisson_hypre.c:118-200: b -= sum(b) * aj / sum(aj)
+__global__ void k_null_scale(Grid g, const double *__restrict__ sums, const double *__restrict__ nvert,
+                             const double *__restrict__ aj, double *__restrict__ b)
+{
+    long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (t >= g.n) return;
+    const int i = t % g.mx;
+    const long long r = t / g.mx;
+    const int j = r % g.my, k = r / g.my;
+    if (i >= 1 && i < g.mx - 1 && j >= 1 && j < g.my - 1 && k >= 1 && k < g.mz - 1 && nvert[t] < 0.1)
+        b[t] += -sums[0] * aj[t] / sums[1];
+}
+
+int poisson_rhs(vfs_ctx *c)
+{
+    const Grid &g = c->g;
+    hipStream_t st = c->stream;
+    const double coeff = 1.;  // time_coeff(), poisson.c:23-31
+    const double fac = -1.0 / c->p.dt * c->p.st * coeff;
+    hipLaunchKernelGGL(k_poisson_rhs, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, st, g, fac, c->f[VFS_UCONT],
+                       c->f[VFS_NVERT], c->f[VFS_AJ], c->f[VFS_B], c->red);
+    hipLaunchKernelGGL(finish_reduce<OpSum>, dim3(1), dim3(RED_THREADS), 0, st, c->red, RED_BLOCKS, c->red + 2 * RED_BLOCKS);
+    hipLaunchKernelGGL(finish_reduce<OpSum>, dim3(1), dim3(RED_THREADS), 0, st, c->red + RED_BLOCKS, RED_BLOCKS,
+                       c->red + 2 * RED_BLOCKS + 1);
+    hipLaunchKernelGGL(k_null_scale, dim3(blocks_for(g.n, QTPB)), dim3(QTPB), 0, st, g, c->red + 2 * RED_BLOCKS,
+                       c->f[VFS_NVERT], c->f[VFS_AJ], c->f[VFS_B]);
+    VFS_LAUNCH_CHECK("poisson rhs");
+    return VFS_OK;
+}
+
+// ---------------------------------------------------------------------------
+// a13: Convert_Phi2_Phi + UpdatePressure (poisson.c:98-186, 3137-3297)
+// ---------------------------------------------------------------------------
+
+__global__ void k_phi_mask(Grid g, double th, const double *__restrict__ nvert, double *__restrict__ phi)
+{
+    long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (t >= g.n) return;
+    const int i = t % g.mx;
+    const long long r = t / g.mx;
+    const int j = r % g.my, k = r / g.my;
+    bool rowc = i >= 1 && i < g.mx - 1 && j >= 1 && j < g.my - 1 && k >= 1 && k < g.mz - 1 && nvert[t] < th;
+    if (!rowc) phi[t] = 0;
+}
+
+__global__ void k_update_p(Grid g, const double *__restrict__ nvert, const double *__restrict__ phi, double *__restrict__ p)
+{
+    long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (t >= g.n) return;
+    const int i = t % g.mx;
+    const long long r = t / g.mx;
+    const int j = r % g.my, k = r / g.my;
+    if (i < 1 || i >= g.mx - 1 || j < 1 || j >= g.my - 1 || k < 1 || k >= g.mz - 1) return;
+    if (nvert[t] < 0.1) p[t] += phi[t];
+    if (nvert[t] > 1.1) p[t] = 0;
+}
+
+int update_pressure(vfs_ctx *c)
+{
+    const Grid &g = c->g;
+    hipStream_t st = c->stream;
+    const unsigned nb = blocks_for(g.n, QTPB);
+    hipLaunchKernelGGL(k_phi_mask, dim3(nb), dim3(QTPB), 0, st, g, c->p.poisson_threshold, c->f[VFS_NVERT], c->f[VFS_PHI]);
+    VFS_LAUNCH_CHECK("phi mask");
+    int rc = periodic_fix_grid(st, g, c->f[VFS_PHI], 1, 0);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_update_p, dim3(nb), dim3(QTPB), 0, st, g, c->f[VFS_NVERT], c->f[VFS_PHI], c->f[VFS_P]);
+    VFS_LAUNCH_CHECK("update pressure");
+    return periodic_fix_grid(st, g, c->f[VFS_P], 1, 0);
+}
+
+// ---------------------------------------------------------------------------
+// a14: Projection (poisson.c:2700-3066)
+// ---------------------------------------------------------------------------
+
+struct ProjArgs {
+    Grid g;
+    double th, fac;  // poisson_threshold; dt*st/coeff
+    const double *phi, *nvert;
+    const double *icsi, *ieta, *izet, *iaj, *jcsi, *jeta, *jzet, *jaj, *kcsi, *keta, *kzet, *kaj;
+    double *ucont;
+};
+
+__global__ void __launch_bounds__(QTPB) k_projection(ProjArgs a)
+{
+    const Grid &g = a.g;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int j = blockIdx.y + 1, k = blockIdx.z + 1;
+    if (i >= g.mx - 1) return;
+    const int mx = g.mx, my = g.my, mz = g.mz, pi_ = g.pi, pj = g.pj, pk = g.pk;
+    const double th = a.th;
+    const long long q = gidx(g, k, j, i);
+#define PH(kk, jj, ii) a.phi[gidx(g, (kk), (jj), (ii))]
+#define NV(kk, jj, ii) a.nvert[gidx(g, (kk), (jj), (ii))]
+    double dpdc, dpde, dpdz;
+    if (i < mx - 2 || (pi_ && i == mx - 2)) {
+        dpdc = PH(k, j, i + 1) - PH(k, j, i);
+        if (i == mx - 2 && pi_) dpdc = PH(k, j, mx + 1) - PH(k, j, i);
+        dpde = 0.; dpdz = 0.;
+        if ((j == my - 2 && !pj) || NV(k, j + 1, i) + NV(k, j + 1, i + 1) > th) {
+            if (NV(k, j - 1, i) + NV(k, j - 1, i + 1) < th && (j != 1))
+                dpde = (PH(k, j, i) + PH(k, j, i + 1) - PH(k, j - 1, i) - PH(k, j - 1, i + 1)) * 0.5;
+        } else if ((j == 1 && !pj) || NV(k, j - 1, i) + NV(k, j - 1, i + 1) > th) {
+            if (NV(k, j + 1, i) + NV(k, j + 1, i + 1) < th)
+                dpde = (PH(k, j + 1, i) + PH(k, j + 1, i + 1) - PH(k, j, i) - PH(k, j, i + 1)) * 0.5;
+        } else
+            dpde = (PH(k, j + 1, i) + PH(k, j + 1, i + 1) - PH(k, j - 1, i) - PH(k, j - 1, i + 1)) * 0.25;
+        if ((k == mz - 2 && !pk) || NV(k + 1, j, i) + NV(k + 1, j, i + 1) > th) {
+            if (NV(k - 1, j, i) + NV(k - 1, j, i + 1) < th && (k != 1))
+                dpdz = (PH(k, j, i) + PH(k, j, i + 1) - PH(k - 1, j, i) - PH(k - 1, j, i + 1)) * 0.5;
+        } else if ((k == 1 && !pk) || NV(k - 1, j, i) + NV(k - 1, j, i + 1) > th) {
+            if (NV(k + 1, j, i) + NV(k + 1, j, i + 1) < th)
+                dpdz = (PH(k + 1, j, i) + PH(k + 1, j, i + 1) - PH(k, j, i) - PH(k, j, i + 1)) * 0.5;
+        } else
+            dpdz = (PH(k + 1, j, i) + PH(k + 1, j, i + 1) - PH(k - 1, j, i) - PH(k - 1, j, i + 1)) * 0.25;
+        if ((NV(k, j, i) + NV(k, j, i + 1)) < th) {
+            const double *ic = a.icsi + 3 * q, *ie = a.ieta + 3 * q, *iz = a.izet + 3 * q;
+            const double iaj = a.iaj[q];
+            a.ucont[3 * q + 0] -= (dpdc * (ic[0] * ic[0] + ic[1] * ic[1] + ic[2] * ic[2]) * iaj +
+                                   dpde * (ie[0] * ic[0] + ie[1] * ic[1] + ie[2] * ic[2]) * iaj +
+                                   dpdz * (iz[0] * ic[0] + iz[1] * ic[1] + iz[2] * ic[2]) * iaj) * a.fac;
+        }
+    }
+    if (j < my - 2 || (pj && j == my - 2)) {
+        dpdc = 0.; dpdz = 0.;
+        if ((i == mx - 2 && !pi_) || NV(k, j, i + 1) + NV(k, j + 1, i + 1) > th) {
+            if (NV(k, j, i - 1) + NV(k, j + 1, i - 1) < th && (i != 1))
+                dpdc = (PH(k, j, i) + PH(k, j + 1, i) - PH(k, j, i - 1) - PH(k, j + 1, i - 1)) * 0.5;
+        } else if ((i == 1 && !pi_) || NV(k, j, i - 1) + NV(k, j + 1, i - 1) > th) {
+            if (NV(k, j, i + 1) + NV(k, j + 1, i + 1) < th)
+                dpdc = (PH(k, j, i + 1) + PH(k, j + 1, i + 1) - PH(k, j, i) - PH(k, j + 1, i)) * 0.5;
+        } else
+            dpdc = (PH(k, j, i + 1) + PH(k, j + 1, i + 1) - PH(k, j, i - 1) - PH(k, j + 1, i - 1)) * 0.25;
+        if (j == my - 2 && pj) dpde = PH(k, my + 1, i) - PH(k, j, i);
+        else dpde = PH(k, j + 1, i) - PH(k, j, i);
+        if ((k == mz - 2 && !pk) || NV(k + 1, j, i) + NV(k + 1, j + 1, i) > th) {
+            if (NV(k - 1, j, i) + NV(k - 1, j + 1, i) < th && (k != 1))
+                dpdz = (PH(k, j, i) + PH(k, j + 1, i) - PH(k - 1, j, i) - PH(k - 1, j + 1, i)) * 0.5;
+        } else if ((k == 1 && !pk) || NV(k - 1, j, i) + NV(k - 1, j + 1, i) > th) {
+            if (NV(k + 1, j, i) + NV(k + 1, j + 1, i) < th)
+                dpdz = (PH(k + 1, j, i) + PH(k + 1, j + 1, i) - PH(k, j, i) - PH(k, j + 1, i)) * 0.5;
+        } else
+            dpdz = (PH(k + 1, j, i) + PH(k + 1, j + 1, i) - PH(k - 1, j, i) - PH(k - 1, j + 1, i)) * 0.25;
+        if ((NV(k, j, i) + NV(k, j + 1, i)) < th || (int)(NV(k, j, i) + NV(k, j + 1, i)) == 5) {
+            const double *jc = a.jcsi + 3 * q, *je = a.jeta + 3 * q, *jz = a.jzet + 3 * q;
+            const double jaj = a.jaj[q];
+            a.ucont[3 * q + 1] -= (dpdc * (jc[0] * je[0] + jc[1] * je[1] + jc[2] * je[2]) * jaj +
+                                   dpde * (je[0] * je[0] + je[1] * je[1] + je[2] * je[2]) * jaj +
+                                   dpdz * (jz[0] * je[0] + jz[1] * je[1] + jz[2] * je[2]) * jaj) * a.fac;
+        }
+    }
+    if (k < mz - 2 || (pk && k == mz - 2)) {
+        dpdc = 0.; dpde = 0.;
+        if ((i == mx - 2 && !pi_) || NV(k, j, i + 1) + NV(k + 1, j, i + 1) > th) {
+            if (NV(k, j, i - 1) + NV(k + 1, j, i - 1) < th && (i != 1))
+                dpdc = (PH(k, j, i) + PH(k + 1, j, i) - PH(k, j, i - 1) - PH(k + 1, j, i - 1)) * 0.5;
+        } else if ((i == 1 && !pi_) || NV(k, j, i - 1) + NV(k + 1, j, i - 1) > th) {
+            if (NV(k, j, i + 1) + NV(k + 1, j, i + 1) < th)
+                dpdc = (PH(k, j, i + 1) + PH(k + 1, j, i + 1) - PH(k, j, i) - PH(k + 1, j, i)) * 0.5;
+        } else
+            dpdc = (PH(k, j, i + 1) + PH(k + 1, j, i + 1) - PH(k, j, i - 1) - PH(k + 1, j, i - 1)) * 0.25;
+        if ((j == my - 2 && !pj) || NV(k, j + 1, i) + NV(k + 1, j + 1, i) > th) {
+            if (NV(k, j - 1, i) + NV(k + 1, j - 1, i) < th && (j != 1))
+                dpde = (PH(k, j, i) + PH(k + 1, j, i) - PH(k, j - 1, i) - PH(k + 1, j - 1, i)) * 0.5;
+        } else if ((j == 1 && !pj) || NV(k, j - 1, i) + NV(k + 1, j - 1, i) > th) {
+            if (NV(k, j + 1, i) + NV(k + 1, j + 1, i) < th)
+                dpde = (PH(k, j + 1, i) + PH(k + 1, j + 1, i) - PH(k, j, i) - PH(k + 1, j, i)) * 0.5;
+        } else
+            dpde = (PH(k, j + 1, i) + PH(k + 1, j + 1, i) - PH(k, j - 1, i) - PH(k + 1, j - 1, i)) * 0.25;
+        if (k == mz - 2 && pk) dpdz = PH(mz + 1, j, i) - PH(k, j, i);
+        else dpdz = PH(k + 1, j, i) - PH(k, j, i);
+        if ((NV(k, j, i) + NV(k + 1, j, i)) < th) {
+            const double *kc = a.kcsi + 3 * q, *ke = a.keta + 3 * q, *kz = a.kzet + 3 * q;
+            const double kaj = a.kaj[q];
+            a.ucont[3 * q + 2] -= (dpdc * (kc[0] * kz[0] + kc[1] * kz[1] + kc[2] * kz[2]) * kaj +
+                                   dpde * (ke[0] * kz[0] + ke[1] * kz[1] + ke[2] * kz[2]) * kaj +
+                                   dpdz * (kz[0] * kz[0] + kz[1] * kz[1] + kz[2] * kz[2]) * kaj) * a.fac;
+        }
+    }
+#undef PH
+#undef NV
+}
+
+int projection(vfs_ctx *c)
+{
+    const Grid &g = c->g;
+    hipStream_t st = c->stream;
+    ProjArgs a;
+    a.g = g;
+    a.th = c->p.poisson_threshold;
+    const double coeff = 1.;
+    a.fac = c->p.dt * c->p.st / coeff;
+    a.phi = c->f[VFS_PHI];
+    a.nvert = c->f[VFS_NVERT];
+    a.icsi = c->f[VFS_ICSI]; a.ieta = c->f[VFS_IETA]; a.izet = c->f[VFS_IZET]; a.iaj = c->f[VFS_IAJ];
+    a.jcsi = c->f[VFS_JCSI]; a.jeta = c->f[VFS_JETA]; a.jzet = c->f[VFS_JZET]; a.jaj = c->f[VFS_JAJ];
+    a.kcsi = c->f[VFS_KCSI]; a.keta = c->f[VFS_KETA]; a.kzet = c->f[VFS_KZET]; a.kaj = c->f[VFS_KAJ];
+    a.ucont = c->f[VFS_UCONT];
+    dim3 grd(blocks_for(g.mx - 2, QTPB), g.my - 2, g.mz - 2);
+    hipLaunchKernelGGL(k_projection, grd, dim3(QTPB), 0, st, a);
+    VFS_LAUNCH_CHECK("projection");
+    int rc = periodic_fix_grid(st, g, c->f[VFS_UCONT], 3, 1);  // poisson.c:3001-3040
+    if (rc) return rc;
+    return c2c(c);  // poisson.c:3061
+}
+
+// ---------------------------------------------------------------------------
+// a15: Divergence (main.c:936-1074)
+// ---------------------------------------------------------------------------
+
+__global__ void k_divergence(Grid g, const double *__restrict__ ucont, const double *__restrict__ aj,
+                             const double *__restrict__ nvert, double *__restrict__ part)
+{
+    const long long ni = (long long)(g.mx - 2) * (g.my - 2) * (g.mz - 2);
+    double v = 0;
+    for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < ni; t += (long long)gridDim.x * blockDim.x) {
+        int i = 1 + t % (g.mx - 2);
+        long long r = t / (g.mx - 2);
+        int j = 1 + r % (g.my - 2), k = 1 + r / (g.my - 2);
+        long long q = gidx(g, k, j, i);
+        double d = fabs((ucont[3 * q] - ucont[3 * gidx(g, k, j, i - 1)] + ucont[3 * q + 1] - ucont[3 * gidx(g, k, j - 1, i) + 1] +
+                         ucont[3 * q + 2] - ucont[3 * gidx(g, k - 1, j, i) + 2]) * aj[q]);
+        if (nvert[q] + nvert[gidx(g, k + 1, j, i)] + nvert[gidx(g, k - 1, j, i)] + nvert[gidx(g, k, j + 1, i)] +
+                nvert[gidx(g, k, j - 1, i)] + nvert[gidx(g, k, j, i + 1)] + nvert[gidx(g, k, j, i - 1)] > 0.1)
+            d = 0.;
+        v = fmax(v, d);
+    }
+    v = block_reduce<OpMax>(v);
+    if (threadIdx.x == 0) part[blockIdx.x] = v;
+}
+
+int divergence(vfs_ctx *c, double *out)
+{
+    hipStream_t st = c->stream;
+    hipLaunchKernelGGL(k_divergence, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, st, c->g, c->f[VFS_UCONT], c->f[VFS_AJ],
+                       c->f[VFS_NVERT], c->red);
+    hipLaunchKernelGGL(finish_reduce<OpMax>, dim3(1), dim3(RED_THREADS), 0, st, c->red, RED_BLOCKS, c->red + RED_BLOCKS);
+    VFS_LAUNCH_CHECK("divergence");
+    VFS_HIP(hipMemcpyAsync(c->red_host, c->red + RED_BLOCKS, sizeof(double), hipMemcpyDeviceToHost, st));
+    VFS_HIP(hipStreamSynchronize(st));
+    *out = c->red_host[0];
+    return VFS_OK;
+}
+
+}  // namespace vfs

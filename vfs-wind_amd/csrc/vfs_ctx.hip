@@ -1,0 +1,421 @@
+// vfs_ctx.hip -- context management and the C ABI entry points.
+//
+// The drivers here follow the reference's call order: RungeKutta
+// (rhs.c:1586-1708) and Flow_Solver (solvers.c:313-759).
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+
+#include "vfs_internal.h"
+
+namespace {
+thread_local std::string g_err;
+
+bool is_vector_field(int f)
+{
+    switch (f) {
+    case VFS_COOR: case VFS_CSI: case VFS_ETA: case VFS_ZET:
+    case VFS_ICSI: case VFS_IETA: case VFS_IZET:
+    case VFS_JCSI: case VFS_JETA: case VFS_JZET:
+    case VFS_KCSI: case VFS_KETA: case VFS_KZET:
+    case VFS_CENT: case VFS_UCONT: case VFS_UCAT: case VFS_UCAT_OLD:
+    case VFS_DP: case VFS_RHS: case VFS_FEUL:
+        return true;
+    default:
+        return false;
+    }
+}
+
+int validate(const vfs_params *p)
+{
+    if (!p) return vfs::set_error(VFS_EINVAL, "null params");
+    if (p->mx < 4 || p->my < 4 || p->mz < 4)
+        return vfs::set_error(VFS_EINVAL, "grid too small: each DA dimension needs at least 4 points");
+    if (p->les != 0 && p->les != 1)
+        return vfs::set_error(VFS_EINVAL, "only -les 0 and -les 1 are supported");
+    if (p->ren <= 0 || p->dt <= 0)
+        return vfs::set_error(VFS_EINVAL, "ren and dt must be positive");
+    if (p->per_j)
+        return vfs::set_error(VFS_EINVAL, "a periodic j direction is not supported (wall-normal line smoother)");
+    return VFS_OK;
+}
+}  // namespace
+
+namespace vfs {
+
+int set_error(int code, const std::string &msg)
+{
+    g_err = msg;
+    return code;
+}
+
+int check_hip(hipError_t e, const char *what)
+{
+    g_err = std::string(what) + ": " + hipGetErrorString(e);
+    return e == hipErrorOutOfMemory ? VFS_ENOMEM : VFS_EHIP;
+}
+
+void prof_begin(vfs_ctx *c, int kernel_id)
+{
+    if (!c->prof_on) return;
+    size_t at = c->ev_recs->size() * 2;
+    while (c->ev_pool->size() < at + 2) {
+        hipEvent_t ev;
+        if (hipEventCreate(&ev) != hipSuccess) { c->prof_on = 0; return; }
+        c->ev_pool->push_back(ev);
+    }
+    hipEventRecord((*c->ev_pool)[at], c->stream);
+    c->ev_recs->push_back({kernel_id, (int)at});
+}
+
+void prof_end(vfs_ctx *c)
+{
+    if (!c->prof_on || c->ev_recs->empty()) return;
+    hipEventRecord((*c->ev_pool)[c->ev_recs->back().second + 1], c->stream);
+}
+
+Grid make_grid(const vfs_params &p)
+{
+    Grid g;
+    g.mx = p.mx; g.my = p.my; g.mz = p.mz;
+    g.pi = p.per_i; g.pj = p.per_j; g.pk = p.per_k;
+    for (int q = 0; q < 6; q++) g.bc[q] = p.bctype[q];
+    g.plane = (long long)p.mx * p.my;
+    g.n = g.plane * p.mz;
+    return g;
+}
+
+}  // namespace vfs
+
+extern "C" {
+
+const char *vfs_last_error(void) { return g_err.c_str(); }
+
+int vfs_ctx_create(const vfs_params *params, vfs_ctx **out)
+{
+    if (!out) return vfs::set_error(VFS_EINVAL, "null output pointer");
+    *out = nullptr;
+    int rc = validate(params);
+    if (rc) return rc;
+    vfs_ctx *c = new vfs_ctx();
+    std::memset(c->f, 0, sizeof(c->f));
+    c->p = *params;
+    c->g = vfs::make_grid(*params);
+    c->face = c->pucont = c->snap = c->red = c->red_host = nullptr;
+    c->ps = nullptr;
+    c->mean_k_area = 0;
+    c->mean_k_flux = 0;
+    c->prof_on = 0;
+    c->ev_pool = new std::vector<hipEvent_t>();
+    c->ev_recs = new std::vector<std::pair<int, int>>();
+    hipError_t e = hipStreamCreate(&c->stream);
+    if (e != hipSuccess) { delete c; return vfs::check_hip(e, "hipStreamCreate"); }
+    const size_t n = (size_t)c->g.n;
+    auto alloc = [&](double **ptr, size_t count) -> int {
+        hipError_t ee = hipMalloc((void **)ptr, count * sizeof(double));
+        if (ee != hipSuccess) return vfs::check_hip(ee, "hipMalloc");
+        ee = hipMemsetAsync(*ptr, 0, count * sizeof(double), c->stream);
+        if (ee != hipSuccess) return vfs::check_hip(ee, "hipMemsetAsync");
+        return VFS_OK;
+    };
+    for (int f = 0; f < VFS_NFIELDS && !rc; f++) rc = alloc(&c->f[f], is_vector_field(f) ? 3 * n : n);
+    if (!rc) rc = alloc(&c->face, 18 * n);
+    if (!rc) rc = alloc(&c->pucont, 3 * n);
+    if (!rc) rc = alloc(&c->snap, 3 * n);
+    if (!rc) rc = alloc(&c->red, 1 << 16);
+    if (!rc) {
+        hipError_t ee = hipHostMalloc((void **)&c->red_host, 4096 * sizeof(double), hipHostMallocDefault);
+        if (ee != hipSuccess) rc = vfs::check_hip(ee, "hipHostMalloc");
+    }
+    if (!rc) {
+        hipError_t ee = hipStreamSynchronize(c->stream);
+        if (ee != hipSuccess) rc = vfs::check_hip(ee, "hipStreamSynchronize");
+    }
+    if (rc) { vfs_ctx_destroy(c); return rc; }
+    *out = c;
+    return VFS_OK;
+}
+
+int vfs_ctx_destroy(vfs_ctx *c)
+{
+    if (!c) return VFS_OK;
+    if (c->stream) hipStreamSynchronize(c->stream);
+    vfs::poisson_free(c);
+    for (int f = 0; f < VFS_NFIELDS; f++) if (c->f[f]) hipFree(c->f[f]);
+    if (c->face) hipFree(c->face);
+    if (c->pucont) hipFree(c->pucont);
+    if (c->snap) hipFree(c->snap);
+    if (c->red) hipFree(c->red);
+    if (c->red_host) hipHostFree(c->red_host);
+    if (c->ev_pool) {
+        for (hipEvent_t ev : *c->ev_pool) hipEventDestroy(ev);
+        delete c->ev_pool;
+    }
+    delete c->ev_recs;
+    if (c->stream) hipStreamDestroy(c->stream);
+    delete c;
+    return VFS_OK;
+}
+
+int vfs_set_params(vfs_ctx *c, const vfs_params *params)
+{
+    if (!c) return vfs::set_error(VFS_EINVAL, "null context");
+    int rc = validate(params);
+    if (rc) return rc;
+    if (params->mx != c->p.mx || params->my != c->p.my || params->mz != c->p.mz ||
+        params->per_i != c->p.per_i || params->per_j != c->p.per_j || params->per_k != c->p.per_k)
+        return vfs::set_error(VFS_EINVAL, "grid size and periodicity are fixed at creation");
+    c->p = *params;
+    c->g = vfs::make_grid(*params);
+    return VFS_OK;
+}
+
+int vfs_get_params(const vfs_ctx *c, vfs_params *params)
+{
+    if (!c || !params) return vfs::set_error(VFS_EINVAL, "null argument");
+    *params = c->p;
+    return VFS_OK;
+}
+
+size_t vfs_field_size(const vfs_ctx *c, vfs_field f)
+{
+    if (!c || f < 0 || f >= VFS_NFIELDS) return 0;
+    return (is_vector_field(f) ? 3 : 1) * (size_t)c->g.n;
+}
+
+int vfs_upload(vfs_ctx *c, vfs_field f, const double *host)
+{
+    if (!c || !host || f < 0 || f >= VFS_NFIELDS) return vfs::set_error(VFS_EINVAL, "bad upload argument");
+    VFS_HIP(hipMemcpyAsync(c->f[f], host, vfs_field_size(c, f) * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    VFS_HIP(hipStreamSynchronize(c->stream));
+    return VFS_OK;
+}
+
+int vfs_download(vfs_ctx *c, vfs_field f, double *host)
+{
+    if (!c || !host || f < 0 || f >= VFS_NFIELDS) return vfs::set_error(VFS_EINVAL, "bad download argument");
+    VFS_HIP(hipMemcpyAsync(host, c->f[f], vfs_field_size(c, f) * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    VFS_HIP(hipStreamSynchronize(c->stream));
+    return VFS_OK;
+}
+
+double *vfs_field_device_ptr(vfs_ctx *c, vfs_field f)
+{
+    if (!c || f < 0 || f >= VFS_NFIELDS) return nullptr;
+    return c->f[f];
+}
+
+int vfs_sync(vfs_ctx *c)
+{
+    if (!c) return vfs::set_error(VFS_EINVAL, "null context");
+    VFS_HIP(hipStreamSynchronize(c->stream));
+    return VFS_OK;
+}
+
+int vfs_form_metrics(vfs_ctx *c)
+{
+    if (!c) return vfs::set_error(VFS_EINVAL, "null context");
+    double *const m[17] = {c->f[VFS_CSI], c->f[VFS_ETA], c->f[VFS_ZET], c->f[VFS_AJ],
+                           c->f[VFS_ICSI], c->f[VFS_IETA], c->f[VFS_IZET], c->f[VFS_IAJ],
+                           c->f[VFS_JCSI], c->f[VFS_JETA], c->f[VFS_JZET], c->f[VFS_JAJ],
+                           c->f[VFS_KCSI], c->f[VFS_KETA], c->f[VFS_KZET], c->f[VFS_KAJ],
+                           c->f[VFS_CENT]};
+    int rc = vfs::launch_metrics(c, c->g, c->f[VFS_COOR], m);
+    if (rc) return rc;
+    // the multigrid hierarchy depends on the metrics; rebuild on next solve
+    vfs::poisson_free(c);
+    VFS_HIP(hipStreamSynchronize(c->stream));
+    return VFS_OK;
+}
+
+int vfs_mean_k_area(vfs_ctx *c, double *area)
+{
+    if (!c) return vfs::set_error(VFS_EINVAL, "null context");
+    double a = 0;
+    int rc = vfs::mean_k_area(c, &a);
+    if (rc) return rc;
+    c->mean_k_area = a;
+    if (area) *area = a;
+    return VFS_OK;
+}
+
+int vfs_contra2cart(vfs_ctx *c)
+{
+    if (!c) return vfs::set_error(VFS_EINVAL, "null context");
+    return vfs::c2c(c);
+}
+
+int vfs_form_bcs(vfs_ctx *c, int outflow_scale)
+{
+    if (!c) return vfs::set_error(VFS_EINVAL, "null context");
+    return vfs::form_bcs(c, outflow_scale);
+}
+
+int vfs_eddy_viscosity(vfs_ctx *c)
+{
+    if (!c) return vfs::set_error(VFS_EINVAL, "null context");
+    return vfs::eddy_viscosity(c);
+}
+
+int vfs_calc_k_flux(vfs_ctx *c, double *mean_k_flux)
+{
+    if (!c) return vfs::set_error(VFS_EINVAL, "null context");
+    double v = 0;
+    int rc = vfs::calc_k_flux(c, &v);
+    if (rc) return rc;
+    c->mean_k_flux = v;
+    if (mean_k_flux) *mean_k_flux = v;
+    return VFS_OK;
+}
+
+int vfs_pressure_gradient(vfs_ctx *c)
+{
+    if (!c) return vfs::set_error(VFS_EINVAL, "null context");
+    if (c->p.per_k && c->mean_k_area == 0) {
+        int rc = vfs_mean_k_area(c, nullptr);
+        if (rc) return rc;
+    }
+    return vfs::pressure_gradient(c);
+}
+
+int vfs_formfunction2(vfs_ctx *c, double scale)
+{
+    if (!c) return vfs::set_error(VFS_EINVAL, "null context");
+    return vfs::formfunction2(c, scale, nullptr, 0.0, nullptr);
+}
+
+int vfs_runge_kutta(vfs_ctx *c)
+{
+    if (!c) return vfs::set_error(VFS_EINVAL, "null context");
+    static const double alfa[4] = {0.25, 1 / 3., 0.5, 1.};
+    int rc = vfs::form_bcs(c, 0);
+    if (rc) return rc;
+    const size_t n3 = 3 * (size_t)c->g.n;
+    VFS_HIP(hipMemcpyAsync(c->pucont, c->f[VFS_UCONT], n3 * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    for (int s = 0; s < 4; s++) {
+        rc = vfs::c2c(c);
+        if (rc) return rc;
+        // Rhs = 0; Formfunction_2(Rhs, 1); Rhs -= dP; Ucont = a*Rhs + pUcont  (rhs.c:1647-1658)
+        rc = vfs::formfunction2(c, 1.0, c->pucont, alfa[s] * c->p.dt * c->p.st, c->f[VFS_UCONT]);
+        if (rc) return rc;
+        rc = vfs::c2c(c);
+        if (rc) return rc;
+    }
+    return vfs::form_bcs(c, 1);
+}
+
+int vfs_min_dt(vfs_ctx *c, double *dt_cfl1)
+{
+    if (!c || !dt_cfl1) return vfs::set_error(VFS_EINVAL, "null argument");
+    return vfs::min_dt(c, dt_cfl1);
+}
+
+int vfs_poisson_rhs(vfs_ctx *c)
+{
+    if (!c) return vfs::set_error(VFS_EINVAL, "null context");
+    return vfs::poisson_rhs(c);
+}
+
+int vfs_poisson_apply(vfs_ctx *c, const double *x, double *y)
+{
+    if (!c || !x || !y) return vfs::set_error(VFS_EINVAL, "null argument");
+    return vfs::poisson_apply(c, x, y);
+}
+
+int vfs_poisson_solve(vfs_ctx *c, const vfs_solve_opts *o, vfs_solve_stats *st)
+{
+    if (!c || !o) return vfs::set_error(VFS_EINVAL, "null argument");
+    return vfs::poisson_solve(c, o, st);
+}
+
+int vfs_update_pressure(vfs_ctx *c)
+{
+    if (!c) return vfs::set_error(VFS_EINVAL, "null context");
+    return vfs::update_pressure(c);
+}
+
+int vfs_projection(vfs_ctx *c)
+{
+    if (!c) return vfs::set_error(VFS_EINVAL, "null context");
+    return vfs::projection(c);
+}
+
+int vfs_divergence(vfs_ctx *c, double *maxdiv)
+{
+    if (!c || !maxdiv) return vfs::set_error(VFS_EINVAL, "null argument");
+    return vfs::divergence(c, maxdiv);
+}
+
+static const char *k_names[vfs::K_NUMK] = {"face_fluxes", "cell_rhs", "line_jacobi_fp32", "line_jacobi_fp64",
+                                           "poisson_apply_fp64", "contra2cart", "pressure_gradient", "eddy_viscosity"};
+
+int vfs_profile(vfs_ctx *c, int enable)
+{
+    if (!c) return vfs::set_error(VFS_EINVAL, "null context");
+    VFS_HIP(hipStreamSynchronize(c->stream));
+    c->ev_recs->clear();
+    c->prof_on = enable ? 1 : 0;
+    return VFS_OK;
+}
+
+int vfs_profile_read(vfs_ctx *c, int kernel, double *total_ms, long long *count, const char **name)
+{
+    if (!c || kernel < 0 || kernel >= vfs::K_NUMK) return vfs::set_error(VFS_EINVAL, "bad profile query");
+    VFS_HIP(hipStreamSynchronize(c->stream));
+    double tot = 0;
+    long long cnt = 0;
+    for (auto &r : *c->ev_recs) {
+        if (r.first != kernel) continue;
+        float ms = 0;
+        VFS_HIP(hipEventElapsedTime(&ms, (*c->ev_pool)[r.second], (*c->ev_pool)[r.second + 1]));
+        tot += ms;
+        cnt++;
+    }
+    if (total_ms) *total_ms = tot;
+    if (count) *count = cnt;
+    if (name) *name = k_names[kernel];
+    return VFS_OK;
+}
+
+int vfs_fractional_step(vfs_ctx *c, const vfs_solve_opts *o, vfs_step_stats *st)
+{
+    if (!c || !o) return vfs::set_error(VFS_EINVAL, "null argument");
+    using clk = std::chrono::steady_clock;
+    auto t0 = clk::now();
+    int rc = vfs_calc_k_flux(c, nullptr);                   // solvers.c:363
+    if (!rc && c->p.les) rc = vfs::c2c(c);                  // solvers.c:368
+    if (!rc && c->p.les) rc = vfs::eddy_viscosity(c);       // solvers.c:370
+    if (!rc) rc = vfs_pressure_gradient(c);                 // solvers.c:438
+    if (!rc) rc = vfs_runge_kutta(c);                       // solvers.c:635
+    if (rc) return rc;
+    VFS_HIP(hipStreamSynchronize(c->stream));
+    auto t1 = clk::now();
+    vfs_solve_stats ps;
+    std::memset(&ps, 0, sizeof(ps));
+    rc = vfs::poisson_rhs(c);                               // solvers.c:650-655
+    if (!rc) rc = vfs::poisson_solve(c, o, &ps);
+    if (rc && rc != VFS_ENOCONV) return rc;
+    int conv = rc;
+    auto t2 = clk::now();
+    rc = vfs::update_pressure(c);                           // solvers.c:662
+    if (!rc) rc = vfs::projection(c);                       // solvers.c:663
+    double md = 0;
+    if (!rc) rc = vfs::divergence(c, &md);                  // solvers.c:684
+    if (rc) return rc;
+    auto t3 = clk::now();
+    if (st) {
+        st->poisson_iterations = ps.iterations;
+        st->poisson_relres = ps.relres;
+        st->maxdiv = md;
+        st->ms_momentum = std::chrono::duration<double, std::milli>(t1 - t0).count();
+        st->ms_poisson = std::chrono::duration<double, std::milli>(t2 - t1).count();
+        st->ms_total = std::chrono::duration<double, std::milli>(t3 - t0).count();
+    }
+    return conv;
+}
+
+}  // extern "C"

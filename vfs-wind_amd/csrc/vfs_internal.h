@@ -1,0 +1,136 @@
+// vfs_internal.h -- shared declarations of the HIP implementation.
+//
+// Layout: one DMDA block per context, global arrays of mz*my*mx points with i
+// fastest; vector fields are AoS {x,y,z} doubles exactly as the host's
+// PETSc arrays, so uploads and downloads are straight copies.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "vfs_wind.h"
+
+namespace vfs {
+
+// Grid geometry and the flags the kernels branch on, passed by value.
+struct Grid {
+    int mx, my, mz;
+    int pi, pj, pk;  // DA-periodic directions
+    int bc[6];
+    long long plane;  // mx*my
+    long long n;      // mx*my*mz
+};
+
+__host__ __device__ inline int wrapi(int i, int m, int per)
+{
+    if (per) {
+        if (i < 0) i += m;
+        else if (i >= m) i -= m;
+    }
+    return i;
+}
+
+// Index of (k,j,i) with the one-rank periodic DA ghost wrap.
+__host__ __device__ inline long long gidx(const Grid &g, int k, int j, int i)
+{
+    i = wrapi(i, g.mx, g.pi);
+    j = wrapi(j, g.my, g.pj);
+    k = wrapi(k, g.mz, g.pk);
+    return ((long long)k * g.my + j) * g.mx + i;
+}
+
+// Source of the reference's periodic ghost fix for cell (k,j,i): cells 0 and
+// m-1 of a periodic direction copy m-2 and 1 (e.g. momentum.c:638-666).
+// Returns false when (k,j,i) is not such a ghost cell.
+__host__ __device__ inline bool periodic_src(const Grid &g, int k, int j, int i, int &c, int &b, int &a,
+                                             int &fi, int &fj, int &fk)
+{
+    a = i; b = j; c = k;
+    fi = fj = fk = 0;
+    if (g.pi && i == 0) { a = g.mx - 2; fi = 1; }
+    else if (g.pi && i == g.mx - 1) { a = 1; fi = 1; }
+    if (g.pj && j == 0) { b = g.my - 2; fj = 1; }
+    else if (g.pj && j == g.my - 1) { b = 1; fj = 1; }
+    if (g.pk && k == 0) { c = g.mz - 2; fk = 1; }
+    else if (g.pk && k == g.mz - 1) { c = 1; fk = 1; }
+    return fi || fj || fk;
+}
+
+Grid make_grid(const vfs_params &p);
+
+// Status plumbing shared by the C ABI.
+int set_error(int code, const std::string &msg);
+int check_hip(hipError_t e, const char *what);
+
+#define VFS_HIP(call)                                          \
+    do {                                                       \
+        hipError_t e_ = (call);                                \
+        if (e_ != hipSuccess) return ::vfs::check_hip(e_, #call); \
+    } while (0)
+
+#define VFS_LAUNCH_CHECK(what)                                \
+    do {                                                      \
+        hipError_t e_ = hipGetLastError();                    \
+        if (e_ != hipSuccess) return ::vfs::check_hip(e_, what); \
+    } while (0)
+
+inline unsigned blocks_for(long long n, int threads) { return (unsigned)((n + threads - 1) / threads); }
+
+}  // namespace vfs
+
+// Multigrid solver state (poisson.hip).
+struct vfs_poisson;
+
+namespace vfs {
+// kernels whose launches can be timed with HIP events on the context stream
+enum KernelId { K_FACE = 0, K_CELL, K_LINE32, K_LINE64, K_APPLY64, K_C2C, K_PGRAD, K_EDDY, K_NUMK };
+}
+
+struct vfs_ctx {
+    vfs_params p;
+    vfs::Grid g;
+    hipStream_t stream;
+    double *f[VFS_NFIELDS];
+    // scratch for Formfunction_2 and the RK driver
+    double *face;     // 18 doubles per point: div1..3, visc1..3 (Cmpnts each)
+    double *pucont;   // RK base state
+    double *snap;     // 3 doubles per point: snapshot for BC passes
+    double *red;      // reduction scratch (device)
+    double *red_host; // pinned host mirror of reduction results
+    double mean_k_area;
+    double mean_k_flux;
+    vfs_poisson *ps;
+    // per-kernel timing (vfs_profile): event pairs recorded around launches
+    int prof_on;
+    std::vector<hipEvent_t> *ev_pool;
+    std::vector<std::pair<int, int>> *ev_recs;  // (kernel id, index of start event)
+};
+
+namespace vfs {
+// kernels driven from the ABI layer; all enqueue on ctx->stream
+int periodic_fix(vfs_ctx *c, double *f, int nc, int comp_mask_mode);
+int launch_metrics(vfs_ctx *c, const Grid &g, const double *coor, double *const m[17]);
+int c2c(vfs_ctx *c);
+int form_bcs(vfs_ctx *c, int outflow_scale);
+int eddy_viscosity(vfs_ctx *c);
+int pressure_gradient(vfs_ctx *c);
+int calc_k_flux(vfs_ctx *c, double *out);
+int mean_k_area(vfs_ctx *c, double *out);
+int formfunction2(vfs_ctx *c, double scale, const double *pucont, double rk_alpha, double *ucont_out);
+int min_dt(vfs_ctx *c, double *out);
+int poisson_rhs(vfs_ctx *c);
+int poisson_apply(vfs_ctx *c, const double *x, double *y);
+int poisson_solve(vfs_ctx *c, const vfs_solve_opts *o, vfs_solve_stats *st);
+void poisson_free(vfs_ctx *c);
+int update_pressure(vfs_ctx *c);
+int projection(vfs_ctx *c);
+int divergence(vfs_ctx *c, double *out);
+// timing of individual launches (no-ops unless vfs_profile enabled it)
+void prof_begin(vfs_ctx *c, int kernel_id);
+void prof_end(vfs_ctx *c);
+}  // namespace vfs

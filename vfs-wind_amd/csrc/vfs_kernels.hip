@@ -1,0 +1,854 @@
+// vfs_kernels.hip -- metrics, Contra2Cart, FormBCS, LES eddy viscosity,
+// Pressure_Gradient and the scalar diagnostics, as HIP kernels for gfx950.
+//
+// Every kernel is one pass over a DMDA array (i fastest, one thread per
+// point, consecutive lanes on consecutive i) so loads and stores coalesce;
+// the boundary work launches only over the boundary planes it touches.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+
+#include "vfs_internal.h"
+#include "vfs_reduce.h"
+
+namespace vfs {
+
+constexpr int TPB = 256;
+
+// ---------------------------------------------------------------------------
+// periodic ghost fix over the two boundary planes of one periodic direction
+// ---------------------------------------------------------------------------
+
+// dir: 0 = i planes, 1 = j planes, 2 = k planes.  mode 0 copies all nc
+// components; mode 1 (Projection, poisson.c:3001-3040) copies component d of
+// a vector only when direction d is a periodic ghost index of the cell.
+__global__ void k_periodic_planes(Grid g, double *__restrict__ f, int nc, int dir, int mode)
+{
+    long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    int i, j, k;
+    if (dir == 0) {
+        long long n = 2LL * g.my * g.mz;
+        if (t >= n) return;
+        int side = t & 1; long long r = t >> 1;
+        j = r % g.my; k = r / g.my; i = side ? g.mx - 1 : 0;
+    } else if (dir == 1) {
+        long long n = 2LL * g.mx * g.mz;
+        if (t >= n) return;
+        i = t % g.mx; long long r = t / g.mx;
+        int side = r & 1; k = r >> 1; j = side ? g.my - 1 : 0;
+    } else {
+        long long n = 2LL * g.mx * g.my;
+        if (t >= n) return;
+        i = t % g.mx; long long r = t / g.mx;
+        j = r % g.my; int side = (int)(r / g.my); k = side ? g.mz - 1 : 0;
+    }
+    int a, b, c, fi, fj, fk;
+    if (!periodic_src(g, k, j, i, c, b, a, fi, fj, fk)) return;
+    long long d = gidx(g, k, j, i), s = gidx(g, c, b, a);
+    if (mode == 0) {
+        for (int q = 0; q < nc; q++) f[nc * d + q] = f[nc * s + q];
+    } else {
+        if (fi) f[3 * d + 0] = f[3 * s + 0];
+        if (fj) f[3 * d + 1] = f[3 * s + 1];
+        if (fk) f[3 * d + 2] = f[3 * s + 2];
+    }
+}
+
+int periodic_fix_grid(hipStream_t st, const Grid &g, double *f, int nc, int mode)
+{
+    const long long np[3] = {2LL * g.my * g.mz, 2LL * g.mx * g.mz, 2LL * g.mx * g.my};
+    const int per[3] = {g.pi, g.pj, g.pk};
+    for (int d = 0; d < 3; d++) {
+        if (!per[d]) continue;
+        hipLaunchKernelGGL(k_periodic_planes, dim3(blocks_for(np[d], TPB)), dim3(TPB), 0, st, g, f, nc, d, mode);
+    }
+    VFS_LAUNCH_CHECK("periodic fix");
+    return VFS_OK;
+}
+
+int periodic_fix(vfs_ctx *c, double *f, int nc, int mode)
+{
+    return periodic_fix_grid(c->stream, c->g, f, nc, mode);
+}
+
+// ---------------------------------------------------------------------------
+// a1: FormMetrics (metrics.c:12-1355)
+// ---------------------------------------------------------------------------
+
+struct Decode {
+    int i, j, k;
+};
+__device__ inline bool decode(const Grid &g, long long t, Decode &d)
+{
+    if (t >= g.n) return false;
+    d.i = t % g.mx;
+    long long r = t / g.mx;
+    d.j = r % g.my;
+    d.k = r / g.my;
+    return true;
+}
+
+#define CRD(c, kk, jj, ii) coor[3 * gidx(g, (kk), (jj), (ii)) + (c)]
+
+// cell centres (metrics.c:79-100), node-plane area vectors csi/eta/zet
+// (metrics.c:116-185) and the Jacobian (metrics.c:193-241)
+__global__ void k_metrics_cell(Grid g, const double *__restrict__ coor, double *__restrict__ csi,
+                               double *__restrict__ eta, double *__restrict__ zet, double *__restrict__ aj,
+                               double *__restrict__ cent)
+{
+    Decode d;
+    if (!decode(g, blockIdx.x * (long long)blockDim.x + threadIdx.x, d)) return;
+    const int i = d.i, j = d.j, k = d.k;
+    const long long q = gidx(g, k, j, i);
+    const bool ii_ = i >= 1 && i < g.mx - 1, jj_ = j >= 1 && j < g.my - 1, kk_ = k >= 1 && k < g.mz - 1;
+    if (ii_ && jj_ && kk_) {
+        for (int c = 0; c < 3; c++) {
+            double s = CRD(c, k, j, i) + CRD(c, k, j - 1, i) + CRD(c, k - 1, j, i) + CRD(c, k - 1, j - 1, i) +
+                       CRD(c, k, j, i - 1) + CRD(c, k, j - 1, i - 1) + CRD(c, k - 1, j, i - 1) +
+                       CRD(c, k - 1, j - 1, i - 1);
+            cent[3 * q + c] = 0.125 * s;
+        }
+    }
+    if (kk_ && jj_ && i < g.mx - 1) {
+        double dxde = 0.5 * (CRD(0, k, j, i) + CRD(0, k - 1, j, i) - CRD(0, k, j - 1, i) - CRD(0, k - 1, j - 1, i));
+        double dyde = 0.5 * (CRD(1, k, j, i) + CRD(1, k - 1, j, i) - CRD(1, k, j - 1, i) - CRD(1, k - 1, j - 1, i));
+        double dzde = 0.5 * (CRD(2, k, j, i) + CRD(2, k - 1, j, i) - CRD(2, k, j - 1, i) - CRD(2, k - 1, j - 1, i));
+        double dxdz = 0.5 * (CRD(0, k, j - 1, i) + CRD(0, k, j, i) - CRD(0, k - 1, j - 1, i) - CRD(0, k - 1, j, i));
+        double dydz = 0.5 * (CRD(1, k, j - 1, i) + CRD(1, k, j, i) - CRD(1, k - 1, j - 1, i) - CRD(1, k - 1, j, i));
+        double dzdz = 0.5 * (CRD(2, k, j - 1, i) + CRD(2, k, j, i) - CRD(2, k - 1, j - 1, i) - CRD(2, k - 1, j, i));
+        csi[3 * q + 0] = dyde * dzdz - dzde * dydz;
+        csi[3 * q + 1] = -dxde * dzdz + dzde * dxdz;
+        csi[3 * q + 2] = dxde * dydz - dyde * dxdz;
+    }
+    if (kk_ && j < g.my - 1 && ii_) {
+        double dxdc = 0.5 * (CRD(0, k, j, i) + CRD(0, k - 1, j, i) - CRD(0, k, j, i - 1) - CRD(0, k - 1, j, i - 1));
+        double dydc = 0.5 * (CRD(1, k, j, i) + CRD(1, k - 1, j, i) - CRD(1, k, j, i - 1) - CRD(1, k - 1, j, i - 1));
+        double dzdc = 0.5 * (CRD(2, k, j, i) + CRD(2, k - 1, j, i) - CRD(2, k, j, i - 1) - CRD(2, k - 1, j, i - 1));
+        double dxdz = 0.5 * (CRD(0, k, j, i) + CRD(0, k, j, i - 1) - CRD(0, k - 1, j, i) - CRD(0, k - 1, j, i - 1));
+        double dydz = 0.5 * (CRD(1, k, j, i) + CRD(1, k, j, i - 1) - CRD(1, k - 1, j, i) - CRD(1, k - 1, j, i - 1));
+        double dzdz = 0.5 * (CRD(2, k, j, i) + CRD(2, k, j, i - 1) - CRD(2, k - 1, j, i) - CRD(2, k - 1, j, i - 1));
+        eta[3 * q + 0] = dydz * dzdc - dzdz * dydc;
+        eta[3 * q + 1] = -dxdz * dzdc + dzdz * dxdc;
+        eta[3 * q + 2] = dxdz * dydc - dydz * dxdc;
+    }
+    if (k < g.mz - 1 && jj_ && ii_) {
+        double dxdc = 0.5 * (CRD(0, k, j, i) + CRD(0, k, j - 1, i) - CRD(0, k, j, i - 1) - CRD(0, k, j - 1, i - 1));
+        double dydc = 0.5 * (CRD(1, k, j, i) + CRD(1, k, j - 1, i) - CRD(1, k, j, i - 1) - CRD(1, k, j - 1, i - 1));
+        double dzdc = 0.5 * (CRD(2, k, j, i) + CRD(2, k, j - 1, i) - CRD(2, k, j, i - 1) - CRD(2, k, j - 1, i - 1));
+        double dxde = 0.5 * (CRD(0, k, j, i) + CRD(0, k, j, i - 1) - CRD(0, k, j - 1, i) - CRD(0, k, j - 1, i - 1));
+        double dyde = 0.5 * (CRD(1, k, j, i) + CRD(1, k, j, i - 1) - CRD(1, k, j - 1, i) - CRD(1, k, j - 1, i - 1));
+        double dzde = 0.5 * (CRD(2, k, j, i) + CRD(2, k, j, i - 1) - CRD(2, k, j - 1, i) - CRD(2, k, j - 1, i - 1));
+        zet[3 * q + 0] = dydc * dzde - dzdc * dyde;
+        zet[3 * q + 1] = -dxdc * dzde + dzdc * dxde;
+        zet[3 * q + 2] = dxdc * dyde - dydc * dxde;
+    }
+    if (ii_ && jj_ && kk_) {
+        double dd[3][3];
+        for (int c = 0; c < 3; c++) {
+            dd[c][0] = 0.25 * (CRD(c, k, j, i) + CRD(c, k, j - 1, i) + CRD(c, k - 1, j, i) + CRD(c, k - 1, j - 1, i) -
+                               CRD(c, k, j, i - 1) - CRD(c, k, j - 1, i - 1) - CRD(c, k - 1, j, i - 1) -
+                               CRD(c, k - 1, j - 1, i - 1));
+            dd[c][1] = 0.25 * (CRD(c, k, j, i) + CRD(c, k, j, i - 1) + CRD(c, k - 1, j, i) + CRD(c, k - 1, j, i - 1) -
+                               CRD(c, k, j - 1, i) - CRD(c, k, j - 1, i - 1) - CRD(c, k - 1, j - 1, i) -
+                               CRD(c, k - 1, j - 1, i - 1));
+            dd[c][2] = 0.25 * (CRD(c, k, j, i) + CRD(c, k, j - 1, i) + CRD(c, k, j, i - 1) + CRD(c, k, j - 1, i - 1) -
+                               CRD(c, k - 1, j, i) - CRD(c, k - 1, j - 1, i) - CRD(c, k - 1, j, i - 1) -
+                               CRD(c, k - 1, j - 1, i - 1));
+        }
+        double dxdc = dd[0][0], dydc = dd[1][0], dzdc = dd[2][0];
+        double dxde = dd[0][1], dyde = dd[1][1], dzde = dd[2][1];
+        double dxdz = dd[0][2], dydz = dd[1][2], dzdz = dd[2][2];
+        double det = dxdc * (dyde * dzdz - dzde * dydz) - dydc * (dxde * dzdz - dzde * dxdz) +
+                     dzdc * (dxde * dydz - dyde * dxdz);
+        aj[q] = 1. / det;
+    }
+}
+
+// ghost mirroring (metrics.c:261-345); `which` selects one of the six
+// sequential loops, each a pass over one boundary plane.
+__global__ void k_metrics_mirror(Grid g, int which, double *__restrict__ csi, double *__restrict__ eta,
+                                 double *__restrict__ zet, double *__restrict__ aj)
+{
+    long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    int axis = which >> 1, hi = which & 1;
+    int i, j, k;
+    if (axis == 0) {
+        if (t >= (long long)g.my * g.mz) return;
+        j = t % g.my; k = t / g.my; i = hi ? g.mx - 1 : 0;
+    } else if (axis == 1) {
+        if (t >= (long long)g.mx * g.mz) return;
+        i = t % g.mx; k = t / g.mx; j = hi ? g.my - 1 : 0;
+    } else {
+        if (t >= (long long)g.mx * g.my) return;
+        i = t % g.mx; j = t / g.mx; k = hi ? g.mz - 1 : 0;
+    }
+    int si = i, sj = j, sk = k;
+    if (axis == 0) si = hi ? g.mx - 2 : 1;
+    if (axis == 1) sj = hi ? g.my - 2 : 1;
+    if (axis == 2) sk = hi ? g.mz - 2 : 1;
+    long long d = gidx(g, k, j, i), s = gidx(g, sk, sj, si);
+    double *A = axis == 0 ? eta : csi;   // i: eta, zet; j: csi, zet; k: eta, csi
+    double *B = axis == 2 ? csi : zet;
+    if (axis == 2) A = eta;
+    for (int c = 0; c < 3; c++) { A[3 * d + c] = A[3 * s + c]; B[3 * d + c] = B[3 * s + c]; }
+    aj[d] = aj[s];
+}
+
+// face centroid of a node plane (metrics.c:483-495, 612-624, 718-730)
+__global__ void k_face_centroid(Grid g, int axis, const double *__restrict__ coor, double *__restrict__ cf)
+{
+    Decode d;
+    if (!decode(g, blockIdx.x * (long long)blockDim.x + threadIdx.x, d)) return;
+    const int i = d.i, j = d.j, k = d.k;
+    bool ok;
+    if (axis == 0) ok = k >= 1 && j >= 1 && i < g.mx - 1;
+    else if (axis == 1) ok = k >= 1 && j < g.my - 1 && i >= 1;
+    else ok = k < g.mz - 1 && j >= 1 && i >= 1;
+    long long q = gidx(g, k, j, i);
+    for (int c = 0; c < 3; c++) {
+        double v = 0;
+        if (ok) {
+            if (axis == 0) v = (CRD(c, k, j, i) + CRD(c, k - 1, j, i) + CRD(c, k, j - 1, i) + CRD(c, k - 1, j - 1, i)) * 0.25;
+            else if (axis == 1) v = (CRD(c, k, j, i) + CRD(c, k - 1, j, i) + CRD(c, k, j, i - 1) + CRD(c, k - 1, j, i - 1)) * 0.25;
+            else v = (CRD(c, k, j, i) + CRD(c, k, j - 1, i) + CRD(c, k, j, i - 1) + CRD(c, k, j - 1, i - 1)) * 0.25;
+        }
+        cf[3 * q + c] = v;
+    }
+}
+
+// face metrics from face-centroid differences (metrics.c:497-818)
+__global__ void k_face_metrics(Grid g, int axis, const double *__restrict__ cf, double *__restrict__ fcsi,
+                               double *__restrict__ feta, double *__restrict__ fzet, double *__restrict__ faj)
+{
+    Decode d;
+    if (!decode(g, blockIdx.x * (long long)blockDim.x + threadIdx.x, d)) return;
+    const int i = d.i, j = d.j, k = d.k;
+    bool ok;
+    if (axis == 0) ok = k >= 1 && k < g.mz - 1 && j >= 1 && j < g.my - 1 && i < g.mx - 1;
+    else if (axis == 1) ok = k >= 1 && k < g.mz - 1 && j < g.my - 1 && i >= 1 && i < g.mx - 1;
+    else ok = k < g.mz - 1 && j >= 1 && j < g.my - 1 && i >= 1 && i < g.mx - 1;
+    if (!ok) return;
+    // lower index at which the own direction switches to a forward difference
+    const int i0 = axis == 0 ? 0 : 1, j0 = axis == 1 ? 0 : 1, k0 = axis == 2 ? 0 : 1;
+    double dc[3], de[3], dz[3];
+#define CF(c, kk, jj, ii) cf[3 * gidx(g, (kk), (jj), (ii)) + (c)]
+    for (int c = 0; c < 3; c++) {
+        if (i == i0) dc[c] = CF(c, k, j, i + 1) - CF(c, k, j, i);
+        else if (i == g.mx - 2) dc[c] = CF(c, k, j, i) - CF(c, k, j, i - 1);
+        else dc[c] = (CF(c, k, j, i + 1) - CF(c, k, j, i - 1)) * 0.5;
+        if (j == j0) de[c] = CF(c, k, j + 1, i) - CF(c, k, j, i);
+        else if (j == g.my - 2) de[c] = CF(c, k, j, i) - CF(c, k, j - 1, i);
+        else de[c] = (CF(c, k, j + 1, i) - CF(c, k, j - 1, i)) * 0.5;
+        if (k == k0) dz[c] = (CF(c, k + 1, j, i) - CF(c, k, j, i));
+        else if (k == g.mz - 2) dz[c] = (CF(c, k, j, i) - CF(c, k - 1, j, i));
+        else dz[c] = (CF(c, k + 1, j, i) - CF(c, k - 1, j, i)) * 0.5;
+    }
+#undef CF
+    const double dxdc = dc[0], dydc = dc[1], dzdc = dc[2];
+    const double dxde = de[0], dyde = de[1], dzde = de[2];
+    const double dxdz = dz[0], dydz = dz[1], dzdz = dz[2];
+    const long long q = gidx(g, k, j, i);
+    fcsi[3 * q + 0] = dyde * dzdz - dzde * dydz;
+    fcsi[3 * q + 1] = -dxde * dzdz + dzde * dxdz;
+    fcsi[3 * q + 2] = dxde * dydz - dyde * dxdz;
+    feta[3 * q + 0] = dydz * dzdc - dzdz * dydc;
+    feta[3 * q + 1] = -dxdz * dzdc + dzdz * dxdc;
+    feta[3 * q + 2] = dxdz * dydc - dydz * dxdc;
+    fzet[3 * q + 0] = dydc * dzde - dzdc * dyde;
+    fzet[3 * q + 1] = -dxdc * dzde + dzdc * dxde;
+    fzet[3 * q + 2] = dxdc * dyde - dydc * dxde;
+    double det = dxdc * (dyde * dzdz - dzde * dydz) - dydc * (dxde * dzdz - dzde * dxdz) +
+                 dzdc * (dxde * dydz - dyde * dxdz);
+    faj[q] = 1. / det;
+}
+#undef CRD
+
+int launch_metrics(vfs_ctx *c, const Grid &g, const double *coor, double *const m[17])
+{
+    hipStream_t st = c->stream;
+    const size_t n = (size_t)g.n;
+    for (int q = 0; q < 17; q++) {
+        bool vec = !(q == 3 || q == 7 || q == 11 || q == 15);
+        VFS_HIP(hipMemsetAsync(m[q], 0, (vec ? 3 : 1) * n * sizeof(double), st));
+    }
+    const unsigned nb = blocks_for(g.n, TPB);
+    hipLaunchKernelGGL(k_metrics_cell, dim3(nb), dim3(TPB), 0, st, g, coor, m[0], m[1], m[2], m[3], m[16]);
+    VFS_LAUNCH_CHECK("metrics cell");
+    const long long pl[3] = {(long long)g.my * g.mz, (long long)g.mx * g.mz, (long long)g.mx * g.my};
+    for (int w = 0; w < 6; w++)
+        hipLaunchKernelGGL(k_metrics_mirror, dim3(blocks_for(pl[w >> 1], TPB)), dim3(TPB), 0, st, g, w, m[0], m[1], m[2], m[3]);
+    VFS_LAUNCH_CHECK("metrics mirror");
+    double *cf = c->snap;  // 3 doubles per point of the finest grid
+    for (int axis = 0; axis < 3; axis++) {
+        hipLaunchKernelGGL(k_face_centroid, dim3(nb), dim3(TPB), 0, st, g, axis, coor, cf);
+        hipLaunchKernelGGL(k_face_metrics, dim3(nb), dim3(TPB), 0, st, g, axis, cf, m[4 + 4 * axis],
+                           m[5 + 4 * axis], m[6 + 4 * axis], m[7 + 4 * axis]);
+    }
+    VFS_LAUNCH_CHECK("face metrics");
+    // final periodic fix of the sixteen arrays (metrics.c:1180-1300)
+    for (int q = 0; q < 16; q++) {
+        bool vec = !(q == 3 || q == 7 || q == 11 || q == 15);
+        int rc = periodic_fix_grid(st, g, m[q], vec ? 3 : 1, 0);
+        if (rc) return rc;
+    }
+    return VFS_OK;
+}
+
+// ---------------------------------------------------------------------------
+// a2: Contra2Cart_2 (rhs.c:65-749)
+// ---------------------------------------------------------------------------
+
+// interior Cramer solve, rhs.c:158-247
+__global__ void k_c2c_interior(Grid g, const double *__restrict__ ucont, const double *__restrict__ csi,
+                               const double *__restrict__ eta, const double *__restrict__ zet,
+                               const double *__restrict__ nvert, double *__restrict__ ucat)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int j = blockIdx.y + 1, k = blockIdx.z + 1;
+    if (i >= g.mx - 1) return;
+    const long long q = gidx(g, k, j, i);
+    if (nvert[q] >= 0.1) return;
+    double q0 = 0.5 * (ucont[3 * gidx(g, k, j, i - 1) + 0] + ucont[3 * q + 0]);
+    double q1 = 0.5 * (ucont[3 * gidx(g, k, j - 1, i) + 1] + ucont[3 * q + 1]);
+    double q2 = 0.5 * (ucont[3 * gidx(g, k - 1, j, i) + 2] + ucont[3 * q + 2]);
+    const double m00 = csi[3 * q], m01 = csi[3 * q + 1], m02 = csi[3 * q + 2];
+    const double m10 = eta[3 * q], m11 = eta[3 * q + 1], m12 = eta[3 * q + 2];
+    const double m20 = zet[3 * q], m21 = zet[3 * q + 1], m22 = zet[3 * q + 2];
+    double det = m00 * (m11 * m22 - m12 * m21) - m01 * (m10 * m22 - m12 * m20) + m02 * (m10 * m21 - m11 * m20);
+    double det0 = q0 * (m11 * m22 - m12 * m21) - q1 * (m01 * m22 - m02 * m21) + q2 * (m01 * m12 - m02 * m11);
+    double det1 = -q0 * (m10 * m22 - m12 * m20) + q1 * (m00 * m22 - m02 * m20) - q2 * (m00 * m12 - m02 * m10);
+    double det2 = q0 * (m10 * m21 - m11 * m20) - q1 * (m00 * m21 - m01 * m20) + q2 * (m00 * m11 - m01 * m10);
+    ucat[3 * q + 0] = det0 / det;
+    ucat[3 * q + 1] = det1 / det;
+    ucat[3 * q + 2] = det2 / det;
+}
+
+// inverse of [csi; eta; zet] (Calculate_Covariant_metrics, rhs2.c:595-612)
+__device__ inline void covariant(const double gm[3][3], double G[3][3])
+{
+    const double a11 = gm[0][0], a12 = gm[0][1], a13 = gm[0][2];
+    const double a21 = gm[1][0], a22 = gm[1][1], a23 = gm[1][2];
+    const double a31 = gm[2][0], a32 = gm[2][1], a33 = gm[2][2];
+    double det = a11 * (a33 * a22 - a32 * a23) - a21 * (a33 * a12 - a32 * a13) + a31 * (a23 * a12 - a22 * a13);
+    G[0][0] = (a33 * a22 - a32 * a23) / det; G[0][1] = -(a33 * a12 - a32 * a13) / det; G[0][2] = (a23 * a12 - a22 * a13) / det;
+    G[1][0] = -(a33 * a21 - a31 * a23) / det; G[1][1] = (a33 * a11 - a31 * a13) / det; G[1][2] = -(a23 * a11 - a21 * a13) / det;
+    G[2][0] = (a32 * a21 - a31 * a22) / det; G[2][1] = -(a32 * a11 - a31 * a12) / det; G[2][2] = (a22 * a11 - a21 * a12) / det;
+}
+
+// slip reflection, rhs.c:447-544
+__device__ inline void slip_fill(const Grid &g, const double *csi, const double *eta, const double *zet,
+                                 const double *uold, int kn, int jn, int in, int dir, double sgn, double *out)
+{
+    long long q = gidx(g, kn, jn, in);
+    double gm[3][3], G[3][3];
+    for (int c = 0; c < 3; c++) { gm[0][c] = csi[3 * q + c]; gm[1][c] = eta[3 * q + c]; gm[2][c] = zet[3 * q + c]; }
+    covariant(gm, G);
+    double nx = sgn * G[0][dir], ny = sgn * G[1][dir], nz = sgn * G[2][dir];
+    double sum = sqrt(nx * nx + ny * ny + nz * nz);
+    nx /= sum; ny /= sum; nz /= sum;
+    double ux = uold[3 * q], uy = uold[3 * q + 1], uz = uold[3 * q + 2];
+    double un = ux * nx + uy * ny + uz * nz;
+    out[0] = ux - 2 * un * nx;
+    out[1] = uy - 2 * un * ny;
+    out[2] = uz - 2 * un * nz;
+}
+
+// ghost-cell BC fill, rhs.c:300-680, for the boundary planes of the
+// non-periodic directions.  Each ghost cell is visited by exactly one thread
+// (planes of a later axis skip cells owned by an earlier axis).
+__global__ void k_c2c_bc(Grid g, int axis, const double *__restrict__ L2, const double *__restrict__ uold,
+                         const double *__restrict__ csi, const double *__restrict__ eta,
+                         const double *__restrict__ zet, const double *__restrict__ nvert, double *__restrict__ ucat)
+{
+    long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    int i, j, k;
+    if (axis == 0) {
+        if (t >= 2LL * g.my * g.mz) return;
+        int side = t & 1; long long r = t >> 1;
+        j = r % g.my; k = r / g.my; i = side ? g.mx - 1 : 0;
+    } else if (axis == 1) {
+        if (t >= 2LL * g.mx * g.mz) return;
+        i = t % g.mx; long long r = t / g.mx;
+        int side = r & 1; k = r >> 1; j = side ? g.my - 1 : 0;
+        if (!g.pi && (i == 0 || i == g.mx - 1)) return;
+    } else {
+        if (t >= 2LL * g.mx * g.my) return;
+        i = t % g.mx; long long r = t / g.mx;
+        j = r % g.my; int side = (int)(r / g.my); k = side ? g.mz - 1 : 0;
+        if (!g.pi && (i == 0 || i == g.mx - 1)) return;
+        if (!g.pj && (j == 0 || j == g.my - 1)) return;
+    }
+    const int *bc = g.bc;
+    const long long q = gidx(g, k, j, i);
+    double u[3] = {ucat[3 * q], ucat[3 * q + 1], ucat[3 * q + 2]};
+    bool touched = false;
+    int solid_flag = 0;
+    const int mx = g.mx, my = g.my, mz = g.mz;
+    if (bc[0] == 10 && i == 0 && (j != 0 && k != 0)) {
+        slip_fill(g, csi, eta, zet, uold, k, j, i + 1, 0, -1.0, u);
+        if (nvert[gidx(g, k, j, i + 1)] > 0.1) u[0] = u[1] = u[2] = 0;
+        if (solid_flag) u[0] = u[1] = u[2] = 0;
+        touched = true;
+    }
+    if (bc[1] == 10 && i == mx - 1 && (j != 0 && k != 0)) {
+        slip_fill(g, csi, eta, zet, uold, k, j, i - 1, 0, 1.0, u);
+        if (nvert[gidx(g, k, j, i - 1)] > 0.1) u[0] = u[1] = u[2] = 0;
+        if (solid_flag) u[0] = u[1] = u[2] = 0;
+        touched = true;
+    }
+    if (bc[2] == 10 && j == 0 && (i != 0 && k != 0)) {
+        slip_fill(g, csi, eta, zet, uold, k, j + 1, i, 1, -1.0, u);
+        if (nvert[gidx(g, k, j + 1, i)] > 0.1) u[0] = u[1] = u[2] = 0;
+        if (solid_flag) u[0] = u[1] = u[2] = 0;
+        touched = true;
+    }
+    if ((bc[3] == 10 || bc[3] == -10) && j == my - 1 && (i != 0 && k != 0)) {
+        slip_fill(g, csi, eta, zet, uold, k, j - 1, i, 1, 1.0, u);
+        if (nvert[gidx(g, k, j - 1, i)] > 0.1) u[0] = u[1] = u[2] = 0;
+        if (solid_flag) u[0] = u[1] = u[2] = 0;
+        touched = true;
+    }
+    if (i == 0 && bc[0] == 1 && (j != 0 && k != 0)) {
+        long long s = gidx(g, k, j, i + 1);
+        for (int c = 0; c < 3; c++) u[c] = -L2[3 * s + c];
+        solid_flag = 1; touched = true;
+    }
+    if (i == mx - 1 && bc[1] == 1 && (j != 0 && k != 0)) {
+        long long s = gidx(g, k, j, i - 1);
+        for (int c = 0; c < 3; c++) u[c] = -L2[3 * s + c];
+        solid_flag = 1; touched = true;
+    }
+    if (j == 0 && bc[2] == 1 && (i != 0 && k != 0)) {
+        long long s = gidx(g, k, j + 1, i);
+        for (int c = 0; c < 3; c++) u[c] = -L2[3 * s + c];
+        solid_flag = 1; touched = true;
+    }
+    if (j == my - 1 && bc[3] == 1 && (i != 0 && k != 0)) {
+        long long s = gidx(g, k, j - 1, i);
+        for (int c = 0; c < 3; c++) u[c] = -L2[3 * s + c];
+        solid_flag = 1; touched = true;
+    }
+    if (k == 0 && bc[4] == 1 && (i != 0 && j != 0)) {
+        long long s = gidx(g, k + 1, j, i);
+        for (int c = 0; c < 3; c++) u[c] = -L2[3 * s + c];
+        solid_flag = 1; touched = true;
+    }
+    if (k == mz - 1 && bc[5] == 1 && (i != 0 && j != 0)) {
+        long long s = gidx(g, k - 1, j, i);
+        for (int c = 0; c < 3; c++) u[c] = -L2[3 * s + c];
+        solid_flag = 1; touched = true;
+    }
+    if (bc[5] == 4 && k == mz - 1 && (i != 0 && i != mx - 1 && j != 0 && j != my - 1)) {
+        if (nvert[gidx(g, k - 1, j, i)] > 0.1) { u[0] = u[1] = u[2] = 0; touched = true; }
+        if (solid_flag) { u[0] = u[1] = u[2] = 0; touched = true; }
+    }
+    if (touched) { ucat[3 * q] = u[0]; ucat[3 * q + 1] = u[1]; ucat[3 * q + 2] = u[2]; }
+}
+
+// interior cells next to two walls (rhs.c:664-669)
+__global__ void k_c2c_corner(Grid g, double *__restrict__ ucat)
+{
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < 1 || k >= g.mz - 2) return;
+    const int *bc = g.bc;
+    if (bc[0] <= 1 && bc[2] <= 1) { long long q = gidx(g, k, 1, 1); ucat[3 * q] = ucat[3 * q + 1] = ucat[3 * q + 2] = 0; }
+    if (bc[1] <= 1 && bc[2] <= 1) { long long q = gidx(g, k, 1, g.mx - 2); ucat[3 * q] = ucat[3 * q + 1] = ucat[3 * q + 2] = 0; }
+}
+
+static bool bc_needs_snapshot(const Grid &g)
+{
+    // a no-slip rule reads the neighbouring plane, which another axis' rule
+    // may write in the same pass only if walls sit on two or more axes
+    const int per[3] = {g.pi, g.pj, g.pk};
+    int axes_noslip = 0, axes_write = 0;
+    for (int a = 0; a < 3; a++) {
+        if (per[a]) continue;
+        bool ns = g.bc[2 * a] == 1 || g.bc[2 * a + 1] == 1;
+        bool wr = ns || g.bc[2 * a] == 10 || g.bc[2 * a + 1] == 10 || (a == 1 && g.bc[3] == -10) ||
+                  (a == 2 && g.bc[5] == 4);
+        axes_noslip += ns;
+        axes_write += wr;
+    }
+    return axes_noslip >= 1 && axes_write >= 2;
+}
+
+static bool corner_rule_active(const Grid &g)
+{
+    return !g.pi && !g.pj && ((g.bc[0] <= 1 && g.bc[2] <= 1) || (g.bc[1] <= 1 && g.bc[2] <= 1));
+}
+
+int c2c(vfs_ctx *c)
+{
+    const Grid &g = c->g;
+    hipStream_t st = c->stream;
+    double *ucont = c->f[VFS_UCONT], *ucat = c->f[VFS_UCAT];
+    int rc = periodic_fix(c, ucont, 3, 0);  // rhs.c:129-156
+    if (rc) return rc;
+    dim3 blk(TPB), grd(blocks_for(g.mx - 2, TPB), g.my - 2, g.mz - 2);
+    prof_begin(c, K_C2C);
+    hipLaunchKernelGGL(k_c2c_interior, grd, blk, 0, st, g, ucont, c->f[VFS_CSI], c->f[VFS_ETA], c->f[VFS_ZET],
+                       c->f[VFS_NVERT], ucat);
+    prof_end(c);
+    VFS_LAUNCH_CHECK("c2c interior");
+    rc = periodic_fix(c, ucat, 3, 0);  // rhs.c:254-291
+    if (rc) return rc;
+    const double *L2 = ucat;
+    const bool corner = corner_rule_active(g);
+    if (bc_needs_snapshot(g) || corner) {
+        VFS_HIP(hipMemcpyAsync(c->snap, ucat, 3 * (size_t)g.n * sizeof(double), hipMemcpyDeviceToDevice, st));
+        L2 = c->snap;
+    }
+    const int per[3] = {g.pi, g.pj, g.pk};
+    const long long np[3] = {2LL * g.my * g.mz, 2LL * g.mx * g.mz, 2LL * g.mx * g.my};
+    for (int a = 0; a < 3; a++) {
+        if (per[a]) continue;
+        hipLaunchKernelGGL(k_c2c_bc, dim3(blocks_for(np[a], TPB)), dim3(TPB), 0, st, g, a, L2, c->f[VFS_UCAT_OLD],
+                           c->f[VFS_CSI], c->f[VFS_ETA], c->f[VFS_ZET], c->f[VFS_NVERT], ucat);
+    }
+    VFS_LAUNCH_CHECK("c2c bc");
+    if (corner) {
+        hipLaunchKernelGGL(k_c2c_corner, dim3(blocks_for(g.mz, TPB)), dim3(TPB), 0, st, g, ucat);
+        VFS_LAUNCH_CHECK("c2c corner");
+    }
+    return periodic_fix(c, ucat, 3, 0);  // rhs.c:712-748
+}
+
+// ---------------------------------------------------------------------------
+// FormBCS subset (bcs.c:1684-2767)
+// ---------------------------------------------------------------------------
+
+// outlet plane sums: FluxOut and the area (bcs.c:2127-2175)
+__global__ void k_outlet_sums(Grid g, const double *__restrict__ ucont, const double *__restrict__ zet,
+                              const double *__restrict__ nvert, double *__restrict__ part)
+{
+    const int k = g.mz - 1;
+    const long long n = (long long)(g.mx - 2) * (g.my - 2);
+    double flux = 0, area = 0;
+    for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < n; t += (long long)gridDim.x * blockDim.x) {
+        int i = 1 + t % (g.mx - 2), j = 1 + t / (g.mx - 2);
+        long long q = gidx(g, k - 1, j, i);
+        if (nvert[q] < 0.1) {
+            flux += ucont[3 * q + 2];
+            area += sqrt(zet[3 * q] * zet[3 * q] + zet[3 * q + 1] * zet[3 * q + 1] + zet[3 * q + 2] * zet[3 * q + 2]);
+        }
+    }
+    flux = block_reduce<OpSum>(flux);
+    area = block_reduce<OpSum>(area);
+    if (threadIdx.x == 0) { part[blockIdx.x] = flux; part[RED_BLOCKS + blockIdx.x] = area; }
+}
+
+__global__ void k_outlet_scale(Grid g, double flux_in, const double *__restrict__ sums, const double *__restrict__ zet,
+                               const double *__restrict__ nvert, double *__restrict__ ucont)
+{
+    const int k = g.mz - 1;
+    long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (t >= (long long)(g.mx - 2) * (g.my - 2)) return;
+    int i = 1 + t % (g.mx - 2), j = 1 + t / (g.mx - 2);
+    long long q = gidx(g, k - 1, j, i);
+    double Area = sqrt(zet[3 * q] * zet[3 * q] + zet[3 * q + 1] * zet[3 * q + 1] + zet[3 * q + 2] * zet[3 * q + 2]);
+    if (nvert[q] < 0.1) ucont[3 * q + 2] += (flux_in - sums[0]) * Area / sums[1];
+}
+
+// wall-normal flux zeroing, bcs.c:2347-2386
+__global__ void k_wall_flux(Grid g, double *__restrict__ ucont)
+{
+    long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    const int *bc = g.bc;
+    // i faces: (k,j) interior
+    long long ni = (long long)(g.my - 2) * (g.mz - 2);
+    if (t < ni) {
+        int j = 1 + t % (g.my - 2), k = 1 + t / (g.my - 2);
+        if (bc[0] == 1 || bc[0] == 10) ucont[3 * gidx(g, k, j, 0) + 0] = 0;
+        if (bc[1] == 1 || bc[1] == 10) ucont[3 * gidx(g, k, j, g.mx - 2) + 0] = 0;
+    }
+    long long nj = (long long)(g.mx - 2) * (g.mz - 2);
+    if (t < nj) {
+        int i = 1 + t % (g.mx - 2), k = 1 + t / (g.mx - 2);
+        if (bc[2] == 1 || bc[2] == 10) ucont[3 * gidx(g, k, 0, i) + 1] = 0;
+        if (bc[3] == 1 || bc[3] == 10) ucont[3 * gidx(g, k, g.my - 2, i) + 1] = 0;
+    }
+}
+
+// zero the twelve ghost edges of Ucat, bcs.c:2630-2760
+__global__ void k_zero_edges(Grid g, double *__restrict__ ucat)
+{
+    long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    auto z = [&](int k, int j, int i) {
+        long long q = gidx(g, k, j, i);
+        ucat[3 * q] = ucat[3 * q + 1] = ucat[3 * q + 2] = 0;
+    };
+    if (t < g.my) { int j = (int)t; z(0, j, 0); z(0, j, g.mx - 1); z(g.mz - 1, j, 0); z(g.mz - 1, j, g.mx - 1); }
+    if (t < g.mx) { int i = (int)t; z(0, 0, i); z(0, g.my - 1, i); z(g.mz - 1, 0, i); z(g.mz - 1, g.my - 1, i); }
+    if (t < g.mz) { int k = (int)t; z(k, 0, 0); z(k, 0, g.mx - 1); z(k, g.my - 1, 0); z(k, g.my - 1, g.mx - 1); }
+}
+
+int form_bcs(vfs_ctx *c, int outflow_scale)
+{
+    const Grid &g = c->g;
+    hipStream_t st = c->stream;
+    int rc = c2c(c);
+    if (rc) return rc;
+    if (g.bc[5] == 4 && !g.pk) {
+        hipLaunchKernelGGL(k_outlet_sums, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, st, g, c->f[VFS_UCONT],
+                           c->f[VFS_ZET], c->f[VFS_NVERT], c->red);
+        hipLaunchKernelGGL(finish_reduce<OpSum>, dim3(1), dim3(RED_THREADS), 0, st, c->red, RED_BLOCKS, c->red + 2 * RED_BLOCKS);
+        hipLaunchKernelGGL(finish_reduce<OpSum>, dim3(1), dim3(RED_THREADS), 0, st, c->red + RED_BLOCKS, RED_BLOCKS,
+                           c->red + 2 * RED_BLOCKS + 1);
+        VFS_LAUNCH_CHECK("outlet sums");
+        if (outflow_scale) {
+            long long n = (long long)(g.mx - 2) * (g.my - 2);
+            hipLaunchKernelGGL(k_outlet_scale, dim3(blocks_for(n, TPB)), dim3(TPB), 0, st, g, c->p.flux_in_sum,
+                               c->red + 2 * RED_BLOCKS, c->f[VFS_ZET], c->f[VFS_NVERT], c->f[VFS_UCONT]);
+            VFS_LAUNCH_CHECK("outlet scale");
+        }
+    }
+    long long nw = (long long)(g.my > g.mx ? g.my : g.mx) * g.mz;
+    hipLaunchKernelGGL(k_wall_flux, dim3(blocks_for(nw, TPB)), dim3(TPB), 0, st, g, c->f[VFS_UCONT]);
+    VFS_LAUNCH_CHECK("wall flux");
+    rc = c2c(c);
+    if (rc) return rc;
+    int ne = g.mx > g.my ? g.mx : g.my;
+    if (g.mz > ne) ne = g.mz;
+    hipLaunchKernelGGL(k_zero_edges, dim3(blocks_for(ne, TPB)), dim3(TPB), 0, st, g, c->f[VFS_UCAT]);
+    VFS_LAUNCH_CHECK("zero edges");
+    return VFS_OK;
+}
+
+// ---------------------------------------------------------------------------
+// a4: Compute_eddy_viscosity_LES (les.c:1143-1361)
+// ---------------------------------------------------------------------------
+
+__global__ void k_eddy(Grid g, double cs2, const double *__restrict__ ucat, const double *__restrict__ csi,
+                       const double *__restrict__ eta, const double *__restrict__ zet, const double *__restrict__ aj,
+                       const double *__restrict__ nvert, double *__restrict__ nut)
+{
+    Decode d;
+    if (!decode(g, blockIdx.x * (long long)blockDim.x + threadIdx.x, d)) return;
+    const int i = d.i, j = d.j, k = d.k;
+    const long long q = gidx(g, k, j, i);
+    if (i < 1 || i >= g.mx - 1 || j < 1 || j >= g.my - 1 || k < 1 || k >= g.mz - 1 || nvert[q] > 1.1) {
+        nut[q] = 0;
+        return;
+    }
+    const double SOL = 0.1;
+#define U(c, kk, jj, ii) ucat[3 * gidx(g, (kk), (jj), (ii)) + (c)]
+#define NV(kk, jj, ii) nvert[gidx(g, (kk), (jj), (ii))]
+    double dd[3][3];
+    for (int c = 0; c < 3; c++) {
+        if (NV(k, j, i + 1) > SOL || (!g.pi && i == g.mx - 2)) dd[c][0] = (U(c, k, j, i) - U(c, k, j, i - 1));
+        else if (NV(k, j, i - 1) > SOL || (!g.pi && i == 1)) dd[c][0] = (U(c, k, j, i + 1) - U(c, k, j, i));
+        else dd[c][0] = (U(c, k, j, i + 1) - U(c, k, j, i - 1)) * 0.5;
+        if (NV(k, j + 1, i) > SOL || (!g.pj && j == g.my - 2)) dd[c][1] = (U(c, k, j, i) - U(c, k, j - 1, i));
+        else if (NV(k, j - 1, i) > SOL || (!g.pj && j == 1)) dd[c][1] = (U(c, k, j + 1, i) - U(c, k, j, i));
+        else dd[c][1] = (U(c, k, j + 1, i) - U(c, k, j - 1, i)) * 0.5;
+        // k-omega.c:390-395 tests k==0 for the lower one-sided case
+        if (NV(k + 1, j, i) > SOL || (!g.pk && k == g.mz - 2)) dd[c][2] = (U(c, k, j, i) - U(c, k - 1, j, i));
+        else if (NV(k - 1, j, i) > SOL || (!g.pk && k == 0)) dd[c][2] = (U(c, k + 1, j, i) - U(c, k, j, i));
+        else dd[c][2] = (U(c, k + 1, j, i) - U(c, k - 1, j, i)) * 0.5;
+    }
+#undef U
+#undef NV
+    const double ajc = aj[q];
+    const double cs[3] = {csi[3 * q], csi[3 * q + 1], csi[3 * q + 2]};
+    const double et[3] = {eta[3 * q], eta[3 * q + 1], eta[3 * q + 2]};
+    const double ze[3] = {zet[3 * q], zet[3 * q + 1], zet[3 * q + 2]};
+    double gr[3][3];
+    for (int c = 0; c < 3; c++)
+        for (int r = 0; r < 3; r++) gr[c][r] = (dd[c][0] * cs[r] + dd[c][1] * et[r] + dd[c][2] * ze[r]) * ajc;
+    double Sxx = 0.5 * (gr[0][0] + gr[0][0]), Sxy = 0.5 * (gr[0][1] + gr[1][0]), Sxz = 0.5 * (gr[0][2] + gr[2][0]);
+    double Syx = Sxy, Syy = 0.5 * (gr[1][1] + gr[1][1]), Syz = 0.5 * (gr[1][2] + gr[2][1]);
+    double Szx = Sxz, Szy = Syz, Szz = 0.5 * (gr[2][2] + gr[2][2]);
+    double Sabs = sqrt(2.0 * (Sxx * Sxx + Sxy * Sxy + Sxz * Sxz + Syx * Syx + Syy * Syy + Syz * Syz + Szx * Szx +
+                              Szy * Szy + Szz * Szz));
+    double filter = pow(1. / ajc, 1. / 3.);
+    nut[q] = cs2 * pow(filter, 2.0) * Sabs;
+}
+
+int eddy_viscosity(vfs_ctx *c)
+{
+    const Grid &g = c->g;
+    if (!c->p.les) {
+        VFS_HIP(hipMemsetAsync(c->f[VFS_NU_T], 0, (size_t)g.n * sizeof(double), c->stream));
+        return VFS_OK;
+    }
+    hipLaunchKernelGGL(k_eddy, dim3(blocks_for(g.n, TPB)), dim3(TPB), 0, c->stream, g, c->p.cs2, c->f[VFS_UCAT],
+                       c->f[VFS_CSI], c->f[VFS_ETA], c->f[VFS_ZET], c->f[VFS_AJ], c->f[VFS_NVERT], c->f[VFS_NU_T]);
+    VFS_LAUNCH_CHECK("eddy viscosity");
+    return periodic_fix(c, c->f[VFS_NU_T], 1, 0);
+}
+
+// ---------------------------------------------------------------------------
+// Calc_k_Flux (rhs2.c:1549-1633), mean_k_area (bcs.c:137-199), min dt
+// ---------------------------------------------------------------------------
+
+// mode 0: sum of ucont.z over interior k faces; 1: sum of |zet|; 2: min of
+// |1/(ucont.c * aj)| over interior cells (Calc_Minimum_dt, momentum.c:157-161)
+__global__ void k_interior_reduce(Grid g, int mode, const double *__restrict__ ucont, const double *__restrict__ zet,
+                                  const double *__restrict__ aj, const double *__restrict__ nvert,
+                                  double *__restrict__ part)
+{
+    const long long ni = (long long)(g.mx - 2) * (g.my - 2) * (g.mz - 2);
+    double v = mode == 2 ? 1.e7 : 0.0;
+    for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < ni; t += (long long)gridDim.x * blockDim.x) {
+        int i = 1 + t % (g.mx - 2);
+        long long r = t / (g.mx - 2);
+        int j = 1 + r % (g.my - 2), k = 1 + r / (g.my - 2);
+        long long q = gidx(g, k, j, i);
+        if (mode == 2) {
+            double a = aj[q];
+            for (int cc = 0; cc < 3; cc++) v = fmin(fabs(1. / ucont[3 * q + cc] / a), v);
+        } else if (nvert[gidx(g, k - 1, j, i)] + nvert[q] < 0.1) {
+            if (mode == 0) v += ucont[3 * q + 2];
+            else v += sqrt(zet[3 * q] * zet[3 * q] + zet[3 * q + 1] * zet[3 * q + 1] + zet[3 * q + 2] * zet[3 * q + 2]);
+        }
+    }
+    v = mode == 2 ? block_reduce<OpMin>(v) : block_reduce<OpSum>(v);
+    if (threadIdx.x == 0) part[blockIdx.x] = v;
+}
+
+static int interior_reduce(vfs_ctx *c, int mode, double *out)
+{
+    hipStream_t st = c->stream;
+    hipLaunchKernelGGL(k_interior_reduce, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, st, c->g, mode, c->f[VFS_UCONT],
+                       c->f[VFS_ZET], c->f[VFS_AJ], c->f[VFS_NVERT], c->red);
+    if (mode == 2) hipLaunchKernelGGL(finish_reduce<OpMin>, dim3(1), dim3(RED_THREADS), 0, st, c->red, RED_BLOCKS, c->red + RED_BLOCKS);
+    else hipLaunchKernelGGL(finish_reduce<OpSum>, dim3(1), dim3(RED_THREADS), 0, st, c->red, RED_BLOCKS, c->red + RED_BLOCKS);
+    VFS_LAUNCH_CHECK("interior reduce");
+    VFS_HIP(hipMemcpyAsync(c->red_host, c->red + RED_BLOCKS, sizeof(double), hipMemcpyDeviceToHost, st));
+    VFS_HIP(hipStreamSynchronize(st));
+    *out = c->red_host[0];
+    return VFS_OK;
+}
+
+int calc_k_flux(vfs_ctx *c, double *out)
+{
+    double s;
+    int rc = interior_reduce(c, 0, &s);
+    if (rc) return rc;
+    *out = s / (c->g.mz - 2);
+    return VFS_OK;
+}
+
+int mean_k_area(vfs_ctx *c, double *out)
+{
+    double s;
+    int rc = interior_reduce(c, 1, &s);
+    if (rc) return rc;
+    *out = s / (c->g.mz - 2);
+    return VFS_OK;
+}
+
+int min_dt(vfs_ctx *c, double *out) { return interior_reduce(c, 2, out); }
+
+// ---------------------------------------------------------------------------
+// a5: Pressure_Gradient (momentum.c:203-439)
+// ---------------------------------------------------------------------------
+
+__global__ void k_pgrad(Grid g, double fdiff, double fdt, double farea, const double *__restrict__ p, const double *__restrict__ nvert,
+                        const double *__restrict__ icsi, const double *__restrict__ ieta, const double *__restrict__ izet,
+                        const double *__restrict__ iaj, const double *__restrict__ jcsi, const double *__restrict__ jeta,
+                        const double *__restrict__ jzet, const double *__restrict__ jaj, const double *__restrict__ kcsi,
+                        const double *__restrict__ keta, const double *__restrict__ kzet, const double *__restrict__ kaj,
+                        double *__restrict__ dp)
+{
+    Decode d;
+    if (!decode(g, blockIdx.x * (long long)blockDim.x + threadIdx.x, d)) return;
+    const int i = d.i, j = d.j, k = d.k;
+    const long long q = gidx(g, k, j, i);
+    const int mx = g.mx, my = g.my, mz = g.mz;
+    if (i < 1 || i >= mx - 1 || j < 1 || j >= my - 1 || k < 1 || k >= mz - 1) {
+        dp[3 * q] = dp[3 * q + 1] = dp[3 * q + 2] = 0;
+        return;
+    }
+#define P(kk, jj, ii) p[gidx(g, (kk), (jj), (ii))]
+#define N1(kk, jj, ii) ((int)(nvert[gidx(g, (kk), (jj), (ii))] + 0.5) == 1)
+#define NVS(kk, jj, ii) nvert[gidx(g, (kk), (jj), (ii))]
+    const int pi_ = g.pi, pj = g.pj, pk = g.pk;
+    const double *ic = icsi + 3 * q, *ie = ieta + 3 * q, *iz = izet + 3 * q;
+    const double *jc = jcsi + 3 * q, *je = jeta + 3 * q, *jz = jzet + 3 * q;
+    const double *kc = kcsi + 3 * q, *ke = keta + 3 * q, *kz = kzet + 3 * q;
+    double g11_i = ic[0] * ic[0] + ic[1] * ic[1] + ic[2] * ic[2];
+    double g12_i = ie[0] * ic[0] + ie[1] * ic[1] + ie[2] * ic[2];
+    double g13_i = iz[0] * ic[0] + iz[1] * ic[1] + iz[2] * ic[2];
+    double g21_j = jc[0] * je[0] + jc[1] * je[1] + jc[2] * je[2];
+    double g22_j = je[0] * je[0] + je[1] * je[1] + je[2] * je[2];
+    double g23_j = jz[0] * je[0] + jz[1] * je[1] + jz[2] * je[2];
+    double g31_k = kc[0] * kz[0] + kc[1] * kz[1] + kc[2] * kz[2];
+    double g32_k = ke[0] * kz[0] + ke[1] * kz[1] + ke[2] * kz[2];
+    double g33_k = kz[0] * kz[0] + kz[1] * kz[1] + kz[2] * kz[2];
+    double dpdc, dpde, dpdz;
+    if (i == mx - 2 && pi_) dpdc = P(k, j, mx + 1) - P(k, j, i);
+    else dpdc = P(k, j, i + 1) - P(k, j, i);
+    if (N1(k, j + 1, i) || N1(k, j + 1, i + 1) || (j == my - 2 && !pj))
+        dpde = (P(k, j, i) - P(k, j - 1, i) + P(k, j, i + 1) - P(k, j - 1, i + 1)) * 0.5;
+    else if (N1(k, j - 1, i) || N1(k, j - 1, i + 1) || (j == 1 && !pj))
+        dpde = (P(k, j + 1, i) - P(k, j, i) + P(k, j + 1, i + 1) - P(k, j, i + 1)) * 0.5;
+    else
+        dpde = (P(k, j + 1, i) - P(k, j - 1, i) + P(k, j + 1, i + 1) - P(k, j - 1, i + 1)) * 0.25;
+    if (N1(k + 1, j, i) || N1(k + 1, j, i + 1) || (k == mz - 2 && !pk))
+        dpdz = (P(k, j, i) - P(k - 1, j, i) + P(k, j, i + 1) - P(k - 1, j, i + 1)) * 0.5;
+    else if (N1(k - 1, j, i) || N1(k - 1, j, i + 1) || (k == 1 && !pk))
+        dpdz = (P(k + 1, j, i) - P(k, j, i) + P(k + 1, j, i + 1) - P(k, j, i + 1)) * 0.5;
+    else
+        dpdz = (P(k + 1, j, i) - P(k - 1, j, i) + P(k + 1, j, i + 1) - P(k - 1, j, i + 1)) * 0.25;
+    double vx = (dpdc * g11_i + dpde * g12_i + dpdz * g13_i) * iaj[q];
+    if (N1(k, j, i + 1) || N1(k, j + 1, i + 1) || (i == mx - 2 && !pi_))
+        dpdc = (P(k, j, i) - P(k, j, i - 1) + P(k, j + 1, i) - P(k, j + 1, i - 1)) * 0.5;
+    else if (N1(k, j, i - 1) || N1(k, j + 1, i - 1) || (i == 1 && !pi_))
+        dpdc = (P(k, j, i + 1) - P(k, j, i) + P(k, j + 1, i + 1) - P(k, j + 1, i)) * 0.5;
+    else
+        dpdc = (P(k, j, i + 1) - P(k, j, i - 1) + P(k, j + 1, i + 1) - P(k, j + 1, i - 1)) * 0.25;
+    if (j == my - 2 && pj) dpde = P(k, my + 1, i) - P(k, j, i);
+    else dpde = P(k, j + 1, i) - P(k, j, i);
+    if (N1(k + 1, j, i) || N1(k + 1, j + 1, i) || (k == mz - 2 && !pk))
+        dpdz = (P(k, j, i) - P(k - 1, j, i) + P(k, j + 1, i) - P(k - 1, j + 1, i)) * 0.5;
+    else if (N1(k - 1, j, i) || N1(k - 1, j + 1, i) || (k == 1 && !pk))
+        dpdz = (P(k + 1, j, i) - P(k, j, i) + P(k + 1, j + 1, i) - P(k, j + 1, i)) * 0.5;
+    else
+        dpdz = (P(k + 1, j, i) - P(k - 1, j, i) + P(k + 1, j + 1, i) - P(k - 1, j + 1, i)) * 0.25;
+    double vy = (dpdc * g21_j + dpde * g22_j + dpdz * g23_j) * jaj[q];
+    if (N1(k, j, i + 1) || N1(k + 1, j, i + 1) || (i == mx - 2 && !pi_))
+        dpdc = (P(k, j, i) - P(k, j, i - 1) + P(k + 1, j, i) - P(k + 1, j, i - 1)) * 0.5;
+    else if (N1(k, j, i - 1) || N1(k + 1, j, i - 1) || (i == 1 && !pi_))
+        dpdc = (P(k, j, i + 1) - P(k, j, i) + P(k + 1, j, i + 1) - P(k + 1, j, i)) * 0.5;
+    else
+        dpdc = (P(k, j, i + 1) - P(k, j, i - 1) + P(k + 1, j, i + 1) - P(k + 1, j, i - 1)) * 0.25;
+    if (N1(k, j + 1, i) || N1(k + 1, j + 1, i) || (j == my - 2 && !pj))
+        dpde = (P(k, j, i) - P(k, j - 1, i) + P(k + 1, j, i) - P(k + 1, j - 1, i)) * 0.5;
+    else if (N1(k, j - 1, i) || N1(k + 1, j - 1, i) || (j == 1 && !pj))
+        dpde = (P(k, j + 1, i) - P(k, j, i) + P(k + 1, j + 1, i) - P(k + 1, j, i)) * 0.5;
+    else
+        dpde = (P(k, j + 1, i) - P(k, j - 1, i) + P(k + 1, j + 1, i) - P(k + 1, j - 1, i)) * 0.25;
+    if (k == mz - 2 && pk) dpdz = P(mz + 1, j, i) - P(k, j, i);
+    else dpdz = (P(k + 1, j, i) - P(k, j, i));
+    if (pk) {
+        double dz = 1. / kaj[q] / sqrt(kz[0] * kz[0] + kz[1] * kz[1] + kz[2] * kz[2]);
+        dpdz += dz * fdiff / fdt / farea;
+    }
+    double vz = (dpdc * g31_k + dpde * g32_k + dpdz * g33_k) * kaj[q];
+    if (i == 0 || NVS(k, j, i) + NVS(k, j, i + 1) > 0.1 || (!pi_ && i == mx - 2)) vx = 0;
+    if (j == 0 || NVS(k, j, i) + NVS(k, j + 1, i) > 0.1 || (!pj && j == my - 2)) vy = 0;
+    if (k == 0 || NVS(k, j, i) + NVS(k + 1, j, i) > 0.1 || (!pk && k == mz - 2)) vz = 0;
+    dp[3 * q] = vx; dp[3 * q + 1] = vy; dp[3 * q + 2] = vz;
+#undef P
+#undef N1
+#undef NVS
+}
+
+int pressure_gradient(vfs_ctx *c)
+{
+    const Grid &g = c->g;
+    int rc = periodic_fix(c, c->f[VFS_P], 1, 0);  // momentum.c:249-279
+    if (rc) return rc;
+    // (mean_k_flux - inlet_flux) / dt / mean_k_area, momentum.c:386-399
+    const double fdiff = c->mean_k_flux - c->p.inlet_flux;
+    hipLaunchKernelGGL(k_pgrad, dim3(blocks_for(g.n, TPB)), dim3(TPB), 0, c->stream, g, fdiff, c->p.dt,
+                       c->mean_k_area, c->f[VFS_P],
+                       c->f[VFS_NVERT], c->f[VFS_ICSI], c->f[VFS_IETA], c->f[VFS_IZET], c->f[VFS_IAJ],
+                       c->f[VFS_JCSI], c->f[VFS_JETA], c->f[VFS_JZET], c->f[VFS_JAJ], c->f[VFS_KCSI],
+                       c->f[VFS_KETA], c->f[VFS_KZET], c->f[VFS_KAJ], c->f[VFS_DP]);
+    VFS_LAUNCH_CHECK("pressure gradient");
+    return VFS_OK;
+}
+
+}  // namespace vfs
