@@ -39,6 +39,13 @@ ALGO_BYTES = {
 }
 
 
+# HBM bytes per launch of the finest-level launch, from rocprofv3 FETCH_SIZE (x2, calibrated on
+# copy kernels of known size) + WRITE_SIZE on this workload at 256x192x256
+MEASURED_TRAFFIC = {
+    "line_jacobi_fp32": 1.497e9,
+}
+
+
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
@@ -135,7 +142,8 @@ def main():
     if dom:
         a = kern[dom]["GBs"]
         roofline = {"bound": "hbm", "kernel": dom, "achieved": round(a, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": round(a / HBM_PEAK_GBS, 4), "traffic": None,
+                    "frac": round(a / HBM_PEAK_GBS, 4),
+                    "traffic": MEASURED_TRAFFIC.get(dom) if (args.nx, args.ny, args.nz) == (256, 192, 256) else None,
                     "bytes_per_point": ALGO_BYTES[dom], "avg_launch_ms": round(kern[dom]["avg_ms"], 4)}
     log("kernels: " + json.dumps({k: {kk: round(vv, 3) for kk, vv in v.items()} for k, v in kern.items()}))
     ctx.close()

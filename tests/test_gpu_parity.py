@@ -140,3 +140,22 @@ def test_update_projection_divergence(pair):
     d_cpu = o.divergence()
     d_gpu = ctx.divergence()
     assert abs(d_gpu - d_cpu) <= 1e-12 * max(1.0, abs(d_cpu))
+
+
+def test_poisson_solve_after_reused_memory():
+    """Regression: buffers of a new solver must not inherit garbage from freed
+    device memory (a NaN left in a never-written ghost cell used to poison the
+    line smoother through a zero coefficient)."""
+    c, o, ctx = channel_pair(24, 16, 20, les=0)
+    shape = o.shape
+    junk = vfs.Context(ctx.params)
+    for name in vfs.FIELDS:
+        junk.upload(name, np.full(junk.field_shape(name), np.nan))
+    junk.close()
+    o.runge_kutta()
+    ctx.upload("UCONT", o["ucont"])
+    ctx.poisson_rhs()
+    ctx.upload("PHI", np.zeros(shape))
+    st = ctx.poisson_solve(vfs.default_solve_opts(tol=1e-8, mg_fp32=1, use_guess=0))
+    assert st.relres <= 1e-8 and np.isfinite(ctx.download("PHI")).all()
+    ctx.close()

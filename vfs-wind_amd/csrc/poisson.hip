@@ -22,6 +22,8 @@
 
 #include <chrono>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -547,11 +549,16 @@ struct vfs_poisson {
 
 namespace vfs {
 
+// Every buffer starts zeroed: the smoother never writes ghost cells of its
+// output vector, whose zero values are then read (times a zero coefficient)
+// by the next sweep; garbage there could hold NaN.
 static int palloc(vfs_poisson *ps, void **p, size_t bytes)
 {
     hipError_t e = hipMalloc(p, bytes);
     if (e != hipSuccess) return check_hip(e, "hipMalloc (poisson)");
     ps->allocs.push_back(*p);
+    e = hipMemset(*p, 0, bytes);
+    if (e != hipSuccess) return check_hip(e, "hipMemset (poisson)");
     return VFS_OK;
 }
 
@@ -817,6 +824,8 @@ int poisson_solve(vfs_ctx *c, const vfs_solve_opts *o, vfs_solve_stats *stats)
     const double tol = o->tol > 0 ? o->tol : 1e-8;
     const int maxit = o->maxit > 0 ? o->maxit : 200;
     bool converged = false;
+    const char *tr = std::getenv("VFS_SOLVER_TRACE");
+    const bool trace = tr && tr[0] == '1';
     while (it < maxit) {
         // r = b - A x into V_0
         hipLaunchKernelGGL(k_apply<double>, dim3(nb), dim3(PTPB), 0, st, L.g, L.c64, L.row, x, b, V);
@@ -824,6 +833,7 @@ int poisson_solve(vfs_ctx *c, const vfs_solve_opts *o, vfs_solve_stats *stats)
         if (rc) return rc;
         double beta = std::sqrt(d[0]);
         rel = beta / bnorm;
+        if (trace) std::fprintf(stderr, "fgmres restart at it %d: true relres %.3e (bnorm %.3e)\n", it, rel, bnorm);
         if (rel <= tol) { converged = true; break; }
         c->red_host[0] = 1.0 / beta;
         VFS_HIP(hipMemcpyAsync(ps->dev_scalars, c->red_host, sizeof(double), hipMemcpyHostToDevice, st));
@@ -878,6 +888,8 @@ int poisson_solve(vfs_ctx *c, const vfs_solve_opts *o, vfs_solve_stats *stats)
             H[(size_t)(jj + 1) * m + jj] = 0;
             gv[jj + 1] = -sn[jj] * gv[jj];
             gv[jj] = cs[jj] * gv[jj];
+            if (trace) std::fprintf(stderr, "  it %d: est relres %.3e  h_{j+1,j} %.3e  ww0 %.3e ww %.3e\n", it,
+                                    std::fabs(gv[jj + 1]) / bnorm, hn, ww0, ww);
             if (std::fabs(gv[jj + 1]) / bnorm <= tol) { jj++; it++; break; }
         }
         for (int i = jj - 1; i >= 0; i--) {
