@@ -25,25 +25,27 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 
-# algorithmic bytes per grid point of each timed kernel: every input read
-# once, every output written once (FP64 unless stated)
-ALGO_BYTES = {
+# Algorithmic bytes per unit of each timed kernel (every input read once, every
+# output written once, FP64 unless stated) and the units one launch covers:
+# "point" = every DA point (the launch grid), "line point" = interior points of
+# one smoother colour, (mz-2)(my-2)ceil((mx-2)/2).
+ALGO = {
     # ucont 24 + ucat 24 + nu_t 8 + nvert 8 + 3 face metric families (9 + 1 doubles) 240; 18 fluxes out 144
-    "face_fluxes": 24 + 24 + 8 + 8 + 240 + 144,
+    "face_fluxes": (24 + 24 + 8 + 8 + 240 + 144, "point"),
     # 18 fluxes 144 + csi/eta/zet 72 + iaj/jaj/kaj 24 + nvert 8 + dP 24 + pUcont 24; Ucont out 24
-    "cell_rhs": 144 + 72 + 24 + 8 + 24 + 24 + 24,
-    # FP32: 19 coefficients 76 + x 4 + b 4 + row mask 1; x out 4
-    "line_jacobi_fp32": 76 + 4 + 4 + 1 + 4,
+    "cell_rhs": (144 + 72 + 24 + 8 + 24 + 24 + 24, "point"),
+    # FP32: 16 off-line coefficients 64 + b 4 + x 4; line rhs out 4
+    "line_rhs_fp32": (64 + 4 + 4 + 4, "line point"),
+    # FP32: Thomas factors lo, 1/pivot, upper 12 + line rhs 4; x out 4
+    "line_solve_fp32": (12 + 4 + 4, "line point"),
     # FP64: 19 coefficients 152 + x 8 + row mask 1; y out 8
-    "poisson_apply_fp64": 152 + 8 + 1 + 8,
+    "poisson_apply_fp64": (152 + 8 + 1 + 8, "point"),
 }
 
 
 # HBM bytes per launch of the finest-level launch, from rocprofv3 FETCH_SIZE (x2, calibrated on
 # copy kernels of known size) + WRITE_SIZE on this workload at 256x192x256
-MEASURED_TRAFFIC = {
-    "line_jacobi_fp32": 1.497e9,
-}
+MEASURED_TRAFFIC = {}
 
 
 def log(msg):
@@ -128,23 +130,26 @@ def main():
     value = cells * args.steps / elapsed / 1e6
 
     # per-kernel timing inside the timed region
-    npts = params.mx * params.my * params.mz
+    units = {"point": params.mx * params.my * params.mz,
+             "line point": (params.mz - 2) * (params.my - 2) * ((params.mx - 1) // 2)}
     kern = {}
-    for name in ALGO_BYTES:
+    for name, (bpu, unit) in ALGO.items():
         ms, cnt = ctx.profile_read(name)
         if cnt:
             avg = ms / cnt
             kern[name] = {"total_ms": ms, "launches": cnt, "avg_ms": avg,
-                          "GBs": ALGO_BYTES[name] * npts / (avg * 1e-3) / 1e9}
+                          "GBs": bpu * units[unit] / (avg * 1e-3) / 1e9}
     ctx.profile(False)
     dom = max(kern, key=lambda k: kern[k]["total_ms"]) if kern else None
     roofline = None
     if dom:
         a = kern[dom]["GBs"]
+        bpu, unit = ALGO[dom]
         roofline = {"bound": "hbm", "kernel": dom, "achieved": round(a, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                     "frac": round(a / HBM_PEAK_GBS, 4),
                     "traffic": MEASURED_TRAFFIC.get(dom) if (args.nx, args.ny, args.nz) == (256, 192, 256) else None,
-                    "bytes_per_point": ALGO_BYTES[dom], "avg_launch_ms": round(kern[dom]["avg_ms"], 4)}
+                    "bytes_per_unit": bpu, "unit_of_work": unit, "units_per_launch": units[unit],
+                    "avg_launch_ms": round(kern[dom]["avg_ms"], 4)}
     log("kernels: " + json.dumps({k: {kk: round(vv, 3) for kk, vv in v.items()} for k, v in kern.items()}))
     ctx.close()
 

@@ -114,9 +114,9 @@ typedef struct {
     double tol;          /* relative residual ||b-A phi||/||b|| (1e-8)       */
     int restart;         /* FGMRES restart length (20)                       */
     int maxit;           /* total iteration cap                              */
-    int levels;          /* multigrid levels (4)                             */
+    int levels;          /* most multigrid levels (6); fewer if a grid stops halving */
     int pre, post;       /* smoothing sweeps (2, 2)                          */
-    int coarse_sweeps;   /* sweeps on the coarsest level                     */
+    int coarse_sweeps;   /* coarsest-level sweeps when it is too big for a dense solve */
     int mg_fp32;         /* 1: FP32 V-cycle inside the FP64 FGMRES           */
     int use_guess;       /* 1: start from VFS_PHI; 0: from zero              */
 } vfs_solve_opts;
@@ -158,9 +158,10 @@ int vfs_fractional_step(vfs_ctx *ctx, const vfs_solve_opts *opts, vfs_step_stats
 /* Enable (1) or disable (0) HIP-event timing of the hot kernels' launches on
  * the context stream and clear earlier records.  For benchmarks only. */
 int vfs_profile(vfs_ctx *ctx, int enable);
-/* Total time and launch count recorded for kernel id 0..7: face_fluxes,
- * cell_rhs, line_jacobi_fp32, line_jacobi_fp64, poisson_apply_fp64,
- * contra2cart, pressure_gradient, eddy_viscosity. */
+/* Total time and launch count recorded for kernel id 0..9 (finest level for
+ * the smoother): face_fluxes, cell_rhs, line_rhs_fp32, line_rhs_fp64,
+ * poisson_apply_fp64, contra2cart, pressure_gradient, eddy_viscosity,
+ * line_solve_fp32, line_solve_fp64. */
 int vfs_profile_read(vfs_ctx *ctx, int kernel, double *total_ms, long long *count, const char **name);
 
 #ifdef __cplusplus

@@ -182,7 +182,7 @@ class Oracle:
         self.lib.orc_poisson_apply(C.byref(self.p), C.byref(self.cm), _ptr(x), _ptr(y))
         return y
 
-    def poisson_solve(self, tol=1e-8, restart=20, maxit=400, levels=4):
+    def poisson_solve(self, tol=1e-8, restart=20, maxit=400, levels=6):
         rel = C.c_double()
         it = self.lib.orc_poisson_solve(C.byref(self.p), C.byref(self.cm), self._f("b"), self._f("phi"), tol,
                                         restart, maxit, levels, C.byref(rel))

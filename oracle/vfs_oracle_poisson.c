@@ -497,48 +497,49 @@ static void mg_free(mg_t *mg)
     }
 }
 
-/* Damped Jacobi over wall-normal j lines (omega = 0.8): every line solves its
- * tridiagonal (SP, CP, NP) part exactly with the off-line couplings taken
- * from the previous iterate; requires a non-periodic j direction. */
-static const double LINE_OMEGA = 0.8;
-
+/* Zebra Gauss-Seidel over wall-normal j lines: the lines (i,k) of colour
+ * (i+k)&1 == 0 are relaxed first, then colour 1.  Within one colour every line
+ * takes its off-line neighbours from the values at the start of that colour's
+ * half-sweep and is solved exactly by the Thomas algorithm. */
 static void smooth_lines(const level_t *L, double *x, const double *b)
 {
     const orc_params *p = &L->p;
     const int my = p->my;
     size_t n = npts(p);
     double *xin = dalloc(n), *cp = dalloc(my), *dp = dalloc(my);
-    memcpy(xin, x, n * sizeof(double));
-    for (int k = 1; k < p->mz - 1; k++)
-        for (int i = 1; i < p->mx - 1; i++) {
-            int have_prev = 0;
-            for (int j = 1; j < my - 1; j++) {
-                size_t q = IX(p, k, j, i);
-                if (!L->row[q]) { cp[j] = 0; dp[j] = 0; have_prev = 0; continue; }
-                const double *c = &L->coef[NSTENCIL * q];
-                double rhs = b[q];
-                for (int t = 0; t < NSTENCIL; t++) {
-                    if (t == CP || t == NP || t == SP) continue;
-                    rhs -= c[t] * xin[nbr(p, k, j, i, t)];
+    for (int col = 0; col < 2; col++) {
+        memcpy(xin, x, n * sizeof(double));
+        for (int k = 1; k < p->mz - 1; k++)
+            for (int i = 1; i < p->mx - 1; i++) {
+                if (((i + k) & 1) != col) continue;
+                int have_prev = 0;
+                for (int j = 1; j < my - 1; j++) {
+                    size_t q = IX(p, k, j, i);
+                    if (!L->row[q]) { cp[j] = 0; dp[j] = 0; have_prev = 0; continue; }
+                    const double *c = &L->coef[NSTENCIL * q];
+                    double rhs = b[q];
+                    for (int t = 0; t < NSTENCIL; t++) {
+                        if (t == CP || t == NP || t == SP) continue;
+                        rhs -= c[t] * xin[nbr(p, k, j, i, t)];
+                    }
+                    double lo = have_prev ? c[SP] : 0.;
+                    double den = c[CP] - lo * (have_prev ? cp[j - 1] : 0.);
+                    cp[j] = c[NP] / den;
+                    dp[j] = (rhs - lo * (have_prev ? dp[j - 1] : 0.)) / den;
+                    have_prev = 1;
                 }
-                double lo = have_prev ? c[SP] : 0.;
-                double up = c[NP];
-                double den = c[CP] - lo * (have_prev ? cp[j - 1] : 0.);
-                cp[j] = up / den;
-                dp[j] = (rhs - lo * (have_prev ? dp[j - 1] : 0.)) / den;
-                have_prev = 1;
+                double next = 0;
+                int next_valid = 0;
+                for (int j = my - 2; j >= 1; j--) {
+                    size_t q = IX(p, k, j, i);
+                    if (!L->row[q]) { x[q] = 0; next_valid = 0; continue; }
+                    double xl = dp[j] - (next_valid ? cp[j] * next : 0.);
+                    next = xl;
+                    next_valid = 1;
+                    x[q] = xl;
+                }
             }
-            double next = 0;
-            int next_valid = 0;
-            for (int j = my - 2; j >= 1; j--) {
-                size_t q = IX(p, k, j, i);
-                if (!L->row[q]) { x[q] = 0; next_valid = 0; continue; }
-                double xl = dp[j] - (next_valid ? cp[j] * next : 0.);
-                next = xl;
-                next_valid = 1;
-                x[q] = xin[q] + LINE_OMEGA * (xl - xin[q]);
-            }
-        }
+    }
     free(xin); free(cp); free(dp);
 }
 
@@ -627,6 +628,13 @@ int orc_poisson_solve(const orc_params *p, const orc_metrics *m, const double *b
 {
     mg_t mg;
     if (!m->coor) levels = 1;
+    {   /* coarsen only while every interior cell count halves exactly */
+        int nl = 1, cx = p->mx - 2, cy = p->my - 2, cz = p->mz - 2;
+        while (nl < levels && nl < 8 && cx % 2 == 0 && cy % 2 == 0 && cz % 2 == 0 && cx >= 4 && cy >= 4 && cz >= 4) {
+            cx /= 2; cy /= 2; cz /= 2; nl++;
+        }
+        levels = nl;
+    }
     mg_build(&mg, p, m, levels);
     level_t *L = &mg.L[0];
     size_t n = npts(p);
@@ -892,7 +900,7 @@ int orc_fractional_step(orc_params *p, const orc_metrics *m, double *ucont, doub
     /* solvers.c:650-655 */
     double *b = dalloc(n);
     orc_poisson_rhs(p, m, ucont, b);
-    int it = orc_poisson_solve(p, m, b, phi, tol, 20, 400, 4, relres);
+    int it = orc_poisson_solve(p, m, b, phi, tol, 20, 400, 6, relres);
     free(b);
     /* solvers.c:660-666 */
     orc_update_pressure(p, m, pres, phi);

@@ -101,8 +101,9 @@ _SIGS = {
                           C.POINTER(C.c_char_p)], C.c_int),
 }
 
-KERNEL_IDS = {"face_fluxes": 0, "cell_rhs": 1, "line_jacobi_fp32": 2, "line_jacobi_fp64": 3,
-              "poisson_apply_fp64": 4, "contra2cart": 5, "pressure_gradient": 6, "eddy_viscosity": 7}
+KERNEL_IDS = {"face_fluxes": 0, "cell_rhs": 1, "line_rhs_fp32": 2, "line_rhs_fp64": 3,
+              "poisson_apply_fp64": 4, "contra2cart": 5, "pressure_gradient": 6, "eddy_viscosity": 7,
+              "line_solve_fp32": 8, "line_solve_fp64": 9}
 
 
 class VFSError(RuntimeError):
@@ -150,7 +151,7 @@ def make_params(mx, my, mz, *, per=(1, 0, 1), bctype=(100, 100, 1, 1, 100, 100),
     return p
 
 
-def default_solve_opts(tol=1e-8, restart=20, maxit=200, levels=4, pre=2, post=2, coarse_sweeps=20, mg_fp32=1,
+def default_solve_opts(tol=1e-8, restart=20, maxit=200, levels=6, pre=2, post=2, coarse_sweeps=20, mg_fp32=1,
                        use_guess=1) -> SolveOpts:
     o = SolveOpts()
     o.tol, o.restart, o.maxit, o.levels = tol, restart, maxit, levels

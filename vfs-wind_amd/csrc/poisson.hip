@@ -37,15 +37,20 @@ namespace vfs {
 constexpr int PTPB = 256;
 constexpr int NST = 19;
 enum { CP, EP, WP, NP, SP, TP, BP, NE, SE, NW, SW, TN, BN, TS, BS, TE, BE, TW, BW };
-__constant__ int c_off[NST][3] = {{0, 0, 0},  {1, 0, 0},   {-1, 0, 0},  {0, 1, 0},  {0, -1, 0},
-                                  {0, 0, 1},  {0, 0, -1},  {1, 1, 0},   {1, -1, 0}, {-1, 1, 0},
-                                  {-1, -1, 0}, {0, 1, 1},  {0, 1, -1},  {0, -1, 1}, {0, -1, -1},
-                                  {1, 0, 1},  {1, 0, -1},  {-1, 0, 1},  {-1, 0, -1}};
+// stencil offsets (di, dj, dk) of each slot
+__host__ __device__ constexpr int soff(int s, int d)
+{
+    constexpr int o[NST][3] = {{0, 0, 0},  {1, 0, 0},   {-1, 0, 0},  {0, 1, 0},  {0, -1, 0},
+                               {0, 0, 1},  {0, 0, -1},  {1, 1, 0},   {1, -1, 0}, {-1, 1, 0},
+                               {-1, -1, 0}, {0, 1, 1},  {0, 1, -1},  {0, -1, 1}, {0, -1, -1},
+                               {1, 0, 1},  {1, 0, -1},  {-1, 0, 1},  {-1, 0, -1}};
+    return o[s][d];
+}
 
 int periodic_fix_grid(hipStream_t st, const Grid &g, double *f, int nc, int mode);
 
 // periodic image of a neighbour index (gid ghost mapping, poisson.c:284-309)
-__device__ inline int canon(int i, int m, int per)
+__host__ __device__ inline int canon(int i, int m, int per)
 {
     if (per) {
         if (i <= 0) i += m - 2;
@@ -54,11 +59,11 @@ __device__ inline int canon(int i, int m, int per)
     return i;
 }
 
-__device__ inline long long nbr_idx(const Grid &g, int k, int j, int i, int s)
+__host__ __device__ inline long long nbr_idx(const Grid &g, int k, int j, int i, int s)
 {
-    int ii = canon(i + c_off[s][0], g.mx, g.pi);
-    int jj = canon(j + c_off[s][1], g.my, g.pj);
-    int kk = canon(k + c_off[s][2], g.mz, g.pk);
+    int ii = canon(i + soff(s, 0), g.mx, g.pi);
+    int jj = canon(j + soff(s, 1), g.my, g.pj);
+    int kk = canon(k + soff(s, 2), g.mz, g.pk);
     return ((long long)kk * g.my + jj) * g.mx + ii;
 }
 
@@ -303,49 +308,149 @@ __global__ void __launch_bounds__(PTPB) k_apply(Grid g, const T *__restrict__ co
     y[t] = b ? b[t] - s : s;
 }
 
-// Damped Jacobi over j lines: each thread owns the line (i,k), assembles the
-// off-line part of every row from xin, solves the tridiagonal (SP, CP, NP)
-// system by the Thomas algorithm and writes xout = xin + w*(xline - xin).
-template <typename T>
-__global__ void __launch_bounds__(PTPB) k_line_jacobi(Grid g, const T *__restrict__ coef, const unsigned char *__restrict__ row,
-                                                      const T *__restrict__ xin, const T *__restrict__ b, T *__restrict__ xout,
-                                                      T *__restrict__ cpw, T *__restrict__ dpw, T omega)
+// ---- zebra line Gauss-Seidel over wall-normal (j) lines --------------------
+// Lines (i,k) are coloured by (i+k) & 1.  A sweep relaxes colour 0 then colour
+// 1; each half-sweep is two kernels: a fully parallel pass that folds the
+// off-line part of every row into a line right-hand side (reading the other
+// colour's current values), then one thread per line runs the forward and
+// backward Thomas recurrences with factors precomputed at setup.  The
+// smoother's coefficients live in a colour-split copy so that both passes
+// read contiguous memory: plane [slot][colour][k][j][h] with i = i0 + 2h.
+constexpr int NOFF = 16;
+__host__ __device__ constexpr int offline_slot(int u)
 {
-    const int ni = g.mx - 2;
+    return u == 0 ? EP : u == 1 ? WP : u == 2 ? TP : u == 3 ? BP : u == 4 ? NE : u == 5 ? SE : u == 6 ? NW : u == 7 ? SW :
+           u == 8 ? TN : u == 9 ? BN : u == 10 ? TS : u == 11 ? BS : u == 12 ? TE : u == 13 ? BE : u == 14 ? TW : BW;
+}
+enum { SM_LO = NOFF, SM_M, SM_CP, NSM };
+
+struct ZGrid {
+    int nkI, njI, nh;      // interior k and j counts, lines per colour per k row
+    long long cplane;      // points per colour = nkI * njI * nh
+};
+
+__host__ __device__ inline int zebra_i0(int k, int c) { return 1 + ((1 + k + c) & 1); }
+
+// colour-split off-line coefficients and Thomas factors (lo, 1/pivot, upper/pivot)
+template <typename T>
+__global__ void k_zebra_setup(Grid g, ZGrid z, const double *__restrict__ c64, const unsigned char *__restrict__ row,
+                              T *__restrict__ sc)
+{
     long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-    if (t >= (long long)ni * (g.mz - 2)) return;
-    const int i = 1 + t % ni, k = 1 + t / ni;
-    const long long n = g.n;
-    bool have_prev = false;
-    T cprev = 0, dprev = 0;
-    for (int j = 1; j < g.my - 1; j++) {
-        const long long q = ((long long)k * g.my + j) * g.mx + i;
-        if (!row[q]) { cpw[q] = 0; dpw[q] = 0; have_prev = false; continue; }
-        T rhs = b[q];
+    if (t >= 2LL * z.nkI * z.nh) return;
+    const int h = t % z.nh;
+    const long long r = t / z.nh;
+    const int kk = r % z.nkI, c = r / z.nkI, k = kk + 1;
+    const int i = zebra_i0(k, c) + 2 * h;
+    const bool valid = i <= g.mx - 2;
+    const long long sstride = 2 * z.cplane;
+    double cpp = 0;
+    for (int jj = 0; jj < z.njI; jj++) {
+        const long long cq = c * z.cplane + ((long long)kk * z.njI + jj) * z.nh + h;
+        const long long q = valid ? ((long long)k * g.my + jj + 1) * g.mx + i : 0;
+        const bool rw = valid && row[q];
 #pragma unroll
-        for (int s = 0; s < NST; s++) {
-            if (s == CP || s == NP || s == SP) continue;
-            rhs -= coef[s * n + q] * xin[nbr_idx(g, k, j, i, s)];
+        for (int u = 0; u < NOFF; u++) sc[u * sstride + cq] = rw ? (T)c64[offline_slot(u) * g.n + q] : T(0);
+        double lo = 0, m = 0, cpn = 0;
+        if (rw) {
+            lo = c64[SP * g.n + q];
+            m = 1.0 / (c64[CP * g.n + q] - lo * cpp);
+            cpn = c64[NP * g.n + q] * m;
         }
-        const T lo = have_prev ? coef[SP * n + q] : T(0);
-        const T up = coef[NP * n + q];  // 0 when j+1 is not a row (assembly)
-        const T den = coef[CP * n + q] - lo * (have_prev ? cprev : T(0));
-        const T cc = up / den;
-        const T dd = (rhs - lo * (have_prev ? dprev : T(0))) / den;
-        cpw[q] = cc; dpw[q] = dd;
-        cprev = cc; dprev = dd;
-        have_prev = true;
+        sc[SM_LO * sstride + cq] = (T)lo;
+        sc[SM_M * sstride + cq] = (T)m;
+        sc[SM_CP * sstride + cq] = (T)cpn;
+        cpp = cpn;
     }
-    T next = 0;
-    bool next_valid = false;
-    for (int j = g.my - 2; j >= 1; j--) {
-        const long long q = ((long long)k * g.my + j) * g.mx + i;
-        if (!row[q]) { xout[q] = 0; next_valid = false; continue; }
-        T xl = dpw[q] - (next_valid ? cpw[q] * next : T(0));
-        next = xl;
-        next_valid = true;
-        xout[q] = xin[q] + omega * (xl - xin[q]);
+}
+
+// line right-hand sides of colour c: rz = b - (off-line part of A) x
+template <typename T>
+__global__ void __launch_bounds__(PTPB) k_zebra_rhs(Grid g, ZGrid z, int c, const T *__restrict__ sc,
+                                                    const T *__restrict__ x, const T *__restrict__ b, T *__restrict__ rz)
+{
+    long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (t >= z.cplane) return;
+    const int h = t % z.nh;
+    const long long r = t / z.nh;
+    const int jj = r % z.njI, kk = r / z.njI;
+    const int k = kk + 1, j = jj + 1, i = zebra_i0(k, c) + 2 * h;
+    if (i > g.mx - 2) return;
+    const long long q = ((long long)k * g.my + j) * g.mx + i;
+    const T *s0 = sc + c * z.cplane + t;
+    const long long sstride = 2 * z.cplane;
+    T acc = b[q];
+#pragma unroll
+    for (int u = 0; u < NOFF; u++) acc -= s0[u * sstride] * x[nbr_idx(g, k, j, i, offline_slot(u))];
+    rz[t] = acc;
+}
+
+// Thomas recurrences of colour c, one thread per line; rz holds the line
+// right-hand sides on entry and the forward-eliminated values afterwards.
+// Loads are issued U points ahead of the dependent chain.
+template <typename T, int U>
+__global__ void __launch_bounds__(PTPB) k_zebra_solve(Grid g, ZGrid z, int c, const T *__restrict__ sc,
+                                                      T *__restrict__ rz, T *__restrict__ x)
+{
+    long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (t >= (long long)z.nkI * z.nh) return;
+    const int h = t % z.nh, kk = t / z.nh, k = kk + 1;
+    const int i = zebra_i0(k, c) + 2 * h;
+    if (i > g.mx - 2) return;
+    const long long sstride = 2 * z.cplane;
+    const long long base = (long long)kk * z.njI * z.nh + h;
+    const T *lo = sc + SM_LO * sstride + c * z.cplane + base;
+    const T *mm = sc + SM_M * sstride + c * z.cplane + base;
+    const T *cp = sc + SM_CP * sstride + c * z.cplane + base;
+    T *rr = rz + base;
+    const int nh = z.nh, nj = z.njI;
+    T d = 0;
+    for (int j0 = 0; j0 < nj; j0 += U) {
+        T a[U], m[U], rv[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int jj = j0 + u;
+            if (jj < nj) { a[u] = lo[(long long)jj * nh]; m[u] = mm[(long long)jj * nh]; rv[u] = rr[(long long)jj * nh]; }
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int jj = j0 + u;
+            if (jj < nj) {
+                d = (rv[u] - a[u] * d) * m[u];
+                rr[(long long)jj * nh] = d;
+            }
+        }
     }
+    const long long xrow = (long long)g.mx, xk = ((long long)k * g.my + 1) * g.mx + i;
+    T xl = 0;
+    for (int j0 = nj - 1; j0 >= 0; j0 -= U) {
+        T cc[U], dd[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int jj = j0 - u;
+            if (jj >= 0) { cc[u] = cp[(long long)jj * nh]; dd[u] = rr[(long long)jj * nh]; }
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int jj = j0 - u;
+            if (jj >= 0) {
+                xl = dd[u] - cc[u] * xl;
+                x[xk + jj * xrow] = xl;
+            }
+        }
+    }
+}
+
+// coarsest level: x = B^{-1} b over the rows, B^{-1} stored column-major
+template <typename T>
+__global__ void k_dense_solve(int nr, const int *__restrict__ ridx, const T *__restrict__ inv, const T *__restrict__ b,
+                              T *__restrict__ x)
+{
+    int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nr) return;
+    T s = 0;
+    for (int c = 0; c < nr; c++) s += inv[(long long)c * nr + r] * b[ridx[c]];
+    x[ridx[r]] = s;
 }
 
 // coarse b = sum of the eight child residuals
@@ -449,21 +554,52 @@ __global__ void k_scale_dev(long long n, const double *__restrict__ a, int inv, 
 
 constexpr int MAXV = 33;
 
-// dots[v] = <w, V_v> for v < nv and dots[nv] = <w, w>
+// dots[v] = <w, V_v> for v < nv and dots[nv] = <w, w>; accumulators stay in
+// registers (the loops unroll over the maximum basis size)
 __global__ void __launch_bounds__(RED_THREADS) k_multidot(long long n, int nv, const double *__restrict__ V, long long ld,
                                                           const double *__restrict__ w, double *__restrict__ part)
 {
-    double acc[MAXV];
-    for (int v = 0; v <= nv; v++) acc[v] = 0;
+    double acc[MAXV - 1];
+#pragma unroll
+    for (int v = 0; v < MAXV - 1; v++) acc[v] = 0;
+    double ww = 0;
     for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < n; t += (long long)gridDim.x * blockDim.x) {
-        double wt = w[t];
-        for (int v = 0; v < nv; v++) acc[v] += V[v * ld + t] * wt;
-        acc[nv] += wt * wt;
+        const double wt = w[t];
+#pragma unroll
+        for (int v = 0; v < MAXV - 1; v++)
+            if (v < nv) acc[v] += V[v * ld + t] * wt;
+        ww += wt * wt;
     }
-    for (int v = 0; v <= nv; v++) {
-        double s = block_reduce<OpSum>(acc[v]);
-        if (threadIdx.x == 0) part[v * RED_BLOCKS + blockIdx.x] = s;
+#pragma unroll
+    for (int v = 0; v < MAXV - 1; v++) {
+        if (v < nv) {
+            const double s = block_reduce<OpSum>(acc[v]);
+            if (threadIdx.x == 0) part[v * RED_BLOCKS + blockIdx.x] = s;
+        }
     }
+    const double s = block_reduce<OpSum>(ww);
+    if (threadIdx.x == 0) part[nv * RED_BLOCKS + blockIdx.x] = s;
+}
+
+// w = w - sum_v h[v] V_v, with per-block partial sums of the new <w, w>
+__global__ void __launch_bounds__(RED_THREADS) k_multiaxpy_norm(long long n, int nv, const double *__restrict__ h,
+                                                                const double *__restrict__ V, long long ld,
+                                                                double *__restrict__ w, double *__restrict__ part)
+{
+    double hv[MAXV - 1];
+#pragma unroll
+    for (int v = 0; v < MAXV - 1; v++) hv[v] = v < nv ? h[v] : 0.0;
+    double ss = 0;
+    for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < n; t += (long long)gridDim.x * blockDim.x) {
+        double s = w[t];
+#pragma unroll
+        for (int v = 0; v < MAXV - 1; v++)
+            if (v < nv) s -= hv[v] * V[v * ld + t];
+        w[t] = s;
+        ss += s * s;
+    }
+    ss = block_reduce<OpSum>(ss);
+    if (threadIdx.x == 0) part[blockIdx.x] = ss;
 }
 
 __global__ void k_finish_many(int nv, const double *__restrict__ part, double *__restrict__ out)
@@ -475,17 +611,6 @@ __global__ void k_finish_many(int nv, const double *__restrict__ part, double *_
     for (int q = threadIdx.x; q < RED_BLOCKS; q += blockDim.x) s += part[v * RED_BLOCKS + q];
     s = block_reduce<OpSum>(s);
     if (threadIdx.x == 0) out[v] = s;
-}
-
-// w = w - sum_v h[v] V_v
-__global__ void k_multiaxpy(long long n, int nv, const double *__restrict__ h, const double *__restrict__ V, long long ld,
-                            double *__restrict__ w)
-{
-    long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-    if (t >= n) return;
-    double s = w[t];
-    for (int v = 0; v < nv; v++) s -= h[v] * V[v * ld + t];
-    w[t] = s;
 }
 
 // x = x + sum_v y[v] Z_v
@@ -528,15 +653,24 @@ __global__ void k_sub_mean(long long n, const unsigned char *__restrict__ row, c
 
 struct PLevel {
     vfs::Grid g;
+    vfs::ZGrid z;
     double *coor = nullptr;
     double *met[17] = {nullptr};
     double *nvert = nullptr;
     unsigned char *row = nullptr;
     double *c64 = nullptr;
     float *c32 = nullptr;
+    // colour-split smoother coefficients + Thomas factors, one-colour line rhs
+    float *sc32 = nullptr, *rz32 = nullptr;
+    double *sc64 = nullptr, *rz64 = nullptr;
     // work vectors
-    float *x32 = nullptr, *y32 = nullptr, *b32 = nullptr, *r32 = nullptr, *cp32 = nullptr, *dp32 = nullptr;
-    double *x64 = nullptr, *y64 = nullptr, *b64 = nullptr, *r64 = nullptr, *cp64 = nullptr, *dp64 = nullptr;
+    float *x32 = nullptr, *b32 = nullptr, *r32 = nullptr;
+    double *x64 = nullptr, *b64 = nullptr, *r64 = nullptr;
+    // dense inverse on the coarsest level (nr rows), column-major
+    int nr = 0;
+    int *ridx = nullptr;
+    float *inv32 = nullptr;
+    double *inv64 = nullptr;
 };
 
 struct vfs_poisson {
@@ -549,9 +683,12 @@ struct vfs_poisson {
 
 namespace vfs {
 
-// Every buffer starts zeroed: the smoother never writes ghost cells of its
-// output vector, whose zero values are then read (times a zero coefficient)
-// by the next sweep; garbage there could hold NaN.
+// largest coarsest level solved by a dense inverse
+constexpr int DENSE_MAX = 1024;
+
+// Every buffer starts zeroed: ghost cells and blanked cells of the work
+// vectors are read (times a zero coefficient) but never written, and garbage
+// there could hold NaN.
 static int palloc(vfs_poisson *ps, void **p, size_t bytes)
 {
     hipError_t e = hipMalloc(p, bytes);
@@ -574,22 +711,25 @@ void poisson_free(vfs_ctx *c)
 static int level_alloc_work(vfs_poisson *ps, PLevel &L)
 {
     size_t n = (size_t)L.g.n;
+    L.z.nkI = L.g.mz - 2;
+    L.z.njI = L.g.my - 2;
+    L.z.nh = (L.g.mx - 2 + 1) / 2;
+    L.z.cplane = (long long)L.z.nkI * L.z.njI * L.z.nh;
+    const size_t cp = (size_t)L.z.cplane;
     int rc = 0;
     rc = rc ? rc : palloc(ps, (void **)&L.x32, n * 4);
-    rc = rc ? rc : palloc(ps, (void **)&L.y32, n * 4);
     rc = rc ? rc : palloc(ps, (void **)&L.b32, n * 4);
     rc = rc ? rc : palloc(ps, (void **)&L.r32, n * 4);
-    rc = rc ? rc : palloc(ps, (void **)&L.cp32, n * 4);
-    rc = rc ? rc : palloc(ps, (void **)&L.dp32, n * 4);
     rc = rc ? rc : palloc(ps, (void **)&L.x64, n * 8);
-    rc = rc ? rc : palloc(ps, (void **)&L.y64, n * 8);
     rc = rc ? rc : palloc(ps, (void **)&L.b64, n * 8);
     rc = rc ? rc : palloc(ps, (void **)&L.r64, n * 8);
-    rc = rc ? rc : palloc(ps, (void **)&L.cp64, n * 8);
-    rc = rc ? rc : palloc(ps, (void **)&L.dp64, n * 8);
     rc = rc ? rc : palloc(ps, (void **)&L.row, n);
     rc = rc ? rc : palloc(ps, (void **)&L.c32, NST * n * 4);
     rc = rc ? rc : palloc(ps, (void **)&L.c64, NST * n * 8);
+    rc = rc ? rc : palloc(ps, (void **)&L.sc32, NSM * 2 * cp * 4);
+    rc = rc ? rc : palloc(ps, (void **)&L.sc64, NSM * 2 * cp * 8);
+    rc = rc ? rc : palloc(ps, (void **)&L.rz32, cp * 4);
+    rc = rc ? rc : palloc(ps, (void **)&L.rz64, cp * 8);
     return rc;
 }
 
@@ -600,7 +740,87 @@ static int level_assemble(vfs_ctx *c, PLevel &L, double *const met[17], const do
     const double th = c->p.poisson_threshold;
     hipLaunchKernelGGL(k_row_mask, dim3(blocks_for(L.g.n, PTPB)), dim3(PTPB), 0, st, L.g, nvert, th, L.row);
     hipLaunchKernelGGL(k_assemble, dim3(blocks_for(L.g.n, PTPB)), dim3(PTPB), 0, st, L.g, F, nvert, th, L.row, L.c64, L.c32);
+    const long long lines = 2LL * L.z.nkI * L.z.nh;
+    hipLaunchKernelGGL(k_zebra_setup<float>, dim3(blocks_for(lines, PTPB)), dim3(PTPB), 0, st, L.g, L.z, L.c64, L.row, L.sc32);
+    hipLaunchKernelGGL(k_zebra_setup<double>, dim3(blocks_for(lines, PTPB)), dim3(PTPB), 0, st, L.g, L.z, L.c64, L.row, L.sc64);
     VFS_LAUNCH_CHECK("assemble");
+    return VFS_OK;
+}
+
+// Dense inverse of B = A + s 1 1^T over the rows of the coarsest level.  A has
+// the constant vector as its null space (rows are sums of face fluxes), so B
+// is regular and B^{-1} b solves A x = b for every consistent b.
+static int level_dense_inverse(vfs_ctx *c, vfs_poisson *ps, PLevel &L)
+{
+    const Grid &g = L.g;
+    const size_t n = (size_t)g.n;
+    std::vector<unsigned char> row(n);
+    VFS_HIP(hipMemcpyAsync(row.data(), L.row, n, hipMemcpyDeviceToHost, c->stream));
+    VFS_HIP(hipStreamSynchronize(c->stream));
+    std::vector<int> map(n, -1), ridx;
+    for (size_t q = 0; q < n; q++)
+        if (row[q]) { map[q] = (int)ridx.size(); ridx.push_back((int)q); }
+    const int nr = (int)ridx.size();
+    if (nr == 0 || nr > DENSE_MAX) return VFS_OK;  // keep smoothing on this level
+    std::vector<double> coef(NST * n);
+    VFS_HIP(hipMemcpyAsync(coef.data(), L.c64, NST * n * 8, hipMemcpyDeviceToHost, c->stream));
+    VFS_HIP(hipStreamSynchronize(c->stream));
+    std::vector<double> A((size_t)nr * nr, 0.0), I((size_t)nr * nr, 0.0);
+    double dmax = 0;
+    for (int r = 0; r < nr; r++) {
+        const long long q = ridx[r];
+        const int i = q % g.mx, j = (q / g.mx) % g.my, k = q / ((long long)g.mx * g.my);
+        for (int s = 0; s < NST; s++) {
+            const double v = coef[(size_t)s * n + q];
+            if (v == 0) continue;
+            const int col = map[nbr_idx(g, k, j, i, s)];
+            if (col >= 0) A[(size_t)r * nr + col] += v;
+        }
+        dmax = std::fmax(dmax, std::fabs(A[(size_t)r * nr + r]));
+        I[(size_t)r * nr + r] = 1.0;
+    }
+    const double shift = dmax / nr;
+    for (auto &v : A) v += shift;
+    // Gauss-Jordan with partial pivoting: A -> identity, I -> B^{-1}
+    for (int col = 0; col < nr; col++) {
+        int piv = col;
+        for (int r = col + 1; r < nr; r++)
+            if (std::fabs(A[(size_t)r * nr + col]) > std::fabs(A[(size_t)piv * nr + col])) piv = r;
+        if (A[(size_t)piv * nr + col] == 0) return set_error(VFS_EINVAL, "singular coarsest Poisson operator");
+        if (piv != col)
+            for (int q = 0; q < nr; q++) {
+                std::swap(A[(size_t)piv * nr + q], A[(size_t)col * nr + q]);
+                std::swap(I[(size_t)piv * nr + q], I[(size_t)col * nr + q]);
+            }
+        const double inv = 1.0 / A[(size_t)col * nr + col];
+        for (int q = 0; q < nr; q++) { A[(size_t)col * nr + q] *= inv; I[(size_t)col * nr + q] *= inv; }
+        for (int r = 0; r < nr; r++) {
+            if (r == col) continue;
+            const double f = A[(size_t)r * nr + col];
+            if (f == 0) continue;
+            for (int q = 0; q < nr; q++) {
+                A[(size_t)r * nr + q] -= f * A[(size_t)col * nr + q];
+                I[(size_t)r * nr + q] -= f * I[(size_t)col * nr + q];
+            }
+        }
+    }
+    // column-major copy: inv[c * nr + r] = B^{-1}[r][c]
+    std::vector<double> t64((size_t)nr * nr);
+    std::vector<float> t32((size_t)nr * nr);
+    for (int r = 0; r < nr; r++)
+        for (int q = 0; q < nr; q++) {
+            t64[(size_t)q * nr + r] = I[(size_t)r * nr + q];
+            t32[(size_t)q * nr + r] = (float)I[(size_t)r * nr + q];
+        }
+    int rc = palloc(ps, (void **)&L.ridx, (size_t)nr * sizeof(int));
+    if (!rc) rc = palloc(ps, (void **)&L.inv64, (size_t)nr * nr * 8);
+    if (!rc) rc = palloc(ps, (void **)&L.inv32, (size_t)nr * nr * 4);
+    if (rc) return rc;
+    VFS_HIP(hipMemcpyAsync(L.ridx, ridx.data(), (size_t)nr * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    VFS_HIP(hipMemcpyAsync(L.inv64, t64.data(), (size_t)nr * nr * 8, hipMemcpyHostToDevice, c->stream));
+    VFS_HIP(hipMemcpyAsync(L.inv32, t32.data(), (size_t)nr * nr * 4, hipMemcpyHostToDevice, c->stream));
+    VFS_HIP(hipStreamSynchronize(c->stream));
+    L.nr = nr;
     return VFS_OK;
 }
 
@@ -616,7 +836,7 @@ static int poisson_setup(vfs_ctx *c, int levels)
     int nl = 1;
     {
         int cx = c->g.mx - 2, cy = c->g.my - 2, cz = c->g.mz - 2;
-        while (nl < levels && cx % 2 == 0 && cy % 2 == 0 && cz % 2 == 0 && cx >= 4 && cy >= 4 && cz >= 4) {
+        while (nl < levels && nl < 8 && cx % 2 == 0 && cy % 2 == 0 && cz % 2 == 0 && cx >= 4 && cy >= 4 && cz >= 4) {
             cx /= 2; cy /= 2; cz /= 2;
             nl++;
         }
@@ -655,7 +875,8 @@ static int poisson_setup(vfs_ctx *c, int levels)
         if (!rc) rc = level_assemble(c, Lc, Lc.met, Lc.nvert);
         if (rc) return rc;
     }
-    return VFS_OK;
+    if (nl > 1) rc = level_dense_inverse(c, ps, ps->L[nl - 1]);
+    return rc;
 }
 
 static int ensure_krylov(vfs_ctx *c, int restart)
@@ -677,41 +898,43 @@ struct LV;
 template <>
 struct LV<float> {
     static float *x(PLevel &L) { return L.x32; }
-    static float *y(PLevel &L) { return L.y32; }
     static float *b(PLevel &L) { return L.b32; }
     static float *r(PLevel &L) { return L.r32; }
-    static float *cp(PLevel &L) { return L.cp32; }
-    static float *dp(PLevel &L) { return L.dp32; }
     static const float *coef(PLevel &L) { return L.c32; }
+    static const float *sc(PLevel &L) { return L.sc32; }
+    static float *rz(PLevel &L) { return L.rz32; }
+    static const float *inv(PLevel &L) { return L.inv32; }
 };
 template <>
 struct LV<double> {
     static double *x(PLevel &L) { return L.x64; }
-    static double *y(PLevel &L) { return L.y64; }
     static double *b(PLevel &L) { return L.b64; }
     static double *r(PLevel &L) { return L.r64; }
-    static double *cp(PLevel &L) { return L.cp64; }
-    static double *dp(PLevel &L) { return L.dp64; }
     static const double *coef(PLevel &L) { return L.c64; }
+    static const double *sc(PLevel &L) { return L.sc64; }
+    static double *rz(PLevel &L) { return L.rz64; }
+    static const double *inv(PLevel &L) { return L.inv64; }
 };
 
-constexpr double LINE_OMEGA = 0.8;
+constexpr int LINE_PREFETCH = 16;
 
 template <typename T>
 static void smooth(vfs_ctx *c, PLevel &L, int sweeps)
 {
     hipStream_t st = c->stream;
-    const long long nl = (long long)(L.g.mx - 2) * (L.g.mz - 2);
+    const long long lines = (long long)L.z.nkI * L.z.nh;
     const bool fine = L.g.n == c->g.n;
-    for (int s = 0; s < sweeps; s++) {
-        if (fine) prof_begin(c, sizeof(T) == 4 ? K_LINE32 : K_LINE64);
-        hipLaunchKernelGGL(k_line_jacobi<T>, dim3(blocks_for(nl, PTPB)), dim3(PTPB), 0, st, L.g, LV<T>::coef(L), L.row,
-                           LV<T>::x(L), LV<T>::b(L), LV<T>::y(L), LV<T>::cp(L), LV<T>::dp(L), T(LINE_OMEGA));
-        if (fine) prof_end(c);
-        // the sweep wrote y; swap the roles of x and y
-        if constexpr (sizeof(T) == 4) std::swap(L.x32, L.y32);
-        else std::swap(L.x64, L.y64);
-    }
+    for (int s = 0; s < sweeps; s++)
+        for (int col = 0; col < 2; col++) {
+            if (fine) prof_begin(c, sizeof(T) == 4 ? K_LINE32 : K_LINE64);
+            hipLaunchKernelGGL(k_zebra_rhs<T>, dim3(blocks_for(L.z.cplane, PTPB)), dim3(PTPB), 0, st, L.g, L.z, col,
+                               LV<T>::sc(L), LV<T>::x(L), LV<T>::b(L), LV<T>::rz(L));
+            if (fine) prof_end(c);
+            if (fine) prof_begin(c, sizeof(T) == 4 ? K_LSOLVE32 : K_LSOLVE64);
+            hipLaunchKernelGGL((k_zebra_solve<T, LINE_PREFETCH>), dim3(blocks_for(lines, PTPB)), dim3(PTPB), 0, st, L.g, L.z,
+                               col, LV<T>::sc(L), LV<T>::rz(L), LV<T>::x(L));
+            if (fine) prof_end(c);
+        }
 }
 
 template <typename T>
@@ -721,7 +944,11 @@ static void vcycle(vfs_ctx *c, vfs_poisson *ps, int l, int pre, int post, int co
     PLevel &L = ps->L[l];
     const unsigned nb = blocks_for(L.g.n, PTPB);
     if (l == ps->nl - 1) {
-        smooth<T>(c, L, coarse);
+        if (L.nr > 0)
+            hipLaunchKernelGGL(k_dense_solve<T>, dim3(blocks_for(L.nr, PTPB)), dim3(PTPB), 0, st, L.nr, L.ridx, LV<T>::inv(L),
+                               LV<T>::b(L), LV<T>::x(L));
+        else
+            smooth<T>(c, L, coarse);
         return;
     }
     smooth<T>(c, L, pre);
@@ -768,6 +995,20 @@ static int multidot(vfs_ctx *c, int nv, const double *V, const double *w, double
     VFS_HIP(hipMemcpyAsync(c->red_host, c->red + MAXV * RED_BLOCKS, (nv + 1) * sizeof(double), hipMemcpyDeviceToHost, st));
     VFS_HIP(hipStreamSynchronize(st));
     std::memcpy(out, c->red_host, (nv + 1) * sizeof(double));
+    return VFS_OK;
+}
+
+// w -= sum_v h_v V_v with h already in dev_scalars; returns the new <w, w>
+static int axpy_norm(vfs_ctx *c, int nv, const double *V, double *w, double *ww)
+{
+    hipStream_t st = c->stream;
+    const long long n = c->g.n;
+    hipLaunchKernelGGL(k_multiaxpy_norm, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, st, n, nv, c->ps->dev_scalars, V, n, w, c->red);
+    hipLaunchKernelGGL(finish_reduce<OpSum>, dim3(1), dim3(RED_THREADS), 0, st, c->red, RED_BLOCKS, c->red + MAXV * RED_BLOCKS);
+    VFS_LAUNCH_CHECK("axpy norm");
+    VFS_HIP(hipMemcpyAsync(c->red_host, c->red + MAXV * RED_BLOCKS, sizeof(double), hipMemcpyDeviceToHost, st));
+    VFS_HIP(hipStreamSynchronize(st));
+    *ww = c->red_host[0];
     return VFS_OK;
 }
 
@@ -856,19 +1097,17 @@ int poisson_solve(vfs_ctx *c, const vfs_solve_opts *o, vfs_solve_stats *stats)
             double ww0 = d[jj + 1];
             std::memcpy(c->red_host, h, (jj + 1) * sizeof(double));
             VFS_HIP(hipMemcpyAsync(ps->dev_scalars, c->red_host, (jj + 1) * sizeof(double), hipMemcpyHostToDevice, st));
-            hipLaunchKernelGGL(k_multiaxpy, dim3(nb), dim3(PTPB), 0, st, n, jj + 1, ps->dev_scalars, V, n, w);
-            rc = multidot(c, jj + 1, V, w, d);
+            double ww;
+            rc = axpy_norm(c, jj + 1, V, w, &ww);
             if (rc) return rc;
-            double ww = d[jj + 1];
             if (ww < 0.25 * ww0) {
+                rc = multidot(c, jj + 1, V, w, d);
+                if (rc) return rc;
                 for (int v = 0; v <= jj; v++) h[v] += d[v];
                 std::memcpy(c->red_host, d, (jj + 1) * sizeof(double));
                 VFS_HIP(hipMemcpyAsync(ps->dev_scalars, c->red_host, (jj + 1) * sizeof(double), hipMemcpyHostToDevice, st));
-                hipLaunchKernelGGL(k_multiaxpy, dim3(nb), dim3(PTPB), 0, st, n, jj + 1, ps->dev_scalars, V, n, w);
-                double dd2[MAXV + 1];
-                rc = multidot(c, 0, V, w, dd2);
+                rc = axpy_norm(c, jj + 1, V, w, &ww);
                 if (rc) return rc;
-                ww = dd2[0];
             }
             double hn = std::sqrt(ww > 0 ? ww : 0);
             for (int v = 0; v <= jj; v++) H[(size_t)v * m + jj] = h[v];

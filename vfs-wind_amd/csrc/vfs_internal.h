@@ -88,7 +88,7 @@ struct vfs_poisson;
 
 namespace vfs {
 // kernels whose launches can be timed with HIP events on the context stream
-enum KernelId { K_FACE = 0, K_CELL, K_LINE32, K_LINE64, K_APPLY64, K_C2C, K_PGRAD, K_EDDY, K_NUMK };
+enum KernelId { K_FACE = 0, K_CELL, K_LINE32, K_LINE64, K_APPLY64, K_C2C, K_PGRAD, K_EDDY, K_LSOLVE32, K_LSOLVE64, K_NUMK };
 }
 
 struct vfs_ctx {
