@@ -45,7 +45,10 @@ ALGO = {
 
 # HBM bytes per launch of the finest-level launch, from rocprofv3 FETCH_SIZE (x2, calibrated on
 # copy kernels of known size) + WRITE_SIZE on this workload at 256x192x256
-MEASURED_TRAFFIC = {}
+MEASURED_TRAFFIC = {
+    "face_fluxes": 7.064e9,
+    "line_rhs_fp32": 0.586e9,
+}
 
 
 def log(msg):

@@ -154,14 +154,77 @@ typedef struct {
  * RungeKutta, Poisson, UpdatePressure, Projection, Divergence. */
 int vfs_fractional_step(vfs_ctx *ctx, const vfs_solve_opts *opts, vfs_step_stats *stats);
 
+/* ---- actuator line (rotor_model.c) ---------------------------------------
+ * One turbine with line elements (rotor_model 3 or 6).  The caller passes
+ * what ACL_read_ucd / airfoil_ACL and the Turbine.inp reader (main.c:2040-2101)
+ * produce: blade nodes in grid units with the hub offset applied, the rotor
+ * axis and hub centre, and per foil type the radial twist/chord table and the
+ * CL/CD polars (angles in degrees, lengths divided by reflength_wt). */
+typedef struct {
+    double nx, ny, nz;      /* rotor axis; normalised here (main.c:2051-2054)  */
+    double xc, yc, zc;      /* hub centre                                       */
+    double angvel;          /* angular velocity (-fixturbineangvel: angvel_fixed) */
+    double pitch;           /* pitch[0], degrees                                */
+    double r_nacelle;       /* r_nacelle / reflength_wt                         */
+    double refangle;        /* refangle_AL in degrees (rotor_model 6)           */
+    int rotor_model;        /* 3 or 6                                           */
+    int deltafunc;          /* -deltafunc: 0 dfunc_2h, 7 dfunc_exp, else s4h    */
+    double halfwidth;       /* -halfwidth_dfunc (deltafunc 7)                   */
+} vfs_turbine;
+
+typedef struct {
+    int n_ac; const double *r_ac, *twist, *chord;   /* FOILnn                      */
+    int n_cl; const double *ang_cl, *cl;            /* CLnn                        */
+    int n_cd; const double *ang_cd, *cd;            /* CDnn                        */
+} vfs_foil;
+
+typedef enum {
+    VFS_ALM_NODES = 0,  /* 3 per node: blade node positions                    */
+    VFS_ALM_NODE_VEL,   /* 3 per node: node velocities (rotor_Rot)             */
+    VFS_ALM_CENT,       /* 3 per element: element centres                      */
+    VFS_ALM_REF_CENT,   /* 3 per element: reference-line centres (model 6)     */
+    VFS_ALM_DA,         /* element lengths                                      */
+    VFS_ALM_TWIST, VFS_ALM_CHORD,
+    VFS_ALM_U_LAGR,     /* 3 per element (Calc_U_lagr)                          */
+    VFS_ALM_F_LAGR,     /* 3 per element (Calc_F_lagr_ACL)                      */
+    VFS_ALM_AOA, VFS_ALM_CL, VFS_ALM_CD,
+    VFS_ALM_BOX,        /* 6 per element: i_min, i_max, j_min, j_max, k_min, k_max */
+    VFS_ALM_REF_BOX     /* the same for the reference line                      */
+} vfs_alm_array;
+
+typedef struct vfs_alm vfs_alm;
+
+/* ACL_read_ucd + airfoil_ACL state (rotor_model.c:659-1318, 2017-2137). */
+int vfs_alm_create(vfs_ctx *ctx, const vfs_turbine *t, int num_blade, int nv_blade, const double *nodes, int nfoil,
+                   const vfs_foil *foils, vfs_alm **out);
+int vfs_alm_destroy(vfs_alm *alm);
+int vfs_alm_counts(const vfs_alm *alm, int *n_elements, int *n_nodes);
+/* rotor_Rot, dimension 1 (rotor_model.c:4741-4867), and refAL_Rot for model 6 (rotor_model.c:5034-5088). */
+int vfs_alm_rotate(vfs_alm *alm, double dt);
+/* Pre_process (rotor_model.c:1849-2011) of the blade (and reference) elements. */
+int vfs_alm_preprocess(vfs_alm *alm);
+/* Calc_U_lagr (rotor_model.c:2937-3146) from VFS_UCAT. */
+int vfs_alm_u_lagr(vfs_alm *alm);
+/* Calc_F_lagr_ACL (rotor_model.c:2370-2681) without the optional tip/3D corrections. */
+int vfs_alm_f_lagr(vfs_alm *alm);
+/* VecSet(F_eul, 0) and Calc_F_eul (rotor_model.c:3668-3960) into VFS_FEUL. */
+int vfs_alm_f_eul(vfs_alm *alm);
+/* The rotor_model 6 block of Flow_Solver (solvers.c:535-563) with a fixed angular velocity. */
+int vfs_alm_step(vfs_alm *alm, double dt);
+/* Copy an element or node array to the host. */
+int vfs_alm_get(vfs_alm *alm, vfs_alm_array what, double *host);
+/* Uref_ACL (rotor_model.c:4373-4396): reference disk elements (Urefdata000) and its mean axial velocity. */
+int vfs_alm_set_disk(vfs_alm *alm, int n, const double *cent, const double *dA);
+int vfs_alm_uref(vfs_alm *alm, double *uref);
+
 /* ---- measurement (no counterpart in the reference) -------------------- */
 /* Enable (1) or disable (0) HIP-event timing of the hot kernels' launches on
  * the context stream and clear earlier records.  For benchmarks only. */
 int vfs_profile(vfs_ctx *ctx, int enable);
-/* Total time and launch count recorded for kernel id 0..9 (finest level for
+/* Total time and launch count recorded for kernel id 0..10 (finest level for
  * the smoother): face_fluxes, cell_rhs, line_rhs_fp32, line_rhs_fp64,
  * poisson_apply_fp64, contra2cart, pressure_gradient, eddy_viscosity,
- * line_solve_fp32, line_solve_fp64. */
+ * line_solve_fp32, line_solve_fp64, cell_fp. */
 int vfs_profile_read(vfs_ctx *ctx, int kernel, double *total_ms, long long *count, const char **name);
 
 #ifdef __cplusplus

@@ -41,6 +41,9 @@ static inline size_t IX(const orc_params *p, int k, int j, int i)
 #define VZ(a, k, j, i) ((a)[3 * IX(p, (k), (j), (i)) + 2])
 #define VC(a, c, k, j, i) ((a)[3 * IX(p, (k), (j), (i)) + (c)])
 
+/* Calculate_Covariant_metrics (rhs2.c:595-612): G = inverse of [csi; eta; zet]. */
+void orc_covariant(const double g[3][3], double G[3][3]);
+
 static inline size_t npts(const orc_params *p) { return (size_t)p->mx * p->my * p->mz; }
 
 static inline double *dalloc(size_t n)

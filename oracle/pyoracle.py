@@ -36,6 +36,19 @@ class OrcMetrics(C.Structure):
     _fields_ = [(n, C.POINTER(C.c_double)) for n in METRIC_NAMES] + [("coor", C.POINTER(C.c_double))]
 
 
+class OrcTurbine(C.Structure):
+    _fields_ = [("nx", C.c_double), ("ny", C.c_double), ("nz", C.c_double), ("xc", C.c_double), ("yc", C.c_double),
+                ("zc", C.c_double), ("angvel", C.c_double), ("pitch", C.c_double), ("r_nacelle", C.c_double),
+                ("refangle", C.c_double)]
+
+
+class OrcFoil(C.Structure):
+    _fields_ = [("n_ac", C.c_int), ("r_ac", C.POINTER(C.c_double)), ("twist", C.POINTER(C.c_double)),
+                ("chord", C.POINTER(C.c_double)), ("n_cl", C.c_int), ("ang_cl", C.POINTER(C.c_double)),
+                ("cl", C.POINTER(C.c_double)), ("n_cd", C.c_int), ("ang_cd", C.POINTER(C.c_double)),
+                ("cd", C.POINTER(C.c_double)), ("r_beg", C.c_double), ("r_end", C.c_double)]
+
+
 _lib = None
 
 
@@ -78,6 +91,16 @@ def load() -> C.CDLL:
         "orc_dfunc_s3h": ([C.c_double], C.c_double),
         "orc_dfunc_s4h": ([C.c_double], C.c_double),
         "orc_dfunc_exp": ([C.c_double, C.c_double], C.c_double),
+        "orc_arbitrary_rotate": ([D, C.c_double, D, D], None),
+        "orc_alm_elements": ([C.c_int, C.c_int, D, D, D], None),
+        "orc_alm_rotate": ([C.POINTER(OrcTurbine), C.c_double, C.c_int, D, D], None),
+        "orc_alm_refline": ([C.POINTER(OrcTurbine), C.c_int, D, D], None),
+        "orc_alm_twist_chord": ([C.POINTER(OrcTurbine), C.c_int, D, C.POINTER(OrcFoil), C.c_int, D, D], None),
+        "orc_alm_preprocess": ([P, M, C.c_int, D, C.POINTER(C.c_int)], None),
+        "orc_alm_ulagr": ([P, M, D, C.c_int, D, C.POINTER(C.c_int), D], None),
+        "orc_alm_flagr": ([C.POINTER(OrcTurbine), C.c_int, C.c_int, D, D, D, D, D, C.POINTER(OrcFoil), C.c_int,
+                           D, D, D, D], None),
+        "orc_alm_feul": ([P, M, C.c_int, D, C.POINTER(C.c_int), D, D, C.c_int, C.c_double, D], None),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)
@@ -204,3 +227,111 @@ class Oracle:
                                           self._f("ucat_old"), self._f("nu_t"), self._f("p"), self._f("phi"),
                                           self._f("dp"), tol, C.byref(md), C.byref(rel))
         return it, md.value, rel.value
+
+
+# ---- actuator line ---------------------------------------------------------
+def _iptr(a: np.ndarray):
+    assert a.dtype == np.int32 and a.flags.c_contiguous
+    return a.ctypes.data_as(C.POINTER(C.c_int))
+
+
+def make_turbine(nx, ny, nz, xc, yc, zc, angvel, pitch=0.0, r_nacelle=0.0, refangle=20.0) -> OrcTurbine:
+    rr = float(np.sqrt(nx * nx + ny * ny + nz * nz))
+    return OrcTurbine(nx / rr, ny / rr, nz / rr, xc, yc, zc, angvel, pitch, r_nacelle, refangle)
+
+
+class Foils:
+    """ctypes array of OrcFoil that keeps its numpy tables alive."""
+
+    def __init__(self, foils: list):
+        self.keep = []
+        self.arr = (OrcFoil * max(1, len(foils)))()
+        self.n = len(foils)
+        for q, f in enumerate(foils):
+            a = [np.ascontiguousarray(f[k], dtype=np.float64) for k in ("r", "twist", "chord", "ang_cl", "cl", "ang_cd", "cd")]
+            self.keep += a
+            o = self.arr[q]
+            o.n_ac, o.r_ac, o.twist, o.chord = len(a[0]), _ptr(a[0]), _ptr(a[1]), _ptr(a[2])
+            o.n_cl, o.ang_cl, o.cl = len(a[3]), _ptr(a[3]), _ptr(a[4])
+            o.n_cd, o.ang_cd, o.cd = len(a[5]), _ptr(a[5]), _ptr(a[6])
+            o.r_beg, o.r_end = float(a[0][0]), float(a[0][-1])
+
+
+def alm_elements(nodes: np.ndarray):
+    """nodes (nb, nvb, 3) -> element centres (n, 3) and lengths (n,)."""
+    lib = load()
+    nb, nvb = nodes.shape[:2]
+    x = np.ascontiguousarray(nodes, dtype=np.float64)
+    cent, dA = np.zeros((nb * (nvb - 1), 3)), np.zeros(nb * (nvb - 1))
+    lib.orc_alm_elements(nb, nvb, _ptr(x), _ptr(cent), _ptr(dA))
+    return cent, dA
+
+
+def alm_rotate(t: OrcTurbine, dt: float, nodes: np.ndarray):
+    """rotor_Rot node loop: returns rotated nodes and node velocities."""
+    lib = load()
+    x = np.ascontiguousarray(nodes, dtype=np.float64).copy()
+    u = np.zeros_like(x)
+    lib.orc_alm_rotate(C.byref(t), dt, x.size // 3, _ptr(x), _ptr(u))
+    return x, u
+
+
+def alm_refline(t: OrcTurbine, nodes: np.ndarray):
+    lib = load()
+    x = np.ascontiguousarray(nodes, dtype=np.float64)
+    r = np.zeros_like(x)
+    lib.orc_alm_refline(C.byref(t), x.size // 3, _ptr(x), _ptr(r))
+    return r
+
+
+def alm_twist_chord(t: OrcTurbine, cent: np.ndarray, foils: Foils):
+    lib = load()
+    c = np.ascontiguousarray(cent, dtype=np.float64)
+    tw, ch = np.zeros(len(c)), np.zeros(len(c))
+    lib.orc_alm_twist_chord(C.byref(t), len(c), _ptr(c), foils.arr, foils.n, _ptr(tw), _ptr(ch))
+    return tw, ch
+
+
+def alm_flagr(t: OrcTurbine, nb: int, nvb: int, cent, ulagr, unode, twist, chord, foils: Foils, cl=None, cd=None):
+    lib = load()
+    n = nb * (nvb - 1)
+    cent, ulagr, unode, twist, chord = (np.ascontiguousarray(a, dtype=np.float64) for a in (cent, ulagr, unode, twist, chord))
+    aoa = np.zeros(n)
+    cl = np.zeros(n) if cl is None else np.ascontiguousarray(cl, dtype=np.float64).copy()
+    cd = np.zeros(n) if cd is None else np.ascontiguousarray(cd, dtype=np.float64).copy()
+    f = np.zeros((n, 3))
+    lib.orc_alm_flagr(C.byref(t), nb, nvb, _ptr(cent), _ptr(ulagr), _ptr(unode), _ptr(twist), _ptr(chord), foils.arr,
+                      foils.n, _ptr(aoa), _ptr(cl), _ptr(cd), _ptr(f))
+    return f, aoa, cl, cd
+
+
+def _alm_methods():
+    def alm_preprocess(self, cent):
+        c = np.ascontiguousarray(cent, dtype=np.float64)
+        box = np.zeros((len(c), 6), dtype=np.int32)
+        self.lib.orc_alm_preprocess(C.byref(self.p), C.byref(self.cm), len(c), _ptr(c), _iptr(box))
+        return box
+
+    def alm_ulagr(self, cent, box):
+        c = np.ascontiguousarray(cent, dtype=np.float64)
+        b = np.ascontiguousarray(box, dtype=np.int32)
+        u = np.zeros((len(c), 3))
+        self.lib.orc_alm_ulagr(C.byref(self.p), C.byref(self.cm), self._f("ucat"), len(c), _ptr(c), _iptr(b), _ptr(u))
+        return u
+
+    def alm_feul(self, cent, box, flagr, dA, df=10, halfwidth=4.0):
+        c = np.ascontiguousarray(cent, dtype=np.float64)
+        b = np.ascontiguousarray(box, dtype=np.int32)
+        f = np.ascontiguousarray(flagr, dtype=np.float64)
+        a = np.ascontiguousarray(dA, dtype=np.float64)
+        feul = np.zeros(self.shape + (3,))
+        self.lib.orc_alm_feul(C.byref(self.p), C.byref(self.cm), len(c), _ptr(c), _iptr(b), _ptr(f), _ptr(a), int(df),
+                              float(halfwidth), _ptr(feul))
+        return feul
+
+    Oracle.alm_preprocess = alm_preprocess
+    Oracle.alm_ulagr = alm_ulagr
+    Oracle.alm_feul = alm_feul
+
+
+_alm_methods()

@@ -325,7 +325,7 @@ static void cramer(const double *csi, const double *eta, const double *zet, cons
 }
 
 /* Inverse of the contravariant matrix g = [csi; eta; zet]: G[r][c] = dx_r/dxi_c. */
-static void covariant(const double g[3][3], double G[3][3])
+void orc_covariant(const double g[3][3], double G[3][3])
 {
     /* Calculate_Covariant_metrics, rhs2.c:595-612 */
     const double a11 = g[0][0], a12 = g[0][1], a13 = g[0][2];
@@ -349,7 +349,7 @@ static void slip_fill(const orc_params *p, const orc_metrics *m, const double *u
         g[1][c] = VC(m->eta, c, kn, jn, in);
         g[2][c] = VC(m->zet, c, kn, jn, in);
     }
-    covariant(g, G);
+    orc_covariant(g, G);
     double nx = sgn * G[0][dir], ny = sgn * G[1][dir], nz = sgn * G[2][dir];
     double sum = sqrt(nx * nx + ny * ny + nz * nz);
     nx /= sum, ny /= sum, nz /= sum;

@@ -144,6 +144,38 @@ double orc_dfunc_s3h(double r);
 double orc_dfunc_s4h(double r);
 double orc_dfunc_exp(double r, double n);
 
+/* actuator line (rotor_model.c): one turbine, line elements */
+typedef struct {
+    double nx, ny, nz;       /* unit rotor axis (Turbine.inp, normalised)          */
+    double xc, yc, zc;       /* hub centre                                        */
+    double angvel;           /* angular velocity about the axis                   */
+    double pitch;            /* pitch[0] (degrees)                                */
+    double r_nacelle;        /* r_nacelle / reflength_wt                          */
+    double refangle;         /* refangle_AL (degrees), rotor_model 6               */
+} orc_turbine;
+
+typedef struct {
+    int n_ac; const double *r_ac, *twist, *chord;  /* FOIL, lengths / reflength_wt */
+    int n_cl; const double *ang_cl, *cl;
+    int n_cd; const double *ang_cd, *cd;
+    double r_beg, r_end;
+} orc_foil;
+
+void orc_arbitrary_rotate(const double p[3], double theta, const double r[3], double q[3]);
+void orc_alm_elements(int nb, int nvb, const double *xbp, double *cent, double *dA);
+void orc_alm_rotate(const orc_turbine *t, double dt, int nv, double *xbp, double *unode);
+void orc_alm_refline(const orc_turbine *t, int nv, const double *xbp, double *xref);
+void orc_alm_twist_chord(const orc_turbine *t, int n, const double *cent, const orc_foil *f, int nfoil,
+                         double *twist, double *chord);
+void orc_alm_preprocess(const orc_params *p, const orc_metrics *m, int n, const double *cent, int *box);
+void orc_alm_ulagr(const orc_params *p, const orc_metrics *m, const double *ucat, int n, const double *cent,
+                   const int *box, double *ulagr);
+void orc_alm_flagr(const orc_turbine *t, int nb, int nvb, const double *cent, const double *ulagr, const double *unode,
+                   const double *twist, const double *chord, const orc_foil *f, int nfoil, double *aoa, double *cl,
+                   double *cd, double *flagr);
+void orc_alm_feul(const orc_params *p, const orc_metrics *m, int n, const double *cent, const int *box,
+                  const double *flagr, const double *dA, int df, double halfwidth, double *feul);
+
 #ifdef __cplusplus
 }
 #endif

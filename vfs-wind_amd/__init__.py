@@ -67,6 +67,36 @@ class StepStats(C.Structure):
     ]
 
 
+class Turbine(C.Structure):
+    """vfs_turbine: one actuator-line turbine (Turbine.inp + control options)."""
+
+    _fields_ = [
+        ("nx", C.c_double), ("ny", C.c_double), ("nz", C.c_double),
+        ("xc", C.c_double), ("yc", C.c_double), ("zc", C.c_double),
+        ("angvel", C.c_double), ("pitch", C.c_double), ("r_nacelle", C.c_double), ("refangle", C.c_double),
+        ("rotor_model", C.c_int), ("deltafunc", C.c_int), ("halfwidth", C.c_double),
+    ]
+
+
+class Foil(C.Structure):
+    """vfs_foil: radial twist/chord table and CL/CD polars of one foil type."""
+
+    _fields_ = [
+        ("n_ac", C.c_int), ("r_ac", C.c_void_p), ("twist", C.c_void_p), ("chord", C.c_void_p),
+        ("n_cl", C.c_int), ("ang_cl", C.c_void_p), ("cl", C.c_void_p),
+        ("n_cd", C.c_int), ("ang_cd", C.c_void_p), ("cd", C.c_void_p),
+    ]
+
+
+# vfs_alm_array enumeration and the number of values per element / node
+ALM_ARRAYS = ["NODES", "NODE_VEL", "CENT", "REF_CENT", "DA", "TWIST", "CHORD", "U_LAGR", "F_LAGR",
+              "AOA", "CL", "CD", "BOX", "REF_BOX"]
+_ALM_SHAPE = {"NODES": ("node", 3), "NODE_VEL": ("node", 3), "CENT": ("elmt", 3), "REF_CENT": ("elmt", 3),
+              "DA": ("elmt", 1), "TWIST": ("elmt", 1), "CHORD": ("elmt", 1), "U_LAGR": ("elmt", 3),
+              "F_LAGR": ("elmt", 3), "AOA": ("elmt", 1), "CL": ("elmt", 1), "CD": ("elmt", 1),
+              "BOX": ("elmt", 6), "REF_BOX": ("elmt", 6)}
+
+
 # entry points and their argument types
 _SIGS = {
     "vfs_last_error": ([], C.c_char_p),
@@ -96,6 +126,19 @@ _SIGS = {
     "vfs_projection": ([C.c_void_p], C.c_int),
     "vfs_divergence": ([C.c_void_p, C.POINTER(C.c_double)], C.c_int),
     "vfs_fractional_step": ([C.c_void_p, C.POINTER(SolveOpts), C.POINTER(StepStats)], C.c_int),
+    "vfs_alm_create": ([C.c_void_p, C.POINTER(Turbine), C.c_int, C.c_int, C.c_void_p, C.c_int, C.c_void_p,
+                        C.POINTER(C.c_void_p)], C.c_int),
+    "vfs_alm_destroy": ([C.c_void_p], C.c_int),
+    "vfs_alm_counts": ([C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int)], C.c_int),
+    "vfs_alm_rotate": ([C.c_void_p, C.c_double], C.c_int),
+    "vfs_alm_preprocess": ([C.c_void_p], C.c_int),
+    "vfs_alm_u_lagr": ([C.c_void_p], C.c_int),
+    "vfs_alm_f_lagr": ([C.c_void_p], C.c_int),
+    "vfs_alm_f_eul": ([C.c_void_p], C.c_int),
+    "vfs_alm_step": ([C.c_void_p, C.c_double], C.c_int),
+    "vfs_alm_get": ([C.c_void_p, C.c_int, C.c_void_p], C.c_int),
+    "vfs_alm_set_disk": ([C.c_void_p, C.c_int, C.c_void_p, C.c_void_p], C.c_int),
+    "vfs_alm_uref": ([C.c_void_p, C.POINTER(C.c_double)], C.c_int),
     "vfs_profile": ([C.c_void_p, C.c_int], C.c_int),
     "vfs_profile_read": ([C.c_void_p, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_longlong),
                           C.POINTER(C.c_char_p)], C.c_int),
@@ -103,7 +146,7 @@ _SIGS = {
 
 KERNEL_IDS = {"face_fluxes": 0, "cell_rhs": 1, "line_rhs_fp32": 2, "line_rhs_fp64": 3,
               "poisson_apply_fp64": 4, "contra2cart": 5, "pressure_gradient": 6, "eddy_viscosity": 7,
-              "line_solve_fp32": 8, "line_solve_fp64": 9}
+              "line_solve_fp32": 8, "line_solve_fp64": 9, "cell_fp": 10}
 
 
 class VFSError(RuntimeError):
@@ -291,3 +334,75 @@ class Context:
         st = StepStats()
         self._check(self.lib.vfs_fractional_step(self.h, C.byref(opts or default_solve_opts()), C.byref(st)))
         return st
+
+
+class ALM:
+    """An actuator-line turbine attached to a Context (the reference's IBMNodes
+    + FSInfo + ACL of one turbine on the rotor_model 3/6 path)."""
+
+    def __init__(self, ctx: Context, turbine: Turbine, nodes: np.ndarray, foils: list):
+        self.ctx, self.lib = ctx, ctx.lib
+        nodes = np.ascontiguousarray(nodes, dtype=np.float64)
+        if nodes.ndim != 3 or nodes.shape[2] != 3:
+            raise ValueError("nodes: expected (num_blade, nodes_per_blade, 3)")
+        nb, nvb = nodes.shape[:2]
+        self._keep = []
+        fs = (Foil * max(1, len(foils)))()
+        for q, f in enumerate(foils):
+            arrs = [np.ascontiguousarray(f[k], dtype=np.float64) for k in ("r", "twist", "chord", "ang_cl", "cl", "ang_cd", "cd")]
+            self._keep += arrs
+            fs[q].n_ac, fs[q].r_ac, fs[q].twist, fs[q].chord = len(arrs[0]), arrs[0].ctypes.data, arrs[1].ctypes.data, arrs[2].ctypes.data
+            fs[q].n_cl, fs[q].ang_cl, fs[q].cl = len(arrs[3]), arrs[3].ctypes.data, arrs[4].ctypes.data
+            fs[q].n_cd, fs[q].ang_cd, fs[q].cd = len(arrs[5]), arrs[5].ctypes.data, arrs[6].ctypes.data
+        h = C.c_void_p()
+        ctx._check(self.lib.vfs_alm_create(ctx.h, C.byref(turbine), nb, nvb, nodes.ctypes.data, len(foils), fs, C.byref(h)))
+        self.h = h
+        ne, nv = C.c_int(), C.c_int()
+        ctx._check(self.lib.vfs_alm_counts(h, C.byref(ne), C.byref(nv)))
+        self.n_elmt, self.n_nodes = ne.value, nv.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.vfs_alm_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def rotate(self, dt: float):
+        self.ctx._check(self.lib.vfs_alm_rotate(self.h, float(dt)))
+
+    def preprocess(self):
+        self.ctx._check(self.lib.vfs_alm_preprocess(self.h))
+
+    def u_lagr(self):
+        self.ctx._check(self.lib.vfs_alm_u_lagr(self.h))
+
+    def f_lagr(self):
+        self.ctx._check(self.lib.vfs_alm_f_lagr(self.h))
+
+    def f_eul(self):
+        self.ctx._check(self.lib.vfs_alm_f_eul(self.h))
+
+    def step(self, dt: float):
+        self.ctx._check(self.lib.vfs_alm_step(self.h, float(dt)))
+
+    def get(self, name: str) -> np.ndarray:
+        kind, w = _ALM_SHAPE[name]
+        n = self.n_elmt if kind == "elmt" else self.n_nodes
+        a = np.empty((n, w) if w > 1 else (n,), dtype=np.float64)
+        self.ctx._check(self.lib.vfs_alm_get(self.h, ALM_ARRAYS.index(name), a.ctypes.data))
+        return a
+
+    def set_disk(self, cent: np.ndarray, dA: np.ndarray):
+        cent = np.ascontiguousarray(cent, dtype=np.float64)
+        dA = np.ascontiguousarray(dA, dtype=np.float64)
+        self.ctx._check(self.lib.vfs_alm_set_disk(self.h, len(dA), cent.ctypes.data, dA.ctypes.data))
+
+    def uref(self) -> float:
+        v = C.c_double()
+        self.ctx._check(self.lib.vfs_alm_uref(self.h, C.byref(v)))
+        return v.value

@@ -126,3 +126,54 @@ def params_for(mod, case: Case, dt: float, les: int = 1, second_order: int = 1, 
     mx, my, mz = case.da
     return mod.make_params(mx, my, mz, per=case.per, bctype=case.bctype, les=les, second_order=second_order,
                            laplacian=laplacian, ren=case.ren, dt=dt, st=1.0, cs2=cs2, inlet_flux=case.inlet_flux)
+
+
+def box(nx: int, ny: int, nz: int, h: float = 1.0, bump: float = 0.0, seed: int = SEED, noise: float = 0.05) -> dict:
+    """Cartesian(-curvilinear) box of nx x ny x nz cells of spacing h, flow
+    along +z (k) with U = 1 plus seeded uniform noise of relative size
+    `noise`; `bump` > 0 displaces every node by bump * h * sin sin sin so the
+    metrics are non-orthogonal.  Returns coor and the cell velocity ucat
+    (ghost cells copy their interior neighbour)."""
+    IM, JM, KM = nx + 1, ny + 1, nz + 1
+    mx, my, mz = nx + 2, ny + 2, nz + 2
+    x, y, z = np.arange(IM) * h, np.arange(JM) * h, np.arange(KM) * h
+    Z, Y, X = np.meshgrid(z, y, x, indexing="ij")
+    Lx, Ly, Lz = nx * h, ny * h, nz * h
+    amp = bump * h * np.sin(np.pi * X / Lx) * np.sin(np.pi * Y / Ly) * np.sin(np.pi * Z / Lz)
+    coor = np.zeros((mz, my, mx, 3))
+    coor[:KM, :JM, :IM, 0] = X + amp
+    coor[:KM, :JM, :IM, 1] = Y + 0.5 * amp
+    coor[:KM, :JM, :IM, 2] = Z - 0.7 * amp
+    rng = np.random.Generator(np.random.PCG64(seed))
+    ucat = np.zeros((mz, my, mx, 3))
+    u = noise * rng.uniform(-1.0, 1.0, size=(nz, ny, nx, 3))
+    u[..., 2] += 1.0
+    ucat[1:-1, 1:-1, 1:-1] = u
+    ucat[0], ucat[-1] = ucat[1], ucat[-2]
+    ucat[:, 0], ucat[:, -1] = ucat[:, 1], ucat[:, -2]
+    ucat[:, :, 0], ucat[:, :, -1] = ucat[:, :, 1], ucat[:, :, -2]
+    return {"cells": (nx, ny, nz), "da": (mx, my, mz), "coor": coor, "ucat": ucat, "h": h}
+
+
+def synthetic_turbine(radius: float, hub, *, num_blade: int = 3, nodes_per_blade: int = 41, tsr: float = 7.0,
+                      uref: float = 1.0, chord_frac: float = 0.1, rotor_model: int = 6, deltafunc: int = 7,
+                      halfwidth: float = 2.0, refangle: float = 20.0, r_nacelle: float = 0.0, pitch: float = 0.0):
+    """The benchmark rotor: axis +z through `hub`, blades radial from 0 to R at
+    120 degree spacing, constant chord chord_frac * R, zero twist, CL = 2 pi
+    alpha (alpha in radians) tabulated every degree over [-90, 90] and
+    clipped to +-1.2, CD = 0.01, angular velocity tsr * uref / R."""
+    from . import turbine as tb  # noqa: E402  (package-relative)
+
+    r = np.linspace(0.0, radius, nodes_per_blade)
+    nodes = np.zeros((num_blade, nodes_per_blade, 3))
+    for b in range(num_blade):
+        th = 2 * np.pi * b / num_blade
+        nodes[b, :, 0] = hub[0] + r * np.cos(th)
+        nodes[b, :, 1] = hub[1] + r * np.sin(th)
+        nodes[b, :, 2] = hub[2]
+    ang = np.arange(-90.0, 91.0, 1.0)
+    foil = {"r": np.array([0.0, radius]), "twist": np.zeros(2), "chord": np.full(2, chord_frac * radius),
+            "ang_cl": ang, "cl": np.clip(2 * np.pi * np.radians(ang), -1.2, 1.2),
+            "ang_cd": np.array([-90.0, 90.0]), "cd": np.array([0.01, 0.01])}
+    return tb.TurbineSetup(0.0, 0.0, 1.0, hub[0], hub[1], hub[2], tsr * uref / radius, pitch, r_nacelle, refangle,
+                           rotor_model, deltafunc, halfwidth, nodes, [foil])

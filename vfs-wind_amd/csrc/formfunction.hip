@@ -36,8 +36,46 @@ struct FaceArgs {
     double *out;  // 18 planes of n doubles: div1,div2,div3 (xyz each), visc1..3
 };
 
-__device__ inline double Uc(const FaceArgs &a, int c, int k, int j, int i) { return a.ucat[3 * gidx(a.g, k, j, i) + c]; }
-__device__ inline double NVd(const FaceArgs &a, int k, int j, int i) { return a.nvert[gidx(a.g, k, j, i)]; }
+// ucat, nvert and nu_t of the points around the block's current plane come
+// from an LDS tile (3 k-planes x (TI+3) x (TJ+3) points, i-1..i+2 and
+// j-1..j+2 of the block's points); anything outside it (the fourth-order
+// k+2 point and the periodic far ghosts) is read from HBM.  The tile holds
+// exactly the values the DMDA index with periodic wrap would return.
+constexpr int FT_I = 32, FT_J = 8, FT_K = 16;
+constexpr int FT_HI = FT_I + 3, FT_HJ = FT_J + 3, FT_HP = FT_HI * FT_HJ;
+
+struct TileAcc {
+    const FaceArgs *a;
+    const double *ux, *uy, *uz, *nv, *nt;
+    int i0, j0, kc, slot0;  // slot0 holds plane kc-1
+    __device__ inline int lidx(int k, int j, int i, bool &in) const
+    {
+        const int di = i - i0 + 1, dj = j - j0 + 1, dk = k - kc + 1;
+        in = (unsigned)di < (unsigned)FT_HI && (unsigned)dj < (unsigned)FT_HJ && (unsigned)dk < 3u;
+        int s = slot0 + dk;
+        if (s >= 3) s -= 3;
+        return (s * FT_HJ + dj) * FT_HI + di;
+    }
+    __device__ inline double U(int c, int k, int j, int i) const
+    {
+        bool in;
+        const int l = lidx(k, j, i, in);
+        if (in) return c == 0 ? ux[l] : (c == 1 ? uy[l] : uz[l]);
+        return a->ucat[3 * gidx(a->g, k, j, i) + c];
+    }
+    __device__ inline double NV(int k, int j, int i) const
+    {
+        bool in;
+        const int l = lidx(k, j, i, in);
+        return in ? nv[l] : a->nvert[gidx(a->g, k, j, i)];
+    }
+    __device__ inline double NT(int k, int j, int i) const
+    {
+        bool in;
+        const int l = lidx(k, j, i, in);
+        return in ? nt[l] : a->nut[gidx(a->g, k, j, i)];
+    }
+};
 
 // viscous face flux, momentum.c:682-696 and 863-902 (and the j/k twins)
 __device__ inline void visc_face(const double cs[3], const double et[3], const double ze[3], const double nrm[3],
@@ -63,15 +101,11 @@ __device__ inline void visc_face(const double cs[3], const double et[3], const d
                    r[2][c] * nrm[2]) * ajc * (nu);
 }
 
-__global__ void __launch_bounds__(FF_TPB) k_face_fluxes(FaceArgs a)
+// the 18 face fluxes of point (i,j,k) (momentum.c:668-1451)
+__device__ inline void face_point(const FaceArgs &a, const TileAcc &T, int i, int j, int k)
 {
     const Grid &g = a.g;
-    const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-    if (t >= g.n) return;
-    const int i = t % g.mx;
-    const long long rr = t / g.mx;
-    const int j = rr % g.my, k = rr / g.my;
-    const long long q = t;
+    const long long q = ((long long)k * g.my + j) * g.mx + i;
     const long long n = g.n;
     const int mx = g.mx, my = g.my, mz = g.mz;
     const double SOL = 0.1;  // k-omega.c:19
@@ -86,38 +120,38 @@ __global__ void __launch_bounds__(FF_TPB) k_face_fluxes(FaceArgs a)
             double d[3][3];
             // Compute_du_i, k-omega.c:56-116
             for (int c = 0; c < 3; c++) {
-                d[c][0] = Uc(a, c, k, j, i + 1) - Uc(a, c, k, j, i);
-                if (NVd(a, k, j + 1, i) > SOL || NVd(a, k, j + 1, i + 1) > SOL)
-                    d[c][1] = (Uc(a, c, k, j, i + 1) + Uc(a, c, k, j, i) - Uc(a, c, k, j - 1, i + 1) - Uc(a, c, k, j - 1, i)) * 0.5;
-                else if (NVd(a, k, j - 1, i) > SOL || NVd(a, k, j - 1, i + 1) > SOL)
-                    d[c][1] = (Uc(a, c, k, j + 1, i + 1) + Uc(a, c, k, j + 1, i) - Uc(a, c, k, j, i + 1) - Uc(a, c, k, j, i)) * 0.5;
+                d[c][0] = T.U(c, k, j, i + 1) - T.U(c, k, j, i);
+                if (T.NV(k, j + 1, i) > SOL || T.NV(k, j + 1, i + 1) > SOL)
+                    d[c][1] = (T.U(c, k, j, i + 1) + T.U(c, k, j, i) - T.U(c, k, j - 1, i + 1) - T.U(c, k, j - 1, i)) * 0.5;
+                else if (T.NV(k, j - 1, i) > SOL || T.NV(k, j - 1, i + 1) > SOL)
+                    d[c][1] = (T.U(c, k, j + 1, i + 1) + T.U(c, k, j + 1, i) - T.U(c, k, j, i + 1) - T.U(c, k, j, i)) * 0.5;
                 else
-                    d[c][1] = (Uc(a, c, k, j + 1, i + 1) + Uc(a, c, k, j + 1, i) - Uc(a, c, k, j - 1, i + 1) - Uc(a, c, k, j - 1, i)) * 0.25;
-                if (NVd(a, k + 1, j, i) > SOL || NVd(a, k + 1, j, i + 1) > SOL)
-                    d[c][2] = (Uc(a, c, k, j, i + 1) + Uc(a, c, k, j, i) - Uc(a, c, k - 1, j, i + 1) - Uc(a, c, k - 1, j, i)) * 0.5;
-                else if (NVd(a, k - 1, j, i) > SOL || NVd(a, k - 1, j, i + 1) > SOL)
-                    d[c][2] = (Uc(a, c, k + 1, j, i + 1) + Uc(a, c, k + 1, j, i) - Uc(a, c, k, j, i + 1) - Uc(a, c, k, j, i)) * 0.5;
+                    d[c][1] = (T.U(c, k, j + 1, i + 1) + T.U(c, k, j + 1, i) - T.U(c, k, j - 1, i + 1) - T.U(c, k, j - 1, i)) * 0.25;
+                if (T.NV(k + 1, j, i) > SOL || T.NV(k + 1, j, i + 1) > SOL)
+                    d[c][2] = (T.U(c, k, j, i + 1) + T.U(c, k, j, i) - T.U(c, k - 1, j, i + 1) - T.U(c, k - 1, j, i)) * 0.5;
+                else if (T.NV(k - 1, j, i) > SOL || T.NV(k - 1, j, i + 1) > SOL)
+                    d[c][2] = (T.U(c, k + 1, j, i + 1) + T.U(c, k + 1, j, i) - T.U(c, k, j, i + 1) - T.U(c, k, j, i)) * 0.5;
                 else
-                    d[c][2] = (Uc(a, c, k + 1, j, i + 1) + Uc(a, c, k + 1, j, i) - Uc(a, c, k - 1, j, i + 1) - Uc(a, c, k - 1, j, i)) * 0.25;
+                    d[c][2] = (T.U(c, k + 1, j, i + 1) + T.U(c, k + 1, j, i) - T.U(c, k - 1, j, i + 1) - T.U(c, k - 1, j, i)) * 0.25;
             }
             int iL = i - 1, iR = i + 2;
             if (i == 0 || i == mx - 2) {
                 if (g.pi && i == mx - 2) iR = mx + 2;
                 else if (g.pi && i == 0) iL = -3;
                 else iL = i, iR = i + 1;
-            } else if (NVd(a, k, j, iL) + NVd(a, k, j, iR) > 0.1) iL = i, iR = i + 1;
+            } else if (T.NV(k, j, iL) + T.NV(k, j, iR) > 0.1) iL = i, iR = i + 1;
             if (a.second_order) iL = i, iR = i + 1;
             const double uc = a.ucont[3 * q + 0];
             for (int c = 0; c < 3; c++) {
-                if (a.second_order) dv[c] = -uc * 0.5 * (Uc(a, c, k, j, i) + Uc(a, c, k, j, i + 1));
-                else dv[c] = -uc * 0.0625 * (-Uc(a, c, k, j, iL) + 9. * Uc(a, c, k, j, i) + 9. * Uc(a, c, k, j, i + 1) - Uc(a, c, k, j, iR));
+                if (a.second_order) dv[c] = -uc * 0.5 * (T.U(c, k, j, i) + T.U(c, k, j, i + 1));
+                else dv[c] = -uc * 0.0625 * (-T.U(c, k, j, iL) + 9. * T.U(c, k, j, i) + 9. * T.U(c, k, j, i + 1) - T.U(c, k, j, iR));
             }
-            if (NVd(a, k, j, i) + NVd(a, k, j, i + 1) > 0.1) dv[0] = dv[1] = dv[2] = 0;
+            if (T.NV(k, j, i) + T.NV(k, j, i + 1) > 0.1) dv[0] = dv[1] = dv[2] = 0;
             double nut = 0;
             if (a.les) {
-                if ((i == 0 && !g.pi) || NVd(a, k, j, i) > 0.1) nut = a.nut[gidx(g, k, j, i + 1)];
-                else if ((i == mx - 2 && !g.pi) || NVd(a, k, j, i + 1) > 0.1) nut = a.nut[q];
-                else nut = 0.5 * (a.nut[q] + a.nut[gidx(g, k, j, i + 1)]);
+                if ((i == 0 && !g.pi) || T.NV(k, j, i) > 0.1) nut = T.NT(k, j, i + 1);
+                else if ((i == mx - 2 && !g.pi) || T.NV(k, j, i + 1) > 0.1) nut = T.NT(k, j, i);
+                else nut = 0.5 * (T.NT(k, j, i) + T.NT(k, j, i + 1));
             }
             visc_face(cs, et, ze, cs, a.fi.aj[q], d, a.nu, nut, a.les, a.laplacian, vs);
         }
@@ -133,38 +167,38 @@ __global__ void __launch_bounds__(FF_TPB) k_face_fluxes(FaceArgs a)
             double d[3][3];
             // Compute_du_j, k-omega.c:141-200
             for (int c = 0; c < 3; c++) {
-                if (NVd(a, k, j, i + 1) > SOL || NVd(a, k, j + 1, i + 1) > SOL)
-                    d[c][0] = (Uc(a, c, k, j + 1, i) + Uc(a, c, k, j, i) - Uc(a, c, k, j + 1, i - 1) - Uc(a, c, k, j, i - 1)) * 0.5;
-                else if (NVd(a, k, j, i - 1) > SOL || NVd(a, k, j + 1, i - 1) > SOL)
-                    d[c][0] = (Uc(a, c, k, j + 1, i + 1) + Uc(a, c, k, j, i + 1) - Uc(a, c, k, j + 1, i) - Uc(a, c, k, j, i)) * 0.5;
+                if (T.NV(k, j, i + 1) > SOL || T.NV(k, j + 1, i + 1) > SOL)
+                    d[c][0] = (T.U(c, k, j + 1, i) + T.U(c, k, j, i) - T.U(c, k, j + 1, i - 1) - T.U(c, k, j, i - 1)) * 0.5;
+                else if (T.NV(k, j, i - 1) > SOL || T.NV(k, j + 1, i - 1) > SOL)
+                    d[c][0] = (T.U(c, k, j + 1, i + 1) + T.U(c, k, j, i + 1) - T.U(c, k, j + 1, i) - T.U(c, k, j, i)) * 0.5;
                 else
-                    d[c][0] = (Uc(a, c, k, j + 1, i + 1) + Uc(a, c, k, j, i + 1) - Uc(a, c, k, j + 1, i - 1) - Uc(a, c, k, j, i - 1)) * 0.25;
-                d[c][1] = Uc(a, c, k, j + 1, i) - Uc(a, c, k, j, i);
-                if (NVd(a, k + 1, j, i) > SOL || NVd(a, k + 1, j + 1, i) > SOL)
-                    d[c][2] = (Uc(a, c, k, j + 1, i) + Uc(a, c, k, j, i) - Uc(a, c, k - 1, j + 1, i) - Uc(a, c, k - 1, j, i)) * 0.5;
-                else if (NVd(a, k - 1, j, i) > SOL || NVd(a, k - 1, j + 1, i) > SOL)
-                    d[c][2] = (Uc(a, c, k + 1, j + 1, i) + Uc(a, c, k + 1, j, i) - Uc(a, c, k, j + 1, i) - Uc(a, c, k, j, i)) * 0.5;
+                    d[c][0] = (T.U(c, k, j + 1, i + 1) + T.U(c, k, j, i + 1) - T.U(c, k, j + 1, i - 1) - T.U(c, k, j, i - 1)) * 0.25;
+                d[c][1] = T.U(c, k, j + 1, i) - T.U(c, k, j, i);
+                if (T.NV(k + 1, j, i) > SOL || T.NV(k + 1, j + 1, i) > SOL)
+                    d[c][2] = (T.U(c, k, j + 1, i) + T.U(c, k, j, i) - T.U(c, k - 1, j + 1, i) - T.U(c, k - 1, j, i)) * 0.5;
+                else if (T.NV(k - 1, j, i) > SOL || T.NV(k - 1, j + 1, i) > SOL)
+                    d[c][2] = (T.U(c, k + 1, j + 1, i) + T.U(c, k + 1, j, i) - T.U(c, k, j + 1, i) - T.U(c, k, j, i)) * 0.5;
                 else
-                    d[c][2] = (Uc(a, c, k + 1, j + 1, i) + Uc(a, c, k + 1, j, i) - Uc(a, c, k - 1, j + 1, i) - Uc(a, c, k - 1, j, i)) * 0.25;
+                    d[c][2] = (T.U(c, k + 1, j + 1, i) + T.U(c, k + 1, j, i) - T.U(c, k - 1, j + 1, i) - T.U(c, k - 1, j, i)) * 0.25;
             }
             int jL = j - 1, jR = j + 2;
             if (j == 0 || j == my - 2) {
                 if (g.pj && j == my - 2) jR = my + 2;
                 else if (g.pj && j == 0) jL = -3;
                 else jL = j, jR = j + 1;
-            } else if (NVd(a, k, jL, i) + NVd(a, k, jR, i) > 0.1) jL = j, jR = j + 1;
+            } else if (T.NV(k, jL, i) + T.NV(k, jR, i) > 0.1) jL = j, jR = j + 1;
             if (a.second_order) jL = j, jR = j + 1;
             const double uc = a.ucont[3 * q + 1];
             for (int c = 0; c < 3; c++) {
-                if (a.second_order) dv[c] = -uc * 0.5 * (Uc(a, c, k, j, i) + Uc(a, c, k, j + 1, i));
-                else dv[c] = -uc * 0.0625 * (-Uc(a, c, k, jL, i) + 9. * Uc(a, c, k, j, i) + 9. * Uc(a, c, k, j + 1, i) - Uc(a, c, k, jR, i));
+                if (a.second_order) dv[c] = -uc * 0.5 * (T.U(c, k, j, i) + T.U(c, k, j + 1, i));
+                else dv[c] = -uc * 0.0625 * (-T.U(c, k, jL, i) + 9. * T.U(c, k, j, i) + 9. * T.U(c, k, j + 1, i) - T.U(c, k, jR, i));
             }
-            if (NVd(a, k, j, i) + NVd(a, k, j + 1, i) > 0.1) dv[0] = dv[1] = dv[2] = 0;
+            if (T.NV(k, j, i) + T.NV(k, j + 1, i) > 0.1) dv[0] = dv[1] = dv[2] = 0;
             double nut = 0;
             if (a.les) {
-                if ((j == 0 && !g.pj) || NVd(a, k, j, i) > 0.1) nut = a.nut[gidx(g, k, j + 1, i)];
-                else if ((j == my - 2 && !g.pj) || NVd(a, k, j + 1, i) > 0.1) nut = a.nut[q];
-                else nut = 0.5 * (a.nut[q] + a.nut[gidx(g, k, j + 1, i)]);
+                if ((j == 0 && !g.pj) || T.NV(k, j, i) > 0.1) nut = T.NT(k, j + 1, i);
+                else if ((j == my - 2 && !g.pj) || T.NV(k, j + 1, i) > 0.1) nut = T.NT(k, j, i);
+                else nut = 0.5 * (T.NT(k, j, i) + T.NT(k, j + 1, i));
             }
             visc_face(cs, et, ze, et, a.fj.aj[q], d, a.nu, nut, a.les, a.laplacian, vs);
         }
@@ -180,42 +214,91 @@ __global__ void __launch_bounds__(FF_TPB) k_face_fluxes(FaceArgs a)
             double d[3][3];
             // Compute_du_k, k-omega.c:227-288
             for (int c = 0; c < 3; c++) {
-                if (NVd(a, k, j, i + 1) > SOL || NVd(a, k + 1, j, i + 1) > SOL)
-                    d[c][0] = (Uc(a, c, k + 1, j, i) + Uc(a, c, k, j, i) - Uc(a, c, k + 1, j, i - 1) - Uc(a, c, k, j, i - 1)) * 0.5;
-                else if (NVd(a, k, j, i - 1) > SOL || NVd(a, k + 1, j, i - 1) > SOL)
-                    d[c][0] = (Uc(a, c, k + 1, j, i + 1) + Uc(a, c, k, j, i + 1) - Uc(a, c, k + 1, j, i) - Uc(a, c, k, j, i)) * 0.5;
+                if (T.NV(k, j, i + 1) > SOL || T.NV(k + 1, j, i + 1) > SOL)
+                    d[c][0] = (T.U(c, k + 1, j, i) + T.U(c, k, j, i) - T.U(c, k + 1, j, i - 1) - T.U(c, k, j, i - 1)) * 0.5;
+                else if (T.NV(k, j, i - 1) > SOL || T.NV(k + 1, j, i - 1) > SOL)
+                    d[c][0] = (T.U(c, k + 1, j, i + 1) + T.U(c, k, j, i + 1) - T.U(c, k + 1, j, i) - T.U(c, k, j, i)) * 0.5;
                 else
-                    d[c][0] = (Uc(a, c, k + 1, j, i + 1) + Uc(a, c, k, j, i + 1) - Uc(a, c, k + 1, j, i - 1) - Uc(a, c, k, j, i - 1)) * 0.25;
-                if (NVd(a, k, j + 1, i) > SOL || NVd(a, k + 1, j + 1, i) > SOL)
-                    d[c][1] = (Uc(a, c, k + 1, j, i) + Uc(a, c, k, j, i) - Uc(a, c, k + 1, j - 1, i) - Uc(a, c, k, j - 1, i)) * 0.5;
-                else if (NVd(a, k, j - 1, i) > SOL || NVd(a, k + 1, j - 1, i) > SOL)
-                    d[c][1] = (Uc(a, c, k + 1, j + 1, i) + Uc(a, c, k, j + 1, i) - Uc(a, c, k + 1, j, i) - Uc(a, c, k, j, i)) * 0.5;
+                    d[c][0] = (T.U(c, k + 1, j, i + 1) + T.U(c, k, j, i + 1) - T.U(c, k + 1, j, i - 1) - T.U(c, k, j, i - 1)) * 0.25;
+                if (T.NV(k, j + 1, i) > SOL || T.NV(k + 1, j + 1, i) > SOL)
+                    d[c][1] = (T.U(c, k + 1, j, i) + T.U(c, k, j, i) - T.U(c, k + 1, j - 1, i) - T.U(c, k, j - 1, i)) * 0.5;
+                else if (T.NV(k, j - 1, i) > SOL || T.NV(k + 1, j - 1, i) > SOL)
+                    d[c][1] = (T.U(c, k + 1, j + 1, i) + T.U(c, k, j + 1, i) - T.U(c, k + 1, j, i) - T.U(c, k, j, i)) * 0.5;
                 else
-                    d[c][1] = (Uc(a, c, k + 1, j + 1, i) + Uc(a, c, k, j + 1, i) - Uc(a, c, k + 1, j - 1, i) - Uc(a, c, k, j - 1, i)) * 0.25;
-                d[c][2] = Uc(a, c, k + 1, j, i) - Uc(a, c, k, j, i);
+                    d[c][1] = (T.U(c, k + 1, j + 1, i) + T.U(c, k, j + 1, i) - T.U(c, k + 1, j - 1, i) - T.U(c, k, j - 1, i)) * 0.25;
+                d[c][2] = T.U(c, k + 1, j, i) - T.U(c, k, j, i);
             }
             int kL = k - 1, kR = k + 2;
             if (k == 0 || k == mz - 2) {
                 if (g.pk && k == mz - 2) kR = mz + 2;
                 else if (g.pk && k == 0) kL = -3;
                 else kL = k, kR = k + 1;
-            } else if (NVd(a, kL, j, i) + NVd(a, kR, j, i) > 0.1) kL = k, kR = k + 1;
+            } else if (T.NV(kL, j, i) + T.NV(kR, j, i) > 0.1) kL = k, kR = k + 1;
             if (a.second_order) kL = k, kR = k + 1;
             const double uc = a.ucont[3 * q + 2];
             for (int c = 0; c < 3; c++) {
-                if (a.second_order) dv[c] = -uc * 0.5 * (Uc(a, c, k, j, i) + Uc(a, c, k + 1, j, i));
-                else dv[c] = -uc * 0.0625 * (-Uc(a, c, kL, j, i) + 9. * Uc(a, c, k, j, i) + 9. * Uc(a, c, k + 1, j, i) - Uc(a, c, kR, j, i));
+                if (a.second_order) dv[c] = -uc * 0.5 * (T.U(c, k, j, i) + T.U(c, k + 1, j, i));
+                else dv[c] = -uc * 0.0625 * (-T.U(c, kL, j, i) + 9. * T.U(c, k, j, i) + 9. * T.U(c, k + 1, j, i) - T.U(c, kR, j, i));
             }
-            if (NVd(a, k, j, i) + NVd(a, k + 1, j, i) > 0.1) dv[0] = dv[1] = dv[2] = 0;
+            if (T.NV(k, j, i) + T.NV(k + 1, j, i) > 0.1) dv[0] = dv[1] = dv[2] = 0;
             double nut = 0;
             if (a.les) {
-                if ((k == 0 && !g.pk) || NVd(a, k, j, i) > 0.1) nut = a.nut[gidx(g, k + 1, j, i)];
-                else if ((k == mz - 2 && !g.pk) || NVd(a, k + 1, j, i) > 0.1) nut = a.nut[q];
-                else nut = 0.5 * (a.nut[q] + a.nut[gidx(g, k + 1, j, i)]);
+                if ((k == 0 && !g.pk) || T.NV(k, j, i) > 0.1) nut = T.NT(k + 1, j, i);
+                else if ((k == mz - 2 && !g.pk) || T.NV(k + 1, j, i) > 0.1) nut = T.NT(k, j, i);
+                else nut = 0.5 * (T.NT(k, j, i) + T.NT(k + 1, j, i));
             }
             visc_face(cs, et, ze, ze, a.fk.aj[q], d, a.nu, nut, a.les, a.laplacian, vs);
         }
         for (int c = 0; c < 3; c++) { a.out[(6 + c) * n + q] = dv[c]; a.out[(15 + c) * n + q] = vs[c]; }
+    }
+}
+
+// One block owns an FT_I x FT_J column of points and marches FT_K planes in
+// k, keeping a rolling 3-plane LDS tile of ucat / nvert / nu_t.
+__global__ void __launch_bounds__(FT_I * FT_J) k_face_fluxes(FaceArgs a)
+{
+    __shared__ double sx[3 * FT_HP], sy[3 * FT_HP], sz[3 * FT_HP], sn[3 * FT_HP], st[3 * FT_HP];
+    const Grid &g = a.g;
+    const int nbi = (g.mx + FT_I - 1) / FT_I, nbj = (g.my + FT_J - 1) / FT_J;
+    int b = blockIdx.x;
+    const int bi = b % nbi;
+    b /= nbi;
+    const int bj = b % nbj, bk = b / nbj;
+    const int i0 = bi * FT_I, j0 = bj * FT_J, k0 = bk * FT_K;
+    const int k1 = k0 + FT_K < g.mz ? k0 + FT_K : g.mz;
+    const int tx = threadIdx.x % FT_I, ty = threadIdx.x / FT_I;
+    const bool les = a.les != 0;
+    auto load_plane = [&](int kk, int slot) {
+        const int wk = wrapi(kk, g.mz, g.pk);
+        for (int p = threadIdx.x; p < FT_HP; p += FT_I * FT_J) {
+            const int di = p % FT_HI, dj = p / FT_HI;
+            const int wi = wrapi(i0 - 1 + di, g.mx, g.pi), wj = wrapi(j0 - 1 + dj, g.my, g.pj);
+            double vx = 0, vy = 0, vz = 0, vn = 0, vt = 0;
+            if (wi >= 0 && wi < g.mx && wj >= 0 && wj < g.my && wk >= 0 && wk < g.mz) {
+                const long long q = ((long long)wk * g.my + wj) * g.mx + wi;
+                vx = a.ucat[3 * q]; vy = a.ucat[3 * q + 1]; vz = a.ucat[3 * q + 2];
+                vn = a.nvert[q];
+                if (les) vt = a.nut[q];
+            }
+            const int l = slot * FT_HP + p;
+            sx[l] = vx; sy[l] = vy; sz[l] = vz; sn[l] = vn; st[l] = vt;
+        }
+    };
+    load_plane(k0 - 1, 0);
+    load_plane(k0, 1);
+    int slot0 = 0;
+    for (int k = k0; k < k1; k++) {
+        int s2 = slot0 + 2;
+        if (s2 >= 3) s2 -= 3;
+        load_plane(k + 1, s2);
+        __syncthreads();
+        const int i = i0 + tx, j = j0 + ty;
+        if (i < g.mx && j < g.my) {
+            TileAcc T{&a, sx, sy, sz, sn, st, i0, j0, k, slot0};
+            face_point(a, T, i, j, k);
+        }
+        __syncthreads();
+        slot0 = slot0 + 1 < 3 ? slot0 + 1 : 0;
     }
 }
 
@@ -315,50 +398,118 @@ __device__ inline void fp_at(const CellArgs &a, int k, int j, int i, double f[3]
     fp_interior(a, k, j, i, f);
 }
 
-__global__ void __launch_bounds__(FF_TPB) k_cell_rhs(CellArgs a)
+// Fp of every point as the projection reads it (fp_at), written once to a
+// 3-plane scratch.  A block owns an FT_I x FT_J column and marches FT_K planes
+// in k; on the second-order path a thread keeps its k-face fluxes in
+// registers, where they become the k-1 face of the next plane.
+__global__ void __launch_bounds__(FT_I * FT_J) k_cell_fp(CellArgs a, double *__restrict__ fp)
 {
     const Grid &g = a.g;
-    const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-    if (t >= g.n) return;
-    const int i = t % g.mx;
-    const long long rr = t / g.mx;
-    const int j = rr % g.my, k = rr / g.my;
-    const long long q = t;
-    const int mx = g.mx, my = g.my, mz = g.mz;
-    double r[3];
-    const bool interior = i >= 1 && i < mx - 1 && j >= 1 && j < my - 1 && k >= 1 && k < mz - 1;
-    bool zero[3] = {true, true, true};
-    if (interior) {
-        double f0[3], fi_[3], fj_[3], fk_[3];
-        fp_at(a, k, j, i, f0);
-        fp_at(a, k, j, i + 1, fi_);
-        fp_at(a, k, j + 1, i, fj_);
-        fp_at(a, k + 1, j, i, fk_);
-        const long long qi = gidx(g, k, j, i + 1), qj = gidx(g, k, j + 1, i), qk = gidx(g, k + 1, j, i);
-        const double *c0 = a.csi + 3 * q, *c1 = a.csi + 3 * qi;
-        const double *e0 = a.eta + 3 * q, *e1 = a.eta + 3 * qj;
-        const double *z0 = a.zet + 3 * q, *z1 = a.zet + 3 * qk;
-        r[0] = a.scale * (0.5 * (c0[0] * f0[0] + c0[1] * f0[1] + c0[2] * f0[2]) +
-                          0.5 * (c1[0] * fi_[0] + c1[1] * fi_[1] + c1[2] * fi_[2])) * a.iaj[q];
-        r[1] = a.scale * (0.5 * (e0[0] * f0[0] + e0[1] * f0[1] + e0[2] * f0[2]) +
-                          0.5 * (e1[0] * fj_[0] + e1[1] * fj_[1] + e1[2] * fj_[2])) * a.jaj[q];
-        r[2] = a.scale * (0.5 * (z0[0] * f0[0] + z0[1] * f0[1] + z0[2] * f0[2]) +
-                          0.5 * (z1[0] * fk_[0] + z1[1] * fk_[1] + z1[2] * fk_[2])) * a.kaj[q];
-        const double *nv = a.nvert;
-        zero[0] = nv[q] + nv[qi] > 0.1 || (!g.pi && i == mx - 2);
-        zero[1] = nv[q] + nv[qj] > 0.1 || (!g.pj && j == my - 2);
-        zero[2] = nv[q] + nv[qk] > 0.1 || (!g.pk && k == mz - 2);
-    }
-    if (a.pucont) {
-        // Rhs = 0 + F; Rhs += -1*dP; Ucont = alpha*Rhs + pUcont  (rhs.c:1647-1658)
-        for (int c = 0; c < 3; c++) {
-            double rv = zero[c] ? 0.0 : (0.0 + r[c]);
-            rv += -1 * a.dp[3 * q + c];
-            a.ucont_out[3 * q + c] = a.alpha * rv + a.pucont[3 * q + c];
+    const int nbi = (g.mx + FT_I - 1) / FT_I, nbj = (g.my + FT_J - 1) / FT_J;
+    int b = blockIdx.x;
+    const int bi = b % nbi;
+    b /= nbi;
+    const int bj = b % nbj, bk = b / nbj;
+    const int i = bi * FT_I + threadIdx.x % FT_I, j = bj * FT_J + threadIdx.x / FT_I, k0 = bk * FT_K;
+    if (i >= g.mx || j >= g.my) return;
+    const int k1 = k0 + FT_K < g.mz ? k0 + FT_K : g.mz;
+    const long long n = g.n;
+    double km1[6];
+    bool have = false;
+    for (int k = k0; k < k1; k++) {
+        const long long q = ((long long)k * g.my + j) * g.mx + i;
+        double f[3];
+        const bool interior = i >= 1 && i < g.mx - 1 && j >= 1 && j < g.my - 1 && k >= 1 && k < g.mz - 1;
+        if (interior && a.second_order) {
+            const long long qi = q - 1, qj = q - g.mx;
+            const double *F = a.face;
+            double own[6];
+            for (int c = 0; c < 3; c++) { own[c] = F[(6 + c) * n + q]; own[3 + c] = F[(15 + c) * n + q]; }
+            if (!have) {
+                const long long qk = q - g.plane;
+                for (int c = 0; c < 3; c++) { km1[c] = F[(6 + c) * n + qk]; km1[3 + c] = F[(15 + c) * n + qk]; }
+            }
+            // same operation order as fp_interior
+            for (int c = 0; c < 3; c++) {
+                f[c] = F[c * n + q] - F[c * n + qi] + F[(3 + c) * n + q] - F[(3 + c) * n + qj] + own[c] - km1[c];
+                f[c] += (F[(9 + c) * n + q] - F[(9 + c) * n + qi] + F[(12 + c) * n + q] - F[(12 + c) * n + qj] + own[3 + c] -
+                         km1[3 + c]) / 1.0;
+            }
+            for (int c = 0; c < 6; c++) km1[c] = own[c];
+            have = true;
+        } else {
+            fp_at(a, k, j, i, f);
+            have = false;
         }
-    } else {
-        for (int c = 0; c < 3; c++) a.rhs[3 * q + c] = zero[c] ? 0.0 : a.rhs[3 * q + c] + r[c];
+        for (int c = 0; c < 3; c++) fp[c * n + q] = f[c];
     }
+}
+
+// Contravariant projection of Fp and boundary zeroing (momentum.c:1715-1938),
+// marching in k with the k+1 neighbour carried in registers.
+__global__ void __launch_bounds__(FT_I * FT_J) k_cell_rhs(CellArgs a, const double *__restrict__ fp)
+{
+    const Grid &g = a.g;
+    const int nbi = (g.mx + FT_I - 1) / FT_I, nbj = (g.my + FT_J - 1) / FT_J;
+    int b = blockIdx.x;
+    const int bi = b % nbi;
+    b /= nbi;
+    const int bj = b % nbj, bk = b / nbj;
+    const int i = bi * FT_I + threadIdx.x % FT_I, j = bj * FT_J + threadIdx.x / FT_I, k0 = bk * FT_K;
+    if (i >= g.mx || j >= g.my) return;
+    const int k1 = k0 + FT_K < g.mz ? k0 + FT_K : g.mz;
+    const long long n = g.n;
+    const int mx = g.mx, my = g.my, mz = g.mz;
+    double fk_[3], z1[3], nvk = 0;
+    bool have = false;
+    for (int k = k0; k < k1; k++) {
+        const long long q = ((long long)k * g.my + j) * g.mx + i;
+        double r[3];
+        const bool interior = i >= 1 && i < mx - 1 && j >= 1 && j < my - 1 && k >= 1 && k < mz - 1;
+        bool zero[3] = {true, true, true};
+        if (interior) {
+            const long long qi = q + 1, qj = q + mx, qk = gidx(g, k + 1, j, i);
+            double f0[3], fi_[3], fj_[3];
+            for (int c = 0; c < 3; c++) {
+                f0[c] = have ? fk_[c] : fp[c * n + q];
+                fi_[c] = fp[c * n + qi];
+                fj_[c] = fp[c * n + qj];
+            }
+            const double nv0 = have ? nvk : a.nvert[q];
+            double fkn[3], zkn[3];
+            for (int c = 0; c < 3; c++) { fkn[c] = fp[c * n + qk]; zkn[c] = a.zet[3 * qk + c]; }
+            const double nvkn = a.nvert[qk];
+            const double *c0 = a.csi + 3 * q, *c1 = a.csi + 3 * qi;
+            const double *e0 = a.eta + 3 * q, *e1 = a.eta + 3 * qj;
+            const double *z0 = a.zet + 3 * q;
+            r[0] = a.scale * (0.5 * (c0[0] * f0[0] + c0[1] * f0[1] + c0[2] * f0[2]) +
+                              0.5 * (c1[0] * fi_[0] + c1[1] * fi_[1] + c1[2] * fi_[2])) * a.iaj[q];
+            r[1] = a.scale * (0.5 * (e0[0] * f0[0] + e0[1] * f0[1] + e0[2] * f0[2]) +
+                              0.5 * (e1[0] * fj_[0] + e1[1] * fj_[1] + e1[2] * fj_[2])) * a.jaj[q];
+            r[2] = a.scale * (0.5 * (z0[0] * f0[0] + z0[1] * f0[1] + z0[2] * f0[2]) +
+                              0.5 * (zkn[0] * fkn[0] + zkn[1] * fkn[1] + zkn[2] * fkn[2])) * a.kaj[q];
+            const double *nv = a.nvert;
+            zero[0] = nv0 + nv[qi] > 0.1 || (!g.pi && i == mx - 2);
+            zero[1] = nv0 + nv[qj] > 0.1 || (!g.pj && j == my - 2);
+            zero[2] = nv0 + nvkn > 0.1 || (!g.pk && k == mz - 2);
+            for (int c = 0; c < 3; c++) { fk_[c] = fkn[c]; z1[c] = zkn[c]; }
+            nvk = nvkn;
+            have = qk == q + g.plane;  // the carried values belong to the next plane
+        } else {
+            have = false;
+        }
+        if (a.pucont) {
+            // Rhs = 0 + F; Rhs += -1*dP; Ucont = alpha*Rhs + pUcont  (rhs.c:1647-1658)
+            for (int c = 0; c < 3; c++) {
+                double rv = zero[c] ? 0.0 : (0.0 + r[c]);
+                rv += -1 * a.dp[3 * q + c];
+                a.ucont_out[3 * q + c] = a.alpha * rv + a.pucont[3 * q + c];
+            }
+        } else {
+            for (int c = 0; c < 3; c++) a.rhs[3 * q + c] = zero[c] ? 0.0 : a.rhs[3 * q + c] + r[c];
+        }
+    }
+    (void)z1;
 }
 
 int formfunction2(vfs_ctx *c, double scale, const double *pucont, double rk_alpha, double *ucont_out)
@@ -382,7 +533,10 @@ int formfunction2(vfs_ctx *c, double scale, const double *pucont, double rk_alph
     fa.laplacian = c->p.laplacian;
     fa.out = c->face;
     prof_begin(c, K_FACE);
-    hipLaunchKernelGGL(k_face_fluxes, dim3(blocks_for(g.n, FF_TPB)), dim3(FF_TPB), 0, st, fa);
+    {
+        const unsigned nb = (unsigned)(((g.mx + FT_I - 1) / FT_I) * ((g.my + FT_J - 1) / FT_J) * ((g.mz + FT_K - 1) / FT_K));
+        hipLaunchKernelGGL(k_face_fluxes, dim3(nb), dim3(FT_I * FT_J), 0, st, fa);
+    }
     prof_end(c);
     VFS_LAUNCH_CHECK("face fluxes");
     const long long np[3] = {2LL * g.my * g.mz, 2LL * g.mx * g.mz, 2LL * g.mx * g.my};
@@ -407,8 +561,12 @@ int formfunction2(vfs_ctx *c, double scale, const double *pucont, double rk_alph
     ca.pucont = pucont;
     ca.alpha = rk_alpha;
     ca.ucont_out = ucont_out;
+    const unsigned nbt = (unsigned)(((g.mx + FT_I - 1) / FT_I) * ((g.my + FT_J - 1) / FT_J) * ((g.mz + FT_K - 1) / FT_K));
+    prof_begin(c, K_CELLFP);
+    hipLaunchKernelGGL(k_cell_fp, dim3(nbt), dim3(FT_I * FT_J), 0, st, ca, c->fpc);
+    prof_end(c);
     prof_begin(c, K_CELL);
-    hipLaunchKernelGGL(k_cell_rhs, dim3(blocks_for(g.n, FF_TPB)), dim3(FF_TPB), 0, st, ca);
+    hipLaunchKernelGGL(k_cell_rhs, dim3(nbt), dim3(FT_I * FT_J), 0, st, ca, (const double *)c->fpc);
     prof_end(c);
     VFS_LAUNCH_CHECK("cell rhs");
     return VFS_OK;

@@ -441,15 +441,22 @@ __global__ void __launch_bounds__(PTPB) k_zebra_solve(Grid g, ZGrid z, int c, co
     }
 }
 
-// coarsest level: x = B^{-1} b over the rows, B^{-1} stored column-major
+// coarsest level: x = B^{-1} b over the rows, B^{-1} stored column-major.
+// Each block gathers b into LDS once, then DENSE_ROWS threads each form one
+// row; the column loop reads a contiguous run of B^{-1} per step.
+constexpr int DENSE_ROWS = 64, DENSE_LDS = 1024;
 template <typename T>
-__global__ void k_dense_solve(int nr, const int *__restrict__ ridx, const T *__restrict__ inv, const T *__restrict__ b,
-                              T *__restrict__ x)
+__global__ void __launch_bounds__(DENSE_ROWS) k_dense_solve(int nr, const int *__restrict__ ridx, const T *__restrict__ inv,
+                                                          const T *__restrict__ b, T *__restrict__ x)
 {
-    int r = blockIdx.x * blockDim.x + threadIdx.x;
+    __shared__ T bs[DENSE_LDS];
+    for (int c = threadIdx.x; c < nr; c += blockDim.x) bs[c] = b[ridx[c]];
+    __syncthreads();
+    const int r = blockIdx.x * DENSE_ROWS + threadIdx.x;
     if (r >= nr) return;
     T s = 0;
-    for (int c = 0; c < nr; c++) s += inv[(long long)c * nr + r] * b[ridx[c]];
+#pragma unroll 8
+    for (int c = 0; c < nr; c++) s += inv[(long long)c * nr + r] * bs[c];
     x[ridx[r]] = s;
 }
 
@@ -685,6 +692,7 @@ namespace vfs {
 
 // largest coarsest level solved by a dense inverse
 constexpr int DENSE_MAX = 1024;
+static_assert(DENSE_MAX <= DENSE_LDS, "k_dense_solve stages the whole coarse right-hand side in LDS");
 
 // Every buffer starts zeroed: ghost cells and blanked cells of the work
 // vectors are read (times a zero coefficient) but never written, and garbage
@@ -945,8 +953,8 @@ static void vcycle(vfs_ctx *c, vfs_poisson *ps, int l, int pre, int post, int co
     const unsigned nb = blocks_for(L.g.n, PTPB);
     if (l == ps->nl - 1) {
         if (L.nr > 0)
-            hipLaunchKernelGGL(k_dense_solve<T>, dim3(blocks_for(L.nr, PTPB)), dim3(PTPB), 0, st, L.nr, L.ridx, LV<T>::inv(L),
-                               LV<T>::b(L), LV<T>::x(L));
+            hipLaunchKernelGGL(k_dense_solve<T>, dim3(blocks_for(L.nr, DENSE_ROWS)), dim3(DENSE_ROWS), 0, st, L.nr, L.ridx,
+                               LV<T>::inv(L), LV<T>::b(L), LV<T>::x(L));
         else
             smooth<T>(c, L, coarse);
         return;

@@ -105,7 +105,7 @@ int vfs_ctx_create(const vfs_params *params, vfs_ctx **out)
     std::memset(c->f, 0, sizeof(c->f));
     c->p = *params;
     c->g = vfs::make_grid(*params);
-    c->face = c->pucont = c->snap = c->red = c->red_host = nullptr;
+    c->face = c->pucont = c->snap = c->fpc = c->red = c->red_host = nullptr;
     c->ps = nullptr;
     c->mean_k_area = 0;
     c->mean_k_flux = 0;
@@ -126,6 +126,7 @@ int vfs_ctx_create(const vfs_params *params, vfs_ctx **out)
     if (!rc) rc = alloc(&c->face, 18 * n);
     if (!rc) rc = alloc(&c->pucont, 3 * n);
     if (!rc) rc = alloc(&c->snap, 3 * n);
+    if (!rc) rc = alloc(&c->fpc, 3 * n);
     if (!rc) rc = alloc(&c->red, 1 << 16);
     if (!rc) {
         hipError_t ee = hipHostMalloc((void **)&c->red_host, 4096 * sizeof(double), hipHostMallocDefault);
@@ -149,6 +150,7 @@ int vfs_ctx_destroy(vfs_ctx *c)
     if (c->face) hipFree(c->face);
     if (c->pucont) hipFree(c->pucont);
     if (c->snap) hipFree(c->snap);
+    if (c->fpc) hipFree(c->fpc);
     if (c->red) hipFree(c->red);
     if (c->red_host) hipHostFree(c->red_host);
     if (c->ev_pool) {
@@ -352,7 +354,7 @@ int vfs_divergence(vfs_ctx *c, double *maxdiv)
 
 static const char *k_names[vfs::K_NUMK] = {"face_fluxes",     "cell_rhs",          "line_rhs_fp32",  "line_rhs_fp64",
                                            "poisson_apply_fp64", "contra2cart",     "pressure_gradient", "eddy_viscosity",
-                                           "line_solve_fp32",  "line_solve_fp64"};
+                                           "line_solve_fp32",  "line_solve_fp64",   "cell_fp"};
 
 int vfs_profile(vfs_ctx *c, int enable)
 {

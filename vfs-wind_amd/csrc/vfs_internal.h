@@ -61,6 +61,19 @@ __host__ __device__ inline bool periodic_src(const Grid &g, int k, int j, int i,
     return fi || fj || fk;
 }
 
+// inverse of [csi; eta; zet] (Calculate_Covariant_metrics, rhs2.c:595-612)
+__host__ __device__ inline void covariant(const double gm[3][3], double G[3][3])
+{
+#pragma clang fp contract(off)
+    const double a11 = gm[0][0], a12 = gm[0][1], a13 = gm[0][2];
+    const double a21 = gm[1][0], a22 = gm[1][1], a23 = gm[1][2];
+    const double a31 = gm[2][0], a32 = gm[2][1], a33 = gm[2][2];
+    double det = a11 * (a33 * a22 - a32 * a23) - a21 * (a33 * a12 - a32 * a13) + a31 * (a23 * a12 - a22 * a13);
+    G[0][0] = (a33 * a22 - a32 * a23) / det; G[0][1] = -(a33 * a12 - a32 * a13) / det; G[0][2] = (a23 * a12 - a22 * a13) / det;
+    G[1][0] = -(a33 * a21 - a31 * a23) / det; G[1][1] = (a33 * a11 - a31 * a13) / det; G[1][2] = -(a23 * a11 - a21 * a13) / det;
+    G[2][0] = (a32 * a21 - a31 * a22) / det; G[2][1] = -(a32 * a11 - a31 * a12) / det; G[2][2] = (a22 * a11 - a21 * a12) / det;
+}
+
 Grid make_grid(const vfs_params &p);
 
 // Status plumbing shared by the C ABI.
@@ -88,7 +101,7 @@ struct vfs_poisson;
 
 namespace vfs {
 // kernels whose launches can be timed with HIP events on the context stream
-enum KernelId { K_FACE = 0, K_CELL, K_LINE32, K_LINE64, K_APPLY64, K_C2C, K_PGRAD, K_EDDY, K_LSOLVE32, K_LSOLVE64, K_NUMK };
+enum KernelId { K_FACE = 0, K_CELL, K_LINE32, K_LINE64, K_APPLY64, K_C2C, K_PGRAD, K_EDDY, K_LSOLVE32, K_LSOLVE64, K_CELLFP, K_NUMK };
 }
 
 struct vfs_ctx {
@@ -100,6 +113,7 @@ struct vfs_ctx {
     double *face;     // 18 doubles per point: div1..3, visc1..3 (Cmpnts each)
     double *pucont;   // RK base state
     double *snap;     // 3 doubles per point: snapshot for BC passes
+    double *fpc;      // 3 planes of n doubles: Fp of every cell (Formfunction_2)
     double *red;      // reduction scratch (device)
     double *red_host; // pinned host mirror of reduction results
     double mean_k_area;

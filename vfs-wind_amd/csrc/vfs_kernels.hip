@@ -323,18 +323,6 @@ __global__ void k_c2c_interior(Grid g, const double *__restrict__ ucont, const d
     ucat[3 * q + 2] = det2 / det;
 }
 
-// inverse of [csi; eta; zet] (Calculate_Covariant_metrics, rhs2.c:595-612)
-__device__ inline void covariant(const double gm[3][3], double G[3][3])
-{
-    const double a11 = gm[0][0], a12 = gm[0][1], a13 = gm[0][2];
-    const double a21 = gm[1][0], a22 = gm[1][1], a23 = gm[1][2];
-    const double a31 = gm[2][0], a32 = gm[2][1], a33 = gm[2][2];
-    double det = a11 * (a33 * a22 - a32 * a23) - a21 * (a33 * a12 - a32 * a13) + a31 * (a23 * a12 - a22 * a13);
-    G[0][0] = (a33 * a22 - a32 * a23) / det; G[0][1] = -(a33 * a12 - a32 * a13) / det; G[0][2] = (a23 * a12 - a22 * a13) / det;
-    G[1][0] = -(a33 * a21 - a31 * a23) / det; G[1][1] = (a33 * a11 - a31 * a13) / det; G[1][2] = -(a23 * a11 - a21 * a13) / det;
-    G[2][0] = (a32 * a21 - a31 * a22) / det; G[2][1] = -(a32 * a11 - a31 * a12) / det; G[2][2] = (a22 * a11 - a21 * a12) / det;
-}
-
 // slip reflection, rhs.c:447-544
 __device__ inline void slip_fill(const Grid &g, const double *csi, const double *eta, const double *zet,
                                  const double *uold, int kn, int jn, int in, int dir, double sgn, double *out)
