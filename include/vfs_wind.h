@@ -68,6 +68,8 @@ typedef enum {
     VFS_RHS,        /* momentum right-hand side (Rhs)                    */
     VFS_B,          /* Poisson right-hand side                           */
     VFS_FEUL,       /* turbine body force on faces (lF_eul)              */
+    VFS_UCONT_O,    /* Ucont at the start of the step (Ucont_o)          */
+    VFS_RHS_O,      /* Formfunction_2 of Ucont_o (RHS_o, solvers.c:627)   */
     VFS_NFIELDS
 } vfs_field;
 
@@ -142,17 +144,48 @@ int vfs_projection(vfs_ctx *ctx);
 /* Divergence (main.c:936-1074): max |div U| * aj over fluid cells. */
 int vfs_divergence(vfs_ctx *ctx, double *maxdiv);
 
+/* ---- implicit momentum, -imp 4 (implicitsolver.c, momentum.c) ---------- */
+typedef struct {
+    double rtol;         /* -imp_tol: SNES and GMRES relative tolerance (1e-4)  */
+    int restart;         /* GMRES restart length (30)                         */
+    int max_newton;      /* Newton iteration cap (50)                         */
+    int max_linear;      /* GMRES iteration cap per Newton step (1000)        */
+    int use_feul;        /* add VFS_FEUL (rotor / nacelle / IB delta models)  */
+} vfs_newton_opts;
+
+typedef struct {
+    int newton_iterations;
+    int linear_iterations;
+    int function_evals;
+    double fnorm0, fnorm;  /* ||R|| at the start and the end                 */
+    double ms;
+} vfs_newton_stats;
+
+/* FormFunction_SNES (momentum.c:2237-2336), time_coeff() == 1:
+ * r = -(u - Ucont_o)/dt + 0.5 F(u) + 0.5 RHS_o - dP (+ F_eul); device arrays of
+ * 3 doubles per point.  Leaves VFS_UCONT/VFS_UCAT at the wall-zeroed u. */
+int vfs_formfunction_snes(vfs_ctx *ctx, const double *u_dev, double *r_dev, int use_feul);
+/* Ucont_o = Ucont and RHS_o = Formfunction_2(Ucont) (solvers.c:627-628). */
+int vfs_store_old_state(vfs_ctx *ctx);
+/* Implicit_MatrixFree (implicitsolver.c:4203-4446): solve R(Ucont) = 0 with
+ * matrix-free Newton-Krylov, then the post-solve boundary fix and FormBCS. */
+int vfs_implicit_matrixfree(vfs_ctx *ctx, const vfs_newton_opts *opts, vfs_newton_stats *stats);
+
 /* ---- one step ------------------------------------------------------------ */
 typedef struct {
     int poisson_iterations;
     double poisson_relres;
     double maxdiv;
     double ms_momentum, ms_poisson, ms_total;
+    int newton_iterations, linear_iterations;  /* implicit step only */
 } vfs_step_stats;
 
 /* Flow_Solver (solvers.c:313-759) for -imp 0: k-flux, LES, Pressure_Gradient,
  * RungeKutta, Poisson, UpdatePressure, Projection, Divergence. */
 int vfs_fractional_step(vfs_ctx *ctx, const vfs_solve_opts *opts, vfs_step_stats *stats);
+/* The same step with the -imp 4 momentum solve (solvers.c:626-630). */
+int vfs_fractional_step_cn(vfs_ctx *ctx, const vfs_newton_opts *nopts, const vfs_solve_opts *opts,
+                           vfs_step_stats *stats);
 
 /* ---- actuator line (rotor_model.c) ---------------------------------------
  * One turbine with line elements (rotor_model 3 or 6).  The caller passes

@@ -91,6 +91,7 @@ def load() -> C.CDLL:
         "orc_dfunc_s3h": ([C.c_double], C.c_double),
         "orc_dfunc_s4h": ([C.c_double], C.c_double),
         "orc_dfunc_exp": ([C.c_double, C.c_double], C.c_double),
+        "orc_formfunction_snes": ([P, M, D, D, D, D, D, D, D, D, D, D], None),
         "orc_arbitrary_rotate": ([D, C.c_double, D, D], None),
         "orc_alm_elements": ([C.c_int, C.c_int, D, D, D], None),
         "orc_alm_rotate": ([C.POINTER(OrcTurbine), C.c_double, C.c_int, D, D], None),
@@ -303,6 +304,25 @@ def alm_flagr(t: OrcTurbine, nb: int, nvb: int, cent, ulagr, unode, twist, chord
     lib.orc_alm_flagr(C.byref(t), nb, nvb, _ptr(cent), _ptr(ulagr), _ptr(unode), _ptr(twist), _ptr(chord), foils.arr,
                       foils.n, _ptr(aoa), _ptr(cl), _ptr(cd), _ptr(f))
     return f, aoa, cl, cd
+
+
+def _snes_methods():
+    def formfunction_snes(self, u, ucont_o, rhs_o, feul=None):
+        """FormFunction_SNES at u; leaves ucont/ucat at the wall-zeroed u."""
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        uo = np.ascontiguousarray(ucont_o, dtype=np.float64)
+        ro = np.ascontiguousarray(rhs_o, dtype=np.float64)
+        fe = None if feul is None else np.ascontiguousarray(feul, dtype=np.float64)
+        r = np.zeros(self.shape + (3,))
+        self.lib.orc_formfunction_snes(C.byref(self.p), C.byref(self.cm), _ptr(u), _ptr(uo), _ptr(ro), self._f("dp"),
+                                       None if fe is None else _ptr(fe), self._f("ucont"), self._f("ucat"),
+                                       self._f("ucat_old"), self._f("nu_t"), _ptr(r))
+        return r
+
+    Oracle.formfunction_snes = formfunction_snes
+
+
+_snes_methods()
 
 
 def _alm_methods():

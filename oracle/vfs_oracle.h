@@ -144,6 +144,11 @@ double orc_dfunc_s3h(double r);
 double orc_dfunc_s4h(double r);
 double orc_dfunc_exp(double r, double n);
 
+/* a7b: CN residual, FormFunction_SNES (momentum.c:2237-2336), time_coeff() == 1 */
+void orc_formfunction_snes(const orc_params *p, const orc_metrics *m, const double *u, const double *ucont_o,
+                           const double *rhs_o, const double *dp, const double *feul, double *ucont, double *ucat,
+                           double *ucat_old, double *nu_t, double *r);
+
 /* actuator line (rotor_model.c): one turbine, line elements */
 typedef struct {
     double nx, ny, nz;       /* unit rotor axis (Turbine.inp, normalised)          */

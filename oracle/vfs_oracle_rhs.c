@@ -347,3 +347,49 @@ double orc_min_dt(const orc_params *p, const orc_metrics *m, const double *ucont
             }
     return ldt;
 }
+
+/* ------------------------------------------------------------------------ */
+/* a7b: FormFunction_SNES (momentum.c:2237-2336), time_coeff() == 1          */
+/* ------------------------------------------------------------------------ */
+
+/* r = -(u - ucont_o)/dt + 0.5 F(u) + 0.5 rhs_o - dp (+ feul when non-null),
+ * with u copied to ucont, the wall fluxes zeroed (momentum.c:2258-2282) and
+ * Contra2Cart applied before Formfunction_2.  PETSc VecAXPY order. */
+void orc_formfunction_snes(const orc_params *p, const orc_metrics *m, const double *u, const double *ucont_o,
+                           const double *rhs_o, const double *dp, const double *feul, double *ucont, double *ucat,
+                           double *ucat_old, double *nu_t, double *r)
+{
+    const size_t n3 = 3 * npts(p);
+    const int mx = p->mx, my = p->my, mz = p->mz;
+    const int *b = p->bctype;
+    memcpy(ucont, u, n3 * sizeof(double));
+    for (int k = 0; k < mz; k++)
+        for (int j = 0; j < my; j++)
+            for (int i = 0; i < mx; i++) {
+                if (i == 0 && b[0] == 1) VX(ucont, k, j, i) = 0;
+                if (i == mx - 1 && b[1] == 1) VX(ucont, k, j, i - 1) = 0;
+                if (j == 0 && b[2] == 1) VY(ucont, k, j, i) = 0;
+                if (j == my - 1 && b[3] == 1) VY(ucont, k, j - 1, i) = 0;
+                if (k == 0 && b[4] == 1) VZ(ucont, k, j, i) = 0;
+                if (k == mz - 1 && b[5] == 1) VZ(ucont, k - 1, j, i) = 0;
+                if (j == my - 1 && b[3] == 2) VY(ucont, k, j - 1, i) = 0;
+                if (j == 0 && b[2] == 12) VY(ucont, k, j, i) = 0;
+                if (j == my - 1 && b[3] == 12) VY(ucont, k, j - 1, i) = 0;
+                if (b[0] == 10 && i == 0 && (j != 0 && j != my - 1 && k != 0 && k != mz - 1)) VX(ucont, k, j, i) = 0;
+                if (b[1] == 10 && i == mx - 1 && (j != 0 && j != my - 1 && k != 0 && k != mz - 1)) VX(ucont, k, j, i - 1) = 0;
+                if (b[2] == 10 && j == 0 && (i != 0 && i != mx - 1 && k != 0 && k != mz - 1)) VY(ucont, k, j, i) = 0;
+                if (abs(b[3]) == 10 && j == my - 1 && (i != 0 && i != mx - 1 && k != 0 && k != mz - 1)) VY(ucont, k, j - 1, i) = 0;
+            }
+    orc_contra2cart(p, m, ucont, ucat, ucat_old);
+    const double dt = p->dt;
+    for (size_t q = 0; q < n3; q++) {
+        r[q] = 0.0 + (-1. / dt) * ucont[q];
+        r[q] = r[q] + (1. / dt) * ucont_o[q];
+    }
+    orc_formfunction2(p, m, ucont, ucat, nu_t, r, 0.5);
+    for (size_t q = 0; q < n3; q++) {
+        r[q] = r[q] + 0.5 * rhs_o[q];
+        r[q] = r[q] + (-1.0) * dp[q];
+        if (feul) r[q] = r[q] + feul[q];
+    }
+}

@@ -209,3 +209,31 @@ def test_dfuncs_partition_of_unity():
     r = np.linspace(-20, 20, 40001)
     v = np.array([lib.orc_dfunc_exp(x, 2.0) for x in r])
     assert abs(np.trapezoid(v, r) - 1.0) < 1e-7
+
+
+def test_snes_residual_time_term_and_consistency():
+    """FormFunction_SNES: at u = Ucont_o with RHS_o = F(Ucont_o) the residual is
+    F(Ucont_o) - dP (the two halves add up); a perturbation d changes it by
+    -d/dt to leading order (sign and scale of the time term)."""
+    from helpers import channel_pair
+    c, o, _ = channel_pair(12, 8, 10, with_gpu=False)
+    o.eddy_viscosity()
+    o.pressure_gradient()
+    uo = o["ucont"].copy()
+    o.set("rhs", np.zeros(o.shape + (3,)))
+    o.formfunction2(1.0)
+    rhso = o["rhs"].copy()
+    r = o.formfunction_snes(uo, uo, rhso)
+    want = rhso - o["dp"]
+    inner = np.zeros(o.shape, dtype=bool)
+    inner[1:-1, 1:-1, 1:-1] = True
+    assert np.allclose(r[inner], want[inner], rtol=1e-12, atol=1e-12 * np.abs(want).max())
+    o.p.dt = dt = 1e-6 * o.p.dt  # the time term dominates: R(u+d) - R(u) = -d/dt + O(|d|)
+    r = o.formfunction_snes(uo, uo, rhso)
+    rng = np.random.default_rng(2)
+    d = np.zeros_like(uo)
+    d[2:-2, 2:-3, 2:-2] = 1e-7 * rng.standard_normal(d[2:-2, 2:-3, 2:-2].shape)
+    r2 = o.formfunction_snes(uo + d, uo, rhso)
+    lin = (r2 - r) * (-dt)
+    sel = np.abs(d) > 0
+    assert np.allclose(lin[sel], d[sel], rtol=0, atol=1e-3 * np.abs(d).max())

@@ -25,11 +25,12 @@ VFS_OK, VFS_EINVAL, VFS_EHIP, VFS_ENOMEM, VFS_ENOCONV = 0, 1, 2, 3, 4
 FIELDS = [
     "COOR", "NVERT", "CSI", "ETA", "ZET", "AJ", "ICSI", "IETA", "IZET", "IAJ",
     "JCSI", "JETA", "JZET", "JAJ", "KCSI", "KETA", "KZET", "KAJ", "CENT",
-    "UCONT", "UCAT", "UCAT_OLD", "NU_T", "P", "PHI", "DP", "RHS", "B", "FEUL",
+    "UCONT", "UCAT", "UCAT_OLD", "NU_T", "P", "PHI", "DP", "RHS", "B", "FEUL", "UCONT_O", "RHS_O",
 ]
 FIELD = {name: i for i, name in enumerate(FIELDS)}
 VECTOR_FIELDS = {"COOR", "CSI", "ETA", "ZET", "ICSI", "IETA", "IZET", "JCSI", "JETA", "JZET",
-                 "KCSI", "KETA", "KZET", "CENT", "UCONT", "UCAT", "UCAT_OLD", "DP", "RHS", "FEUL"}
+                 "KCSI", "KETA", "KZET", "CENT", "UCONT", "UCAT", "UCAT_OLD", "DP", "RHS", "FEUL",
+                 "UCONT_O", "RHS_O"}
 METRIC_FIELDS = ["CSI", "ETA", "ZET", "AJ", "ICSI", "IETA", "IZET", "IAJ", "JCSI", "JETA", "JZET",
                  "JAJ", "KCSI", "KETA", "KZET", "KAJ", "CENT"]
 
@@ -64,7 +65,18 @@ class StepStats(C.Structure):
     _fields_ = [
         ("poisson_iterations", C.c_int), ("poisson_relres", C.c_double), ("maxdiv", C.c_double),
         ("ms_momentum", C.c_double), ("ms_poisson", C.c_double), ("ms_total", C.c_double),
+        ("newton_iterations", C.c_int), ("linear_iterations", C.c_int),
     ]
+
+
+class NewtonOpts(C.Structure):
+    _fields_ = [("rtol", C.c_double), ("restart", C.c_int), ("max_newton", C.c_int), ("max_linear", C.c_int),
+                ("use_feul", C.c_int)]
+
+
+class NewtonStats(C.Structure):
+    _fields_ = [("newton_iterations", C.c_int), ("linear_iterations", C.c_int), ("function_evals", C.c_int),
+                ("fnorm0", C.c_double), ("fnorm", C.c_double), ("ms", C.c_double)]
 
 
 class Turbine(C.Structure):
@@ -139,6 +151,10 @@ _SIGS = {
     "vfs_alm_get": ([C.c_void_p, C.c_int, C.c_void_p], C.c_int),
     "vfs_alm_set_disk": ([C.c_void_p, C.c_int, C.c_void_p, C.c_void_p], C.c_int),
     "vfs_alm_uref": ([C.c_void_p, C.POINTER(C.c_double)], C.c_int),
+    "vfs_formfunction_snes": ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_int], C.c_int),
+    "vfs_store_old_state": ([C.c_void_p], C.c_int),
+    "vfs_implicit_matrixfree": ([C.c_void_p, C.POINTER(NewtonOpts), C.POINTER(NewtonStats)], C.c_int),
+    "vfs_fractional_step_cn": ([C.c_void_p, C.POINTER(NewtonOpts), C.POINTER(SolveOpts), C.POINTER(StepStats)], C.c_int),
     "vfs_profile": ([C.c_void_p, C.c_int], C.c_int),
     "vfs_profile_read": ([C.c_void_p, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_longlong),
                           C.POINTER(C.c_char_p)], C.c_int),
@@ -178,6 +194,12 @@ def load_library(path: str | os.PathLike | None = None) -> C.CDLL:
 
 def exported_symbols() -> list[str]:
     return list(_SIGS)
+
+
+def default_newton_opts(rtol=1e-4, restart=30, max_newton=50, max_linear=1000, use_feul=0) -> NewtonOpts:
+    o = NewtonOpts()
+    o.rtol, o.restart, o.max_newton, o.max_linear, o.use_feul = rtol, restart, max_newton, max_linear, use_feul
+    return o
 
 
 def make_params(mx, my, mz, *, per=(1, 0, 1), bctype=(100, 100, 1, 1, 100, 100), les=1, second_order=1,
@@ -333,6 +355,26 @@ class Context:
     def fractional_step(self, opts: SolveOpts | None = None) -> StepStats:
         st = StepStats()
         self._check(self.lib.vfs_fractional_step(self.h, C.byref(opts or default_solve_opts()), C.byref(st)))
+        return st
+
+    def fractional_step_cn(self, nopts: NewtonOpts | None = None, opts: SolveOpts | None = None) -> StepStats:
+        st = StepStats()
+        self._check(self.lib.vfs_fractional_step_cn(self.h, C.byref(nopts or default_newton_opts()),
+                                                    C.byref(opts or default_solve_opts()), C.byref(st)))
+        return st
+
+    # -- implicit momentum (-imp 4) ---------------------------------------------
+    def store_old_state(self):
+        self._check(self.lib.vfs_store_old_state(self.h))
+
+    def formfunction_snes(self, u_dev: int, r_dev: int, use_feul: int = 0):
+        self._check(self.lib.vfs_formfunction_snes(self.h, u_dev, r_dev, int(use_feul)))
+
+    def implicit_matrixfree(self, nopts: NewtonOpts | None = None, check: bool = True) -> NewtonStats:
+        st = NewtonStats()
+        rc = self.lib.vfs_implicit_matrixfree(self.h, C.byref(nopts or default_newton_opts()), C.byref(st))
+        if check:
+            self._check(rc)
         return st
 
 

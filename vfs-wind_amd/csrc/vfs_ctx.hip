@@ -23,7 +23,7 @@ bool is_vector_field(int f)
     case VFS_JCSI: case VFS_JETA: case VFS_JZET:
     case VFS_KCSI: case VFS_KETA: case VFS_KZET:
     case VFS_CENT: case VFS_UCONT: case VFS_UCAT: case VFS_UCAT_OLD:
-    case VFS_DP: case VFS_RHS: case VFS_FEUL:
+    case VFS_DP: case VFS_RHS: case VFS_FEUL: case VFS_UCONT_O: case VFS_RHS_O:
         return true;
     default:
         return false;
@@ -384,16 +384,24 @@ int vfs_profile_read(vfs_ctx *c, int kernel, double *total_ms, long long *count,
     return VFS_OK;
 }
 
-int vfs_fractional_step(vfs_ctx *c, const vfs_solve_opts *o, vfs_step_stats *st)
+static int step_common(vfs_ctx *c, const vfs_solve_opts *o, const vfs_newton_opts *no, vfs_step_stats *st)
 {
-    if (!c || !o) return vfs::set_error(VFS_EINVAL, "null argument");
     using clk = std::chrono::steady_clock;
     auto t0 = clk::now();
     int rc = vfs_calc_k_flux(c, nullptr);                   // solvers.c:363
     if (!rc && c->p.les) rc = vfs::c2c(c);                  // solvers.c:368
     if (!rc && c->p.les) rc = vfs::eddy_viscosity(c);       // solvers.c:370
     if (!rc) rc = vfs_pressure_gradient(c);                 // solvers.c:438
-    if (!rc) rc = vfs_runge_kutta(c);                       // solvers.c:635
+    vfs_newton_stats ns;
+    std::memset(&ns, 0, sizeof(ns));
+    if (!rc) {
+        if (no) {                                           // solvers.c:626-630, -imp 4
+            rc = vfs_store_old_state(c);
+            if (!rc) rc = vfs_implicit_matrixfree(c, no, &ns);
+        } else {
+            rc = vfs_runge_kutta(c);                        // solvers.c:635
+        }
+    }
     if (rc) return rc;
     VFS_HIP(hipStreamSynchronize(c->stream));
     auto t1 = clk::now();
@@ -417,8 +425,22 @@ int vfs_fractional_step(vfs_ctx *c, const vfs_solve_opts *o, vfs_step_stats *st)
         st->ms_momentum = std::chrono::duration<double, std::milli>(t1 - t0).count();
         st->ms_poisson = std::chrono::duration<double, std::milli>(t2 - t1).count();
         st->ms_total = std::chrono::duration<double, std::milli>(t3 - t0).count();
+        st->newton_iterations = ns.newton_iterations;
+        st->linear_iterations = ns.linear_iterations;
     }
     return conv;
+}
+
+int vfs_fractional_step(vfs_ctx *c, const vfs_solve_opts *o, vfs_step_stats *st)
+{
+    if (!c || !o) return vfs::set_error(VFS_EINVAL, "null argument");
+    return step_common(c, o, nullptr, st);
+}
+
+int vfs_fractional_step_cn(vfs_ctx *c, const vfs_newton_opts *no, const vfs_solve_opts *o, vfs_step_stats *st)
+{
+    if (!c || !o || !no) return vfs::set_error(VFS_EINVAL, "null argument");
+    return step_common(c, o, no, st);
 }
 
 }  // extern "C"
