@@ -250,6 +250,35 @@ int vfs_alm_get(vfs_alm *alm, vfs_alm_array what, double *host);
 int vfs_alm_set_disk(vfs_alm *alm, int n, const double *cent, const double *dA);
 int vfs_alm_uref(vfs_alm *alm, double *uref);
 
+/* ---- learned Poisson stage (no counterpart in the reference) ------------
+ * A 3D convolutional U-Net (BF16 on the matrix cores, FP32 accumulation and
+ * master weights) on the interior cells of the context's grid, trained on
+ * (Poisson right-hand side, solution) pairs the solver produces.  It gives
+ * vfs_poisson_solve_learned its initial guess and one preconditioner stage. */
+typedef struct vfs_unet vfs_unet;
+/* base channels (16..64, multiple of 16), levels 2..5; interior cell counts
+ * divisible by 2^(levels-1). */
+int vfs_unet_create(vfs_ctx *ctx, int base_channels, int levels, unsigned long long seed, vfs_unet **out);
+int vfs_unet_destroy(vfs_unet *net);
+/* phi ~ out_scale * rms(b) * net(b / rms(b)); out_scale from the training data. */
+int vfs_unet_set_output_scale(vfs_unet *net, double out_scale);
+/* y = net(b) on DA-layout device arrays (ghost points of y set to 0). */
+int vfs_unet_forward(vfs_unet *net, const double *b_dev, double *y_dev);
+/* forward + mean-squared loss against phi + backward; gradients kept. */
+int vfs_unet_gradients(vfs_unet *net, const double *b_dev, const double *phi_dev, double *loss);
+/* vfs_unet_gradients + one Adam step with learning rate lr. */
+int vfs_unet_train_step(vfs_unet *net, const double *b_dev, const double *phi_dev, double lr, double *loss);
+long long vfs_unet_param_count(vfs_unet *net);
+/* which = 0: FP32 master weights, 1: last gradients; flattened layer by layer. */
+int vfs_unet_get_params(vfs_unet *net, int which, float *host);
+int vfs_unet_set_params(vfs_unet *net, const float *host);
+/* vfs_poisson_solve with the network: x0 = guess + alpha * net(b - A guess)
+ * (alpha by least squares on the residual), the first FGMRES direction
+ * preconditioned by the network, the rest by the V-cycle; when after three
+ * iterations the residual has not dropped below half of its starting value
+ * the solve restarts from the guess with the plain V-cycle (stats->fallback = 1). */
+int vfs_poisson_solve_learned(vfs_ctx *ctx, vfs_unet *net, const vfs_solve_opts *opts, vfs_solve_stats *stats);
+
 /* ---- measurement (no counterpart in the reference) -------------------- */
 /* Enable (1) or disable (0) HIP-event timing of the hot kernels' launches on
  * the context stream and clear earlier records.  For benchmarks only. */

@@ -155,6 +155,16 @@ _SIGS = {
     "vfs_store_old_state": ([C.c_void_p], C.c_int),
     "vfs_implicit_matrixfree": ([C.c_void_p, C.POINTER(NewtonOpts), C.POINTER(NewtonStats)], C.c_int),
     "vfs_fractional_step_cn": ([C.c_void_p, C.POINTER(NewtonOpts), C.POINTER(SolveOpts), C.POINTER(StepStats)], C.c_int),
+    "vfs_unet_create": ([C.c_void_p, C.c_int, C.c_int, C.c_ulonglong, C.POINTER(C.c_void_p)], C.c_int),
+    "vfs_unet_destroy": ([C.c_void_p], C.c_int),
+    "vfs_unet_set_output_scale": ([C.c_void_p, C.c_double], C.c_int),
+    "vfs_unet_forward": ([C.c_void_p, C.c_void_p, C.c_void_p], C.c_int),
+    "vfs_unet_gradients": ([C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(C.c_double)], C.c_int),
+    "vfs_unet_train_step": ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_double, C.POINTER(C.c_double)], C.c_int),
+    "vfs_unet_param_count": ([C.c_void_p], C.c_longlong),
+    "vfs_unet_get_params": ([C.c_void_p, C.c_int, C.c_void_p], C.c_int),
+    "vfs_unet_set_params": ([C.c_void_p, C.c_void_p], C.c_int),
+    "vfs_poisson_solve_learned": ([C.c_void_p, C.c_void_p, C.POINTER(SolveOpts), C.POINTER(SolveStats)], C.c_int),
     "vfs_profile": ([C.c_void_p, C.c_int], C.c_int),
     "vfs_profile_read": ([C.c_void_p, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_longlong),
                           C.POINTER(C.c_char_p)], C.c_int),
@@ -448,3 +458,60 @@ class ALM:
         v = C.c_double()
         self.ctx._check(self.lib.vfs_alm_uref(self.h, C.byref(v)))
         return v.value
+
+
+class UNet:
+    """The learned Poisson stage on a Context's grid (BF16 U-Net on the matrix cores)."""
+
+    def __init__(self, ctx: Context, base: int = 16, levels: int = 4, seed: int = 20261202):
+        self.ctx, self.lib = ctx, ctx.lib
+        h = C.c_void_p()
+        ctx._check(self.lib.vfs_unet_create(ctx.h, base, levels, seed, C.byref(h)))
+        self.h = h
+        self.base, self.levels = base, levels
+        self.n_params = int(self.lib.vfs_unet_param_count(h))
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.vfs_unet_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_output_scale(self, s: float):
+        self.ctx._check(self.lib.vfs_unet_set_output_scale(self.h, float(s)))
+
+    def forward(self, b_dev: int, y_dev: int):
+        self.ctx._check(self.lib.vfs_unet_forward(self.h, b_dev, y_dev))
+
+    def gradients(self, b_dev: int, phi_dev: int) -> float:
+        v = C.c_double()
+        self.ctx._check(self.lib.vfs_unet_gradients(self.h, b_dev, phi_dev, C.byref(v)))
+        return v.value
+
+    def train_step(self, b_dev: int, phi_dev: int, lr: float) -> float:
+        v = C.c_double()
+        self.ctx._check(self.lib.vfs_unet_train_step(self.h, b_dev, phi_dev, float(lr), C.byref(v)))
+        return v.value
+
+    def params(self, which: int = 0) -> np.ndarray:
+        a = np.empty(self.n_params, dtype=np.float32)
+        self.ctx._check(self.lib.vfs_unet_get_params(self.h, int(which), a.ctypes.data))
+        return a
+
+    def set_params(self, a: np.ndarray):
+        a = np.ascontiguousarray(a, dtype=np.float32)
+        if a.size != self.n_params:
+            raise ValueError(f"expected {self.n_params} parameters, got {a.size}")
+        self.ctx._check(self.lib.vfs_unet_set_params(self.h, a.ctypes.data))
+
+    def poisson_solve(self, opts: SolveOpts | None = None, check: bool = True) -> SolveStats:
+        st = SolveStats()
+        rc = self.lib.vfs_poisson_solve_learned(self.ctx.h, self.h, C.byref(opts or default_solve_opts()), C.byref(st))
+        if check:
+            self.ctx._check(rc)
+        return st

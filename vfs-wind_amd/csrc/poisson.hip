@@ -1045,7 +1045,10 @@ int poisson_apply(vfs_ctx *c, const double *x, double *y)
     return VFS_OK;
 }
 
-int poisson_solve(vfs_ctx *c, const vfs_solve_opts *o, vfs_solve_stats *stats)
+// FGMRES; with a network: warm start and first preconditioner stage from the
+// network, and *fell_back set (solve abandoned) when after three iterations
+// the residual is above half of its value after the warm start
+static int fgmres(vfs_ctx *c, const vfs_solve_opts *o, vfs_unet *net, vfs_solve_stats *stats, bool *fell_back)
 {
     using clk = std::chrono::steady_clock;
     auto t0 = clk::now();
@@ -1062,6 +1065,24 @@ int poisson_solve(vfs_ctx *c, const vfs_solve_opts *o, vfs_solve_stats *stats)
     double *V = ps->V, *Z = ps->Z, *w = ps->w;
     if (!o->use_guess) hipLaunchKernelGGL(k_zero<double>, dim3(nb), dim3(PTPB), 0, st, n, x);
     double d[MAXV + 1];
+    if (net) {
+        // x += alpha y, y = net(b - A x), alpha = <r, A y> / <A y, A y>
+        hipLaunchKernelGGL(k_apply<double>, dim3(nb), dim3(PTPB), 0, st, L.g, L.c64, L.row, x, b, w);
+        rc = vfs_unet_forward(net, w, Z);
+        if (rc) return rc;
+        hipLaunchKernelGGL(k_apply<double>, dim3(nb), dim3(PTPB), 0, st, L.g, L.c64, L.row, (const double *)Z,
+                           (const double *)nullptr, V + n);
+        rc = multidot(c, 1, V + n, w, d);
+        if (rc) return rc;
+        const double rAy = d[0];
+        rc = multidot(c, 0, V, V + n, d);
+        if (rc) return rc;
+        c->red_host[0] = d[0] > 0 ? rAy / d[0] : 0.0;
+        VFS_HIP(hipMemcpyAsync(ps->dev_scalars, c->red_host, sizeof(double), hipMemcpyHostToDevice, st));
+        hipLaunchKernelGGL(k_multiupdate, dim3(nb), dim3(PTPB), 0, st, n, 1, ps->dev_scalars, Z, n, x);
+        VFS_LAUNCH_CHECK("learned warm start");
+    }
+    double rel_start = -1;
     // ||b||: multidot with no basis vectors returns <b,b>
     rc = multidot(c, 0, V, b, d);
     if (rc) return rc;
@@ -1082,6 +1103,7 @@ int poisson_solve(vfs_ctx *c, const vfs_solve_opts *o, vfs_solve_stats *stats)
         if (rc) return rc;
         double beta = std::sqrt(d[0]);
         rel = beta / bnorm;
+        if (rel_start < 0) rel_start = rel;
         if (trace) std::fprintf(stderr, "fgmres restart at it %d: true relres %.3e (bnorm %.3e)\n", it, rel, bnorm);
         if (rel <= tol) { converged = true; break; }
         c->red_host[0] = 1.0 / beta;
@@ -1092,7 +1114,7 @@ int poisson_solve(vfs_ctx *c, const vfs_solve_opts *o, vfs_solve_stats *stats)
         int jj;
         for (jj = 0; jj < m && it < maxit; jj++, it++) {
             double *vj = V + (size_t)jj * n, *zj = Z + (size_t)jj * n;
-            rc = precondition(c, o, vj, zj);
+            rc = net && it == 0 ? vfs_unet_forward(net, vj, zj) : precondition(c, o, vj, zj);
             if (rc) return rc;
             prof_begin(c, K_APPLY64);
             hipLaunchKernelGGL(k_apply<double>, dim3(nb), dim3(PTPB), 0, st, L.g, L.c64, L.row, zj, (const double *)nullptr, w);
@@ -1138,6 +1160,10 @@ int poisson_solve(vfs_ctx *c, const vfs_solve_opts *o, vfs_solve_stats *stats)
             if (trace) std::fprintf(stderr, "  it %d: est relres %.3e  h_{j+1,j} %.3e  ww0 %.3e ww %.3e\n", it,
                                     std::fabs(gv[jj + 1]) / bnorm, hn, ww0, ww);
             if (std::fabs(gv[jj + 1]) / bnorm <= tol) { jj++; it++; break; }
+            if (net && it == 2 && std::fabs(gv[jj + 1]) / bnorm > 0.5 * rel_start) {
+                *fell_back = true;
+                return VFS_OK;
+            }
         }
         for (int i = jj - 1; i >= 0; i--) {
             double s = gv[i];
@@ -1167,6 +1193,48 @@ int poisson_solve(vfs_ctx *c, const vfs_solve_opts *o, vfs_solve_stats *stats)
     }
     if (!converged) return set_error(VFS_ENOCONV, "Poisson FGMRES did not reach the tolerance");
     return VFS_OK;
+}
+
+int poisson_solve(vfs_ctx *c, const vfs_solve_opts *o, vfs_solve_stats *stats)
+{
+    bool fb = false;
+    return fgmres(c, o, nullptr, stats, &fb);
+}
+
+int poisson_solve_learned(vfs_ctx *c, vfs_unet *net, const vfs_solve_opts *o, vfs_solve_stats *stats)
+{
+    using clk = std::chrono::steady_clock;
+    auto t0 = clk::now();
+    const int m = o->restart > 0 ? (o->restart < MAXV - 1 ? o->restart : MAXV - 2) : 20;
+    if (m < 4) return set_error(VFS_EINVAL, "learned Poisson solve: restart must be at least 4");
+    int rc = poisson_setup(c, o->levels > 0 ? o->levels : 4);
+    if (!rc) rc = ensure_krylov(c, m);
+    if (rc) return rc;
+    const long long n = c->g.n;
+    // keep the initial guess for a fallback (in the last Krylov direction slot)
+    double *saved = c->ps->Z + (size_t)(m - 1) * n;
+    VFS_HIP(hipMemcpyAsync(saved, c->f[VFS_PHI], n * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    bool fb = false;
+    vfs_solve_stats s1;
+    std::memset(&s1, 0, sizeof(s1));
+    rc = fgmres(c, o, net, &s1, &fb);
+    if (rc && rc != VFS_ENOCONV) return rc;
+    if (!fb) {
+        if (stats) { *stats = s1; stats->ms = std::chrono::duration<double, std::milli>(clk::now() - t0).count(); }
+        return rc;
+    }
+    VFS_HIP(hipMemcpyAsync(c->f[VFS_PHI], saved, n * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    vfs_solve_opts o2 = *o;
+    vfs_solve_stats s2;
+    std::memset(&s2, 0, sizeof(s2));
+    rc = fgmres(c, &o2, nullptr, &s2, &fb);
+    if (stats) {
+        *stats = s2;
+        stats->iterations += 3;
+        stats->fallback = 1;
+        stats->ms = std::chrono::duration<double, std::milli>(clk::now() - t0).count();
+    }
+    return rc;
 }
 
 }  // namespace vfs

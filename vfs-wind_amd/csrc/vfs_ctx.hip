@@ -334,6 +334,12 @@ int vfs_poisson_solve(vfs_ctx *c, const vfs_solve_opts *o, vfs_solve_stats *st)
     return vfs::poisson_solve(c, o, st);
 }
 
+int vfs_poisson_solve_learned(vfs_ctx *c, vfs_unet *net, const vfs_solve_opts *o, vfs_solve_stats *st)
+{
+    if (!c || !net || !o) return vfs::set_error(VFS_EINVAL, "null argument");
+    return vfs::poisson_solve_learned(c, net, o, st);
+}
+
 int vfs_update_pressure(vfs_ctx *c)
 {
     if (!c) return vfs::set_error(VFS_EINVAL, "null context");

@@ -140,6 +140,7 @@ int min_dt(vfs_ctx *c, double *out);
 int poisson_rhs(vfs_ctx *c);
 int poisson_apply(vfs_ctx *c, const double *x, double *y);
 int poisson_solve(vfs_ctx *c, const vfs_solve_opts *o, vfs_solve_stats *st);
+int poisson_solve_learned(vfs_ctx *c, vfs_unet *net, const vfs_solve_opts *o, vfs_solve_stats *st);
 void poisson_free(vfs_ctx *c);
 int update_pressure(vfs_ctx *c);
 int projection(vfs_ctx *c);
