@@ -20,6 +20,8 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
@@ -80,6 +82,48 @@ def cpu_baseline(nx=64, ny=48, nz=64, steps=3):
                       f"{dt:.1f} s"}
 
 
+def poisson_comparison(vfs, ctx, reps=3):
+    """ms per Poisson solve on the same right-hand side: plain FP64 V-cycle,
+    FP32 V-cycle, FP32 V-cycle + learned warm start / first stage.  The
+    right-hand side is that of one more step of the run (momentum stage, then
+    PoissonRHS); the previous step's correction is the initial guess, as in
+    the time loop, and each variant is also run from a zero guess."""
+    learned = importlib.import_module("vfs-wind_amd.learned")
+    ctx.calc_k_flux()
+    if ctx.params.les:
+        ctx.contra2cart()
+        ctx.eddy_viscosity()
+    ctx.pressure_gradient()
+    ctx.runge_kutta()
+    ctx.poisson_rhs()
+    guess = ctx.download("PHI")
+    zero = np.zeros_like(guess)
+    net = learned.make_net(ctx)
+    variants = [("fp64_mg", 0, None), ("fp32_mg", 1, None)]
+    if net is not None:
+        variants.append(("fp32_mg_net", 1, net))
+    out = {}
+    for start, g in (("previous_guess", guess), ("zero_guess", zero)):
+        res = {}
+        for name, fp32, nn in variants:
+            ms, its, fb, rel = [], [], [], []
+            for _ in range(reps):
+                ctx.upload("PHI", g)
+                ctx.sync()
+                opts = vfs.default_solve_opts(tol=1e-8, mg_fp32=fp32, use_guess=1)
+                st = nn.poisson_solve(opts) if nn is not None else ctx.poisson_solve(opts)
+                ms.append(st.ms); its.append(st.iterations); fb.append(st.fallback); rel.append(st.relres)
+            res[name] = {"ms": round(float(np.median(ms)), 3), "iterations": int(np.median(its)),
+                         "fallback": int(max(fb)), "relres": float(max(rel))}
+        base = res["fp64_mg"]["ms"]
+        for name in res:
+            res[name]["speedup_vs_fp64_mg"] = round(base / res[name]["ms"], 3)
+        out[start] = res
+    if net is not None:
+        net.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -90,6 +134,7 @@ def main():
     ap.add_argument("--nz", type=int, default=256)
     ap.add_argument("--mg-fp64", action="store_true", help="plain FP64 V-cycle instead of FP32")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-poisson-comparison", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for {args.gpus})", file=sys.stderr)
@@ -154,6 +199,11 @@ def main():
                     "bytes_per_unit": bpu, "unit_of_work": unit, "units_per_launch": units[unit],
                     "avg_launch_ms": round(kern[dom]["avg_ms"], 4)}
     log("kernels: " + json.dumps({k: {kk: round(vv, 3) for kk, vv in v.items()} for k, v in kern.items()}))
+    pois = None
+    if not args.no_poisson_comparison:
+        log("Poisson solve comparison (same right-hand side)")
+        pois = poisson_comparison(vfs, ctx)
+        log("poisson: " + json.dumps(pois))
     ctx.close()
 
     cpu = None
@@ -184,6 +234,7 @@ def main():
         },
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "poisson_ms_per_solve": pois,
     }
     print(json.dumps(out), flush=True)
 
