@@ -98,6 +98,9 @@ int vfs_mean_k_area(vfs_ctx *ctx, double *area);
 int vfs_contra2cart(vfs_ctx *ctx);
 /* FormBCS (bcs.c:1684-2767) for codes 1, 10, 4 and periodic directions. */
 int vfs_form_bcs(vfs_ctx *ctx, int outflow_scale);
+/* InflowFlux (bcs.c:247-626) for a uniform inlet (-inlet 1, code 5 on the k- face):
+ * inlet fluxes and ghost velocities; FluxInSum returned and kept for FormBCS. */
+int vfs_inflow_flux(vfs_ctx *ctx, double *flux_in);
 /* Compute_eddy_viscosity_LES (les.c:1143-1361) with lCs = params.cs2. */
 int vfs_eddy_viscosity(vfs_ctx *ctx);
 /* Calc_k_Flux (rhs2.c:1549-1633): mean_k_flux, stored in the context. */

@@ -91,6 +91,8 @@ def load() -> C.CDLL:
         "orc_dfunc_s3h": ([C.c_double], C.c_double),
         "orc_dfunc_s4h": ([C.c_double], C.c_double),
         "orc_dfunc_exp": ([C.c_double, C.c_double], C.c_double),
+        "orc_inlet_area": ([P, M], C.c_double),
+        "orc_inflow_flux": ([P, M, D, D], C.c_double),
         "orc_formfunction_snes": ([P, M, D, D, D, D, D, D, D, D, D, D], None),
         "orc_arbitrary_rotate": ([D, C.c_double, D, D], None),
         "orc_alm_elements": ([C.c_int, C.c_int, D, D, D], None),
@@ -193,6 +195,9 @@ class Oracle:
         self.lib.orc_runge_kutta(C.byref(self.p), C.byref(self.cm), self._f("ucont"), self._f("ucat"),
                                  self._f("ucat_old"), self._f("nu_t"), self._f("dp"),
                                  C.cast(None, C.POINTER(C.c_double)))
+
+    def inflow_flux(self):
+        return self.lib.orc_inflow_flux(C.byref(self.p), C.byref(self.cm), self._f("ucont"), self._f("ucat"))
 
     def min_dt(self):
         return self.lib.orc_min_dt(C.byref(self.p), C.byref(self.cm), self._f("ucont"))

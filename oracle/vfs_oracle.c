@@ -84,49 +84,12 @@ void orc_form_metrics(const orc_params *p, const double *coor, orc_metrics *m)
                     VC(m->cent, c, k, j, i) = 0.125 * s;
                 }
 
-    /* csi on node planes i, metrics.c:116-137 */
-    for (int k = 1; k < mz - 1; k++)
-        for (int j = 1; j < my - 1; j++)
-            for (int i = 0; i < mx - 1; i++) {
-                double dxde = 0.5 * (CX(k, j, i) + CX(k - 1, j, i) - CX(k, j - 1, i) - CX(k - 1, j - 1, i));
-                double dyde = 0.5 * (CY(k, j, i) + CY(k - 1, j, i) - CY(k, j - 1, i) - CY(k - 1, j - 1, i));
-                double dzde = 0.5 * (CZ(k, j, i) + CZ(k - 1, j, i) - CZ(k, j - 1, i) - CZ(k - 1, j - 1, i));
-                double dxdz = 0.5 * (CX(k, j - 1, i) + CX(k, j, i) - CX(k - 1, j - 1, i) - CX(k - 1, j, i));
-                double dydz = 0.5 * (CY(k, j - 1, i) + CY(k, j, i) - CY(k - 1, j - 1, i) - CY(k - 1, j, i));
-                double dzdz = 0.5 * (CZ(k, j - 1, i) + CZ(k, j, i) - CZ(k - 1, j - 1, i) - CZ(k - 1, j, i));
-                VX(csi, k, j, i) = dyde * dzdz - dzde * dydz;
-                VY(csi, k, j, i) = -dxde * dzdz + dzde * dxdz;
-                VZ(csi, k, j, i) = dxde * dydz - dyde * dxdz;
-            }
-    /* eta on node planes j, metrics.c:139-161 */
-    for (int k = 1; k < mz - 1; k++)
-        for (int j = 0; j < my - 1; j++)
-            for (int i = 1; i < mx - 1; i++) {
-                double dxdc = 0.5 * (CX(k, j, i) + CX(k - 1, j, i) - CX(k, j, i - 1) - CX(k - 1, j, i - 1));
-                double dydc = 0.5 * (CY(k, j, i) + CY(k - 1, j, i) - CY(k, j, i - 1) - CY(k - 1, j, i - 1));
-                double dzdc = 0.5 * (CZ(k, j, i) + CZ(k - 1, j, i) - CZ(k, j, i - 1) - CZ(k - 1, j, i - 1));
-                double dxdz = 0.5 * (CX(k, j, i) + CX(k, j, i - 1) - CX(k - 1, j, i) - CX(k - 1, j, i - 1));
-                double dydz = 0.5 * (CY(k, j, i) + CY(k, j, i - 1) - CY(k - 1, j, i) - CY(k - 1, j, i - 1));
-                double dzdz = 0.5 * (CZ(k, j, i) + CZ(k, j, i - 1) - CZ(k - 1, j, i) - CZ(k - 1, j, i - 1));
-                VX(eta, k, j, i) = dydz * dzdc - dzdz * dydc;
-                VY(eta, k, j, i) = -dxdz * dzdc + dzdz * dxdc;
-                VZ(eta, k, j, i) = dxdz * dydc - dydz * dxdc;
-            }
-    /* zet on node planes k, metrics.c:163-185 */
-    for (int k = 0; k < mz - 1; k++)
-        for (int j = 1; j < my - 1; j++)
-            for (int i = 1; i < mx - 1; i++) {
-                double dxdc = 0.5 * (CX(k, j, i) + CX(k, j - 1, i) - CX(k, j, i - 1) - CX(k, j - 1, i - 1));
-                double dydc = 0.5 * (CY(k, j, i) + CY(k, j - 1, i) - CY(k, j, i - 1) - CY(k, j - 1, i - 1));
-                double dzdc = 0.5 * (CZ(k, j, i) + CZ(k, j - 1, i) - CZ(k, j, i - 1) - CZ(k, j - 1, i - 1));
-                double dxde = 0.5 * (CX(k, j, i) + CX(k, j, i - 1) - CX(k, j - 1, i) - CX(k, j - 1, i - 1));
-                double dyde = 0.5 * (CY(k, j, i) + CY(k, j, i - 1) - CY(k, j - 1, i) - CY(k, j - 1, i - 1));
-                double dzde = 0.5 * (CZ(k, j, i) + CZ(k, j, i - 1) - CZ(k, j - 1, i) - CZ(k, j - 1, i - 1));
-                VX(zet, k, j, i) = dydc * dzde - dzdc * dyde;
-                VY(zet, k, j, i) = -dxdc * dzde + dzdc * dxde;
-                VZ(zet, k, j, i) = dxdc * dyde - dydc * dxde;
-            }
-    /* Jacobian, metrics.c:193-241 */
+    /* The reference compiles FormMetrics with NEWMETRIC defined
+     * (metrics.c:10), so the node-plane formulas of metrics.c:115-191 are
+     * dead code: the cell metrics csi/eta/zet are the cross products of the
+     * cell-averaged edge vectors, computed beside the Jacobian
+     * (metrics.c:193-257), and each face metric is the mean of the two cell
+     * metrics it separates (metrics.c:588-593, 696-701, 804-809). */
     for (int k = 1; k < mz - 1; k++)
         for (int j = 1; j < my - 1; j++)
             for (int i = 1; i < mx - 1; i++) {
@@ -144,154 +107,55 @@ void orc_form_metrics(const orc_params *p, const double *coor, orc_metrics *m)
                                       CC(k - 1, j - 1, i - 1));
 #undef CC
                 }
-                double dxdc = d[0][0], dydc = d[1][0], dzdc = d[2][0];
-                double dxde = d[0][1], dyde = d[1][1], dzde = d[2][1];
-                double dxdz = d[0][2], dydz = d[1][2], dzdz = d[2][2];
-                double det = dxdc * (dyde * dzdz - dzde * dydz) - dydc * (dxde * dzdz - dzde * dxdz) +
-                             dzdc * (dxde * dydz - dyde * dxdz);
-                S(aj, k, j, i) = 1. / det;
+                size_t q = IX(p, k, j, i);
+                cross_metrics(d[0][0], d[1][0], d[2][0], d[0][1], d[1][1], d[2][1], d[0][2], d[1][2], d[2][2],
+                              &csi[3 * q], &eta[3 * q], &zet[3 * q], &aj[q]);
             }
-    /* ghost-cell mirroring, metrics.c:261-345, in the reference's order */
-    for (int k = 0; k < mz; k++)
-        for (int j = 0; j < my; j++) {
-            for (int c = 0; c < 3; c++) {
-                VC(eta, c, k, j, 0) = VC(eta, c, k, j, 1);
-                VC(zet, c, k, j, 0) = VC(zet, c, k, j, 1);
-            }
-            S(aj, k, j, 0) = S(aj, k, j, 1);
-        }
-    for (int k = 0; k < mz; k++)
-        for (int j = 0; j < my; j++) {
-            for (int c = 0; c < 3; c++) {
-                VC(eta, c, k, j, mx - 1) = VC(eta, c, k, j, mx - 2);
-                VC(zet, c, k, j, mx - 1) = VC(zet, c, k, j, mx - 2);
-            }
-            S(aj, k, j, mx - 1) = S(aj, k, j, mx - 2);
-        }
-    for (int k = 0; k < mz; k++)
-        for (int i = 0; i < mx; i++) {
-            for (int c = 0; c < 3; c++) {
-                VC(csi, c, k, 0, i) = VC(csi, c, k, 1, i);
-                VC(zet, c, k, 0, i) = VC(zet, c, k, 1, i);
-            }
-            S(aj, k, 0, i) = S(aj, k, 1, i);
-        }
-    for (int k = 0; k < mz; k++)
-        for (int i = 0; i < mx; i++) {
-            for (int c = 0; c < 3; c++) {
-                VC(csi, c, k, my - 1, i) = VC(csi, c, k, my - 2, i);
-                VC(zet, c, k, my - 1, i) = VC(zet, c, k, my - 2, i);
-            }
-            S(aj, k, my - 1, i) = S(aj, k, my - 2, i);
-        }
-    for (int j = 0; j < my; j++)
-        for (int i = 0; i < mx; i++) {
-            for (int c = 0; c < 3; c++) {
-                VC(eta, c, 0, j, i) = VC(eta, c, 1, j, i);
-                VC(csi, c, 0, j, i) = VC(csi, c, 1, j, i);
-            }
-            S(aj, 0, j, i) = S(aj, 1, j, i);
-        }
-    for (int j = 0; j < my; j++)
-        for (int i = 0; i < mx; i++) {
-            for (int c = 0; c < 3; c++) {
-                VC(eta, c, mz - 1, j, i) = VC(eta, c, mz - 2, j, i);
-                VC(csi, c, mz - 1, j, i) = VC(csi, c, mz - 2, j, i);
-            }
-            S(aj, mz - 1, j, i) = S(aj, mz - 2, j, i);
-        }
+    /* ghost-cell mirroring, metrics.c:261-345: each boundary plane copies all
+     * four cell arrays from its neighbour plane, in the order i-, i+, j-, j+,
+     * k-, k+ (edges and corners take what the later planes copy). */
+    double *cell[4] = {csi, eta, zet, aj};
+    const int ncomp[4] = {3, 3, 3, 1};
+    for (int side = 0; side < 6; side++) {
+        const int axis = side >> 1, hi = side & 1;
+        const int lim[3] = {mx, my, mz};
+        const int to = hi ? lim[axis] - 1 : 0, from = hi ? lim[axis] - 2 : 1;
+        for (int k = 0; k < mz; k++)
+            for (int j = 0; j < my; j++)
+                for (int i = 0; i < mx; i++) {
+                    int ijk[3] = {i, j, k};
+                    if (ijk[axis] != to) continue;
+                    ijk[axis] = from;
+                    size_t d = IX(p, k, j, i), s = IX(p, ijk[2], ijk[1], ijk[0]);
+                    for (int a = 0; a < 4; a++)
+                        for (int c = 0; c < ncomp[a]; c++) cell[a][ncomp[a] * d + c] = cell[a][ncomp[a] * s + c];
+                }
+    }
+    /* periodic copy into the ghosted cell arrays before the face averages
+     * read them, metrics.c:365-395 */
+    orc_periodic_fix(p, csi, 3);
+    orc_periodic_fix(p, eta, 3);
+    orc_periodic_fix(p, zet, 3);
+    orc_periodic_fix(p, aj, 1);
 
-    /* face metrics.  centx/centy/centz are face centroids of node planes
-     * (metrics.c:483-495, 612-624, 718-730). */
-    double *cf = dalloc(3 * n);
-    /* I faces, metrics.c:497-597 */
-    for (int k = 1; k < mz; k++)
-        for (int j = 1; j < my; j++)
-            for (int i = 0; i < mx - 1; i++)
-                for (int c = 0; c < 3; c++) {
-#define CC(kk, jj, ii) coor[3 * IX(p, (kk), (jj), (ii)) + c]
-                    VC(cf, c, k, j, i) = (CC(k, j, i) + CC(k - 1, j, i) + CC(k, j - 1, i) + CC(k - 1, j - 1, i)) * 0.25;
-#undef CC
+    /* face metrics: F = 0.5 X + 0.5 Y (AxByC, momentum.c:40-44) of the two
+     * cells, and the harmonic mean of their aj; I faces i = 0..mx-2 over the
+     * interior j, k (metrics.c:497, 588-593), J faces (626, 696-701), K faces
+     * (731, 804-809). */
+    for (int axis = 0; axis < 3; axis++) {
+        double *fv[3] = {face_v[3 * axis], face_v[3 * axis + 1], face_v[3 * axis + 2]};
+        double *fa = face_s[axis];
+        const size_t step = axis == 0 ? 1 : axis == 1 ? (size_t)mx : (size_t)mx * my;
+        for (int k = (axis == 2 ? 0 : 1); k < mz - 1; k++)
+            for (int j = (axis == 1 ? 0 : 1); j < my - 1; j++)
+                for (int i = (axis == 0 ? 0 : 1); i < mx - 1; i++) {
+                    size_t q = IX(p, k, j, i), r = q + step;
+                    for (int a = 0; a < 3; a++)
+                        for (int c = 0; c < 3; c++)
+                            fv[a][3 * q + c] = 0.5 * cell[a][3 * q + c] + 0.5 * cell[a][3 * r + c];
+                    fa[q] = 2. / (1. / aj[q] + 1. / aj[r]);
                 }
-    for (int k = 1; k < mz - 1; k++)
-        for (int j = 1; j < my - 1; j++)
-            for (int i = 0; i < mx - 1; i++) {
-                double dc[3], de[3], dz[3];
-                for (int c = 0; c < 3; c++) {
-                    if (i == 0) dc[c] = VC(cf, c, k, j, i + 1) - VC(cf, c, k, j, i);
-                    else if (i == mx - 2) dc[c] = VC(cf, c, k, j, i) - VC(cf, c, k, j, i - 1);
-                    else dc[c] = (VC(cf, c, k, j, i + 1) - VC(cf, c, k, j, i - 1)) * 0.5;
-                    if (j == 1) de[c] = VC(cf, c, k, j + 1, i) - VC(cf, c, k, j, i);
-                    else if (j == my - 2) de[c] = VC(cf, c, k, j, i) - VC(cf, c, k, j - 1, i);
-                    else de[c] = (VC(cf, c, k, j + 1, i) - VC(cf, c, k, j - 1, i)) * 0.5;
-                    if (k == 1) dz[c] = (VC(cf, c, k + 1, j, i) - VC(cf, c, k, j, i));
-                    else if (k == mz - 2) dz[c] = (VC(cf, c, k, j, i) - VC(cf, c, k - 1, j, i));
-                    else dz[c] = (VC(cf, c, k + 1, j, i) - VC(cf, c, k - 1, j, i)) * 0.5;
-                }
-                size_t q = IX(p, k, j, i);
-                cross_metrics(dc[0], dc[1], dc[2], de[0], de[1], de[2], dz[0], dz[1], dz[2],
-                              &m->icsi[3 * q], &m->ieta[3 * q], &m->izet[3 * q], &m->iaj[q]);
-            }
-    /* J faces, metrics.c:606-709 */
-    memset(cf, 0, 3 * n * sizeof(double));
-    for (int k = 1; k < mz; k++)
-        for (int j = 0; j < my - 1; j++)
-            for (int i = 1; i < mx; i++)
-                for (int c = 0; c < 3; c++) {
-#define CC(kk, jj, ii) coor[3 * IX(p, (kk), (jj), (ii)) + c]
-                    VC(cf, c, k, j, i) = (CC(k, j, i) + CC(k - 1, j, i) + CC(k, j, i - 1) + CC(k - 1, j, i - 1)) * 0.25;
-#undef CC
-                }
-    for (int k = 1; k < mz - 1; k++)
-        for (int j = 0; j < my - 1; j++)
-            for (int i = 1; i < mx - 1; i++) {
-                double dc[3], de[3], dz[3];
-                for (int c = 0; c < 3; c++) {
-                    if (i == 1) dc[c] = VC(cf, c, k, j, i + 1) - VC(cf, c, k, j, i);
-                    else if (i == mx - 2) dc[c] = VC(cf, c, k, j, i) - VC(cf, c, k, j, i - 1);
-                    else dc[c] = (VC(cf, c, k, j, i + 1) - VC(cf, c, k, j, i - 1)) * 0.5;
-                    if (j == 0) de[c] = VC(cf, c, k, j + 1, i) - VC(cf, c, k, j, i);
-                    else if (j == my - 2) de[c] = VC(cf, c, k, j, i) - VC(cf, c, k, j - 1, i);
-                    else de[c] = (VC(cf, c, k, j + 1, i) - VC(cf, c, k, j - 1, i)) * 0.5;
-                    if (k == 1) dz[c] = (VC(cf, c, k + 1, j, i) - VC(cf, c, k, j, i));
-                    else if (k == mz - 2) dz[c] = (VC(cf, c, k, j, i) - VC(cf, c, k - 1, j, i));
-                    else dz[c] = (VC(cf, c, k + 1, j, i) - VC(cf, c, k - 1, j, i)) * 0.5;
-                }
-                size_t q = IX(p, k, j, i);
-                cross_metrics(dc[0], dc[1], dc[2], de[0], de[1], de[2], dz[0], dz[1], dz[2],
-                              &m->jcsi[3 * q], &m->jeta[3 * q], &m->jzet[3 * q], &m->jaj[q]);
-            }
-    /* K faces, metrics.c:711-818 */
-    memset(cf, 0, 3 * n * sizeof(double));
-    for (int k = 0; k < mz - 1; k++)
-        for (int j = 1; j < my; j++)
-            for (int i = 1; i < mx; i++)
-                for (int c = 0; c < 3; c++) {
-#define CC(kk, jj, ii) coor[3 * IX(p, (kk), (jj), (ii)) + c]
-                    VC(cf, c, k, j, i) = (CC(k, j, i) + CC(k, j - 1, i) + CC(k, j, i - 1) + CC(k, j - 1, i - 1)) * 0.25;
-#undef CC
-                }
-    for (int k = 0; k < mz - 1; k++)
-        for (int j = 1; j < my - 1; j++)
-            for (int i = 1; i < mx - 1; i++) {
-                double dc[3], de[3], dz[3];
-                for (int c = 0; c < 3; c++) {
-                    if (i == 1) dc[c] = VC(cf, c, k, j, i + 1) - VC(cf, c, k, j, i);
-                    else if (i == mx - 2) dc[c] = VC(cf, c, k, j, i) - VC(cf, c, k, j, i - 1);
-                    else dc[c] = (VC(cf, c, k, j, i + 1) - VC(cf, c, k, j, i - 1)) * 0.5;
-                    if (j == 1) de[c] = VC(cf, c, k, j + 1, i) - VC(cf, c, k, j, i);
-                    else if (j == my - 2) de[c] = VC(cf, c, k, j, i) - VC(cf, c, k, j - 1, i);
-                    else de[c] = (VC(cf, c, k, j + 1, i) - VC(cf, c, k, j - 1, i)) * 0.5;
-                    if (k == 0) dz[c] = (VC(cf, c, k + 1, j, i) - VC(cf, c, k, j, i));
-                    else if (k == mz - 2) dz[c] = (VC(cf, c, k, j, i) - VC(cf, c, k - 1, j, i));
-                    else dz[c] = (VC(cf, c, k + 1, j, i) - VC(cf, c, k - 1, j, i)) * 0.5;
-                }
-                size_t q = IX(p, k, j, i);
-                cross_metrics(dc[0], dc[1], dc[2], de[0], de[1], de[2], dz[0], dz[1], dz[2],
-                              &m->kcsi[3 * q], &m->keta[3 * q], &m->kzet[3 * q], &m->kaj[q]);
-            }
-    free(cf);
-
+    }
     /* final periodic fix of all sixteen arrays, metrics.c:1180-1300 */
     orc_periodic_fix(p, m->csi, 3);
     orc_periodic_fix(p, m->eta, 3);
@@ -715,4 +579,51 @@ void orc_pressure_gradient(const orc_params *p, const orc_metrics *m, double *pr
                 if (j == 0 || S(nv, k, j, i) + S(nv, k, j + 1, i) > 0.1 || (!pj && j == my - 2)) VY(dp, k, j, i) = 0;
                 if (k == 0 || S(nv, k, j, i) + S(nv, k + 1, j, i) > 0.1 || (!pk && k == mz - 2)) VZ(dp, k, j, i) = 0;
             }
+}
+
+/* ------------------------------------------------------------------------ */
+/* InflowFlux (bcs.c:247-626), uniform inlet (-inlet 1) on the k- face       */
+/* ------------------------------------------------------------------------ */
+
+/* Calc_Inlet_Area's inletArea (bcs.c:201-234): |zet| of the k = 0 faces
+ * whose first cell is fluid. */
+double orc_inlet_area(const orc_params *p, const orc_metrics *m)
+{
+    double a = 0;
+    for (int j = 1; j < p->my - 1; j++)
+        for (int i = 1; i < p->mx - 1; i++) {
+            double k_area = sqrt(VX(m->zet, 0, j, i) * VX(m->zet, 0, j, i) + VY(m->zet, 0, j, i) * VY(m->zet, 0, j, i) +
+                                 VZ(m->zet, 0, j, i) * VZ(m->zet, 0, j, i));
+            if (S(m->nvert, 1, j, i) < 0.1) a += k_area;
+        }
+    return a;
+}
+
+/* Sets the inlet fluxes ucont(0,j,i).z = uin |zet| (uin = 1 when no -flux is
+ * given, inlet_flux / inletArea otherwise), the ghost velocity
+ * ucat(0) = 2 u - ucat(1) with u from Contra2Cart_single, and returns FluxInSum. */
+double orc_inflow_flux(const orc_params *p, const orc_metrics *m, double *ucont, double *ucat)
+{
+    if (p->bctype[4] != 5) return p->flux_in_sum;
+    const double uin = p->inlet_flux < 0 ? 1. : p->inlet_flux / orc_inlet_area(p, m);
+    double flux = 0;
+    for (int j = 1; j < p->my - 1; j++)
+        for (int i = 1; i < p->mx - 1; i++) {
+            const int k = 0;
+            double area = sqrt(VX(m->zet, k, j, i) * VX(m->zet, k, j, i) + VY(m->zet, k, j, i) * VY(m->zet, k, j, i) +
+                               VZ(m->zet, k, j, i) * VZ(m->zet, k, j, i));
+            if (S(m->nvert, k + 1, j, i) < 0.1) {
+                VZ(ucont, k, j, i) = uin * area;
+                VX(ucont, k, j, i) = 0;
+                VY(ucont, k, j, i) = 0;
+                double q[3] = {0, 0, VZ(ucont, k, j, i)}, u[3];
+                cramer(&VX(m->csi, k, j, i), &VX(m->eta, k, j, i), &VX(m->zet, k, j, i), q, u);
+                for (int c = 0; c < 3; c++) VC(ucat, c, k, j, i) = -VC(ucat, c, k + 1, j, i) + 2 * u[c];
+                flux += VZ(ucont, k, j, i);
+            } else {
+                VZ(ucat, k, j, i) = 0;
+                VZ(ucont, k, j, i) = 0;
+            }
+        }
+    return flux;
 }

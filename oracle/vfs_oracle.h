@@ -144,6 +144,10 @@ double orc_dfunc_s3h(double r);
 double orc_dfunc_s4h(double r);
 double orc_dfunc_exp(double r, double n);
 
+/* InflowFlux (bcs.c:247-626), uniform inlet on the k- face: returns FluxInSum */
+double orc_inlet_area(const orc_params *p, const orc_metrics *m);
+double orc_inflow_flux(const orc_params *p, const orc_metrics *m, double *ucont, double *ucat);
+
 /* a7b: CN residual, FormFunction_SNES (momentum.c:2237-2336), time_coeff() == 1 */
 void orc_formfunction_snes(const orc_params *p, const orc_metrics *m, const double *u, const double *ucont_o,
                            const double *rhs_o, const double *dp, const double *feul, double *ucont, double *ucat,

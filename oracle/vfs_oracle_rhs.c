@@ -307,9 +307,13 @@ void orc_formfunction2(const orc_params *p, const orc_metrics *m, double *ucont,
 }
 
 /* RungeKutta, rhs.c:1586-1708 (InflowFlux acts only on bctype 5 inlets). */
-void orc_runge_kutta(const orc_params *p, const orc_metrics *m, double *ucont, double *ucat,
+void orc_runge_kutta(const orc_params *p_in, const orc_metrics *m, double *ucont, double *ucat,
                      const double *ucat_old, const double *nu_t, const double *dp, double *duold)
 {
+    /* InflowFlux first (rhs.c:1612); its FluxInSum feeds the outlet balance */
+    orc_params pq = *p_in;
+    const orc_params *p = &pq;
+    if (pq.bctype[4] == 5) pq.flux_in_sum = orc_inflow_flux(&pq, m, ucont, ucat);
     const size_t n3 = 3 * npts(p);
     const double alfa[4] = {0.25, 1 / 3., 0.5, 1.};
     double *ucont_o = dalloc(n3), *pucont = dalloc(n3), *rhs = dalloc(n3);

@@ -36,31 +36,56 @@ def test_metrics_closed_form_cartesian():
         np.testing.assert_allclose(o.m[f[0] + "aj"][s], 1 / (hx * hy * hz), rtol=1e-12)
 
 
-def test_metrics_gcl_channel():
-    """Discrete geometric conservation: the node-plane area vectors of each
-    interior cell sum to zero (closed cell surface)."""
-    c = cases.channel(16, 12, 16)
-    p = po.make_params(*c.da, per=c.per, bctype=c.bctype)
-    o = po.Oracle(p, c.coor)
-    csi, eta, zet = o.m["csi"], o.m["eta"], o.m["zet"]
-    s = np.s_[2:-2, 2:-2, 2:-2]
-    sm = np.s_[2:-2, 2:-2, 1:-3]
-    net = (csi[s] - csi[sm]) + (eta[s] - eta[2:-2, 1:-3, 2:-2]) + (zet[s] - zet[1:-3, 2:-2, 2:-2])
-    assert np.abs(net).max() < 1e-15
+def test_metrics_closed_form_sheared():
+    """y = eta * hy * (1 + a * xi): the cell-averaged edge vectors of cell
+    (i, j, k) are exact, giving csi = (hy hz (1 + a (i - 1/2)), 0, 0),
+    eta = (-hz hy a (j - 1/2), hz hx, 0), zet = (0, 0, hx hy (1 + a (i - 1/2)))
+    and aj = 1 / (hx hy hz (1 + a (i - 1/2))).  The node-plane formulas of
+    the non-NEWMETRIC build would put i (not i - 1/2) here, so this separates
+    the two; face metrics are the means of the two cells, ghosts copy their
+    neighbours in the order i-, i+, j-, j+, k-, k+."""
+    hx, hy, hz, a = 0.1, 0.2, 0.05, 0.15
+    nx, ny, nz = 6, 5, 7
+    mx, my, mz = nx + 2, ny + 2, nz + 2
+    coor = np.zeros((mz, my, mx, 3))
+    K, J, I = np.meshgrid(np.arange(nz + 1), np.arange(ny + 1), np.arange(nx + 1), indexing="ij")
+    coor[: nz + 1, : ny + 1, : nx + 1, 0] = I * hx
+    coor[: nz + 1, : ny + 1, : nx + 1, 1] = J * hy * (1 + a * I)
+    coor[: nz + 1, : ny + 1, : nx + 1, 2] = K * hz
+    p = po.make_params(mx, my, mz, per=(0, 0, 0), bctype=(1, 1, 1, 1, 1, 1))
+    o = po.Oracle(p, coor)
+    k, j, i = np.meshgrid(np.arange(mz), np.arange(my), np.arange(mx), indexing="ij")
+    ic = np.clip(i, 1, mx - 2).astype(float)
+    jc = np.clip(j, 1, my - 2).astype(float)
+    s = 1 + a * (ic - 0.5)
+    exp = {
+        "csi": np.stack([hy * hz * s, 0 * s, 0 * s], -1),
+        "eta": np.stack([-hz * hy * a * (jc - 0.5), hz * hx + 0 * s, 0 * s], -1),
+        "zet": np.stack([0 * s, 0 * s, hx * hy * s], -1),
+    }
+    for f, e in exp.items():  # every cell, ghosts included (mirrored)
+        np.testing.assert_allclose(o.m[f], e, rtol=1e-13, atol=1e-16, err_msg=f)
+    np.testing.assert_allclose(o.m["aj"], 1 / (hx * hy * hz * s), rtol=1e-13)
+    si = slice(1, -1)
+    # I faces 0..mx-2: mean of cells i and i+1
+    fi = 1 + a * (0.5 * (np.clip(np.arange(mx - 1), 1, mx - 2) + np.clip(np.arange(1, mx), 1, mx - 2)) - 0.5)
+    np.testing.assert_allclose(o.m["icsi"][si, si, : mx - 1, 0], np.broadcast_to(hy * hz * fi, (nz, ny, mx - 1)),
+                               rtol=1e-13)
+    vol = 0.5 * (1 / o.m["aj"][si, si, : mx - 1] + 1 / o.m["aj"][si, si, 1:])
+    np.testing.assert_allclose(o.m["iaj"][si, si, : mx - 1], 1 / vol, rtol=1e-13)
+    # J faces 0..my-2: eta.x is the mean of the two cells' -hz hy a (j - 1/2)
+    fj = 0.5 * (np.clip(np.arange(my - 1), 1, my - 2) + np.clip(np.arange(1, my), 1, my - 2)) - 0.5
+    np.testing.assert_allclose(o.m["jeta"][si, : my - 1, si, 0],
+                               np.broadcast_to((-hz * hy * a * fj)[None, :, None], (nz, my - 1, nx)), rtol=1e-13)
+    # K faces: zet is k independent
+    np.testing.assert_allclose(o.m["kzet"][: mz - 1, si, si], o.m["zet"][: mz - 1, si, si], rtol=1e-13, atol=1e-16)
 
 
 def test_contra2cart_recovers_uniform_flow():
     """Ucont = u . (node-plane area vector) on any grid gives back u exactly."""
-    c = cases.channel(12, 10, 12)
-    p = po.make_params(*c.da, per=c.per, bctype=c.bctype)
-    o = po.Oracle(p, c.coor)
     u = np.array([0.3, -0.2, 1.1])
-    ucont = np.zeros(o["ucont"].shape)
-    ucont[..., 0] = o.m["csi"] @ u
-    ucont[..., 1] = o.m["eta"] @ u
-    ucont[..., 2] = o.m["zet"] @ u
-    # the cell solve averages faces i-1 and i, which hold the same csi only on
-    # an affine grid; use the Cartesian box for an exact statement
+    # Contra2Cart averages the fluxes of faces i-1 and i, which equal
+    # u . csi of the cell only on an affine grid: use the Cartesian box
     p, o = cartesian(5, 4, 6, per=(1, 0, 1), bctype=(100, 100, 1, 1, 100, 100))
     ucont = np.zeros(o["ucont"].shape)
     ucont[..., 0] = o.m["csi"] @ u

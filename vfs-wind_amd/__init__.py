@@ -126,6 +126,7 @@ _SIGS = {
     "vfs_contra2cart": ([C.c_void_p], C.c_int),
     "vfs_form_bcs": ([C.c_void_p, C.c_int], C.c_int),
     "vfs_eddy_viscosity": ([C.c_void_p], C.c_int),
+    "vfs_inflow_flux": ([C.c_void_p, C.POINTER(C.c_double)], C.c_int),
     "vfs_calc_k_flux": ([C.c_void_p, C.POINTER(C.c_double)], C.c_int),
     "vfs_pressure_gradient": ([C.c_void_p], C.c_int),
     "vfs_formfunction2": ([C.c_void_p, C.c_double], C.c_int),
@@ -306,6 +307,11 @@ class Context:
 
     def form_bcs(self, outflow_scale: int = 0):
         self._check(self.lib.vfs_form_bcs(self.h, int(outflow_scale)))
+
+    def inflow_flux(self) -> float:
+        v = C.c_double()
+        self._check(self.lib.vfs_inflow_flux(self.h, C.byref(v)))
+        return v.value
 
     def eddy_viscosity(self):
         self._check(self.lib.vfs_eddy_viscosity(self.h))

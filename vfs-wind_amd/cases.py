@@ -177,3 +177,27 @@ def synthetic_turbine(radius: float, hub, *, num_blade: int = 3, nodes_per_blade
             "ang_cd": np.array([-90.0, 90.0]), "cd": np.array([0.01, 0.01])}
     return tb.TurbineSetup(0.0, 0.0, 1.0, hub[0], hub[1], hub[2], tsr * uref / radius, pitch, r_nacelle, refangle,
                            rotor_model, deltafunc, halfwidth, nodes, [foil])
+
+
+def box_flow(nx: int, ny: int, nz: int, h: float = 1.0, bump: float = 0.3, seed: int = SEED, noise: float = 0.05,
+             ren: float = 2800.0) -> Case:
+    """Uniform-inflow box of the actuator-line configuration: slip walls on the
+    i and j faces (bcs 10), uniform inlet on k- (5), outlet on k+ (4), U = 1
+    along +z plus `noise` seeded isotropic noise; face fluxes from the cell
+    velocities with zero flux through the slip walls."""
+    b = box(nx, ny, nz, h=h, bump=bump, seed=seed, noise=noise)
+    mx, my, mz = b["da"]
+    IM, JM, KM = nx + 1, ny + 1, nz + 1
+    coor, ucat = b["coor"], b["ucat"]
+    ai, aj, ak = _face_areas(coor[:KM, :JM, :IM])
+    ucont = np.zeros((mz, my, mx, 3))
+    for_i = 0.5 * (ucat[1:-1, 1:-1, :-1] + ucat[1:-1, 1:-1, 1:])
+    ucont[1:-1, 1:-1, :-1, 0] = np.einsum("kjic,kjic->kji", for_i, ai[:, :, :IM])
+    for_j = 0.5 * (ucat[1:-1, :-1, 1:-1] + ucat[1:-1, 1:, 1:-1])
+    ucont[1:-1, :-1, 1:-1, 1] = np.einsum("kjic,kjic->kji", for_j, aj[:, :JM, :])
+    for_k = 0.5 * (ucat[:-1, 1:-1, 1:-1] + ucat[1:, 1:-1, 1:-1])
+    ucont[:-1, 1:-1, 1:-1, 2] = np.einsum("kjic,kjic->kji", for_k, ak[:KM, :, :])
+    ucont[:, :, 0, 0] = ucont[:, :, mx - 2, 0] = 0.0   # slip walls
+    ucont[:, 0, :, 1] = ucont[:, my - 2, :, 1] = 0.0
+    return Case(name=f"box{nx}x{ny}x{nz}", cells=(nx, ny, nz), per=(0, 0, 0), bctype=(10, 10, 10, 10, 5, 4), ren=ren,
+                coor=coor, ucont=ucont, inlet_flux=-1.0)

@@ -451,8 +451,9 @@ int vfs_implicit_matrixfree(vfs_ctx *c, const vfs_newton_opts *o, vfs_newton_sta
     const long long n = s.n3;
     const unsigned nb = blocks_for(n, IM_TPB);
     auto fail = [&](int code) { im_free(s); return code; };
-    // InflowFlux is not on this path (no inlet faces); FormBCS with outflow_scale = 0
-    rc = form_bcs(c, 0);
+    // InflowFlux, then FormBCS with outflow_scale = 0 (implicitsolver.c:4293-4297)
+    rc = inflow_flux(c, nullptr);
+    if (!rc) rc = form_bcs(c, 0);
     if (rc) return fail(rc);
     VFS_HIP(hipMemcpyAsync(s.U, c->f[VFS_UCONT], vb, hipMemcpyDeviceToDevice, st));
     int nfunc = 0, newton = 0, lin_total = 0;

@@ -41,6 +41,9 @@ int validate(const vfs_params *p)
         return vfs::set_error(VFS_EINVAL, "ren and dt must be positive");
     if (p->per_j)
         return vfs::set_error(VFS_EINVAL, "a periodic j direction is not supported (wall-normal line smoother)");
+    for (int f = 0; f < 6; f++)
+        if (p->bctype[f] == 5 && (f != 4 || p->per_k))
+            return vfs::set_error(VFS_EINVAL, "an inlet (bcs code 5) is supported on the k- face of a non-periodic k direction");
     return VFS_OK;
 }
 }  // namespace
@@ -294,7 +297,8 @@ int vfs_runge_kutta(vfs_ctx *c)
 {
     if (!c) return vfs::set_error(VFS_EINVAL, "null context");
     static const double alfa[4] = {0.25, 1 / 3., 0.5, 1.};
-    int rc = vfs::form_bcs(c, 0);
+    int rc = vfs::inflow_flux(c, nullptr);  // rhs.c:1612
+    if (!rc) rc = vfs::form_bcs(c, 0);
     if (rc) return rc;
     const size_t n3 = 3 * (size_t)c->g.n;
     VFS_HIP(hipMemcpyAsync(c->pucont, c->f[VFS_UCONT], n3 * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
@@ -308,6 +312,12 @@ int vfs_runge_kutta(vfs_ctx *c)
         if (rc) return rc;
     }
     return vfs::form_bcs(c, 1);
+}
+
+int vfs_inflow_flux(vfs_ctx *c, double *flux_in)
+{
+    if (!c) return vfs::set_error(VFS_EINVAL, "null context");
+    return vfs::inflow_flux(c, flux_in);
 }
 
 int vfs_min_dt(vfs_ctx *c, double *dt_cfl1)

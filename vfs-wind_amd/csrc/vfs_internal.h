@@ -131,6 +131,7 @@ int periodic_fix(vfs_ctx *c, double *f, int nc, int comp_mask_mode);
 int launch_metrics(vfs_ctx *c, const Grid &g, const double *coor, double *const m[17]);
 int c2c(vfs_ctx *c);
 int form_bcs(vfs_ctx *c, int outflow_scale);
+int inflow_flux(vfs_ctx *c, double *flux_in);
 int eddy_viscosity(vfs_ctx *c);
 int pressure_gradient(vfs_ctx *c);
 int calc_k_flux(vfs_ctx *c, double *out);

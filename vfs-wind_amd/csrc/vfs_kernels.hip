@@ -90,8 +90,11 @@ __device__ inline bool decode(const Grid &g, long long t, Decode &d)
 
 #define CRD(c, kk, jj, ii) coor[3 * gidx(g, (kk), (jj), (ii)) + (c)]
 
-// cell centres (metrics.c:79-100), node-plane area vectors csi/eta/zet
-// (metrics.c:116-185) and the Jacobian (metrics.c:193-241)
+// cell centres (metrics.c:79-100) and, at interior cells, the Jacobian with
+// the cell metrics csi/eta/zet.  The reference compiles FormMetrics with
+// NEWMETRIC defined (metrics.c:10): its node-plane formulas
+// (metrics.c:115-191) are dead code, and the cell metrics are the cross
+// products of the cell-averaged edge vectors (metrics.c:193-257).
 __global__ void k_metrics_cell(Grid g, const double *__restrict__ coor, double *__restrict__ csi,
                                double *__restrict__ eta, double *__restrict__ zet, double *__restrict__ aj,
                                double *__restrict__ cent)
@@ -100,72 +103,45 @@ __global__ void k_metrics_cell(Grid g, const double *__restrict__ coor, double *
     if (!decode(g, blockIdx.x * (long long)blockDim.x + threadIdx.x, d)) return;
     const int i = d.i, j = d.j, k = d.k;
     const long long q = gidx(g, k, j, i);
-    const bool ii_ = i >= 1 && i < g.mx - 1, jj_ = j >= 1 && j < g.my - 1, kk_ = k >= 1 && k < g.mz - 1;
-    if (ii_ && jj_ && kk_) {
-        for (int c = 0; c < 3; c++) {
-            double s = CRD(c, k, j, i) + CRD(c, k, j - 1, i) + CRD(c, k - 1, j, i) + CRD(c, k - 1, j - 1, i) +
-                       CRD(c, k, j, i - 1) + CRD(c, k, j - 1, i - 1) + CRD(c, k - 1, j, i - 1) +
-                       CRD(c, k - 1, j - 1, i - 1);
-            cent[3 * q + c] = 0.125 * s;
-        }
+    if (!(i >= 1 && i < g.mx - 1 && j >= 1 && j < g.my - 1 && k >= 1 && k < g.mz - 1)) return;
+    for (int c = 0; c < 3; c++) {
+        double s = CRD(c, k, j, i) + CRD(c, k, j - 1, i) + CRD(c, k - 1, j, i) + CRD(c, k - 1, j - 1, i) +
+                   CRD(c, k, j, i - 1) + CRD(c, k, j - 1, i - 1) + CRD(c, k - 1, j, i - 1) +
+                   CRD(c, k - 1, j - 1, i - 1);
+        cent[3 * q + c] = 0.125 * s;
     }
-    if (kk_ && jj_ && i < g.mx - 1) {
-        double dxde = 0.5 * (CRD(0, k, j, i) + CRD(0, k - 1, j, i) - CRD(0, k, j - 1, i) - CRD(0, k - 1, j - 1, i));
-        double dyde = 0.5 * (CRD(1, k, j, i) + CRD(1, k - 1, j, i) - CRD(1, k, j - 1, i) - CRD(1, k - 1, j - 1, i));
-        double dzde = 0.5 * (CRD(2, k, j, i) + CRD(2, k - 1, j, i) - CRD(2, k, j - 1, i) - CRD(2, k - 1, j - 1, i));
-        double dxdz = 0.5 * (CRD(0, k, j - 1, i) + CRD(0, k, j, i) - CRD(0, k - 1, j - 1, i) - CRD(0, k - 1, j, i));
-        double dydz = 0.5 * (CRD(1, k, j - 1, i) + CRD(1, k, j, i) - CRD(1, k - 1, j - 1, i) - CRD(1, k - 1, j, i));
-        double dzdz = 0.5 * (CRD(2, k, j - 1, i) + CRD(2, k, j, i) - CRD(2, k - 1, j - 1, i) - CRD(2, k - 1, j, i));
-        csi[3 * q + 0] = dyde * dzdz - dzde * dydz;
-        csi[3 * q + 1] = -dxde * dzdz + dzde * dxdz;
-        csi[3 * q + 2] = dxde * dydz - dyde * dxdz;
+    double dd[3][3];
+    for (int c = 0; c < 3; c++) {
+        dd[c][0] = 0.25 * (CRD(c, k, j, i) + CRD(c, k, j - 1, i) + CRD(c, k - 1, j, i) + CRD(c, k - 1, j - 1, i) -
+                           CRD(c, k, j, i - 1) - CRD(c, k, j - 1, i - 1) - CRD(c, k - 1, j, i - 1) -
+                           CRD(c, k - 1, j - 1, i - 1));
+        dd[c][1] = 0.25 * (CRD(c, k, j, i) + CRD(c, k, j, i - 1) + CRD(c, k - 1, j, i) + CRD(c, k - 1, j, i - 1) -
+                           CRD(c, k, j - 1, i) - CRD(c, k, j - 1, i - 1) - CRD(c, k - 1, j - 1, i) -
+                           CRD(c, k - 1, j - 1, i - 1));
+        dd[c][2] = 0.25 * (CRD(c, k, j, i) + CRD(c, k, j - 1, i) + CRD(c, k, j, i - 1) + CRD(c, k, j - 1, i - 1) -
+                           CRD(c, k - 1, j, i) - CRD(c, k - 1, j - 1, i) - CRD(c, k - 1, j, i - 1) -
+                           CRD(c, k - 1, j - 1, i - 1));
     }
-    if (kk_ && j < g.my - 1 && ii_) {
-        double dxdc = 0.5 * (CRD(0, k, j, i) + CRD(0, k - 1, j, i) - CRD(0, k, j, i - 1) - CRD(0, k - 1, j, i - 1));
-        double dydc = 0.5 * (CRD(1, k, j, i) + CRD(1, k - 1, j, i) - CRD(1, k, j, i - 1) - CRD(1, k - 1, j, i - 1));
-        double dzdc = 0.5 * (CRD(2, k, j, i) + CRD(2, k - 1, j, i) - CRD(2, k, j, i - 1) - CRD(2, k - 1, j, i - 1));
-        double dxdz = 0.5 * (CRD(0, k, j, i) + CRD(0, k, j, i - 1) - CRD(0, k - 1, j, i) - CRD(0, k - 1, j, i - 1));
-        double dydz = 0.5 * (CRD(1, k, j, i) + CRD(1, k, j, i - 1) - CRD(1, k - 1, j, i) - CRD(1, k - 1, j, i - 1));
-        double dzdz = 0.5 * (CRD(2, k, j, i) + CRD(2, k, j, i - 1) - CRD(2, k - 1, j, i) - CRD(2, k - 1, j, i - 1));
-        eta[3 * q + 0] = dydz * dzdc - dzdz * dydc;
-        eta[3 * q + 1] = -dxdz * dzdc + dzdz * dxdc;
-        eta[3 * q + 2] = dxdz * dydc - dydz * dxdc;
-    }
-    if (k < g.mz - 1 && jj_ && ii_) {
-        double dxdc = 0.5 * (CRD(0, k, j, i) + CRD(0, k, j - 1, i) - CRD(0, k, j, i - 1) - CRD(0, k, j - 1, i - 1));
-        double dydc = 0.5 * (CRD(1, k, j, i) + CRD(1, k, j - 1, i) - CRD(1, k, j, i - 1) - CRD(1, k, j - 1, i - 1));
-        double dzdc = 0.5 * (CRD(2, k, j, i) + CRD(2, k, j - 1, i) - CRD(2, k, j, i - 1) - CRD(2, k, j - 1, i - 1));
-        double dxde = 0.5 * (CRD(0, k, j, i) + CRD(0, k, j, i - 1) - CRD(0, k, j - 1, i) - CRD(0, k, j - 1, i - 1));
-        double dyde = 0.5 * (CRD(1, k, j, i) + CRD(1, k, j, i - 1) - CRD(1, k, j - 1, i) - CRD(1, k, j - 1, i - 1));
-        double dzde = 0.5 * (CRD(2, k, j, i) + CRD(2, k, j, i - 1) - CRD(2, k, j - 1, i) - CRD(2, k, j - 1, i - 1));
-        zet[3 * q + 0] = dydc * dzde - dzdc * dyde;
-        zet[3 * q + 1] = -dxdc * dzde + dzdc * dxde;
-        zet[3 * q + 2] = dxdc * dyde - dydc * dxde;
-    }
-    if (ii_ && jj_ && kk_) {
-        double dd[3][3];
-        for (int c = 0; c < 3; c++) {
-            dd[c][0] = 0.25 * (CRD(c, k, j, i) + CRD(c, k, j - 1, i) + CRD(c, k - 1, j, i) + CRD(c, k - 1, j - 1, i) -
-                               CRD(c, k, j, i - 1) - CRD(c, k, j - 1, i - 1) - CRD(c, k - 1, j, i - 1) -
-                               CRD(c, k - 1, j - 1, i - 1));
-            dd[c][1] = 0.25 * (CRD(c, k, j, i) + CRD(c, k, j, i - 1) + CRD(c, k - 1, j, i) + CRD(c, k - 1, j, i - 1) -
-                               CRD(c, k, j - 1, i) - CRD(c, k, j - 1, i - 1) - CRD(c, k - 1, j - 1, i) -
-                               CRD(c, k - 1, j - 1, i - 1));
-            dd[c][2] = 0.25 * (CRD(c, k, j, i) + CRD(c, k, j - 1, i) + CRD(c, k, j, i - 1) + CRD(c, k, j - 1, i - 1) -
-                               CRD(c, k - 1, j, i) - CRD(c, k - 1, j - 1, i) - CRD(c, k - 1, j, i - 1) -
-                               CRD(c, k - 1, j - 1, i - 1));
-        }
-        double dxdc = dd[0][0], dydc = dd[1][0], dzdc = dd[2][0];
-        double dxde = dd[0][1], dyde = dd[1][1], dzde = dd[2][1];
-        double dxdz = dd[0][2], dydz = dd[1][2], dzdz = dd[2][2];
-        double det = dxdc * (dyde * dzdz - dzde * dydz) - dydc * (dxde * dzdz - dzde * dxdz) +
-                     dzdc * (dxde * dydz - dyde * dxdz);
-        aj[q] = 1. / det;
-    }
+    const double dxdc = dd[0][0], dydc = dd[1][0], dzdc = dd[2][0];
+    const double dxde = dd[0][1], dyde = dd[1][1], dzde = dd[2][1];
+    const double dxdz = dd[0][2], dydz = dd[1][2], dzdz = dd[2][2];
+    double det = dxdc * (dyde * dzdz - dzde * dydz) - dydc * (dxde * dzdz - dzde * dxdz) +
+                 dzdc * (dxde * dydz - dyde * dxdz);
+    aj[q] = 1. / det;
+    csi[3 * q + 0] = dyde * dzdz - dzde * dydz;
+    csi[3 * q + 1] = -dxde * dzdz + dzde * dxdz;
+    csi[3 * q + 2] = dxde * dydz - dyde * dxdz;
+    eta[3 * q + 0] = dydz * dzdc - dzdz * dydc;
+    eta[3 * q + 1] = -dxdz * dzdc + dzdz * dxdc;
+    eta[3 * q + 2] = dxdz * dydc - dydz * dxdc;
+    zet[3 * q + 0] = dydc * dzde - dzdc * dyde;
+    zet[3 * q + 1] = -dxdc * dzde + dzdc * dxde;
+    zet[3 * q + 2] = dxdc * dyde - dydc * dxde;
 }
 
-// ghost mirroring (metrics.c:261-345); `which` selects one of the six
-// sequential loops, each a pass over one boundary plane.
+// ghost mirroring (metrics.c:261-345): `which` selects one of the six
+// sequential plane passes (i-, i+, j-, j+, k-, k+), each copying all four
+// cell arrays from the neighbour plane; later passes overwrite edges.
 __global__ void k_metrics_mirror(Grid g, int which, double *__restrict__ csi, double *__restrict__ eta,
                                  double *__restrict__ zet, double *__restrict__ aj)
 {
@@ -187,79 +163,36 @@ __global__ void k_metrics_mirror(Grid g, int which, double *__restrict__ csi, do
     if (axis == 1) sj = hi ? g.my - 2 : 1;
     if (axis == 2) sk = hi ? g.mz - 2 : 1;
     long long d = gidx(g, k, j, i), s = gidx(g, sk, sj, si);
-    double *A = axis == 0 ? eta : csi;   // i: eta, zet; j: csi, zet; k: eta, csi
-    double *B = axis == 2 ? csi : zet;
-    if (axis == 2) A = eta;
-    for (int c = 0; c < 3; c++) { A[3 * d + c] = A[3 * s + c]; B[3 * d + c] = B[3 * s + c]; }
+    for (int c = 0; c < 3; c++) {
+        csi[3 * d + c] = csi[3 * s + c];
+        eta[3 * d + c] = eta[3 * s + c];
+        zet[3 * d + c] = zet[3 * s + c];
+    }
     aj[d] = aj[s];
 }
 
-// face centroid of a node plane (metrics.c:483-495, 612-624, 718-730)
-__global__ void k_face_centroid(Grid g, int axis, const double *__restrict__ coor, double *__restrict__ cf)
+// face metrics (metrics.c:588-593, 696-701, 804-809): the mean of the two
+// cell metrics a face separates (AxByC, momentum.c:40-44) and the harmonic
+// mean of their aj.  I faces cover i = 0..mx-2 over the interior j, k; J and
+// K faces likewise in their own direction (metrics.c:497, 626, 731).
+__global__ void k_face_metrics(Grid g, int axis, const double *__restrict__ csi, const double *__restrict__ eta,
+                               const double *__restrict__ zet, const double *__restrict__ aj,
+                               double *__restrict__ fcsi, double *__restrict__ feta, double *__restrict__ fzet,
+                               double *__restrict__ faj)
 {
     Decode d;
     if (!decode(g, blockIdx.x * (long long)blockDim.x + threadIdx.x, d)) return;
     const int i = d.i, j = d.j, k = d.k;
-    bool ok;
-    if (axis == 0) ok = k >= 1 && j >= 1 && i < g.mx - 1;
-    else if (axis == 1) ok = k >= 1 && j < g.my - 1 && i >= 1;
-    else ok = k < g.mz - 1 && j >= 1 && i >= 1;
-    long long q = gidx(g, k, j, i);
-    for (int c = 0; c < 3; c++) {
-        double v = 0;
-        if (ok) {
-            if (axis == 0) v = (CRD(c, k, j, i) + CRD(c, k - 1, j, i) + CRD(c, k, j - 1, i) + CRD(c, k - 1, j - 1, i)) * 0.25;
-            else if (axis == 1) v = (CRD(c, k, j, i) + CRD(c, k - 1, j, i) + CRD(c, k, j, i - 1) + CRD(c, k - 1, j, i - 1)) * 0.25;
-            else v = (CRD(c, k, j, i) + CRD(c, k, j - 1, i) + CRD(c, k, j, i - 1) + CRD(c, k, j - 1, i - 1)) * 0.25;
-        }
-        cf[3 * q + c] = v;
-    }
-}
-
-// face metrics from face-centroid differences (metrics.c:497-818)
-__global__ void k_face_metrics(Grid g, int axis, const double *__restrict__ cf, double *__restrict__ fcsi,
-                               double *__restrict__ feta, double *__restrict__ fzet, double *__restrict__ faj)
-{
-    Decode d;
-    if (!decode(g, blockIdx.x * (long long)blockDim.x + threadIdx.x, d)) return;
-    const int i = d.i, j = d.j, k = d.k;
-    bool ok;
-    if (axis == 0) ok = k >= 1 && k < g.mz - 1 && j >= 1 && j < g.my - 1 && i < g.mx - 1;
-    else if (axis == 1) ok = k >= 1 && k < g.mz - 1 && j < g.my - 1 && i >= 1 && i < g.mx - 1;
-    else ok = k < g.mz - 1 && j >= 1 && j < g.my - 1 && i >= 1 && i < g.mx - 1;
-    if (!ok) return;
-    // lower index at which the own direction switches to a forward difference
     const int i0 = axis == 0 ? 0 : 1, j0 = axis == 1 ? 0 : 1, k0 = axis == 2 ? 0 : 1;
-    double dc[3], de[3], dz[3];
-#define CF(c, kk, jj, ii) cf[3 * gidx(g, (kk), (jj), (ii)) + (c)]
-    for (int c = 0; c < 3; c++) {
-        if (i == i0) dc[c] = CF(c, k, j, i + 1) - CF(c, k, j, i);
-        else if (i == g.mx - 2) dc[c] = CF(c, k, j, i) - CF(c, k, j, i - 1);
-        else dc[c] = (CF(c, k, j, i + 1) - CF(c, k, j, i - 1)) * 0.5;
-        if (j == j0) de[c] = CF(c, k, j + 1, i) - CF(c, k, j, i);
-        else if (j == g.my - 2) de[c] = CF(c, k, j, i) - CF(c, k, j - 1, i);
-        else de[c] = (CF(c, k, j + 1, i) - CF(c, k, j - 1, i)) * 0.5;
-        if (k == k0) dz[c] = (CF(c, k + 1, j, i) - CF(c, k, j, i));
-        else if (k == g.mz - 2) dz[c] = (CF(c, k, j, i) - CF(c, k - 1, j, i));
-        else dz[c] = (CF(c, k + 1, j, i) - CF(c, k - 1, j, i)) * 0.5;
-    }
-#undef CF
-    const double dxdc = dc[0], dydc = dc[1], dzdc = dc[2];
-    const double dxde = de[0], dyde = de[1], dzde = de[2];
-    const double dxdz = dz[0], dydz = dz[1], dzdz = dz[2];
+    if (!(i >= i0 && i < g.mx - 1 && j >= j0 && j < g.my - 1 && k >= k0 && k < g.mz - 1)) return;
     const long long q = gidx(g, k, j, i);
-    fcsi[3 * q + 0] = dyde * dzdz - dzde * dydz;
-    fcsi[3 * q + 1] = -dxde * dzdz + dzde * dxdz;
-    fcsi[3 * q + 2] = dxde * dydz - dyde * dxdz;
-    feta[3 * q + 0] = dydz * dzdc - dzdz * dydc;
-    feta[3 * q + 1] = -dxdz * dzdc + dzdz * dxdc;
-    feta[3 * q + 2] = dxdz * dydc - dydz * dxdc;
-    fzet[3 * q + 0] = dydc * dzde - dzdc * dyde;
-    fzet[3 * q + 1] = -dxdc * dzde + dzdc * dxde;
-    fzet[3 * q + 2] = dxdc * dyde - dydc * dxde;
-    double det = dxdc * (dyde * dzdz - dzde * dydz) - dydc * (dxde * dzdz - dzde * dxdz) +
-                 dzdc * (dxde * dydz - dyde * dxdz);
-    faj[q] = 1. / det;
+    const long long r = q + (axis == 0 ? 1LL : axis == 1 ? (long long)g.mx : (long long)g.mx * g.my);
+    for (int c = 0; c < 3; c++) {
+        fcsi[3 * q + c] = 0.5 * csi[3 * q + c] + 0.5 * csi[3 * r + c];
+        feta[3 * q + c] = 0.5 * eta[3 * q + c] + 0.5 * eta[3 * r + c];
+        fzet[3 * q + c] = 0.5 * zet[3 * q + c] + 0.5 * zet[3 * r + c];
+    }
+    faj[q] = 2. / (1. / aj[q] + 1. / aj[r]);
 }
 #undef CRD
 
@@ -278,14 +211,17 @@ int launch_metrics(vfs_ctx *c, const Grid &g, const double *coor, double *const 
     for (int w = 0; w < 6; w++)
         hipLaunchKernelGGL(k_metrics_mirror, dim3(blocks_for(pl[w >> 1], TPB)), dim3(TPB), 0, st, g, w, m[0], m[1], m[2], m[3]);
     VFS_LAUNCH_CHECK("metrics mirror");
-    double *cf = c->snap;  // 3 doubles per point of the finest grid
-    for (int axis = 0; axis < 3; axis++) {
-        hipLaunchKernelGGL(k_face_centroid, dim3(nb), dim3(TPB), 0, st, g, axis, coor, cf);
-        hipLaunchKernelGGL(k_face_metrics, dim3(nb), dim3(TPB), 0, st, g, axis, cf, m[4 + 4 * axis],
-                           m[5 + 4 * axis], m[6 + 4 * axis], m[7 + 4 * axis]);
+    // periodic copy into the cell arrays before the face means read them
+    // (metrics.c:365-395)
+    for (int q = 0; q < 4; q++) {
+        int rc = periodic_fix_grid(st, g, m[q], q == 3 ? 1 : 3, 0);
+        if (rc) return rc;
     }
+    for (int axis = 0; axis < 3; axis++)
+        hipLaunchKernelGGL(k_face_metrics, dim3(nb), dim3(TPB), 0, st, g, axis, m[0], m[1], m[2], m[3],
+                           m[4 + 4 * axis], m[5 + 4 * axis], m[6 + 4 * axis], m[7 + 4 * axis]);
     VFS_LAUNCH_CHECK("face metrics");
-    // final periodic fix of the sixteen arrays (metrics.c:1180-1300)
+    // final periodic fix of the sixteen arrays (metrics.c:1233-1280)
     for (int q = 0; q < 16; q++) {
         bool vec = !(q == 3 || q == 7 || q == 11 || q == 15);
         int rc = periodic_fix_grid(st, g, m[q], vec ? 3 : 1, 0);
@@ -568,6 +504,84 @@ __global__ void k_zero_edges(Grid g, double *__restrict__ ucat)
     if (t < g.my) { int j = (int)t; z(0, j, 0); z(0, j, g.mx - 1); z(g.mz - 1, j, 0); z(g.mz - 1, j, g.mx - 1); }
     if (t < g.mx) { int i = (int)t; z(0, 0, i); z(0, g.my - 1, i); z(g.mz - 1, 0, i); z(g.mz - 1, g.my - 1, i); }
     if (t < g.mz) { int k = (int)t; z(k, 0, 0); z(k, 0, g.mx - 1); z(k, g.my - 1, 0); z(k, g.my - 1, g.mx - 1); }
+}
+
+// ---------------------------------------------------------------------------
+// InflowFlux (bcs.c:247-626): uniform inlet (-inlet 1) on the k- face
+// ---------------------------------------------------------------------------
+
+// inletArea (bcs.c:201-234) and, with mode 1, the inlet fluxes themselves:
+// ucont(0).z = uin |zet|, ucat(0) = 2 u - ucat(1) with u = Contra2Cart_single
+// (rhs.c:42-63) of (0, 0, ucont.z); per-block partial sums of the area (mode
+// 0) or of the fluid inlet flux (mode 1)
+__global__ void k_inflow(Grid g, int mode, double uin, const double *__restrict__ csi, const double *__restrict__ eta,
+                         const double *__restrict__ zet, const double *__restrict__ nvert, double *__restrict__ ucont,
+                         double *__restrict__ ucat, double *__restrict__ part)
+{
+    const long long n = (long long)(g.mx - 2) * (g.my - 2);
+    double s = 0;
+    for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < n; t += (long long)gridDim.x * blockDim.x) {
+        const int i = 1 + t % (g.mx - 2), j = 1 + t / (g.mx - 2);
+        const long long q = gidx(g, 0, j, i), q1 = gidx(g, 1, j, i);
+        const double *z = zet + 3 * q;
+        const double area = sqrt(z[0] * z[0] + z[1] * z[1] + z[2] * z[2]);
+        const bool fluid = nvert[q1] < 0.1;
+        if (mode == 0) {
+            if (fluid) s += area;
+            continue;
+        }
+        if (fluid) {
+            const double q2 = uin * area;
+            ucont[3 * q + 2] = q2;
+            ucont[3 * q] = 0;
+            ucont[3 * q + 1] = 0;
+            const double m00 = csi[3 * q], m01 = csi[3 * q + 1], m02 = csi[3 * q + 2];
+            const double m10 = eta[3 * q], m11 = eta[3 * q + 1], m12 = eta[3 * q + 2];
+            const double m20 = z[0], m21 = z[1], m22 = z[2];
+            const double q0 = 0, qq1 = 0;
+            const double det = m00 * (m11 * m22 - m12 * m21) - m01 * (m10 * m22 - m12 * m20) + m02 * (m10 * m21 - m11 * m20);
+            const double det0 = q0 * (m11 * m22 - m12 * m21) - qq1 * (m01 * m22 - m02 * m21) + q2 * (m01 * m12 - m02 * m11);
+            const double det1 = -q0 * (m10 * m22 - m12 * m20) + qq1 * (m00 * m22 - m02 * m20) - q2 * (m00 * m12 - m02 * m10);
+            const double det2 = q0 * (m10 * m21 - m11 * m20) - qq1 * (m00 * m21 - m01 * m20) + q2 * (m00 * m11 - m01 * m10);
+            const double u[3] = {det0 / det, det1 / det, det2 / det};
+            for (int c = 0; c < 3; c++) ucat[3 * q + c] = -ucat[3 * q1 + c] + 2 * u[c];
+            s += q2;
+        } else {
+            ucat[3 * q + 2] = 0;
+            ucont[3 * q + 2] = 0;
+        }
+    }
+    s = block_reduce<OpSum>(s);
+    if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+int inflow_flux(vfs_ctx *c, double *flux_in)
+{
+    const Grid &g = c->g;
+    if (g.bc[4] != 5) {
+        if (flux_in) *flux_in = c->p.flux_in_sum;
+        return VFS_OK;
+    }
+    hipStream_t st = c->stream;
+    double uin = 1.0;
+    if (c->p.inlet_flux >= 0) {
+        hipLaunchKernelGGL(k_inflow, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, st, g, 0, 0.0, c->f[VFS_CSI], c->f[VFS_ETA],
+                           c->f[VFS_ZET], c->f[VFS_NVERT], c->f[VFS_UCONT], c->f[VFS_UCAT], c->red);
+        hipLaunchKernelGGL(finish_reduce<OpSum>, dim3(1), dim3(RED_THREADS), 0, st, c->red, RED_BLOCKS, c->red + RED_BLOCKS);
+        VFS_LAUNCH_CHECK("inlet area");
+        VFS_HIP(hipMemcpyAsync(c->red_host, c->red + RED_BLOCKS, sizeof(double), hipMemcpyDeviceToHost, st));
+        VFS_HIP(hipStreamSynchronize(st));
+        uin = c->p.inlet_flux / c->red_host[0];
+    }
+    hipLaunchKernelGGL(k_inflow, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, st, g, 1, uin, c->f[VFS_CSI], c->f[VFS_ETA],
+                       c->f[VFS_ZET], c->f[VFS_NVERT], c->f[VFS_UCONT], c->f[VFS_UCAT], c->red);
+    hipLaunchKernelGGL(finish_reduce<OpSum>, dim3(1), dim3(RED_THREADS), 0, st, c->red, RED_BLOCKS, c->red + RED_BLOCKS);
+    VFS_LAUNCH_CHECK("inflow flux");
+    VFS_HIP(hipMemcpyAsync(c->red_host, c->red + RED_BLOCKS, sizeof(double), hipMemcpyDeviceToHost, st));
+    VFS_HIP(hipStreamSynchronize(st));
+    c->p.flux_in_sum = c->red_host[0];  // FluxInSum, read by the outlet balance of FormBCS
+    if (flux_in) *flux_in = c->red_host[0];
+    return VFS_OK;
 }
 
 int form_bcs(vfs_ctx *c, int outflow_scale)
