@@ -96,7 +96,8 @@ int vfs_mean_k_area(vfs_ctx *ctx, double *area);
 /* ---- momentum (rhs.c, momentum.c, les.c) ------------------------------- */
 /* Contra2Cart (rhs.c:35 -> Contra2Cart_2, rhs.c:65-749). */
 int vfs_contra2cart(vfs_ctx *ctx);
-/* FormBCS (bcs.c:1684-2767) for codes 1, 10, 4 and periodic directions. */
+/* FormBCS (bcs.c:1684-2767) for codes 1 (wall), 10 (slip), 4 (outlet), 5 (uniform inlet,
+   whose fluxes vfs_inflow_flux sets) and periodic directions. */
 int vfs_form_bcs(vfs_ctx *ctx, int outflow_scale);
 /* InflowFlux (bcs.c:247-626) for a uniform inlet (-inlet 1, code 5 on the k- face):
  * inlet fluxes and ghost velocities; FluxInSum returned and kept for FormBCS. */
