@@ -121,7 +121,7 @@ typedef struct {
     int restart;         /* FGMRES restart length (20)                       */
     int maxit;           /* total iteration cap                              */
     int levels;          /* most multigrid levels (6); fewer if a grid stops halving */
-    int pre, post;       /* smoothing sweeps (2, 2)                          */
+    int pre, post;       /* smoothing sweeps (2, 3)                          */
     int coarse_sweeps;   /* coarsest-level sweeps when it is too big for a dense solve */
     int mg_fp32;         /* 1: FP32 V-cycle inside the FP64 FGMRES           */
     int use_guess;       /* 1: start from VFS_PHI; 0: from zero              */

@@ -99,6 +99,25 @@ def test_contra2cart_recovers_uniform_flow():
     np.testing.assert_allclose(o["ucat"][1:-1, 0, 1:-1], -o["ucat"][1:-1, 1, 1:-1], rtol=0, atol=0)
 
 
+@pytest.mark.parametrize("case", ["channel", "box"])
+def test_contra2cart_idempotent(case):
+    """A second Contra2Cart on an unchanged Ucont rewrites the same Ucat
+    (ghosts included): the device RungeKutta relies on this to drop the
+    stage-start call that repeats the previous stage's closing one."""
+    c = cases.channel(24, 16, 20) if case == "channel" else cases.box_flow(24, 20, 32)
+    p = po.make_params(*c.da, per=c.per, bctype=c.bctype, inlet_flux=c.inlet_flux)
+    o = po.Oracle(p, c.coor)
+    o.set("ucont", c.ucont)
+    o.contra2cart()
+    o.set("ucat_old", 0.9 * o["ucat"])
+    rng = np.random.default_rng(1)
+    o.set("ucont", c.ucont * (1 + 0.01 * rng.standard_normal(c.ucont.shape)))
+    o.contra2cart()
+    u1 = o["ucat"].copy()
+    o.contra2cart()
+    assert np.array_equal(o["ucat"], u1)
+
+
 def test_eddy_viscosity_linear_field():
     """u = A x on a Cartesian grid: central differences are exact, so
     nu_t = Cs^2 * V^(2/3) * sqrt(2 S:S) with S = sym(A)."""

@@ -227,7 +227,7 @@ def make_params(mx, my, mz, *, per=(1, 0, 1), bctype=(100, 100, 1, 1, 100, 100),
     return p
 
 
-def default_solve_opts(tol=1e-8, restart=20, maxit=200, levels=6, pre=2, post=2, coarse_sweeps=20, mg_fp32=1,
+def default_solve_opts(tol=1e-8, restart=20, maxit=200, levels=6, pre=2, post=3, coarse_sweeps=20, mg_fp32=1,
                        use_guess=1) -> SolveOpts:
     o = SolveOpts()
     o.tol, o.restart, o.maxit, o.levels = tol, restart, maxit, levels

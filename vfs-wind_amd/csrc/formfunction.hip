@@ -37,24 +37,36 @@ struct FaceArgs {
 };
 
 // ucat, nvert and nu_t of the points around the block's current plane come
-// from an LDS tile (3 k-planes x (TI+3) x (TJ+3) points, i-1..i+2 and
-// j-1..j+2 of the block's points); anything outside it (the fourth-order
-// k+2 point and the periodic far ghosts) is read from HBM.  The tile holds
-// exactly the values the DMDA index with periodic wrap would return.
+// from an LDS tile of 3 k-planes around the block's FT_I x FT_J points.  With
+// the second-order convection (-second_order 1) every face of the block
+// reads only the points i-1..i+1, j-1..j+1, k-1..k+1 of its own point, so the
+// tile spans i0-1..i0+FT_I and j0-1..j0+FT_J and the accessor reads LDS only.
+// The fourth-order QUICK path also reads i+2 / j+2 (one more tile column and
+// row), the k+2 plane and, at a periodic boundary, the far ghosts; those
+// reads outside the tile go to HBM.  The tile holds exactly the values the
+// DMDA index with periodic wrap would return.  nvert (blanking codes, small
+// integers) is kept as float in LDS, which represents them exactly.
 constexpr int FT_I = 32, FT_J = 8, FT_K = 16;
-constexpr int FT_HI = FT_I + 3, FT_HJ = FT_J + 3, FT_HP = FT_HI * FT_HJ;
 
+template <bool SO>
+struct TileDims {
+    static constexpr int HI = FT_I + (SO ? 2 : 3), HJ = FT_J + (SO ? 2 : 3), HP = HI * HJ;
+};
+
+template <bool SO>
 struct TileAcc {
+    using D = TileDims<SO>;
     const FaceArgs *a;
-    const double *ux, *uy, *uz, *nv, *nt;
+    const double *ux, *uy, *uz, *nt;
+    const float *nv;
     int i0, j0, kc, slot0;  // slot0 holds plane kc-1
     __device__ inline int lidx(int k, int j, int i, bool &in) const
     {
         const int di = i - i0 + 1, dj = j - j0 + 1, dk = k - kc + 1;
-        in = (unsigned)di < (unsigned)FT_HI && (unsigned)dj < (unsigned)FT_HJ && (unsigned)dk < 3u;
+        in = SO || ((unsigned)di < (unsigned)D::HI && (unsigned)dj < (unsigned)D::HJ && (unsigned)dk < 3u);
         int s = slot0 + dk;
         if (s >= 3) s -= 3;
-        return (s * FT_HJ + dj) * FT_HI + di;
+        return (s * D::HJ + dj) * D::HI + di;
     }
     __device__ inline double U(int c, int k, int j, int i) const
     {
@@ -67,7 +79,7 @@ struct TileAcc {
     {
         bool in;
         const int l = lidx(k, j, i, in);
-        return in ? nv[l] : a->nvert[gidx(a->g, k, j, i)];
+        return in ? (double)nv[l] : a->nvert[gidx(a->g, k, j, i)];
     }
     __device__ inline double NT(int k, int j, int i) const
     {
@@ -101,8 +113,10 @@ __device__ inline void visc_face(const double cs[3], const double et[3], const d
                    r[2][c] * nrm[2]) * ajc * (nu);
 }
 
-// the 18 face fluxes of point (i,j,k) (momentum.c:668-1451)
-__device__ inline void face_point(const FaceArgs &a, const TileAcc &T, int i, int j, int k)
+// the 18 face fluxes of point (i,j,k) (momentum.c:668-1451); SO selects the
+// second-order convection at compile time (a.second_order is then 1)
+template <bool SO>
+__device__ inline void face_point(const FaceArgs &a, const TileAcc<SO> &T, int i, int j, int k)
 {
     const Grid &g = a.g;
     const long long q = ((long long)k * g.my + j) * g.mx + i;
@@ -140,10 +154,10 @@ __device__ inline void face_point(const FaceArgs &a, const TileAcc &T, int i, in
                 else if (g.pi && i == 0) iL = -3;
                 else iL = i, iR = i + 1;
             } else if (T.NV(k, j, iL) + T.NV(k, j, iR) > 0.1) iL = i, iR = i + 1;
-            if (a.second_order) iL = i, iR = i + 1;
+            if (SO) iL = i, iR = i + 1;
             const double uc = a.ucont[3 * q + 0];
             for (int c = 0; c < 3; c++) {
-                if (a.second_order) dv[c] = -uc * 0.5 * (T.U(c, k, j, i) + T.U(c, k, j, i + 1));
+                if (SO) dv[c] = -uc * 0.5 * (T.U(c, k, j, i) + T.U(c, k, j, i + 1));
                 else dv[c] = -uc * 0.0625 * (-T.U(c, k, j, iL) + 9. * T.U(c, k, j, i) + 9. * T.U(c, k, j, i + 1) - T.U(c, k, j, iR));
             }
             if (T.NV(k, j, i) + T.NV(k, j, i + 1) > 0.1) dv[0] = dv[1] = dv[2] = 0;
@@ -187,10 +201,10 @@ __device__ inline void face_point(const FaceArgs &a, const TileAcc &T, int i, in
                 else if (g.pj && j == 0) jL = -3;
                 else jL = j, jR = j + 1;
             } else if (T.NV(k, jL, i) + T.NV(k, jR, i) > 0.1) jL = j, jR = j + 1;
-            if (a.second_order) jL = j, jR = j + 1;
+            if (SO) jL = j, jR = j + 1;
             const double uc = a.ucont[3 * q + 1];
             for (int c = 0; c < 3; c++) {
-                if (a.second_order) dv[c] = -uc * 0.5 * (T.U(c, k, j, i) + T.U(c, k, j + 1, i));
+                if (SO) dv[c] = -uc * 0.5 * (T.U(c, k, j, i) + T.U(c, k, j + 1, i));
                 else dv[c] = -uc * 0.0625 * (-T.U(c, k, jL, i) + 9. * T.U(c, k, j, i) + 9. * T.U(c, k, j + 1, i) - T.U(c, k, jR, i));
             }
             if (T.NV(k, j, i) + T.NV(k, j + 1, i) > 0.1) dv[0] = dv[1] = dv[2] = 0;
@@ -234,10 +248,10 @@ __device__ inline void face_point(const FaceArgs &a, const TileAcc &T, int i, in
                 else if (g.pk && k == 0) kL = -3;
                 else kL = k, kR = k + 1;
             } else if (T.NV(kL, j, i) + T.NV(kR, j, i) > 0.1) kL = k, kR = k + 1;
-            if (a.second_order) kL = k, kR = k + 1;
+            if (SO) kL = k, kR = k + 1;
             const double uc = a.ucont[3 * q + 2];
             for (int c = 0; c < 3; c++) {
-                if (a.second_order) dv[c] = -uc * 0.5 * (T.U(c, k, j, i) + T.U(c, k + 1, j, i));
+                if (SO) dv[c] = -uc * 0.5 * (T.U(c, k, j, i) + T.U(c, k + 1, j, i));
                 else dv[c] = -uc * 0.0625 * (-T.U(c, kL, j, i) + 9. * T.U(c, k, j, i) + 9. * T.U(c, k + 1, j, i) - T.U(c, kR, j, i));
             }
             if (T.NV(k, j, i) + T.NV(k + 1, j, i) > 0.1) dv[0] = dv[1] = dv[2] = 0;
@@ -255,9 +269,12 @@ __device__ inline void face_point(const FaceArgs &a, const TileAcc &T, int i, in
 
 // One block owns an FT_I x FT_J column of points and marches FT_K planes in
 // k, keeping a rolling 3-plane LDS tile of ucat / nvert / nu_t.
+template <bool SO>
 __global__ void __launch_bounds__(FT_I * FT_J) k_face_fluxes(FaceArgs a)
 {
-    __shared__ double sx[3 * FT_HP], sy[3 * FT_HP], sz[3 * FT_HP], sn[3 * FT_HP], st[3 * FT_HP];
+    using D = TileDims<SO>;
+    __shared__ double sx[3 * D::HP], sy[3 * D::HP], sz[3 * D::HP], st[3 * D::HP];
+    __shared__ float sn[3 * D::HP];
     const Grid &g = a.g;
     const int nbi = (g.mx + FT_I - 1) / FT_I, nbj = (g.my + FT_J - 1) / FT_J;
     int b = blockIdx.x;
@@ -270,17 +287,18 @@ __global__ void __launch_bounds__(FT_I * FT_J) k_face_fluxes(FaceArgs a)
     const bool les = a.les != 0;
     auto load_plane = [&](int kk, int slot) {
         const int wk = wrapi(kk, g.mz, g.pk);
-        for (int p = threadIdx.x; p < FT_HP; p += FT_I * FT_J) {
-            const int di = p % FT_HI, dj = p / FT_HI;
+        for (int p = threadIdx.x; p < D::HP; p += FT_I * FT_J) {
+            const int di = p % D::HI, dj = p / D::HI;
             const int wi = wrapi(i0 - 1 + di, g.mx, g.pi), wj = wrapi(j0 - 1 + dj, g.my, g.pj);
-            double vx = 0, vy = 0, vz = 0, vn = 0, vt = 0;
+            double vx = 0, vy = 0, vz = 0, vt = 0;
+            float vn = 0;
             if (wi >= 0 && wi < g.mx && wj >= 0 && wj < g.my && wk >= 0 && wk < g.mz) {
                 const long long q = ((long long)wk * g.my + wj) * g.mx + wi;
                 vx = a.ucat[3 * q]; vy = a.ucat[3 * q + 1]; vz = a.ucat[3 * q + 2];
-                vn = a.nvert[q];
+                vn = (float)a.nvert[q];
                 if (les) vt = a.nut[q];
             }
-            const int l = slot * FT_HP + p;
+            const int l = slot * D::HP + p;
             sx[l] = vx; sy[l] = vy; sz[l] = vz; sn[l] = vn; st[l] = vt;
         }
     };
@@ -294,8 +312,8 @@ __global__ void __launch_bounds__(FT_I * FT_J) k_face_fluxes(FaceArgs a)
         __syncthreads();
         const int i = i0 + tx, j = j0 + ty;
         if (i < g.mx && j < g.my) {
-            TileAcc T{&a, sx, sy, sz, sn, st, i0, j0, k, slot0};
-            face_point(a, T, i, j, k);
+            TileAcc<SO> T{&a, sx, sy, sz, st, sn, i0, j0, k, slot0};
+            face_point<SO>(a, T, i, j, k);
         }
         __syncthreads();
         slot0 = slot0 + 1 < 3 ? slot0 + 1 : 0;
@@ -344,7 +362,8 @@ __device__ inline double Fv(const CellArgs &a, int plane, int k, int j, int i)
     return a.face[plane * a.g.n + gidx(a.g, k, j, i)];
 }
 
-// Fp of an interior cell, momentum.c:1548-1678
+// Fp of an interior cell, momentum.c:1548-1678; SO is a.second_order
+template <bool SO>
 __device__ void fp_interior(const CellArgs &a, int k, int j, int i, double f[3])
 {
     const Grid &g = a.g;
@@ -353,7 +372,7 @@ __device__ void fp_interior(const CellArgs &a, int k, int j, int i, double f[3])
     for (int c = 0; c < 3; c++)
         dv[c] = (Fv(a, c, k, j, i) - Fv(a, c, k, j, i - 1) + Fv(a, 3 + c, k, j, i) - Fv(a, 3 + c, k, j - 1, i) +
                  Fv(a, 6 + c, k, j, i) - Fv(a, 6 + c, k - 1, j, i));
-    if (a.second_order) {
+    if (SO) {
         for (int c = 0; c < 3; c++) f[c] = dv[c];
     } else {
         int iR = i + 1, iL = i - 2, jR = j + 1, jL = j - 2, kR = k + 1, kL = k - 2;
@@ -386,6 +405,7 @@ __device__ void fp_interior(const CellArgs &a, int k, int j, int i, double f[3])
 
 // Fp as the projection loop reads it: interior cells directly, periodic ghost
 // cells through the periodic fix (momentum.c:1685-1713), other ghosts 0.
+template <bool SO>
 __device__ inline void fp_at(const CellArgs &a, int k, int j, int i, double f[3])
 {
     const Grid &g = a.g;
@@ -395,13 +415,14 @@ __device__ inline void fp_at(const CellArgs &a, int k, int j, int i, double f[3]
         f[0] = f[1] = f[2] = 0;
         return;
     }
-    fp_interior(a, k, j, i, f);
+    fp_interior<SO>(a, k, j, i, f);
 }
 
 // Fp of every point as the projection reads it (fp_at), written once to a
 // 3-plane scratch.  A block owns an FT_I x FT_J column and marches FT_K planes
 // in k; on the second-order path a thread keeps its k-face fluxes in
 // registers, where they become the k-1 face of the next plane.
+template <bool SO>
 __global__ void __launch_bounds__(FT_I * FT_J) k_cell_fp(CellArgs a, double *__restrict__ fp)
 {
     const Grid &g = a.g;
@@ -420,7 +441,7 @@ __global__ void __launch_bounds__(FT_I * FT_J) k_cell_fp(CellArgs a, double *__r
         const long long q = ((long long)k * g.my + j) * g.mx + i;
         double f[3];
         const bool interior = i >= 1 && i < g.mx - 1 && j >= 1 && j < g.my - 1 && k >= 1 && k < g.mz - 1;
-        if (interior && a.second_order) {
+        if (interior && SO) {
             const long long qi = q - 1, qj = q - g.mx;
             const double *F = a.face;
             double own[6];
@@ -438,7 +459,7 @@ __global__ void __launch_bounds__(FT_I * FT_J) k_cell_fp(CellArgs a, double *__r
             for (int c = 0; c < 6; c++) km1[c] = own[c];
             have = true;
         } else {
-            fp_at(a, k, j, i, f);
+            fp_at<SO>(a, k, j, i, f);
             have = false;
         }
         for (int c = 0; c < 3; c++) fp[c * n + q] = f[c];
@@ -535,7 +556,10 @@ int formfunction2(vfs_ctx *c, double scale, const double *pucont, double rk_alph
     prof_begin(c, K_FACE);
     {
         const unsigned nb = (unsigned)(((g.mx + FT_I - 1) / FT_I) * ((g.my + FT_J - 1) / FT_J) * ((g.mz + FT_K - 1) / FT_K));
-        hipLaunchKernelGGL(k_face_fluxes, dim3(nb), dim3(FT_I * FT_J), 0, st, fa);
+        if (fa.second_order)
+            hipLaunchKernelGGL(k_face_fluxes<true>, dim3(nb), dim3(FT_I * FT_J), 0, st, fa);
+        else
+            hipLaunchKernelGGL(k_face_fluxes<false>, dim3(nb), dim3(FT_I * FT_J), 0, st, fa);
     }
     prof_end(c);
     VFS_LAUNCH_CHECK("face fluxes");
@@ -563,7 +587,10 @@ int formfunction2(vfs_ctx *c, double scale, const double *pucont, double rk_alph
     ca.ucont_out = ucont_out;
     const unsigned nbt = (unsigned)(((g.mx + FT_I - 1) / FT_I) * ((g.my + FT_J - 1) / FT_J) * ((g.mz + FT_K - 1) / FT_K));
     prof_begin(c, K_CELLFP);
-    hipLaunchKernelGGL(k_cell_fp, dim3(nbt), dim3(FT_I * FT_J), 0, st, ca, c->fpc);
+    if (ca.second_order)
+        hipLaunchKernelGGL(k_cell_fp<true>, dim3(nbt), dim3(FT_I * FT_J), 0, st, ca, c->fpc);
+    else
+        hipLaunchKernelGGL(k_cell_fp<false>, dim3(nbt), dim3(FT_I * FT_J), 0, st, ca, c->fpc);
     prof_end(c);
     prof_begin(c, K_CELL);
     hipLaunchKernelGGL(k_cell_rhs, dim3(nbt), dim3(FT_I * FT_J), 0, st, ca, (const double *)c->fpc);

@@ -441,6 +441,111 @@ __global__ void __launch_bounds__(PTPB) k_zebra_solve(Grid g, ZGrid z, int c, co
     }
 }
 
+// Line solve of colour c with the line right-hand sides rz (from
+// k_zebra_rhs).  A block owns 32 lines; each line is cut into nseg segments of
+// at most SLM points, one thread per (line, segment), so the whole solve runs
+// with one thread per ~SLM line points instead of one per line.  The Thomas
+// forward recurrence d_j = (r_j - lo_j d_{j-1}) m_j is affine in d_{j-1}: each
+// thread composes its segment's map (d_out = A d_in + B), the maps are chained
+// across segments through LDS, and each thread then reruns the recurrence
+// from its own d_in; the backward recurrence x_j = d_j - cp_j x_{j+1} is done
+// the same way from the top segment down.  The segment's values stay in
+// registers between the passes.
+constexpr int LINE_LINES = 32, LINE_MAXSEG = 32;
+// points per segment: the segment's right-hand sides and its three Thomas
+// factors are held in registers (4 x SLM values)
+template <typename T>
+constexpr int line_slm()
+{
+    return sizeof(T) == 4 ? 16 : 8;
+}
+
+template <typename T, int SLM>
+__global__ void __launch_bounds__(LINE_LINES * LINE_MAXSEG) k_zebra_scan(Grid g, ZGrid z, int c, int nseg, int sl,
+                                                                        const T *__restrict__ sc, const T *__restrict__ rz,
+                                                                        T *__restrict__ x)
+{
+    __shared__ T sA[LINE_MAXSEG * LINE_LINES], sB[LINE_MAXSEG * LINE_LINES];
+    const int ln = threadIdx.x % LINE_LINES, seg = threadIdx.x / LINE_LINES;
+    const long long nlines = (long long)z.nkI * z.nh;
+    const long long lid = (long long)blockIdx.x * LINE_LINES + ln;
+    int kk = 0, h = 0, k = 1, i = 1;
+    bool act = lid < nlines;
+    if (act) {
+        kk = (int)(lid / z.nh);
+        h = (int)(lid % z.nh);
+        k = kk + 1;
+        i = zebra_i0(k, c) + 2 * h;
+        act = i <= g.mx - 2;
+    }
+    const int j0 = seg * sl;
+    const int j1 = j0 + sl < z.njI ? j0 + sl : z.njI;
+    const long long sstride = 2 * z.cplane;
+    const long long base = (long long)kk * z.njI * z.nh + h;  // + jj * nh
+    const T *s0 = sc + c * z.cplane + base;
+    const T *lo = s0 + SM_LO * sstride, *mm = s0 + SM_M * sstride, *cp = s0 + SM_CP * sstride;
+    const T *rr = rz + base;
+    const long long nh = z.nh;
+    T r[SLM], l[SLM], m[SLM], u[SLM];
+#pragma unroll
+    for (int v = 0; v < SLM; v++) {
+        const int jj = j0 + v;
+        if (act && jj < j1) {
+            r[v] = rr[jj * nh];
+            l[v] = lo[jj * nh];
+            m[v] = mm[jj * nh];
+            u[v] = cp[jj * nh];
+        } else {
+            r[v] = 0; l[v] = 0; m[v] = 0; u[v] = 0;
+        }
+    }
+    // forward map of the segment: d_out = A d_in + B
+    T A = 1, B = 0;
+#pragma unroll
+    for (int v = 0; v < SLM; v++)
+        if (j0 + v < j1) {
+            B = (r[v] - l[v] * B) * m[v];
+            A = (-l[v] * A) * m[v];
+        }
+    const int me = seg * LINE_LINES + ln;
+    sA[me] = A;
+    sB[me] = B;
+    __syncthreads();
+    T d = 0;
+    for (int s = 0; s < seg; s++) d = sA[s * LINE_LINES + ln] * d + sB[s * LINE_LINES + ln];
+#pragma unroll
+    for (int v = 0; v < SLM; v++)
+        if (j0 + v < j1) {
+            d = (r[v] - l[v] * d) * m[v];
+            r[v] = d;
+        }
+    __syncthreads();
+    // backward map of the segment: x_first = A x_above + B
+    A = 1;
+    B = 0;
+#pragma unroll
+    for (int v = SLM - 1; v >= 0; v--)
+        if (j0 + v < j1) {
+            B = r[v] - u[v] * B;
+            A = -u[v] * A;
+        }
+    sA[me] = A;
+    sB[me] = B;
+    __syncthreads();
+    T xl = 0;
+    for (int s = nseg - 1; s > seg; s--) xl = sA[s * LINE_LINES + ln] * xl + sB[s * LINE_LINES + ln];
+    if (act) {
+#pragma unroll
+        for (int v = SLM - 1; v >= 0; v--) {
+            const int jj = j0 + v;
+            if (jj < j1) {
+                xl = r[v] - u[v] * xl;
+                x[((long long)k * g.my + jj + 1) * g.mx + i] = xl;
+            }
+        }
+    }
+}
+
 // coarsest level: x = B^{-1} b over the rows, B^{-1} stored column-major.
 // Each block gathers b into LDS once, then DENSE_ROWS threads each form one
 // row; the column loop reads a contiguous run of B^{-1} per step.
@@ -932,6 +1037,10 @@ static void smooth(vfs_ctx *c, PLevel &L, int sweeps)
     hipStream_t st = c->stream;
     const long long lines = (long long)L.z.nkI * L.z.nh;
     const bool fine = L.g.n == c->g.n;
+    constexpr int SLM = line_slm<T>();
+    const int nseg = (L.z.njI + SLM - 1) / SLM;
+    const bool scan = nseg <= LINE_MAXSEG;  // lines up to 32 * SLM points
+    const int sl = (L.z.njI + nseg - 1) / nseg;
     for (int s = 0; s < sweeps; s++)
         for (int col = 0; col < 2; col++) {
             if (fine) prof_begin(c, sizeof(T) == 4 ? K_LINE32 : K_LINE64);
@@ -939,8 +1048,13 @@ static void smooth(vfs_ctx *c, PLevel &L, int sweeps)
                                LV<T>::sc(L), LV<T>::x(L), LV<T>::b(L), LV<T>::rz(L));
             if (fine) prof_end(c);
             if (fine) prof_begin(c, sizeof(T) == 4 ? K_LSOLVE32 : K_LSOLVE64);
-            hipLaunchKernelGGL((k_zebra_solve<T, LINE_PREFETCH>), dim3(blocks_for(lines, PTPB)), dim3(PTPB), 0, st, L.g, L.z,
-                               col, LV<T>::sc(L), LV<T>::rz(L), LV<T>::x(L));
+            if (scan)
+                hipLaunchKernelGGL((k_zebra_scan<T, SLM>), dim3((unsigned)((lines + LINE_LINES - 1) / LINE_LINES)),
+                                   dim3(LINE_LINES * nseg), 0, st, L.g, L.z, col, nseg, sl, LV<T>::sc(L),
+                                   (const T *)LV<T>::rz(L), LV<T>::x(L));
+            else
+                hipLaunchKernelGGL((k_zebra_solve<T, LINE_PREFETCH>), dim3(blocks_for(lines, PTPB)), dim3(PTPB), 0, st, L.g,
+                                   L.z, col, LV<T>::sc(L), LV<T>::rz(L), LV<T>::x(L));
             if (fine) prof_end(c);
         }
 }

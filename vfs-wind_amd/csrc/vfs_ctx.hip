@@ -303,7 +303,11 @@ int vfs_runge_kutta(vfs_ctx *c)
     const size_t n3 = 3 * (size_t)c->g.n;
     VFS_HIP(hipMemcpyAsync(c->pucont, c->f[VFS_UCONT], n3 * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
     for (int s = 0; s < 4; s++) {
-        rc = vfs::c2c(c);
+        // the reference calls Contra2Cart at the start of every stage
+        // (rhs.c:1642); from the second stage on Ucont is unchanged since the
+        // previous stage's closing call and Contra2Cart is idempotent, so
+        // that call would rewrite the same Ucat
+        if (s == 0) rc = vfs::c2c(c);
         if (rc) return rc;
         // Rhs = 0; Formfunction_2(Rhs, 1); Rhs -= dP; Ucont = a*Rhs + pUcont  (rhs.c:1647-1658)
         rc = vfs::formfunction2(c, 1.0, c->pucont, alfa[s] * c->p.dt * c->p.st, c->f[VFS_UCONT]);
