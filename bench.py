@@ -29,13 +29,21 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 
 # Algorithmic bytes per unit of each timed kernel (every input read once, every
 # output written once, FP64 unless stated) and the units one launch covers:
-# "point" = every DA point (the launch grid), "line point" = interior points of
-# one smoother colour, (mz-2)(my-2)ceil((mx-2)/2).
+# "point" = every DA point (the launch grid), "cell" = interior cells,
+# "line point" = interior points of one smoother colour, (mz-2)(my-2)ceil((mx-2)/2).
 ALGO = {
     # ucont 24 + ucat 24 + nu_t 8 + nvert 8 + 3 face metric families (9 + 1 doubles) 240; 18 fluxes out 144
     "face_fluxes": (24 + 24 + 8 + 8 + 240 + 144, "point"),
-    # 18 fluxes 144 + csi/eta/zet 72 + iaj/jaj/kaj 24 + nvert 8 + dP 24 + pUcont 24; Ucont out 24
-    "cell_rhs": (144 + 72 + 24 + 8 + 24 + 24 + 24, "point"),
+    # second-order path: 18 fluxes 144; Fp out 24
+    "cell_fp": (144 + 24, "point"),
+    # Fp 24 + csi/eta/zet 72 + iaj/jaj/kaj 24 + nvert 8 + dP 24 + pUcont 24; Ucont out 24
+    "cell_rhs": (24 + 72 + 24 + 8 + 24 + 24 + 24, "point"),
+    # ucont 24 + csi/eta/zet 72 + nvert 8; ucat out 24 (interior cells)
+    "contra2cart": (24 + 72 + 8 + 24, "cell"),
+    # ucat 24 + csi/eta/zet 72 + aj 8 + nvert 8; nu_t out 8
+    "eddy_viscosity": (24 + 72 + 8 + 8 + 8, "point"),
+    # p 8 + nvert 8 + 3 face metric families 240; dP out 24
+    "pressure_gradient": (8 + 8 + 240 + 24, "point"),
     # FP32: 16 off-line coefficients 64 + b 4 + x 4; line rhs out 4
     "line_rhs_fp32": (64 + 4 + 4 + 4, "line point"),
     # FP32: Thomas factors lo, 1/pivot, upper 12 + line rhs 4; x out 4
@@ -57,7 +65,7 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(nx=64, ny=48, nz=64, steps=3):
+def cpu_baseline(nx=64, ny=48, nz=64, steps=12):
     """FP64 CPU oracle (single thread) on the 64x48x64 channel: Mcell-steps/s."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle as po
@@ -178,7 +186,7 @@ def main():
     value = cells * args.steps / elapsed / 1e6
 
     # per-kernel timing inside the timed region
-    units = {"point": params.mx * params.my * params.mz,
+    units = {"point": params.mx * params.my * params.mz, "cell": cells,
              "line point": (params.mz - 2) * (params.my - 2) * ((params.mx - 1) // 2)}
     kern = {}
     for name, (bpu, unit) in ALGO.items():

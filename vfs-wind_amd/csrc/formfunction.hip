@@ -123,14 +123,32 @@ __device__ inline void face_point(const FaceArgs &a, const TileAcc<SO> &T, int i
     const long long n = g.n;
     const int mx = g.mx, my = g.my, mz = g.mz;
     const double SOL = 0.1;  // k-omega.c:19
+    // the point's three face-metric families and fluxes, loaded together up
+    // front (q is a valid point for every face, used or not) so that the 33
+    // HBM loads are in flight at once
+    double M[3][10], UC[3];
+    {
+        const Metrics9 *fam[3] = {&a.fi, &a.fj, &a.fk};
+#pragma unroll
+        for (int f = 0; f < 3; f++) {
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                M[f][c] = fam[f]->csi[3 * q + c];
+                M[f][3 + c] = fam[f]->eta[3 * q + c];
+                M[f][6 + c] = fam[f]->zet[3 * q + c];
+            }
+            M[f][9] = fam[f]->aj[q];
+            UC[f] = a.ucont[3 * q + f];
+        }
+    }
 
     // ---- i face (momentum.c:668-928): i in [0,mx-2], j,k interior
     {
         double dv[3] = {0, 0, 0}, vs[3] = {0, 0, 0};
         if (i < mx - 1 && j >= 1 && j < my - 1 && k >= 1 && k < mz - 1) {
-            const double cs[3] = {a.fi.csi[3 * q], a.fi.csi[3 * q + 1], a.fi.csi[3 * q + 2]};
-            const double et[3] = {a.fi.eta[3 * q], a.fi.eta[3 * q + 1], a.fi.eta[3 * q + 2]};
-            const double ze[3] = {a.fi.zet[3 * q], a.fi.zet[3 * q + 1], a.fi.zet[3 * q + 2]};
+            const double cs[3] = {M[0][0], M[0][1], M[0][2]};
+            const double et[3] = {M[0][3], M[0][4], M[0][5]};
+            const double ze[3] = {M[0][6], M[0][7], M[0][8]};
             double d[3][3];
             // Compute_du_i, k-omega.c:56-116
             for (int c = 0; c < 3; c++) {
@@ -155,7 +173,7 @@ __device__ inline void face_point(const FaceArgs &a, const TileAcc<SO> &T, int i
                 else iL = i, iR = i + 1;
             } else if (T.NV(k, j, iL) + T.NV(k, j, iR) > 0.1) iL = i, iR = i + 1;
             if (SO) iL = i, iR = i + 1;
-            const double uc = a.ucont[3 * q + 0];
+            const double uc = UC[0];
             for (int c = 0; c < 3; c++) {
                 if (SO) dv[c] = -uc * 0.5 * (T.U(c, k, j, i) + T.U(c, k, j, i + 1));
                 else dv[c] = -uc * 0.0625 * (-T.U(c, k, j, iL) + 9. * T.U(c, k, j, i) + 9. * T.U(c, k, j, i + 1) - T.U(c, k, j, iR));
@@ -167,7 +185,7 @@ __device__ inline void face_point(const FaceArgs &a, const TileAcc<SO> &T, int i
                 else if ((i == mx - 2 && !g.pi) || T.NV(k, j, i + 1) > 0.1) nut = T.NT(k, j, i);
                 else nut = 0.5 * (T.NT(k, j, i) + T.NT(k, j, i + 1));
             }
-            visc_face(cs, et, ze, cs, a.fi.aj[q], d, a.nu, nut, a.les, a.laplacian, vs);
+            visc_face(cs, et, ze, cs, M[0][9], d, a.nu, nut, a.les, a.laplacian, vs);
         }
         for (int c = 0; c < 3; c++) { a.out[(0 + c) * n + q] = dv[c]; a.out[(9 + c) * n + q] = vs[c]; }
     }
@@ -175,9 +193,9 @@ __device__ inline void face_point(const FaceArgs &a, const TileAcc<SO> &T, int i
     {
         double dv[3] = {0, 0, 0}, vs[3] = {0, 0, 0};
         if (i >= 1 && i < mx - 1 && j < my - 1 && k >= 1 && k < mz - 1) {
-            const double cs[3] = {a.fj.csi[3 * q], a.fj.csi[3 * q + 1], a.fj.csi[3 * q + 2]};
-            const double et[3] = {a.fj.eta[3 * q], a.fj.eta[3 * q + 1], a.fj.eta[3 * q + 2]};
-            const double ze[3] = {a.fj.zet[3 * q], a.fj.zet[3 * q + 1], a.fj.zet[3 * q + 2]};
+            const double cs[3] = {M[1][0], M[1][1], M[1][2]};
+            const double et[3] = {M[1][3], M[1][4], M[1][5]};
+            const double ze[3] = {M[1][6], M[1][7], M[1][8]};
             double d[3][3];
             // Compute_du_j, k-omega.c:141-200
             for (int c = 0; c < 3; c++) {
@@ -202,7 +220,7 @@ __device__ inline void face_point(const FaceArgs &a, const TileAcc<SO> &T, int i
                 else jL = j, jR = j + 1;
             } else if (T.NV(k, jL, i) + T.NV(k, jR, i) > 0.1) jL = j, jR = j + 1;
             if (SO) jL = j, jR = j + 1;
-            const double uc = a.ucont[3 * q + 1];
+            const double uc = UC[1];
             for (int c = 0; c < 3; c++) {
                 if (SO) dv[c] = -uc * 0.5 * (T.U(c, k, j, i) + T.U(c, k, j + 1, i));
                 else dv[c] = -uc * 0.0625 * (-T.U(c, k, jL, i) + 9. * T.U(c, k, j, i) + 9. * T.U(c, k, j + 1, i) - T.U(c, k, jR, i));
@@ -214,7 +232,7 @@ __device__ inline void face_point(const FaceArgs &a, const TileAcc<SO> &T, int i
                 else if ((j == my - 2 && !g.pj) || T.NV(k, j + 1, i) > 0.1) nut = T.NT(k, j, i);
                 else nut = 0.5 * (T.NT(k, j, i) + T.NT(k, j + 1, i));
             }
-            visc_face(cs, et, ze, et, a.fj.aj[q], d, a.nu, nut, a.les, a.laplacian, vs);
+            visc_face(cs, et, ze, et, M[1][9], d, a.nu, nut, a.les, a.laplacian, vs);
         }
         for (int c = 0; c < 3; c++) { a.out[(3 + c) * n + q] = dv[c]; a.out[(12 + c) * n + q] = vs[c]; }
     }
@@ -222,9 +240,9 @@ __device__ inline void face_point(const FaceArgs &a, const TileAcc<SO> &T, int i
     {
         double dv[3] = {0, 0, 0}, vs[3] = {0, 0, 0};
         if (i >= 1 && i < mx - 1 && j >= 1 && j < my - 1 && k < mz - 1) {
-            const double cs[3] = {a.fk.csi[3 * q], a.fk.csi[3 * q + 1], a.fk.csi[3 * q + 2]};
-            const double et[3] = {a.fk.eta[3 * q], a.fk.eta[3 * q + 1], a.fk.eta[3 * q + 2]};
-            const double ze[3] = {a.fk.zet[3 * q], a.fk.zet[3 * q + 1], a.fk.zet[3 * q + 2]};
+            const double cs[3] = {M[2][0], M[2][1], M[2][2]};
+            const double et[3] = {M[2][3], M[2][4], M[2][5]};
+            const double ze[3] = {M[2][6], M[2][7], M[2][8]};
             double d[3][3];
             // Compute_du_k, k-omega.c:227-288
             for (int c = 0; c < 3; c++) {
@@ -249,7 +267,7 @@ __device__ inline void face_point(const FaceArgs &a, const TileAcc<SO> &T, int i
                 else kL = k, kR = k + 1;
             } else if (T.NV(kL, j, i) + T.NV(kR, j, i) > 0.1) kL = k, kR = k + 1;
             if (SO) kL = k, kR = k + 1;
-            const double uc = a.ucont[3 * q + 2];
+            const double uc = UC[2];
             for (int c = 0; c < 3; c++) {
                 if (SO) dv[c] = -uc * 0.5 * (T.U(c, k, j, i) + T.U(c, k + 1, j, i));
                 else dv[c] = -uc * 0.0625 * (-T.U(c, kL, j, i) + 9. * T.U(c, k, j, i) + 9. * T.U(c, k + 1, j, i) - T.U(c, kR, j, i));
@@ -261,7 +279,7 @@ __device__ inline void face_point(const FaceArgs &a, const TileAcc<SO> &T, int i
                 else if ((k == mz - 2 && !g.pk) || T.NV(k + 1, j, i) > 0.1) nut = T.NT(k, j, i);
                 else nut = 0.5 * (T.NT(k, j, i) + T.NT(k + 1, j, i));
             }
-            visc_face(cs, et, ze, ze, a.fk.aj[q], d, a.nu, nut, a.les, a.laplacian, vs);
+            visc_face(cs, et, ze, ze, M[2][9], d, a.nu, nut, a.les, a.laplacian, vs);
         }
         for (int c = 0; c < 3; c++) { a.out[(6 + c) * n + q] = dv[c]; a.out[(15 + c) * n + q] = vs[c]; }
     }
@@ -443,20 +461,32 @@ __global__ void __launch_bounds__(FT_I * FT_J) k_cell_fp(CellArgs a, double *__r
         const bool interior = i >= 1 && i < g.mx - 1 && j >= 1 && j < g.my - 1 && k >= 1 && k < g.mz - 1;
         if (interior && SO) {
             const long long qi = q - 1, qj = q - g.mx;
-            const double *F = a.face;
-            double own[6];
-            for (int c = 0; c < 3; c++) { own[c] = F[(6 + c) * n + q]; own[3 + c] = F[(15 + c) * n + q]; }
+            const double *__restrict__ F = a.face;
+            // all loads first (18 own planes, 6 at i-1, 6 at j-1), then the
+            // arithmetic, so the loads are in flight together
+            double own[18], im[6], jm[6];
+#pragma unroll
+            for (int p = 0; p < 18; p++) own[p] = F[p * n + q];
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                im[c] = F[c * n + qi];
+                im[3 + c] = F[(9 + c) * n + qi];
+                jm[c] = F[(3 + c) * n + qj];
+                jm[3 + c] = F[(12 + c) * n + qj];
+            }
             if (!have) {
                 const long long qk = q - g.plane;
+#pragma unroll
                 for (int c = 0; c < 3; c++) { km1[c] = F[(6 + c) * n + qk]; km1[3 + c] = F[(15 + c) * n + qk]; }
             }
             // same operation order as fp_interior
+#pragma unroll
             for (int c = 0; c < 3; c++) {
-                f[c] = F[c * n + q] - F[c * n + qi] + F[(3 + c) * n + q] - F[(3 + c) * n + qj] + own[c] - km1[c];
-                f[c] += (F[(9 + c) * n + q] - F[(9 + c) * n + qi] + F[(12 + c) * n + q] - F[(12 + c) * n + qj] + own[3 + c] -
-                         km1[3 + c]) / 1.0;
+                f[c] = own[c] - im[c] + own[3 + c] - jm[c] + own[6 + c] - km1[c];
+                f[c] += (own[9 + c] - im[3 + c] + own[12 + c] - jm[3 + c] + own[15 + c] - km1[3 + c]) / 1.0;
             }
-            for (int c = 0; c < 6; c++) km1[c] = own[c];
+#pragma unroll
+            for (int c = 0; c < 3; c++) { km1[c] = own[6 + c]; km1[3 + c] = own[15 + c]; }
             have = true;
         } else {
             fp_at<SO>(a, k, j, i, f);

@@ -673,8 +673,10 @@ int eddy_viscosity(vfs_ctx *c)
         VFS_HIP(hipMemsetAsync(c->f[VFS_NU_T], 0, (size_t)g.n * sizeof(double), c->stream));
         return VFS_OK;
     }
+    prof_begin(c, K_EDDY);
     hipLaunchKernelGGL(k_eddy, dim3(blocks_for(g.n, TPB)), dim3(TPB), 0, c->stream, g, c->p.cs2, c->f[VFS_UCAT],
                        c->f[VFS_CSI], c->f[VFS_ETA], c->f[VFS_ZET], c->f[VFS_AJ], c->f[VFS_NVERT], c->f[VFS_NU_T]);
+    prof_end(c);
     VFS_LAUNCH_CHECK("eddy viscosity");
     return periodic_fix(c, c->f[VFS_NU_T], 1, 0);
 }
@@ -844,11 +846,13 @@ int pressure_gradient(vfs_ctx *c)
     if (rc) return rc;
     // (mean_k_flux - inlet_flux) / dt / mean_k_area, momentum.c:386-399
     const double fdiff = c->mean_k_flux - c->p.inlet_flux;
+    prof_begin(c, K_PGRAD);
     hipLaunchKernelGGL(k_pgrad, dim3(blocks_for(g.n, TPB)), dim3(TPB), 0, c->stream, g, fdiff, c->p.dt,
                        c->mean_k_area, c->f[VFS_P],
                        c->f[VFS_NVERT], c->f[VFS_ICSI], c->f[VFS_IETA], c->f[VFS_IZET], c->f[VFS_IAJ],
                        c->f[VFS_JCSI], c->f[VFS_JETA], c->f[VFS_JZET], c->f[VFS_JAJ], c->f[VFS_KCSI],
                        c->f[VFS_KETA], c->f[VFS_KZET], c->f[VFS_KAJ], c->f[VFS_DP]);
+    prof_end(c);
     VFS_LAUNCH_CHECK("pressure gradient");
     return VFS_OK;
 }
