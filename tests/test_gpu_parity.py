@@ -73,6 +73,28 @@ def test_formfunction2(pair):
     assert relerr(ctx.download("RHS"), o["rhs"]) <= RHS_TOL
 
 
+def test_formfunction2_stored_face_metrics(pair):
+    """The face pass reads the stored face-metric arrays once a metric array
+    may have been written from outside (here: its device pointer was handed
+    out); the result is the same as with face metrics rebuilt from the cells."""
+    c, o, ctx = pair
+    o.eddy_viscosity()
+    ctx.upload("NU_T", o["nu_t"])
+    rhs0 = np.zeros(o["rhs"].shape)
+    ctx.upload("RHS", rhs0)
+    ctx.formfunction2(1.0)
+    derived = ctx.download("RHS")
+    ctx.device_ptr("ICSI")
+    ctx.upload("RHS", rhs0)
+    ctx.formfunction2(1.0)
+    stored = ctx.download("RHS")
+    assert np.array_equal(derived, stored)
+    o.set("rhs", rhs0)
+    o.formfunction2(1.0)
+    assert relerr(stored, o["rhs"]) <= RHS_TOL
+    ctx.form_metrics()  # the same metrics again, face pass back on the cell metrics
+
+
 def test_runge_kutta(pair):
     c, o, ctx = pair
     o.eddy_viscosity(); ctx.upload("NU_T", o["nu_t"])

@@ -31,6 +31,7 @@ struct FaceArgs {
     Grid g;
     const double *ucont, *ucat, *nut, *nvert;
     Metrics9 fi, fj, fk;
+    Metrics9 cell;  // cell metrics csi, eta, zet, aj
     double nu;
     int les, second_order, laplacian;
     double *out;  // 18 planes of n doubles: div1,div2,div3 (xyz each), visc1..3
@@ -113,10 +114,35 @@ __device__ inline void visc_face(const double cs[3], const double et[3], const d
                    r[2][c] * nrm[2]) * ajc * (nu);
 }
 
+// the 10 metric values (csi, eta, zet, aj) of point q of one family
+__device__ inline void load_metrics(const Metrics9 &m, long long q, double v[10])
+{
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        v[c] = m.csi[3 * q + c];
+        v[3 + c] = m.eta[3 * q + c];
+        v[6 + c] = m.zet[3 * q + c];
+    }
+    v[9] = m.aj[q];
+}
+
+// a face metric as FormMetrics builds it from the two cells the face
+// separates: 0.5 X + 0.5 Y (AxByC, momentum.c:40-44) and 2 / (1/aj + 1/aj)
+// (metrics.c:588-593, 696-701, 804-809)
+__device__ inline void face_mean(const double a[10], const double b[10], double f[10])
+{
+#pragma unroll
+    for (int c = 0; c < 9; c++) f[c] = 0.5 * a[c] + 0.5 * b[c];
+    f[9] = 2. / (1. / a[9] + 1. / b[9]);
+}
+
 // the 18 face fluxes of point (i,j,k) (momentum.c:668-1451); SO selects the
-// second-order convection at compile time (a.second_order is then 1)
-template <bool SO>
-__device__ inline void face_point(const FaceArgs &a, const TileAcc<SO> &T, int i, int j, int k)
+// second-order convection at compile time (a.second_order is then 1).  With
+// DER the face metrics are rebuilt from the cell metrics: `cur` holds the
+// point's own cell metrics and `nk` those of cell k+1.
+template <bool SO, bool DER>
+__device__ inline void face_point(const FaceArgs &a, const TileAcc<SO> &T, int i, int j, int k, const double *cur,
+                                  const double *nk)
 {
     const Grid &g = a.g;
     const long long q = ((long long)k * g.my + j) * g.mx + i;
@@ -127,19 +153,21 @@ __device__ inline void face_point(const FaceArgs &a, const TileAcc<SO> &T, int i
     // front (q is a valid point for every face, used or not) so that the 33
     // HBM loads are in flight at once
     double M[3][10], UC[3];
-    {
-        const Metrics9 *fam[3] = {&a.fi, &a.fj, &a.fk};
 #pragma unroll
-        for (int f = 0; f < 3; f++) {
-#pragma unroll
-            for (int c = 0; c < 3; c++) {
-                M[f][c] = fam[f]->csi[3 * q + c];
-                M[f][3 + c] = fam[f]->eta[3 * q + c];
-                M[f][6 + c] = fam[f]->zet[3 * q + c];
-            }
-            M[f][9] = fam[f]->aj[q];
-            UC[f] = a.ucont[3 * q + f];
-        }
+    for (int f = 0; f < 3; f++) UC[f] = a.ucont[3 * q + f];
+    if (DER) {
+        // neighbour cells i+1 and j+1 (the point itself where that face is
+        // not computed, which keeps the loads inside the array)
+        double ni[10], nj[10];
+        load_metrics(a.cell, i < mx - 1 ? q + 1 : q, ni);
+        load_metrics(a.cell, j < my - 1 ? q + mx : q, nj);
+        face_mean(cur, ni, M[0]);
+        face_mean(cur, nj, M[1]);
+        face_mean(cur, nk, M[2]);
+    } else {
+        load_metrics(a.fi, q, M[0]);
+        load_metrics(a.fj, q, M[1]);
+        load_metrics(a.fk, q, M[2]);
     }
 
     // ---- i face (momentum.c:668-928): i in [0,mx-2], j,k interior
@@ -287,7 +315,7 @@ __device__ inline void face_point(const FaceArgs &a, const TileAcc<SO> &T, int i
 
 // One block owns an FT_I x FT_J column of points and marches FT_K planes in
 // k, keeping a rolling 3-plane LDS tile of ucat / nvert / nu_t.
-template <bool SO>
+template <bool SO, bool DER>
 __global__ void __launch_bounds__(FT_I * FT_J) k_face_fluxes(FaceArgs a)
 {
     using D = TileDims<SO>;
@@ -323,6 +351,10 @@ __global__ void __launch_bounds__(FT_I * FT_J) k_face_fluxes(FaceArgs a)
     load_plane(k0 - 1, 0);
     load_plane(k0, 1);
     int slot0 = 0;
+    // with DER the cell metrics of k+1 loaded for one plane are the point's
+    // own on the next
+    double cur[DER ? 10 : 1], nk[DER ? 10 : 1];
+    bool have = false;
     for (int k = k0; k < k1; k++) {
         int s2 = slot0 + 2;
         if (s2 >= 3) s2 -= 3;
@@ -330,8 +362,18 @@ __global__ void __launch_bounds__(FT_I * FT_J) k_face_fluxes(FaceArgs a)
         __syncthreads();
         const int i = i0 + tx, j = j0 + ty;
         if (i < g.mx && j < g.my) {
+            if (DER) {
+                const long long q = ((long long)k * g.my + j) * g.mx + i;
+                if (!have) load_metrics(a.cell, q, cur);
+                load_metrics(a.cell, k < g.mz - 1 ? q + g.plane : q, nk);
+            }
             TileAcc<SO> T{&a, sx, sy, sz, st, sn, i0, j0, k, slot0};
-            face_point<SO>(a, T, i, j, k);
+            face_point<SO, DER>(a, T, i, j, k, cur, nk);
+            if (DER) {
+#pragma unroll
+                for (int c = 0; c < 10; c++) cur[c] = nk[c];
+                have = true;
+            }
         }
         __syncthreads();
         slot0 = slot0 + 1 < 3 ? slot0 + 1 : 0;
@@ -578,6 +620,7 @@ int formfunction2(vfs_ctx *c, double scale, const double *pucont, double rk_alph
     fa.fi = {c->f[VFS_ICSI], c->f[VFS_IETA], c->f[VFS_IZET], c->f[VFS_IAJ]};
     fa.fj = {c->f[VFS_JCSI], c->f[VFS_JETA], c->f[VFS_JZET], c->f[VFS_JAJ]};
     fa.fk = {c->f[VFS_KCSI], c->f[VFS_KETA], c->f[VFS_KZET], c->f[VFS_KAJ]};
+    fa.cell = {c->f[VFS_CSI], c->f[VFS_ETA], c->f[VFS_ZET], c->f[VFS_AJ]};
     fa.nu = 1. / c->p.ren;
     fa.les = c->p.les;
     fa.second_order = c->p.second_order;
@@ -586,10 +629,12 @@ int formfunction2(vfs_ctx *c, double scale, const double *pucont, double rk_alph
     prof_begin(c, K_FACE);
     {
         const unsigned nb = (unsigned)(((g.mx + FT_I - 1) / FT_I) * ((g.my + FT_J - 1) / FT_J) * ((g.mz + FT_K - 1) / FT_K));
-        if (fa.second_order)
-            hipLaunchKernelGGL(k_face_fluxes<true>, dim3(nb), dim3(FT_I * FT_J), 0, st, fa);
+        if (fa.second_order && c->face_derived)
+            hipLaunchKernelGGL((k_face_fluxes<true, true>), dim3(nb), dim3(FT_I * FT_J), 0, st, fa);
+        else if (fa.second_order)
+            hipLaunchKernelGGL((k_face_fluxes<true, false>), dim3(nb), dim3(FT_I * FT_J), 0, st, fa);
         else
-            hipLaunchKernelGGL(k_face_fluxes<false>, dim3(nb), dim3(FT_I * FT_J), 0, st, fa);
+            hipLaunchKernelGGL((k_face_fluxes<false, false>), dim3(nb), dim3(FT_I * FT_J), 0, st, fa);
     }
     prof_end(c);
     VFS_LAUNCH_CHECK("face fluxes");

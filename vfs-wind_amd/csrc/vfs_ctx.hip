@@ -112,6 +112,7 @@ int vfs_ctx_create(const vfs_params *params, vfs_ctx **out)
     c->ps = nullptr;
     c->mean_k_area = 0;
     c->mean_k_flux = 0;
+    c->face_derived = false;
     c->prof_on = 0;
     c->ev_pool = new std::vector<hipEvent_t>();
     c->ev_recs = new std::vector<std::pair<int, int>>();
@@ -195,6 +196,7 @@ size_t vfs_field_size(const vfs_ctx *c, vfs_field f)
 int vfs_upload(vfs_ctx *c, vfs_field f, const double *host)
 {
     if (!c || !host || f < 0 || f >= VFS_NFIELDS) return vfs::set_error(VFS_EINVAL, "bad upload argument");
+    if (vfs::is_metric_field(f)) c->face_derived = false;
     VFS_HIP(hipMemcpyAsync(c->f[f], host, vfs_field_size(c, f) * sizeof(double), hipMemcpyHostToDevice, c->stream));
     VFS_HIP(hipStreamSynchronize(c->stream));
     return VFS_OK;
@@ -211,6 +213,7 @@ int vfs_download(vfs_ctx *c, vfs_field f, double *host)
 double *vfs_field_device_ptr(vfs_ctx *c, vfs_field f)
 {
     if (!c || f < 0 || f >= VFS_NFIELDS) return nullptr;
+    if (vfs::is_metric_field(f)) c->face_derived = false;  // the caller may write through it
     return c->f[f];
 }
 
@@ -231,6 +234,7 @@ int vfs_form_metrics(vfs_ctx *c)
                            c->f[VFS_CENT]};
     int rc = vfs::launch_metrics(c, c->g, c->f[VFS_COOR], m);
     if (rc) return rc;
+    c->face_derived = true;
     // the multigrid hierarchy depends on the metrics; rebuild on next solve
     vfs::poisson_free(c);
     VFS_HIP(hipStreamSynchronize(c->stream));

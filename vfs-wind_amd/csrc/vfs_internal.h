@@ -118,6 +118,10 @@ struct vfs_ctx {
     double *red_host; // pinned host mirror of reduction results
     double mean_k_area;
     double mean_k_flux;
+    // true while the face metrics are the cell-metric means vfs_form_metrics
+    // derived (no metric array written since): kernels may then rebuild a
+    // face metric from the two cell metrics instead of reading it
+    bool face_derived;
     vfs_poisson *ps;
     // per-kernel timing (vfs_profile): event pairs recorded around launches
     int prof_on;
@@ -129,6 +133,7 @@ namespace vfs {
 // kernels driven from the ABI layer; all enqueue on ctx->stream
 int periodic_fix(vfs_ctx *c, double *f, int nc, int comp_mask_mode);
 int launch_metrics(vfs_ctx *c, const Grid &g, const double *coor, double *const m[17]);
+inline bool is_metric_field(int f) { return f >= VFS_CSI && f <= VFS_KAJ; }
 int c2c(vfs_ctx *c);
 int form_bcs(vfs_ctx *c, int outflow_scale);
 int inflow_flux(vfs_ctx *c, double *flux_in);
