@@ -73,26 +73,48 @@ def test_formfunction2(pair):
     assert relerr(ctx.download("RHS"), o["rhs"]) <= RHS_TOL
 
 
-def test_formfunction2_stored_face_metrics(pair):
-    """The face pass reads the stored face-metric arrays once a metric array
-    may have been written from outside (here: its device pointer was handed
-    out); the result is the same as with face metrics rebuilt from the cells."""
+def test_stored_face_metrics_paths(pair):
+    """Once a metric array may have been written from outside (here: its
+    device pointer was handed out) the face pass reads the stored face-metric
+    arrays instead of rebuilding face metrics from the cells; both ways give
+    the same bits, and the oracle's result.  Pressure_Gradient and Projection
+    (which always read the stored arrays) are run in both states too."""
     c, o, ctx = pair
+    saved = {f: ctx.download(f) for f in ("P", "PHI", "RHS", "DP", "UCONT", "UCAT", "NU_T")}
     o.eddy_viscosity()
     ctx.upload("NU_T", o["nu_t"])
-    rhs0 = np.zeros(o["rhs"].shape)
-    ctx.upload("RHS", rhs0)
-    ctx.formfunction2(1.0)
-    derived = ctx.download("RHS")
+    rng = np.random.default_rng(12)
+    p = rng.standard_normal(o.shape)
+    phi = rng.standard_normal(o.shape)
+    ucont0 = ctx.download("UCONT")
+
+    def run():
+        ctx.upload("UCONT", ucont0)
+        ctx.contra2cart()
+        ctx.upload("RHS", np.zeros(o["rhs"].shape))
+        ctx.formfunction2(1.0)
+        ctx.upload("P", p)
+        ctx.pressure_gradient()
+        ctx.upload("PHI", phi)
+        ctx.projection()
+        out = [ctx.download(f) for f in ("RHS", "DP", "UCONT")]
+        ctx.upload("UCONT", ucont0)
+        ctx.contra2cart()
+        return out
+
+    derived = run()
     ctx.device_ptr("ICSI")
-    ctx.upload("RHS", rhs0)
-    ctx.formfunction2(1.0)
-    stored = ctx.download("RHS")
-    assert np.array_equal(derived, stored)
-    o.set("rhs", rhs0)
+    stored = run()
+    for d, s_ in zip(derived, stored):
+        assert np.array_equal(d, s_)
+    o.set("ucont", ucont0)
+    o.contra2cart()
+    o.set("rhs", np.zeros(o["rhs"].shape))
     o.formfunction2(1.0)
-    assert relerr(stored, o["rhs"]) <= RHS_TOL
-    ctx.form_metrics()  # the same metrics again, face pass back on the cell metrics
+    assert relerr(stored[0], o["rhs"]) <= RHS_TOL
+    ctx.form_metrics()  # the same metrics again: back on the cell-metric path
+    for f, v in saved.items():  # the fixture is shared by the module's tests
+        ctx.upload(f, v)
 
 
 def test_runge_kutta(pair):

@@ -11,6 +11,9 @@
 #include "vfs_internal.h"
 #include "vfs_reduce.h"
 
+// FP64 expressions keep the reference's operation order: no contraction to FMA
+#pragma clang fp contract(off)
+
 namespace vfs {
 
 constexpr int TPB = 256;
@@ -768,9 +771,17 @@ __global__ void k_pgrad(Grid g, double fdiff, double fdt, double farea, const do
 #define N1(kk, jj, ii) ((int)(nvert[gidx(g, (kk), (jj), (ii))] + 0.5) == 1)
 #define NVS(kk, jj, ii) nvert[gidx(g, (kk), (jj), (ii))]
     const int pi_ = g.pi, pj = g.pj, pk = g.pk;
-    const double *ic = icsi + 3 * q, *ie = ieta + 3 * q, *iz = izet + 3 * q;
-    const double *jc = jcsi + 3 * q, *je = jeta + 3 * q, *jz = jzet + 3 * q;
-    const double *kc = kcsi + 3 * q, *ke = keta + 3 * q, *kz = kzet + 3 * q;
+    // the three face-metric families of the point
+    double FI[10], FJ[10], FK[10];
+    for (int c = 0; c < 3; c++) {
+        FI[c] = icsi[3 * q + c]; FI[3 + c] = ieta[3 * q + c]; FI[6 + c] = izet[3 * q + c];
+        FJ[c] = jcsi[3 * q + c]; FJ[3 + c] = jeta[3 * q + c]; FJ[6 + c] = jzet[3 * q + c];
+        FK[c] = kcsi[3 * q + c]; FK[3 + c] = keta[3 * q + c]; FK[6 + c] = kzet[3 * q + c];
+    }
+    FI[9] = iaj[q]; FJ[9] = jaj[q]; FK[9] = kaj[q];
+    const double *ic = FI, *ie = FI + 3, *iz = FI + 6;
+    const double *jc = FJ, *je = FJ + 3, *jz = FJ + 6;
+    const double *kc = FK, *ke = FK + 3, *kz = FK + 6;
     double g11_i = ic[0] * ic[0] + ic[1] * ic[1] + ic[2] * ic[2];
     double g12_i = ie[0] * ic[0] + ie[1] * ic[1] + ie[2] * ic[2];
     double g13_i = iz[0] * ic[0] + iz[1] * ic[1] + iz[2] * ic[2];
@@ -795,7 +806,7 @@ __global__ void k_pgrad(Grid g, double fdiff, double fdt, double farea, const do
         dpdz = (P(k + 1, j, i) - P(k, j, i) + P(k + 1, j, i + 1) - P(k, j, i + 1)) * 0.5;
     else
         dpdz = (P(k + 1, j, i) - P(k - 1, j, i) + P(k + 1, j, i + 1) - P(k - 1, j, i + 1)) * 0.25;
-    double vx = (dpdc * g11_i + dpde * g12_i + dpdz * g13_i) * iaj[q];
+    double vx = (dpdc * g11_i + dpde * g12_i + dpdz * g13_i) * FI[9];
     if (N1(k, j, i + 1) || N1(k, j + 1, i + 1) || (i == mx - 2 && !pi_))
         dpdc = (P(k, j, i) - P(k, j, i - 1) + P(k, j + 1, i) - P(k, j + 1, i - 1)) * 0.5;
     else if (N1(k, j, i - 1) || N1(k, j + 1, i - 1) || (i == 1 && !pi_))
@@ -810,7 +821,7 @@ __global__ void k_pgrad(Grid g, double fdiff, double fdt, double farea, const do
         dpdz = (P(k + 1, j, i) - P(k, j, i) + P(k + 1, j + 1, i) - P(k, j + 1, i)) * 0.5;
     else
         dpdz = (P(k + 1, j, i) - P(k - 1, j, i) + P(k + 1, j + 1, i) - P(k - 1, j + 1, i)) * 0.25;
-    double vy = (dpdc * g21_j + dpde * g22_j + dpdz * g23_j) * jaj[q];
+    double vy = (dpdc * g21_j + dpde * g22_j + dpdz * g23_j) * FJ[9];
     if (N1(k, j, i + 1) || N1(k + 1, j, i + 1) || (i == mx - 2 && !pi_))
         dpdc = (P(k, j, i) - P(k, j, i - 1) + P(k + 1, j, i) - P(k + 1, j, i - 1)) * 0.5;
     else if (N1(k, j, i - 1) || N1(k + 1, j, i - 1) || (i == 1 && !pi_))
@@ -826,10 +837,10 @@ __global__ void k_pgrad(Grid g, double fdiff, double fdt, double farea, const do
     if (k == mz - 2 && pk) dpdz = P(mz + 1, j, i) - P(k, j, i);
     else dpdz = (P(k + 1, j, i) - P(k, j, i));
     if (pk) {
-        double dz = 1. / kaj[q] / sqrt(kz[0] * kz[0] + kz[1] * kz[1] + kz[2] * kz[2]);
+        double dz = 1. / FK[9] / sqrt(kz[0] * kz[0] + kz[1] * kz[1] + kz[2] * kz[2]);
         dpdz += dz * fdiff / fdt / farea;
     }
-    double vz = (dpdc * g31_k + dpde * g32_k + dpdz * g33_k) * kaj[q];
+    double vz = (dpdc * g31_k + dpde * g32_k + dpdz * g33_k) * FK[9];
     if (i == 0 || NVS(k, j, i) + NVS(k, j, i + 1) > 0.1 || (!pi_ && i == mx - 2)) vx = 0;
     if (j == 0 || NVS(k, j, i) + NVS(k, j + 1, i) > 0.1 || (!pj && j == my - 2)) vy = 0;
     if (k == 0 || NVS(k, j, i) + NVS(k + 1, j, i) > 0.1 || (!pk && k == mz - 2)) vz = 0;
@@ -847,11 +858,12 @@ int pressure_gradient(vfs_ctx *c)
     // (mean_k_flux - inlet_flux) / dt / mean_k_area, momentum.c:386-399
     const double fdiff = c->mean_k_flux - c->p.inlet_flux;
     prof_begin(c, K_PGRAD);
-    hipLaunchKernelGGL(k_pgrad, dim3(blocks_for(g.n, TPB)), dim3(TPB), 0, c->stream, g, fdiff, c->p.dt,
-                       c->mean_k_area, c->f[VFS_P],
-                       c->f[VFS_NVERT], c->f[VFS_ICSI], c->f[VFS_IETA], c->f[VFS_IZET], c->f[VFS_IAJ],
-                       c->f[VFS_JCSI], c->f[VFS_JETA], c->f[VFS_JZET], c->f[VFS_JAJ], c->f[VFS_KCSI],
-                       c->f[VFS_KETA], c->f[VFS_KZET], c->f[VFS_KAJ], c->f[VFS_DP]);
+    {
+        hipLaunchKernelGGL(k_pgrad, dim3(blocks_for(g.n, TPB)), dim3(TPB), 0, c->stream, g, fdiff, c->p.dt, c->mean_k_area,
+                           c->f[VFS_P], c->f[VFS_NVERT], c->f[VFS_ICSI], c->f[VFS_IETA], c->f[VFS_IZET], c->f[VFS_IAJ],
+                           c->f[VFS_JCSI], c->f[VFS_JETA], c->f[VFS_JZET], c->f[VFS_JAJ], c->f[VFS_KCSI], c->f[VFS_KETA],
+                           c->f[VFS_KZET], c->f[VFS_KAJ], c->f[VFS_DP]);
+    }
     prof_end(c);
     VFS_LAUNCH_CHECK("pressure gradient");
     return VFS_OK;
