@@ -287,10 +287,11 @@ int vfs_poisson_solve_learned(vfs_ctx *ctx, vfs_unet *net, const vfs_solve_opts 
 /* Enable (1) or disable (0) HIP-event timing of the hot kernels' launches on
  * the context stream and clear earlier records.  For benchmarks only. */
 int vfs_profile(vfs_ctx *ctx, int enable);
-/* Total time and launch count recorded for kernel id 0..10 (finest level for
+/* Total time and launch count recorded for kernel id 0..11 (finest level for
  * the smoother): face_fluxes, cell_rhs, line_rhs_fp32, line_rhs_fp64,
  * poisson_apply_fp64, contra2cart, pressure_gradient, eddy_viscosity,
- * line_solve_fp32, line_solve_fp64, cell_fp. */
+ * line_solve_fp32, line_solve_fp64, cell_fp, and formfunction2 (one whole
+ * Formfunction_2 call: face pass, its periodic fix and both cell passes). */
 int vfs_profile_read(vfs_ctx *ctx, int kernel, double *total_ms, long long *count, const char **name);
 
 #ifdef __cplusplus

@@ -165,6 +165,28 @@ def test_poisson_solve(pair, fp32):
     assert relerr(ctx.download("PHI"), o["phi"]) <= 1e-6
 
 
+@pytest.mark.parametrize("fp32", [1, 0])
+def test_poisson_graph_replays(pair, fp32, monkeypatch):
+    """The coarse V-cycle levels run as a captured HIP graph: every replay
+    (with other eager work and synchronisations in between) gives the same
+    bits as launching the same kernels directly."""
+    c, o, ctx = pair
+    ctx.poisson_rhs()
+    opts = vfs.default_solve_opts(tol=1e-10, mg_fp32=fp32, use_guess=0)
+    monkeypatch.setenv("VFS_MG_GRAPH", "0")
+    ctx.upload("PHI", np.zeros(o.shape))
+    ctx.poisson_solve(opts)
+    eager = ctx.download("PHI")
+    monkeypatch.delenv("VFS_MG_GRAPH")
+    for rep in range(3):
+        ctx.upload("PHI", np.zeros(o.shape))
+        st = ctx.poisson_solve(opts)
+        assert st.relres <= 1e-10
+        assert np.array_equal(ctx.download("PHI"), eager), rep
+        ctx.contra2cart()
+        ctx.sync()
+
+
 def test_update_projection_divergence(pair):
     c, o, ctx = pair
     o.poisson_rhs()

@@ -609,6 +609,7 @@ int formfunction2(vfs_ctx *c, double scale, const double *pucont, double rk_alph
 {
     const Grid &g = c->g;
     hipStream_t st = c->stream;
+    const int whole = prof_begin(c, K_FF);
     int rc = periodic_fix(c, c->f[VFS_UCONT], 3, 0);  // momentum.c:638-666
     if (rc) return rc;
     FaceArgs fa;
@@ -670,6 +671,7 @@ int formfunction2(vfs_ctx *c, double scale, const double *pucont, double rk_alph
     prof_begin(c, K_CELL);
     hipLaunchKernelGGL(k_cell_rhs, dim3(nbt), dim3(FT_I * FT_J), 0, st, ca, (const double *)c->fpc);
     prof_end(c);
+    prof_end(c, whole);
     VFS_LAUNCH_CHECK("cell rhs");
     return VFS_OK;
 }

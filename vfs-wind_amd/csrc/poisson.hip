@@ -785,12 +785,20 @@ struct PLevel {
     double *inv64 = nullptr;
 };
 
+// a captured coarse sub-cycle (levels 1..nl-1 of one V-cycle) and the
+// smoothing counts it was captured with
+struct SubGraph {
+    hipGraphExec_t exec = nullptr;
+    int pre = -1, post = -1, coarse = -1;
+};
+
 struct vfs_poisson {
     int nl = 0;
     int restart = 0;
     PLevel L[8];
     double *V = nullptr, *Z = nullptr, *w = nullptr, *dev_scalars = nullptr;
     std::vector<void *> allocs;
+    SubGraph sub[2];  // [0] FP32, [1] FP64 hierarchy
 };
 
 namespace vfs {
@@ -816,6 +824,8 @@ void poisson_free(vfs_ctx *c)
 {
     if (!c || !c->ps) return;
     hipStreamSynchronize(c->stream);
+    for (SubGraph &s : c->ps->sub)
+        if (s.exec) hipGraphExecDestroy(s.exec);
     for (void *p : c->ps->allocs) hipFree(p);
     delete c->ps;
     c->ps = nullptr;
@@ -1060,6 +1070,40 @@ static void smooth(vfs_ctx *c, PLevel &L, int sweeps)
 }
 
 template <typename T>
+static void vcycle(vfs_ctx *c, vfs_poisson *ps, int l, int pre, int post, int coarse);
+
+// The coarse part of a V-cycle (zero x on level 1, then levels 1..nl-1) is a
+// fixed sequence of small kernels on fixed buffers: it is captured once into
+// a HIP graph per hierarchy and smoothing setting and replayed, which cuts the
+// per-launch gaps of the ~4 us coarse-level kernels.  It holds kernel nodes
+// only.  VFS_MG_GRAPH=0 launches the same kernels directly.
+template <typename T>
+static void coarse_subcycle(vfs_ctx *c, vfs_poisson *ps, int pre, int post, int coarse)
+{
+    hipStream_t st = c->stream;
+    PLevel &C = ps->L[1];
+    auto direct = [&]() {
+        hipLaunchKernelGGL(k_zero<T>, dim3(blocks_for(C.g.n, PTPB)), dim3(PTPB), 0, st, C.g.n, LV<T>::x(C));
+        vcycle<T>(c, ps, 1, pre, post, coarse);
+    };
+    const char *env = std::getenv("VFS_MG_GRAPH");
+    if (env && env[0] == '0') { direct(); return; }
+    SubGraph &sg = ps->sub[sizeof(T) == 4 ? 0 : 1];
+    if (!sg.exec || sg.pre != pre || sg.post != post || sg.coarse != coarse) {
+        if (sg.exec) { hipGraphExecDestroy(sg.exec); sg.exec = nullptr; }
+        hipGraph_t graph = nullptr;
+        if (hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) != hipSuccess) { direct(); return; }
+        direct();
+        if (hipStreamEndCapture(st, &graph) != hipSuccess || !graph) { direct(); return; }
+        hipError_t e = hipGraphInstantiate(&sg.exec, graph, nullptr, nullptr, 0);
+        hipGraphDestroy(graph);
+        if (e != hipSuccess) { sg.exec = nullptr; direct(); return; }
+        sg.pre = pre; sg.post = post; sg.coarse = coarse;
+    }
+    hipGraphLaunch(sg.exec, st);
+}
+
+template <typename T>
 static void vcycle(vfs_ctx *c, vfs_poisson *ps, int l, int pre, int post, int coarse)
 {
     hipStream_t st = c->stream;
@@ -1077,8 +1121,12 @@ static void vcycle(vfs_ctx *c, vfs_poisson *ps, int l, int pre, int post, int co
     hipLaunchKernelGGL(k_apply<T>, dim3(nb), dim3(PTPB), 0, st, L.g, LV<T>::coef(L), L.row, LV<T>::x(L), LV<T>::b(L), LV<T>::r(L));
     PLevel &C = ps->L[l + 1];
     hipLaunchKernelGGL(k_restrict<T>, dim3(blocks_for(C.g.n, PTPB)), dim3(PTPB), 0, st, C.g, L.g, C.row, L.row, LV<T>::r(L), LV<T>::b(C));
-    hipLaunchKernelGGL(k_zero<T>, dim3(blocks_for(C.g.n, PTPB)), dim3(PTPB), 0, st, C.g.n, LV<T>::x(C));
-    vcycle<T>(c, ps, l + 1, pre, post, coarse);
+    if (l == 0) {
+        coarse_subcycle<T>(c, ps, pre, post, coarse);
+    } else {
+        hipLaunchKernelGGL(k_zero<T>, dim3(blocks_for(C.g.n, PTPB)), dim3(PTPB), 0, st, C.g.n, LV<T>::x(C));
+        vcycle<T>(c, ps, l + 1, pre, post, coarse);
+    }
     hipLaunchKernelGGL(k_prolong<T>, dim3(nb), dim3(PTPB), 0, st, C.g, L.g, L.row, LV<T>::x(C), LV<T>::x(L));
     smooth<T>(c, L, post);
 }

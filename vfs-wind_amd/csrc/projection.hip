@@ -135,8 +135,10 @@ struct ProjArgs {
 __global__ void __launch_bounds__(QTPB) k_projection(ProjArgs a)
 {
     const Grid &g = a.g;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
-    const int j = blockIdx.y + 1, k = blockIdx.z + 1;
+    unsigned bx, by, bz;
+    xcd_block3(bx, by, bz);
+    const int i = bx * blockDim.x + threadIdx.x + 1;
+    const int j = by + 1, k = bz + 1;
     if (i >= g.mx - 1) return;
     const int mx = g.mx, my = g.my, mz = g.mz, pi_ = g.pi, pj = g.pj, pk = g.pk;
     const double th = a.th;

@@ -62,23 +62,26 @@ int check_hip(hipError_t e, const char *what)
     return e == hipErrorOutOfMemory ? VFS_ENOMEM : VFS_EHIP;
 }
 
-void prof_begin(vfs_ctx *c, int kernel_id)
+int prof_begin(vfs_ctx *c, int kernel_id)
 {
-    if (!c->prof_on) return;
+    if (!c->prof_on) return -1;
     size_t at = c->ev_recs->size() * 2;
     while (c->ev_pool->size() < at + 2) {
         hipEvent_t ev;
-        if (hipEventCreate(&ev) != hipSuccess) { c->prof_on = 0; return; }
+        if (hipEventCreate(&ev) != hipSuccess) { c->prof_on = 0; return -1; }
         c->ev_pool->push_back(ev);
     }
     hipEventRecord((*c->ev_pool)[at], c->stream);
     c->ev_recs->push_back({kernel_id, (int)at});
+    return (int)c->ev_recs->size() - 1;
 }
 
-void prof_end(vfs_ctx *c)
+void prof_end(vfs_ctx *c, int idx)
 {
     if (!c->prof_on || c->ev_recs->empty()) return;
-    hipEventRecord((*c->ev_pool)[c->ev_recs->back().second + 1], c->stream);
+    if (idx >= (int)c->ev_recs->size()) return;
+    const auto &r = idx < 0 ? c->ev_recs->back() : (*c->ev_recs)[idx];
+    hipEventRecord((*c->ev_pool)[r.second + 1], c->stream);
 }
 
 Grid make_grid(const vfs_params &p)
@@ -378,7 +381,7 @@ int vfs_divergence(vfs_ctx *c, double *maxdiv)
 
 static const char *k_names[vfs::K_NUMK] = {"face_fluxes",     "cell_rhs",          "line_rhs_fp32",  "line_rhs_fp64",
                                            "poisson_apply_fp64", "contra2cart",     "pressure_gradient", "eddy_viscosity",
-                                           "line_solve_fp32",  "line_solve_fp64",   "cell_fp"};
+                                           "line_solve_fp32",  "line_solve_fp64",   "cell_fp",           "formfunction2"};
 
 int vfs_profile(vfs_ctx *c, int enable)
 {

@@ -94,6 +94,29 @@ int check_hip(hipError_t e, const char *what);
 
 inline unsigned blocks_for(long long n, int threads) { return (unsigned)((n + threads - 1) / threads); }
 
+// XCD-aware block order.  The dispatcher hands workgroup b (x fastest, then
+// y, then z) to XCD b % 8, so consecutive blocks land on different XCDs and
+// their shared halos miss in each XCD's own L2.  xcd_block() maps the
+// hardware block id to a logical one such that every XCD works on one
+// contiguous range of logical blocks (a bijection on [0, nb)).
+constexpr unsigned NXCD = 8;
+__device__ inline unsigned xcd_block(unsigned b, unsigned nb)
+{
+    const unsigned q = nb / NXCD, rem = nb % NXCD, x = b % NXCD, r = b / NXCD;
+    return x * q + (x < rem ? x : rem) + r;
+}
+// the logical block of a 1D launch
+__device__ inline unsigned xcd_block1() { return xcd_block(blockIdx.x, gridDim.x); }
+// the logical (x, y, z) block of a 3D launch
+__device__ inline void xcd_block3(unsigned &bx, unsigned &by, unsigned &bz)
+{
+    const unsigned nb = gridDim.x * gridDim.y * gridDim.z;
+    const unsigned l = xcd_block(blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z), nb);
+    bx = l % gridDim.x;
+    by = (l / gridDim.x) % gridDim.y;
+    bz = l / (gridDim.x * gridDim.y);
+}
+
 }  // namespace vfs
 
 // Multigrid solver state (poisson.hip).
@@ -101,7 +124,11 @@ struct vfs_poisson;
 
 namespace vfs {
 // kernels whose launches can be timed with HIP events on the context stream
-enum KernelId { K_FACE = 0, K_CELL, K_LINE32, K_LINE64, K_APPLY64, K_C2C, K_PGRAD, K_EDDY, K_LSOLVE32, K_LSOLVE64, K_CELLFP, K_NUMK };
+enum KernelId {
+    K_FACE = 0, K_CELL, K_LINE32, K_LINE64, K_APPLY64, K_C2C, K_PGRAD, K_EDDY, K_LSOLVE32, K_LSOLVE64, K_CELLFP,
+    K_FF,  // a whole Formfunction_2 (face pass, periodic fix, both cell passes)
+    K_NUMK
+};
 }
 
 struct vfs_ctx {
@@ -152,6 +179,8 @@ int update_pressure(vfs_ctx *c);
 int projection(vfs_ctx *c);
 int divergence(vfs_ctx *c, double *out);
 // timing of individual launches (no-ops unless vfs_profile enabled it)
-void prof_begin(vfs_ctx *c, int kernel_id);
-void prof_end(vfs_ctx *c);
+// prof_begin returns the record index (-1 when profiling is off); prof_end
+// closes that record, or the most recent one when idx < 0
+int prof_begin(vfs_ctx *c, int kernel_id);
+void prof_end(vfs_ctx *c, int idx = -1);
 }  // namespace vfs

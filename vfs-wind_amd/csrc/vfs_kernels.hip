@@ -628,7 +628,7 @@ __global__ void k_eddy(Grid g, double cs2, const double *__restrict__ ucat, cons
                        const double *__restrict__ nvert, double *__restrict__ nut)
 {
     Decode d;
-    if (!decode(g, blockIdx.x * (long long)blockDim.x + threadIdx.x, d)) return;
+    if (!decode(g, xcd_block1() * (long long)blockDim.x + threadIdx.x, d)) return;
     const int i = d.i, j = d.j, k = d.k;
     const long long q = gidx(g, k, j, i);
     if (i < 1 || i >= g.mx - 1 || j < 1 || j >= g.my - 1 || k < 1 || k >= g.mz - 1 || nvert[q] > 1.1) {
@@ -759,7 +759,7 @@ __global__ void k_pgrad(Grid g, double fdiff, double fdt, double farea, const do
                         double *__restrict__ dp)
 {
     Decode d;
-    if (!decode(g, blockIdx.x * (long long)blockDim.x + threadIdx.x, d)) return;
+    if (!decode(g, xcd_block1() * (long long)blockDim.x + threadIdx.x, d)) return;
     const int i = d.i, j = d.j, k = d.k;
     const long long q = gidx(g, k, j, i);
     const int mx = g.mx, my = g.my, mz = g.mz;
