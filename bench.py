@@ -37,9 +37,10 @@ ALGO = {
     # cell metrics 9 + face metrics 30 and writes Rhs 3 (400 B/cell); the fused update adds pUcont 3 +
     # dP 3 in and Ucont 3 out (72 B/cell).  Timed as one region: face pass, its periodic fix, both cell passes.
     "formfunction2": (400 + 72, "cell"),
-    # the face pass as it runs here: ucont 24 + ucat 24 + nu_t 8 + nvert 8 + cell metrics 80 (the face
-    # metrics are rebuilt from them); 18 fluxes out 144
-    "face_fluxes": (24 + 24 + 8 + 8 + 80 + 144, "point"),
+    # the face pass as it runs on this workload (second order, face metrics rebuilt from the cell
+    # metrics, flux divergence formed in the same kernel): ucont 24 + ucat 24 + nu_t 8 + nvert 8 +
+    # cell metrics 80; Fp out 24
+    "face_fluxes": (24 + 24 + 8 + 8 + 80 + 24, "point"),
     # second-order path: 18 fluxes 144; Fp out 24
     "cell_fp": (144 + 24, "point"),
     # Fp 24 + csi/eta/zet 72 + iaj/jaj/kaj 24 + nvert 8 + dP 24 + pUcont 24; Ucont out 24

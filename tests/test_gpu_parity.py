@@ -117,6 +117,34 @@ def test_stored_face_metrics_paths(pair):
         ctx.upload(f, v)
 
 
+def test_fused_flux_divergence(pair, monkeypatch):
+    """The second-order face pass that forms Fp itself (no flux planes in
+    HBM) gives the same bits as the face pass + flux-plane periodic fix +
+    cell Fp pass, for Formfunction_2 and for a whole RK4 step."""
+    c, o, ctx = pair
+    saved = {f: ctx.download(f) for f in ("RHS", "UCONT", "UCAT", "NU_T", "DP")}
+    o.eddy_viscosity()
+    ctx.upload("NU_T", o["nu_t"])
+
+    def run():
+        for f in ("UCONT", "UCAT", "DP"):
+            ctx.upload(f, saved[f])
+        ctx.upload("RHS", np.zeros(o["rhs"].shape))
+        ctx.formfunction2(1.0)
+        rhs = ctx.download("RHS")
+        ctx.runge_kutta()
+        return rhs, ctx.download("UCONT"), ctx.download("UCAT")
+
+    fused = run()
+    monkeypatch.setenv("VFS_FUSED_FP", "0")
+    planes = run()
+    monkeypatch.delenv("VFS_FUSED_FP")
+    for a_, b_ in zip(fused, planes):
+        assert np.array_equal(a_, b_)
+    for f, v in saved.items():  # the fixture is shared by the module's tests
+        ctx.upload(f, v)
+
+
 def test_runge_kutta(pair):
     c, o, ctx = pair
     o.eddy_viscosity(); ctx.upload("NU_T", o["nu_t"])

@@ -15,6 +15,8 @@
 // a few ulps.
 #include <hip/hip_runtime.h>
 
+#include <cstdlib>
+
 #include "vfs_internal.h"
 
 #pragma clang fp contract(off)
@@ -136,6 +138,171 @@ __device__ inline void face_mean(const double a[10], const double b[10], double 
     f[9] = 2. / (1. / a[9] + 1. / b[9]);
 }
 
+// convective and viscous flux of the i face of point (i,j,k)
+// (momentum.c:668-928): i in [0,mx-2], j,k interior; zero elsewhere
+template <bool SO>
+__device__ inline void flux_i(const FaceArgs &a, const TileAcc<SO> &T, int i, int j, int k, const double *Mf, double ucf,
+                              double dv[3], double vs[3])
+{
+    const Grid &g = a.g;
+    const int mx = g.mx, my = g.my, mz = g.mz;
+    const double SOL = 0.1;  // k-omega.c:19
+    (void)SOL; (void)my; (void)mz;
+    for (int c = 0; c < 3; c++) dv[c] = vs[c] = 0;
+    if (i < mx - 1 && j >= 1 && j < my - 1 && k >= 1 && k < mz - 1) {
+        const double cs[3] = {Mf[0], Mf[1], Mf[2]};
+        const double et[3] = {Mf[3], Mf[4], Mf[5]};
+        const double ze[3] = {Mf[6], Mf[7], Mf[8]};
+        double d[3][3];
+        // Compute_du_i, k-omega.c:56-116
+        for (int c = 0; c < 3; c++) {
+            d[c][0] = T.U(c, k, j, i + 1) - T.U(c, k, j, i);
+            if (T.NV(k, j + 1, i) > SOL || T.NV(k, j + 1, i + 1) > SOL)
+                d[c][1] = (T.U(c, k, j, i + 1) + T.U(c, k, j, i) - T.U(c, k, j - 1, i + 1) - T.U(c, k, j - 1, i)) * 0.5;
+            else if (T.NV(k, j - 1, i) > SOL || T.NV(k, j - 1, i + 1) > SOL)
+                d[c][1] = (T.U(c, k, j + 1, i + 1) + T.U(c, k, j + 1, i) - T.U(c, k, j, i + 1) - T.U(c, k, j, i)) * 0.5;
+            else
+                d[c][1] = (T.U(c, k, j + 1, i + 1) + T.U(c, k, j + 1, i) - T.U(c, k, j - 1, i + 1) - T.U(c, k, j - 1, i)) * 0.25;
+            if (T.NV(k + 1, j, i) > SOL || T.NV(k + 1, j, i + 1) > SOL)
+                d[c][2] = (T.U(c, k, j, i + 1) + T.U(c, k, j, i) - T.U(c, k - 1, j, i + 1) - T.U(c, k - 1, j, i)) * 0.5;
+            else if (T.NV(k - 1, j, i) > SOL || T.NV(k - 1, j, i + 1) > SOL)
+                d[c][2] = (T.U(c, k + 1, j, i + 1) + T.U(c, k + 1, j, i) - T.U(c, k, j, i + 1) - T.U(c, k, j, i)) * 0.5;
+            else
+                d[c][2] = (T.U(c, k + 1, j, i + 1) + T.U(c, k + 1, j, i) - T.U(c, k - 1, j, i + 1) - T.U(c, k - 1, j, i)) * 0.25;
+        }
+        int iL = i - 1, iR = i + 2;
+        if (i == 0 || i == mx - 2) {
+            if (g.pi && i == mx - 2) iR = mx + 2;
+            else if (g.pi && i == 0) iL = -3;
+            else iL = i, iR = i + 1;
+        } else if (T.NV(k, j, iL) + T.NV(k, j, iR) > 0.1) iL = i, iR = i + 1;
+        if (SO) iL = i, iR = i + 1;
+        const double uc = ucf;
+        for (int c = 0; c < 3; c++) {
+            if (SO) dv[c] = -uc * 0.5 * (T.U(c, k, j, i) + T.U(c, k, j, i + 1));
+            else dv[c] = -uc * 0.0625 * (-T.U(c, k, j, iL) + 9. * T.U(c, k, j, i) + 9. * T.U(c, k, j, i + 1) - T.U(c, k, j, iR));
+        }
+        if (T.NV(k, j, i) + T.NV(k, j, i + 1) > 0.1) dv[0] = dv[1] = dv[2] = 0;
+        double nut = 0;
+        if (a.les) {
+            if ((i == 0 && !g.pi) || T.NV(k, j, i) > 0.1) nut = T.NT(k, j, i + 1);
+            else if ((i == mx - 2 && !g.pi) || T.NV(k, j, i + 1) > 0.1) nut = T.NT(k, j, i);
+            else nut = 0.5 * (T.NT(k, j, i) + T.NT(k, j, i + 1));
+        }
+        visc_face(cs, et, ze, cs, Mf[9], d, a.nu, nut, a.les, a.laplacian, vs);
+    }
+}
+
+// convective and viscous flux of the j face of point (i,j,k)
+// (momentum.c:930-1211): j in [0,my-2], i,k interior; zero elsewhere
+template <bool SO>
+__device__ inline void flux_j(const FaceArgs &a, const TileAcc<SO> &T, int i, int j, int k, const double *Mf, double ucf,
+                              double dv[3], double vs[3])
+{
+    const Grid &g = a.g;
+    const int mx = g.mx, my = g.my, mz = g.mz;
+    const double SOL = 0.1;  // k-omega.c:19
+    (void)SOL; (void)my; (void)mz;
+    for (int c = 0; c < 3; c++) dv[c] = vs[c] = 0;
+    if (i >= 1 && i < mx - 1 && j < my - 1 && k >= 1 && k < mz - 1) {
+        const double cs[3] = {Mf[0], Mf[1], Mf[2]};
+        const double et[3] = {Mf[3], Mf[4], Mf[5]};
+        const double ze[3] = {Mf[6], Mf[7], Mf[8]};
+        double d[3][3];
+        // Compute_du_j, k-omega.c:141-200
+        for (int c = 0; c < 3; c++) {
+            if (T.NV(k, j, i + 1) > SOL || T.NV(k, j + 1, i + 1) > SOL)
+                d[c][0] = (T.U(c, k, j + 1, i) + T.U(c, k, j, i) - T.U(c, k, j + 1, i - 1) - T.U(c, k, j, i - 1)) * 0.5;
+            else if (T.NV(k, j, i - 1) > SOL || T.NV(k, j + 1, i - 1) > SOL)
+                d[c][0] = (T.U(c, k, j + 1, i + 1) + T.U(c, k, j, i + 1) - T.U(c, k, j + 1, i) - T.U(c, k, j, i)) * 0.5;
+            else
+                d[c][0] = (T.U(c, k, j + 1, i + 1) + T.U(c, k, j, i + 1) - T.U(c, k, j + 1, i - 1) - T.U(c, k, j, i - 1)) * 0.25;
+            d[c][1] = T.U(c, k, j + 1, i) - T.U(c, k, j, i);
+            if (T.NV(k + 1, j, i) > SOL || T.NV(k + 1, j + 1, i) > SOL)
+                d[c][2] = (T.U(c, k, j + 1, i) + T.U(c, k, j, i) - T.U(c, k - 1, j + 1, i) - T.U(c, k - 1, j, i)) * 0.5;
+            else if (T.NV(k - 1, j, i) > SOL || T.NV(k - 1, j + 1, i) > SOL)
+                d[c][2] = (T.U(c, k + 1, j + 1, i) + T.U(c, k + 1, j, i) - T.U(c, k, j + 1, i) - T.U(c, k, j, i)) * 0.5;
+            else
+                d[c][2] = (T.U(c, k + 1, j + 1, i) + T.U(c, k + 1, j, i) - T.U(c, k - 1, j + 1, i) - T.U(c, k - 1, j, i)) * 0.25;
+        }
+        int jL = j - 1, jR = j + 2;
+        if (j == 0 || j == my - 2) {
+            if (g.pj && j == my - 2) jR = my + 2;
+            else if (g.pj && j == 0) jL = -3;
+            else jL = j, jR = j + 1;
+        } else if (T.NV(k, jL, i) + T.NV(k, jR, i) > 0.1) jL = j, jR = j + 1;
+        if (SO) jL = j, jR = j + 1;
+        const double uc = ucf;
+        for (int c = 0; c < 3; c++) {
+            if (SO) dv[c] = -uc * 0.5 * (T.U(c, k, j, i) + T.U(c, k, j + 1, i));
+            else dv[c] = -uc * 0.0625 * (-T.U(c, k, jL, i) + 9. * T.U(c, k, j, i) + 9. * T.U(c, k, j + 1, i) - T.U(c, k, jR, i));
+        }
+        if (T.NV(k, j, i) + T.NV(k, j + 1, i) > 0.1) dv[0] = dv[1] = dv[2] = 0;
+        double nut = 0;
+        if (a.les) {
+            if ((j == 0 && !g.pj) || T.NV(k, j, i) > 0.1) nut = T.NT(k, j + 1, i);
+            else if ((j == my - 2 && !g.pj) || T.NV(k, j + 1, i) > 0.1) nut = T.NT(k, j, i);
+            else nut = 0.5 * (T.NT(k, j, i) + T.NT(k, j + 1, i));
+        }
+        visc_face(cs, et, ze, et, Mf[9], d, a.nu, nut, a.les, a.laplacian, vs);
+    }
+}
+
+// convective and viscous flux of the k face of point (i,j,k)
+// (momentum.c:1212-1451): k in [0,mz-2], i,j interior; zero elsewhere
+template <bool SO>
+__device__ inline void flux_k(const FaceArgs &a, const TileAcc<SO> &T, int i, int j, int k, const double *Mf, double ucf,
+                              double dv[3], double vs[3])
+{
+    const Grid &g = a.g;
+    const int mx = g.mx, my = g.my, mz = g.mz;
+    const double SOL = 0.1;  // k-omega.c:19
+    (void)SOL; (void)my; (void)mz;
+    for (int c = 0; c < 3; c++) dv[c] = vs[c] = 0;
+    if (i >= 1 && i < mx - 1 && j >= 1 && j < my - 1 && k < mz - 1) {
+        const double cs[3] = {Mf[0], Mf[1], Mf[2]};
+        const double et[3] = {Mf[3], Mf[4], Mf[5]};
+        const double ze[3] = {Mf[6], Mf[7], Mf[8]};
+        double d[3][3];
+        // Compute_du_k, k-omega.c:227-288
+        for (int c = 0; c < 3; c++) {
+            if (T.NV(k, j, i + 1) > SOL || T.NV(k + 1, j, i + 1) > SOL)
+                d[c][0] = (T.U(c, k + 1, j, i) + T.U(c, k, j, i) - T.U(c, k + 1, j, i - 1) - T.U(c, k, j, i - 1)) * 0.5;
+            else if (T.NV(k, j, i - 1) > SOL || T.NV(k + 1, j, i - 1) > SOL)
+                d[c][0] = (T.U(c, k + 1, j, i + 1) + T.U(c, k, j, i + 1) - T.U(c, k + 1, j, i) - T.U(c, k, j, i)) * 0.5;
+            else
+                d[c][0] = (T.U(c, k + 1, j, i + 1) + T.U(c, k, j, i + 1) - T.U(c, k + 1, j, i - 1) - T.U(c, k, j, i - 1)) * 0.25;
+            if (T.NV(k, j + 1, i) > SOL || T.NV(k + 1, j + 1, i) > SOL)
+                d[c][1] = (T.U(c, k + 1, j, i) + T.U(c, k, j, i) - T.U(c, k + 1, j - 1, i) - T.U(c, k, j - 1, i)) * 0.5;
+            else if (T.NV(k, j - 1, i) > SOL || T.NV(k + 1, j - 1, i) > SOL)
+                d[c][1] = (T.U(c, k + 1, j + 1, i) + T.U(c, k, j + 1, i) - T.U(c, k + 1, j, i) - T.U(c, k, j, i)) * 0.5;
+            else
+                d[c][1] = (T.U(c, k + 1, j + 1, i) + T.U(c, k, j + 1, i) - T.U(c, k + 1, j - 1, i) - T.U(c, k, j - 1, i)) * 0.25;
+            d[c][2] = T.U(c, k + 1, j, i) - T.U(c, k, j, i);
+        }
+        int kL = k - 1, kR = k + 2;
+        if (k == 0 || k == mz - 2) {
+            if (g.pk && k == mz - 2) kR = mz + 2;
+            else if (g.pk && k == 0) kL = -3;
+            else kL = k, kR = k + 1;
+        } else if (T.NV(kL, j, i) + T.NV(kR, j, i) > 0.1) kL = k, kR = k + 1;
+        if (SO) kL = k, kR = k + 1;
+        const double uc = ucf;
+        for (int c = 0; c < 3; c++) {
+            if (SO) dv[c] = -uc * 0.5 * (T.U(c, k, j, i) + T.U(c, k + 1, j, i));
+            else dv[c] = -uc * 0.0625 * (-T.U(c, kL, j, i) + 9. * T.U(c, k, j, i) + 9. * T.U(c, k + 1, j, i) - T.U(c, kR, j, i));
+        }
+        if (T.NV(k, j, i) + T.NV(k + 1, j, i) > 0.1) dv[0] = dv[1] = dv[2] = 0;
+        double nut = 0;
+        if (a.les) {
+            if ((k == 0 && !g.pk) || T.NV(k, j, i) > 0.1) nut = T.NT(k + 1, j, i);
+            else if ((k == mz - 2 && !g.pk) || T.NV(k + 1, j, i) > 0.1) nut = T.NT(k, j, i);
+            else nut = 0.5 * (T.NT(k, j, i) + T.NT(k + 1, j, i));
+        }
+        visc_face(cs, et, ze, ze, Mf[9], d, a.nu, nut, a.les, a.laplacian, vs);
+    }
+}
+
 // the 18 face fluxes of point (i,j,k) (momentum.c:668-1451); SO selects the
 // second-order convection at compile time (a.second_order is then 1).  With
 // DER the face metrics are rebuilt from the cell metrics: `cur` holds the
@@ -170,147 +337,13 @@ __device__ inline void face_point(const FaceArgs &a, const TileAcc<SO> &T, int i
         load_metrics(a.fk, q, M[2]);
     }
 
-    // ---- i face (momentum.c:668-928): i in [0,mx-2], j,k interior
-    {
-        double dv[3] = {0, 0, 0}, vs[3] = {0, 0, 0};
-        if (i < mx - 1 && j >= 1 && j < my - 1 && k >= 1 && k < mz - 1) {
-            const double cs[3] = {M[0][0], M[0][1], M[0][2]};
-            const double et[3] = {M[0][3], M[0][4], M[0][5]};
-            const double ze[3] = {M[0][6], M[0][7], M[0][8]};
-            double d[3][3];
-            // Compute_du_i, k-omega.c:56-116
-            for (int c = 0; c < 3; c++) {
-                d[c][0] = T.U(c, k, j, i + 1) - T.U(c, k, j, i);
-                if (T.NV(k, j + 1, i) > SOL || T.NV(k, j + 1, i + 1) > SOL)
-                    d[c][1] = (T.U(c, k, j, i + 1) + T.U(c, k, j, i) - T.U(c, k, j - 1, i + 1) - T.U(c, k, j - 1, i)) * 0.5;
-                else if (T.NV(k, j - 1, i) > SOL || T.NV(k, j - 1, i + 1) > SOL)
-                    d[c][1] = (T.U(c, k, j + 1, i + 1) + T.U(c, k, j + 1, i) - T.U(c, k, j, i + 1) - T.U(c, k, j, i)) * 0.5;
-                else
-                    d[c][1] = (T.U(c, k, j + 1, i + 1) + T.U(c, k, j + 1, i) - T.U(c, k, j - 1, i + 1) - T.U(c, k, j - 1, i)) * 0.25;
-                if (T.NV(k + 1, j, i) > SOL || T.NV(k + 1, j, i + 1) > SOL)
-                    d[c][2] = (T.U(c, k, j, i + 1) + T.U(c, k, j, i) - T.U(c, k - 1, j, i + 1) - T.U(c, k - 1, j, i)) * 0.5;
-                else if (T.NV(k - 1, j, i) > SOL || T.NV(k - 1, j, i + 1) > SOL)
-                    d[c][2] = (T.U(c, k + 1, j, i + 1) + T.U(c, k + 1, j, i) - T.U(c, k, j, i + 1) - T.U(c, k, j, i)) * 0.5;
-                else
-                    d[c][2] = (T.U(c, k + 1, j, i + 1) + T.U(c, k + 1, j, i) - T.U(c, k - 1, j, i + 1) - T.U(c, k - 1, j, i)) * 0.25;
-            }
-            int iL = i - 1, iR = i + 2;
-            if (i == 0 || i == mx - 2) {
-                if (g.pi && i == mx - 2) iR = mx + 2;
-                else if (g.pi && i == 0) iL = -3;
-                else iL = i, iR = i + 1;
-            } else if (T.NV(k, j, iL) + T.NV(k, j, iR) > 0.1) iL = i, iR = i + 1;
-            if (SO) iL = i, iR = i + 1;
-            const double uc = UC[0];
-            for (int c = 0; c < 3; c++) {
-                if (SO) dv[c] = -uc * 0.5 * (T.U(c, k, j, i) + T.U(c, k, j, i + 1));
-                else dv[c] = -uc * 0.0625 * (-T.U(c, k, j, iL) + 9. * T.U(c, k, j, i) + 9. * T.U(c, k, j, i + 1) - T.U(c, k, j, iR));
-            }
-            if (T.NV(k, j, i) + T.NV(k, j, i + 1) > 0.1) dv[0] = dv[1] = dv[2] = 0;
-            double nut = 0;
-            if (a.les) {
-                if ((i == 0 && !g.pi) || T.NV(k, j, i) > 0.1) nut = T.NT(k, j, i + 1);
-                else if ((i == mx - 2 && !g.pi) || T.NV(k, j, i + 1) > 0.1) nut = T.NT(k, j, i);
-                else nut = 0.5 * (T.NT(k, j, i) + T.NT(k, j, i + 1));
-            }
-            visc_face(cs, et, ze, cs, M[0][9], d, a.nu, nut, a.les, a.laplacian, vs);
-        }
-        for (int c = 0; c < 3; c++) { a.out[(0 + c) * n + q] = dv[c]; a.out[(9 + c) * n + q] = vs[c]; }
-    }
-    // ---- j face (momentum.c:930-1211): j in [0,my-2], i,k interior
-    {
-        double dv[3] = {0, 0, 0}, vs[3] = {0, 0, 0};
-        if (i >= 1 && i < mx - 1 && j < my - 1 && k >= 1 && k < mz - 1) {
-            const double cs[3] = {M[1][0], M[1][1], M[1][2]};
-            const double et[3] = {M[1][3], M[1][4], M[1][5]};
-            const double ze[3] = {M[1][6], M[1][7], M[1][8]};
-            double d[3][3];
-            // Compute_du_j, k-omega.c:141-200
-            for (int c = 0; c < 3; c++) {
-                if (T.NV(k, j, i + 1) > SOL || T.NV(k, j + 1, i + 1) > SOL)
-                    d[c][0] = (T.U(c, k, j + 1, i) + T.U(c, k, j, i) - T.U(c, k, j + 1, i - 1) - T.U(c, k, j, i - 1)) * 0.5;
-                else if (T.NV(k, j, i - 1) > SOL || T.NV(k, j + 1, i - 1) > SOL)
-                    d[c][0] = (T.U(c, k, j + 1, i + 1) + T.U(c, k, j, i + 1) - T.U(c, k, j + 1, i) - T.U(c, k, j, i)) * 0.5;
-                else
-                    d[c][0] = (T.U(c, k, j + 1, i + 1) + T.U(c, k, j, i + 1) - T.U(c, k, j + 1, i - 1) - T.U(c, k, j, i - 1)) * 0.25;
-                d[c][1] = T.U(c, k, j + 1, i) - T.U(c, k, j, i);
-                if (T.NV(k + 1, j, i) > SOL || T.NV(k + 1, j + 1, i) > SOL)
-                    d[c][2] = (T.U(c, k, j + 1, i) + T.U(c, k, j, i) - T.U(c, k - 1, j + 1, i) - T.U(c, k - 1, j, i)) * 0.5;
-                else if (T.NV(k - 1, j, i) > SOL || T.NV(k - 1, j + 1, i) > SOL)
-                    d[c][2] = (T.U(c, k + 1, j + 1, i) + T.U(c, k + 1, j, i) - T.U(c, k, j + 1, i) - T.U(c, k, j, i)) * 0.5;
-                else
-                    d[c][2] = (T.U(c, k + 1, j + 1, i) + T.U(c, k + 1, j, i) - T.U(c, k - 1, j + 1, i) - T.U(c, k - 1, j, i)) * 0.25;
-            }
-            int jL = j - 1, jR = j + 2;
-            if (j == 0 || j == my - 2) {
-                if (g.pj && j == my - 2) jR = my + 2;
-                else if (g.pj && j == 0) jL = -3;
-                else jL = j, jR = j + 1;
-            } else if (T.NV(k, jL, i) + T.NV(k, jR, i) > 0.1) jL = j, jR = j + 1;
-            if (SO) jL = j, jR = j + 1;
-            const double uc = UC[1];
-            for (int c = 0; c < 3; c++) {
-                if (SO) dv[c] = -uc * 0.5 * (T.U(c, k, j, i) + T.U(c, k, j + 1, i));
-                else dv[c] = -uc * 0.0625 * (-T.U(c, k, jL, i) + 9. * T.U(c, k, j, i) + 9. * T.U(c, k, j + 1, i) - T.U(c, k, jR, i));
-            }
-            if (T.NV(k, j, i) + T.NV(k, j + 1, i) > 0.1) dv[0] = dv[1] = dv[2] = 0;
-            double nut = 0;
-            if (a.les) {
-                if ((j == 0 && !g.pj) || T.NV(k, j, i) > 0.1) nut = T.NT(k, j + 1, i);
-                else if ((j == my - 2 && !g.pj) || T.NV(k, j + 1, i) > 0.1) nut = T.NT(k, j, i);
-                else nut = 0.5 * (T.NT(k, j, i) + T.NT(k, j + 1, i));
-            }
-            visc_face(cs, et, ze, et, M[1][9], d, a.nu, nut, a.les, a.laplacian, vs);
-        }
-        for (int c = 0; c < 3; c++) { a.out[(3 + c) * n + q] = dv[c]; a.out[(12 + c) * n + q] = vs[c]; }
-    }
-    // ---- k face (momentum.c:1212-1451): k in [0,mz-2], i,j interior
-    {
-        double dv[3] = {0, 0, 0}, vs[3] = {0, 0, 0};
-        if (i >= 1 && i < mx - 1 && j >= 1 && j < my - 1 && k < mz - 1) {
-            const double cs[3] = {M[2][0], M[2][1], M[2][2]};
-            const double et[3] = {M[2][3], M[2][4], M[2][5]};
-            const double ze[3] = {M[2][6], M[2][7], M[2][8]};
-            double d[3][3];
-            // Compute_du_k, k-omega.c:227-288
-            for (int c = 0; c < 3; c++) {
-                if (T.NV(k, j, i + 1) > SOL || T.NV(k + 1, j, i + 1) > SOL)
-                    d[c][0] = (T.U(c, k + 1, j, i) + T.U(c, k, j, i) - T.U(c, k + 1, j, i - 1) - T.U(c, k, j, i - 1)) * 0.5;
-                else if (T.NV(k, j, i - 1) > SOL || T.NV(k + 1, j, i - 1) > SOL)
-                    d[c][0] = (T.U(c, k + 1, j, i + 1) + T.U(c, k, j, i + 1) - T.U(c, k + 1, j, i) - T.U(c, k, j, i)) * 0.5;
-                else
-                    d[c][0] = (T.U(c, k + 1, j, i + 1) + T.U(c, k, j, i + 1) - T.U(c, k + 1, j, i - 1) - T.U(c, k, j, i - 1)) * 0.25;
-                if (T.NV(k, j + 1, i) > SOL || T.NV(k + 1, j + 1, i) > SOL)
-                    d[c][1] = (T.U(c, k + 1, j, i) + T.U(c, k, j, i) - T.U(c, k + 1, j - 1, i) - T.U(c, k, j - 1, i)) * 0.5;
-                else if (T.NV(k, j - 1, i) > SOL || T.NV(k + 1, j - 1, i) > SOL)
-                    d[c][1] = (T.U(c, k + 1, j + 1, i) + T.U(c, k, j + 1, i) - T.U(c, k + 1, j, i) - T.U(c, k, j, i)) * 0.5;
-                else
-                    d[c][1] = (T.U(c, k + 1, j + 1, i) + T.U(c, k, j + 1, i) - T.U(c, k + 1, j - 1, i) - T.U(c, k, j - 1, i)) * 0.25;
-                d[c][2] = T.U(c, k + 1, j, i) - T.U(c, k, j, i);
-            }
-            int kL = k - 1, kR = k + 2;
-            if (k == 0 || k == mz - 2) {
-                if (g.pk && k == mz - 2) kR = mz + 2;
-                else if (g.pk && k == 0) kL = -3;
-                else kL = k, kR = k + 1;
-            } else if (T.NV(kL, j, i) + T.NV(kR, j, i) > 0.1) kL = k, kR = k + 1;
-            if (SO) kL = k, kR = k + 1;
-            const double uc = UC[2];
-            for (int c = 0; c < 3; c++) {
-                if (SO) dv[c] = -uc * 0.5 * (T.U(c, k, j, i) + T.U(c, k + 1, j, i));
-                else dv[c] = -uc * 0.0625 * (-T.U(c, kL, j, i) + 9. * T.U(c, k, j, i) + 9. * T.U(c, k + 1, j, i) - T.U(c, kR, j, i));
-            }
-            if (T.NV(k, j, i) + T.NV(k + 1, j, i) > 0.1) dv[0] = dv[1] = dv[2] = 0;
-            double nut = 0;
-            if (a.les) {
-                if ((k == 0 && !g.pk) || T.NV(k, j, i) > 0.1) nut = T.NT(k + 1, j, i);
-                else if ((k == mz - 2 && !g.pk) || T.NV(k + 1, j, i) > 0.1) nut = T.NT(k, j, i);
-                else nut = 0.5 * (T.NT(k, j, i) + T.NT(k + 1, j, i));
-            }
-            visc_face(cs, et, ze, ze, M[2][9], d, a.nu, nut, a.les, a.laplacian, vs);
-        }
-        for (int c = 0; c < 3; c++) { a.out[(6 + c) * n + q] = dv[c]; a.out[(15 + c) * n + q] = vs[c]; }
-    }
+    double dv[3], vs[3];
+    flux_i<SO>(a, T, i, j, k, M[0], UC[0], dv, vs);
+    for (int c = 0; c < 3; c++) { a.out[(0 + c) * n + q] = dv[c]; a.out[(9 + c) * n + q] = vs[c]; }
+    flux_j<SO>(a, T, i, j, k, M[1], UC[1], dv, vs);
+    for (int c = 0; c < 3; c++) { a.out[(3 + c) * n + q] = dv[c]; a.out[(12 + c) * n + q] = vs[c]; }
+    flux_k<SO>(a, T, i, j, k, M[2], UC[2], dv, vs);
+    for (int c = 0; c < 3; c++) { a.out[(6 + c) * n + q] = dv[c]; a.out[(15 + c) * n + q] = vs[c]; }
 }
 
 // One block owns an FT_I x FT_J column of points and marches FT_K planes in
@@ -374,6 +407,145 @@ __global__ void __launch_bounds__(FT_I * FT_J) k_face_fluxes(FaceArgs a)
                 for (int c = 0; c < 10; c++) cur[c] = nk[c];
                 have = true;
             }
+        }
+        __syncthreads();
+        slot0 = slot0 + 1 < 3 ? slot0 + 1 : 0;
+    }
+}
+
+// The face pass fused with the flux divergence (momentum.c:1548-1678) for
+// second-order convection with the face metrics rebuilt from the cell
+// metrics.  A block owns an FT_I x FT_J column and marches in k as
+// k_face_fluxes does; on each plane the i- and j-face fluxes of its points go
+// to LDS together with one extra column (the i face at i0-1) and one extra row
+// (the j face at j0-1), and the k-face flux of the plane below stays in
+// registers, so Fp = (div_i(i) - div_i(i-1) + ... ) + (visc ...) is formed in
+// the reference's order without the 18 flux planes ever reaching HBM.  A
+// block starting above k = 0 first computes the k face of plane k0-1 only.
+// Periodic ghost cells of the face fluxes are copies of the interior faces
+// they stand for (momentum.c:1504-1546); with the periodic ghosts of ucat,
+// nu_t, nvert, ucont and the metrics holding their periodic copies (as
+// Contra2Cart, the eddy viscosity, Formfunction_2's own Ucont fix and
+// FormMetrics leave them) the faces at index 0 computed here are bitwise
+// those copies, so that fix is not applied.  Fp is written for every point of
+// the block (zero at ghosts); the periodic ghosts are filled afterwards.
+constexpr int FX_I = FT_I + 1, FX_J = FT_J + 1;
+__global__ void __launch_bounds__(FT_I * FT_J) k_face_fp(FaceArgs a, double *__restrict__ fp)
+{
+    using D = TileDims<true>;
+    __shared__ double sx[3 * D::HP], sy[3 * D::HP], sz[3 * D::HP], st[3 * D::HP];
+    __shared__ float sn[3 * D::HP];
+    __shared__ double fxi[6][FT_J][FX_I];  // i-face div xyz, visc xyz; column 0 = face i0-1
+    __shared__ double fxj[6][FX_J][FT_I];  // j-face div xyz, visc xyz; row 0 = face j0-1
+    const Grid &g = a.g;
+    const int nbi = (g.mx + FT_I - 1) / FT_I, nbj = (g.my + FT_J - 1) / FT_J;
+    int b = blockIdx.x;
+    const int bi = b % nbi;
+    b /= nbi;
+    const int bj = b % nbj, bk = b / nbj;
+    const int i0 = bi * FT_I, j0 = bj * FT_J, k0 = bk * FT_K;
+    const int k1 = k0 + FT_K < g.mz ? k0 + FT_K : g.mz;
+    const int tx = threadIdx.x % FT_I, ty = threadIdx.x / FT_I;
+    const int i = i0 + tx, j = j0 + ty;
+    const bool inb = i < g.mx && j < g.my;
+    const long long n = g.n;
+    const bool les = a.les != 0;
+    auto load_plane = [&](int kk, int slot) {
+        const int wk = wrapi(kk, g.mz, g.pk);
+        for (int p = threadIdx.x; p < D::HP; p += FT_I * FT_J) {
+            const int di = p % D::HI, dj = p / D::HI;
+            const int wi = wrapi(i0 - 1 + di, g.mx, g.pi), wj = wrapi(j0 - 1 + dj, g.my, g.pj);
+            double vx = 0, vy = 0, vz = 0, vt = 0;
+            float vn = 0;
+            if (wi >= 0 && wi < g.mx && wj >= 0 && wj < g.my && wk >= 0 && wk < g.mz) {
+                const long long q = ((long long)wk * g.my + wj) * g.mx + wi;
+                vx = a.ucat[3 * q]; vy = a.ucat[3 * q + 1]; vz = a.ucat[3 * q + 2];
+                vn = (float)a.nvert[q];
+                if (les) vt = a.nut[q];
+            }
+            const int l = slot * D::HP + p;
+            sx[l] = vx; sy[l] = vy; sz[l] = vz; sn[l] = vn; st[l] = vt;
+        }
+    };
+    const int kstart = k0 > 0 ? k0 - 1 : k0;
+    load_plane(kstart - 1, 0);
+    load_plane(kstart, 1);
+    int slot0 = 0;
+    double cur[10], nk[10], kprev[6];
+    for (int c = 0; c < 6; c++) kprev[c] = 0;
+    bool have = false;
+    for (int k = kstart; k < k1; k++) {
+        int s2 = slot0 + 2;
+        if (s2 >= 3) s2 -= 3;
+        load_plane(k + 1, s2);
+        __syncthreads();
+        TileAcc<true> T{&a, sx, sy, sz, st, sn, i0, j0, k, slot0};
+        const bool full = k >= k0;  // the plane below the block: its k face only
+        const long long q = ((long long)k * g.my + j) * g.mx + i;
+        double kf[6] = {0, 0, 0, 0, 0, 0};
+        if (inb) {
+            if (!have) load_metrics(a.cell, q, cur);
+            load_metrics(a.cell, k < g.mz - 1 ? q + g.plane : q, nk);
+            double M[10];
+            face_mean(cur, nk, M);
+            flux_k<true>(a, T, i, j, k, M, a.ucont[3 * q + 2], kf, kf + 3);
+        }
+        if (full) {
+            double f6[6] = {0, 0, 0, 0, 0, 0};
+            if (inb) {
+                double nb_[10], M[10];
+                load_metrics(a.cell, i < g.mx - 1 ? q + 1 : q, nb_);
+                face_mean(cur, nb_, M);
+                flux_i<true>(a, T, i, j, k, M, a.ucont[3 * q + 0], f6, f6 + 3);
+            }
+            for (int c = 0; c < 6; c++) fxi[c][ty][tx + 1] = f6[c];
+            for (int c = 0; c < 6; c++) f6[c] = 0;
+            if (inb) {
+                double nb_[10], M[10];
+                load_metrics(a.cell, j < g.my - 1 ? q + g.mx : q, nb_);
+                face_mean(cur, nb_, M);
+                flux_j<true>(a, T, i, j, k, M, a.ucont[3 * q + 1], f6, f6 + 3);
+            }
+            for (int c = 0; c < 6; c++) fxj[c][ty + 1][tx] = f6[c];
+            if (tx == 0) {  // i face at i0-1 of this row
+                for (int c = 0; c < 6; c++) f6[c] = 0;
+                if (i0 >= 1 && j < g.my) {
+                    const long long p = q - 1;
+                    double cp[10], M[10];
+                    load_metrics(a.cell, p, cp);
+                    face_mean(cp, cur, M);
+                    flux_i<true>(a, T, i0 - 1, j, k, M, a.ucont[3 * p + 0], f6, f6 + 3);
+                }
+                for (int c = 0; c < 6; c++) fxi[c][ty][0] = f6[c];
+            }
+            if (ty == 0) {  // j face at j0-1 of this column
+                for (int c = 0; c < 6; c++) f6[c] = 0;
+                if (j0 >= 1 && i < g.mx) {
+                    const long long p = q - g.mx;
+                    double cp[10], M[10];
+                    load_metrics(a.cell, p, cp);
+                    face_mean(cp, cur, M);
+                    flux_j<true>(a, T, i, j0 - 1, k, M, a.ucont[3 * p + 1], f6, f6 + 3);
+                }
+                for (int c = 0; c < 6; c++) fxj[c][0][tx] = f6[c];
+            }
+        }
+        __syncthreads();
+        if (full && inb) {
+            double f[3] = {0, 0, 0};
+            if (i >= 1 && i < g.mx - 1 && j >= 1 && j < g.my - 1 && k >= 1 && k < g.mz - 1) {
+                for (int c = 0; c < 3; c++) {
+                    f[c] = fxi[c][ty][tx + 1] - fxi[c][ty][tx] + fxj[c][ty + 1][tx] - fxj[c][ty][tx] + kf[c] - kprev[c];
+                    f[c] += (fxi[3 + c][ty][tx + 1] - fxi[3 + c][ty][tx] + fxj[3 + c][ty + 1][tx] - fxj[3 + c][ty][tx] +
+                             kf[3 + c] - kprev[3 + c]) / 1.0;
+                }
+            }
+            for (int c = 0; c < 3; c++) fp[c * n + q] = f[c];
+        }
+        for (int c = 0; c < 6; c++) kprev[c] = kf[c];
+        if (inb) {
+            for (int c = 0; c < 10; c++) cur[c] = nk[c];
+            have = true;
         }
         __syncthreads();
         slot0 = slot0 + 1 < 3 ? slot0 + 1 : 0;
@@ -627,22 +799,31 @@ int formfunction2(vfs_ctx *c, double scale, const double *pucont, double rk_alph
     fa.second_order = c->p.second_order;
     fa.laplacian = c->p.laplacian;
     fa.out = c->face;
-    prof_begin(c, K_FACE);
-    {
-        const unsigned nb = (unsigned)(((g.mx + FT_I - 1) / FT_I) * ((g.my + FT_J - 1) / FT_J) * ((g.mz + FT_K - 1) / FT_K));
-        if (fa.second_order && c->face_derived)
-            hipLaunchKernelGGL((k_face_fluxes<true, true>), dim3(nb), dim3(FT_I * FT_J), 0, st, fa);
-        else if (fa.second_order)
-            hipLaunchKernelGGL((k_face_fluxes<true, false>), dim3(nb), dim3(FT_I * FT_J), 0, st, fa);
-        else
-            hipLaunchKernelGGL((k_face_fluxes<false, false>), dim3(nb), dim3(FT_I * FT_J), 0, st, fa);
-    }
-    prof_end(c);
-    VFS_LAUNCH_CHECK("face fluxes");
+    const unsigned nbt = (unsigned)(((g.mx + FT_I - 1) / FT_I) * ((g.my + FT_J - 1) / FT_J) * ((g.mz + FT_K - 1) / FT_K));
     const long long np[3] = {2LL * g.my * g.mz, 2LL * g.mx * g.mz, 2LL * g.mx * g.my};
     const int per[3] = {g.pi, g.pj, g.pk};
-    for (int d = 0; d < 3; d++)
-        if (per[d]) hipLaunchKernelGGL(k_periodic_soa, dim3(blocks_for(np[d], FF_TPB)), dim3(FF_TPB), 0, st, g, c->face, 18, d);
+    // VFS_FUSED_FP=0 keeps the flux planes (the face pass, then k_cell_fp)
+    const char *env = std::getenv("VFS_FUSED_FP");
+    const bool fused = fa.second_order && c->face_derived && !(env && env[0] == '0');
+    prof_begin(c, K_FACE);
+    if (fused)
+        hipLaunchKernelGGL(k_face_fp, dim3(nbt), dim3(FT_I * FT_J), 0, st, fa, c->fpc);
+    else if (fa.second_order && c->face_derived)
+        hipLaunchKernelGGL((k_face_fluxes<true, true>), dim3(nbt), dim3(FT_I * FT_J), 0, st, fa);
+    else if (fa.second_order)
+        hipLaunchKernelGGL((k_face_fluxes<true, false>), dim3(nbt), dim3(FT_I * FT_J), 0, st, fa);
+    else
+        hipLaunchKernelGGL((k_face_fluxes<false, false>), dim3(nbt), dim3(FT_I * FT_J), 0, st, fa);
+    prof_end(c);
+    VFS_LAUNCH_CHECK("face fluxes");
+    if (fused) {
+        // Fp of the periodic ghost cells: copies of the cells they stand for
+        for (int d = 0; d < 3; d++)
+            if (per[d]) hipLaunchKernelGGL(k_periodic_soa, dim3(blocks_for(np[d], FF_TPB)), dim3(FF_TPB), 0, st, g, c->fpc, 3, d);
+    } else {
+        for (int d = 0; d < 3; d++)
+            if (per[d]) hipLaunchKernelGGL(k_periodic_soa, dim3(blocks_for(np[d], FF_TPB)), dim3(FF_TPB), 0, st, g, c->face, 18, d);
+    }
     VFS_LAUNCH_CHECK("face periodic");
     CellArgs ca;
     ca.g = g;
@@ -661,13 +842,14 @@ int formfunction2(vfs_ctx *c, double scale, const double *pucont, double rk_alph
     ca.pucont = pucont;
     ca.alpha = rk_alpha;
     ca.ucont_out = ucont_out;
-    const unsigned nbt = (unsigned)(((g.mx + FT_I - 1) / FT_I) * ((g.my + FT_J - 1) / FT_J) * ((g.mz + FT_K - 1) / FT_K));
-    prof_begin(c, K_CELLFP);
-    if (ca.second_order)
-        hipLaunchKernelGGL(k_cell_fp<true>, dim3(nbt), dim3(FT_I * FT_J), 0, st, ca, c->fpc);
-    else
-        hipLaunchKernelGGL(k_cell_fp<false>, dim3(nbt), dim3(FT_I * FT_J), 0, st, ca, c->fpc);
-    prof_end(c);
+    if (!fused) {
+        prof_begin(c, K_CELLFP);
+        if (ca.second_order)
+            hipLaunchKernelGGL(k_cell_fp<true>, dim3(nbt), dim3(FT_I * FT_J), 0, st, ca, c->fpc);
+        else
+            hipLaunchKernelGGL(k_cell_fp<false>, dim3(nbt), dim3(FT_I * FT_J), 0, st, ca, c->fpc);
+        prof_end(c);
+    }
     prof_begin(c, K_CELL);
     hipLaunchKernelGGL(k_cell_rhs, dim3(nbt), dim3(FT_I * FT_J), 0, st, ca, (const double *)c->fpc);
     prof_end(c);
