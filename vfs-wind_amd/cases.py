@@ -39,8 +39,8 @@ class Case:
 
 
 def _face_areas(coor: np.ndarray):
-    """Area vectors of the i, j and k node planes (the cross products of the
-    face diagonals, as the node-plane metrics of metrics.c:116-185)."""
+    """Area vectors of the i, j and k node-plane faces (cross products of the
+    face's averaged edge vectors); used only to build the initial fluxes."""
     X = coor
     # i faces between cells i and i+1 at node plane i: corners (k,j),(k-1,j),(k,j-1),(k-1,j-1)
     a = X[1:, 1:, :, :]

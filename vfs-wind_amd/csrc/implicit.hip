@@ -215,22 +215,35 @@ struct ImpState {
     long long n3 = 0;
     int m = 0;
     double *U = nullptr, *R = nullptr, *W = nullptr, *FW = nullptr, *dx = nullptr, *V = nullptr, *h = nullptr;
-    std::vector<void *> allocs;
 };
 
-static int im_alloc(ImpState &s, void **p, size_t bytes)
+// Carve the solver's vectors out of the context's persistent workspace
+// (5 + m + 1 vectors of n3 doubles and IM_MAXV scalars), allocating or
+// growing it first when needed.  Every vector is written before it is read.
+static int im_workspace(vfs_ctx *c, ImpState &s)
 {
-    hipError_t e = hipMalloc(p, bytes);
-    if (e != hipSuccess) return check_hip(e, "hipMalloc (implicit)");
-    s.allocs.push_back(*p);
-    e = hipMemset(*p, 0, bytes);
-    return e == hipSuccess ? VFS_OK : check_hip(e, "hipMemset (implicit)");
-}
-
-static void im_free(ImpState &s)
-{
-    for (void *p : s.allocs) hipFree(p);
-    s.allocs.clear();
+    const size_t vb = (size_t)s.n3 * sizeof(double);
+    const size_t need = (size_t)(6 + s.m) * vb + IM_MAXV * sizeof(double);
+    if (c->imp_ws_bytes < need) {
+        VFS_HIP(hipStreamSynchronize(c->stream));
+        if (c->imp_ws) hipFree(c->imp_ws);
+        c->imp_ws = nullptr;
+        c->imp_ws_bytes = 0;
+        hipError_t e = hipMalloc((void **)&c->imp_ws, need);
+        if (e != hipSuccess) { c->imp_ws = nullptr; return check_hip(e, "hipMalloc (implicit)"); }
+        e = hipMemsetAsync(c->imp_ws, 0, need, c->stream);
+        if (e != hipSuccess) return check_hip(e, "hipMemsetAsync (implicit)");
+        c->imp_ws_bytes = need;
+    }
+    double *p = c->imp_ws;
+    s.U = p; p += s.n3;
+    s.R = p; p += s.n3;
+    s.W = p; p += s.n3;
+    s.FW = p; p += s.n3;
+    s.dx = p; p += s.n3;
+    s.V = p; p += (size_t)(s.m + 1) * s.n3;
+    s.h = p;
+    return VFS_OK;
 }
 
 // R(u) into r (FormFunction_SNES); leaves UCONT/UCAT at the BC-zeroed u
@@ -438,19 +451,12 @@ int vfs_implicit_matrixfree(vfs_ctx *c, const vfs_newton_opts *o, vfs_newton_sta
     s.n3 = 3 * c->g.n;
     s.m = m;
     const size_t vb = (size_t)s.n3 * sizeof(double);
-    int rc = 0;
-    rc = rc ? rc : im_alloc(s, (void **)&s.U, vb);
-    rc = rc ? rc : im_alloc(s, (void **)&s.R, vb);
-    rc = rc ? rc : im_alloc(s, (void **)&s.W, vb);
-    rc = rc ? rc : im_alloc(s, (void **)&s.FW, vb);
-    rc = rc ? rc : im_alloc(s, (void **)&s.dx, vb);
-    rc = rc ? rc : im_alloc(s, (void **)&s.V, (size_t)(m + 1) * vb);
-    rc = rc ? rc : im_alloc(s, (void **)&s.h, IM_MAXV * sizeof(double));
-    if (rc) { im_free(s); return rc; }
+    int rc = im_workspace(c, s);
+    if (rc) return rc;
     hipStream_t st = c->stream;
     const long long n = s.n3;
     const unsigned nb = blocks_for(n, IM_TPB);
-    auto fail = [&](int code) { im_free(s); return code; };
+    auto fail = [&](int code) { return code; };
     // InflowFlux, then FormBCS with outflow_scale = 0 (implicitsolver.c:4293-4297)
     rc = inflow_flux(c, nullptr);
     if (!rc) rc = form_bcs(c, 0);
@@ -515,7 +521,6 @@ int vfs_implicit_matrixfree(vfs_ctx *c, const vfs_newton_opts *o, vfs_newton_sta
     rc = form_bcs(c, 1);
     if (rc) return fail(rc);
     VFS_HIP(hipStreamSynchronize(st));
-    im_free(s);
     if (stats) {
         stats->newton_iterations = newton;
         stats->linear_iterations = lin_total;

@@ -116,6 +116,8 @@ int vfs_ctx_create(const vfs_params *params, vfs_ctx **out)
     c->mean_k_area = 0;
     c->mean_k_flux = 0;
     c->face_derived = false;
+    c->imp_ws = nullptr;
+    c->imp_ws_bytes = 0;
     c->prof_on = 0;
     c->ev_pool = new std::vector<hipEvent_t>();
     c->ev_recs = new std::vector<std::pair<int, int>>();
@@ -160,6 +162,7 @@ int vfs_ctx_destroy(vfs_ctx *c)
     if (c->fpc) hipFree(c->fpc);
     if (c->red) hipFree(c->red);
     if (c->red_host) hipHostFree(c->red_host);
+    if (c->imp_ws) hipFree(c->imp_ws);
     if (c->ev_pool) {
         for (hipEvent_t ev : *c->ev_pool) hipEventDestroy(ev);
         delete c->ev_pool;

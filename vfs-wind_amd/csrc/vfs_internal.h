@@ -149,6 +149,10 @@ struct vfs_ctx {
     // derived (no metric array written since): kernels may then rebuild a
     // face metric from the two cell metrics instead of reading it
     bool face_derived;
+    // Newton-GMRES vectors of the implicit momentum solve, kept between calls
+    // (allocated on first use, grown when a longer restart asks for more)
+    double *imp_ws;
+    size_t imp_ws_bytes;
     vfs_poisson *ps;
     // per-kernel timing (vfs_profile): event pairs recorded around launches
     int prof_on;
