@@ -39,12 +39,13 @@ ALGO = {
     "formfunction2": (400 + 72, "cell"),
     # the face pass as it runs on this workload (second order, face metrics rebuilt from the cell
     # metrics, flux divergence formed in the same kernel): ucont 24 + ucat 24 + nu_t 8 + nvert 8 +
-    # cell metrics 80; Fp out 24
+    # cell metrics 80; csi.Fp, eta.Fp, zet.Fp out 24
     "face_fluxes": (24 + 24 + 8 + 8 + 80 + 24, "point"),
-    # second-order path: 18 fluxes 144; Fp out 24
+    # (the path with flux planes) 18 fluxes 144; Fp out 24
     "cell_fp": (144 + 24, "point"),
-    # Fp 24 + csi/eta/zet 72 + iaj/jaj/kaj 24 + nvert 8 + dP 24 + pUcont 24; Ucont out 24
-    "cell_rhs": (24 + 72 + 24 + 8 + 24 + 24 + 24, "point"),
+    # (after the fused face pass) csi.Fp, eta.Fp, zet.Fp 24 + iaj/jaj/kaj 24 + nvert 8 + dP 24 + pUcont 24;
+    # Ucont out 24
+    "cell_rhs": (24 + 24 + 8 + 24 + 24 + 24, "point"),
     # ucont 24 + csi/eta/zet 72 + nvert 8; ucat out 24 (interior cells)
     "contra2cart": (24 + 72 + 8 + 24, "cell"),
     # ucat 24 + csi/eta/zet 72 + aj 8 + nvert 8; nu_t out 8
@@ -106,7 +107,10 @@ def poisson_comparison(vfs, ctx, reps=3):
     FP32 V-cycle, FP32 V-cycle + learned warm start / first stage.  The
     right-hand side is that of one more step of the run (momentum stage, then
     PoissonRHS); the previous step's correction is the initial guess, as in
-    the time loop, and each variant is also run from a zero guess."""
+    the time loop, and each variant is also run from a zero guess.  For the
+    first solve of each variant the correction is also applied (pressure
+    update and projection of the same pre-projection Ucont), and the pressure,
+    velocity and divergence are compared with the all-FP64 variant."""
     learned = importlib.import_module("vfs-wind_amd.learned")
     ctx.calc_k_flux()
     if ctx.params.les:
@@ -117,27 +121,44 @@ def poisson_comparison(vfs, ctx, reps=3):
     ctx.poisson_rhs()
     guess = ctx.download("PHI")
     zero = np.zeros_like(guess)
+    ucont_pre, p_pre = ctx.download("UCONT"), ctx.download("P")
     net = learned.make_net(ctx)
     variants = [("fp64_mg", 0, None), ("fp32_mg", 1, None)]
     if net is not None:
         variants.append(("fp32_mg_net", 1, net))
     out = {}
+    fields = {}
     for start, g in (("previous_guess", guess), ("zero_guess", zero)):
         res = {}
         for name, fp32, nn in variants:
             ms, its, fb, rel = [], [], [], []
-            for _ in range(reps):
+            for rep in range(reps):
                 ctx.upload("PHI", g)
                 ctx.sync()
                 opts = vfs.default_solve_opts(tol=1e-8, mg_fp32=fp32, use_guess=1)
                 st = nn.poisson_solve(opts) if nn is not None else ctx.poisson_solve(opts)
                 ms.append(st.ms); its.append(st.iterations); fb.append(st.fallback); rel.append(st.relres)
+                if rep == 0 and start == "previous_guess":
+                    ctx.update_pressure()
+                    ctx.projection()
+                    fields[name] = (ctx.download("P"), ctx.download("UCONT"), ctx.divergence())
+                    ctx.upload("UCONT", ucont_pre)
+                    ctx.upload("P", p_pre)
+                    ctx.contra2cart()
             res[name] = {"ms": round(float(np.median(ms)), 3), "iterations": int(np.median(its)),
                          "fallback": int(max(fb)), "relres": float(max(rel))}
         base = res["fp64_mg"]["ms"]
         for name in res:
             res[name]["speedup_vs_fp64_mg"] = round(base / res[name]["ms"], 3)
         out[start] = res
+    # errors against the all-FP64 variant, after the pressure update and projection
+    p64, u64, d64 = fields["fp64_mg"]
+    err = {"fp64_mg": {"maxdiv": d64}}
+    rel = lambda a, b: float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+    for name, (p_, u_, d_) in fields.items():
+        if name != "fp64_mg":
+            err[name] = {"p_rel_vs_fp64": rel(p_, p64), "ucont_rel_vs_fp64": rel(u_, u64), "maxdiv": d_}
+    out["errors_after_projection"] = err
     if net is not None:
         net.close()
     return out

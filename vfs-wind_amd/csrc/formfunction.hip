@@ -420,15 +420,21 @@ __global__ void __launch_bounds__(FT_I * FT_J) k_face_fluxes(FaceArgs a)
 // to LDS together with one extra column (the i face at i0-1) and one extra row
 // (the j face at j0-1), and the k-face flux of the plane below stays in
 // registers, so Fp = (div_i(i) - div_i(i-1) + ... ) + (visc ...) is formed in
-// the reference's order without the 18 flux planes ever reaching HBM.  A
-// block starting above k = 0 first computes the k face of plane k0-1 only.
+// the reference's order without the 18 flux planes ever reaching HBM.  What
+// is stored per cell is not Fp but its three projections on the cell's own
+// contravariant directions, csi.Fp, eta.Fp and zet.Fp: the projection pass
+// (momentum.c:1715-1841) uses Fp only through these products (of a cell's
+// Fp with that cell's metrics), evaluated here with the same operands in the
+// same order.  A block starting above k = 0 first computes the k face of
+// plane k0-1 only.
 // Periodic ghost cells of the face fluxes are copies of the interior faces
 // they stand for (momentum.c:1504-1546); with the periodic ghosts of ucat,
 // nu_t, nvert, ucont and the metrics holding their periodic copies (as
 // Contra2Cart, the eddy viscosity, Formfunction_2's own Ucont fix and
 // FormMetrics leave them) the faces at index 0 computed here are bitwise
-// those copies, so that fix is not applied.  Fp is written for every point of
-// the block (zero at ghosts); the periodic ghosts are filled afterwards.
+// those copies, so that fix is not applied.  The projections are written for
+// every point of the block (zero at ghosts); the periodic ghosts are filled
+// afterwards.
 constexpr int FX_I = FT_I + 1, FX_J = FT_J + 1;
 __global__ void __launch_bounds__(FT_I * FT_J) k_face_fp(FaceArgs a, double *__restrict__ fp)
 {
@@ -533,14 +539,17 @@ __global__ void __launch_bounds__(FT_I * FT_J) k_face_fp(FaceArgs a, double *__r
         __syncthreads();
         if (full && inb) {
             double f[3] = {0, 0, 0};
+            double pr[3] = {0, 0, 0};
             if (i >= 1 && i < g.mx - 1 && j >= 1 && j < g.my - 1 && k >= 1 && k < g.mz - 1) {
                 for (int c = 0; c < 3; c++) {
                     f[c] = fxi[c][ty][tx + 1] - fxi[c][ty][tx] + fxj[c][ty + 1][tx] - fxj[c][ty][tx] + kf[c] - kprev[c];
                     f[c] += (fxi[3 + c][ty][tx + 1] - fxi[3 + c][ty][tx] + fxj[3 + c][ty + 1][tx] - fxj[3 + c][ty][tx] +
                              kf[3 + c] - kprev[3 + c]) / 1.0;
                 }
+                // csi.Fp, eta.Fp, zet.Fp of this cell
+                for (int r = 0; r < 3; r++) pr[r] = cur[3 * r] * f[0] + cur[3 * r + 1] * f[1] + cur[3 * r + 2] * f[2];
             }
-            for (int c = 0; c < 3; c++) fp[c * n + q] = f[c];
+            for (int r = 0; r < 3; r++) fp[r * n + q] = pr[r];
         }
         for (int c = 0; c < 6; c++) kprev[c] = kf[c];
         if (inb) {
@@ -777,6 +786,62 @@ __global__ void __launch_bounds__(FT_I * FT_J) k_cell_rhs(CellArgs a, const doub
     (void)z1;
 }
 
+// k_cell_rhs on the projections written by k_face_fp: pr[0..2] planes hold
+// csi.Fp, eta.Fp, zet.Fp of every cell, so the contravariant projection
+// reads no cell metrics (the same products, hence the same bits).
+__global__ void __launch_bounds__(FT_I * FT_J) k_cell_rhs_pr(CellArgs a, const double *__restrict__ pr)
+{
+    const Grid &g = a.g;
+    const int nbi = (g.mx + FT_I - 1) / FT_I, nbj = (g.my + FT_J - 1) / FT_J;
+    int b = blockIdx.x;
+    const int bi = b % nbi;
+    b /= nbi;
+    const int bj = b % nbj, bk = b / nbj;
+    const int i = bi * FT_I + threadIdx.x % FT_I, j = bj * FT_J + threadIdx.x / FT_I, k0 = bk * FT_K;
+    if (i >= g.mx || j >= g.my) return;
+    const int k1 = k0 + FT_K < g.mz ? k0 + FT_K : g.mz;
+    const long long n = g.n;
+    const int mx = g.mx, my = g.my, mz = g.mz;
+    double pk_next = 0, nvk = 0;
+    bool have = false;
+    for (int k = k0; k < k1; k++) {
+        const long long q = ((long long)k * g.my + j) * g.mx + i;
+        double r[3];
+        const bool interior = i >= 1 && i < mx - 1 && j >= 1 && j < my - 1 && k >= 1 && k < mz - 1;
+        bool zero[3] = {true, true, true};
+        if (interior) {
+            const long long qi = q + 1, qj = q + mx, qk = gidx(g, k + 1, j, i);
+            const double pi0 = pr[q], pi1 = pr[qi];
+            const double pj0 = pr[n + q], pj1 = pr[n + qj];
+            const double pk0 = have ? pk_next : pr[2 * n + q], pk1 = pr[2 * n + qk];
+            const double nv0 = have ? nvk : a.nvert[q];
+            const double nvkn = a.nvert[qk];
+            r[0] = a.scale * (0.5 * pi0 + 0.5 * pi1) * a.iaj[q];
+            r[1] = a.scale * (0.5 * pj0 + 0.5 * pj1) * a.jaj[q];
+            r[2] = a.scale * (0.5 * pk0 + 0.5 * pk1) * a.kaj[q];
+            const double *nv = a.nvert;
+            zero[0] = nv0 + nv[qi] > 0.1 || (!g.pi && i == mx - 2);
+            zero[1] = nv0 + nv[qj] > 0.1 || (!g.pj && j == my - 2);
+            zero[2] = nv0 + nvkn > 0.1 || (!g.pk && k == mz - 2);
+            nvk = nvkn;
+            pk_next = pk1;
+            have = qk == q + g.plane;  // the carried values belong to the next plane
+        } else {
+            have = false;
+        }
+        if (a.pucont) {
+            // Rhs = 0 + F; Rhs += -1*dP; Ucont = alpha*Rhs + pUcont  (rhs.c:1647-1658)
+            for (int c = 0; c < 3; c++) {
+                double rv = zero[c] ? 0.0 : (0.0 + r[c]);
+                rv += -1 * a.dp[3 * q + c];
+                a.ucont_out[3 * q + c] = a.alpha * rv + a.pucont[3 * q + c];
+            }
+        } else {
+            for (int c = 0; c < 3; c++) a.rhs[3 * q + c] = zero[c] ? 0.0 : a.rhs[3 * q + c] + r[c];
+        }
+    }
+}
+
 int formfunction2(vfs_ctx *c, double scale, const double *pucont, double rk_alpha, double *ucont_out)
 {
     const Grid &g = c->g;
@@ -817,7 +882,7 @@ int formfunction2(vfs_ctx *c, double scale, const double *pucont, double rk_alph
     prof_end(c);
     VFS_LAUNCH_CHECK("face fluxes");
     if (fused) {
-        // Fp of the periodic ghost cells: copies of the cells they stand for
+        // projections of the periodic ghost cells: copies of the cells they stand for
         for (int d = 0; d < 3; d++)
             if (per[d]) hipLaunchKernelGGL(k_periodic_soa, dim3(blocks_for(np[d], FF_TPB)), dim3(FF_TPB), 0, st, g, c->fpc, 3, d);
     } else {
@@ -851,7 +916,10 @@ int formfunction2(vfs_ctx *c, double scale, const double *pucont, double rk_alph
         prof_end(c);
     }
     prof_begin(c, K_CELL);
-    hipLaunchKernelGGL(k_cell_rhs, dim3(nbt), dim3(FT_I * FT_J), 0, st, ca, (const double *)c->fpc);
+    if (fused)
+        hipLaunchKernelGGL(k_cell_rhs_pr, dim3(nbt), dim3(FT_I * FT_J), 0, st, ca, (const double *)c->fpc);
+    else
+        hipLaunchKernelGGL(k_cell_rhs, dim3(nbt), dim3(FT_I * FT_J), 0, st, ca, (const double *)c->fpc);
     prof_end(c);
     prof_end(c, whole);
     VFS_LAUNCH_CHECK("cell rhs");
