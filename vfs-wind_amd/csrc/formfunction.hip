@@ -456,26 +456,48 @@ __global__ void __launch_bounds__(FT_I * FT_J) k_face_fp(FaceArgs a, double *__r
     const bool inb = i < g.mx && j < g.my;
     const long long n = g.n;
     const bool les = a.les != 0;
-    auto load_plane = [&](int kk, int slot) {
+    // A tile plane is fetched into registers one plane ahead and written to
+    // LDS at the top of the next iteration, so the global loads of plane k+2
+    // are in flight while plane k is computed.  Each thread owns tile points
+    // tid and tid + FT_I*FT_J.
+    constexpr int TPP = (D::HP + FT_I * FT_J - 1) / (FT_I * FT_J);  // tile points per thread
+    double rv[TPP][4];
+    float rn[TPP];
+    auto fetch_plane = [&](int kk) {
         const int wk = wrapi(kk, g.mz, g.pk);
-        for (int p = threadIdx.x; p < D::HP; p += FT_I * FT_J) {
-            const int di = p % D::HI, dj = p / D::HI;
-            const int wi = wrapi(i0 - 1 + di, g.mx, g.pi), wj = wrapi(j0 - 1 + dj, g.my, g.pj);
-            double vx = 0, vy = 0, vz = 0, vt = 0;
-            float vn = 0;
-            if (wi >= 0 && wi < g.mx && wj >= 0 && wj < g.my && wk >= 0 && wk < g.mz) {
-                const long long q = ((long long)wk * g.my + wj) * g.mx + wi;
-                vx = a.ucat[3 * q]; vy = a.ucat[3 * q + 1]; vz = a.ucat[3 * q + 2];
-                vn = (float)a.nvert[q];
-                if (les) vt = a.nut[q];
+#pragma unroll
+        for (int r = 0; r < TPP; r++) {
+            const int p = threadIdx.x + r * FT_I * FT_J;
+            rv[r][0] = rv[r][1] = rv[r][2] = rv[r][3] = 0;
+            rn[r] = 0;
+            if (p < D::HP) {
+                const int di = p % D::HI, dj = p / D::HI;
+                const int wi = wrapi(i0 - 1 + di, g.mx, g.pi), wj = wrapi(j0 - 1 + dj, g.my, g.pj);
+                if (wi >= 0 && wi < g.mx && wj >= 0 && wj < g.my && wk >= 0 && wk < g.mz) {
+                    const long long q = ((long long)wk * g.my + wj) * g.mx + wi;
+                    rv[r][0] = a.ucat[3 * q]; rv[r][1] = a.ucat[3 * q + 1]; rv[r][2] = a.ucat[3 * q + 2];
+                    rn[r] = (float)a.nvert[q];
+                    if (les) rv[r][3] = a.nut[q];
+                }
             }
-            const int l = slot * D::HP + p;
-            sx[l] = vx; sy[l] = vy; sz[l] = vz; sn[l] = vn; st[l] = vt;
+        }
+    };
+    auto store_plane = [&](int slot) {
+#pragma unroll
+        for (int r = 0; r < TPP; r++) {
+            const int p = threadIdx.x + r * FT_I * FT_J;
+            if (p < D::HP) {
+                const int l = slot * D::HP + p;
+                sx[l] = rv[r][0]; sy[l] = rv[r][1]; sz[l] = rv[r][2]; st[l] = rv[r][3]; sn[l] = rn[r];
+            }
         }
     };
     const int kstart = k0 > 0 ? k0 - 1 : k0;
-    load_plane(kstart - 1, 0);
-    load_plane(kstart, 1);
+    fetch_plane(kstart - 1);
+    store_plane(0);
+    fetch_plane(kstart);
+    store_plane(1);
+    fetch_plane(kstart + 1);
     int slot0 = 0;
     double cur[10], nk[10], kprev[6];
     for (int c = 0; c < 6; c++) kprev[c] = 0;
@@ -483,8 +505,9 @@ __global__ void __launch_bounds__(FT_I * FT_J) k_face_fp(FaceArgs a, double *__r
     for (int k = kstart; k < k1; k++) {
         int s2 = slot0 + 2;
         if (s2 >= 3) s2 -= 3;
-        load_plane(k + 1, s2);
+        store_plane(s2);  // plane k+1
         __syncthreads();
+        if (k + 1 < k1) fetch_plane(k + 2);
         TileAcc<true> T{&a, sx, sy, sz, st, sn, i0, j0, k, slot0};
         const bool full = k >= k0;  // the plane below the block: its k face only
         const long long q = ((long long)k * g.my + j) * g.mx + i;
