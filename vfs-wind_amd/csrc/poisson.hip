@@ -714,6 +714,43 @@ __global__ void __launch_bounds__(RED_THREADS) k_multiaxpy_norm(long long n, int
     if (threadIdx.x == 0) part[blockIdx.x] = ss;
 }
 
+// w = w - sum_v h[v] V_v with per-block partial sums of <V_v, w_new> (v < nv)
+// and of <w_new, w_new>: the first Gram-Schmidt pass together with the dot
+// products a second (re-orthogonalisation) pass needs, in one sweep over V.
+__global__ void __launch_bounds__(RED_THREADS) k_multiaxpy_dots(long long n, int nv, const double *__restrict__ h,
+                                                                const double *__restrict__ V, long long ld,
+                                                                double *__restrict__ w, double *__restrict__ part)
+{
+    double acc[MAXV - 1];
+#pragma unroll
+    for (int v = 0; v < MAXV - 1; v++) acc[v] = 0;
+    double ss = 0;
+    for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < n; t += (long long)gridDim.x * blockDim.x) {
+        double vt[MAXV - 1];
+        double s = w[t];
+#pragma unroll
+        for (int v = 0; v < MAXV - 1; v++)
+            if (v < nv) {
+                vt[v] = V[v * ld + t];
+                s -= h[v] * vt[v];
+            }
+        w[t] = s;
+#pragma unroll
+        for (int v = 0; v < MAXV - 1; v++)
+            if (v < nv) acc[v] += vt[v] * s;
+        ss += s * s;
+    }
+#pragma unroll
+    for (int v = 0; v < MAXV - 1; v++) {
+        if (v < nv) {
+            const double r = block_reduce<OpSum>(acc[v]);
+            if (threadIdx.x == 0) part[v * RED_BLOCKS + blockIdx.x] = r;
+        }
+    }
+    ss = block_reduce<OpSum>(ss);
+    if (threadIdx.x == 0) part[nv * RED_BLOCKS + blockIdx.x] = ss;
+}
+
 __global__ void k_finish_many(int nv, const double *__restrict__ part, double *__restrict__ out)
 {
     // one block per vector
@@ -1168,6 +1205,22 @@ static int multidot(vfs_ctx *c, int nv, const double *V, const double *w, double
     return VFS_OK;
 }
 
+// w -= sum_v h_v V_v with h already in dev_scalars; returns <V_v, w_new> for
+// v < nv in d[0..nv-1] and <w_new, w_new> in *ww
+static int axpy_dots(vfs_ctx *c, int nv, const double *V, double *w, double *d, double *ww)
+{
+    hipStream_t st = c->stream;
+    const long long n = c->g.n;
+    hipLaunchKernelGGL(k_multiaxpy_dots, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, st, n, nv, c->ps->dev_scalars, V, n, w, c->red);
+    hipLaunchKernelGGL(k_finish_many, dim3(nv + 1), dim3(RED_THREADS), 0, st, nv, c->red, c->red + MAXV * RED_BLOCKS);
+    VFS_LAUNCH_CHECK("axpy dots");
+    VFS_HIP(hipMemcpyAsync(c->red_host, c->red + MAXV * RED_BLOCKS, (nv + 1) * sizeof(double), hipMemcpyDeviceToHost, st));
+    VFS_HIP(hipStreamSynchronize(st));
+    std::memcpy(d, c->red_host, nv * sizeof(double));
+    *ww = c->red_host[nv];
+    return VFS_OK;
+}
+
 // w -= sum_v h_v V_v with h already in dev_scalars; returns the new <w, w>
 static int axpy_norm(vfs_ctx *c, int nv, const double *V, double *w, double *ww)
 {
@@ -1289,12 +1342,11 @@ static int fgmres(vfs_ctx *c, const vfs_solve_opts *o, vfs_unet *net, vfs_solve_
             double ww0 = d[jj + 1];
             std::memcpy(c->red_host, h, (jj + 1) * sizeof(double));
             VFS_HIP(hipMemcpyAsync(ps->dev_scalars, c->red_host, (jj + 1) * sizeof(double), hipMemcpyHostToDevice, st));
+            // first pass, with the dot products of a second one in the same sweep
             double ww;
-            rc = axpy_norm(c, jj + 1, V, w, &ww);
+            rc = axpy_dots(c, jj + 1, V, w, d, &ww);
             if (rc) return rc;
             if (ww < 0.25 * ww0) {
-                rc = multidot(c, jj + 1, V, w, d);
-                if (rc) return rc;
                 for (int v = 0; v <= jj; v++) h[v] += d[v];
                 std::memcpy(c->red_host, d, (jj + 1) * sizeof(double));
                 VFS_HIP(hipMemcpyAsync(ps->dev_scalars, c->red_host, (jj + 1) * sizeof(double), hipMemcpyHostToDevice, st));
