@@ -149,6 +149,10 @@ struct vfs_ctx {
     // derived (no metric array written since): kernels may then rebuild a
     // face metric from the two cell metrics instead of reading it
     bool face_derived;
+    // true while VFS_UCAT is what Contra2Cart would produce from the current
+    // VFS_UCONT / VFS_UCAT_OLD / metrics / nvert (set by c2c; cleared by
+    // every write to those arrays): c2c is idempotent and is then skipped
+    bool ucat_fresh;
     // Newton-GMRES vectors of the implicit momentum solve, kept between calls
     // (allocated on first use, grown when a longer restart asks for more)
     double *imp_ws;
@@ -165,6 +169,11 @@ namespace vfs {
 int periodic_fix(vfs_ctx *c, double *f, int nc, int comp_mask_mode);
 int launch_metrics(vfs_ctx *c, const Grid &g, const double *coor, double *const m[17]);
 inline bool is_metric_field(int f) { return f >= VFS_CSI && f <= VFS_KAJ; }
+// the arrays Contra2Cart reads or writes
+inline bool is_c2c_field(int f)
+{
+    return f == VFS_COOR || f == VFS_NVERT || is_metric_field(f) || f == VFS_UCONT || f == VFS_UCAT || f == VFS_UCAT_OLD;
+}
 int c2c(vfs_ctx *c);
 int form_bcs(vfs_ctx *c, int outflow_scale);
 int inflow_flux(vfs_ctx *c, double *flux_in);

@@ -408,6 +408,7 @@ int c2c(vfs_ctx *c)
 {
     const Grid &g = c->g;
     hipStream_t st = c->stream;
+    if (c->ucat_fresh) return VFS_OK;  // would rewrite the same values
     double *ucont = c->f[VFS_UCONT], *ucat = c->f[VFS_UCAT];
     int rc = periodic_fix(c, ucont, 3, 0);  // rhs.c:129-156
     if (rc) return rc;
@@ -437,7 +438,9 @@ int c2c(vfs_ctx *c)
         hipLaunchKernelGGL(k_c2c_corner, dim3(blocks_for(g.mz, TPB)), dim3(TPB), 0, st, g, ucat);
         VFS_LAUNCH_CHECK("c2c corner");
     }
-    return periodic_fix(c, ucat, 3, 0);  // rhs.c:712-748
+    rc = periodic_fix(c, ucat, 3, 0);  // rhs.c:712-748
+    if (!rc) c->ucat_fresh = true;
+    return rc;
 }
 
 // ---------------------------------------------------------------------------
@@ -578,6 +581,7 @@ int inflow_flux(vfs_ctx *c, double *flux_in)
     }
     hipLaunchKernelGGL(k_inflow, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, st, g, 1, uin, c->f[VFS_CSI], c->f[VFS_ETA],
                        c->f[VFS_ZET], c->f[VFS_NVERT], c->f[VFS_UCONT], c->f[VFS_UCAT], c->red);
+    c->ucat_fresh = false;
     hipLaunchKernelGGL(finish_reduce<OpSum>, dim3(1), dim3(RED_THREADS), 0, st, c->red, RED_BLOCKS, c->red + RED_BLOCKS);
     VFS_LAUNCH_CHECK("inflow flux");
     VFS_HIP(hipMemcpyAsync(c->red_host, c->red + RED_BLOCKS, sizeof(double), hipMemcpyDeviceToHost, st));
@@ -604,17 +608,20 @@ int form_bcs(vfs_ctx *c, int outflow_scale)
             long long n = (long long)(g.mx - 2) * (g.my - 2);
             hipLaunchKernelGGL(k_outlet_scale, dim3(blocks_for(n, TPB)), dim3(TPB), 0, st, g, c->p.flux_in_sum,
                                c->red + 2 * RED_BLOCKS, c->f[VFS_ZET], c->f[VFS_NVERT], c->f[VFS_UCONT]);
+            c->ucat_fresh = false;
             VFS_LAUNCH_CHECK("outlet scale");
         }
     }
     long long nw = (long long)(g.my > g.mx ? g.my : g.mx) * g.mz;
     hipLaunchKernelGGL(k_wall_flux, dim3(blocks_for(nw, TPB)), dim3(TPB), 0, st, g, c->f[VFS_UCONT]);
+    c->ucat_fresh = false;
     VFS_LAUNCH_CHECK("wall flux");
     rc = c2c(c);
     if (rc) return rc;
     int ne = g.mx > g.my ? g.mx : g.my;
     if (g.mz > ne) ne = g.mz;
     hipLaunchKernelGGL(k_zero_edges, dim3(blocks_for(ne, TPB)), dim3(TPB), 0, st, g, c->f[VFS_UCAT]);
+    c->ucat_fresh = false;
     VFS_LAUNCH_CHECK("zero edges");
     return VFS_OK;
 }
