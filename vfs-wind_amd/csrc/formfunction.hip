@@ -128,16 +128,6 @@ __device__ inline void load_metrics(const Metrics9 &m, long long q, double v[10]
     v[9] = m.aj[q];
 }
 
-// a face metric as FormMetrics builds it from the two cells the face
-// separates: 0.5 X + 0.5 Y (AxByC, momentum.c:40-44) and 2 / (1/aj + 1/aj)
-// (metrics.c:588-593, 696-701, 804-809)
-__device__ inline void face_mean(const double a[10], const double b[10], double f[10])
-{
-#pragma unroll
-    for (int c = 0; c < 9; c++) f[c] = 0.5 * a[c] + 0.5 * b[c];
-    f[9] = 2. / (1. / a[9] + 1. / b[9]);
-}
-
 // convective and viscous flux of the i face of point (i,j,k)
 // (momentum.c:668-928): i in [0,mx-2], j,k interior; zero elsewhere
 template <bool SO>

@@ -61,6 +61,29 @@ __host__ __device__ inline bool periodic_src(const Grid &g, int k, int j, int i,
     return fi || fj || fk;
 }
 
+// the cell metrics of point q into v[0..9]: csi xyz, eta xyz, zet xyz, aj
+__device__ inline void load_cell_metrics(const double *__restrict__ csi, const double *__restrict__ eta,
+                                         const double *__restrict__ zet, const double *__restrict__ aj, long long q,
+                                         double v[10])
+{
+    for (int c = 0; c < 3; c++) {
+        v[c] = csi[3 * q + c];
+        v[3 + c] = eta[3 * q + c];
+        v[6 + c] = zet[3 * q + c];
+    }
+    v[9] = aj[q];
+}
+
+// a face metric as FormMetrics builds it from the two cells the face
+// separates: 0.5 X + 0.5 Y (AxByC, momentum.c:40-44) and 2 / (1/aj + 1/aj)
+// (metrics.c:588-593, 696-701, 804-809)
+__device__ inline void face_mean(const double a[10], const double b[10], double f[10])
+{
+#pragma clang fp contract(off)
+    for (int c = 0; c < 9; c++) f[c] = 0.5 * a[c] + 0.5 * b[c];
+    f[9] = 2. / (1. / a[9] + 1. / b[9]);
+}
+
 // inverse of [csi; eta; zet] (Calculate_Covariant_metrics, rhs2.c:595-612)
 __host__ __device__ inline void covariant(const double gm[3][3], double G[3][3])
 {
