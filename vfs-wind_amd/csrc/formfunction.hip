@@ -425,14 +425,17 @@ __global__ void __launch_bounds__(FT_I * FT_J) k_face_fluxes(FaceArgs a)
 // those copies, so that fix is not applied.  The projections are written for
 // every point of the block (zero at ghosts); the periodic ghosts are filled
 // afterwards.
-constexpr int FX_I = FT_I + 1, FX_J = FT_J + 1;
-__global__ void __launch_bounds__(FT_I * FT_J) k_face_fp(FaceArgs a, double *__restrict__ fp)
+constexpr int FX_J = FT_J + 1;
+static_assert(FT_I == 32, "a row of the tile is half a wavefront (the i-face exchange shuffles within it)");
+constexpr int HX_T0 = 64;  // first thread of the second wavefront
+static_assert(FT_I * FT_J >= HX_T0 + FT_J && FT_J * 2 <= 64, "halo lanes of the i faces");
+__global__ void __launch_bounds__(FT_I * FT_J) __attribute__((amdgpu_waves_per_eu(2))) k_face_fp(FaceArgs a, double *__restrict__ fp)
 {
     using D = TileDims<true>;
     __shared__ double sx[3 * D::HP], sy[3 * D::HP], sz[3 * D::HP], st[3 * D::HP];
     __shared__ float sn[3 * D::HP];
-    __shared__ double fxi[6][FT_J][FX_I];  // i-face div xyz, visc xyz; column 0 = face i0-1
     __shared__ double fxj[6][FX_J][FT_I];  // j-face div xyz, visc xyz; row 0 = face j0-1
+    __shared__ double hx[6][FT_J];         // i-face div xyz, visc xyz at i0-1 of each row
     const Grid &g = a.g;
     const int nbi = (g.mx + FT_I - 1) / FT_I, nbj = (g.my + FT_J - 1) / FT_J;
     int b = blockIdx.x;
@@ -501,15 +504,13 @@ __global__ void __launch_bounds__(FT_I * FT_J) k_face_fp(FaceArgs a, double *__r
         TileAcc<true> T{&a, sx, sy, sz, st, sn, i0, j0, k, slot0};
         const bool full = k >= k0;  // the plane below the block: its k face only
         const long long q = ((long long)k * g.my + j) * g.mx + i;
-        double kf[6] = {0, 0, 0, 0, 0, 0};
-        if (inb) {
-            if (!have) load_metrics(a.cell, q, cur);
-            load_metrics(a.cell, k < g.mz - 1 ? q + g.plane : q, nk);
-            double M[10];
-            face_mean(cur, nk, M);
-            flux_k<true>(a, T, i, j, k, M, a.ucont[3 * q + 2], kf, kf + 3);
-        }
+        if (inb && !have) load_metrics(a.cell, q, cur);
+        double di[6] = {0, 0, 0, 0, 0, 0};  // i-face differences, div xyz and visc xyz
         if (full) {
+            // i faces: this point's face i+1/2 and, from the lane before it
+            // in the row (a row of FT_I points is FT_I consecutive lanes of
+            // one wavefront), the face i-1/2; the row's first lane takes the
+            // face i0-1/2 from LDS after the barrier.
             double f6[6] = {0, 0, 0, 0, 0, 0};
             if (inb) {
                 double nb_[10], M[10];
@@ -517,7 +518,9 @@ __global__ void __launch_bounds__(FT_I * FT_J) k_face_fp(FaceArgs a, double *__r
                 face_mean(cur, nb_, M);
                 flux_i<true>(a, T, i, j, k, M, a.ucont[3 * q + 0], f6, f6 + 3);
             }
-            for (int c = 0; c < 6; c++) fxi[c][ty][tx + 1] = f6[c];
+            double fl[6];
+            for (int c = 0; c < 6; c++) fl[c] = __shfl_up(f6[c], 1, FT_I);
+            for (int c = 0; c < 6; c++) di[c] = tx == 0 ? f6[c] : f6[c] - fl[c];
             for (int c = 0; c < 6; c++) f6[c] = 0;
             if (inb) {
                 double nb_[10], M[10];
@@ -526,17 +529,6 @@ __global__ void __launch_bounds__(FT_I * FT_J) k_face_fp(FaceArgs a, double *__r
                 flux_j<true>(a, T, i, j, k, M, a.ucont[3 * q + 1], f6, f6 + 3);
             }
             for (int c = 0; c < 6; c++) fxj[c][ty + 1][tx] = f6[c];
-            if (tx == 0) {  // i face at i0-1 of this row
-                for (int c = 0; c < 6; c++) f6[c] = 0;
-                if (i0 >= 1 && j < g.my) {
-                    const long long p = q - 1;
-                    double cp[10], M[10];
-                    load_metrics(a.cell, p, cp);
-                    face_mean(cp, cur, M);
-                    flux_i<true>(a, T, i0 - 1, j, k, M, a.ucont[3 * p + 0], f6, f6 + 3);
-                }
-                for (int c = 0; c < 6; c++) fxi[c][ty][0] = f6[c];
-            }
             if (ty == 0) {  // j face at j0-1 of this column
                 for (int c = 0; c < 6; c++) f6[c] = 0;
                 if (j0 >= 1 && i < g.mx) {
@@ -548,16 +540,41 @@ __global__ void __launch_bounds__(FT_I * FT_J) k_face_fp(FaceArgs a, double *__r
                 }
                 for (int c = 0; c < 6; c++) fxj[c][0][tx] = f6[c];
             }
+            // the face i0-1/2 of each row, on lanes of the second wavefront
+            // (the first one computes the faces j0-1/2), so that no
+            // wavefront evaluates more than one halo face
+            const int r = (int)threadIdx.x - HX_T0;
+            if (r >= 0 && r < FT_J) {
+                const int jr = j0 + r;
+                for (int c = 0; c < 6; c++) f6[c] = 0;
+                if (i0 >= 1 && jr < g.my) {
+                    const long long p = ((long long)k * g.my + jr) * g.mx + (i0 - 1);
+                    double cp[10], cn[10], M[10];
+                    load_metrics(a.cell, p, cp);
+                    load_metrics(a.cell, p + 1, cn);
+                    face_mean(cp, cn, M);
+                    flux_i<true>(a, T, i0 - 1, jr, k, M, a.ucont[3 * p + 0], f6, f6 + 3);
+                }
+                for (int c = 0; c < 6; c++) hx[c][r] = f6[c];
+            }
+        }
+        double kf[6] = {0, 0, 0, 0, 0, 0};
+        if (inb) {
+            load_metrics(a.cell, k < g.mz - 1 ? q + g.plane : q, nk);
+            double M[10];
+            face_mean(cur, nk, M);
+            flux_k<true>(a, T, i, j, k, M, a.ucont[3 * q + 2], kf, kf + 3);
         }
         __syncthreads();
         if (full && inb) {
             double f[3] = {0, 0, 0};
             double pr[3] = {0, 0, 0};
+            if (tx == 0)
+                for (int c = 0; c < 6; c++) di[c] = di[c] - hx[c][ty];
             if (i >= 1 && i < g.mx - 1 && j >= 1 && j < g.my - 1 && k >= 1 && k < g.mz - 1) {
                 for (int c = 0; c < 3; c++) {
-                    f[c] = fxi[c][ty][tx + 1] - fxi[c][ty][tx] + fxj[c][ty + 1][tx] - fxj[c][ty][tx] + kf[c] - kprev[c];
-                    f[c] += (fxi[3 + c][ty][tx + 1] - fxi[3 + c][ty][tx] + fxj[3 + c][ty + 1][tx] - fxj[3 + c][ty][tx] +
-                             kf[3 + c] - kprev[3 + c]) / 1.0;
+                    f[c] = di[c] + fxj[c][ty + 1][tx] - fxj[c][ty][tx] + kf[c] - kprev[c];
+                    f[c] += (di[3 + c] + fxj[3 + c][ty + 1][tx] - fxj[3 + c][ty][tx] + kf[3 + c] - kprev[3 + c]) / 1.0;
                 }
                 // csi.Fp, eta.Fp, zet.Fp of this cell
                 for (int r = 0; r < 3; r++) pr[r] = cur[3 * r] * f[0] + cur[3 * r + 1] * f[1] + cur[3 * r + 2] * f[2];
