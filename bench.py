@@ -52,8 +52,9 @@ ALGO = {
     "eddy_viscosity": (24 + 72 + 8 + 8 + 8, "point"),
     # p 8 + nvert 8 + 3 face metric families 240; dP out 24
     "pressure_gradient": (8 + 8 + 240 + 24, "point"),
-    # FP32: 16 off-line coefficients 64 + b 4 + x 4; line rhs out 4
-    "line_rhs_fp32": (64 + 4 + 4 + 4, "line point"),
+    # FP32 smoother: 16 off-line coefficient / diagonal ratios as FP16 32 + diagonal 4 + b 4 + x 4;
+    # line rhs out 4
+    "line_rhs_fp32": (32 + 4 + 4 + 4 + 4, "line point"),
     # FP32: Thomas factors lo, 1/pivot, upper 12 + line rhs 4; x out 4
     "line_solve_fp32": (12 + 4 + 4, "line point"),
     # FP64: 19 coefficients 152 + x 8 + row mask 1; y out 8

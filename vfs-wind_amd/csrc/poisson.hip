@@ -331,10 +331,28 @@ struct ZGrid {
 
 __host__ __device__ inline int zebra_i0(int k, int c) { return 1 + ((1 + k + c) & 1); }
 
-// colour-split off-line coefficients and Thomas factors (lo, 1/pivot, upper/pivot)
+// The FP32 smoother keeps the 16 off-line coefficients of a row divided by
+// its diagonal, as FP16 pairs (8 planes of 32-bit words, slots 2p and 2p+1
+// of offline_slot), with the diagonal in an FP32 plane: 36 bytes per point
+// instead of 64, for the pass that reads them on every sweep.  The ratios are
+// O(1) for the Poisson stencil and FP16 keeps 11 bits of them; the smoother
+// is a preconditioner of the FP64 FGMRES, whose residual stays exact.
+typedef _Float16 h2_t __attribute__((ext_vector_type(2)));
+
+// a / d as a float within the FP16 range (the clamp only guards degenerate rows)
+__device__ inline float half_ratio(double a, double d)
+{
+    const float v = (float)(a / d);
+    return v > 65504.f ? 65504.f : (v < -65504.f ? -65504.f : v);
+}
+
+// colour-split off-line coefficients and Thomas factors (lo, 1/pivot,
+// upper/pivot).  T = double: sc holds the 16 off-line planes, then the three
+// factor planes.  T = float: sc holds the three factor planes, sh the FP16
+// coefficient pairs, sd the diagonal.
 template <typename T>
 __global__ void k_zebra_setup(Grid g, ZGrid z, const double *__restrict__ c64, const unsigned char *__restrict__ row,
-                              T *__restrict__ sc)
+                              T *__restrict__ sc, h2_t *__restrict__ sh, float *__restrict__ sd)
 {
     long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
     if (t >= 2LL * z.nkI * z.nh) return;
@@ -349,17 +367,32 @@ __global__ void k_zebra_setup(Grid g, ZGrid z, const double *__restrict__ c64, c
         const long long cq = c * z.cplane + ((long long)kk * z.njI + jj) * z.nh + h;
         const long long q = valid ? ((long long)k * g.my + jj + 1) * g.mx + i : 0;
         const bool rw = valid && row[q];
+        if constexpr (sizeof(T) == 8) {
 #pragma unroll
-        for (int u = 0; u < NOFF; u++) sc[u * sstride + cq] = rw ? (T)c64[offline_slot(u) * g.n + q] : T(0);
+            for (int u = 0; u < NOFF; u++) sc[u * sstride + cq] = rw ? (T)c64[offline_slot(u) * g.n + q] : T(0);
+        } else {
+            const double dg = rw ? c64[CP * g.n + q] : 1.0;
+#pragma unroll
+            for (int u = 0; u < NOFF; u += 2) {
+                h2_t w = {(_Float16)0.f, (_Float16)0.f};
+                if (rw) {
+                    w.x = (_Float16)half_ratio(c64[offline_slot(u) * g.n + q], dg);
+                    w.y = (_Float16)half_ratio(c64[offline_slot(u + 1) * g.n + q], dg);
+                }
+                sh[(u / 2) * sstride + cq] = w;
+            }
+            sd[cq] = rw ? (float)dg : 0.f;
+        }
+        T *sf = sizeof(T) == 8 ? sc + SM_LO * sstride : sc;  // lo, m, cp planes
         double lo = 0, m = 0, cpn = 0;
         if (rw) {
             lo = c64[SP * g.n + q];
             m = 1.0 / (c64[CP * g.n + q] - lo * cpp);
             cpn = c64[NP * g.n + q] * m;
         }
-        sc[SM_LO * sstride + cq] = (T)lo;
-        sc[SM_M * sstride + cq] = (T)m;
-        sc[SM_CP * sstride + cq] = (T)cpn;
+        sf[0 * sstride + cq] = (T)lo;
+        sf[1 * sstride + cq] = (T)m;
+        sf[2 * sstride + cq] = (T)cpn;
         cpp = cpn;
     }
 }
@@ -385,11 +418,39 @@ __global__ void __launch_bounds__(PTPB) k_zebra_rhs(Grid g, ZGrid z, int c, cons
     rz[t] = acc;
 }
 
+// the same for the FP32 smoother: rz = b - d * sum (a/d) x over the off-line
+// points, with the FP16 ratios
+__global__ void __launch_bounds__(PTPB) k_zebra_rhs_h(Grid g, ZGrid z, int c, const h2_t *__restrict__ sh,
+                                                      const float *__restrict__ sd, const float *__restrict__ x,
+                                                      const float *__restrict__ b, float *__restrict__ rz)
+{
+    long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (t >= z.cplane) return;
+    const int h = t % z.nh;
+    const long long r = t / z.nh;
+    const int jj = r % z.njI, kk = r / z.njI;
+    const int k = kk + 1, j = jj + 1, i = zebra_i0(k, c) + 2 * h;
+    if (i > g.mx - 2) return;
+    const long long q = ((long long)k * g.my + j) * g.mx + i;
+    const h2_t *s0 = sh + c * z.cplane + t;
+    const long long sstride = 2 * z.cplane;
+    h2_t w[NOFF / 2];
+#pragma unroll
+    for (int p = 0; p < NOFF / 2; p++) w[p] = s0[p * sstride];
+    float acc = 0;
+#pragma unroll
+    for (int p = 0; p < NOFF / 2; p++) {
+        acc += (float)w[p].x * x[nbr_idx(g, k, j, i, offline_slot(2 * p))];
+        acc += (float)w[p].y * x[nbr_idx(g, k, j, i, offline_slot(2 * p + 1))];
+    }
+    rz[t] = b[q] - sd[c * z.cplane + t] * acc;
+}
+
 // Thomas recurrences of colour c, one thread per line; rz holds the line
 // right-hand sides on entry and the forward-eliminated values afterwards.
 // Loads are issued U points ahead of the dependent chain.
 template <typename T, int U>
-__global__ void __launch_bounds__(PTPB) k_zebra_solve(Grid g, ZGrid z, int c, const T *__restrict__ sc,
+__global__ void __launch_bounds__(PTPB) k_zebra_solve(Grid g, ZGrid z, int c, const T *__restrict__ sf,
                                                       T *__restrict__ rz, T *__restrict__ x)
 {
     long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
@@ -399,9 +460,9 @@ __global__ void __launch_bounds__(PTPB) k_zebra_solve(Grid g, ZGrid z, int c, co
     if (i > g.mx - 2) return;
     const long long sstride = 2 * z.cplane;
     const long long base = (long long)kk * z.njI * z.nh + h;
-    const T *lo = sc + SM_LO * sstride + c * z.cplane + base;
-    const T *mm = sc + SM_M * sstride + c * z.cplane + base;
-    const T *cp = sc + SM_CP * sstride + c * z.cplane + base;
+    const T *lo = sf + c * z.cplane + base;
+    const T *mm = sf + sstride + c * z.cplane + base;
+    const T *cp = sf + 2 * sstride + c * z.cplane + base;
     T *rr = rz + base;
     const int nh = z.nh, nj = z.njI;
     T d = 0;
@@ -462,7 +523,7 @@ constexpr int line_slm()
 
 template <typename T, int SLM>
 __global__ void __launch_bounds__(LINE_LINES * LINE_MAXSEG) k_zebra_scan(Grid g, ZGrid z, int c, int nseg, int sl,
-                                                                        const T *__restrict__ sc, const T *__restrict__ rz,
+                                                                        const T *__restrict__ sf, const T *__restrict__ rz,
                                                                         T *__restrict__ x)
 {
     __shared__ T sA[LINE_MAXSEG * LINE_LINES], sB[LINE_MAXSEG * LINE_LINES];
@@ -482,8 +543,8 @@ __global__ void __launch_bounds__(LINE_LINES * LINE_MAXSEG) k_zebra_scan(Grid g,
     const int j1 = j0 + sl < z.njI ? j0 + sl : z.njI;
     const long long sstride = 2 * z.cplane;
     const long long base = (long long)kk * z.njI * z.nh + h;  // + jj * nh
-    const T *s0 = sc + c * z.cplane + base;
-    const T *lo = s0 + SM_LO * sstride, *mm = s0 + SM_M * sstride, *cp = s0 + SM_CP * sstride;
+    const T *s0 = sf + c * z.cplane + base;
+    const T *lo = s0, *mm = s0 + sstride, *cp = s0 + 2 * sstride;
     const T *rr = rz + base;
     const long long nh = z.nh;
     T r[SLM], l[SLM], m[SLM], u[SLM];
@@ -810,7 +871,9 @@ struct PLevel {
     double *c64 = nullptr;
     float *c32 = nullptr;
     // colour-split smoother coefficients + Thomas factors, one-colour line rhs
-    float *sc32 = nullptr, *rz32 = nullptr;
+    // (FP32: sc32 = factors, sh16 = FP16 coefficient ratios, sd32 = diagonal)
+    float *sc32 = nullptr, *rz32 = nullptr, *sd32 = nullptr;
+    vfs::h2_t *sh16 = nullptr;
     double *sc64 = nullptr, *rz64 = nullptr;
     // work vectors
     float *x32 = nullptr, *b32 = nullptr, *r32 = nullptr;
@@ -886,7 +949,9 @@ static int level_alloc_work(vfs_poisson *ps, PLevel &L)
     rc = rc ? rc : palloc(ps, (void **)&L.row, n);
     rc = rc ? rc : palloc(ps, (void **)&L.c32, NST * n * 4);
     rc = rc ? rc : palloc(ps, (void **)&L.c64, NST * n * 8);
-    rc = rc ? rc : palloc(ps, (void **)&L.sc32, NSM * 2 * cp * 4);
+    rc = rc ? rc : palloc(ps, (void **)&L.sc32, 3 * 2 * cp * 4);
+    rc = rc ? rc : palloc(ps, (void **)&L.sh16, NOFF / 2 * 2 * cp * 4);
+    rc = rc ? rc : palloc(ps, (void **)&L.sd32, 2 * cp * 4);
     rc = rc ? rc : palloc(ps, (void **)&L.sc64, NSM * 2 * cp * 8);
     rc = rc ? rc : palloc(ps, (void **)&L.rz32, cp * 4);
     rc = rc ? rc : palloc(ps, (void **)&L.rz64, cp * 8);
@@ -901,8 +966,10 @@ static int level_assemble(vfs_ctx *c, PLevel &L, double *const met[17], const do
     hipLaunchKernelGGL(k_row_mask, dim3(blocks_for(L.g.n, PTPB)), dim3(PTPB), 0, st, L.g, nvert, th, L.row);
     hipLaunchKernelGGL(k_assemble, dim3(blocks_for(L.g.n, PTPB)), dim3(PTPB), 0, st, L.g, F, nvert, th, L.row, L.c64, L.c32);
     const long long lines = 2LL * L.z.nkI * L.z.nh;
-    hipLaunchKernelGGL(k_zebra_setup<float>, dim3(blocks_for(lines, PTPB)), dim3(PTPB), 0, st, L.g, L.z, L.c64, L.row, L.sc32);
-    hipLaunchKernelGGL(k_zebra_setup<double>, dim3(blocks_for(lines, PTPB)), dim3(PTPB), 0, st, L.g, L.z, L.c64, L.row, L.sc64);
+    hipLaunchKernelGGL(k_zebra_setup<float>, dim3(blocks_for(lines, PTPB)), dim3(PTPB), 0, st, L.g, L.z, L.c64, L.row, L.sc32,
+                       L.sh16, L.sd32);
+    hipLaunchKernelGGL(k_zebra_setup<double>, dim3(blocks_for(lines, PTPB)), dim3(PTPB), 0, st, L.g, L.z, L.c64, L.row, L.sc64,
+                       nullptr, nullptr);
     VFS_LAUNCH_CHECK("assemble");
     return VFS_OK;
 }
@@ -1061,7 +1128,7 @@ struct LV<float> {
     static float *b(PLevel &L) { return L.b32; }
     static float *r(PLevel &L) { return L.r32; }
     static const float *coef(PLevel &L) { return L.c32; }
-    static const float *sc(PLevel &L) { return L.sc32; }
+    static const float *sf(PLevel &L) { return L.sc32; }
     static float *rz(PLevel &L) { return L.rz32; }
     static const float *inv(PLevel &L) { return L.inv32; }
 };
@@ -1071,7 +1138,7 @@ struct LV<double> {
     static double *b(PLevel &L) { return L.b64; }
     static double *r(PLevel &L) { return L.r64; }
     static const double *coef(PLevel &L) { return L.c64; }
-    static const double *sc(PLevel &L) { return L.sc64; }
+    static const double *sf(PLevel &L) { return L.sc64 + SM_LO * 2 * L.z.cplane; }
     static double *rz(PLevel &L) { return L.rz64; }
     static const double *inv(PLevel &L) { return L.inv64; }
 };
@@ -1091,17 +1158,21 @@ static void smooth(vfs_ctx *c, PLevel &L, int sweeps)
     for (int s = 0; s < sweeps; s++)
         for (int col = 0; col < 2; col++) {
             if (fine) prof_begin(c, sizeof(T) == 4 ? K_LINE32 : K_LINE64);
-            hipLaunchKernelGGL(k_zebra_rhs<T>, dim3(blocks_for(L.z.cplane, PTPB)), dim3(PTPB), 0, st, L.g, L.z, col,
-                               LV<T>::sc(L), LV<T>::x(L), LV<T>::b(L), LV<T>::rz(L));
+            if constexpr (sizeof(T) == 4)
+                hipLaunchKernelGGL(k_zebra_rhs_h, dim3(blocks_for(L.z.cplane, PTPB)), dim3(PTPB), 0, st, L.g, L.z, col,
+                                   L.sh16, L.sd32, L.x32, L.b32, L.rz32);
+            else
+                hipLaunchKernelGGL(k_zebra_rhs<double>, dim3(blocks_for(L.z.cplane, PTPB)), dim3(PTPB), 0, st, L.g, L.z,
+                                   col, L.sc64, L.x64, L.b64, L.rz64);
             if (fine) prof_end(c);
             if (fine) prof_begin(c, sizeof(T) == 4 ? K_LSOLVE32 : K_LSOLVE64);
             if (scan)
                 hipLaunchKernelGGL((k_zebra_scan<T, SLM>), dim3((unsigned)((lines + LINE_LINES - 1) / LINE_LINES)),
-                                   dim3(LINE_LINES * nseg), 0, st, L.g, L.z, col, nseg, sl, LV<T>::sc(L),
+                                   dim3(LINE_LINES * nseg), 0, st, L.g, L.z, col, nseg, sl, LV<T>::sf(L),
                                    (const T *)LV<T>::rz(L), LV<T>::x(L));
             else
                 hipLaunchKernelGGL((k_zebra_solve<T, LINE_PREFETCH>), dim3(blocks_for(lines, PTPB)), dim3(PTPB), 0, st, L.g,
-                                   L.z, col, LV<T>::sc(L), LV<T>::rz(L), LV<T>::x(L));
+                                   L.z, col, LV<T>::sf(L), LV<T>::rz(L), LV<T>::x(L));
             if (fine) prof_end(c);
         }
 }
