@@ -727,89 +727,127 @@ __global__ void k_scale_dev(long long n, const double *__restrict__ a, int inv, 
 
 constexpr int MAXV = 33;
 
-// dots[v] = <w, V_v> for v < nv and dots[nv] = <w, w>; accumulators stay in
-// registers (the loops unroll over the maximum basis size)
+// The Gram-Schmidt sweeps over the Krylov basis.  Each is instantiated for a
+// maximum basis count NVM (4, 8, 16 or 32; the host picks the smallest that
+// holds nv), so that the per-vector accumulators stay in registers without
+// spills, and each thread keeps U grid-stride elements in flight.
+
+// dots[v] = <w, V_v> for v < nv and dots[nv] = <w, w> (per-block partials)
+template <int NVM, int U>
 __global__ void __launch_bounds__(RED_THREADS) k_multidot(long long n, int nv, const double *__restrict__ V, long long ld,
                                                           const double *__restrict__ w, double *__restrict__ part)
 {
-    double acc[MAXV - 1];
+    double acc[NVM];
 #pragma unroll
-    for (int v = 0; v < MAXV - 1; v++) acc[v] = 0;
+    for (int v = 0; v < NVM; v++) acc[v] = 0;
     double ww = 0;
-    for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < n; t += (long long)gridDim.x * blockDim.x) {
+    const long long S = (long long)gridDim.x * blockDim.x;
+    long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    for (; t + (U - 1) * S < n; t += U * S) {
+        double wt[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) wt[u] = w[t + u * S];
+#pragma unroll
+        for (int v = 0; v < NVM; v++)
+            if (v < nv) {
+                double vt[U];
+#pragma unroll
+                for (int u = 0; u < U; u++) vt[u] = V[v * ld + t + u * S];
+#pragma unroll
+                for (int u = 0; u < U; u++) acc[v] += vt[u] * wt[u];
+            }
+#pragma unroll
+        for (int u = 0; u < U; u++) ww += wt[u] * wt[u];
+    }
+    for (; t < n; t += S) {
         const double wt = w[t];
 #pragma unroll
-        for (int v = 0; v < MAXV - 1; v++)
+        for (int v = 0; v < NVM; v++)
             if (v < nv) acc[v] += V[v * ld + t] * wt;
         ww += wt * wt;
     }
 #pragma unroll
-    for (int v = 0; v < MAXV - 1; v++) {
-        if (v < nv) {
-            const double s = block_reduce<OpSum>(acc[v]);
-            if (threadIdx.x == 0) part[v * RED_BLOCKS + blockIdx.x] = s;
-        }
-    }
-    const double s = block_reduce<OpSum>(ww);
-    if (threadIdx.x == 0) part[nv * RED_BLOCKS + blockIdx.x] = s;
-}
-
-// w = w - sum_v h[v] V_v, with per-block partial sums of the new <w, w>
-__global__ void __launch_bounds__(RED_THREADS) k_multiaxpy_norm(long long n, int nv, const double *__restrict__ h,
-                                                                const double *__restrict__ V, long long ld,
-                                                                double *__restrict__ w, double *__restrict__ part)
-{
-    double hv[MAXV - 1];
-#pragma unroll
-    for (int v = 0; v < MAXV - 1; v++) hv[v] = v < nv ? h[v] : 0.0;
-    double ss = 0;
-    for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < n; t += (long long)gridDim.x * blockDim.x) {
-        double s = w[t];
-#pragma unroll
-        for (int v = 0; v < MAXV - 1; v++)
-            if (v < nv) s -= hv[v] * V[v * ld + t];
-        w[t] = s;
-        ss += s * s;
-    }
-    ss = block_reduce<OpSum>(ss);
-    if (threadIdx.x == 0) part[blockIdx.x] = ss;
-}
-
-// w = w - sum_v h[v] V_v with per-block partial sums of <V_v, w_new> (v < nv)
-// and of <w_new, w_new>: the first Gram-Schmidt pass together with the dot
-// products a second (re-orthogonalisation) pass needs, in one sweep over V.
-__global__ void __launch_bounds__(RED_THREADS) k_multiaxpy_dots(long long n, int nv, const double *__restrict__ h,
-                                                                const double *__restrict__ V, long long ld,
-                                                                double *__restrict__ w, double *__restrict__ part)
-{
-    double acc[MAXV - 1];
-#pragma unroll
-    for (int v = 0; v < MAXV - 1; v++) acc[v] = 0;
-    double ss = 0;
-    for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < n; t += (long long)gridDim.x * blockDim.x) {
-        double vt[MAXV - 1];
-        double s = w[t];
-#pragma unroll
-        for (int v = 0; v < MAXV - 1; v++)
-            if (v < nv) {
-                vt[v] = V[v * ld + t];
-                s -= h[v] * vt[v];
-            }
-        w[t] = s;
-#pragma unroll
-        for (int v = 0; v < MAXV - 1; v++)
-            if (v < nv) acc[v] += vt[v] * s;
-        ss += s * s;
-    }
-#pragma unroll
-    for (int v = 0; v < MAXV - 1; v++) {
+    for (int v = 0; v < NVM; v++) {
         if (v < nv) {
             const double r = block_reduce<OpSum>(acc[v]);
             if (threadIdx.x == 0) part[v * RED_BLOCKS + blockIdx.x] = r;
         }
     }
+    const double r = block_reduce<OpSum>(ww);
+    if (threadIdx.x == 0) part[nv * RED_BLOCKS + blockIdx.x] = r;
+}
+
+// w = w - sum_v h[v] V_v; with DOTS the per-block partials of <V_v, w_new>
+// (v < nv), the dot products a second (re-orthogonalisation) pass needs, in
+// the same sweep; always the partials of <w_new, w_new> (at slot nv with
+// DOTS, slot 0 without)
+template <int NVM, int U, bool DOTS>
+__global__ void __launch_bounds__(RED_THREADS) k_multiaxpy(long long n, int nv, const double *__restrict__ h,
+                                                           const double *__restrict__ V, long long ld,
+                                                           double *__restrict__ w, double *__restrict__ part)
+{
+    double hv[NVM], acc[NVM];
+#pragma unroll
+    for (int v = 0; v < NVM; v++) {
+        hv[v] = v < nv ? h[v] : 0.0;
+        acc[v] = 0;
+    }
+    double ss = 0;
+    const long long S = (long long)gridDim.x * blockDim.x;
+    long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    for (; t + (U - 1) * S < n; t += U * S) {
+        double s[U], vt[NVM][U];
+#pragma unroll
+        for (int u = 0; u < U; u++) s[u] = w[t + u * S];
+#pragma unroll
+        for (int v = 0; v < NVM; v++)
+            if (v < nv)
+#pragma unroll
+                for (int u = 0; u < U; u++) vt[v][u] = V[v * ld + t + u * S];
+#pragma unroll
+        for (int v = 0; v < NVM; v++)
+            if (v < nv)
+#pragma unroll
+                for (int u = 0; u < U; u++) s[u] -= hv[v] * vt[v][u];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            w[t + u * S] = s[u];
+            ss += s[u] * s[u];
+        }
+        if (DOTS)
+#pragma unroll
+            for (int v = 0; v < NVM; v++)
+                if (v < nv)
+#pragma unroll
+                    for (int u = 0; u < U; u++) acc[v] += vt[v][u] * s[u];
+    }
+    for (; t < n; t += S) {
+        double vt[NVM];
+        double s = w[t];
+#pragma unroll
+        for (int v = 0; v < NVM; v++)
+            if (v < nv) {
+                vt[v] = V[v * ld + t];
+                s -= hv[v] * vt[v];
+            }
+        w[t] = s;
+        ss += s * s;
+        if (DOTS)
+#pragma unroll
+            for (int v = 0; v < NVM; v++)
+                if (v < nv) acc[v] += vt[v] * s;
+    }
+    if (DOTS) {
+#pragma unroll
+        for (int v = 0; v < NVM; v++) {
+            if (v < nv) {
+                const double r = block_reduce<OpSum>(acc[v]);
+                if (threadIdx.x == 0) part[v * RED_BLOCKS + blockIdx.x] = r;
+            }
+        }
+    }
     ss = block_reduce<OpSum>(ss);
-    if (threadIdx.x == 0) part[nv * RED_BLOCKS + blockIdx.x] = ss;
+    if (threadIdx.x == 0) part[(DOTS ? nv : 0) * RED_BLOCKS + blockIdx.x] = ss;
 }
 
 __global__ void k_finish_many(int nv, const double *__restrict__ part, double *__restrict__ out)
@@ -1267,7 +1305,11 @@ static int multidot(vfs_ctx *c, int nv, const double *V, const double *w, double
 {
     hipStream_t st = c->stream;
     const long long n = c->g.n;
-    hipLaunchKernelGGL(k_multidot, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, st, n, nv, V, n, w, c->red);
+    const dim3 gb(RED_BLOCKS), tb(RED_THREADS);
+    if (nv <= 4) hipLaunchKernelGGL((k_multidot<4, 4>), gb, tb, 0, st, n, nv, V, n, w, c->red);
+    else if (nv <= 8) hipLaunchKernelGGL((k_multidot<8, 4>), gb, tb, 0, st, n, nv, V, n, w, c->red);
+    else if (nv <= 16) hipLaunchKernelGGL((k_multidot<16, 2>), gb, tb, 0, st, n, nv, V, n, w, c->red);
+    else hipLaunchKernelGGL((k_multidot<MAXV - 1, 1>), gb, tb, 0, st, n, nv, V, n, w, c->red);
     hipLaunchKernelGGL(k_finish_many, dim3(nv + 1), dim3(RED_THREADS), 0, st, nv, c->red, c->red + MAXV * RED_BLOCKS);
     VFS_LAUNCH_CHECK("multidot");
     VFS_HIP(hipMemcpyAsync(c->red_host, c->red + MAXV * RED_BLOCKS, (nv + 1) * sizeof(double), hipMemcpyDeviceToHost, st));
@@ -1276,13 +1318,27 @@ static int multidot(vfs_ctx *c, int nv, const double *V, const double *w, double
     return VFS_OK;
 }
 
+// w -= sum_v h_v V_v (h in dev_scalars) with the partials of k_multiaxpy
+template <bool DOTS>
+static void launch_multiaxpy(vfs_ctx *c, int nv, const double *V, double *w)
+{
+    hipStream_t st = c->stream;
+    const long long n = c->g.n;
+    const double *h = c->ps->dev_scalars;
+    const dim3 gb(RED_BLOCKS), tb(RED_THREADS);
+    if (nv <= 4) hipLaunchKernelGGL((k_multiaxpy<4, 4, DOTS>), gb, tb, 0, st, n, nv, h, V, n, w, c->red);
+    else if (nv <= 8) hipLaunchKernelGGL((k_multiaxpy<8, 4, DOTS>), gb, tb, 0, st, n, nv, h, V, n, w, c->red);
+    else if (nv <= 16) hipLaunchKernelGGL((k_multiaxpy<16, 2, DOTS>), gb, tb, 0, st, n, nv, h, V, n, w, c->red);
+    else hipLaunchKernelGGL((k_multiaxpy<MAXV - 1, 1, DOTS>), gb, tb, 0, st, n, nv, h, V, n, w, c->red);
+}
+
 // w -= sum_v h_v V_v with h already in dev_scalars; returns <V_v, w_new> for
 // v < nv in d[0..nv-1] and <w_new, w_new> in *ww
 static int axpy_dots(vfs_ctx *c, int nv, const double *V, double *w, double *d, double *ww)
 {
     hipStream_t st = c->stream;
     const long long n = c->g.n;
-    hipLaunchKernelGGL(k_multiaxpy_dots, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, st, n, nv, c->ps->dev_scalars, V, n, w, c->red);
+    launch_multiaxpy<true>(c, nv, V, w);
     hipLaunchKernelGGL(k_finish_many, dim3(nv + 1), dim3(RED_THREADS), 0, st, nv, c->red, c->red + MAXV * RED_BLOCKS);
     VFS_LAUNCH_CHECK("axpy dots");
     VFS_HIP(hipMemcpyAsync(c->red_host, c->red + MAXV * RED_BLOCKS, (nv + 1) * sizeof(double), hipMemcpyDeviceToHost, st));
@@ -1297,7 +1353,7 @@ static int axpy_norm(vfs_ctx *c, int nv, const double *V, double *w, double *ww)
 {
     hipStream_t st = c->stream;
     const long long n = c->g.n;
-    hipLaunchKernelGGL(k_multiaxpy_norm, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, st, n, nv, c->ps->dev_scalars, V, n, w, c->red);
+    launch_multiaxpy<false>(c, nv, V, w);
     hipLaunchKernelGGL(finish_reduce<OpSum>, dim3(1), dim3(RED_THREADS), 0, st, c->red, RED_BLOCKS, c->red + MAXV * RED_BLOCKS);
     VFS_LAUNCH_CHECK("axpy norm");
     VFS_HIP(hipMemcpyAsync(c->red_host, c->red + MAXV * RED_BLOCKS, sizeof(double), hipMemcpyDeviceToHost, st));
