@@ -1,0 +1,46 @@
+"""Contra2Cart after writes that leave most of Ucat valid: after FormBCS's
+wall-flux reset only the cells next to the wall faces are recomputed, after
+the ghost-edge zeroing none; the ghost-cell passes run in full.  The results
+are the bits of a full recompute (VFS_C2C_PARTIAL=0), over whole fractional
+steps of the channel (no-slip j walls, periodic i and k) and of the
+slip/inlet/outlet box (slip walls on i and j, ghost snapshot path)."""
+import importlib
+
+import numpy as np
+import pytest
+
+from helpers import vfs
+
+cases = importlib.import_module("vfs-wind_amd.cases")
+pytestmark = pytest.mark.gpu
+
+
+def _context(kind):
+    c = cases.channel(24, 16, 20) if kind == "channel" else cases.box_flow(24, 20, 32)
+    p = cases.params_for(vfs, c, dt=1e-3)
+    ctx = vfs.Context(p)
+    ctx.upload("COOR", c.coor)
+    ctx.form_metrics()
+    if kind == "channel":
+        ctx.mean_k_area()
+    ctx.upload("UCONT", c.ucont)
+    ctx.contra2cart()
+    ctx.upload("UCAT_OLD", ctx.download("UCAT"))
+    p.dt = 0.3 * ctx.min_dt()
+    ctx.set_params(p)
+    return ctx
+
+
+@pytest.mark.parametrize("kind", ["channel", "box"])
+def test_partial_contra2cart_bitwise(kind, monkeypatch):
+    runs = []
+    for partial in ("1", "0"):
+        monkeypatch.setenv("VFS_C2C_PARTIAL", partial)
+        ctx = _context(kind)
+        for _ in range(2):
+            st = ctx.fractional_step(vfs.default_solve_opts(tol=1e-10))
+            assert st.poisson_relres <= 1e-10
+        runs.append([ctx.download(f) for f in ("UCONT", "UCAT", "P")])
+        ctx.close()
+    for a, b in zip(*runs):
+        assert np.array_equal(a, b)

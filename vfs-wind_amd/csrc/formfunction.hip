@@ -952,7 +952,7 @@ int formfunction2(vfs_ctx *c, double scale, const double *pucont, double rk_alph
         hipLaunchKernelGGL(k_cell_rhs, dim3(nbt), dim3(FT_I * FT_J), 0, st, ca, (const double *)c->fpc);
     prof_end(c);
     prof_end(c, whole);
-    if (pucont) c->ucat_fresh = false;  // the RK stage wrote Ucont
+    if (pucont) ucat_stale(c);  // the RK stage wrote Ucont
     VFS_LAUNCH_CHECK("cell rhs");
     return VFS_OK;
 }

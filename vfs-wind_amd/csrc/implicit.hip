@@ -255,7 +255,7 @@ int snes_function(vfs_ctx *c, const double *u, double *r, int use_feul)
     if (u != c->f[VFS_UCONT])
         VFS_HIP(hipMemcpyAsync(c->f[VFS_UCONT], u, n3 * sizeof(double), hipMemcpyDeviceToDevice, st));
     hipLaunchKernelGGL(k_snes_bc, dim3(blocks_for(g.n, IM_TPB)), dim3(IM_TPB), 0, st, g, c->f[VFS_UCONT]);
-    c->ucat_fresh = false;
+    ucat_stale(c);
     VFS_LAUNCH_CHECK("snes bc");
     int rc = c2c(c);
     if (rc) return rc;
@@ -514,12 +514,12 @@ int vfs_implicit_matrixfree(vfs_ctx *c, const vfs_newton_opts *o, vfs_newton_sta
     // post-solve boundary fix and FormBCS(outflow_scale = 1)
     VFS_HIP(hipMemcpyAsync(c->f[VFS_UCONT], s.U, vb, hipMemcpyDeviceToDevice, st));
     hipLaunchKernelGGL(k_snes_bc, dim3(blocks_for(c->g.n, IM_TPB)), dim3(IM_TPB), 0, st, c->g, c->f[VFS_UCONT]);
-    c->ucat_fresh = false;
+    ucat_stale(c);
     rc = c2c(c);
     if (rc) return fail(rc);
     hipLaunchKernelGGL(k_imp_post, dim3(blocks_for(c->g.n, IM_TPB)), dim3(IM_TPB), 0, st, c->g, c->f[VFS_ETA], c->f[VFS_ZET],
                        c->f[VFS_UCONT], c->f[VFS_UCAT]);
-    c->ucat_fresh = false;
+    ucat_stale(c);
     VFS_LAUNCH_CHECK("implicit post");
     rc = form_bcs(c, 1);
     if (rc) return fail(rc);

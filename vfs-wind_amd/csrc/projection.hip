@@ -251,7 +251,7 @@ int projection(vfs_ctx *c)
     a.ucont = c->f[VFS_UCONT];
     dim3 grd(blocks_for(g.mx - 2, QTPB), g.my - 2, g.mz - 2);
     hipLaunchKernelGGL(k_projection, grd, dim3(QTPB), 0, st, a);
-    c->ucat_fresh = false;
+    ucat_stale(c);
     VFS_LAUNCH_CHECK("projection");
     int rc = periodic_fix_grid(st, g, c->f[VFS_UCONT], 3, 1);  // poisson.c:3001-3040
     if (rc) return rc;

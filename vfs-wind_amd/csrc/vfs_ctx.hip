@@ -116,7 +116,7 @@ int vfs_ctx_create(const vfs_params *params, vfs_ctx **out)
     c->mean_k_area = 0;
     c->mean_k_flux = 0;
     c->face_derived = false;
-    c->ucat_fresh = false;
+    vfs::ucat_stale(c);
     c->imp_ws = nullptr;
     c->imp_ws_bytes = 0;
     c->prof_on = 0;
@@ -184,7 +184,7 @@ int vfs_set_params(vfs_ctx *c, const vfs_params *params)
         return vfs::set_error(VFS_EINVAL, "grid size and periodicity are fixed at creation");
     c->p = *params;
     c->g = vfs::make_grid(*params);
-    c->ucat_fresh = false;  // the boundary codes may have changed
+    vfs::ucat_stale(c);  // the boundary codes may have changed
     return VFS_OK;
 }
 
@@ -205,7 +205,7 @@ int vfs_upload(vfs_ctx *c, vfs_field f, const double *host)
 {
     if (!c || !host || f < 0 || f >= VFS_NFIELDS) return vfs::set_error(VFS_EINVAL, "bad upload argument");
     if (vfs::is_metric_field(f)) c->face_derived = false;
-    if (vfs::is_c2c_field(f)) c->ucat_fresh = false;
+    if (vfs::is_c2c_field(f)) vfs::ucat_stale(c);
     VFS_HIP(hipMemcpyAsync(c->f[f], host, vfs_field_size(c, f) * sizeof(double), hipMemcpyHostToDevice, c->stream));
     VFS_HIP(hipStreamSynchronize(c->stream));
     return VFS_OK;
@@ -223,7 +223,7 @@ double *vfs_field_device_ptr(vfs_ctx *c, vfs_field f)
 {
     if (!c || f < 0 || f >= VFS_NFIELDS) return nullptr;
     if (vfs::is_metric_field(f)) c->face_derived = false;  // the caller may write through it
-    if (vfs::is_c2c_field(f)) c->ucat_fresh = false;
+    if (vfs::is_c2c_field(f)) vfs::ucat_stale(c);
     return c->f[f];
 }
 
@@ -245,7 +245,7 @@ int vfs_form_metrics(vfs_ctx *c)
     int rc = vfs::launch_metrics(c, c->g, c->f[VFS_COOR], m);
     if (rc) return rc;
     c->face_derived = true;
-    c->ucat_fresh = false;
+    vfs::ucat_stale(c);
     // the multigrid hierarchy depends on the metrics; rebuild on next solve
     vfs::poisson_free(c);
     VFS_HIP(hipStreamSynchronize(c->stream));

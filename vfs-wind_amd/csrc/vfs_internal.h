@@ -176,6 +176,12 @@ struct vfs_ctx {
     // VFS_UCONT / VFS_UCAT_OLD / metrics / nvert (set by c2c; cleared by
     // every write to those arrays): c2c is idempotent and is then skipped
     bool ucat_fresh;
+    // what is still known of VFS_UCAT while it is not fresh: UCAT_INTERIOR =
+    // every interior cell holds Contra2Cart's value (only ghost cells were
+    // written since), UCAT_WALLS = the same except the cells next to the wall
+    // faces k_wall_flux resets; c2c then recomputes only those cells before
+    // its ghost-cell passes
+    int ucat_part;
     // Newton-GMRES vectors of the implicit momentum solve, kept between calls
     // (allocated on first use, grown when a longer restart asks for more)
     double *imp_ws;
@@ -198,6 +204,26 @@ inline bool is_c2c_field(int f)
     return f == VFS_COOR || f == VFS_NVERT || is_metric_field(f) || f == VFS_UCONT || f == VFS_UCAT || f == VFS_UCAT_OLD;
 }
 int c2c(vfs_ctx *c);
+enum { UCAT_STALE = 0, UCAT_INTERIOR = 1, UCAT_WALLS = 2 };
+// a write that may change any cell of VFS_UCAT's Contra2Cart inputs or of
+// VFS_UCAT itself
+inline void ucat_stale(vfs_ctx *c)
+{
+    c->ucat_fresh = false;
+    c->ucat_part = UCAT_STALE;
+}
+// a write of ghost cells of VFS_UCAT only
+inline void ucat_ghosts_written(vfs_ctx *c)
+{
+    if (c->ucat_fresh) c->ucat_part = UCAT_INTERIOR;
+    c->ucat_fresh = false;
+}
+// k_wall_flux reset the wall-face fluxes of VFS_UCONT
+inline void ucat_walls_written(vfs_ctx *c)
+{
+    if (c->ucat_fresh || c->ucat_part == UCAT_INTERIOR) c->ucat_part = UCAT_WALLS;
+    c->ucat_fresh = false;
+}
 int form_bcs(vfs_ctx *c, int outflow_scale);
 int inflow_flux(vfs_ctx *c, double *flux_in);
 int eddy_viscosity(vfs_ctx *c);

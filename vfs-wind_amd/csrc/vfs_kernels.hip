@@ -238,13 +238,11 @@ int launch_metrics(vfs_ctx *c, const Grid &g, const double *coor, double *const 
 // ---------------------------------------------------------------------------
 
 // interior Cramer solve, rhs.c:158-247
-__global__ void k_c2c_interior(Grid g, const double *__restrict__ ucont, const double *__restrict__ csi,
-                               const double *__restrict__ eta, const double *__restrict__ zet,
-                               const double *__restrict__ nvert, double *__restrict__ ucat)
+__device__ inline void c2c_point(const Grid &g, int k, int j, int i, const double *__restrict__ ucont,
+                                 const double *__restrict__ csi, const double *__restrict__ eta,
+                                 const double *__restrict__ zet, const double *__restrict__ nvert,
+                                 double *__restrict__ ucat)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
-    const int j = blockIdx.y + 1, k = blockIdx.z + 1;
-    if (i >= g.mx - 1) return;
     const long long q = gidx(g, k, j, i);
     if (nvert[q] >= 0.1) return;
     double q0 = 0.5 * (ucont[3 * gidx(g, k, j, i - 1) + 0] + ucont[3 * q + 0]);
@@ -260,6 +258,45 @@ __global__ void k_c2c_interior(Grid g, const double *__restrict__ ucont, const d
     ucat[3 * q + 0] = det0 / det;
     ucat[3 * q + 1] = det1 / det;
     ucat[3 * q + 2] = det2 / det;
+}
+
+__global__ void k_c2c_interior(Grid g, const double *__restrict__ ucont, const double *__restrict__ csi,
+                               const double *__restrict__ eta, const double *__restrict__ zet,
+                               const double *__restrict__ nvert, double *__restrict__ ucat)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int j = blockIdx.y + 1, k = blockIdx.z + 1;
+    if (i >= g.mx - 1) return;
+    c2c_point(g, k, j, i, ucont, csi, eta, zet, nvert, ucat);
+}
+
+// the interior cells whose Contra2Cart inputs k_wall_flux may change: the
+// planes i = 1 / i = mx-2 / j = 1 / j = my-2 next to a wall face (bc 1 or 10)
+__device__ inline bool wall_face(const Grid &g, int side) { return g.bc[side] == 1 || g.bc[side] == 10; }
+__global__ void k_c2c_walls(Grid g, const double *__restrict__ ucont, const double *__restrict__ csi,
+                            const double *__restrict__ eta, const double *__restrict__ zet,
+                            const double *__restrict__ nvert, double *__restrict__ ucat)
+{
+    long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    const long long ni = (long long)(g.my - 2) * (g.mz - 2), nj = (long long)(g.mx - 2) * (g.mz - 2);
+    const int side = (int)(t / (ni > nj ? ni : nj));
+    const long long r = t % (ni > nj ? ni : nj);
+    if (side >= 4 || !wall_face(g, side)) return;
+    int i, j, k;
+    if (side < 2) {
+        if (r >= ni) return;
+        j = 1 + (int)(r % (g.my - 2));
+        k = 1 + (int)(r / (g.my - 2));
+        i = side == 0 ? 1 : g.mx - 2;
+    } else {
+        if (r >= nj) return;
+        i = 1 + (int)(r % (g.mx - 2));
+        k = 1 + (int)(r / (g.mx - 2));
+        j = side == 2 ? 1 : g.my - 2;
+        // cells of an i-wall plane are done by that plane's threads
+        if ((i == 1 && wall_face(g, 0)) || (i == g.mx - 2 && wall_face(g, 1))) return;
+    }
+    c2c_point(g, k, j, i, ucont, csi, eta, zet, nvert, ucat);
 }
 
 // slip reflection, rhs.c:447-544
@@ -412,12 +449,26 @@ int c2c(vfs_ctx *c)
     double *ucont = c->f[VFS_UCONT], *ucat = c->f[VFS_UCAT];
     int rc = periodic_fix(c, ucont, 3, 0);  // rhs.c:129-156
     if (rc) return rc;
-    dim3 blk(TPB), grd(blocks_for(g.mx - 2, TPB), g.my - 2, g.mz - 2);
-    prof_begin(c, K_C2C);
-    hipLaunchKernelGGL(k_c2c_interior, grd, blk, 0, st, g, ucont, c->f[VFS_CSI], c->f[VFS_ETA], c->f[VFS_ZET],
-                       c->f[VFS_NVERT], ucat);
-    prof_end(c);
-    VFS_LAUNCH_CHECK("c2c interior");
+    // The interior cells are a pointwise function of Ucont, the metrics and
+    // nvert; where they are known to hold that function already, only the
+    // cells whose inputs changed are recomputed.  The ghost-cell passes below
+    // then see the same interior and the same ghost values as after a full
+    // interior pass.
+    const char *pe = std::getenv("VFS_C2C_PARTIAL");
+    const int part = pe && pe[0] == '0' ? (int)UCAT_STALE : c->ucat_part;
+    if (part == UCAT_WALLS) {
+        const long long ni = (long long)(g.my - 2) * (g.mz - 2), nj = (long long)(g.mx - 2) * (g.mz - 2);
+        hipLaunchKernelGGL(k_c2c_walls, dim3(blocks_for(4 * (ni > nj ? ni : nj), TPB)), dim3(TPB), 0, st, g, ucont,
+                           c->f[VFS_CSI], c->f[VFS_ETA], c->f[VFS_ZET], c->f[VFS_NVERT], ucat);
+        VFS_LAUNCH_CHECK("c2c walls");
+    } else if (part != UCAT_INTERIOR) {
+        dim3 blk(TPB), grd(blocks_for(g.mx - 2, TPB), g.my - 2, g.mz - 2);
+        prof_begin(c, K_C2C);
+        hipLaunchKernelGGL(k_c2c_interior, grd, blk, 0, st, g, ucont, c->f[VFS_CSI], c->f[VFS_ETA], c->f[VFS_ZET],
+                           c->f[VFS_NVERT], ucat);
+        prof_end(c);
+        VFS_LAUNCH_CHECK("c2c interior");
+    }
     rc = periodic_fix(c, ucat, 3, 0);  // rhs.c:254-291
     if (rc) return rc;
     const double *L2 = ucat;
@@ -439,7 +490,10 @@ int c2c(vfs_ctx *c)
         VFS_LAUNCH_CHECK("c2c corner");
     }
     rc = periodic_fix(c, ucat, 3, 0);  // rhs.c:712-748
-    if (!rc) c->ucat_fresh = true;
+    if (!rc) {
+        c->ucat_fresh = true;
+        c->ucat_part = UCAT_STALE;
+    }
     return rc;
 }
 
@@ -581,7 +635,7 @@ int inflow_flux(vfs_ctx *c, double *flux_in)
     }
     hipLaunchKernelGGL(k_inflow, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, st, g, 1, uin, c->f[VFS_CSI], c->f[VFS_ETA],
                        c->f[VFS_ZET], c->f[VFS_NVERT], c->f[VFS_UCONT], c->f[VFS_UCAT], c->red);
-    c->ucat_fresh = false;
+    ucat_stale(c);
     hipLaunchKernelGGL(finish_reduce<OpSum>, dim3(1), dim3(RED_THREADS), 0, st, c->red, RED_BLOCKS, c->red + RED_BLOCKS);
     VFS_LAUNCH_CHECK("inflow flux");
     VFS_HIP(hipMemcpyAsync(c->red_host, c->red + RED_BLOCKS, sizeof(double), hipMemcpyDeviceToHost, st));
@@ -608,20 +662,20 @@ int form_bcs(vfs_ctx *c, int outflow_scale)
             long long n = (long long)(g.mx - 2) * (g.my - 2);
             hipLaunchKernelGGL(k_outlet_scale, dim3(blocks_for(n, TPB)), dim3(TPB), 0, st, g, c->p.flux_in_sum,
                                c->red + 2 * RED_BLOCKS, c->f[VFS_ZET], c->f[VFS_NVERT], c->f[VFS_UCONT]);
-            c->ucat_fresh = false;
+            ucat_stale(c);
             VFS_LAUNCH_CHECK("outlet scale");
         }
     }
     long long nw = (long long)(g.my > g.mx ? g.my : g.mx) * g.mz;
     hipLaunchKernelGGL(k_wall_flux, dim3(blocks_for(nw, TPB)), dim3(TPB), 0, st, g, c->f[VFS_UCONT]);
-    c->ucat_fresh = false;
+    ucat_walls_written(c);
     VFS_LAUNCH_CHECK("wall flux");
     rc = c2c(c);
     if (rc) return rc;
     int ne = g.mx > g.my ? g.mx : g.my;
     if (g.mz > ne) ne = g.mz;
     hipLaunchKernelGGL(k_zero_edges, dim3(blocks_for(ne, TPB)), dim3(TPB), 0, st, g, c->f[VFS_UCAT]);
-    c->ucat_fresh = false;
+    ucat_ghosts_written(c);
     VFS_LAUNCH_CHECK("zero edges");
     return VFS_OK;
 }
