@@ -287,6 +287,10 @@ int vfs_poisson_solve_learned(vfs_ctx *ctx, vfs_unet *net, const vfs_solve_opts 
 /* Enable (1) or disable (0) HIP-event timing of the hot kernels' launches on
  * the context stream and clear earlier records.  For benchmarks only. */
 int vfs_profile(vfs_ctx *ctx, int enable);
+/* The same, recording only the kernel ids whose bit (1u << id) is set in
+ * kernel_mask (0 disables); nested regions of unselected ids record nothing.
+ * Fewer event pairs on the stream perturb the timed run less. */
+int vfs_profile_select(vfs_ctx *ctx, unsigned int kernel_mask);
 /* Total time and launch count recorded for kernel id 0..11 (finest level for
  * the smoother): face_fluxes, cell_rhs, line_rhs_fp32, line_rhs_fp64,
  * poisson_apply_fp64, contra2cart, pressure_gradient, eddy_viscosity,

@@ -167,6 +167,7 @@ _SIGS = {
     "vfs_unet_set_params": ([C.c_void_p, C.c_void_p], C.c_int),
     "vfs_poisson_solve_learned": ([C.c_void_p, C.c_void_p, C.POINTER(SolveOpts), C.POINTER(SolveStats)], C.c_int),
     "vfs_profile": ([C.c_void_p, C.c_int], C.c_int),
+    "vfs_profile_select": ([C.c_void_p, C.c_uint], C.c_int),
     "vfs_profile_read": ([C.c_void_p, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_longlong),
                           C.POINTER(C.c_char_p)], C.c_int),
 }
@@ -359,8 +360,16 @@ class Context:
         self._check(self.lib.vfs_divergence(self.h, C.byref(v)))
         return v.value
 
-    def profile(self, enable: bool):
-        self._check(self.lib.vfs_profile(self.h, int(enable)))
+    def profile(self, enable: bool, only=None):
+        """Start (clearing earlier records) or stop HIP-event timing; `only`
+        restricts it to the named kernels (KERNEL_IDS)."""
+        if enable and only:
+            mask = 0
+            for k in only:
+                mask |= 1 << KERNEL_IDS[k]
+            self._check(self.lib.vfs_profile_select(self.h, mask))
+        else:
+            self._check(self.lib.vfs_profile(self.h, int(enable)))
 
     def profile_read(self, kernel: str):
         """(total ms, launch count) recorded for a kernel since profile(True)."""

@@ -62,9 +62,15 @@ int check_hip(hipError_t e, const char *what)
     return e == hipErrorOutOfMemory ? VFS_ENOMEM : VFS_EHIP;
 }
 
+// Regions nest; each prof_begin pushes its record index (or -1 when the
+// kernel is not selected) and the matching prof_end pops it.
 int prof_begin(vfs_ctx *c, int kernel_id)
 {
     if (!c->prof_on) return -1;
+    if (!((c->prof_mask >> kernel_id) & 1u)) {
+        c->prof_stack->push_back(-1);
+        return -1;
+    }
     size_t at = c->ev_recs->size() * 2;
     while (c->ev_pool->size() < at + 2) {
         hipEvent_t ev;
@@ -73,15 +79,19 @@ int prof_begin(vfs_ctx *c, int kernel_id)
     }
     hipEventRecord((*c->ev_pool)[at], c->stream);
     c->ev_recs->push_back({kernel_id, (int)at});
-    return (int)c->ev_recs->size() - 1;
+    const int idx = (int)c->ev_recs->size() - 1;
+    c->prof_stack->push_back(idx);
+    return idx;
 }
 
 void prof_end(vfs_ctx *c, int idx)
 {
-    if (!c->prof_on || c->ev_recs->empty()) return;
-    if (idx >= (int)c->ev_recs->size()) return;
-    const auto &r = idx < 0 ? c->ev_recs->back() : (*c->ev_recs)[idx];
-    hipEventRecord((*c->ev_pool)[r.second + 1], c->stream);
+    if (!c->prof_on || c->prof_stack->empty()) return;
+    const int top = c->prof_stack->back();
+    c->prof_stack->pop_back();
+    if (idx < 0) idx = top;
+    if (idx < 0 || idx >= (int)c->ev_recs->size()) return;
+    hipEventRecord((*c->ev_pool)[(*c->ev_recs)[idx].second + 1], c->stream);
 }
 
 Grid make_grid(const vfs_params &p)
@@ -120,8 +130,10 @@ int vfs_ctx_create(const vfs_params *params, vfs_ctx **out)
     c->imp_ws = nullptr;
     c->imp_ws_bytes = 0;
     c->prof_on = 0;
+    c->prof_mask = 0;
     c->ev_pool = new std::vector<hipEvent_t>();
     c->ev_recs = new std::vector<std::pair<int, int>>();
+    c->prof_stack = new std::vector<int>();
     hipError_t e = hipStreamCreate(&c->stream);
     if (e != hipSuccess) { delete c; return vfs::check_hip(e, "hipStreamCreate"); }
     const size_t n = (size_t)c->g.n;
@@ -169,6 +181,7 @@ int vfs_ctx_destroy(vfs_ctx *c)
         delete c->ev_pool;
     }
     delete c->ev_recs;
+    delete c->prof_stack;
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;
     return VFS_OK;
@@ -393,10 +406,17 @@ static const char *k_names[vfs::K_NUMK] = {"face_fluxes",     "cell_rhs",       
 
 int vfs_profile(vfs_ctx *c, int enable)
 {
+    return vfs_profile_select(c, enable ? ~0u : 0u);
+}
+
+int vfs_profile_select(vfs_ctx *c, unsigned int kernel_mask)
+{
     if (!c) return vfs::set_error(VFS_EINVAL, "null context");
     VFS_HIP(hipStreamSynchronize(c->stream));
     c->ev_recs->clear();
-    c->prof_on = enable ? 1 : 0;
+    c->prof_stack->clear();
+    c->prof_mask = kernel_mask;
+    c->prof_on = kernel_mask ? 1 : 0;
     return VFS_OK;
 }
 

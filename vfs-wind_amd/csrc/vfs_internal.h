@@ -189,6 +189,8 @@ struct vfs_ctx {
     vfs_poisson *ps;
     // per-kernel timing (vfs_profile): event pairs recorded around launches
     int prof_on;
+    unsigned prof_mask;                // kernel ids recorded while prof_on (bit id)
+    std::vector<int> *prof_stack;      // open regions: record index, or -1 when not recorded
     std::vector<hipEvent_t> *ev_pool;
     std::vector<std::pair<int, int>> *ev_recs;  // (kernel id, index of start event)
 };
