@@ -819,15 +819,19 @@ __global__ void __launch_bounds__(FT_I * FT_J) k_cell_rhs(CellArgs a, const doub
 // k_cell_rhs on the projections written by k_face_fp: pr[0..2] planes hold
 // csi.Fp, eta.Fp, zet.Fp of every cell, so the contravariant projection
 // reads no cell metrics (the same products, hence the same bits).
-__global__ void __launch_bounds__(FT_I * FT_J) k_cell_rhs_pr(CellArgs a, const double *__restrict__ pr)
+// The cell pass needs no LDS tile; rows of 64 points (1.5 KB of each AoS
+// array) leave fewer partly written cache lines at block edges than the
+// face pass's 32-point rows (measured: 0.372 -> 0.344 ms at 256x192x256).
+constexpr int CR_I = 64, CR_J = 4;
+__global__ void __launch_bounds__(CR_I * CR_J) k_cell_rhs_pr(CellArgs a, const double *__restrict__ pr)
 {
     const Grid &g = a.g;
-    const int nbi = (g.mx + FT_I - 1) / FT_I, nbj = (g.my + FT_J - 1) / FT_J;
+    const int nbi = (g.mx + CR_I - 1) / CR_I, nbj = (g.my + CR_J - 1) / CR_J;
     int b = blockIdx.x;
     const int bi = b % nbi;
     b /= nbi;
     const int bj = b % nbj, bk = b / nbj;
-    const int i = bi * FT_I + threadIdx.x % FT_I, j = bj * FT_J + threadIdx.x / FT_I, k0 = bk * FT_K;
+    const int i = bi * CR_I + threadIdx.x % CR_I, j = bj * CR_J + threadIdx.x / CR_I, k0 = bk * FT_K;
     if (i >= g.mx || j >= g.my) return;
     const int k1 = k0 + FT_K < g.mz ? k0 + FT_K : g.mz;
     const long long n = g.n;
@@ -947,7 +951,9 @@ int formfunction2(vfs_ctx *c, double scale, const double *pucont, double rk_alph
     }
     prof_begin(c, K_CELL);
     if (fused)
-        hipLaunchKernelGGL(k_cell_rhs_pr, dim3(nbt), dim3(FT_I * FT_J), 0, st, ca, (const double *)c->fpc);
+        hipLaunchKernelGGL(k_cell_rhs_pr,
+                           dim3((unsigned)(((g.mx + CR_I - 1) / CR_I) * ((g.my + CR_J - 1) / CR_J) * ((g.mz + FT_K - 1) / FT_K))),
+                           dim3(CR_I * CR_J), 0, st, ca, (const double *)c->fpc);
     else
         hipLaunchKernelGGL(k_cell_rhs, dim3(nbt), dim3(FT_I * FT_J), 0, st, ca, (const double *)c->fpc);
     prof_end(c);

@@ -346,6 +346,46 @@ __device__ inline float half_ratio(double a, double d)
     return v > 65504.f ? 65504.f : (v < -65504.f ? -65504.f : v);
 }
 
+// Line right-hand sides of the FP32 smoother, rz = b - d * sum (a/d) x over
+// the off-line points with the FP16 ratios, for two consecutive points
+// (j, j+1) of a line per thread: their off-line neighbours share the rows j
+// and j+1, so a pair reads 24 values of x instead of 32.
+__global__ void __launch_bounds__(PTPB) k_zebra_rhs_h2(Grid g, ZGrid z, int c, const h2_t *__restrict__ sh,
+                                                       const float *__restrict__ sd, const float *__restrict__ x,
+                                                       const float *__restrict__ b, float *__restrict__ rz)
+{
+    long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    const int npair = (z.njI + 1) >> 1;
+    if (t >= (long long)z.nkI * npair * z.nh) return;
+    const int h = t % z.nh;
+    const long long r = t / z.nh;
+    const int jp = r % npair, kk = r / npair;
+    const int k = kk + 1, j = 2 * jp + 1, i = zebra_i0(k, c) + 2 * h;
+    if (i > g.mx - 2) return;
+    const int np = j + 1 <= g.my - 2 ? 2 : 1;
+    const int I[3] = {canon(i - 1, g.mx, g.pi), i, canon(i + 1, g.mx, g.pi)};
+    const int K[3] = {canon(k - 1, g.mz, g.pk), k, canon(k + 1, g.mz, g.pk)};
+    const int J[4] = {canon(j - 1, g.my, g.pj), j, canon(j + 1, g.my, g.pj), np == 2 ? canon(j + 2, g.my, g.pj) : j};
+    const long long sstride = 2 * z.cplane;
+    const long long cl0 = ((long long)kk * z.njI + 2 * jp) * z.nh + h;  // colour-local index of point j
+#pragma unroll
+    for (int p = 0; p < 2; p++) {
+        if (p >= np) break;
+        const long long cl = cl0 + p * z.nh, cq = c * z.cplane + cl;
+        h2_t w[NOFF / 2];
+#pragma unroll
+        for (int u = 0; u < NOFF / 2; u++) w[u] = sh[u * sstride + cq];
+        float acc = 0;
+#pragma unroll
+        for (int u = 0; u < NOFF; u++) {
+            const int sl = offline_slot(u);
+            const float xv = x[((long long)K[soff(sl, 2) + 1] * g.my + J[p + soff(sl, 1) + 1]) * g.mx + I[soff(sl, 0) + 1]];
+            acc += (float)((u & 1) ? w[u / 2].y : w[u / 2].x) * xv;
+        }
+        rz[cl] = b[((long long)k * g.my + j + p) * g.mx + i] - sd[cq] * acc;
+    }
+}
+
 // colour-split off-line coefficients and Thomas factors (lo, 1/pivot,
 // upper/pivot).  T = double: sc holds the 16 off-line planes, then the three
 // factor planes.  T = float: sc holds the three factor planes, sh the FP16
@@ -418,33 +458,6 @@ __global__ void __launch_bounds__(PTPB) k_zebra_rhs(Grid g, ZGrid z, int c, cons
     rz[t] = acc;
 }
 
-// the same for the FP32 smoother: rz = b - d * sum (a/d) x over the off-line
-// points, with the FP16 ratios
-__global__ void __launch_bounds__(PTPB) k_zebra_rhs_h(Grid g, ZGrid z, int c, const h2_t *__restrict__ sh,
-                                                      const float *__restrict__ sd, const float *__restrict__ x,
-                                                      const float *__restrict__ b, float *__restrict__ rz)
-{
-    long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-    if (t >= z.cplane) return;
-    const int h = t % z.nh;
-    const long long r = t / z.nh;
-    const int jj = r % z.njI, kk = r / z.njI;
-    const int k = kk + 1, j = jj + 1, i = zebra_i0(k, c) + 2 * h;
-    if (i > g.mx - 2) return;
-    const long long q = ((long long)k * g.my + j) * g.mx + i;
-    const h2_t *s0 = sh + c * z.cplane + t;
-    const long long sstride = 2 * z.cplane;
-    h2_t w[NOFF / 2];
-#pragma unroll
-    for (int p = 0; p < NOFF / 2; p++) w[p] = s0[p * sstride];
-    float acc = 0;
-#pragma unroll
-    for (int p = 0; p < NOFF / 2; p++) {
-        acc += (float)w[p].x * x[nbr_idx(g, k, j, i, offline_slot(2 * p))];
-        acc += (float)w[p].y * x[nbr_idx(g, k, j, i, offline_slot(2 * p + 1))];
-    }
-    rz[t] = b[q] - sd[c * z.cplane + t] * acc;
-}
 
 // Thomas recurrences of colour c, one thread per line; rz holds the line
 // right-hand sides on entry and the forward-eliminated values afterwards.
@@ -1197,8 +1210,9 @@ static void smooth(vfs_ctx *c, PLevel &L, int sweeps)
         for (int col = 0; col < 2; col++) {
             if (fine) prof_begin(c, sizeof(T) == 4 ? K_LINE32 : K_LINE64);
             if constexpr (sizeof(T) == 4)
-                hipLaunchKernelGGL(k_zebra_rhs_h, dim3(blocks_for(L.z.cplane, PTPB)), dim3(PTPB), 0, st, L.g, L.z, col,
-                                   L.sh16, L.sd32, L.x32, L.b32, L.rz32);
+                hipLaunchKernelGGL(k_zebra_rhs_h2,
+                                   dim3(blocks_for((long long)L.z.nkI * ((L.z.njI + 1) / 2) * L.z.nh, PTPB)), dim3(PTPB),
+                                   0, st, L.g, L.z, col, L.sh16, L.sd32, L.x32, L.b32, L.rz32);
             else
                 hipLaunchKernelGGL(k_zebra_rhs<double>, dim3(blocks_for(L.z.cplane, PTPB)), dim3(PTPB), 0, st, L.g, L.z,
                                    col, L.sc64, L.x64, L.b64, L.rz64);
