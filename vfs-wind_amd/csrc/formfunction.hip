@@ -49,7 +49,7 @@ struct FaceArgs {
 // reads outside the tile go to HBM.  The tile holds exactly the values the
 // DMDA index with periodic wrap would return.  nvert (blanking codes, small
 // integers) is kept as float in LDS, which represents them exactly.
-constexpr int FT_I = 32, FT_J = 8, FT_K = 16;
+constexpr int FT_I = 32, FT_J = 16, FT_K = 16;
 
 template <bool SO>
 struct TileDims {
