@@ -145,6 +145,10 @@ __global__ void __launch_bounds__(QTPB) k_projection(ProjArgs a)
     const long long q = gidx(g, k, j, i);
 #define PH(kk, jj, ii) a.phi[gidx(g, (kk), (jj), (ii))]
 #define NV(kk, jj, ii) a.nvert[gidx(g, (kk), (jj), (ii))]
+    // the three components are read and written together (separate
+    // read-modify-writes of one component of the AoS line each evicted
+    // partly written lines)
+    double u[3] = {a.ucont[3 * q + 0], a.ucont[3 * q + 1], a.ucont[3 * q + 2]};
     double dpdc, dpde, dpdz;
     if (i < mx - 2 || (pi_ && i == mx - 2)) {
         dpdc = PH(k, j, i + 1) - PH(k, j, i);
@@ -169,9 +173,9 @@ __global__ void __launch_bounds__(QTPB) k_projection(ProjArgs a)
         if ((NV(k, j, i) + NV(k, j, i + 1)) < th) {
             const double *ic = a.icsi + 3 * q, *ie = a.ieta + 3 * q, *iz = a.izet + 3 * q;
             const double iaj = a.iaj[q];
-            a.ucont[3 * q + 0] -= (dpdc * (ic[0] * ic[0] + ic[1] * ic[1] + ic[2] * ic[2]) * iaj +
-                                   dpde * (ie[0] * ic[0] + ie[1] * ic[1] + ie[2] * ic[2]) * iaj +
-                                   dpdz * (iz[0] * ic[0] + iz[1] * ic[1] + iz[2] * ic[2]) * iaj) * a.fac;
+            u[0] -= (dpdc * (ic[0] * ic[0] + ic[1] * ic[1] + ic[2] * ic[2]) * iaj +
+                     dpde * (ie[0] * ic[0] + ie[1] * ic[1] + ie[2] * ic[2]) * iaj +
+                     dpdz * (iz[0] * ic[0] + iz[1] * ic[1] + iz[2] * ic[2]) * iaj) * a.fac;
         }
     }
     if (j < my - 2 || (pj && j == my - 2)) {
@@ -197,9 +201,9 @@ __global__ void __launch_bounds__(QTPB) k_projection(ProjArgs a)
         if ((NV(k, j, i) + NV(k, j + 1, i)) < th || (int)(NV(k, j, i) + NV(k, j + 1, i)) == 5) {
             const double *jc = a.jcsi + 3 * q, *je = a.jeta + 3 * q, *jz = a.jzet + 3 * q;
             const double jaj = a.jaj[q];
-            a.ucont[3 * q + 1] -= (dpdc * (jc[0] * je[0] + jc[1] * je[1] + jc[2] * je[2]) * jaj +
-                                   dpde * (je[0] * je[0] + je[1] * je[1] + je[2] * je[2]) * jaj +
-                                   dpdz * (jz[0] * je[0] + jz[1] * je[1] + jz[2] * je[2]) * jaj) * a.fac;
+            u[1] -= (dpdc * (jc[0] * je[0] + jc[1] * je[1] + jc[2] * je[2]) * jaj +
+                     dpde * (je[0] * je[0] + je[1] * je[1] + je[2] * je[2]) * jaj +
+                     dpdz * (jz[0] * je[0] + jz[1] * je[1] + jz[2] * je[2]) * jaj) * a.fac;
         }
     }
     if (k < mz - 2 || (pk && k == mz - 2)) {
@@ -225,11 +229,14 @@ __global__ void __launch_bounds__(QTPB) k_projection(ProjArgs a)
         if ((NV(k, j, i) + NV(k + 1, j, i)) < th) {
             const double *kc = a.kcsi + 3 * q, *ke = a.keta + 3 * q, *kz = a.kzet + 3 * q;
             const double kaj = a.kaj[q];
-            a.ucont[3 * q + 2] -= (dpdc * (kc[0] * kz[0] + kc[1] * kz[1] + kc[2] * kz[2]) * kaj +
-                                   dpde * (ke[0] * kz[0] + ke[1] * kz[1] + ke[2] * kz[2]) * kaj +
-                                   dpdz * (kz[0] * kz[0] + kz[1] * kz[1] + kz[2] * kz[2]) * kaj) * a.fac;
+            u[2] -= (dpdc * (kc[0] * kz[0] + kc[1] * kz[1] + kc[2] * kz[2]) * kaj +
+                     dpde * (ke[0] * kz[0] + ke[1] * kz[1] + ke[2] * kz[2]) * kaj +
+                     dpdz * (kz[0] * kz[0] + kz[1] * kz[1] + kz[2] * kz[2]) * kaj) * a.fac;
         }
     }
+    a.ucont[3 * q + 0] = u[0];
+    a.ucont[3 * q + 1] = u[1];
+    a.ucont[3 * q + 2] = u[2];
 #undef PH
 #undef NV
 }
