@@ -215,6 +215,31 @@ def test_poisson_graph_replays(pair, fp32, monkeypatch):
         ctx.sync()
 
 
+def test_k_marching_kernels(pair, monkeypatch):
+    """The k-marching eddy viscosity and divergence give the bits of the
+    thread-per-point kernels (VFS_EDDY_KM=0, VFS_REDUCE_KM=0); the k-marching
+    Poisson right-hand side sums in another order (1e-14)."""
+    c, o, ctx = pair
+    saved = {f: ctx.download(f) for f in ("NU_T", "B")}
+    out = {}
+    for km in ("1", "0"):
+        monkeypatch.setenv("VFS_EDDY_KM", km)
+        monkeypatch.setenv("VFS_REDUCE_KM", km)
+        ctx.upload("NU_T", np.full(o.shape, 7.0))
+        ctx.upload("B", np.full(o.shape, 7.0))
+        ctx.eddy_viscosity()
+        d = ctx.divergence()
+        ctx.poisson_rhs()
+        out[km] = (ctx.download("NU_T"), d, ctx.download("B"))
+    monkeypatch.delenv("VFS_EDDY_KM")
+    monkeypatch.delenv("VFS_REDUCE_KM")
+    assert np.array_equal(out["1"][0], out["0"][0])
+    assert out["1"][1] == out["0"][1]
+    assert relerr(out["1"][2], out["0"][2]) <= 1e-14
+    for f, v in saved.items():  # the fixture is shared by the module's tests
+        ctx.upload(f, v)
+
+
 def test_update_projection_divergence(pair):
     c, o, ctx = pair
     o.poisson_rhs()

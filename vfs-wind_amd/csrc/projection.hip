@@ -49,6 +49,61 @@ __global__ void k_poisson_rhs(Grid g, double fac, const double *__restrict__ uco
     if (threadIdx.x == 0) { part[blockIdx.x] = sb; part[RED_BLOCKS + blockIdx.x] = sa; }
 }
 
+// The same over the interior with k-marching tiles (ucont.z of plane k-1
+// carried in registers); the ghost cells of b get 0 from k_zero_ghosts.
+__global__ void __launch_bounds__(KM_I * KM_J) k_poisson_rhs_km(Grid g, int kc, double fac, const double *__restrict__ ucont,
+                                                               const double *__restrict__ nvert,
+                                                               const double *__restrict__ aj, double *__restrict__ b,
+                                                               double *__restrict__ part)
+{
+    double sb = 0, sa = 0;
+    int i, j, k0, k1;
+    if (km_tile(g, kc, i, j, k0, k1)) {
+        const long long P = g.plane;
+        long long t = gidx(g, k0, j, i);
+        double wz = k0 == 1 && g.pk ? ucont[3 * gidx(g, -2, j, i) + 2] : ucont[3 * (t - P) + 2];
+        for (int k = k0; k < k1; k++, t += P) {
+            const double uz = ucont[3 * t + 2];
+            double val = 0;
+            if (nvert[t] < 0.1) {
+                val -= ucont[3 * t + 0];
+                if (i == 1 && g.pi) val += ucont[3 * gidx(g, k, j, -2) + 0];
+                else val += ucont[3 * (t - 1) + 0];
+                val -= ucont[3 * t + 1];
+                if (j == 1 && g.pj) val += ucont[3 * gidx(g, k, -2, i) + 1];
+                else val += ucont[3 * (t - g.mx) + 1];
+                val -= uz;
+                val += wz;
+                val *= fac;
+                sb += val;
+                sa += aj[t];
+            }
+            b[t] = val;
+            wz = uz;
+        }
+    }
+    sb = block_reduce<OpSum>(sb);
+    sa = block_reduce<OpSum>(sa);
+    if (threadIdx.x == 0) { part[blockIdx.x] = sb; part[RED_BLOCKS + blockIdx.x] = sa; }
+}
+
+// planes per block of a k-marching reduction so that the blocks fit the
+// RED_BLOCKS partials (0 when even whole columns do not)
+static int km_chunk(const Grid &g)
+{
+    const long long cols = (long long)((g.mx - 2 + KM_I - 1) / KM_I) * ((g.my - 2 + KM_J - 1) / KM_J);
+    if (cols > RED_BLOCKS) return 0;
+    int kc = 1;
+    while (cols * ((g.mz - 2 + kc - 1) / kc) > RED_BLOCKS) kc++;
+    return kc;
+}
+
+static bool km_reductions()
+{
+    const char *e = std::getenv("VFS_REDUCE_KM");
+    return !(e && e[0] == '0');
+}
+
 // Remove_Nullspace_Scale, poisson_hypre.c:118-200: b -= sum(b) * aj / sum(aj)
 __global__ void k_null_scale(Grid g, const double *__restrict__ sums, const double *__restrict__ nvert,
                              const double *__restrict__ aj, double *__restrict__ b)
@@ -68,10 +123,20 @@ int poisson_rhs(vfs_ctx *c)
     hipStream_t st = c->stream;
     const double coeff = 1.;  // time_coeff(), poisson.c:23-31
     const double fac = -1.0 / c->p.dt * c->p.st * coeff;
-    hipLaunchKernelGGL(k_poisson_rhs, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, st, g, fac, c->f[VFS_UCONT],
-                       c->f[VFS_NVERT], c->f[VFS_AJ], c->f[VFS_B], c->red);
-    hipLaunchKernelGGL(finish_reduce<OpSum>, dim3(1), dim3(RED_THREADS), 0, st, c->red, RED_BLOCKS, c->red + 2 * RED_BLOCKS);
-    hipLaunchKernelGGL(finish_reduce<OpSum>, dim3(1), dim3(RED_THREADS), 0, st, c->red + RED_BLOCKS, RED_BLOCKS,
+    const int kc = km_reductions() ? km_chunk(g) : 0;
+    int nparts = RED_BLOCKS;
+    if (kc > 0) {
+        int rc = zero_ghosts(c, c->f[VFS_B]);
+        if (rc) return rc;
+        nparts = (int)km_blocks(g, kc);
+        hipLaunchKernelGGL(k_poisson_rhs_km, dim3(nparts), dim3(KM_I * KM_J), 0, st, g, kc, fac, c->f[VFS_UCONT],
+                           c->f[VFS_NVERT], c->f[VFS_AJ], c->f[VFS_B], c->red);
+    } else {
+        hipLaunchKernelGGL(k_poisson_rhs, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, st, g, fac, c->f[VFS_UCONT],
+                           c->f[VFS_NVERT], c->f[VFS_AJ], c->f[VFS_B], c->red);
+    }
+    hipLaunchKernelGGL(finish_reduce<OpSum>, dim3(1), dim3(RED_THREADS), 0, st, c->red, nparts, c->red + 2 * RED_BLOCKS);
+    hipLaunchKernelGGL(finish_reduce<OpSum>, dim3(1), dim3(RED_THREADS), 0, st, c->red + RED_BLOCKS, nparts,
                        c->red + 2 * RED_BLOCKS + 1);
     hipLaunchKernelGGL(k_null_scale, dim3(blocks_for(g.n, QTPB)), dim3(QTPB), 0, st, g, c->red + 2 * RED_BLOCKS,
                        c->f[VFS_NVERT], c->f[VFS_AJ], c->f[VFS_B]);
@@ -290,12 +355,46 @@ __global__ void k_divergence(Grid g, const double *__restrict__ ucont, const dou
     if (threadIdx.x == 0) part[blockIdx.x] = v;
 }
 
+// The same with k-marching tiles: ucont.z and nvert of the planes k-1, k,
+// k+1 at the thread's own (i, j) carried in registers
+__global__ void __launch_bounds__(KM_I * KM_J) k_divergence_km(Grid g, int kc, const double *__restrict__ ucont,
+                                                              const double *__restrict__ aj,
+                                                              const double *__restrict__ nvert, double *__restrict__ part)
+{
+    double v = 0;
+    int i, j, k0, k1;
+    if (km_tile(g, kc, i, j, k0, k1)) {
+        const long long P = g.plane;
+        long long q = gidx(g, k0, j, i);
+        double wz = ucont[3 * (q - P) + 2], nvm = nvert[q - P], nv0 = nvert[q];
+        for (int k = k0; k < k1; k++, q += P) {
+            const double uz = ucont[3 * q + 2], nvp = nvert[q + P];
+            double d = fabs((ucont[3 * q] - ucont[3 * (q - 1)] + ucont[3 * q + 1] - ucont[3 * (q - g.mx) + 1] + uz - wz) * aj[q]);
+            if (nv0 + nvp + nvm + nvert[q + g.mx] + nvert[q - g.mx] + nvert[q + 1] + nvert[q - 1] > 0.1) d = 0.;
+            v = fmax(v, d);
+            wz = uz;
+            nvm = nv0;
+            nv0 = nvp;
+        }
+    }
+    v = block_reduce<OpMax>(v);
+    if (threadIdx.x == 0) part[blockIdx.x] = v;
+}
+
 int divergence(vfs_ctx *c, double *out)
 {
     hipStream_t st = c->stream;
-    hipLaunchKernelGGL(k_divergence, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, st, c->g, c->f[VFS_UCONT], c->f[VFS_AJ],
-                       c->f[VFS_NVERT], c->red);
-    hipLaunchKernelGGL(finish_reduce<OpMax>, dim3(1), dim3(RED_THREADS), 0, st, c->red, RED_BLOCKS, c->red + RED_BLOCKS);
+    const int kc = km_reductions() ? km_chunk(c->g) : 0;
+    int nparts = RED_BLOCKS;
+    if (kc > 0) {
+        nparts = (int)km_blocks(c->g, kc);
+        hipLaunchKernelGGL(k_divergence_km, dim3(nparts), dim3(KM_I * KM_J), 0, st, c->g, kc, c->f[VFS_UCONT], c->f[VFS_AJ],
+                           c->f[VFS_NVERT], c->red);
+    } else {
+        hipLaunchKernelGGL(k_divergence, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, st, c->g, c->f[VFS_UCONT], c->f[VFS_AJ],
+                           c->f[VFS_NVERT], c->red);
+    }
+    hipLaunchKernelGGL(finish_reduce<OpMax>, dim3(1), dim3(RED_THREADS), 0, st, c->red, nparts, c->red + RED_BLOCKS);
     VFS_LAUNCH_CHECK("divergence");
     VFS_HIP(hipMemcpyAsync(c->red_host, c->red + RED_BLOCKS, sizeof(double), hipMemcpyDeviceToHost, st));
     VFS_HIP(hipStreamSynchronize(st));

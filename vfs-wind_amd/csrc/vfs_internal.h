@@ -140,6 +140,29 @@ __device__ inline void xcd_block3(unsigned &bx, unsigned &by, unsigned &bz)
     bz = l / (gridDim.x * gridDim.y);
 }
 
+
+// k-marching tiles over the interior: one thread per (i, j) of a KM_I x KM_J
+// tile, marching kc planes; blocks are numbered i-tile fastest, then j-tile,
+// then k-chunk
+constexpr int KM_I = 64, KM_J = 4, KM_K = 16;
+__host__ __device__ inline unsigned km_blocks(const Grid &g, int kc)
+{
+    return (unsigned)(((g.mx - 2 + KM_I - 1) / KM_I) * ((g.my - 2 + KM_J - 1) / KM_J) * ((g.mz - 2 + kc - 1) / kc));
+}
+// this thread's interior (i, j) and planes [k0, k1); false when outside
+__device__ inline bool km_tile(const Grid &g, int kc, int &i, int &j, int &k0, int &k1)
+{
+    const int nbi = (g.mx - 2 + KM_I - 1) / KM_I, nbj = (g.my - 2 + KM_J - 1) / KM_J;
+    int b = blockIdx.x;
+    const int bi = b % nbi;
+    b /= nbi;
+    const int bj = b % nbj, bk = b / nbj;
+    i = 1 + bi * KM_I + (int)threadIdx.x % KM_I;
+    j = 1 + bj * KM_J + (int)threadIdx.x / KM_I;
+    k0 = 1 + bk * kc;
+    k1 = k0 + kc < g.mz - 1 ? k0 + kc : g.mz - 1;
+    return i < g.mx - 1 && j < g.my - 1 && k0 < k1;
+}
 }  // namespace vfs
 
 // Multigrid solver state (poisson.hip).
@@ -206,6 +229,7 @@ inline bool is_c2c_field(int f)
     return f == VFS_COOR || f == VFS_NVERT || is_metric_field(f) || f == VFS_UCONT || f == VFS_UCAT || f == VFS_UCAT_OLD;
 }
 int c2c(vfs_ctx *c);
+int zero_ghosts(vfs_ctx *c, double *f);  // f = 0 on the six boundary planes
 enum { UCAT_STALE = 0, UCAT_INTERIOR = 1, UCAT_WALLS = 2 };
 // a write that may change any cell of VFS_UCAT's Contra2Cart inputs or of
 // VFS_UCAT itself

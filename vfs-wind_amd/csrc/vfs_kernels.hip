@@ -684,6 +684,26 @@ int form_bcs(vfs_ctx *c, int outflow_scale)
 // a4: Compute_eddy_viscosity_LES (les.c:1143-1361)
 // ---------------------------------------------------------------------------
 
+// Smagorinsky nu_t from the velocity gradient dd[c][d] (d/dcsi, d/deta,
+// d/dzet of component c) and the cell metrics, les.c:1143-1361
+__device__ inline double smagorinsky(double cs2, const double dd[3][3], const double *csi, const double *eta,
+                                     const double *zet, double ajc)
+{
+    const double cs[3] = {csi[0], csi[1], csi[2]};
+    const double et[3] = {eta[0], eta[1], eta[2]};
+    const double ze[3] = {zet[0], zet[1], zet[2]};
+    double gr[3][3];
+    for (int c = 0; c < 3; c++)
+        for (int r = 0; r < 3; r++) gr[c][r] = (dd[c][0] * cs[r] + dd[c][1] * et[r] + dd[c][2] * ze[r]) * ajc;
+    double Sxx = 0.5 * (gr[0][0] + gr[0][0]), Sxy = 0.5 * (gr[0][1] + gr[1][0]), Sxz = 0.5 * (gr[0][2] + gr[2][0]);
+    double Syx = Sxy, Syy = 0.5 * (gr[1][1] + gr[1][1]), Syz = 0.5 * (gr[1][2] + gr[2][1]);
+    double Szx = Sxz, Szy = Syz, Szz = 0.5 * (gr[2][2] + gr[2][2]);
+    double Sabs = sqrt(2.0 * (Sxx * Sxx + Sxy * Sxy + Sxz * Sxz + Syx * Syx + Syy * Syy + Syz * Syz + Szx * Szx +
+                              Szy * Szy + Szz * Szz));
+    double filter = pow(1. / ajc, 1. / 3.);
+    return cs2 * pow(filter, 2.0) * Sabs;
+}
+
 __global__ void k_eddy(Grid g, double cs2, const double *__restrict__ ucat, const double *__restrict__ csi,
                        const double *__restrict__ eta, const double *__restrict__ zet, const double *__restrict__ aj,
                        const double *__restrict__ nvert, double *__restrict__ nut)
@@ -714,20 +734,82 @@ __global__ void k_eddy(Grid g, double cs2, const double *__restrict__ ucat, cons
     }
 #undef U
 #undef NV
-    const double ajc = aj[q];
-    const double cs[3] = {csi[3 * q], csi[3 * q + 1], csi[3 * q + 2]};
-    const double et[3] = {eta[3 * q], eta[3 * q + 1], eta[3 * q + 2]};
-    const double ze[3] = {zet[3 * q], zet[3 * q + 1], zet[3 * q + 2]};
-    double gr[3][3];
-    for (int c = 0; c < 3; c++)
-        for (int r = 0; r < 3; r++) gr[c][r] = (dd[c][0] * cs[r] + dd[c][1] * et[r] + dd[c][2] * ze[r]) * ajc;
-    double Sxx = 0.5 * (gr[0][0] + gr[0][0]), Sxy = 0.5 * (gr[0][1] + gr[1][0]), Sxz = 0.5 * (gr[0][2] + gr[2][0]);
-    double Syx = Sxy, Syy = 0.5 * (gr[1][1] + gr[1][1]), Syz = 0.5 * (gr[1][2] + gr[2][1]);
-    double Szx = Sxz, Szy = Syz, Szz = 0.5 * (gr[2][2] + gr[2][2]);
-    double Sabs = sqrt(2.0 * (Sxx * Sxx + Sxy * Sxy + Sxz * Sxz + Syx * Syx + Syy * Syy + Syz * Syz + Szx * Szx +
-                              Szy * Szy + Szz * Szz));
-    double filter = pow(1. / ajc, 1. / 3.);
-    nut[q] = cs2 * pow(filter, 2.0) * Sabs;
+    nut[q] = smagorinsky(cs2, dd, csi + 3 * q, eta + 3 * q, zet + 3 * q, aj[q]);
+}
+
+// The same per interior point, with one thread per (i, j) of a 64 x 4 tile
+// marching KM_K planes: the thread's own ucat and nvert of the planes k-1,
+// k, k+1 stay in registers, so every value of those arrays leaves HBM once
+// (the thread-per-point form re-read the k +- 1 planes).  The ghost cells
+// get nu_t = 0 from k_zero_ghosts.
+__global__ void __launch_bounds__(KM_I * KM_J) k_eddy_km(Grid g, double cs2, const double *__restrict__ ucat,
+                                                        const double *__restrict__ csi, const double *__restrict__ eta,
+                                                        const double *__restrict__ zet, const double *__restrict__ aj,
+                                                        const double *__restrict__ nvert, double *__restrict__ nut)
+{
+    int i, j, k0, k1;
+    if (!km_tile(g, KM_K, i, j, k0, k1)) return;
+    const double SOL = 0.1;
+    const long long P = g.plane;
+    long long q = gidx(g, k0, j, i);
+    double um[3], u0[3], up[3];
+    for (int c = 0; c < 3; c++) { um[c] = ucat[3 * (q - P) + c]; u0[c] = ucat[3 * q + c]; }
+    double nvm = nvert[q - P], nv0 = nvert[q];
+    for (int k = k0; k < k1; k++, q += P) {
+        for (int c = 0; c < 3; c++) up[c] = ucat[3 * (q + P) + c];
+        const double nvp = nvert[q + P];
+        if (nv0 > 1.1) {
+            nut[q] = 0;
+        } else {
+            const double nve = nvert[q + 1], nvw = nvert[q - 1], nvn = nvert[q + g.mx], nvs = nvert[q - g.mx];
+            double dd[3][3];
+            for (int c = 0; c < 3; c++) {
+                const double ue = ucat[3 * (q + 1) + c], uw = ucat[3 * (q - 1) + c];
+                const double un = ucat[3 * (q + g.mx) + c], us = ucat[3 * (q - g.mx) + c];
+                if (nve > SOL || (!g.pi && i == g.mx - 2)) dd[c][0] = (u0[c] - uw);
+                else if (nvw > SOL || (!g.pi && i == 1)) dd[c][0] = (ue - u0[c]);
+                else dd[c][0] = (ue - uw) * 0.5;
+                if (nvn > SOL || (!g.pj && j == g.my - 2)) dd[c][1] = (u0[c] - us);
+                else if (nvs > SOL || (!g.pj && j == 1)) dd[c][1] = (un - u0[c]);
+                else dd[c][1] = (un - us) * 0.5;
+                // k-omega.c:390-395 tests k==0 for the lower one-sided case
+                if (nvp > SOL || (!g.pk && k == g.mz - 2)) dd[c][2] = (u0[c] - um[c]);
+                else if (nvm > SOL || (!g.pk && k == 0)) dd[c][2] = (up[c] - u0[c]);
+                else dd[c][2] = (up[c] - um[c]) * 0.5;
+            }
+            nut[q] = smagorinsky(cs2, dd, csi + 3 * q, eta + 3 * q, zet + 3 * q, aj[q]);
+        }
+        for (int c = 0; c < 3; c++) { um[c] = u0[c]; u0[c] = up[c]; }
+        nvm = nv0;
+        nv0 = nvp;
+    }
+}
+
+// f = 0 on the six boundary planes of the DA
+__global__ void k_zero_ghosts(Grid g, double *__restrict__ f)
+{
+    long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    const long long ni = (long long)g.my * g.mz, nj = (long long)g.mx * g.mz, nk = (long long)g.mx * g.my;
+    int i, j, k;
+    if (t < 2 * ni) {
+        j = (int)((t >> 1) % g.my); k = (int)((t >> 1) / g.my); i = (t & 1) ? g.mx - 1 : 0;
+    } else if ((t -= 2 * ni) < 2 * nj) {
+        i = (int)((t >> 1) % g.mx); k = (int)((t >> 1) / g.mx); j = (t & 1) ? g.my - 1 : 0;
+    } else if ((t -= 2 * nj) < 2 * nk) {
+        i = (int)((t >> 1) % g.mx); j = (int)((t >> 1) / g.mx); k = (t & 1) ? g.mz - 1 : 0;
+    } else {
+        return;
+    }
+    f[((long long)k * g.my + j) * g.mx + i] = 0;
+}
+
+int zero_ghosts(vfs_ctx *c, double *f)
+{
+    const Grid &g = c->g;
+    const long long n = 2LL * ((long long)g.my * g.mz + (long long)g.mx * g.mz + (long long)g.mx * g.my);
+    hipLaunchKernelGGL(k_zero_ghosts, dim3(blocks_for(n, TPB)), dim3(TPB), 0, c->stream, g, f);
+    VFS_LAUNCH_CHECK("zero ghosts");
+    return VFS_OK;
 }
 
 int eddy_viscosity(vfs_ctx *c)
@@ -738,8 +820,17 @@ int eddy_viscosity(vfs_ctx *c)
         return VFS_OK;
     }
     prof_begin(c, K_EDDY);
-    hipLaunchKernelGGL(k_eddy, dim3(blocks_for(g.n, TPB)), dim3(TPB), 0, c->stream, g, c->p.cs2, c->f[VFS_UCAT],
-                       c->f[VFS_CSI], c->f[VFS_ETA], c->f[VFS_ZET], c->f[VFS_AJ], c->f[VFS_NVERT], c->f[VFS_NU_T]);
+    const char *e = std::getenv("VFS_EDDY_KM");
+    if (e && e[0] == '0') {
+        hipLaunchKernelGGL(k_eddy, dim3(blocks_for(g.n, TPB)), dim3(TPB), 0, c->stream, g, c->p.cs2, c->f[VFS_UCAT],
+                           c->f[VFS_CSI], c->f[VFS_ETA], c->f[VFS_ZET], c->f[VFS_AJ], c->f[VFS_NVERT], c->f[VFS_NU_T]);
+    } else {
+        int rc = zero_ghosts(c, c->f[VFS_NU_T]);
+        if (rc) return rc;
+        hipLaunchKernelGGL(k_eddy_km, dim3(km_blocks(g, KM_K)), dim3(KM_I * KM_J), 0, c->stream, g, c->p.cs2,
+                           c->f[VFS_UCAT], c->f[VFS_CSI], c->f[VFS_ETA], c->f[VFS_ZET], c->f[VFS_AJ], c->f[VFS_NVERT],
+                           c->f[VFS_NU_T]);
+    }
     prof_end(c);
     VFS_LAUNCH_CHECK("eddy viscosity");
     return periodic_fix(c, c->f[VFS_NU_T], 1, 0);
