@@ -586,7 +586,11 @@ __global__ void __launch_bounds__(FT_I * FT_J) __attribute__((amdgpu_waves_per_e
             for (int c = 0; c < 10; c++) cur[c] = nk[c];
             have = true;
         }
-        __syncthreads();
+        // No barrier here: the next store_plane overwrites the slot of plane
+        // k-1, which only the flux evaluations read (all done before the
+        // barrier above), and fxj / hx are next written after the next
+        // iteration's first barrier, which every wavefront reaches only once
+        // it has finished reading them here.
         slot0 = slot0 + 1 < 3 ? slot0 + 1 : 0;
     }
 }
