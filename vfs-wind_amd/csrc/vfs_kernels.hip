@@ -25,25 +25,28 @@ constexpr int TPB = 256;
 // dir: 0 = i planes, 1 = j planes, 2 = k planes.  mode 0 copies all nc
 // components; mode 1 (Projection, poisson.c:3001-3040) copies component d of
 // a vector only when direction d is a periodic ghost index of the cell.
-__global__ void k_periodic_planes(Grid g, double *__restrict__ f, int nc, int dir, int mode)
+// One launch over the ghost planes of every periodic direction (t runs over
+// the planes of the first periodic direction, then the next ...).  A ghost
+// cell's source wraps all periodic directions at once (periodic_src), so it
+// is never a periodic ghost itself and the planes do not depend on each
+// other; a cell on two planes is written twice with the same value.
+__global__ void k_periodic_planes(Grid g, double *__restrict__ f, int nc, int mode)
 {
     long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
     int i, j, k;
-    if (dir == 0) {
-        long long n = 2LL * g.my * g.mz;
-        if (t >= n) return;
+    const long long n0 = g.pi ? 2LL * g.my * g.mz : 0, n1 = g.pj ? 2LL * g.mx * g.mz : 0;
+    const long long n2 = g.pk ? 2LL * g.mx * g.my : 0;
+    if (t < n0) {
         int side = t & 1; long long r = t >> 1;
         j = r % g.my; k = r / g.my; i = side ? g.mx - 1 : 0;
-    } else if (dir == 1) {
-        long long n = 2LL * g.mx * g.mz;
-        if (t >= n) return;
+    } else if ((t -= n0) < n1) {
         i = t % g.mx; long long r = t / g.mx;
         int side = r & 1; k = r >> 1; j = side ? g.my - 1 : 0;
-    } else {
-        long long n = 2LL * g.mx * g.my;
-        if (t >= n) return;
+    } else if ((t -= n1) < n2) {
         i = t % g.mx; long long r = t / g.mx;
         j = r % g.my; int side = (int)(r / g.my); k = side ? g.mz - 1 : 0;
+    } else {
+        return;
     }
     int a, b, c, fi, fj, fk;
     if (!periodic_src(g, k, j, i, c, b, a, fi, fj, fk)) return;
@@ -59,12 +62,9 @@ __global__ void k_periodic_planes(Grid g, double *__restrict__ f, int nc, int di
 
 int periodic_fix_grid(hipStream_t st, const Grid &g, double *f, int nc, int mode)
 {
-    const long long np[3] = {2LL * g.my * g.mz, 2LL * g.mx * g.mz, 2LL * g.mx * g.my};
-    const int per[3] = {g.pi, g.pj, g.pk};
-    for (int d = 0; d < 3; d++) {
-        if (!per[d]) continue;
-        hipLaunchKernelGGL(k_periodic_planes, dim3(blocks_for(np[d], TPB)), dim3(TPB), 0, st, g, f, nc, d, mode);
-    }
+    const long long n = (g.pi ? 2LL * g.my * g.mz : 0) + (g.pj ? 2LL * g.mx * g.mz : 0) + (g.pk ? 2LL * g.mx * g.my : 0);
+    if (n == 0) return VFS_OK;
+    hipLaunchKernelGGL(k_periodic_planes, dim3(blocks_for(n, TPB)), dim3(TPB), 0, st, g, f, nc, mode);
     VFS_LAUNCH_CHECK("periodic fix");
     return VFS_OK;
 }
