@@ -738,6 +738,32 @@ __global__ void k_scale_dev(long long n, const double *__restrict__ a, int inv, 
     if (t < n) y[t] = s * x[t];
 }
 
+// out = <w, w> of the second Gram-Schmidt pass when it ran (gate as in
+// k_multiaxpy), else the first pass's *gww
+__global__ void __launch_bounds__(RED_THREADS) k_finish_gated(const double *__restrict__ part, int n,
+                                                              const double *__restrict__ gww, const double *__restrict__ gww0,
+                                                              double *__restrict__ out)
+{
+    if (!(gww[0] < 0.25 * gww0[0])) {
+        if (threadIdx.x == 0) out[0] = gww[0];
+        return;
+    }
+    double v = OpSum::init();
+    for (int q = threadIdx.x; q < n; q += blockDim.x) v = OpSum::op(v, part[q]);
+    v = block_reduce<OpSum>(v);
+    if (threadIdx.x == 0) out[0] = v;
+}
+
+// y = x / sqrt(ww) (0 when ww <= 0), ww read from device memory
+__global__ void k_scale_rnorm(long long n, const double *__restrict__ ww, const double *__restrict__ x,
+                              double *__restrict__ y)
+{
+    long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    const double hn = sqrt(ww[0] > 0 ? ww[0] : 0);
+    const double s = hn > 0 ? 1.0 / hn : 0.0;
+    if (t < n) y[t] = s * x[t];
+}
+
 constexpr int MAXV = 33;
 
 // The Gram-Schmidt sweeps over the Krylov basis.  Each is instantiated for a
@@ -797,8 +823,11 @@ __global__ void __launch_bounds__(RED_THREADS) k_multidot(long long n, int nv, c
 template <int NVM, int U, bool DOTS>
 __global__ void __launch_bounds__(RED_THREADS) k_multiaxpy(long long n, int nv, const double *__restrict__ h,
                                                            const double *__restrict__ V, long long ld,
-                                                           double *__restrict__ w, double *__restrict__ part)
+                                                           double *__restrict__ w, double *__restrict__ part,
+                                                           const double *__restrict__ gww, const double *__restrict__ gww0)
 {
+    // gate: run only when *gww < *gww0 / 4 (the re-orthogonalisation test)
+    if (gww && !(gww[0] < 0.25 * gww0[0])) return;
     double hv[NVM], acc[NVM];
 #pragma unroll
     for (int v = 0; v < NVM; v++) {
@@ -1334,45 +1363,48 @@ static int multidot(vfs_ctx *c, int nv, const double *V, const double *w, double
 
 // w -= sum_v h_v V_v (h in dev_scalars) with the partials of k_multiaxpy
 template <bool DOTS>
-static void launch_multiaxpy(vfs_ctx *c, int nv, const double *V, double *w)
+static void launch_multiaxpy(vfs_ctx *c, int nv, const double *V, double *w, const double *h = nullptr,
+                             const double *gww = nullptr, const double *gww0 = nullptr)
 {
     hipStream_t st = c->stream;
     const long long n = c->g.n;
-    const double *h = c->ps->dev_scalars;
+    if (!h) h = c->ps->dev_scalars;
     const dim3 gb(RED_BLOCKS), tb(RED_THREADS);
-    if (nv <= 4) hipLaunchKernelGGL((k_multiaxpy<4, 4, DOTS>), gb, tb, 0, st, n, nv, h, V, n, w, c->red);
-    else if (nv <= 8) hipLaunchKernelGGL((k_multiaxpy<8, 4, DOTS>), gb, tb, 0, st, n, nv, h, V, n, w, c->red);
-    else if (nv <= 16) hipLaunchKernelGGL((k_multiaxpy<16, 2, DOTS>), gb, tb, 0, st, n, nv, h, V, n, w, c->red);
-    else hipLaunchKernelGGL((k_multiaxpy<MAXV - 1, 1, DOTS>), gb, tb, 0, st, n, nv, h, V, n, w, c->red);
+    if (nv <= 4) hipLaunchKernelGGL((k_multiaxpy<4, 4, DOTS>), gb, tb, 0, st, n, nv, h, V, n, w, c->red, gww, gww0);
+    else if (nv <= 8) hipLaunchKernelGGL((k_multiaxpy<8, 4, DOTS>), gb, tb, 0, st, n, nv, h, V, n, w, c->red, gww, gww0);
+    else if (nv <= 16) hipLaunchKernelGGL((k_multiaxpy<16, 2, DOTS>), gb, tb, 0, st, n, nv, h, V, n, w, c->red, gww, gww0);
+    else hipLaunchKernelGGL((k_multiaxpy<MAXV - 1, 1, DOTS>), gb, tb, 0, st, n, nv, h, V, n, w, c->red, gww, gww0);
 }
 
-// w -= sum_v h_v V_v with h already in dev_scalars; returns <V_v, w_new> for
-// v < nv in d[0..nv-1] and <w_new, w_new> in *ww
-static int axpy_dots(vfs_ctx *c, int nv, const double *V, double *w, double *d, double *ww)
+// One classical Gram-Schmidt step of w against V_0..V_{nv-1} with the
+// re-orthogonalisation pass when <w, w> drops below a quarter, and
+// V_nv = w / ||w||, all on the device: the coefficients of each pass are the
+// previous pass's device results, so the host synchronises once.  out (host)
+// receives h (nv), ww0, the second-pass dots (nv), ww1 and the final <w, w>
+// at [0..nv], [40..40+nv], [80].
+constexpr int GS_D2 = 40, GS_D3 = 80;
+static_assert(GS_D2 >= MAXV && GS_D3 >= GS_D2 + MAXV, "Gram-Schmidt result slots");
+static int gs_step(vfs_ctx *c, int nv, const double *V, double *w, double *vnext, double *out)
 {
     hipStream_t st = c->stream;
     const long long n = c->g.n;
-    launch_multiaxpy<true>(c, nv, V, w);
-    hipLaunchKernelGGL(k_finish_many, dim3(nv + 1), dim3(RED_THREADS), 0, st, nv, c->red, c->red + MAXV * RED_BLOCKS);
-    VFS_LAUNCH_CHECK("axpy dots");
-    VFS_HIP(hipMemcpyAsync(c->red_host, c->red + MAXV * RED_BLOCKS, (nv + 1) * sizeof(double), hipMemcpyDeviceToHost, st));
+    double *R = c->red + MAXV * RED_BLOCKS;
+    const dim3 gb(RED_BLOCKS), tb(RED_THREADS);
+    if (nv <= 4) hipLaunchKernelGGL((k_multidot<4, 4>), gb, tb, 0, st, n, nv, V, n, w, c->red);
+    else if (nv <= 8) hipLaunchKernelGGL((k_multidot<8, 4>), gb, tb, 0, st, n, nv, V, n, w, c->red);
+    else if (nv <= 16) hipLaunchKernelGGL((k_multidot<16, 2>), gb, tb, 0, st, n, nv, V, n, w, c->red);
+    else hipLaunchKernelGGL((k_multidot<MAXV - 1, 1>), gb, tb, 0, st, n, nv, V, n, w, c->red);
+    hipLaunchKernelGGL(k_finish_many, dim3(nv + 1), dim3(RED_THREADS), 0, st, nv, c->red, R);
+    launch_multiaxpy<true>(c, nv, V, w, R);
+    hipLaunchKernelGGL(k_finish_many, dim3(nv + 1), dim3(RED_THREADS), 0, st, nv, c->red, R + GS_D2);
+    launch_multiaxpy<false>(c, nv, V, w, R + GS_D2, R + GS_D2 + nv, R + nv);
+    hipLaunchKernelGGL(k_finish_gated, dim3(1), dim3(RED_THREADS), 0, st, c->red, RED_BLOCKS, R + GS_D2 + nv, R + nv,
+                       R + GS_D3);
+    hipLaunchKernelGGL(k_scale_rnorm, dim3(blocks_for(n, PTPB)), dim3(PTPB), 0, st, n, R + GS_D3, w, vnext);
+    VFS_LAUNCH_CHECK("gram-schmidt");
+    VFS_HIP(hipMemcpyAsync(c->red_host, R, (GS_D3 + 1) * sizeof(double), hipMemcpyDeviceToHost, st));
     VFS_HIP(hipStreamSynchronize(st));
-    std::memcpy(d, c->red_host, nv * sizeof(double));
-    *ww = c->red_host[nv];
-    return VFS_OK;
-}
-
-// w -= sum_v h_v V_v with h already in dev_scalars; returns the new <w, w>
-static int axpy_norm(vfs_ctx *c, int nv, const double *V, double *w, double *ww)
-{
-    hipStream_t st = c->stream;
-    const long long n = c->g.n;
-    launch_multiaxpy<false>(c, nv, V, w);
-    hipLaunchKernelGGL(finish_reduce<OpSum>, dim3(1), dim3(RED_THREADS), 0, st, c->red, RED_BLOCKS, c->red + MAXV * RED_BLOCKS);
-    VFS_LAUNCH_CHECK("axpy norm");
-    VFS_HIP(hipMemcpyAsync(c->red_host, c->red + MAXV * RED_BLOCKS, sizeof(double), hipMemcpyDeviceToHost, st));
-    VFS_HIP(hipStreamSynchronize(st));
-    *ww = c->red_host[0];
+    std::memcpy(out, c->red_host, (GS_D3 + 1) * sizeof(double));
     return VFS_OK;
 }
 
@@ -1475,31 +1507,22 @@ static int fgmres(vfs_ctx *c, const vfs_solve_opts *o, vfs_unet *net, vfs_solve_
             prof_begin(c, K_APPLY64);
             hipLaunchKernelGGL(k_apply<double>, dim3(nb), dim3(PTPB), 0, st, L.g, L.c64, L.row, zj, (const double *)nullptr, w);
             prof_end(c);
-            // classical Gram-Schmidt, re-orthogonalised when the norm drops
-            rc = multidot(c, jj + 1, V, w, d);
+            // classical Gram-Schmidt, re-orthogonalised when the norm drops;
+            // the first pass also forms the dot products of the second
+            double gs[GS_D3 + 1];
+            rc = gs_step(c, jj + 1, V, w, V + (size_t)(jj + 1) * n, gs);
             if (rc) return rc;
             double h[MAXV];
-            for (int v = 0; v <= jj; v++) h[v] = d[v];
-            double ww0 = d[jj + 1];
-            std::memcpy(c->red_host, h, (jj + 1) * sizeof(double));
-            VFS_HIP(hipMemcpyAsync(ps->dev_scalars, c->red_host, (jj + 1) * sizeof(double), hipMemcpyHostToDevice, st));
-            // first pass, with the dot products of a second one in the same sweep
-            double ww;
-            rc = axpy_dots(c, jj + 1, V, w, d, &ww);
-            if (rc) return rc;
-            if (ww < 0.25 * ww0) {
-                for (int v = 0; v <= jj; v++) h[v] += d[v];
-                std::memcpy(c->red_host, d, (jj + 1) * sizeof(double));
-                VFS_HIP(hipMemcpyAsync(ps->dev_scalars, c->red_host, (jj + 1) * sizeof(double), hipMemcpyHostToDevice, st));
-                rc = axpy_norm(c, jj + 1, V, w, &ww);
-                if (rc) return rc;
+            for (int v = 0; v <= jj; v++) h[v] = gs[v];
+            const double ww0 = gs[jj + 1], ww1 = gs[GS_D2 + jj + 1];
+            double ww = ww1;
+            if (ww1 < 0.25 * ww0) {
+                for (int v = 0; v <= jj; v++) h[v] += gs[GS_D2 + v];
+                ww = gs[GS_D3];
             }
             double hn = std::sqrt(ww > 0 ? ww : 0);
             for (int v = 0; v <= jj; v++) H[(size_t)v * m + jj] = h[v];
             H[(size_t)(jj + 1) * m + jj] = hn;
-            c->red_host[0] = hn > 0 ? 1.0 / hn : 0.0;
-            VFS_HIP(hipMemcpyAsync(ps->dev_scalars, c->red_host, sizeof(double), hipMemcpyHostToDevice, st));
-            hipLaunchKernelGGL(k_scale_dev, dim3(nb), dim3(PTPB), 0, st, n, ps->dev_scalars, 0, w, V + (size_t)(jj + 1) * n);
             for (int i = 0; i < jj; i++) {
                 double tmp = cs[i] * H[(size_t)i * m + jj] + sn[i] * H[(size_t)(i + 1) * m + jj];
                 H[(size_t)(i + 1) * m + jj] = -sn[i] * H[(size_t)i * m + jj] + cs[i] * H[(size_t)(i + 1) * m + jj];
