@@ -178,7 +178,7 @@ def main():
     ap.add_argument("--no-poisson-comparison", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for {args.gpus})", file=sys.stderr)
+        print(f"bench.py measures one GPU: the library has no multi-GPU path (asked for {args.gpus})", file=sys.stderr)
         sys.exit(2)
 
     vfs = importlib.import_module("vfs-wind_amd")

@@ -404,7 +404,7 @@ void orc_remove_mean(const orc_params *p, const orc_metrics *m, double *phi)
 }
 
 /* ------------------------------------------------------------------------ */
-/* Geometric multigrid (design of this build, mirrored by the HIP solver):   */
+/* Geometric multigrid, the design of the HIP solver restated:               */
 /*  - coarse grids take every other node (init.c:402-455), their operators   */
 /*    are rediscretised with FormMetrics on the coarse nodes;               */
 /*  - restriction sums the 8 child residuals (the rows are flux sums);      */
