@@ -65,11 +65,11 @@ ALGO = {
 # HBM bytes per launch of the finest-level launch, from rocprofv3 FETCH_SIZE (x2, calibrated on
 # copy kernels of known size) + WRITE_SIZE on this workload at 256x192x256
 MEASURED_TRAFFIC = {
-    "formfunction2": 6.287e9,
-    "face_fluxes": 3.578e9,
-    "cell_rhs": 2.655e9,
-    "line_rhs_fp32": 0.410e9,
-    "poisson_apply_fp64": 2.889e9,
+    "formfunction2": 5.755e9,
+    "face_fluxes": 3.290e9,
+    "cell_rhs": 2.408e9,
+    "line_rhs_fp32": 0.382e9,
+    "poisson_apply_fp64": 2.890e9,
     "contra2cart": 1.993e9,
 }
 
