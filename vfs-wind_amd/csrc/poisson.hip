@@ -293,10 +293,11 @@ __global__ void k_assemble(Grid g, FaceSet F, const double *__restrict__ nvert, 
     }
 }
 
-// y = A x (rows only), or r = b - A x when b != nullptr
-template <typename T>
+// y = A x (rows only), or r = b - A x when b != nullptr; x may be stored in a
+// narrower type X (an FP32 preconditioned vector, exact in FP64)
+template <typename T, typename X = T>
 __global__ void __launch_bounds__(PTPB) k_apply(Grid g, const T *__restrict__ coef, const unsigned char *__restrict__ row,
-                                                const T *__restrict__ x, const T *__restrict__ b, T *__restrict__ y)
+                                                const X *__restrict__ x, const T *__restrict__ b, T *__restrict__ y)
 {
     long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
     if (t >= g.n) return;
@@ -304,7 +305,7 @@ __global__ void __launch_bounds__(PTPB) k_apply(Grid g, const T *__restrict__ co
     int i = t % g.mx; long long r = t / g.mx; int j = r % g.my, k = r / g.my;
     T s = 0;
 #pragma unroll
-    for (int q = 0; q < NST; q++) s += coef[q * g.n + t] * x[nbr_idx(g, k, j, i, q)];
+    for (int q = 0; q < NST; q++) s += coef[q * g.n + t] * (T)x[nbr_idx(g, k, j, i, q)];
     y[t] = b ? b[t] - s : s;
 }
 
@@ -903,14 +904,15 @@ __global__ void k_finish_many(int nv, const double *__restrict__ part, double *_
     if (threadIdx.x == 0) out[v] = s;
 }
 
-// x = x + sum_v y[v] Z_v
-__global__ void k_multiupdate(long long n, int nv, const double *__restrict__ y, const double *__restrict__ Z, long long ld,
+// x = x + sum_v y[v] Z_v (Z in FP64, or FP32 from the FP32 V-cycle)
+template <typename Z_T>
+__global__ void k_multiupdate(long long n, int nv, const double *__restrict__ y, const Z_T *__restrict__ Z, long long ld,
                               double *__restrict__ x)
 {
     long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
     if (t >= n) return;
     double s = x[t];
-    for (int v = 0; v < nv; v++) s += y[v] * Z[v * ld + t];
+    for (int v = 0; v < nv; v++) s += y[v] * (double)Z[v * ld + t];
     x[t] = s;
 }
 
@@ -1321,7 +1323,10 @@ static void vcycle(vfs_ctx *c, vfs_poisson *ps, int l, int pre, int post, int co
 }
 
 // z = M^{-1} v (V-cycle from zero); FP32 or FP64 hierarchy
-static int precondition(vfs_ctx *c, const vfs_solve_opts *o, const double *v, double *z)
+// With z32 the FP32 V-cycle's finest iterate is z32 itself (no FP64 copy);
+// the FP64 V-cycle iterates in z directly.  The finest level's kernels take
+// the iterate pointer at launch (the captured coarse levels never see it).
+static int precondition(vfs_ctx *c, const vfs_solve_opts *o, const double *v, double *z, float *z32 = nullptr)
 {
     vfs_poisson *ps = c->ps;
     hipStream_t st = c->stream;
@@ -1330,14 +1335,19 @@ static int precondition(vfs_ctx *c, const vfs_solve_opts *o, const double *v, do
     const unsigned nb = blocks_for(n, PTPB);
     if (o->mg_fp32) {
         hipLaunchKernelGGL((k_convert<double, float>), dim3(nb), dim3(PTPB), 0, st, n, v, L.b32);
+        float *x0 = L.x32;
+        if (z32) L.x32 = z32;
         hipLaunchKernelGGL(k_zero<float>, dim3(nb), dim3(PTPB), 0, st, n, L.x32);
         vcycle<float>(c, ps, 0, o->pre, o->post, o->coarse_sweeps);
-        hipLaunchKernelGGL((k_convert<float, double>), dim3(nb), dim3(PTPB), 0, st, n, L.x32, z);
+        if (!z32) hipLaunchKernelGGL((k_convert<float, double>), dim3(nb), dim3(PTPB), 0, st, n, L.x32, z);
+        L.x32 = x0;
     } else {
         hipMemcpyAsync(L.b64, v, n * 8, hipMemcpyDeviceToDevice, st);
+        double *x0 = L.x64;
+        L.x64 = z;
         hipLaunchKernelGGL(k_zero<double>, dim3(nb), dim3(PTPB), 0, st, n, L.x64);
         vcycle<double>(c, ps, 0, o->pre, o->post, o->coarse_sweeps);
-        hipMemcpyAsync(z, L.x64, n * 8, hipMemcpyDeviceToDevice, st);
+        L.x64 = x0;
     }
     VFS_LAUNCH_CHECK("v-cycle");
     return VFS_OK;
@@ -1451,6 +1461,9 @@ static int fgmres(vfs_ctx *c, const vfs_solve_opts *o, vfs_unet *net, vfs_solve_
     const unsigned nb = blocks_for(n, PTPB);
     double *x = c->f[VFS_PHI], *b = c->f[VFS_B];
     double *V = ps->V, *Z = ps->Z, *w = ps->w;
+    // the FP32 V-cycle's outputs are kept as FP32 vectors in Z's memory (exact)
+    const bool z32 = o->mg_fp32 && !net;
+    float *Z32 = reinterpret_cast<float *>(Z);
     if (!o->use_guess) hipLaunchKernelGGL(k_zero<double>, dim3(nb), dim3(PTPB), 0, st, n, x);
     double d[MAXV + 1];
     if (net) {
@@ -1467,7 +1480,7 @@ static int fgmres(vfs_ctx *c, const vfs_solve_opts *o, vfs_unet *net, vfs_solve_
         if (rc) return rc;
         c->red_host[0] = d[0] > 0 ? rAy / d[0] : 0.0;
         VFS_HIP(hipMemcpyAsync(ps->dev_scalars, c->red_host, sizeof(double), hipMemcpyHostToDevice, st));
-        hipLaunchKernelGGL(k_multiupdate, dim3(nb), dim3(PTPB), 0, st, n, 1, ps->dev_scalars, Z, n, x);
+        hipLaunchKernelGGL(k_multiupdate<double>, dim3(nb), dim3(PTPB), 0, st, n, 1, ps->dev_scalars, (const double *)Z, n, x);
         VFS_LAUNCH_CHECK("learned warm start");
     }
     double rel_start = -1;
@@ -1502,10 +1515,15 @@ static int fgmres(vfs_ctx *c, const vfs_solve_opts *o, vfs_unet *net, vfs_solve_
         int jj;
         for (jj = 0; jj < m && it < maxit; jj++, it++) {
             double *vj = V + (size_t)jj * n, *zj = Z + (size_t)jj * n;
-            rc = net && it == 0 ? vfs_unet_forward(net, vj, zj) : precondition(c, o, vj, zj);
+            float *zj32 = Z32 + (size_t)jj * n;
+            rc = net && it == 0 ? vfs_unet_forward(net, vj, zj) : precondition(c, o, vj, zj, z32 ? zj32 : nullptr);
             if (rc) return rc;
             prof_begin(c, K_APPLY64);
-            hipLaunchKernelGGL(k_apply<double>, dim3(nb), dim3(PTPB), 0, st, L.g, L.c64, L.row, zj, (const double *)nullptr, w);
+            if (z32)
+                hipLaunchKernelGGL((k_apply<double, float>), dim3(nb), dim3(PTPB), 0, st, L.g, L.c64, L.row, (const float *)zj32,
+                                   (const double *)nullptr, w);
+            else
+                hipLaunchKernelGGL(k_apply<double>, dim3(nb), dim3(PTPB), 0, st, L.g, L.c64, L.row, zj, (const double *)nullptr, w);
             prof_end(c);
             // classical Gram-Schmidt, re-orthogonalised when the norm drops;
             // the first pass also forms the dot products of the second
@@ -1550,7 +1568,10 @@ static int fgmres(vfs_ctx *c, const vfs_solve_opts *o, vfs_unet *net, vfs_solve_
         }
         std::memcpy(c->red_host, y.data(), jj * sizeof(double));
         VFS_HIP(hipMemcpyAsync(ps->dev_scalars, c->red_host, jj * sizeof(double), hipMemcpyHostToDevice, st));
-        hipLaunchKernelGGL(k_multiupdate, dim3(nb), dim3(PTPB), 0, st, n, jj, ps->dev_scalars, Z, n, x);
+        if (z32)
+            hipLaunchKernelGGL(k_multiupdate<float>, dim3(nb), dim3(PTPB), 0, st, n, jj, ps->dev_scalars, (const float *)Z32, n, x);
+        else
+            hipLaunchKernelGGL(k_multiupdate<double>, dim3(nb), dim3(PTPB), 0, st, n, jj, ps->dev_scalars, (const double *)Z, n, x);
         VFS_LAUNCH_CHECK("fgmres update");
     }
     if (!converged) {
