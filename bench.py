@@ -57,6 +57,9 @@ ALGO = {
     "line_rhs_fp32": (32 + 4 + 4 + 4 + 4, "line point"),
     # FP32: Thomas factors lo, 1/pivot, upper 12 + line rhs 4; x out 4
     "line_solve_fp32": (12 + 4 + 4, "line point"),
+    # compact FP32 smoother (weak cross couplings): four axis ratios as FP16 8 + diagonal 4 + b 4 + x 4;
+    # line rhs out 4
+    "line_rhs_compact_fp32": (8 + 4 + 4 + 4 + 4, "line point"),
     # FP64: 19 coefficients 152 + x 8 + row mask 1; y out 8
     "poisson_apply_fp64": (152 + 8 + 1 + 8, "point"),
 }

@@ -339,6 +339,19 @@ __host__ __device__ inline int zebra_i0(int k, int c) { return 1 + ((1 + k + c) 
 // O(1) for the Poisson stencil and FP16 keeps 11 bits of them; the smoother
 // is a preconditioner of the FP64 FGMRES, whose residual stays exact.
 typedef _Float16 h2_t __attribute__((ext_vector_type(2)));
+// Compact FP32 smoother, for levels whose cross couplings (the twelve
+// off-line neighbours off the i and k axes: edges through j and the i-k
+// diagonals) are weak: only the ratios of the four axis neighbours
+// (i+1, i-1, k+1, k-1 in the line's own row) are kept, as one 8-byte word.
+typedef _Float16 h4_t __attribute__((ext_vector_type(4)));
+// Cross-coupling strength (max over rows of sum |a_cross| / |a_P|) up to
+// which a level takes the compact smoother.  On a 128x96x128 channel whose
+// nodes were displaced by short-wavelength sines of growing amplitude the
+// compact form took as many FGMRES iterations as the full one at strength
+// 0.0085 (7 % less solve time), one more per solve from 0.033 (slower), 8-9
+// instead of 6 at 0.083.  A channel whose only distortion spans the whole
+// box (2 % of the spacing) measures 0.0012.
+constexpr double COMPACT_MAX_CROSS = 0.01;
 
 // a / d as a float within the FP16 range (the clamp only guards degenerate rows)
 __device__ inline float half_ratio(double a, double d)
@@ -387,13 +400,50 @@ __global__ void __launch_bounds__(PTPB) k_zebra_rhs_h2(Grid g, ZGrid z, int c, c
     }
 }
 
+// Compact line right-hand side of one point: b - d (r_E x_E + r_W x_W +
+// r_T x_T + r_B x_B), the neighbours in the point's own row j
+__device__ inline float c4_rhs(const Grid &g, const int I[3], const int K[3], int j, h4_t w, float d, const float *__restrict__ x,
+                               float bq)
+{
+    const long long row0 = (long long)j * g.mx;
+    const float xe = x[(long long)K[1] * g.plane + row0 + I[2]];
+    const float xw = x[(long long)K[1] * g.plane + row0 + I[0]];
+    const float xt = x[(long long)K[2] * g.plane + row0 + I[1]];
+    const float xb = x[(long long)K[0] * g.plane + row0 + I[1]];
+    float acc = 0;
+    acc += (float)w.x * xe;
+    acc += (float)w.y * xw;
+    acc += (float)w.z * xt;
+    acc += (float)w.w * xb;
+    return bq - d * acc;
+}
+
+// compact line right-hand sides of colour c, one thread per line point
+__global__ void __launch_bounds__(PTPB) k_zebra_rhs_c4(Grid g, ZGrid z, int c, const h4_t *__restrict__ sh,
+                                                       const float *__restrict__ sd, const float *__restrict__ x,
+                                                       const float *__restrict__ b, float *__restrict__ rz)
+{
+    long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (t >= z.cplane) return;
+    const int h = t % z.nh;
+    const long long r = t / z.nh;
+    const int jj = r % z.njI, kk = r / z.njI;
+    const int k = kk + 1, j = jj + 1, i = zebra_i0(k, c) + 2 * h;
+    if (i > g.mx - 2) return;
+    const int I[3] = {canon(i - 1, g.mx, g.pi), i, canon(i + 1, g.mx, g.pi)};
+    const int K[3] = {canon(k - 1, g.mz, g.pk), k, canon(k + 1, g.mz, g.pk)};
+    const long long cq = c * z.cplane + t;
+    rz[t] = c4_rhs(g, I, K, j, sh[cq], sd[cq], x, b[((long long)k * g.my + j) * g.mx + i]);
+}
+
 // colour-split off-line coefficients and Thomas factors (lo, 1/pivot,
 // upper/pivot).  T = double: sc holds the 16 off-line planes, then the three
 // factor planes.  T = float: sc holds the three factor planes, sh the FP16
-// coefficient pairs, sd the diagonal.
+// coefficient pairs (compact: one h4_t word of the four axis ratios per
+// point), sd the diagonal.
 template <typename T>
 __global__ void k_zebra_setup(Grid g, ZGrid z, const double *__restrict__ c64, const unsigned char *__restrict__ row,
-                              T *__restrict__ sc, h2_t *__restrict__ sh, float *__restrict__ sd)
+                              T *__restrict__ sc, h2_t *__restrict__ sh, float *__restrict__ sd, int compact)
 {
     long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
     if (t >= 2LL * z.nkI * z.nh) return;
@@ -413,14 +463,25 @@ __global__ void k_zebra_setup(Grid g, ZGrid z, const double *__restrict__ c64, c
             for (int u = 0; u < NOFF; u++) sc[u * sstride + cq] = rw ? (T)c64[offline_slot(u) * g.n + q] : T(0);
         } else {
             const double dg = rw ? c64[CP * g.n + q] : 1.0;
-#pragma unroll
-            for (int u = 0; u < NOFF; u += 2) {
-                h2_t w = {(_Float16)0.f, (_Float16)0.f};
+            if (compact) {
+                h4_t w = {(_Float16)0.f, (_Float16)0.f, (_Float16)0.f, (_Float16)0.f};
                 if (rw) {
-                    w.x = (_Float16)half_ratio(c64[offline_slot(u) * g.n + q], dg);
-                    w.y = (_Float16)half_ratio(c64[offline_slot(u + 1) * g.n + q], dg);
+                    w.x = (_Float16)half_ratio(c64[offline_slot(0) * g.n + q], dg);
+                    w.y = (_Float16)half_ratio(c64[offline_slot(1) * g.n + q], dg);
+                    w.z = (_Float16)half_ratio(c64[offline_slot(2) * g.n + q], dg);
+                    w.w = (_Float16)half_ratio(c64[offline_slot(3) * g.n + q], dg);
                 }
-                sh[(u / 2) * sstride + cq] = w;
+                reinterpret_cast<h4_t *>(sh)[cq] = w;
+            } else {
+#pragma unroll
+                for (int u = 0; u < NOFF; u += 2) {
+                    h2_t w = {(_Float16)0.f, (_Float16)0.f};
+                    if (rw) {
+                        w.x = (_Float16)half_ratio(c64[offline_slot(u) * g.n + q], dg);
+                        w.y = (_Float16)half_ratio(c64[offline_slot(u + 1) * g.n + q], dg);
+                    }
+                    sh[(u / 2) * sstride + cq] = w;
+                }
             }
             sd[cq] = rw ? (float)dg : 0.f;
         }
@@ -956,6 +1017,10 @@ struct PLevel {
     // (FP32: sc32 = factors, sh16 = FP16 coefficient ratios, sd32 = diagonal)
     float *sc32 = nullptr, *rz32 = nullptr, *sd32 = nullptr;
     vfs::h2_t *sh16 = nullptr;
+    // FP32 smoother form: compact (four axis ratios in sh16's memory, as
+    // h4_t) when the level's cross-coupling strength xs is small
+    int compact = 0;
+    double xs = 0;
     double *sc64 = nullptr, *rz64 = nullptr;
     // work vectors
     float *x32 = nullptr, *b32 = nullptr, *r32 = nullptr;
@@ -1040,6 +1105,45 @@ static int level_alloc_work(vfs_poisson *ps, PLevel &L)
     return rc;
 }
 
+// cross-coupling strength of a level: max over its rows of sum |a_u| / |a_P|
+// over the twelve off-line neighbours off the i and k axes
+__global__ void __launch_bounds__(RED_THREADS) k_cross_strength(Grid g, const double *__restrict__ c64,
+                                                                const unsigned char *__restrict__ row, double *__restrict__ part)
+{
+    double v = 0;
+    for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < g.n; t += (long long)gridDim.x * blockDim.x) {
+        if (!row[t]) continue;
+        double s = 0;
+        for (int u = 4; u < NOFF; u++) s += fabs(c64[offline_slot(u) * g.n + t]);
+        const double d = fabs(c64[CP * g.n + t]);
+        if (d > 0) v = fmax(v, s / d);
+    }
+    v = block_reduce<OpMax>(v);
+    if (threadIdx.x == 0) part[blockIdx.x] = v;
+}
+
+// The FP32 smoother's form for a level: compact when its cross couplings are
+// weak (VFS_SMOOTH_COMPACT=0 or 1 forces the full or the compact form)
+static int choose_smoother(vfs_ctx *c, PLevel &L)
+{
+    hipStream_t st = c->stream;
+    hipLaunchKernelGGL(k_cross_strength, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, st, L.g, L.c64, L.row, c->red);
+    hipLaunchKernelGGL(finish_reduce<OpMax>, dim3(1), dim3(RED_THREADS), 0, st, c->red, RED_BLOCKS, c->red + RED_BLOCKS);
+    VFS_LAUNCH_CHECK("cross-coupling strength");
+    VFS_HIP(hipMemcpyAsync(c->red_host, c->red + RED_BLOCKS, sizeof(double), hipMemcpyDeviceToHost, st));
+    VFS_HIP(hipStreamSynchronize(st));
+    L.xs = c->red_host[0];
+    const char *e = std::getenv("VFS_SMOOTH_COMPACT");
+    if (e && e[0] == '0') L.compact = 0;
+    else if (e && e[0] == '1') L.compact = 1;
+    else L.compact = L.xs <= COMPACT_MAX_CROSS;
+    const char *tr = std::getenv("VFS_SOLVER_TRACE");
+    if (tr && tr[0] == '1')
+        std::fprintf(stderr, "level %dx%dx%d: cross-coupling strength %.9g, %s FP32 smoother\n", L.g.mx, L.g.my, L.g.mz, L.xs,
+                     L.compact ? "compact" : "full");
+    return VFS_OK;
+}
+
 static int level_assemble(vfs_ctx *c, PLevel &L, double *const met[17], const double *nvert)
 {
     hipStream_t st = c->stream;
@@ -1047,12 +1151,15 @@ static int level_assemble(vfs_ctx *c, PLevel &L, double *const met[17], const do
     const double th = c->p.poisson_threshold;
     hipLaunchKernelGGL(k_row_mask, dim3(blocks_for(L.g.n, PTPB)), dim3(PTPB), 0, st, L.g, nvert, th, L.row);
     hipLaunchKernelGGL(k_assemble, dim3(blocks_for(L.g.n, PTPB)), dim3(PTPB), 0, st, L.g, F, nvert, th, L.row, L.c64, L.c32);
+    VFS_LAUNCH_CHECK("assemble");
+    int rc = choose_smoother(c, L);
+    if (rc) return rc;
     const long long lines = 2LL * L.z.nkI * L.z.nh;
     hipLaunchKernelGGL(k_zebra_setup<float>, dim3(blocks_for(lines, PTPB)), dim3(PTPB), 0, st, L.g, L.z, L.c64, L.row, L.sc32,
-                       L.sh16, L.sd32);
+                       L.sh16, L.sd32, L.compact);
     hipLaunchKernelGGL(k_zebra_setup<double>, dim3(blocks_for(lines, PTPB)), dim3(PTPB), 0, st, L.g, L.z, L.c64, L.row, L.sc64,
-                       nullptr, nullptr);
-    VFS_LAUNCH_CHECK("assemble");
+                       nullptr, nullptr, 0);
+    VFS_LAUNCH_CHECK("smoother setup");
     return VFS_OK;
 }
 
@@ -1237,22 +1344,28 @@ static void smooth(vfs_ctx *c, PLevel &L, int sweeps)
     const int nseg = (L.z.njI + SLM - 1) / SLM;
     const bool scan = nseg <= LINE_MAXSEG;  // lines up to 32 * SLM points
     const int sl = (L.z.njI + nseg - 1) / nseg;
+    const bool c4 = sizeof(T) == 4 && L.compact;
+    const h4_t *sh4 = reinterpret_cast<const h4_t *>(L.sh16);
+    const unsigned scan_blocks = (unsigned)((lines + LINE_LINES - 1) / LINE_LINES);
     for (int s = 0; s < sweeps; s++)
         for (int col = 0; col < 2; col++) {
-            if (fine) prof_begin(c, sizeof(T) == 4 ? K_LINE32 : K_LINE64);
-            if constexpr (sizeof(T) == 4)
-                hipLaunchKernelGGL(k_zebra_rhs_h2,
-                                   dim3(blocks_for((long long)L.z.nkI * ((L.z.njI + 1) / 2) * L.z.nh, PTPB)), dim3(PTPB),
-                                   0, st, L.g, L.z, col, L.sh16, L.sd32, L.x32, L.b32, L.rz32);
-            else
+            if (fine) prof_begin(c, sizeof(T) == 4 ? (c4 ? K_LINEC32 : K_LINE32) : K_LINE64);
+            if constexpr (sizeof(T) == 4) {
+                if (c4)
+                    hipLaunchKernelGGL(k_zebra_rhs_c4, dim3(blocks_for(L.z.cplane, PTPB)), dim3(PTPB), 0, st, L.g, L.z, col, sh4,
+                                       L.sd32, L.x32, L.b32, L.rz32);
+                else
+                    hipLaunchKernelGGL(k_zebra_rhs_h2,
+                                       dim3(blocks_for((long long)L.z.nkI * ((L.z.njI + 1) / 2) * L.z.nh, PTPB)), dim3(PTPB),
+                                       0, st, L.g, L.z, col, L.sh16, L.sd32, L.x32, L.b32, L.rz32);
+            } else
                 hipLaunchKernelGGL(k_zebra_rhs<double>, dim3(blocks_for(L.z.cplane, PTPB)), dim3(PTPB), 0, st, L.g, L.z,
                                    col, L.sc64, L.x64, L.b64, L.rz64);
             if (fine) prof_end(c);
             if (fine) prof_begin(c, sizeof(T) == 4 ? K_LSOLVE32 : K_LSOLVE64);
             if (scan)
-                hipLaunchKernelGGL((k_zebra_scan<T, SLM>), dim3((unsigned)((lines + LINE_LINES - 1) / LINE_LINES)),
-                                   dim3(LINE_LINES * nseg), 0, st, L.g, L.z, col, nseg, sl, LV<T>::sf(L),
-                                   (const T *)LV<T>::rz(L), LV<T>::x(L));
+                hipLaunchKernelGGL((k_zebra_scan<T, SLM>), dim3(scan_blocks), dim3(LINE_LINES * nseg), 0, st, L.g, L.z, col, nseg,
+                                   sl, LV<T>::sf(L), (const T *)LV<T>::rz(L), LV<T>::x(L));
             else
                 hipLaunchKernelGGL((k_zebra_solve<T, LINE_PREFETCH>), dim3(blocks_for(lines, PTPB)), dim3(PTPB), 0, st, L.g,
                                    L.z, col, LV<T>::sf(L), LV<T>::rz(L), LV<T>::x(L));
