@@ -72,7 +72,8 @@ MEASURED_TRAFFIC = {
     "face_fluxes": 3.290e9,
     "cell_rhs": 2.408e9,
     "line_rhs_fp32": 0.382e9,
-    "poisson_apply_fp64": 2.890e9,
+    "line_rhs_compact_fp32": 0.208e9,
+    "poisson_apply_fp64": 2.501e9,
     "contra2cart": 1.993e9,
 }
 
