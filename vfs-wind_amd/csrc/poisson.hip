@@ -294,8 +294,10 @@ __global__ void k_assemble(Grid g, FaceSet F, const double *__restrict__ nvert, 
 }
 
 // y = A x (rows only), or r = b - A x when b != nullptr; x may be stored in a
-// narrower type X (an FP32 preconditioned vector, exact in FP64)
-template <typename T, typename X = T>
+// narrower type X (an FP32 preconditioned vector, exact in FP64).  NQ = 7
+// applies only the centre and the six axis neighbours (the compact levels'
+// V-cycle operator; the stencil's first seven slots).
+template <typename T, typename X = T, int NQ = NST>
 __global__ void __launch_bounds__(PTPB) k_apply(Grid g, const T *__restrict__ coef, const unsigned char *__restrict__ row,
                                                 const X *__restrict__ x, const T *__restrict__ b, T *__restrict__ y)
 {
@@ -305,7 +307,7 @@ __global__ void __launch_bounds__(PTPB) k_apply(Grid g, const T *__restrict__ co
     int i = t % g.mx; long long r = t / g.mx; int j = r % g.my, k = r / g.my;
     T s = 0;
 #pragma unroll
-    for (int q = 0; q < NST; q++) s += coef[q * g.n + t] * (T)x[nbr_idx(g, k, j, i, q)];
+    for (int q = 0; q < NQ; q++) s += coef[q * g.n + t] * (T)x[nbr_idx(g, k, j, i, q)];
     y[t] = b ? b[t] - s : s;
 }
 
@@ -339,18 +341,20 @@ __host__ __device__ inline int zebra_i0(int k, int c) { return 1 + ((1 + k + c) 
 // O(1) for the Poisson stencil and FP16 keeps 11 bits of them; the smoother
 // is a preconditioner of the FP64 FGMRES, whose residual stays exact.
 typedef _Float16 h2_t __attribute__((ext_vector_type(2)));
-// Compact FP32 smoother, for levels whose cross couplings (the twelve
+// Compact FP32 V-cycle level, for levels whose cross couplings (the twelve
 // off-line neighbours off the i and k axes: edges through j and the i-k
-// diagonals) are weak: only the ratios of the four axis neighbours
-// (i+1, i-1, k+1, k-1 in the line's own row) are kept, as one 8-byte word.
+// diagonals) are weak: the level works on the axis part of its operator.
+// Its smoother keeps the ratios of the four axis neighbours (i+1, i-1, k+1,
+// k-1 in the line's own row) as one 8-byte word, and its residual reads the
+// centre and the six axis coefficients (7 of the 19 FP32 planes).
 typedef _Float16 h4_t __attribute__((ext_vector_type(4)));
 // Cross-coupling strength (max over rows of sum |a_cross| / |a_P|) up to
-// which a level takes the compact smoother.  On a 128x96x128 channel whose
-// nodes were displaced by short-wavelength sines of growing amplitude the
-// compact form took as many FGMRES iterations as the full one at strength
-// 0.0085 (7 % less solve time), one more per solve from 0.033 (slower), 8-9
-// instead of 6 at 0.083.  A channel whose only distortion spans the whole
-// box (2 % of the spacing) measures 0.0012.
+// which a level takes the compact form.  On a 128x96x128 channel whose nodes
+// were displaced by short-wavelength sines of growing amplitude the compact
+// smoother took as many FGMRES iterations as the full one at strength 0.0085
+// (7 % less solve time; 11 % with the axis-only residual too), one more per
+// solve from 0.033 (slower), 8-9 instead of 6 at 0.083.  A channel whose only
+// distortion spans the whole box (2 % of the spacing) measures 0.0012.
 constexpr double COMPACT_MAX_CROSS = 0.01;
 
 // a / d as a float within the FP16 range (the clamp only guards degenerate rows)
@@ -1122,7 +1126,7 @@ __global__ void __launch_bounds__(RED_THREADS) k_cross_strength(Grid g, const do
     if (threadIdx.x == 0) part[blockIdx.x] = v;
 }
 
-// The FP32 smoother's form for a level: compact when its cross couplings are
+// The FP32 V-cycle's form for a level: compact when its cross couplings are
 // weak (VFS_SMOOTH_COMPACT=0 or 1 forces the full or the compact form)
 static int choose_smoother(vfs_ctx *c, PLevel &L)
 {
@@ -1422,7 +1426,14 @@ static void vcycle(vfs_ctx *c, vfs_poisson *ps, int l, int pre, int post, int co
         return;
     }
     smooth<T>(c, L, pre);
-    hipLaunchKernelGGL(k_apply<T>, dim3(nb), dim3(PTPB), 0, st, L.g, LV<T>::coef(L), L.row, LV<T>::x(L), LV<T>::b(L), LV<T>::r(L));
+    // a compact level's V-cycle works on the axis part of its operator: the
+    // residual it restricts takes the centre and the six axis neighbours
+    if (sizeof(T) == 4 && L.compact)
+        hipLaunchKernelGGL((k_apply<T, T, 7>), dim3(nb), dim3(PTPB), 0, st, L.g, LV<T>::coef(L), L.row, LV<T>::x(L), LV<T>::b(L),
+                           LV<T>::r(L));
+    else
+        hipLaunchKernelGGL(k_apply<T>, dim3(nb), dim3(PTPB), 0, st, L.g, LV<T>::coef(L), L.row, LV<T>::x(L), LV<T>::b(L),
+                           LV<T>::r(L));
     PLevel &C = ps->L[l + 1];
     hipLaunchKernelGGL(k_restrict<T>, dim3(blocks_for(C.g.n, PTPB)), dim3(PTPB), 0, st, C.g, L.g, C.row, L.row, LV<T>::r(L), LV<T>::b(C));
     if (l == 0) {
