@@ -77,6 +77,48 @@ def test_formfunction2(pair):
     assert relerr(ctx.download("RHS"), o["rhs"]) <= RHS_TOL
 
 
+def test_face_pass_blanking(pair, monkeypatch):
+    """With no blanked point the face pass runs an instance without blanking
+    tests: the same bits as the instance with them (VFS_FACE_NB=0).  Blanked
+    cells (nvert = 1 on a small block) reset that choice, and the face pass
+    with its tests matches the oracle."""
+    c, o, ctx = pair
+    saved = {f: ctx.download(f) for f in ("RHS", "NU_T")}
+    o.eddy_viscosity()
+    ctx.upload("NU_T", o["nu_t"])
+    zero = np.zeros(o["rhs"].shape)
+
+    def ff():
+        ctx.upload("RHS", zero)
+        ctx.formfunction2(1.0)
+        return ctx.download("RHS")
+
+    no_tests = ff()
+    monkeypatch.setenv("VFS_FACE_NB", "0")
+    with_tests = ff()
+    monkeypatch.delenv("VFS_FACE_NB")
+    assert np.array_equal(no_tests, with_tests)
+    o.set("rhs", zero)
+    o.formfunction2(1.0)
+    assert relerr(no_tests, o["rhs"]) <= RHS_TOL
+    mz, my, mx = o.shape
+    nv = np.zeros(o.shape)
+    nv[mz // 2, my // 2: my // 2 + 2, 3:6] = 1.0
+    o.m["nvert"][...] = nv
+    ctx.upload("NVERT", nv)
+    try:
+        blanked = ff()
+        o.set("rhs", zero)
+        o.formfunction2(1.0)
+        assert relerr(blanked, o["rhs"]) <= RHS_TOL
+        assert not np.array_equal(blanked, no_tests)
+    finally:
+        o.m["nvert"][...] = 0.0
+        ctx.upload("NVERT", np.zeros(o.shape))
+        for f, v in saved.items():  # the fixture is shared by the module's tests
+            ctx.upload(f, v)
+
+
 def test_stored_face_metrics_paths(pair):
     """Once a metric array may have been written from outside (here: its
     device pointer was handed out) the face pass reads the stored face-metric

@@ -18,6 +18,7 @@
 #include <cstdlib>
 
 #include "vfs_internal.h"
+#include "vfs_reduce.h"
 
 #pragma clang fp contract(off)
 
@@ -56,7 +57,10 @@ struct TileDims {
     static constexpr int HI = FT_I + (SO ? 2 : 3), HJ = FT_J + (SO ? 2 : 3), HP = HI * HJ;
 };
 
-template <bool SO>
+// NB: no point of the grid is blanked (every nvert is 0), so NV is the
+// constant 0 and every blanking test folds away at compile time; the tests
+// then take the branches they take for nvert = 0, so the result is the same.
+template <bool SO, bool NB = false>
 struct TileAcc {
     using D = TileDims<SO>;
     const FaceArgs *a;
@@ -80,6 +84,7 @@ struct TileAcc {
     }
     __device__ inline double NV(int k, int j, int i) const
     {
+        if constexpr (NB) return 0.0;
         bool in;
         const int l = lidx(k, j, i, in);
         return in ? (double)nv[l] : a->nvert[gidx(a->g, k, j, i)];
@@ -130,8 +135,8 @@ __device__ inline void load_metrics(const Metrics9 &m, long long q, double v[10]
 
 // convective and viscous flux of the i face of point (i,j,k)
 // (momentum.c:668-928): i in [0,mx-2], j,k interior; zero elsewhere
-template <bool SO>
-__device__ inline void flux_i(const FaceArgs &a, const TileAcc<SO> &T, int i, int j, int k, const double *Mf, double ucf,
+template <bool SO, bool NB = false>
+__device__ inline void flux_i(const FaceArgs &a, const TileAcc<SO, NB> &T, int i, int j, int k, const double *Mf, double ucf,
                               double dv[3], double vs[3])
 {
     const Grid &g = a.g;
@@ -185,8 +190,8 @@ __device__ inline void flux_i(const FaceArgs &a, const TileAcc<SO> &T, int i, in
 
 // convective and viscous flux of the j face of point (i,j,k)
 // (momentum.c:930-1211): j in [0,my-2], i,k interior; zero elsewhere
-template <bool SO>
-__device__ inline void flux_j(const FaceArgs &a, const TileAcc<SO> &T, int i, int j, int k, const double *Mf, double ucf,
+template <bool SO, bool NB = false>
+__device__ inline void flux_j(const FaceArgs &a, const TileAcc<SO, NB> &T, int i, int j, int k, const double *Mf, double ucf,
                               double dv[3], double vs[3])
 {
     const Grid &g = a.g;
@@ -240,8 +245,8 @@ __device__ inline void flux_j(const FaceArgs &a, const TileAcc<SO> &T, int i, in
 
 // convective and viscous flux of the k face of point (i,j,k)
 // (momentum.c:1212-1451): k in [0,mz-2], i,j interior; zero elsewhere
-template <bool SO>
-__device__ inline void flux_k(const FaceArgs &a, const TileAcc<SO> &T, int i, int j, int k, const double *Mf, double ucf,
+template <bool SO, bool NB = false>
+__device__ inline void flux_k(const FaceArgs &a, const TileAcc<SO, NB> &T, int i, int j, int k, const double *Mf, double ucf,
                               double dv[3], double vs[3])
 {
     const Grid &g = a.g;
@@ -429,6 +434,7 @@ constexpr int FX_J = FT_J + 1;
 static_assert(FT_I == 32, "a row of the tile is half a wavefront (the i-face exchange shuffles within it)");
 constexpr int HX_T0 = 64;  // first thread of the second wavefront
 static_assert(FT_I * FT_J >= HX_T0 + FT_J && FT_J * 2 <= 64, "halo lanes of the i faces");
+template <bool NB>
 __global__ void __launch_bounds__(FT_I * FT_J) __attribute__((amdgpu_waves_per_eu(2))) k_face_fp(FaceArgs a, double *__restrict__ fp)
 {
     using D = TileDims<true>;
@@ -469,7 +475,7 @@ __global__ void __launch_bounds__(FT_I * FT_J) __attribute__((amdgpu_waves_per_e
                 if (wi >= 0 && wi < g.mx && wj >= 0 && wj < g.my && wk >= 0 && wk < g.mz) {
                     const long long q = ((long long)wk * g.my + wj) * g.mx + wi;
                     rv[r][0] = a.ucat[3 * q]; rv[r][1] = a.ucat[3 * q + 1]; rv[r][2] = a.ucat[3 * q + 2];
-                    rn[r] = (float)a.nvert[q];
+                    if (!NB) rn[r] = (float)a.nvert[q];
                     if (les) rv[r][3] = a.nut[q];
                 }
             }
@@ -481,7 +487,8 @@ __global__ void __launch_bounds__(FT_I * FT_J) __attribute__((amdgpu_waves_per_e
             const int p = threadIdx.x + r * FT_I * FT_J;
             if (p < D::HP) {
                 const int l = slot * D::HP + p;
-                sx[l] = rv[r][0]; sy[l] = rv[r][1]; sz[l] = rv[r][2]; st[l] = rv[r][3]; sn[l] = rn[r];
+                sx[l] = rv[r][0]; sy[l] = rv[r][1]; sz[l] = rv[r][2]; st[l] = rv[r][3];
+                if (!NB) sn[l] = rn[r];
             }
         }
     };
@@ -501,7 +508,7 @@ __global__ void __launch_bounds__(FT_I * FT_J) __attribute__((amdgpu_waves_per_e
         store_plane(s2);  // plane k+1
         __syncthreads();
         if (k + 1 < k1) fetch_plane(k + 2);
-        TileAcc<true> T{&a, sx, sy, sz, st, sn, i0, j0, k, slot0};
+        TileAcc<true, NB> T{&a, sx, sy, sz, st, sn, i0, j0, k, slot0};
         const bool full = k >= k0;  // the plane below the block: its k face only
         const long long q = ((long long)k * g.my + j) * g.mx + i;
         if (inb && !have) load_metrics(a.cell, q, cur);
@@ -516,7 +523,7 @@ __global__ void __launch_bounds__(FT_I * FT_J) __attribute__((amdgpu_waves_per_e
                 double nb_[10], M[10];
                 load_metrics(a.cell, i < g.mx - 1 ? q + 1 : q, nb_);
                 face_mean(cur, nb_, M);
-                flux_i<true>(a, T, i, j, k, M, a.ucont[3 * q + 0], f6, f6 + 3);
+                flux_i<true, NB>(a, T, i, j, k, M, a.ucont[3 * q + 0], f6, f6 + 3);
             }
             double fl[6];
             for (int c = 0; c < 6; c++) fl[c] = __shfl_up(f6[c], 1, FT_I);
@@ -526,7 +533,7 @@ __global__ void __launch_bounds__(FT_I * FT_J) __attribute__((amdgpu_waves_per_e
                 double nb_[10], M[10];
                 load_metrics(a.cell, j < g.my - 1 ? q + g.mx : q, nb_);
                 face_mean(cur, nb_, M);
-                flux_j<true>(a, T, i, j, k, M, a.ucont[3 * q + 1], f6, f6 + 3);
+                flux_j<true, NB>(a, T, i, j, k, M, a.ucont[3 * q + 1], f6, f6 + 3);
             }
             for (int c = 0; c < 6; c++) fxj[c][ty + 1][tx] = f6[c];
             if (ty == 0) {  // j face at j0-1 of this column
@@ -536,7 +543,7 @@ __global__ void __launch_bounds__(FT_I * FT_J) __attribute__((amdgpu_waves_per_e
                     double cp[10], M[10];
                     load_metrics(a.cell, p, cp);
                     face_mean(cp, cur, M);
-                    flux_j<true>(a, T, i, j0 - 1, k, M, a.ucont[3 * p + 1], f6, f6 + 3);
+                    flux_j<true, NB>(a, T, i, j0 - 1, k, M, a.ucont[3 * p + 1], f6, f6 + 3);
                 }
                 for (int c = 0; c < 6; c++) fxj[c][0][tx] = f6[c];
             }
@@ -553,7 +560,7 @@ __global__ void __launch_bounds__(FT_I * FT_J) __attribute__((amdgpu_waves_per_e
                     load_metrics(a.cell, p, cp);
                     load_metrics(a.cell, p + 1, cn);
                     face_mean(cp, cn, M);
-                    flux_i<true>(a, T, i0 - 1, jr, k, M, a.ucont[3 * p + 0], f6, f6 + 3);
+                    flux_i<true, NB>(a, T, i0 - 1, jr, k, M, a.ucont[3 * p + 0], f6, f6 + 3);
                 }
                 for (int c = 0; c < 6; c++) hx[c][r] = f6[c];
             }
@@ -563,7 +570,7 @@ __global__ void __launch_bounds__(FT_I * FT_J) __attribute__((amdgpu_waves_per_e
             load_metrics(a.cell, k < g.mz - 1 ? q + g.plane : q, nk);
             double M[10];
             face_mean(cur, nk, M);
-            flux_k<true>(a, T, i, j, k, M, a.ucont[3 * q + 2], kf, kf + 3);
+            flux_k<true, NB>(a, T, i, j, k, M, a.ucont[3 * q + 2], kf, kf + 3);
         }
         __syncthreads();
         if (full && inb) {
@@ -880,6 +887,33 @@ __global__ void __launch_bounds__(CR_I * CR_J) k_cell_rhs_pr(CellArgs a, const d
     }
 }
 
+// max |nvert| over every point (ghosts included), one block partial each
+__global__ void __launch_bounds__(RED_THREADS) k_nvert_absmax(long long n, const double *__restrict__ nvert,
+                                                             double *__restrict__ part)
+{
+    double v = 0;
+    for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < n; t += (long long)gridDim.x * blockDim.x)
+        v = fmax(v, fabs(nvert[t]));
+    v = block_reduce<OpMax>(v);
+    if (threadIdx.x == 0) part[blockIdx.x] = v;
+}
+
+// whether every nvert is 0, measured once after each write of VFS_NVERT
+static int nvert_all_zero(vfs_ctx *c, bool *zero)
+{
+    if (c->nvert_zero < 0) {
+        hipStream_t st = c->stream;
+        hipLaunchKernelGGL(k_nvert_absmax, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, st, c->g.n, c->f[VFS_NVERT], c->red);
+        hipLaunchKernelGGL(finish_reduce<OpMax>, dim3(1), dim3(RED_THREADS), 0, st, c->red, RED_BLOCKS, c->red + RED_BLOCKS);
+        VFS_LAUNCH_CHECK("nvert scan");
+        VFS_HIP(hipMemcpyAsync(c->red_host, c->red + RED_BLOCKS, sizeof(double), hipMemcpyDeviceToHost, st));
+        VFS_HIP(hipStreamSynchronize(st));
+        c->nvert_zero = c->red_host[0] == 0.0 ? 1 : 0;
+    }
+    *zero = c->nvert_zero == 1;
+    return VFS_OK;
+}
+
 int formfunction2(vfs_ctx *c, double scale, const double *pucont, double rk_alpha, double *ucont_out)
 {
     const Grid &g = c->g;
@@ -908,9 +942,20 @@ int formfunction2(vfs_ctx *c, double scale, const double *pucont, double rk_alph
     // VFS_FUSED_FP=0 keeps the flux planes (the face pass, then k_cell_fp)
     const char *env = std::getenv("VFS_FUSED_FP");
     const bool fused = fa.second_order && c->face_derived && !(env && env[0] == '0');
+    // a grid with no blanked point takes the face pass without blanking tests
+    // (VFS_FACE_NB=0 keeps the tests)
+    bool nb = false;
+    if (fused) {
+        int rc = nvert_all_zero(c, &nb);
+        if (rc) return rc;
+        const char *e = std::getenv("VFS_FACE_NB");
+        if (e && e[0] == '0') nb = false;
+    }
     prof_begin(c, K_FACE);
-    if (fused)
-        hipLaunchKernelGGL(k_face_fp, dim3(nbt), dim3(FT_I * FT_J), 0, st, fa, c->fpc);
+    if (fused && nb)
+        hipLaunchKernelGGL(k_face_fp<true>, dim3(nbt), dim3(FT_I * FT_J), 0, st, fa, c->fpc);
+    else if (fused)
+        hipLaunchKernelGGL(k_face_fp<false>, dim3(nbt), dim3(FT_I * FT_J), 0, st, fa, c->fpc);
     else if (fa.second_order && c->face_derived)
         hipLaunchKernelGGL((k_face_fluxes<true, true>), dim3(nbt), dim3(FT_I * FT_J), 0, st, fa);
     else if (fa.second_order)

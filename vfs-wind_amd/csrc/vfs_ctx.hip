@@ -127,6 +127,7 @@ int vfs_ctx_create(const vfs_params *params, vfs_ctx **out)
     c->mean_k_flux = 0;
     c->face_derived = false;
     vfs::ucat_stale(c);
+    c->nvert_zero = -1;
     c->imp_ws = nullptr;
     c->imp_ws_bytes = 0;
     c->prof_on = 0;
@@ -219,6 +220,7 @@ int vfs_upload(vfs_ctx *c, vfs_field f, const double *host)
     if (!c || !host || f < 0 || f >= VFS_NFIELDS) return vfs::set_error(VFS_EINVAL, "bad upload argument");
     if (vfs::is_metric_field(f)) c->face_derived = false;
     if (vfs::is_c2c_field(f)) vfs::ucat_stale(c);
+    if (f == VFS_NVERT) c->nvert_zero = -1;
     VFS_HIP(hipMemcpyAsync(c->f[f], host, vfs_field_size(c, f) * sizeof(double), hipMemcpyHostToDevice, c->stream));
     VFS_HIP(hipStreamSynchronize(c->stream));
     return VFS_OK;
@@ -237,6 +239,7 @@ double *vfs_field_device_ptr(vfs_ctx *c, vfs_field f)
     if (!c || f < 0 || f >= VFS_NFIELDS) return nullptr;
     if (vfs::is_metric_field(f)) c->face_derived = false;  // the caller may write through it
     if (vfs::is_c2c_field(f)) vfs::ucat_stale(c);
+    if (f == VFS_NVERT) c->nvert_zero = -1;
     return c->f[f];
 }
 

@@ -206,6 +206,10 @@ struct vfs_ctx {
     // faces k_wall_flux resets; c2c then recomputes only those cells before
     // its ghost-cell passes
     int ucat_part;
+    // 1 when every nvert is 0 (no blanked point: the face pass then runs
+    // without blanking tests), 0 when some is not, -1 until measured; reset
+    // by every write of VFS_NVERT
+    int nvert_zero;
     // Newton-GMRES vectors of the implicit momentum solve, kept between calls
     // (allocated on first use, grown when a longer restart asks for more)
     double *imp_ws;
