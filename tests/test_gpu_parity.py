@@ -77,46 +77,66 @@ def test_formfunction2(pair):
     assert relerr(ctx.download("RHS"), o["rhs"]) <= RHS_TOL
 
 
-def test_face_pass_blanking(pair, monkeypatch):
-    """With no blanked point the face pass runs an instance without blanking
-    tests: the same bits as the instance with them (VFS_FACE_NB=0).  Blanked
-    cells (nvert = 1 on a small block) reset that choice, and the face pass
-    with its tests matches the oracle."""
+def test_blanking_free_instances(pair, monkeypatch):
+    """With no blanked point the face pass, the pressure gradient and the
+    projection run instances without blanking tests: the same bits as the
+    instances with them (VFS_NOBLANK=0).  Blanked cells (nvert = 1 on a small
+    block) reset that choice, and the kernels with their tests match the
+    oracle."""
     c, o, ctx = pair
-    saved = {f: ctx.download(f) for f in ("RHS", "NU_T")}
+    names = ("RHS", "NU_T", "UCONT", "UCAT", "P", "PHI", "DP")
+    saved = {f: ctx.download(f) for f in names}
+    saved_o = {f: o[f].copy() for f in o.fields}
     o.eddy_viscosity()
     ctx.upload("NU_T", o["nu_t"])
+    rng = np.random.default_rng(21)
+    p = rng.standard_normal(o.shape)
+    phi = rng.standard_normal(o.shape)
     zero = np.zeros(o["rhs"].shape)
 
-    def ff():
-        ctx.upload("RHS", zero)
+    def gpu():
+        for f, v in (("UCONT", saved["UCONT"]), ("UCAT", saved["UCAT"]), ("P", p), ("PHI", phi), ("RHS", zero)):
+            ctx.upload(f, v)
         ctx.formfunction2(1.0)
-        return ctx.download("RHS")
+        ctx.calc_k_flux()
+        ctx.pressure_gradient()
+        ctx.projection()
+        return [ctx.download(f) for f in ("RHS", "DP", "UCONT")]
 
-    no_tests = ff()
-    monkeypatch.setenv("VFS_FACE_NB", "0")
-    with_tests = ff()
-    monkeypatch.delenv("VFS_FACE_NB")
-    assert np.array_equal(no_tests, with_tests)
-    o.set("rhs", zero)
-    o.formfunction2(1.0)
-    assert relerr(no_tests, o["rhs"]) <= RHS_TOL
-    mz, my, mx = o.shape
-    nv = np.zeros(o.shape)
-    nv[mz // 2, my // 2: my // 2 + 2, 3:6] = 1.0
-    o.m["nvert"][...] = nv
-    ctx.upload("NVERT", nv)
-    try:
-        blanked = ff()
-        o.set("rhs", zero)
+    def cpu():
+        for f, v in (("ucont", saved["UCONT"]), ("ucat", saved["UCAT"]), ("p", p), ("phi", phi), ("rhs", zero)):
+            o.set(f, v)
         o.formfunction2(1.0)
-        assert relerr(blanked, o["rhs"]) <= RHS_TOL
-        assert not np.array_equal(blanked, no_tests)
+        o.calc_k_flux()
+        o.pressure_gradient()
+        o.projection()
+        return [o[f] for f in ("rhs", "dp", "ucont")]
+
+    try:
+        no_tests = gpu()
+        monkeypatch.setenv("VFS_NOBLANK", "0")
+        with_tests = gpu()
+        monkeypatch.delenv("VFS_NOBLANK")
+        for a_, b_ in zip(no_tests, with_tests):
+            assert np.array_equal(a_, b_)
+        for a_, b_ in zip(no_tests, cpu()):
+            assert relerr(a_, b_) <= RHS_TOL
+        mz, my, mx = o.shape
+        nv = np.zeros(o.shape)
+        nv[mz // 2, my // 2: my // 2 + 2, 3:6] = 1.0
+        o.m["nvert"][...] = nv
+        ctx.upload("NVERT", nv)
+        blanked = gpu()
+        for a_, b_ in zip(blanked, cpu()):
+            assert relerr(a_, b_) <= RHS_TOL
+        assert not np.array_equal(blanked[0], no_tests[0])
     finally:
         o.m["nvert"][...] = 0.0
         ctx.upload("NVERT", np.zeros(o.shape))
         for f, v in saved.items():  # the fixture is shared by the module's tests
             ctx.upload(f, v)
+        for f, v in saved_o.items():
+            o.set(f, v)
 
 
 def test_stored_face_metrics_paths(pair):

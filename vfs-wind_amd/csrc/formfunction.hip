@@ -898,8 +898,7 @@ __global__ void __launch_bounds__(RED_THREADS) k_nvert_absmax(long long n, const
     if (threadIdx.x == 0) part[blockIdx.x] = v;
 }
 
-// whether every nvert is 0, measured once after each write of VFS_NVERT
-static int nvert_all_zero(vfs_ctx *c, bool *zero)
+int nvert_all_zero(vfs_ctx *c, bool *zero)
 {
     if (c->nvert_zero < 0) {
         hipStream_t st = c->stream;
@@ -910,7 +909,8 @@ static int nvert_all_zero(vfs_ctx *c, bool *zero)
         VFS_HIP(hipStreamSynchronize(st));
         c->nvert_zero = c->red_host[0] == 0.0 ? 1 : 0;
     }
-    *zero = c->nvert_zero == 1;
+    const char *e = std::getenv("VFS_NOBLANK");
+    *zero = c->nvert_zero == 1 && !(e && e[0] == '0');
     return VFS_OK;
 }
 
@@ -943,13 +943,10 @@ int formfunction2(vfs_ctx *c, double scale, const double *pucont, double rk_alph
     const char *env = std::getenv("VFS_FUSED_FP");
     const bool fused = fa.second_order && c->face_derived && !(env && env[0] == '0');
     // a grid with no blanked point takes the face pass without blanking tests
-    // (VFS_FACE_NB=0 keeps the tests)
     bool nb = false;
     if (fused) {
         int rc = nvert_all_zero(c, &nb);
         if (rc) return rc;
-        const char *e = std::getenv("VFS_FACE_NB");
-        if (e && e[0] == '0') nb = false;
     }
     prof_begin(c, K_FACE);
     if (fused && nb)

@@ -197,6 +197,8 @@ struct ProjArgs {
     double *ucont;
 };
 
+// NB: no point is blanked (every nvert 0): NV is the constant 0
+template <bool NB>
 __global__ void __launch_bounds__(QTPB) k_projection(ProjArgs a)
 {
     const Grid &g = a.g;
@@ -209,7 +211,7 @@ __global__ void __launch_bounds__(QTPB) k_projection(ProjArgs a)
     const double th = a.th;
     const long long q = gidx(g, k, j, i);
 #define PH(kk, jj, ii) a.phi[gidx(g, (kk), (jj), (ii))]
-#define NV(kk, jj, ii) a.nvert[gidx(g, (kk), (jj), (ii))]
+#define NV(kk, jj, ii) (NB ? 0.0 : a.nvert[gidx(g, (kk), (jj), (ii))])
     // the three components are read and written together (separate
     // read-modify-writes of one component of the AoS line each evicted
     // partly written lines)
@@ -322,7 +324,11 @@ int projection(vfs_ctx *c)
     a.kcsi = c->f[VFS_KCSI]; a.keta = c->f[VFS_KETA]; a.kzet = c->f[VFS_KZET]; a.kaj = c->f[VFS_KAJ];
     a.ucont = c->f[VFS_UCONT];
     dim3 grd(blocks_for(g.mx - 2, QTPB), g.my - 2, g.mz - 2);
-    hipLaunchKernelGGL(k_projection, grd, dim3(QTPB), 0, st, a);
+    bool nb = false;
+    int rc0 = nvert_all_zero(c, &nb);
+    if (rc0) return rc0;
+    if (nb) hipLaunchKernelGGL(k_projection<true>, grd, dim3(QTPB), 0, st, a);
+    else hipLaunchKernelGGL(k_projection<false>, grd, dim3(QTPB), 0, st, a);
     ucat_stale(c);
     VFS_LAUNCH_CHECK("projection");
     int rc = periodic_fix_grid(st, g, c->f[VFS_UCONT], 3, 1);  // poisson.c:3001-3040

@@ -234,6 +234,10 @@ inline bool is_c2c_field(int f)
     return f == VFS_COOR || f == VFS_NVERT || is_metric_field(f) || f == VFS_UCONT || f == VFS_UCAT || f == VFS_UCAT_OLD;
 }
 int c2c(vfs_ctx *c);
+// whether every nvert of the grid is 0 (measured once after each write of
+// VFS_NVERT); kernels then run instances without blanking tests, which give
+// the same bits.  VFS_NOBLANK=0 answers false (keeps the tests).
+int nvert_all_zero(vfs_ctx *c, bool *zero);
 int zero_ghosts(vfs_ctx *c, double *f);  // f = 0 on the six boundary planes
 enum { UCAT_STALE = 0, UCAT_INTERIOR = 1, UCAT_WALLS = 2 };
 // a write that may change any cell of VFS_UCAT's Contra2Cart inputs or of
