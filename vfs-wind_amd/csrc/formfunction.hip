@@ -859,14 +859,15 @@ __global__ void __launch_bounds__(CR_I * CR_J) k_cell_rhs_pr(CellArgs a, const d
             const double pi0 = pr[q], pi1 = pr[qi];
             const double pj0 = pr[n + q], pj1 = pr[n + qj];
             const double pk0 = have ? pk_next : pr[2 * n + q], pk1 = pr[2 * n + qk];
-            const double nv0 = have ? nvk : a.nvert[q];
-            const double nvkn = a.nvert[qk];
+            // a.nvert == nullptr: no point is blanked (every nvert 0)
+            const double *nv = a.nvert;
+            const double nv0 = have ? nvk : (nv ? nv[q] : 0.0);
+            const double nvkn = nv ? nv[qk] : 0.0;
             r[0] = a.scale * (0.5 * pi0 + 0.5 * pi1) * a.iaj[q];
             r[1] = a.scale * (0.5 * pj0 + 0.5 * pj1) * a.jaj[q];
             r[2] = a.scale * (0.5 * pk0 + 0.5 * pk1) * a.kaj[q];
-            const double *nv = a.nvert;
-            zero[0] = nv0 + nv[qi] > 0.1 || (!g.pi && i == mx - 2);
-            zero[1] = nv0 + nv[qj] > 0.1 || (!g.pj && j == my - 2);
+            zero[0] = nv0 + (nv ? nv[qi] : 0.0) > 0.1 || (!g.pi && i == mx - 2);
+            zero[1] = nv0 + (nv ? nv[qj] : 0.0) > 0.1 || (!g.pj && j == my - 2);
             zero[2] = nv0 + nvkn > 0.1 || (!g.pk && k == mz - 2);
             nvk = nvkn;
             pk_next = pk1;
@@ -996,6 +997,7 @@ int formfunction2(vfs_ctx *c, double scale, const double *pucont, double rk_alph
         prof_end(c);
     }
     prof_begin(c, K_CELL);
+    if (fused && nb) ca.nvert = nullptr;  // the cell pass then skips its blanking tests
     if (fused)
         hipLaunchKernelGGL(k_cell_rhs_pr,
                            dim3((unsigned)(((g.mx + CR_I - 1) / CR_I) * ((g.my + CR_J - 1) / CR_J) * ((g.mz + FT_K - 1) / FT_K))),

@@ -244,7 +244,7 @@ __device__ inline void c2c_point(const Grid &g, int k, int j, int i, const doubl
                                  double *__restrict__ ucat)
 {
     const long long q = gidx(g, k, j, i);
-    if (nvert[q] >= 0.1) return;
+    if (nvert && nvert[q] >= 0.1) return;  // nvert == nullptr: no point is blanked
     double q0 = 0.5 * (ucont[3 * gidx(g, k, j, i - 1) + 0] + ucont[3 * q + 0]);
     double q1 = 0.5 * (ucont[3 * gidx(g, k, j - 1, i) + 1] + ucont[3 * q + 1]);
     double q2 = 0.5 * (ucont[3 * gidx(g, k - 1, j, i) + 2] + ucont[3 * q + 2]);
@@ -456,16 +456,20 @@ int c2c(vfs_ctx *c)
     // interior pass.
     const char *pe = std::getenv("VFS_C2C_PARTIAL");
     const int part = pe && pe[0] == '0' ? (int)UCAT_STALE : c->ucat_part;
+    bool nb = false;
+    rc = nvert_all_zero(c, &nb);
+    if (rc) return rc;
+    const double *nvi = nb ? nullptr : c->f[VFS_NVERT];  // interior passes skip the blanking test
     if (part == UCAT_WALLS) {
         const long long ni = (long long)(g.my - 2) * (g.mz - 2), nj = (long long)(g.mx - 2) * (g.mz - 2);
         hipLaunchKernelGGL(k_c2c_walls, dim3(blocks_for(4 * (ni > nj ? ni : nj), TPB)), dim3(TPB), 0, st, g, ucont,
-                           c->f[VFS_CSI], c->f[VFS_ETA], c->f[VFS_ZET], c->f[VFS_NVERT], ucat);
+                           c->f[VFS_CSI], c->f[VFS_ETA], c->f[VFS_ZET], nvi, ucat);
         VFS_LAUNCH_CHECK("c2c walls");
     } else if (part != UCAT_INTERIOR) {
         dim3 blk(TPB), grd(blocks_for(g.mx - 2, TPB), g.my - 2, g.mz - 2);
         prof_begin(c, K_C2C);
-        hipLaunchKernelGGL(k_c2c_interior, grd, blk, 0, st, g, ucont, c->f[VFS_CSI], c->f[VFS_ETA], c->f[VFS_ZET],
-                           c->f[VFS_NVERT], ucat);
+        hipLaunchKernelGGL(k_c2c_interior, grd, blk, 0, st, g, ucont, c->f[VFS_CSI], c->f[VFS_ETA], c->f[VFS_ZET], nvi,
+                           ucat);
         prof_end(c);
         VFS_LAUNCH_CHECK("c2c interior");
     }
