@@ -78,11 +78,11 @@ def test_formfunction2(pair):
 
 
 def test_blanking_free_instances(pair, monkeypatch):
-    """With no blanked point the face pass, the pressure gradient and the
-    projection run instances without blanking tests: the same bits as the
-    instances with them (VFS_NOBLANK=0).  Blanked cells (nvert = 1 on a small
-    block) reset that choice, and the kernels with their tests match the
-    oracle."""
+    """With no blanked point the eddy viscosity, the face and cell passes,
+    the pressure gradient, the projection and Contra2Cart run without their
+    blanking tests: the same bits as with them (VFS_NOBLANK=0).  Blanked
+    cells (nvert = 1 on a small block) reset that choice, and the kernels with
+    their tests match the oracle."""
     c, o, ctx = pair
     names = ("RHS", "NU_T", "UCONT", "UCAT", "P", "PHI", "DP")
     saved = {f: ctx.download(f) for f in names}
@@ -97,20 +97,22 @@ def test_blanking_free_instances(pair, monkeypatch):
     def gpu():
         for f, v in (("UCONT", saved["UCONT"]), ("UCAT", saved["UCAT"]), ("P", p), ("PHI", phi), ("RHS", zero)):
             ctx.upload(f, v)
+        ctx.eddy_viscosity()
         ctx.formfunction2(1.0)
         ctx.calc_k_flux()
         ctx.pressure_gradient()
         ctx.projection()
-        return [ctx.download(f) for f in ("RHS", "DP", "UCONT")]
+        return [ctx.download(f) for f in ("NU_T", "RHS", "DP", "UCONT")]
 
     def cpu():
         for f, v in (("ucont", saved["UCONT"]), ("ucat", saved["UCAT"]), ("p", p), ("phi", phi), ("rhs", zero)):
             o.set(f, v)
+        o.eddy_viscosity()
         o.formfunction2(1.0)
         o.calc_k_flux()
         o.pressure_gradient()
         o.projection()
-        return [o[f] for f in ("rhs", "dp", "ucont")]
+        return [o[f] for f in ("nu_t", "rhs", "dp", "ucont")]
 
     try:
         no_tests = gpu()
@@ -129,7 +131,7 @@ def test_blanking_free_instances(pair, monkeypatch):
         blanked = gpu()
         for a_, b_ in zip(blanked, cpu()):
             assert relerr(a_, b_) <= RHS_TOL
-        assert not np.array_equal(blanked[0], no_tests[0])
+        assert not np.array_equal(blanked[1], no_tests[1])
     finally:
         o.m["nvert"][...] = 0.0
         ctx.upload("NVERT", np.zeros(o.shape))

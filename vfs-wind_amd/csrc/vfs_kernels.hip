@@ -746,6 +746,8 @@ __global__ void k_eddy(Grid g, double cs2, const double *__restrict__ ucat, cons
 // k, k+1 stay in registers, so every value of those arrays leaves HBM once
 // (the thread-per-point form re-read the k +- 1 planes).  The ghost cells
 // get nu_t = 0 from k_zero_ghosts.
+// NB: no point is blanked (every nvert 0): the blanking tests fold away
+template <bool NB>
 __global__ void __launch_bounds__(KM_I * KM_J) k_eddy_km(Grid g, double cs2, const double *__restrict__ ucat,
                                                         const double *__restrict__ csi, const double *__restrict__ eta,
                                                         const double *__restrict__ zet, const double *__restrict__ aj,
@@ -758,14 +760,15 @@ __global__ void __launch_bounds__(KM_I * KM_J) k_eddy_km(Grid g, double cs2, con
     long long q = gidx(g, k0, j, i);
     double um[3], u0[3], up[3];
     for (int c = 0; c < 3; c++) { um[c] = ucat[3 * (q - P) + c]; u0[c] = ucat[3 * q + c]; }
-    double nvm = nvert[q - P], nv0 = nvert[q];
+#define NVR(x) (NB ? 0.0 : nvert[x])
+    double nvm = NVR(q - P), nv0 = NVR(q);
     for (int k = k0; k < k1; k++, q += P) {
         for (int c = 0; c < 3; c++) up[c] = ucat[3 * (q + P) + c];
-        const double nvp = nvert[q + P];
+        const double nvp = NVR(q + P);
         if (nv0 > 1.1) {
             nut[q] = 0;
         } else {
-            const double nve = nvert[q + 1], nvw = nvert[q - 1], nvn = nvert[q + g.mx], nvs = nvert[q - g.mx];
+            const double nve = NVR(q + 1), nvw = NVR(q - 1), nvn = NVR(q + g.mx), nvs = NVR(q - g.mx);
             double dd[3][3];
             for (int c = 0; c < 3; c++) {
                 const double ue = ucat[3 * (q + 1) + c], uw = ucat[3 * (q - 1) + c];
@@ -787,6 +790,7 @@ __global__ void __launch_bounds__(KM_I * KM_J) k_eddy_km(Grid g, double cs2, con
         nvm = nv0;
         nv0 = nvp;
     }
+#undef NVR
 }
 
 // f = 0 on the six boundary planes of the DA
@@ -831,7 +835,10 @@ int eddy_viscosity(vfs_ctx *c)
     } else {
         int rc = zero_ghosts(c, c->f[VFS_NU_T]);
         if (rc) return rc;
-        hipLaunchKernelGGL(k_eddy_km, dim3(km_blocks(g, KM_K)), dim3(KM_I * KM_J), 0, c->stream, g, c->p.cs2,
+        bool nb = false;
+        rc = nvert_all_zero(c, &nb);
+        if (rc) return rc;
+        hipLaunchKernelGGL(nb ? k_eddy_km<true> : k_eddy_km<false>, dim3(km_blocks(g, KM_K)), dim3(KM_I * KM_J), 0, c->stream, g, c->p.cs2,
                            c->f[VFS_UCAT], c->f[VFS_CSI], c->f[VFS_ETA], c->f[VFS_ZET], c->f[VFS_AJ], c->f[VFS_NVERT],
                            c->f[VFS_NU_T]);
     }
