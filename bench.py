@@ -68,13 +68,13 @@ ALGO = {
 # HBM bytes per launch of the finest-level launch, from rocprofv3 FETCH_SIZE (x2, calibrated on
 # copy kernels of known size) + WRITE_SIZE on this workload at 256x192x256
 MEASURED_TRAFFIC = {
-    "formfunction2": 5.755e9,
-    "face_fluxes": 3.290e9,
-    "cell_rhs": 2.408e9,
+    "formfunction2": 5.307e9,
+    "face_fluxes": 3.119e9,
+    "cell_rhs": 2.132e9,
     "line_rhs_fp32": 0.382e9,
     "line_rhs_compact_fp32": 0.208e9,
     "poisson_apply_fp64": 2.501e9,
-    "contra2cart": 1.993e9,
+    "contra2cart": 1.883e9,
 }
 
 
