@@ -267,7 +267,7 @@ __global__ void k_row_mask(Grid g, const double *__restrict__ nvert, double th, 
 {
     long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
     if (t >= g.n) return;
-    int i = t % g.mx; long long r = t / g.mx; int j = r % g.my, k = r / g.my;
+    const int t32 = (int)t; int i = t32 % g.mx; int r = t32 / g.mx; int j = r % g.my, k = r / g.my;
     row[t] = is_row(g, nvert, th, k, j, i) ? 1 : 0;
 }
 
@@ -278,7 +278,7 @@ __global__ void k_assemble(Grid g, FaceSet F, const double *__restrict__ nvert, 
 {
     long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
     if (t >= g.n) return;
-    int i = t % g.mx; long long r = t / g.mx; int j = r % g.my, k = r / g.my;
+    const int t32 = (int)t; int i = t32 % g.mx; int r = t32 / g.mx; int j = r % g.my, k = r / g.my;
     double vol[NST];
     if (row[t]) {
         assemble_row(g, F, nvert, th, k, j, i, vol);
@@ -304,7 +304,7 @@ __global__ void __launch_bounds__(PTPB) k_apply(Grid g, const T *__restrict__ co
     long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
     if (t >= g.n) return;
     if (!row[t]) { y[t] = 0; return; }
-    int i = t % g.mx; long long r = t / g.mx; int j = r % g.my, k = r / g.my;
+    const int t32 = (int)t; int i = t32 % g.mx; int r = t32 / g.mx; int j = r % g.my, k = r / g.my;
     T s = 0;
 #pragma unroll
     for (int q = 0; q < NQ; q++) s += coef[q * g.n + t] * (T)x[nbr_idx(g, k, j, i, q)];
@@ -375,8 +375,8 @@ __global__ void __launch_bounds__(PTPB) k_zebra_rhs_h2(Grid g, ZGrid z, int c, c
     long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
     const int npair = (z.njI + 1) >> 1;
     if (t >= (long long)z.nkI * npair * z.nh) return;
-    const int h = t % z.nh;
-    const long long r = t / z.nh;
+    const int h = (int)t % z.nh;
+    const int r = (int)t / z.nh;
     const int jp = r % npair, kk = r / npair;
     const int k = kk + 1, j = 2 * jp + 1, i = zebra_i0(k, c) + 2 * h;
     if (i > g.mx - 2) return;
@@ -429,8 +429,8 @@ __global__ void __launch_bounds__(PTPB) k_zebra_rhs_c4(Grid g, ZGrid z, int c, c
 {
     long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
     if (t >= z.cplane) return;
-    const int h = t % z.nh;
-    const long long r = t / z.nh;
+    const int h = (int)t % z.nh;
+    const int r = (int)t / z.nh;
     const int jj = r % z.njI, kk = r / z.njI;
     const int k = kk + 1, j = jj + 1, i = zebra_i0(k, c) + 2 * h;
     if (i > g.mx - 2) return;
@@ -451,8 +451,8 @@ __global__ void k_zebra_setup(Grid g, ZGrid z, const double *__restrict__ c64, c
 {
     long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
     if (t >= 2LL * z.nkI * z.nh) return;
-    const int h = t % z.nh;
-    const long long r = t / z.nh;
+    const int h = (int)t % z.nh;
+    const int r = (int)t / z.nh;
     const int kk = r % z.nkI, c = r / z.nkI, k = kk + 1;
     const int i = zebra_i0(k, c) + 2 * h;
     const bool valid = i <= g.mx - 2;
@@ -510,8 +510,8 @@ __global__ void __launch_bounds__(PTPB) k_zebra_rhs(Grid g, ZGrid z, int c, cons
 {
     long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
     if (t >= z.cplane) return;
-    const int h = t % z.nh;
-    const long long r = t / z.nh;
+    const int h = (int)t % z.nh;
+    const int r = (int)t / z.nh;
     const int jj = r % z.njI, kk = r / z.njI;
     const int k = kk + 1, j = jj + 1, i = zebra_i0(k, c) + 2 * h;
     if (i > g.mx - 2) return;
@@ -716,7 +716,7 @@ __global__ void k_restrict(Grid gc, Grid gf, const unsigned char *__restrict__ r
     long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
     if (t >= gc.n) return;
     if (!rowc[t]) { bc[t] = 0; return; }
-    int i = t % gc.mx; long long r = t / gc.mx; int j = r % gc.my, k = r / gc.my;
+    const int t32 = (int)t; int i = t32 % gc.mx; int r = t32 / gc.mx; int j = r % gc.my, k = r / gc.my;
     T s = 0;
     for (int dk = 0; dk < 2; dk++)
         for (int dj = 0; dj < 2; dj++)
@@ -743,7 +743,7 @@ __global__ void k_prolong(Grid gc, Grid gf, const unsigned char *__restrict__ ro
     long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
     if (t >= gf.n) return;
     if (!rowf[t]) return;
-    int i = t % gf.mx; long long r = t / gf.mx; int j = r % gf.my, k = r / gf.my;
+    const int t32 = (int)t; int i = t32 % gf.mx; int r = t32 / gf.mx; int j = r % gf.my, k = r / gf.my;
     int I = (i + 1) / 2, J = (j + 1) / 2, K = (k + 1) / 2;
     int I2 = parent_nb(I, (i & 1) ? -1 : 1, gc.mx, gc.pi);
     int J2 = parent_nb(J, (j & 1) ? -1 : 1, gc.my, gc.pj);
@@ -760,7 +760,7 @@ __global__ void k_coarse_coor(Grid gc, Grid gf, const double *__restrict__ cf, d
 {
     long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
     if (t >= gc.n) return;
-    int i = t % gc.mx; long long r = t / gc.mx; int j = r % gc.my, k = r / gc.my;
+    const int t32 = (int)t; int i = t32 % gc.mx; int r = t32 / gc.mx; int j = r % gc.my, k = r / gc.my;
     if (i >= gc.mx - 1 || j >= gc.my - 1 || k >= gc.mz - 1) {
         cc[3 * t] = cc[3 * t + 1] = cc[3 * t + 2] = 0;
         return;
@@ -774,7 +774,7 @@ __global__ void k_coarse_nvert(Grid gc, Grid gf, const double *__restrict__ nf, 
 {
     long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
     if (t >= gc.n) return;
-    int i = t % gc.mx; long long r = t / gc.mx; int j = r % gc.my, k = r / gc.my;
+    const int t32 = (int)t; int i = t32 % gc.mx; int r = t32 / gc.mx; int j = r % gc.my, k = r / gc.my;
     if (i < 1 || i > gc.mx - 2 || j < 1 || j > gc.my - 2 || k < 1 || k > gc.mz - 2) { nc[t] = 0; return; }
     double mn = 1e30;
     for (int dk = 0; dk < 2; dk++)
@@ -1250,6 +1250,8 @@ static int level_dense_inverse(vfs_ctx *c, vfs_poisson *ps, PLevel &L)
 static int poisson_setup(vfs_ctx *c, int levels)
 {
     if (c->ps && c->ps->nl == levels) return VFS_OK;
+    // the solver's point kernels decode (i, j, k) in 32-bit arithmetic
+    if (c->g.n >= (1LL << 31)) return set_error(VFS_EINVAL, "Poisson solver: grid of 2^31 points or more");
     poisson_free(c);
     vfs_poisson *ps = new vfs_poisson();
     c->ps = ps;
