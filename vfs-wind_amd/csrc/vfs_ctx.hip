@@ -35,6 +35,9 @@ int validate(const vfs_params *p)
     if (!p) return vfs::set_error(VFS_EINVAL, "null params");
     if (p->mx < 4 || p->my < 4 || p->mz < 4)
         return vfs::set_error(VFS_EINVAL, "grid too small: each DA dimension needs at least 4 points");
+    // point kernels decode (i, j, k) from a point index in 32-bit arithmetic
+    if ((long long)p->mx * p->my * p->mz >= (1LL << 31))
+        return vfs::set_error(VFS_EINVAL, "grid too large: the DA must hold fewer than 2^31 points");
     if (p->les != 0 && p->les != 1)
         return vfs::set_error(VFS_EINVAL, "only -les 0 and -les 1 are supported");
     if (p->ren <= 0 || p->dt <= 0)

@@ -84,8 +84,9 @@ struct Decode {
 __device__ inline bool decode(const Grid &g, long long t, Decode &d)
 {
     if (t >= g.n) return false;
-    d.i = t % g.mx;
-    long long r = t / g.mx;
+    const int t32 = (int)t;  // g.n < 2^31 (checked when the context is created)
+    d.i = t32 % g.mx;
+    const int r = t32 / g.mx;
     d.j = r % g.my;
     d.k = r / g.my;
     return true;
