@@ -824,13 +824,19 @@ __global__ void __launch_bounds__(RED_THREADS) k_finish_gated(const double *__re
 }
 
 // y = x / sqrt(ww) (0 when ww <= 0), ww read from device memory
+// y = x / sqrt(ww[0]) (0 when ww[0] <= 0); y32, when given, receives y as
+// FP32 (the next FP32 V-cycle's right-hand side)
 __global__ void k_scale_rnorm(long long n, const double *__restrict__ ww, const double *__restrict__ x,
-                              double *__restrict__ y)
+                              double *__restrict__ y, float *__restrict__ y32)
 {
     long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
     const double hn = sqrt(ww[0] > 0 ? ww[0] : 0);
     const double s = hn > 0 ? 1.0 / hn : 0.0;
-    if (t < n) y[t] = s * x[t];
+    if (t < n) {
+        const double v = s * x[t];
+        y[t] = v;
+        if (y32) y32[t] = (float)v;
+    }
 }
 
 constexpr int MAXV = 33;
@@ -1461,7 +1467,8 @@ static void vcycle(vfs_ctx *c, vfs_poisson *ps, int l, int pre, int post, int co
 // With z32 the FP32 V-cycle's finest iterate is z32 itself (no FP64 copy);
 // the FP64 V-cycle iterates in z directly.  The finest level's kernels take
 // the iterate pointer at launch (the captured coarse levels never see it).
-static int precondition(vfs_ctx *c, const vfs_solve_opts *o, const double *v, double *z, float *z32 = nullptr)
+static int precondition(vfs_ctx *c, const vfs_solve_opts *o, const double *v, double *z, float *z32 = nullptr,
+                        bool b32_ready = false)
 {
     vfs_poisson *ps = c->ps;
     hipStream_t st = c->stream;
@@ -1469,7 +1476,7 @@ static int precondition(vfs_ctx *c, const vfs_solve_opts *o, const double *v, do
     const long long n = L.g.n;
     const unsigned nb = blocks_for(n, PTPB);
     if (o->mg_fp32) {
-        hipLaunchKernelGGL((k_convert<double, float>), dim3(nb), dim3(PTPB), 0, st, n, v, L.b32);
+        if (!b32_ready) hipLaunchKernelGGL((k_convert<double, float>), dim3(nb), dim3(PTPB), 0, st, n, v, L.b32);
         float *x0 = L.x32;
         if (z32) L.x32 = z32;
         hipLaunchKernelGGL(k_zero<float>, dim3(nb), dim3(PTPB), 0, st, n, L.x32);
@@ -1529,7 +1536,7 @@ static void launch_multiaxpy(vfs_ctx *c, int nv, const double *V, double *w, con
 // at [0..nv], [40..40+nv], [80].
 constexpr int GS_D2 = 40, GS_D3 = 80;
 static_assert(GS_D2 >= MAXV && GS_D3 >= GS_D2 + MAXV, "Gram-Schmidt result slots");
-static int gs_step(vfs_ctx *c, int nv, const double *V, double *w, double *vnext, double *out)
+static int gs_step(vfs_ctx *c, int nv, const double *V, double *w, double *vnext, double *out, float *vnext32)
 {
     hipStream_t st = c->stream;
     const long long n = c->g.n;
@@ -1545,7 +1552,7 @@ static int gs_step(vfs_ctx *c, int nv, const double *V, double *w, double *vnext
     launch_multiaxpy<false>(c, nv, V, w, R + GS_D2, R + GS_D2 + nv, R + nv);
     hipLaunchKernelGGL(k_finish_gated, dim3(1), dim3(RED_THREADS), 0, st, c->red, RED_BLOCKS, R + GS_D2 + nv, R + nv,
                        R + GS_D3);
-    hipLaunchKernelGGL(k_scale_rnorm, dim3(blocks_for(n, PTPB)), dim3(PTPB), 0, st, n, R + GS_D3, w, vnext);
+    hipLaunchKernelGGL(k_scale_rnorm, dim3(blocks_for(n, PTPB)), dim3(PTPB), 0, st, n, R + GS_D3, w, vnext, vnext32);
     VFS_LAUNCH_CHECK("gram-schmidt");
     VFS_HIP(hipMemcpyAsync(c->red_host, R, (GS_D3 + 1) * sizeof(double), hipMemcpyDeviceToHost, st));
     VFS_HIP(hipStreamSynchronize(st));
@@ -1651,7 +1658,8 @@ static int fgmres(vfs_ctx *c, const vfs_solve_opts *o, vfs_unet *net, vfs_solve_
         for (jj = 0; jj < m && it < maxit; jj++, it++) {
             double *vj = V + (size_t)jj * n, *zj = Z + (size_t)jj * n;
             float *zj32 = Z32 + (size_t)jj * n;
-            rc = net && it == 0 ? vfs_unet_forward(net, vj, zj) : precondition(c, o, vj, zj, z32 ? zj32 : nullptr);
+            rc = net && it == 0 ? vfs_unet_forward(net, vj, zj)
+                                : precondition(c, o, vj, zj, z32 ? zj32 : nullptr, z32 && jj > 0);
             if (rc) return rc;
             prof_begin(c, K_APPLY64);
             if (z32)
@@ -1663,7 +1671,9 @@ static int fgmres(vfs_ctx *c, const vfs_solve_opts *o, vfs_unet *net, vfs_solve_
             // classical Gram-Schmidt, re-orthogonalised when the norm drops;
             // the first pass also forms the dot products of the second
             double gs[GS_D3 + 1];
-            rc = gs_step(c, jj + 1, V, w, V + (size_t)(jj + 1) * n, gs);
+            // with FP32 V-cycles the new basis vector is also written as the
+            // next V-cycle's FP32 right-hand side
+            rc = gs_step(c, jj + 1, V, w, V + (size_t)(jj + 1) * n, gs, z32 ? L.b32 : nullptr);
             if (rc) return rc;
             double h[MAXV];
             for (int v = 0; v <= jj; v++) h[v] = gs[v];
