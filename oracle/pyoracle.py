@@ -25,6 +25,7 @@ class OrcParams(C.Structure):
         ("ren", C.c_double), ("dt", C.c_double), ("st", C.c_double), ("cs2", C.c_double),
         ("inlet_flux", C.c_double), ("mean_k_area", C.c_double), ("mean_k_flux", C.c_double),
         ("poisson_threshold", C.c_double), ("flux_in_sum", C.c_double),
+        ("max_cs", C.c_double), ("i_homo", C.c_int), ("j_homo", C.c_int), ("k_homo", C.c_int),
     ]
 
 
@@ -72,6 +73,8 @@ def load() -> C.CDLL:
         "orc_contra2cart": ([P, M, D, D, D], None),
         "orc_form_bcs": ([P, M, D, D, D, C.c_int], None),
         "orc_eddy_viscosity": ([P, M, D, D], None),
+        "orc_eddy_viscosity_cs": ([P, M, D, D, D], None),
+        "orc_smagorinsky_constant": ([P, M, D, C.c_int, C.c_int, C.c_int, D], None),
         "orc_calc_k_flux": ([P, M, D], C.c_double),
         "orc_mean_k_area": ([P, M], C.c_double),
         "orc_pressure_gradient": ([P, M, D, D], None),
@@ -120,7 +123,7 @@ def _ptr(a: np.ndarray):
 
 def make_params(mx, my, mz, *, per=(1, 0, 1), bctype=(100, 100, 1, 1, 100, 100), les=1, second_order=1,
                 laplacian=1, ren=2800.0, dt=1e-3, st=1.0, cs2=0.01, inlet_flux=-1.0, poisson_threshold=0.1,
-                flux_in_sum=0.0) -> OrcParams:
+                flux_in_sum=0.0, max_cs=0.5, homo=(0, 0, 0)) -> OrcParams:
     p = OrcParams()
     p.mx, p.my, p.mz = mx, my, mz
     p.per_i, p.per_j, p.per_k = per
@@ -130,6 +133,8 @@ def make_params(mx, my, mz, *, per=(1, 0, 1), bctype=(100, 100, 1, 1, 100, 100),
     p.ren, p.dt, p.st, p.cs2 = ren, dt, st, cs2
     p.inlet_flux, p.poisson_threshold, p.flux_in_sum = inlet_flux, poisson_threshold, flux_in_sum
     p.mean_k_area, p.mean_k_flux = 0.0, 0.0
+    p.max_cs = max_cs
+    p.i_homo, p.j_homo, p.k_homo = (int(h) for h in homo)
     return p
 
 
@@ -181,8 +186,20 @@ class Oracle:
         self.lib.orc_form_bcs(C.byref(self.p), C.byref(self.cm), self._f("ucont"), self._f("ucat"),
                               self._f("ucat_old"), int(outflow_scale))
 
-    def eddy_viscosity(self):
-        self.lib.orc_eddy_viscosity(C.byref(self.p), C.byref(self.cm), self._f("ucat"), self._f("nu_t"))
+    def eddy_viscosity(self, cs: np.ndarray | None = None):
+        if cs is None:
+            self.lib.orc_eddy_viscosity(C.byref(self.p), C.byref(self.cm), self._f("ucat"), self._f("nu_t"))
+        else:
+            cs = np.ascontiguousarray(cs, dtype=np.float64)
+            self.lib.orc_eddy_viscosity_cs(C.byref(self.p), C.byref(self.cm), self._f("ucat"), _ptr(cs),
+                                           self._f("nu_t"))
+
+    def smagorinsky_constant(self, ti=2, tistart=0, rstart=0) -> np.ndarray:
+        """Compute_Smagorinsky_Constant_1 on the current Ucat: lCs per point."""
+        cs = np.zeros(self.shape)
+        self.lib.orc_smagorinsky_constant(C.byref(self.p), C.byref(self.cm), self._f("ucat"), int(ti), int(tistart),
+                                          int(rstart), _ptr(cs))
+        return cs
 
     def pressure_gradient(self):
         self.lib.orc_pressure_gradient(C.byref(self.p), C.byref(self.cm), self._f("p"), self._f("dp"))

@@ -399,8 +399,8 @@ void orc_form_bcs(const orc_params *p, const orc_metrics *m, double *ucont, doub
 
 static const double SOLID = 0.1; /* k-omega.c:19 */
 
-static void du_center(const orc_params *p, const double *nvert, const double *ucat, int k, int j,
-                      int i, double d[3][3])
+void orc_du_center(const orc_params *p, const double *nvert, const double *ucat, int k, int j, int i,
+                   double d[3][3])
 {
     /* d[c][0] = d u_c / d csi, d[c][1] = d/d eta, d[c][2] = d/d zeta */
     const int mx = p->mx, my = p->my, mz = p->mz;
@@ -429,6 +429,12 @@ static void du_center(const orc_params *p, const double *nvert, const double *uc
 
 void orc_eddy_viscosity(const orc_params *p, const orc_metrics *m, const double *ucat, double *nu_t)
 {
+    orc_eddy_viscosity_cs(p, m, ucat, NULL, nu_t);
+}
+
+void orc_eddy_viscosity_cs(const orc_params *p, const orc_metrics *m, const double *ucat, const double *lcs,
+                           double *nu_t)
+{
     const int mx = p->mx, my = p->my, mz = p->mz;
     memset(nu_t, 0, npts(p) * sizeof(double));
     if (!p->les) return;
@@ -440,7 +446,7 @@ void orc_eddy_viscosity(const orc_params *p, const orc_metrics *m, const double 
                 double ajc = m->aj[q];
                 const double *cs = &m->csi[3 * q], *et = &m->eta[3 * q], *ze = &m->zet[3 * q];
                 double d[3][3];
-                du_center(p, m->nvert, ucat, k, j, i, d);
+                orc_du_center(p, m->nvert, ucat, k, j, i, d);
                 /* Compute_du_dxyz: du_c/dx_r = (d[c][0]*csi_r + d[c][1]*eta_r + d[c][2]*zet_r)*aj */
                 double g[3][3];
                 for (int c = 0; c < 3; c++)
@@ -452,7 +458,7 @@ void orc_eddy_viscosity(const orc_params *p, const orc_metrics *m, const double 
                 double Sabs = sqrt(2.0 * (Sxx * Sxx + Sxy * Sxy + Sxz * Sxz + Syx * Syx + Syy * Syy + Syz * Syz +
                                           Szx * Szx + Szy * Szy + Szz * Szz));
                 double filter = pow(1. / ajc, 1. / 3.);
-                S(nu_t, k, j, i) = p->cs2 * pow(filter, 2.0) * Sabs;
+                S(nu_t, k, j, i) = (lcs ? lcs[q] : p->cs2) * pow(filter, 2.0) * Sabs;
             }
     orc_periodic_fix(p, nu_t, 1);
 }

@@ -27,7 +27,7 @@ typedef struct {
     int mx, my, mz;                 /* DA sizes = cells + 2 (IM+1, JM+1, KM+1) */
     int per_i, per_j, per_k;        /* ii_/jj_/kk_periodic (main.c:283-288)    */
     int bctype[6];                  /* bcs.dat: i-, i+, j-, j+, k-, k+          */
-    int les;                        /* 0 or 1 (constant Smagorinsky)            */
+    int les;                        /* 0, 1 (constant) or 2 (dynamic Smagorinsky)*/
     int second_order;               /* main.c:1408                              */
     int laplacian;                  /* main.c:277                               */
     double ren, dt, st;             /* Reynolds number, time step, Strouhal     */
@@ -37,6 +37,8 @@ typedef struct {
     double mean_k_flux;             /* Calc_k_Flux, rhs2.c:1549-1633            */
     double poisson_threshold;       /* main.c:124, 0.1                          */
     double flux_in_sum;             /* FluxInSum from InflowFlux (outlet scaling)*/
+    double max_cs;                  /* -max_cs (main.c:308, 1728), les == 2      */
+    int i_homo, j_homo, k_homo;     /* -i/j/k_homo_filter (main.c:324-326)       */
 } orc_params;
 
 /* Metric set as the momentum/Poisson kernels read it (lCsi ... lKAj after the
@@ -68,6 +70,18 @@ void orc_form_bcs(const orc_params *p, const orc_metrics *m, double *ucont,
 /* a4: Compute_eddy_viscosity_LES (les.c:1143-1361) with constant lCs. */
 void orc_eddy_viscosity(const orc_params *p, const orc_metrics *m,
                         const double *ucat, double *nu_t);
+/* The same with a per-point lCs (cs == NULL: the constant p->cs2). */
+void orc_eddy_viscosity_cs(const orc_params *p, const orc_metrics *m,
+                           const double *ucat, const double *cs, double *nu_t);
+
+/* a3 / f1: Compute_Smagorinsky_Constant_1 (les.c:75-1141) into lCs: 0 while
+ * ti < 2 on a fresh start (T1), p->cs2 for les == 1, the dynamic model for
+ * les == 2.  ucat: the local Ucat with its ghost cells. */
+void orc_smagorinsky_constant(const orc_params *p, const orc_metrics *m, const double *ucat,
+                              int ti, int tistart, int rstart, double *cs);
+/* Compute_du_center (k-omega.c:313-431): d[c][0..2] = d u_c / d(csi, eta, zeta). */
+void orc_du_center(const orc_params *p, const double *nvert, const double *ucat, int k, int j, int i,
+                   double d[3][3]);
 
 /* Calc_k_Flux (rhs2.c:1549-1633): mean streamwise flux per k plane. */
 double orc_calc_k_flux(const orc_params *p, const orc_metrics *m, const double *ucont);

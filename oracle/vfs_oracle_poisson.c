@@ -891,7 +891,15 @@ int orc_fractional_step(orc_params *p, const orc_metrics *m, double *ucont, doub
     /* LES, solvers.c:365-371 */
     if (p->les) {
         orc_contra2cart(p, m, ucont, ucat, ucat_old);
-        orc_eddy_viscosity(p, m, ucat, nu_t);
+        if (p->les == 2) {
+            /* dynamic_freq 1, after the first steps (ti >= 2) */
+            double *cs = dalloc(n);
+            orc_smagorinsky_constant(p, m, ucat, 2, 0, 0, cs);
+            orc_eddy_viscosity_cs(p, m, ucat, cs, nu_t);
+            free(cs);
+        } else {
+            orc_eddy_viscosity(p, m, ucat, nu_t);
+        }
     }
     /* solvers.c:438 */
     orc_pressure_gradient(p, m, pres, dp);
