@@ -39,7 +39,7 @@ typedef struct {
     int mx, my, mz;            /* DA sizes: IM+1, JM+1, KM+1 (init.c:157)       */
     int per_i, per_j, per_k;   /* -ii_periodic, -jj_periodic, -kk_periodic      */
     int bctype[6];             /* bcs.dat codes: i-, i+, j-, j+, k-, k+         */
-    int les;                   /* -les: 0 or 1 (constant Smagorinsky)          */
+    int les;                   /* -les: 0, 1 (constant) or 2 (dynamic Smagorinsky) */
     int second_order;          /* -second_order                                */
     int laplacian;             /* -laplacian                                   */
     double ren, dt, st;        /* -ren, -dt, Strouhal (init.c:59-72)           */
@@ -47,6 +47,10 @@ typedef struct {
     double inlet_flux;         /* -flux; < 0 when unset                        */
     double poisson_threshold;  /* poisson_threshold, main.c:124 (0.1)          */
     double flux_in_sum;        /* FluxInSum of InflowFlux (outlet balance)     */
+    double max_cs;             /* -max_cs: upper clip of the dynamic lCs (0.5) */
+    int dynamic_freq;          /* -dynamic_freq: steps between dynamic updates (1) */
+    int i_homo_filter, j_homo_filter, k_homo_filter;  /* -i/j/k_homo_filter: average
+                                  LM, MM over homogeneous directions (les.c:799-958) */
 } vfs_params;
 
 /* Device-resident arrays of a context. */
@@ -70,6 +74,7 @@ typedef enum {
     VFS_FEUL,       /* turbine body force on faces (lF_eul)              */
     VFS_UCONT_O,    /* Ucont at the start of the step (Ucont_o)          */
     VFS_RHS_O,      /* Formfunction_2 of Ucont_o (RHS_o, solvers.c:627)   */
+    VFS_CS,         /* Smagorinsky coefficient lCs (= Cs^2) per point     */
     VFS_NFIELDS
 } vfs_field;
 
@@ -102,7 +107,13 @@ int vfs_form_bcs(vfs_ctx *ctx, int outflow_scale);
 /* InflowFlux (bcs.c:247-626) for a uniform inlet (-inlet 1, code 5 on the k- face):
  * inlet fluxes and ghost velocities; FluxInSum returned and kept for FormBCS. */
 int vfs_inflow_flux(vfs_ctx *ctx, double *flux_in);
-/* Compute_eddy_viscosity_LES (les.c:1143-1361) with lCs = params.cs2. */
+/* Compute_Smagorinsky_Constant_1 (les.c:75-1141) from VFS_UCAT into lCs:
+ * 0 while ti < 2 on a fresh start (tistart == 0 and no restart), params.cs2
+ * (0.01) for les == 1, the dynamic model for les == 2 (3-D Simpson test
+ * filter, homogeneous averaging per the *_homo_filter flags, clipped to
+ * [0, max_cs]).  Until it is first called lCs holds params.cs2. */
+int vfs_smagorinsky_constant(vfs_ctx *ctx, int ti, int tistart, int rstart_flg);
+/* Compute_eddy_viscosity_LES (les.c:1143-1361) with the current lCs. */
 int vfs_eddy_viscosity(vfs_ctx *ctx);
 /* Calc_k_Flux (rhs2.c:1549-1633): mean_k_flux, stored in the context. */
 int vfs_calc_k_flux(vfs_ctx *ctx, double *mean_k_flux);
@@ -184,6 +195,12 @@ typedef struct {
     int newton_iterations, linear_iterations;  /* implicit step only */
 } vfs_step_stats;
 
+/* The time-step index the step drivers pass to Compute_Smagorinsky_Constant_1
+ * (the reference's globals ti, tistart, rstart_flg).  Each step calls it when
+ * ti % dynamic_freq == 0 or ti == tistart, then advances ti by one.  A new
+ * context starts at ti = tistart = 0 with rstart_flg = 1 (the first-steps
+ * zero coefficient off). */
+int vfs_set_time_index(vfs_ctx *ctx, int ti, int tistart, int rstart_flg);
 /* Flow_Solver (solvers.c:313-759) for -imp 0: k-flux, LES, Pressure_Gradient,
  * RungeKutta, Poisson, UpdatePressure, Projection, Divergence. */
 int vfs_fractional_step(vfs_ctx *ctx, const vfs_solve_opts *opts, vfs_step_stats *stats);

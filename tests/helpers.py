@@ -21,12 +21,13 @@ def relerr(a: np.ndarray, b: np.ndarray) -> float:
     return num / den
 
 
-def channel_pair(nx=32, ny=24, nz=32, les=1, second_order=1, laplacian=1, with_gpu=True, seed=cases.SEED):
+def channel_pair(nx=32, ny=24, nz=32, les=1, second_order=1, laplacian=1, with_gpu=True, seed=cases.SEED,
+                 max_cs=0.5, homo=(0, 0, 0)):
     """Oracle and (optionally) GPU context on the same channel case, metrics
     formed, initial Ucat from Contra2Cart, dt = 0.3 dt_CFL1."""
     c = cases.channel(nx, ny, nz, seed=seed)
     op = po.make_params(*c.da, per=c.per, bctype=c.bctype, ren=c.ren, dt=1e-3, inlet_flux=c.inlet_flux,
-                        les=les, second_order=second_order, laplacian=laplacian)
+                        les=les, second_order=second_order, laplacian=laplacian, max_cs=max_cs, homo=homo)
     o = po.Oracle(op, c.coor)
     o.set("ucont", c.ucont)
     o.contra2cart()
@@ -36,7 +37,8 @@ def channel_pair(nx=32, ny=24, nz=32, les=1, second_order=1, laplacian=1, with_g
     o.mean_k_area()
     ctx = None
     if with_gpu:
-        gp = cases.params_for(vfs, c, dt, les=les, second_order=second_order, laplacian=laplacian)
+        gp = cases.params_for(vfs, c, dt, les=les, second_order=second_order, laplacian=laplacian, max_cs=max_cs,
+                              homo=homo)
         ctx = vfs.Context(gp)
         ctx.upload("COOR", c.coor)
         ctx.form_metrics()

@@ -1,0 +1,108 @@
+"""Parity of Compute_Smagorinsky_Constant_1 (les.c:75-1141) on the GPU with
+the oracle: the dynamic coefficient (les 2) with the 3-D Simpson test filter
+and with each homogeneous average, blanked and near-wall cells, the eddy
+viscosity it feeds, the first-steps zero coefficient, and whole steps.
+
+Tolerances: the per-cell passes keep the reference's operation order, so Cs
+agrees to 1e-12 of max|Cs|; the homogeneous averages sum in a different
+(fixed) order than the oracle's sequential loops: 1e-10.
+"""
+import numpy as np
+import pytest
+
+from helpers import cases, channel_pair, relerr, vfs
+import pyoracle as po
+
+pytestmark = pytest.mark.gpu
+
+
+def box_pair(nx=20, ny=16, nz=24, max_cs=0.5, homo=(0, 0, 0)):
+    """The slip/inlet/outlet box (no periodic direction), les 2."""
+    c = cases.box_flow(nx, ny, nz)
+    op = po.make_params(*c.da, per=c.per, bctype=c.bctype, ren=c.ren, dt=1e-3, les=2, max_cs=max_cs, homo=homo)
+    o = po.Oracle(op, c.coor)
+    o.set("ucont", c.ucont)
+    o.contra2cart()
+    o.set("ucat_old", o["ucat"])
+    ctx = vfs.Context(cases.params_for(vfs, c, 1e-3, les=2, max_cs=max_cs, homo=homo))
+    ctx.upload("COOR", c.coor)
+    ctx.form_metrics()
+    ctx.upload("UCONT", c.ucont)
+    ctx.contra2cart()
+    ctx.upload("UCAT_OLD", ctx.download("UCAT"))
+    return c, o, ctx
+
+
+HOMO = {"none": (0, 0, 0), "ik": (1, 0, 1), "i": (1, 0, 0), "j": (0, 1, 0), "k": (0, 0, 1)}
+
+
+@pytest.mark.parametrize("homo", list(HOMO))
+@pytest.mark.parametrize("case", ["channel", "box"])
+def test_dynamic_coefficient(case, homo):
+    h = HOMO[homo]
+    if case == "channel":
+        c, o, ctx = channel_pair(24, 16, 20, les=2, max_cs=0.2, homo=h)
+    else:
+        c, o, ctx = box_pair(max_cs=0.2, homo=h)
+    with ctx:
+        assert relerr(ctx.download("UCAT"), o["ucat"]) <= 1e-13
+        ref = o.smagorinsky_constant(ti=5)
+        ctx.smagorinsky_constant(ti=5)
+        got = ctx.download("CS")
+        tol = 1e-12 if homo == "none" else 1e-10
+        assert np.abs(ref).max() > 1e-4 and (ref > 0).mean() > 0.1
+        assert relerr(got, ref) <= tol
+        o.eddy_viscosity(ref)
+        ctx.eddy_viscosity()
+        assert relerr(ctx.download("NU_T"), o["nu_t"]) <= 1e-11
+
+
+def test_dynamic_coefficient_blanked():
+    """Solid cells (nvert 3) get Cs = 0, cells next to them (nvert 1) at
+    least wall_cs; the filters drop the blanked neighbours."""
+    c, o, ctx = channel_pair(24, 16, 20, les=2, max_cs=0.2)
+    with ctx:
+        mz, my, mx = o.shape
+        nv = np.zeros(o.shape)
+        nv[8:12, 5:9, 6:11] = 1.0
+        nv[9:11, 6:8, 7:10] = 3.0
+        o.m["nvert"][...] = nv
+        ctx.upload("NVERT", nv)
+        ref = o.smagorinsky_constant(ti=5)
+        ctx.smagorinsky_constant(ti=5)
+        got = ctx.download("CS")
+        assert np.all(ref[nv > 1.1] == 0) and np.all(ref[(nv > 0.1) & (nv < 1.1)] >= 0.001)
+        assert relerr(got, ref) <= 1e-12
+
+
+def test_first_steps_and_constant_modes():
+    c, o, ctx = channel_pair(16, 12, 16, les=2)
+    with ctx:
+        ctx.smagorinsky_constant(ti=0, tistart=0, rstart=0)  # les.c:77-80
+        assert np.all(ctx.download("CS") == 0)
+        ctx.eddy_viscosity()
+        assert np.all(ctx.download("NU_T") == 0)
+        ctx.smagorinsky_constant(ti=1, tistart=0, rstart=1)  # a restart: the dynamic value
+        assert np.abs(ctx.download("CS")).max() > 0
+    c, o, ctx = channel_pair(16, 12, 16, les=1)
+    with ctx:
+        ctx.smagorinsky_constant(ti=3)
+        assert np.all(ctx.download("CS") == 0.01)
+        o.eddy_viscosity()
+        ctx.eddy_viscosity()
+        assert relerr(ctx.download("NU_T"), o["nu_t"]) <= 1e-12
+
+
+@pytest.mark.parametrize("homo", ["none", "ik"])
+def test_fractional_steps_les2(homo):
+    """Two full steps with the dynamic model against the oracle's steps (which
+    recompute Cs every step, as dynamic_freq 1 after the first steps)."""
+    c, o, ctx = channel_pair(16, 12, 16, les=2, max_cs=0.2, homo=HOMO[homo])
+    with ctx:
+        ctx.set_time_index(2, 0, 0)
+        for s in range(2):
+            st = ctx.fractional_step(vfs.default_solve_opts(tol=1e-10))
+            o.fractional_step(tol=1e-10)
+            assert st.poisson_relres <= 1e-10
+        err = relerr(ctx.download("UCONT"), o["ucont"])
+        assert err <= 1e-8, err

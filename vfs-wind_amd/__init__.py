@@ -25,7 +25,7 @@ VFS_OK, VFS_EINVAL, VFS_EHIP, VFS_ENOMEM, VFS_ENOCONV = 0, 1, 2, 3, 4
 FIELDS = [
     "COOR", "NVERT", "CSI", "ETA", "ZET", "AJ", "ICSI", "IETA", "IZET", "IAJ",
     "JCSI", "JETA", "JZET", "JAJ", "KCSI", "KETA", "KZET", "KAJ", "CENT",
-    "UCONT", "UCAT", "UCAT_OLD", "NU_T", "P", "PHI", "DP", "RHS", "B", "FEUL", "UCONT_O", "RHS_O",
+    "UCONT", "UCAT", "UCAT_OLD", "NU_T", "P", "PHI", "DP", "RHS", "B", "FEUL", "UCONT_O", "RHS_O", "CS",
 ]
 FIELD = {name: i for i, name in enumerate(FIELDS)}
 VECTOR_FIELDS = {"COOR", "CSI", "ETA", "ZET", "ICSI", "IETA", "IZET", "JCSI", "JETA", "JZET",
@@ -46,6 +46,8 @@ class Params(C.Structure):
         ("ren", C.c_double), ("dt", C.c_double), ("st", C.c_double),
         ("cs2", C.c_double), ("inlet_flux", C.c_double),
         ("poisson_threshold", C.c_double), ("flux_in_sum", C.c_double),
+        ("max_cs", C.c_double), ("dynamic_freq", C.c_int),
+        ("i_homo_filter", C.c_int), ("j_homo_filter", C.c_int), ("k_homo_filter", C.c_int),
     ]
 
 
@@ -126,6 +128,8 @@ _SIGS = {
     "vfs_contra2cart": ([C.c_void_p], C.c_int),
     "vfs_form_bcs": ([C.c_void_p, C.c_int], C.c_int),
     "vfs_eddy_viscosity": ([C.c_void_p], C.c_int),
+    "vfs_smagorinsky_constant": ([C.c_void_p, C.c_int, C.c_int, C.c_int], C.c_int),
+    "vfs_set_time_index": ([C.c_void_p, C.c_int, C.c_int, C.c_int], C.c_int),
     "vfs_inflow_flux": ([C.c_void_p, C.POINTER(C.c_double)], C.c_int),
     "vfs_calc_k_flux": ([C.c_void_p, C.POINTER(C.c_double)], C.c_int),
     "vfs_pressure_gradient": ([C.c_void_p], C.c_int),
@@ -217,7 +221,7 @@ def default_newton_opts(rtol=1e-4, restart=30, max_newton=50, max_linear=1000, u
 
 def make_params(mx, my, mz, *, per=(1, 0, 1), bctype=(100, 100, 1, 1, 100, 100), les=1, second_order=1,
                 laplacian=1, ren=2800.0, dt=1e-3, st=1.0, cs2=0.01, inlet_flux=-1.0,
-                poisson_threshold=0.1, flux_in_sum=0.0) -> Params:
+                poisson_threshold=0.1, flux_in_sum=0.0, max_cs=0.5, dynamic_freq=1, homo=(0, 0, 0)) -> Params:
     p = Params()
     p.mx, p.my, p.mz = mx, my, mz
     p.per_i, p.per_j, p.per_k = per
@@ -226,6 +230,8 @@ def make_params(mx, my, mz, *, per=(1, 0, 1), bctype=(100, 100, 1, 1, 100, 100),
     p.les, p.second_order, p.laplacian = les, second_order, laplacian
     p.ren, p.dt, p.st, p.cs2 = ren, dt, st, cs2
     p.inlet_flux, p.poisson_threshold, p.flux_in_sum = inlet_flux, poisson_threshold, flux_in_sum
+    p.max_cs, p.dynamic_freq = max_cs, dynamic_freq
+    p.i_homo_filter, p.j_homo_filter, p.k_homo_filter = (int(h) for h in homo)
     return p
 
 
@@ -317,6 +323,12 @@ class Context:
 
     def eddy_viscosity(self):
         self._check(self.lib.vfs_eddy_viscosity(self.h))
+
+    def smagorinsky_constant(self, ti: int = 2, tistart: int = 0, rstart: int = 0):
+        self._check(self.lib.vfs_smagorinsky_constant(self.h, int(ti), int(tistart), int(rstart)))
+
+    def set_time_index(self, ti: int, tistart: int = 0, rstart: int = 0):
+        self._check(self.lib.vfs_set_time_index(self.h, int(ti), int(tistart), int(rstart)))
 
     def calc_k_flux(self) -> float:
         v = C.c_double()

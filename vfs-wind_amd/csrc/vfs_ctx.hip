@@ -38,8 +38,10 @@ int validate(const vfs_params *p)
     // point kernels decode (i, j, k) from a point index in 32-bit arithmetic
     if ((long long)p->mx * p->my * p->mz >= (1LL << 31))
         return vfs::set_error(VFS_EINVAL, "grid too large: the DA must hold fewer than 2^31 points");
-    if (p->les != 0 && p->les != 1)
-        return vfs::set_error(VFS_EINVAL, "only -les 0 and -les 1 are supported");
+    if (p->les != 0 && p->les != 1 && p->les != 2)
+        return vfs::set_error(VFS_EINVAL, "only -les 0, 1 and 2 are supported");
+    if (p->les == 2 && !(p->max_cs >= 0))
+        return vfs::set_error(VFS_EINVAL, "max_cs must be >= 0 with -les 2");
     if (p->ren <= 0 || p->dt <= 0)
         return vfs::set_error(VFS_EINVAL, "ren and dt must be positive");
     if (p->per_j)
@@ -126,6 +128,11 @@ int vfs_ctx_create(const vfs_params *params, vfs_ctx **out)
     c->g = vfs::make_grid(*params);
     c->face = c->pucont = c->snap = c->fpc = c->red = c->red_host = nullptr;
     c->ps = nullptr;
+    c->cs_mode = vfs::CS_PARAM;
+    c->dyn = nullptr;
+    c->ti = 0;
+    c->tistart = 0;
+    c->rstart = 1;
     c->mean_k_area = 0;
     c->mean_k_flux = 0;
     c->face_derived = false;
@@ -180,6 +187,7 @@ int vfs_ctx_destroy(vfs_ctx *c)
     if (c->red) hipFree(c->red);
     if (c->red_host) hipHostFree(c->red_host);
     if (c->imp_ws) hipFree(c->imp_ws);
+    if (c->dyn) hipFree(c->dyn);
     if (c->ev_pool) {
         for (hipEvent_t ev : *c->ev_pool) hipEventDestroy(ev);
         delete c->ev_pool;
@@ -224,6 +232,7 @@ int vfs_upload(vfs_ctx *c, vfs_field f, const double *host)
     if (vfs::is_metric_field(f)) c->face_derived = false;
     if (vfs::is_c2c_field(f)) vfs::ucat_stale(c);
     if (f == VFS_NVERT) c->nvert_zero = -1;
+    if (f == VFS_CS) c->cs_mode = vfs::CS_FIELD;
     VFS_HIP(hipMemcpyAsync(c->f[f], host, vfs_field_size(c, f) * sizeof(double), hipMemcpyHostToDevice, c->stream));
     VFS_HIP(hipStreamSynchronize(c->stream));
     return VFS_OK;
@@ -232,6 +241,10 @@ int vfs_upload(vfs_ctx *c, vfs_field f, const double *host)
 int vfs_download(vfs_ctx *c, vfs_field f, double *host)
 {
     if (!c || !host || f < 0 || f >= VFS_NFIELDS) return vfs::set_error(VFS_EINVAL, "bad download argument");
+    if (f == VFS_CS) {
+        int rc = vfs::materialize_cs(c);
+        if (rc) return rc;
+    }
     VFS_HIP(hipMemcpyAsync(host, c->f[f], vfs_field_size(c, f) * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     VFS_HIP(hipStreamSynchronize(c->stream));
     return VFS_OK;
@@ -243,6 +256,7 @@ double *vfs_field_device_ptr(vfs_ctx *c, vfs_field f)
     if (vfs::is_metric_field(f)) c->face_derived = false;  // the caller may write through it
     if (vfs::is_c2c_field(f)) vfs::ucat_stale(c);
     if (f == VFS_NVERT) c->nvert_zero = -1;
+    if (f == VFS_CS && vfs::materialize_cs(c)) return nullptr;
     return c->f[f];
 }
 
@@ -292,6 +306,21 @@ int vfs_form_bcs(vfs_ctx *c, int outflow_scale)
 {
     if (!c) return vfs::set_error(VFS_EINVAL, "null context");
     return vfs::form_bcs(c, outflow_scale);
+}
+
+int vfs_smagorinsky_constant(vfs_ctx *c, int ti, int tistart, int rstart_flg)
+{
+    if (!c) return vfs::set_error(VFS_EINVAL, "null context");
+    return vfs::smagorinsky_constant(c, ti, tistart, rstart_flg);
+}
+
+int vfs_set_time_index(vfs_ctx *c, int ti, int tistart, int rstart_flg)
+{
+    if (!c) return vfs::set_error(VFS_EINVAL, "null context");
+    c->ti = ti;
+    c->tistart = tistart;
+    c->rstart = rstart_flg;
+    return VFS_OK;
 }
 
 int vfs_eddy_viscosity(vfs_ctx *c)
@@ -452,6 +481,9 @@ static int step_common(vfs_ctx *c, const vfs_solve_opts *o, const vfs_newton_opt
     auto t0 = clk::now();
     int rc = vfs_calc_k_flux(c, nullptr);                   // solvers.c:363
     if (!rc && c->p.les) rc = vfs::c2c(c);                  // solvers.c:368
+    const int freq = c->p.dynamic_freq < 1 ? 1 : c->p.dynamic_freq;  // main.c:1439
+    if (!rc && c->p.les && (c->ti % freq == 0 || c->ti == c->tistart))
+        rc = vfs::smagorinsky_constant(c, c->ti, c->tistart, c->rstart);  // solvers.c:369
     if (!rc && c->p.les) rc = vfs::eddy_viscosity(c);       // solvers.c:370
     if (!rc) rc = vfs_pressure_gradient(c);                 // solvers.c:438
     vfs_newton_stats ns;
@@ -480,6 +512,7 @@ static int step_common(vfs_ctx *c, const vfs_solve_opts *o, const vfs_newton_opt
     if (!rc) rc = vfs::divergence(c, &md);                  // solvers.c:684
     if (rc) return rc;
     auto t3 = clk::now();
+    c->ti++;
     if (st) {
         st->poisson_iterations = ps.iterations;
         st->poisson_relres = ps.relres;

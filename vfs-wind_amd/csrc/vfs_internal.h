@@ -215,6 +215,12 @@ struct vfs_ctx {
     double *imp_ws;
     size_t imp_ws_bytes;
     vfs_poisson *ps;
+    // lCs: which value the eddy-viscosity pass reads (CS_PARAM: params.cs2,
+    // CS_ZERO: 0, CS_FIELD: the VFS_CS array); scratch of the dynamic model
+    int cs_mode;
+    double *dyn;
+    // time-step index of the step drivers (ti, tistart, rstart_flg)
+    int ti, tistart, rstart;
     // per-kernel timing (vfs_profile): event pairs recorded around launches
     int prof_on;
     unsigned prof_mask;                // kernel ids recorded while prof_on (bit id)
@@ -259,6 +265,11 @@ inline void ucat_walls_written(vfs_ctx *c)
     if (c->ucat_fresh || c->ucat_part == UCAT_INTERIOR) c->ucat_part = UCAT_WALLS;
     c->ucat_fresh = false;
 }
+enum { CS_PARAM = 0, CS_ZERO = 1, CS_FIELD = 2 };
+// Compute_Smagorinsky_Constant_1 (les.hip); materialize_cs writes the
+// current lCs into VFS_CS when it is a constant
+int smagorinsky_constant(vfs_ctx *c, int ti, int tistart, int rstart);
+int materialize_cs(vfs_ctx *c);
 int form_bcs(vfs_ctx *c, int outflow_scale);
 int inflow_flux(vfs_ctx *c, double *flux_in);
 int eddy_viscosity(vfs_ctx *c);

@@ -709,9 +709,9 @@ __device__ inline double smagorinsky(double cs2, const double dd[3][3], const do
     return cs2 * pow(filter, 2.0) * Sabs;
 }
 
-__global__ void k_eddy(Grid g, double cs2, const double *__restrict__ ucat, const double *__restrict__ csi,
-                       const double *__restrict__ eta, const double *__restrict__ zet, const double *__restrict__ aj,
-                       const double *__restrict__ nvert, double *__restrict__ nut)
+__global__ void k_eddy(Grid g, double cs2, const double *__restrict__ csf, const double *__restrict__ ucat,
+                       const double *__restrict__ csi, const double *__restrict__ eta, const double *__restrict__ zet,
+                       const double *__restrict__ aj, const double *__restrict__ nvert, double *__restrict__ nut)
 {
     Decode d;
     if (!decode(g, xcd_block1() * (long long)blockDim.x + threadIdx.x, d)) return;
@@ -739,7 +739,7 @@ __global__ void k_eddy(Grid g, double cs2, const double *__restrict__ ucat, cons
     }
 #undef U
 #undef NV
-    nut[q] = smagorinsky(cs2, dd, csi + 3 * q, eta + 3 * q, zet + 3 * q, aj[q]);
+    nut[q] = smagorinsky(csf ? csf[q] : cs2, dd, csi + 3 * q, eta + 3 * q, zet + 3 * q, aj[q]);
 }
 
 // The same per interior point, with one thread per (i, j) of a 64 x 4 tile
@@ -747,9 +747,11 @@ __global__ void k_eddy(Grid g, double cs2, const double *__restrict__ ucat, cons
 // k, k+1 stay in registers, so every value of those arrays leaves HBM once
 // (the thread-per-point form re-read the k +- 1 planes).  The ghost cells
 // get nu_t = 0 from k_zero_ghosts.
-// NB: no point is blanked (every nvert 0): the blanking tests fold away
-template <bool NB>
-__global__ void __launch_bounds__(KM_I * KM_J) k_eddy_km(Grid g, double cs2, const double *__restrict__ ucat,
+// NB: no point is blanked (every nvert 0): the blanking tests fold away;
+// CSF: lCs is the per-point array csf (dynamic model), else the scalar cs2
+template <bool NB, bool CSF>
+__global__ void __launch_bounds__(KM_I * KM_J) k_eddy_km(Grid g, double cs2, const double *__restrict__ csf,
+                                                        const double *__restrict__ ucat,
                                                         const double *__restrict__ csi, const double *__restrict__ eta,
                                                         const double *__restrict__ zet, const double *__restrict__ aj,
                                                         const double *__restrict__ nvert, double *__restrict__ nut)
@@ -785,7 +787,7 @@ __global__ void __launch_bounds__(KM_I * KM_J) k_eddy_km(Grid g, double cs2, con
                 else if (nvm > SOL || (!g.pk && k == 0)) dd[c][2] = (up[c] - u0[c]);
                 else dd[c][2] = (up[c] - um[c]) * 0.5;
             }
-            nut[q] = smagorinsky(cs2, dd, csi + 3 * q, eta + 3 * q, zet + 3 * q, aj[q]);
+            nut[q] = smagorinsky(CSF ? csf[q] : cs2, dd, csi + 3 * q, eta + 3 * q, zet + 3 * q, aj[q]);
         }
         for (int c = 0; c < 3; c++) { um[c] = u0[c]; u0[c] = up[c]; }
         nvm = nv0;
@@ -828,10 +830,14 @@ int eddy_viscosity(vfs_ctx *c)
         VFS_HIP(hipMemsetAsync(c->f[VFS_NU_T], 0, (size_t)g.n * sizeof(double), c->stream));
         return VFS_OK;
     }
+    // lCs: the per-point array of the dynamic model, else one value
+    const bool csf = c->cs_mode == CS_FIELD;
+    const double cs2 = c->cs_mode == CS_ZERO ? 0.0 : c->p.cs2;
+    const double *csp = csf ? c->f[VFS_CS] : nullptr;
     prof_begin(c, K_EDDY);
     const char *e = std::getenv("VFS_EDDY_KM");
     if (e && e[0] == '0') {
-        hipLaunchKernelGGL(k_eddy, dim3(blocks_for(g.n, TPB)), dim3(TPB), 0, c->stream, g, c->p.cs2, c->f[VFS_UCAT],
+        hipLaunchKernelGGL(k_eddy, dim3(blocks_for(g.n, TPB)), dim3(TPB), 0, c->stream, g, cs2, csp, c->f[VFS_UCAT],
                            c->f[VFS_CSI], c->f[VFS_ETA], c->f[VFS_ZET], c->f[VFS_AJ], c->f[VFS_NVERT], c->f[VFS_NU_T]);
     } else {
         int rc = zero_ghosts(c, c->f[VFS_NU_T]);
@@ -839,7 +845,9 @@ int eddy_viscosity(vfs_ctx *c)
         bool nb = false;
         rc = nvert_all_zero(c, &nb);
         if (rc) return rc;
-        hipLaunchKernelGGL(nb ? k_eddy_km<true> : k_eddy_km<false>, dim3(km_blocks(g, KM_K)), dim3(KM_I * KM_J), 0, c->stream, g, c->p.cs2,
+        auto kern = nb ? (csf ? k_eddy_km<true, true> : k_eddy_km<true, false>)
+                       : (csf ? k_eddy_km<false, true> : k_eddy_km<false, false>);
+        hipLaunchKernelGGL(kern, dim3(km_blocks(g, KM_K)), dim3(KM_I * KM_J), 0, c->stream, g, cs2, csp,
                            c->f[VFS_UCAT], c->f[VFS_CSI], c->f[VFS_ETA], c->f[VFS_ZET], c->f[VFS_AJ], c->f[VFS_NVERT],
                            c->f[VFS_NU_T]);
     }
