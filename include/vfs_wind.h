@@ -89,7 +89,15 @@ int vfs_get_params(const vfs_ctx *ctx, vfs_params *params);
 size_t vfs_field_size(const vfs_ctx *ctx, vfs_field f);      /* doubles */
 int vfs_upload(vfs_ctx *ctx, vfs_field f, const double *host);
 int vfs_download(vfs_ctx *ctx, vfs_field f, double *host);
+/* Device address of a field.  The context treats the field as written when
+ * the pointer is handed out; a host that keeps the pointer and writes through
+ * it after a later library call must call vfs_field_written before the next
+ * call that reads the field, or results computed from its old contents
+ * (Ucat, the Poisson operator) are reused. */
 double *vfs_field_device_ptr(vfs_ctx *ctx, vfs_field f);
+/* Declares that f was written outside the library (through a device pointer):
+ * drops every result the context derived from it. */
+int vfs_field_written(vfs_ctx *ctx, vfs_field f);
 int vfs_sync(vfs_ctx *ctx);
 
 /* ---- setup (metrics.c, bcs.c) ------------------------------------------ */

@@ -17,6 +17,7 @@
 void orc_periodic_fix(const orc_params *p, double *f, int nc)
 {
     if (!(p->per_i || p->per_j || p->per_k)) return;
+#pragma omp parallel for schedule(static)
     for (int k = 0; k < p->mz; k++)
         for (int j = 0; j < p->my; j++)
             for (int i = 0; i < p->mx; i++) {
@@ -72,6 +73,7 @@ void orc_form_metrics(const orc_params *p, const double *coor, orc_metrics *m)
     double *csi = m->csi, *eta = m->eta, *zet = m->zet, *aj = m->aj;
 
     /* cell centres, metrics.c:79-100 */
+#pragma omp parallel for schedule(static)
     for (int k = 1; k < mz - 1; k++)
         for (int j = 1; j < my - 1; j++)
             for (int i = 1; i < mx - 1; i++)
@@ -90,6 +92,7 @@ void orc_form_metrics(const orc_params *p, const double *coor, orc_metrics *m)
      * cell-averaged edge vectors, computed beside the Jacobian
      * (metrics.c:193-257), and each face metric is the mean of the two cell
      * metrics it separates (metrics.c:588-593, 696-701, 804-809). */
+#pragma omp parallel for schedule(static)
     for (int k = 1; k < mz - 1; k++)
         for (int j = 1; j < my - 1; j++)
             for (int i = 1; i < mx - 1; i++) {
@@ -146,6 +149,7 @@ void orc_form_metrics(const orc_params *p, const double *coor, orc_metrics *m)
         double *fv[3] = {face_v[3 * axis], face_v[3 * axis + 1], face_v[3 * axis + 2]};
         double *fa = face_s[axis];
         const size_t step = axis == 0 ? 1 : axis == 1 ? (size_t)mx : (size_t)mx * my;
+#pragma omp parallel for schedule(static)
         for (int k = (axis == 2 ? 0 : 1); k < mz - 1; k++)
             for (int j = (axis == 1 ? 0 : 1); j < my - 1; j++)
                 for (int i = (axis == 0 ? 0 : 1); i < mx - 1; i++) {
@@ -234,6 +238,7 @@ void orc_contra2cart(const orc_params *p, const orc_metrics *m, double *ucont, d
     /* rhs.c:129-156 */
     orc_periodic_fix(p, ucont, 3);
     /* rhs.c:158-247 */
+#pragma omp parallel for schedule(static)
     for (int k = 1; k < mz - 1; k++)
         for (int j = 1; j < my - 1; j++)
             for (int i = 1; i < mx - 1; i++) {
@@ -251,6 +256,7 @@ void orc_contra2cart(const orc_params *p, const orc_metrics *m, double *ucont, d
     double *L2 = dalloc(3 * n);
     memcpy(L2, ucat, 3 * n * sizeof(double));
     const int *bc = p->bctype;
+#pragma omp parallel for schedule(static)
     for (int k = 0; k < mz; k++)
         for (int j = 0; j < my; j++)
             for (int i = 0; i < mx; i++) {
@@ -438,6 +444,7 @@ void orc_eddy_viscosity_cs(const orc_params *p, const orc_metrics *m, const doub
     const int mx = p->mx, my = p->my, mz = p->mz;
     memset(nu_t, 0, npts(p) * sizeof(double));
     if (!p->les) return;
+#pragma omp parallel for schedule(static)
     for (int k = 1; k < mz - 1; k++)
         for (int j = 1; j < my - 1; j++)
             for (int i = 1; i < mx - 1; i++) {
@@ -509,6 +516,7 @@ void orc_pressure_gradient(const orc_params *p, const orc_metrics *m, double *pr
     orc_periodic_fix(p, pres, 1); /* momentum.c:249-279 */
     memset(dp, 0, 3 * npts(p) * sizeof(double));
     const int pi_ = p->per_i, pj = p->per_j, pk = p->per_k;
+#pragma omp parallel for schedule(static)
     for (int k = 1; k < mz - 1; k++)
         for (int j = 1; j < my - 1; j++)
             for (int i = 1; i < mx - 1; i++) {

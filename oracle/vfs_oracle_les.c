@@ -134,6 +134,7 @@ void orc_smagorinsky_constant(const orc_params *p, const orc_metrics *m, const d
     /* les.c:258-261: gradients, |S| and the test-filtered velocity */
     double *Ax = dalloc(3 * n), *Ay = dalloc(3 * n), *Az = dalloc(3 * n), *Sabs = dalloc(n), *uf = dalloc(3 * n);
     double *LM = dalloc(n), *MM = dalloc(n);
+#pragma omp parallel for schedule(static)
     for (int k = 1; k < mz - 1; k++)
         for (int j = 1; j < my - 1; j++)
             for (int i = 1; i < mx - 1; i++) {
@@ -158,6 +159,7 @@ void orc_smagorinsky_constant(const orc_params *p, const orc_metrics *m, const d
     static const double coef[3][3][3] = {{{0.125, 0.250, 0.125}, {0.250, 0.500, 0.250}, {0.125, 0.250, 0.125}},
                                          {{0.250, 0.500, 0.250}, {0.500, 1.000, 0.500}, {0.250, 0.500, 0.250}},
                                          {{0.125, 0.250, 0.125}, {0.250, 0.500, 0.250}, {0.125, 0.250, 0.125}}};
+#pragma omp parallel for schedule(static)
     for (int k = 1; k < mz - 1; k++)
         for (int j = 1; j < my - 1; j++)
             for (int i = 1; i < mx - 1; i++) {
@@ -239,6 +241,7 @@ void orc_smagorinsky_constant(const orc_params *p, const orc_metrics *m, const d
 
     /* les.c:733-797: filtered (or local, with a homogeneous filter) LM / MM */
     const int homo = p->i_homo || p->j_homo || p->k_homo;
+#pragma omp parallel for schedule(static)
     for (int k = 1; k < mz - 1; k++)
         for (int j = 1; j < my - 1; j++)
             for (int i = 1; i < mx - 1; i++) {

@@ -298,6 +298,7 @@ static void level_build_operator(level_t *L)
     size_t n = npts(p);
     L->coef = dalloc(NSTENCIL * n);
     L->row = (unsigned char *)calloc(n, 1);
+#pragma omp parallel for schedule(static)
     for (int k = 1; k < p->mz - 1; k++)
         for (int j = 1; j < p->my - 1; j++)
             for (int i = 1; i < p->mx - 1; i++) {
@@ -305,6 +306,7 @@ static void level_build_operator(level_t *L)
                 size_t q = IX(p, k, j, i);
                 L->row[q] = 1;
             }
+#pragma omp parallel for schedule(static)
     for (int k = 1; k < p->mz - 1; k++)
         for (int j = 1; j < p->my - 1; j++)
             for (int i = 1; i < p->mx - 1; i++) {
@@ -322,6 +324,7 @@ static void level_apply(const level_t *L, const double *x, double *y)
 {
     const orc_params *p = &L->p;
     memset(y, 0, npts(p) * sizeof(double));
+#pragma omp parallel for schedule(static)
     for (int k = 1; k < p->mz - 1; k++)
         for (int j = 1; j < p->my - 1; j++)
             for (int i = 1; i < p->mx - 1; i++) {
@@ -358,6 +361,7 @@ void orc_poisson_rhs(const orc_params *p, const orc_metrics *m, const double *uc
     const int mx = p->mx, my = p->my, mz = p->mz;
     memset(b, 0, npts(p) * sizeof(double));
     const double coeff = 1.; /* time_coeff(), poisson.c:23-31 */
+#pragma omp parallel for schedule(static)
     for (int k = 1; k < mz - 1; k++)
         for (int j = 1; j < my - 1; j++)
             for (int i = 1; i < mx - 1; i++) {
@@ -506,9 +510,14 @@ static void smooth_lines(const level_t *L, double *x, const double *b)
     const orc_params *p = &L->p;
     const int my = p->my;
     size_t n = npts(p);
-    double *xin = dalloc(n), *cp = dalloc(my), *dp = dalloc(my);
+    double *xin = dalloc(n);
     for (int col = 0; col < 2; col++) {
         memcpy(xin, x, n * sizeof(double));
+        /* the lines of one colour are independent: they read xin only */
+#pragma omp parallel
+        {
+        double *cp = dalloc(my), *dp = dalloc(my);
+#pragma omp for schedule(static) collapse(2)
         for (int k = 1; k < p->mz - 1; k++)
             for (int i = 1; i < p->mx - 1; i++) {
                 if (((i + k) & 1) != col) continue;
@@ -539,14 +548,17 @@ static void smooth_lines(const level_t *L, double *x, const double *b)
                     x[q] = xl;
                 }
             }
+        free(cp); free(dp);
+        }
     }
-    free(xin); free(cp); free(dp);
+    free(xin);
 }
 
 static void restrict_sum(const level_t *Lf, const double *rf, const level_t *Lc, double *bc)
 {
     const orc_params *pc = &Lc->p, *pf = &Lf->p;
     memset(bc, 0, npts(pc) * sizeof(double));
+#pragma omp parallel for schedule(static)
     for (int k = 1; k < pc->mz - 1; k++)
         for (int j = 1; j < pc->my - 1; j++)
             for (int i = 1; i < pc->mx - 1; i++) {
@@ -575,6 +587,7 @@ static inline int parent_nb(int I, int d, int mc, int per)
 static void prolong_add(const level_t *Lc, const double *xc, const level_t *Lf, double *xf)
 {
     const orc_params *pc = &Lc->p, *pf = &Lf->p;
+#pragma omp parallel for schedule(static)
     for (int k = 1; k < pf->mz - 1; k++)
         for (int j = 1; j < pf->my - 1; j++)
             for (int i = 1; i < pf->mx - 1; i++) {
@@ -596,13 +609,14 @@ static void residual(const level_t *L, const double *x, const double *b, double 
 {
     level_apply(L, x, r);
     size_t n = npts(&L->p);
+#pragma omp parallel for schedule(static)
     for (size_t q = 0; q < n; q++) r[q] = L->row[q] ? b[q] - r[q] : 0.;
 }
 
 static void vcycle(mg_t *mg, int l, int pre, int post, int coarse_sweeps)
 {
     level_t *L = &mg->L[l];
-    if (l == mg->nl - 1) {
+    if (l == mg->nl - 1 || l == 7) {
         for (int s = 0; s < coarse_sweeps; s++) smooth_lines(L, mg->x[l], mg->b[l]);
         return;
     }
@@ -617,9 +631,20 @@ static void vcycle(mg_t *mg, int l, int pre, int post, int coarse_sweeps)
 
 static double dot(const level_t *L, const double *a, const double *b)
 {
+    /* partial sums over fixed chunks of 4096 points, added in chunk order:
+     * the same value whatever the number of threads */
+    const size_t CH = 4096, n = npts(&L->p), nch = (n + CH - 1) / CH;
+    double *part = dalloc(nch);
+#pragma omp parallel for schedule(static)
+    for (size_t c = 0; c < nch; c++) {
+        double s = 0;
+        const size_t e = (c + 1) * CH < n ? (c + 1) * CH : n;
+        for (size_t q = c * CH; q < e; q++) if (L->row[q]) s += a[q] * b[q];
+        part[c] = s;
+    }
     double s = 0;
-    size_t n = npts(&L->p);
-    for (size_t q = 0; q < n; q++) if (L->row[q]) s += a[q] * b[q];
+    for (size_t c = 0; c < nch; c++) s += part[c];
+    free(part);
     return s;
 }
 
@@ -655,6 +680,7 @@ int orc_poisson_solve(const orc_params *p, const orc_metrics *m, const double *b
         double beta = sqrt(dot(L, V[0], V[0]));
         rel = beta / bnorm;
         if (rel <= tol) break;
+#pragma omp parallel for schedule(static)
         for (size_t q = 0; q < n; q++) V[0][q] /= beta;
         memset(g, 0, (mr + 1) * sizeof(double));
         g[0] = beta;
@@ -671,10 +697,12 @@ int orc_poisson_solve(const orc_params *p, const orc_metrics *m, const double *b
             for (int i = 0; i <= jj; i++) {
                 double h = dot(L, w, V[i]);
                 H[i * mr + jj] = h;
+#pragma omp parallel for schedule(static)
                 for (size_t q = 0; q < n; q++) w[q] -= h * V[i][q];
             }
             double hn = sqrt(dot(L, w, w));
             H[(jj + 1) * mr + jj] = hn;
+#pragma omp parallel for schedule(static)
             for (size_t q = 0; q < n; q++) V[jj + 1][q] = hn > 0 ? w[q] / hn : 0;
             for (int i = 0; i < jj; i++) {
                 double t = cs[i] * H[i * mr + jj] + sn[i] * H[(i + 1) * mr + jj];
@@ -697,6 +725,7 @@ int orc_poisson_solve(const orc_params *p, const orc_metrics *m, const double *b
             y[i] = s / H[i * mr + i];
         }
         for (int i = 0; i < jj; i++)
+#pragma omp parallel for schedule(static)
             for (size_t q = 0; q < n; q++) phi[q] += y[i] * Z[i][q];
     }
     residual(L, phi, b, w);
@@ -745,6 +774,7 @@ void orc_projection(const orc_params *p, const orc_metrics *m, const double *phi
     const double coeff = 1.;
 #define PH(kk, jj, ii) S(phi, kk, jj, ii)
 #define NV(kk, jj, ii) S(nv, kk, jj, ii)
+#pragma omp parallel for schedule(static)
     for (int k = 1; k < mz - 1; k++)
         for (int j = 1; j < my - 1; j++)
             for (int i = 1; i < mx - 1; i++) {

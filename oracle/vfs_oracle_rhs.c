@@ -118,6 +118,7 @@ void orc_formfunction2(const orc_params *p, const orc_metrics *m, double *ucont,
     orc_periodic_fix(p, ucont, 3); /* momentum.c:638-666 */
 
     /* i faces, momentum.c:668-928 */
+#pragma omp parallel for schedule(static)
     for (int k = 1; k < mz - 1; k++)
         for (int j = 1; j < my - 1; j++)
             for (int i = 0; i < mx - 1; i++) {
@@ -152,6 +153,7 @@ void orc_formfunction2(const orc_params *p, const orc_metrics *m, double *ucont,
                 visc_face(cs, et, ze, cs, ajc, d, nu, nut, p->les, p->laplacian, &visc[0][3 * q]);
             }
     /* j faces, momentum.c:930-1211 */
+#pragma omp parallel for schedule(static)
     for (int k = 1; k < mz - 1; k++)
         for (int j = 0; j < my - 1; j++)
             for (int i = 1; i < mx - 1; i++) {
@@ -186,6 +188,7 @@ void orc_formfunction2(const orc_params *p, const orc_metrics *m, double *ucont,
                 visc_face(cs, et, ze, et, ajc, d, nu, nut, p->les, p->laplacian, &visc[1][3 * q]);
             }
     /* k faces, momentum.c:1212-1451 */
+#pragma omp parallel for schedule(static)
     for (int k = 0; k < mz - 1; k++)
         for (int j = 1; j < my - 1; j++)
             for (int i = 1; i < mx - 1; i++) {
@@ -223,6 +226,7 @@ void orc_formfunction2(const orc_params *p, const orc_metrics *m, double *ucont,
     for (int d = 0; d < 3; d++) { orc_periodic_fix(p, div[d], 3); orc_periodic_fix(p, visc[d], 3); }
 
     /* cell loop, momentum.c:1548-1678 */
+#pragma omp parallel for schedule(static)
     for (int k = 1; k < mz - 1; k++)
         for (int j = 1; j < my - 1; j++)
             for (int i = 1; i < mx - 1; i++) {
@@ -263,6 +267,7 @@ void orc_formfunction2(const orc_params *p, const orc_metrics *m, double *ucont,
     orc_periodic_fix(p, fp, 3); /* momentum.c:1685-1713 */
 
     /* contravariant projection, momentum.c:1715-1841 */
+#pragma omp parallel for schedule(static)
     for (int k = 1; k < mz - 1; k++)
         for (int j = 1; j < my - 1; j++)
             for (int i = 1; i < mx - 1; i++) {
@@ -324,8 +329,10 @@ void orc_runge_kutta(const orc_params *p_in, const orc_metrics *m, double *ucont
         orc_contra2cart(p, m, ucont, ucat, ucat_old);
         memset(rhs, 0, n3 * sizeof(double));
         orc_formfunction2(p, m, ucont, ucat, nu_t, rhs, 1.0);
+#pragma omp parallel for schedule(static)
         for (size_t q = 0; q < n3; q++) rhs[q] += -1 * dp[q];
         double a = alfa[s] * p->dt * p->st;
+#pragma omp parallel for schedule(static)
         for (size_t q = 0; q < n3; q++) ucont[q] = a * rhs[q] + pucont[q];
         orc_contra2cart(p, m, ucont, ucat, ucat_old);
     }
@@ -386,11 +393,13 @@ void orc_formfunction_snes(const orc_params *p, const orc_metrics *m, const doub
             }
     orc_contra2cart(p, m, ucont, ucat, ucat_old);
     const double dt = p->dt;
+#pragma omp parallel for schedule(static)
     for (size_t q = 0; q < n3; q++) {
         r[q] = 0.0 + (-1. / dt) * ucont[q];
         r[q] = r[q] + (1. / dt) * ucont_o[q];
     }
     orc_formfunction2(p, m, ucont, ucat, nu_t, r, 0.5);
+#pragma omp parallel for schedule(static)
     for (size_t q = 0; q < n3; q++) {
         r[q] = r[q] + 0.5 * rhs_o[q];
         r[q] = r[q] + (-1.0) * dp[q];

@@ -346,3 +346,86 @@ def test_poisson_solve_after_reused_memory():
     st = ctx.poisson_solve(vfs.default_solve_opts(tol=1e-8, mg_fp32=1, use_guess=0))
     assert st.relres <= 1e-8 and np.isfinite(ctx.download("PHI")).all()
     ctx.close()
+
+
+def _blank(o, ctx):
+    """A solid block (nvert 3) wrapped in a layer of boundary cells (nvert 1)
+    inside the channel, in the oracle and the context."""
+    nv = np.zeros(o.shape)
+    mz, my, mx = o.shape
+    nv[mz // 2 - 3: mz // 2 + 3, 3: 9, mx // 2 - 4: mx // 2 + 3] = 1.0
+    nv[mz // 2 - 2: mz // 2 + 2, 4: 8, mx // 2 - 3: mx // 2 + 2] = 3.0
+    o.m["nvert"][...] = nv
+    ctx.upload("NVERT", nv)
+    return nv
+
+
+@pytest.mark.parametrize("fp32", [1, 0])
+def test_poisson_blanked(fp32):
+    """setup_lidx2's unknowns (the row mask), PoissonLHSNew's rows next to
+    blanked cells (one-sided cross terms) and the blanked coarse levels, on a
+    fresh context whose hierarchy is built with the blanking: right-hand side
+    and operator at 1e-12, the solve to 1e-8 with the oracle's operator."""
+    c, o, ctx = channel_pair(24, 16, 20, les=0)
+    with ctx:
+        nv = _blank(o, ctx)
+        o.runge_kutta()
+        ctx.upload("UCONT", o["ucont"])
+        o.poisson_rhs()
+        ctx.poisson_rhs()
+        assert relerr(ctx.download("B"), o["b"]) <= RHS_TOL
+        assert np.all(o["b"][nv > 0.1] == 0)
+        rng = np.random.default_rng(4)
+        x = rng.standard_normal(o.shape)
+        ctx.upload("PHI", x)
+        ctx.poisson_apply_dev(ctx.device_ptr("PHI"), ctx.device_ptr("RHS"))
+        y = ctx.download("RHS").reshape(-1)[: x.size].reshape(o.shape)
+        assert relerr(y, o.poisson_apply(x)) <= RHS_TOL
+        ctx.upload("PHI", np.zeros(o.shape))
+        st = ctx.poisson_solve(vfs.default_solve_opts(tol=1e-8, mg_fp32=fp32, use_guess=0, maxit=400))
+        assert st.relres <= 1e-8
+        phi = ctx.download("PHI")
+        b = o["b"]
+        assert np.linalg.norm(b - o.poisson_apply(phi)) / np.linalg.norm(b) <= 1.01e-8
+
+
+def test_poisson_rebuilt_after_blanking_changes():
+    """A solve, then an upload of a blanked nvert, then a solve again on the
+    same context: the operator hierarchy is rebuilt from the new blanking
+    (PoissonLHSNew is re-run whenever nvert may change, poisson_hypre.c:534-550)
+    and the second solution satisfies the oracle's blanked operator."""
+    c, o, ctx = channel_pair(24, 16, 20, les=0)
+    with ctx:
+        opts = vfs.default_solve_opts(tol=1e-8, mg_fp32=1, use_guess=0, maxit=400)
+        ctx.poisson_rhs()
+        ctx.upload("PHI", np.zeros(o.shape))
+        assert ctx.poisson_solve(opts).relres <= 1e-8
+        _blank(o, ctx)
+        o.poisson_rhs()
+        ctx.poisson_rhs()
+        ctx.upload("PHI", np.zeros(o.shape))
+        st = ctx.poisson_solve(opts)
+        phi = ctx.download("PHI")
+        b = o["b"]
+        assert st.relres <= 1e-8
+        assert np.linalg.norm(b - o.poisson_apply(phi)) / np.linalg.norm(b) <= 1.01e-8
+
+
+def test_write_through_kept_pointer():
+    """Ucont written through a device pointer kept from an earlier call, then
+    declared with vfs_field_written: the next Contra2Cart recomputes Ucat."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+    c, o, ctx = channel_pair(16, 12, 16, les=0)
+    with ctx:
+        ptr = ctx.device_ptr("UCONT")
+        ctx.contra2cart()  # Ucat fresh for the current Ucont
+        u2 = 0.5 * o["ucont"]
+        ctx.sync()
+        assert hip.hipMemcpy(ptr, u2.ctypes.data, u2.nbytes, 1) == 0  # hipMemcpyHostToDevice
+        ctx.field_written("UCONT")
+        ctx.contra2cart()
+        o.set("ucont", u2)
+        o.contra2cart()
+        assert relerr(ctx.download("UCAT"), o["ucat"]) <= RHS_TOL

@@ -44,6 +44,33 @@ def test_smoother_forms_converge(monkeypatch, cells, compact):
     assert np.isfinite(phi).all()
 
 
+@pytest.mark.parametrize("fp32", [1, 0], ids=["fp32", "fp64"])
+@pytest.mark.parametrize("path", ["short", "long", "solve"])
+def test_line_solve_paths(monkeypatch, path, fp32):
+    """Every line-solve form (8-point scan, 16-point FP32 / 8-point FP64 scan,
+    one thread per line: the forms long lines take at large grids) forced on
+    small grids preconditions to the tolerance, checked with the oracle's
+    operator, and agrees with the oracle's converged solution."""
+    for cells in ((24, 16, 20), (18, 13, 22)):
+        monkeypatch.setenv("VFS_LINE_PATH", path)
+        c, o, ctx = channel_pair(*cells, les=0)
+        try:
+            o.poisson_rhs()
+            ctx.poisson_rhs()
+            ctx.upload("PHI", np.zeros(o.shape))
+            st = ctx.poisson_solve(vfs.default_solve_opts(tol=1e-10, mg_fp32=fp32, use_guess=0, maxit=400))
+            phi = ctx.download("PHI")
+            b = o["b"]
+            assert st.relres <= 1e-10
+            assert np.linalg.norm(b - o.poisson_apply(phi)) / np.linalg.norm(b) <= 1.01e-10
+            o.set("phi", np.zeros(o.shape))
+            o.poisson_solve(tol=1e-10)
+            assert relerr(phi, o["phi"]) <= 1e-5
+        finally:
+            ctx.close()
+            monkeypatch.delenv("VFS_LINE_PATH")
+
+
 def test_compact_smoother_odd_periodic(monkeypatch):
     """An odd periodic interior count (19 cells in x) puts lines of one colour
     next to each other across the wrap; the right-hand side pass reads the

@@ -122,6 +122,7 @@ _SIGS = {
     "vfs_upload": ([C.c_void_p, C.c_int, C.c_void_p], C.c_int),
     "vfs_download": ([C.c_void_p, C.c_int, C.c_void_p], C.c_int),
     "vfs_field_device_ptr": ([C.c_void_p, C.c_int], C.c_void_p),
+    "vfs_field_written": ([C.c_void_p, C.c_int], C.c_int),
     "vfs_sync": ([C.c_void_p], C.c_int),
     "vfs_form_metrics": ([C.c_void_p], C.c_int),
     "vfs_mean_k_area": ([C.c_void_p, C.POINTER(C.c_double)], C.c_int),
@@ -293,6 +294,10 @@ class Context:
 
     def device_ptr(self, name: str) -> int:
         return int(self.lib.vfs_field_device_ptr(self.h, FIELD[name]) or 0)
+
+    def field_written(self, name: str):
+        """Declare a write through a kept device pointer (vfs_field_written)."""
+        self._check(self.lib.vfs_field_written(self.h, FIELD[name]))
 
     def set_params(self, params: Params):
         self._check(self.lib.vfs_set_params(self.h, C.byref(params)))

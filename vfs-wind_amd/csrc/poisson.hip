@@ -1356,10 +1356,17 @@ static void smooth(vfs_ctx *c, PLevel &L, int sweeps)
     const long long lines = (long long)L.z.nkI * L.z.nh;
     const bool fine = L.g.n == c->g.n;
     constexpr int SLM = line_slm<T>();
-    const bool short_seg = sizeof(T) == 4 && L.z.njI <= LINE_SLM_SHORT * LINE_MAXSEG;
+    // VFS_LINE_PATH forces one line-solve form (tests reach every form on
+    // small grids): "short" the 8-point FP32 scan, "long" the SLM-point scan,
+    // "solve" one thread per line; otherwise the line length chooses
+    const char *lp = std::getenv("VFS_LINE_PATH");
+    const int force = !lp ? 0 : lp[0] == 's' && lp[1] == 'h' ? 1 : lp[0] == 'l' ? 2 : lp[0] == 's' ? 3 : 0;
+    bool short_seg = sizeof(T) == 4 && L.z.njI <= LINE_SLM_SHORT * LINE_MAXSEG;
+    if (force == 1) short_seg = sizeof(T) == 4;
+    if (force == 2) short_seg = false;
     const int slm = short_seg ? LINE_SLM_SHORT : SLM;
     const int nseg = (L.z.njI + slm - 1) / slm;
-    const bool scan = nseg <= LINE_MAXSEG;  // lines up to 32 * slm points
+    const bool scan = nseg <= LINE_MAXSEG && force != 3;  // lines up to 32 * slm points
     const int sl = (L.z.njI + nseg - 1) / nseg;
     const bool c4 = sizeof(T) == 4 && L.compact;
     const h4_t *sh4 = reinterpret_cast<const h4_t *>(L.sh16);

@@ -207,9 +207,12 @@ int vfs_set_params(vfs_ctx *c, const vfs_params *params)
     if (params->mx != c->p.mx || params->my != c->p.my || params->mz != c->p.mz ||
         params->per_i != c->p.per_i || params->per_j != c->p.per_j || params->per_k != c->p.per_k)
         return vfs::set_error(VFS_EINVAL, "grid size and periodicity are fixed at creation");
+    bool op_changed = params->poisson_threshold != c->p.poisson_threshold;
+    for (int q = 0; q < 6; q++) op_changed = op_changed || params->bctype[q] != c->p.bctype[q];
     c->p = *params;
     c->g = vfs::make_grid(*params);
     vfs::ucat_stale(c);  // the boundary codes may have changed
+    if (op_changed) vfs::poisson_free(c);  // the operator's rows depend on both
     return VFS_OK;
 }
 
@@ -226,13 +229,31 @@ size_t vfs_field_size(const vfs_ctx *c, vfs_field f)
     return (is_vector_field(f) ? 3 : 1) * (size_t)c->g.n;
 }
 
-int vfs_upload(vfs_ctx *c, vfs_field f, const double *host)
+// What a write of field f (upload, or through a handed-out device pointer)
+// invalidates: the derived-face-metric flag, the fresh-Ucat state, the
+// all-fluid flag, the Poisson operator hierarchy (built from the node
+// coordinates, the metrics and nvert; PoissonLHSNew is rebuilt whenever the
+// blanking can change, poisson_hypre.c:534-550) and the lCs mode.
+static void field_written(vfs_ctx *c, int f)
 {
-    if (!c || !host || f < 0 || f >= VFS_NFIELDS) return vfs::set_error(VFS_EINVAL, "bad upload argument");
     if (vfs::is_metric_field(f)) c->face_derived = false;
     if (vfs::is_c2c_field(f)) vfs::ucat_stale(c);
     if (f == VFS_NVERT) c->nvert_zero = -1;
+    if (f == VFS_NVERT || f == VFS_COOR || vfs::is_metric_field(f)) vfs::poisson_free(c);
     if (f == VFS_CS) c->cs_mode = vfs::CS_FIELD;
+}
+
+int vfs_field_written(vfs_ctx *c, vfs_field f)
+{
+    if (!c || f < 0 || f >= VFS_NFIELDS) return vfs::set_error(VFS_EINVAL, "bad field");
+    field_written(c, f);
+    return VFS_OK;
+}
+
+int vfs_upload(vfs_ctx *c, vfs_field f, const double *host)
+{
+    if (!c || !host || f < 0 || f >= VFS_NFIELDS) return vfs::set_error(VFS_EINVAL, "bad upload argument");
+    field_written(c, f);
     VFS_HIP(hipMemcpyAsync(c->f[f], host, vfs_field_size(c, f) * sizeof(double), hipMemcpyHostToDevice, c->stream));
     VFS_HIP(hipStreamSynchronize(c->stream));
     return VFS_OK;
@@ -253,10 +274,8 @@ int vfs_download(vfs_ctx *c, vfs_field f, double *host)
 double *vfs_field_device_ptr(vfs_ctx *c, vfs_field f)
 {
     if (!c || f < 0 || f >= VFS_NFIELDS) return nullptr;
-    if (vfs::is_metric_field(f)) c->face_derived = false;  // the caller may write through it
-    if (vfs::is_c2c_field(f)) vfs::ucat_stale(c);
-    if (f == VFS_NVERT) c->nvert_zero = -1;
     if (f == VFS_CS && vfs::materialize_cs(c)) return nullptr;
+    field_written(c, f);  // the caller may write through it before the next call
     return c->f[f];
 }
 
