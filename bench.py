@@ -17,6 +17,7 @@ import argparse
 import importlib
 import json
 import os
+import subprocess
 import sys
 import time
 
@@ -82,13 +83,42 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(nx=64, ny=48, nz=64, steps=12):
-    """FP64 CPU oracle (single thread) on the 64x48x64 channel: Mcell-steps/s."""
+def _host_info():
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        cc = subprocess.run(["gcc", "--version"], capture_output=True, text=True).stdout.splitlines()[0]
+    except (OSError, IndexError):
+        cc = "gcc"
+    return model, cc
+
+
+def cpu_baseline(c, timed_steps=1):
+    """The restated reference CPU path: the FP64 oracle (OpenMP, fixed-order
+    reductions) built -O3 -march=native -fopenmp on this host, on the same
+    channel as the GPU run.  One untimed step (it builds the multigrid
+    hierarchy), then `timed_steps` steps on every OpenMP thread and one step
+    on one thread.  Mcell-steps/s of full steps (RK4 + FGMRES/MG to 1e-8 +
+    projection)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle as po
 
-    cases = importlib.import_module("vfs-wind_amd.cases")
-    c = cases.channel(nx, ny, nz)
+    flags = "-O3 -march=native -fopenmp -ffp-contract=off"
+    try:
+        path = po.build_native()
+    except (OSError, subprocess.CalledProcessError) as e:
+        log(f"native oracle build failed ({e}); using the portable build")
+        path, flags = None, "-O3 -fopenmp -ffp-contract=off"
+    po.load(path)
+    threads = po.set_threads(0)
+    nx, ny, nz = c.cells
     p = po.make_params(*c.da, per=c.per, bctype=c.bctype, ren=c.ren, dt=1e-3, inlet_flux=c.inlet_flux)
     o = po.Oracle(p, c.coor)
     o.set("ucont", c.ucont)
@@ -96,15 +126,27 @@ def cpu_baseline(nx=64, ny=48, nz=64, steps=12):
     o.set("ucat_old", o["ucat"])
     p.dt = 0.3 * o.min_dt()
     o.mean_k_area()
-    o.fractional_step(tol=1e-8)  # warm-up (first solve starts from zero)
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        o.fractional_step(tol=1e-8)
-    dt = time.perf_counter() - t0
+    o.fractional_step(tol=1e-8)  # hierarchy set-up; the first solve starts from zero
     cells = nx * ny * nz
-    return {"value": cells * steps / dt / 1e6, "unit": "Mcell-steps/s", "cores": 1, "kind": "port",
-            "sample": f"FP64 CPU oracle, channel {nx}x{ny}x{nz}, {steps} full steps (RK4 + FGMRES/MG 1e-8 + projection), "
-                      f"{dt:.1f} s"}
+    t0 = time.perf_counter()
+    for _ in range(timed_steps):
+        o.fractional_step(tol=1e-8)
+    t_all = (time.perf_counter() - t0) / timed_steps
+    log(f"cpu baseline: {threads} threads {t_all:.2f} s/step")
+    po.set_threads(1)
+    t0 = time.perf_counter()
+    o.fractional_step(tol=1e-8)
+    t_one = time.perf_counter() - t0
+    po.set_threads(threads)
+    log(f"cpu baseline: 1 thread {t_one:.2f} s/step")
+    model, cc = _host_info()
+    return {"value": round(cells / t_all / 1e6, 4), "unit": "Mcell-steps/s", "cores": threads, "kind": "port",
+            "sample": f"restated reference CPU path (FP64 oracle), channel {nx}x{ny}x{nz}, {timed_steps} full "
+                      f"step(s) (RK4 + FGMRES/MG 1e-8 + projection) on {threads} threads after one untimed step: "
+                      f"{t_all:.2f} s/step; one step on 1 thread: {t_one:.2f} s",
+            "one_core_value": round(cells / t_one / 1e6, 4),
+            "cpu_model": model, "nproc": os.cpu_count(), "omp_num_threads": os.environ.get("OMP_NUM_THREADS"),
+            "compiler": cc, "flags": flags}
 
 
 def poisson_comparison(vfs, ctx, reps=3):
@@ -199,7 +241,6 @@ def main():
     ctx.upload("UCAT_OLD", ctx.download("UCAT"))
     params.dt = 0.3 * ctx.min_dt()
     ctx.set_params(params)
-    del c
     opts = vfs.default_solve_opts(tol=1e-8, mg_fp32=0 if args.mg_fp64 else 1)
     log(f"dt = {params.dt:.4e}; warmup {args.warmup} steps")
     for s in range(args.warmup):
@@ -268,7 +309,7 @@ def main():
     cpu = None
     if not args.no_cpu_baseline:
         log("cpu baseline (FP64 oracle, bounded sample)")
-        cpu = cpu_baseline()
+        cpu = cpu_baseline(c)
 
     out = {
         "metric": "Mcell-steps/s at 1e-8 Poisson resid; ms/Poisson solve plain MG vs FP32+net",

@@ -308,6 +308,37 @@ int vfs_unet_set_params(vfs_unet *net, const float *host);
  * the solve restarts from the guess with the plain V-cycle (stats->fallback = 1). */
 int vfs_poisson_solve_learned(vfs_ctx *ctx, vfs_unet *net, const vfs_solve_opts *opts, vfs_solve_stats *stats);
 
+/* ---- file formats (host side) ----------------------------------------- */
+/* PETSc binary Vec files as VecView writes them and VecLoadIntoVector reads
+ * them (PETSc 3.1, the reference's PetscViewerBinaryOpen calls in
+ * main.c:376-934): big-endian int32 class cookie 1211214, int32 length n,
+ * n big-endian doubles; a DMDA vector in natural (= one-rank global) order.
+ * vfs_vec_read fails unless the file holds exactly n values. */
+int vfs_vec_write(const char *path, const double *host, long long n);
+int vfs_vec_read(const char *path, double *host, long long n);
+int vfs_vec_length(const char *path, long long *n);
+/* Ucont_P_Binary_Output (main.c:684-770, -qcr 0, no averaging): writes
+ * dir/vfield%06d_%1d.dat (Ucont), ufield (Ucat), pfield (P) and nvfield
+ * (Nvert) for time index ti and block number block. */
+int vfs_checkpoint_write(vfs_ctx *ctx, const char *dir, int ti, int block);
+/* Ucont_P_Binary_Input (main.c:376-426, no averaging): loads Ucont, P and Ucat,
+ * then Ucont_o = Ucont and Ucat_old = Ucat; the blanking is zeroed unless
+ * immersed != 0, in which case nvfield is loaded into VFS_NVERT (the
+ * immersed-boundary search that recomputes it is not part of this library). */
+int vfs_checkpoint_read(vfs_ctx *ctx, const char *dir, int ti, int block, int immersed);
+/* Grid files of MG_Initial (init.c:258-394): grid.dat as text (block count,
+ * then per block "IM JM KM" and all x, all y, all z values in (k, j, i)
+ * order), grid.dat binary (-binary_input: the same as native int / double),
+ * xyz.dat (-xyz_input: "IM JM KM", then IM lines whose first column is x,
+ * JM lines whose second is y, KM lines whose third is z). */
+enum { VFS_GRID_TEXT = 0, VFS_GRID_BINARY = 1, VFS_GRID_XYZ = 2 };
+int vfs_grid_dims(const char *path, int format, int block, int *IM, int *JM, int *KM);
+/* Node coordinates of one block into a DA-layout coordinate array (mx = IM+1,
+ * my = JM+1, mz = KM+1; nodes at indices 0..IM-1 etc., the last plane zero),
+ * each value divided by cl and multiplied by L_dim as the reference does. */
+int vfs_grid_read(const char *path, int format, int block, double cl, double L_dim, int mx, int my, int mz,
+                  double *coor);
+
 /* ---- measurement (no counterpart in the reference) -------------------- */
 /* Enable (1) or disable (0) HIP-event timing of the hot kernels' launches on
  * the context stream and clear earlier records.  For benchmarks only. */

@@ -60,13 +60,24 @@ def build(force: bool = False) -> Path:
     return LIB_PATH
 
 
-def load() -> C.CDLL:
+def build_native() -> Path:
+    """Builds liborc_native.so (-march=native) for the host this runs on."""
+    path = ORACLE_DIR / "liborc_native.so"
+    subprocess.run(["make", "-C", str(ORACLE_DIR), "-B", path.name], check=True, capture_output=True, text=True)
+    return path
+
+
+def load(path: str | os.PathLike | None = None) -> C.CDLL:
+    """The oracle library (liborc.so unless `path` names another build of the
+    same sources); the first call fixes which one the process uses."""
     global _lib
     if _lib is not None:
         return _lib
-    if not LIB_PATH.exists():
-        build()
-    lib = C.CDLL(str(LIB_PATH))
+    if path is None:
+        if not LIB_PATH.exists():
+            build()
+        path = LIB_PATH
+    lib = C.CDLL(str(path))
     P, M, D = C.POINTER(OrcParams), C.POINTER(OrcMetrics), C.POINTER(C.c_double)
     sig = {
         "orc_form_metrics": ([P, D, M], None),
@@ -97,6 +108,7 @@ def load() -> C.CDLL:
         "orc_inlet_area": ([P, M], C.c_double),
         "orc_inflow_flux": ([P, M, D, D], C.c_double),
         "orc_formfunction_snes": ([P, M, D, D, D, D, D, D, D, D, D, D], None),
+        "orc_set_threads": ([C.c_int], C.c_int),
         "orc_arbitrary_rotate": ([D, C.c_double, D, D], None),
         "orc_alm_elements": ([C.c_int, C.c_int, D, D, D], None),
         "orc_alm_rotate": ([C.POINTER(OrcTurbine), C.c_double, C.c_int, D, D], None),
@@ -114,6 +126,11 @@ def load() -> C.CDLL:
         fn.restype = res
     _lib = lib
     return lib
+
+
+def set_threads(n: int = 0) -> int:
+    """OpenMP team size of the oracle (n > 0 sets it); returns the current one."""
+    return load().orc_set_threads(int(n))
 
 
 def _ptr(a: np.ndarray):

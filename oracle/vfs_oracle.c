@@ -11,6 +11,16 @@
  * laplacian 0/1, no level set, no RANS, no immersed boundary, no wall model.
  */
 #include "orc_internal.h"
+#include <omp.h>
+
+/* OpenMP team size of the oracle's parallel loops (the CPU baseline times it
+ * on all cores and on one).  Every reduction keeps a fixed order, so results
+ * do not depend on the team size. */
+int orc_set_threads(int n)
+{
+    if (n > 0) omp_set_num_threads(n);
+    return omp_get_max_threads();
+}
 
 /* The fix reads only cells whose periodic indices are interior, which the fix
  * never writes, so an in-place pass equals the reference's local-array pass. */

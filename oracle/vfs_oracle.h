@@ -23,6 +23,9 @@
 extern "C" {
 #endif
 
+/* Sets the OpenMP team size (n > 0) and returns the current maximum. */
+int orc_set_threads(int n);
+
 typedef struct {
     int mx, my, mz;                 /* DA sizes = cells + 2 (IM+1, JM+1, KM+1) */
     int per_i, per_j, per_k;        /* ii_/jj_/kk_periodic (main.c:283-288)    */

@@ -46,6 +46,13 @@ def test_dynamic_coefficient(case, homo):
         c, o, ctx = box_pair(max_cs=0.2, homo=h)
     with ctx:
         assert relerr(ctx.download("UCAT"), o["ucat"]) <= 1e-13
+        if case == "box" and h[2]:
+            # under the uniform inflow U = 1 every streamwise line averages to a negative LM, so C clips to 0
+            # everywhere (les.c:967-980); the k-averaged box cases run on the fluctuation field instead
+            u = o["ucat"].copy()
+            u[..., 2] -= 1.0
+            o.set("ucat", u)
+            ctx.upload("UCAT", u)
         ref = o.smagorinsky_constant(ti=5)
         ctx.smagorinsky_constant(ti=5)
         got = ctx.download("CS")

@@ -171,6 +171,15 @@ _SIGS = {
     "vfs_unet_get_params": ([C.c_void_p, C.c_int, C.c_void_p], C.c_int),
     "vfs_unet_set_params": ([C.c_void_p, C.c_void_p], C.c_int),
     "vfs_poisson_solve_learned": ([C.c_void_p, C.c_void_p, C.POINTER(SolveOpts), C.POINTER(SolveStats)], C.c_int),
+    "vfs_vec_write": ([C.c_char_p, C.c_void_p, C.c_longlong], C.c_int),
+    "vfs_vec_read": ([C.c_char_p, C.c_void_p, C.c_longlong], C.c_int),
+    "vfs_vec_length": ([C.c_char_p, C.POINTER(C.c_longlong)], C.c_int),
+    "vfs_checkpoint_write": ([C.c_void_p, C.c_char_p, C.c_int, C.c_int], C.c_int),
+    "vfs_checkpoint_read": ([C.c_void_p, C.c_char_p, C.c_int, C.c_int, C.c_int], C.c_int),
+    "vfs_grid_dims": ([C.c_char_p, C.c_int, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)],
+                      C.c_int),
+    "vfs_grid_read": ([C.c_char_p, C.c_int, C.c_int, C.c_double, C.c_double, C.c_int, C.c_int, C.c_int, C.c_void_p],
+                      C.c_int),
     "vfs_profile": ([C.c_void_p, C.c_int], C.c_int),
     "vfs_profile_select": ([C.c_void_p, C.c_uint], C.c_int),
     "vfs_profile_read": ([C.c_void_p, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_longlong),
@@ -212,6 +221,56 @@ def load_library(path: str | os.PathLike | None = None) -> C.CDLL:
 
 def exported_symbols() -> list[str]:
     return list(_SIGS)
+
+
+def _raise(lib, rc):
+    if rc != VFS_OK:
+        raise VFSError(rc, lib.vfs_last_error().decode())
+
+
+# ---- file formats (host side; no device needed) -------------------------------
+GRID_FORMATS = {"text": 0, "binary": 1, "xyz": 2}
+
+
+def vec_write(path, arr: np.ndarray):
+    """Writes a PETSc binary Vec file (VecView's format) holding arr's values."""
+    lib = load_library()
+    a = np.ascontiguousarray(arr, dtype=np.float64).reshape(-1)
+    _raise(lib, lib.vfs_vec_write(os.fsencode(path), a.ctypes.data, a.size))
+
+
+def vec_length(path) -> int:
+    lib = load_library()
+    n = C.c_longlong()
+    _raise(lib, lib.vfs_vec_length(os.fsencode(path), C.byref(n)))
+    return n.value
+
+
+def vec_read(path, shape=None) -> np.ndarray:
+    """Reads a PETSc binary Vec file; `shape` (if given) must hold exactly its values."""
+    lib = load_library()
+    n = vec_length(path) if shape is None else int(np.prod(shape))
+    out = np.empty(n)
+    _raise(lib, lib.vfs_vec_read(os.fsencode(path), out.ctypes.data, n))
+    return out if shape is None else out.reshape(shape)
+
+
+def grid_dims(path, fmt: str = "text", block: int = 0):
+    """(IM, JM, KM) node counts of one block of a grid.dat / xyz.dat file."""
+    lib = load_library()
+    im, jm, km = C.c_int(), C.c_int(), C.c_int()
+    _raise(lib, lib.vfs_grid_dims(os.fsencode(path), GRID_FORMATS[fmt], block, C.byref(im), C.byref(jm), C.byref(km)))
+    return im.value, jm.value, km.value
+
+
+def grid_read(path, fmt: str = "text", block: int = 0, cl: float = 1.0, L_dim: float = 1.0) -> np.ndarray:
+    """Node coordinates of one block as the DA coordinate array (KM+1, JM+1, IM+1, 3)."""
+    lib = load_library()
+    im, jm, km = grid_dims(path, fmt, block)
+    out = np.zeros((km + 1, jm + 1, im + 1, 3))
+    _raise(lib, lib.vfs_grid_read(os.fsencode(path), GRID_FORMATS[fmt], block, cl, L_dim, im + 1, jm + 1, km + 1,
+                                  out.ctypes.data))
+    return out
 
 
 def default_newton_opts(rtol=1e-4, restart=30, max_newton=50, max_linear=1000, use_feul=0) -> NewtonOpts:
@@ -405,6 +464,13 @@ class Context:
         self._check(self.lib.vfs_fractional_step_cn(self.h, C.byref(nopts or default_newton_opts()),
                                                     C.byref(opts or default_solve_opts()), C.byref(st)))
         return st
+
+    # -- checkpoints (vfield/ufield/pfield/nvfield) -----------------------------
+    def checkpoint_write(self, directory, ti: int, block: int = 0):
+        self._check(self.lib.vfs_checkpoint_write(self.h, os.fsencode(directory), ti, block))
+
+    def checkpoint_read(self, directory, ti: int, block: int = 0, immersed: int = 0):
+        self._check(self.lib.vfs_checkpoint_read(self.h, os.fsencode(directory), ti, block, immersed))
 
     # -- implicit momentum (-imp 4) ---------------------------------------------
     def store_old_state(self):
