@@ -51,6 +51,9 @@ typedef struct {
     int dynamic_freq;          /* -dynamic_freq: steps between dynamic updates (1) */
     int i_homo_filter, j_homo_filter, k_homo_filter;  /* -i/j/k_homo_filter: average
                                   LM, MM over homogeneous directions (les.c:799-958) */
+    int inletprofile;          /* -inlet (main.c:24): profile of the k- inlet (1)  */
+    int pseudo_periodic;       /* -pseudo (main.c:289): the k- inlet takes the
+                                  velocity of plane (6 mz) / 10 (bcs.c:29-95)    */
 } vfs_params;
 
 /* Device-resident arrays of a context. */
@@ -112,8 +115,14 @@ int vfs_contra2cart(vfs_ctx *ctx);
 /* FormBCS (bcs.c:1684-2767) for codes 1 (wall), 10 (slip), 4 (outlet), 5 (uniform inlet,
    whose fluxes vfs_inflow_flux sets) and periodic directions. */
 int vfs_form_bcs(vfs_ctx *ctx, int outflow_scale);
-/* InflowFlux (bcs.c:247-626) for a uniform inlet (-inlet 1, code 5 on the k- face):
- * inlet fluxes and ghost velocities; FluxInSum returned and kept for FormBCS. */
+/* InflowFlux (bcs.c:247-626) for the inlet (code 5) on the k- face with -inlet
+ * 0 and 3 (the constant Flux_in of main.c:13), 1 and 2 (uniform), 10 (power
+ * law), 12 (pipe), 14 (tabulated pipe profile), 15 (round jet), 21
+ * (Poiseuille) and 100 or -pseudo 1 (the inflow plane, vfs_inflow_set_plane /
+ * vfs_pseudo_periodic_plane), with the held bulk flux of -inlet 14, 100 and
+ * -pseudo when -flux is given: inlet fluxes and ghost velocities; FluxInSum
+ * returned and kept for FormBCS.  Profile 11 (time-seeded noise) and the
+ * profiles that leave the inlet velocity unset (13, 16, 20, 101) are refused. */
 int vfs_inflow_flux(vfs_ctx *ctx, double *flux_in);
 /* Compute_Smagorinsky_Constant_1 (les.c:75-1141) from VFS_UCAT into lCs:
  * 0 while ti < 2 on a fresh start (tistart == 0 and no restart), params.cs2
@@ -307,6 +316,37 @@ int vfs_unet_set_params(vfs_unet *net, const float *host);
  * iterations the residual has not dropped below half of its starting value
  * the solve restarts from the guess with the plain V-cycle (stats->fallback = 1). */
 int vfs_poisson_solve_learned(vfs_ctx *ctx, vfs_unet *net, const vfs_solve_opts *opts, vfs_solve_stats *stats);
+
+/* ---- inflow planes (-inlet 100, -pseudo 1) ----------------------------- */
+/* The reference's user->ucont_plane: my x mx Cmpnts of Cartesian velocity
+ * that InflowFlux copies into the k = 0 ghost cells for -inlet 100 and the
+ * pseudo-periodic inlet (bcs.c:379-390).  vfs_inflow_set_plane uploads a
+ * plane (read_inflow_section's result, rhs2.c:766-862); vfs_inflow_get_plane
+ * downloads the current one (for save_inflow_section, rhs2.c:716-764). */
+int vfs_inflow_set_plane(vfs_ctx *ctx, const double *host);
+int vfs_inflow_get_plane(vfs_ctx *ctx, double *host);
+/* pseudo_periodic_BC (bcs.c:29-95): the plane := Ucat of plane k = (6 mz) / 10,
+ * on the device.  The step drivers call it first when -pseudo is set
+ * (solvers.c:355-358). */
+int vfs_pseudo_periodic_plane(vfs_ctx *ctx);
+/* save_inflow_section (rhs2.c:716-764), host side: appends the plane to
+ * dir/inflow_%06d_dt=%g.dat (the file of time index ti; the file is started
+ * afresh when ti - save_inflow_minus == tistart or is 1 modulo the period). */
+int vfs_inflow_write_section(const char *dir, int ti, int tistart, int save_inflow_period, int save_inflow_minus,
+                             double dt, int mx, int my, const double *plane);
+/* read_inflow_section (rhs2.c:766-862), host side: a reader keeps the open
+ * file of path_inflow/inflow/inflow_%06d_dt=%g.dat between calls and returns
+ * the plane of time index ti with the reference's recycling (-recycle period),
+ * file period (-save_inflow_period) and offset (-save_inflow_minus) rules;
+ * the first call must be for ti = tistart, later calls for ti + 1, ti + 2, ...
+ * As in the reference, the plane is zero at ti = 1 of a fresh start, and
+ * values past the end of a file (which a restart not at 1 modulo the file
+ * period reaches, since files are switched only after tistart + 90) are zero. */
+typedef struct vfs_inflow_reader vfs_inflow_reader;
+int vfs_inflow_reader_open(const char *path_inflow, double dt_inflow, int save_inflow_period, int save_inflow_minus,
+                           int recycle_period, int tistart, int mx, int my, vfs_inflow_reader **out);
+int vfs_inflow_reader_next(vfs_inflow_reader *r, int ti, double *plane);
+int vfs_inflow_reader_close(vfs_inflow_reader *r);
 
 /* ---- file formats (host side) ----------------------------------------- */
 /* PETSc binary Vec files as VecView writes them and VecLoadIntoVector reads

@@ -26,6 +26,7 @@ class OrcParams(C.Structure):
         ("inlet_flux", C.c_double), ("mean_k_area", C.c_double), ("mean_k_flux", C.c_double),
         ("poisson_threshold", C.c_double), ("flux_in_sum", C.c_double),
         ("max_cs", C.c_double), ("i_homo", C.c_int), ("j_homo", C.c_int), ("k_homo", C.c_int),
+        ("inletprofile", C.c_int), ("pseudo_periodic", C.c_int), ("inflow_plane", C.c_void_p), ("ti", C.c_int),
     ]
 
 
@@ -107,6 +108,7 @@ def load(path: str | os.PathLike | None = None) -> C.CDLL:
         "orc_dfunc_exp": ([C.c_double, C.c_double], C.c_double),
         "orc_inlet_area": ([P, M], C.c_double),
         "orc_inflow_flux": ([P, M, D, D], C.c_double),
+        "orc_pseudo_periodic_plane": ([P, D, D], None),
         "orc_formfunction_snes": ([P, M, D, D, D, D, D, D, D, D, D, D], None),
         "orc_set_threads": ([C.c_int], C.c_int),
         "orc_arbitrary_rotate": ([D, C.c_double, D, D], None),
@@ -140,7 +142,7 @@ def _ptr(a: np.ndarray):
 
 def make_params(mx, my, mz, *, per=(1, 0, 1), bctype=(100, 100, 1, 1, 100, 100), les=1, second_order=1,
                 laplacian=1, ren=2800.0, dt=1e-3, st=1.0, cs2=0.01, inlet_flux=-1.0, poisson_threshold=0.1,
-                flux_in_sum=0.0, max_cs=0.5, homo=(0, 0, 0)) -> OrcParams:
+                flux_in_sum=0.0, max_cs=0.5, homo=(0, 0, 0), inletprofile=1, pseudo=0) -> OrcParams:
     p = OrcParams()
     p.mx, p.my, p.mz = mx, my, mz
     p.per_i, p.per_j, p.per_k = per
@@ -152,6 +154,9 @@ def make_params(mx, my, mz, *, per=(1, 0, 1), bctype=(100, 100, 1, 1, 100, 100),
     p.mean_k_area, p.mean_k_flux = 0.0, 0.0
     p.max_cs = max_cs
     p.i_homo, p.j_homo, p.k_homo = (int(h) for h in homo)
+    p.inletprofile, p.pseudo_periodic = int(inletprofile), int(pseudo)
+    p.inflow_plane = None
+    p.ti = 0
     return p
 
 
@@ -229,6 +234,19 @@ class Oracle:
         self.lib.orc_runge_kutta(C.byref(self.p), C.byref(self.cm), self._f("ucont"), self._f("ucat"),
                                  self._f("ucat_old"), self._f("nu_t"), self._f("dp"),
                                  C.cast(None, C.POINTER(C.c_double)))
+
+    def set_inflow_plane(self, plane: np.ndarray | None):
+        """user->ucont_plane: (my, mx, 3) Cartesian velocity of the inlet plane."""
+        if plane is None:
+            self._plane, self.p.inflow_plane = None, None
+            return
+        self._plane = np.ascontiguousarray(plane, dtype=np.float64).reshape(self.shape[1], self.shape[2], 3)
+        self.p.inflow_plane = self._plane.ctypes.data
+
+    def pseudo_periodic_plane(self) -> np.ndarray:
+        out = np.zeros((self.shape[1], self.shape[2], 3))
+        self.lib.orc_pseudo_periodic_plane(C.byref(self.p), self._f("ucat"), _ptr(out))
+        return out
 
     def inflow_flux(self):
         return self.lib.orc_inflow_flux(C.byref(self.p), C.byref(self.cm), self._f("ucont"), self._f("ucat"))

@@ -322,6 +322,29 @@ void orc_contra2cart(const orc_params *p, const orc_metrics *m, double *ucont, d
                     for (int c = 0; c < 3; c++) u[c] = -VC(L2, c, k - 1, j, i);
                     solid_flag = 1;
                 }
+                /* lid-driven cavity j+, rhs.c:593-605: the wall moves with u = 1 */
+                if (j == my - 1 && bc[3] == 2) {
+                    const double *nb = solid_flag ? &VX(ucat_old, k, j - 1, i) : &VX(L2, k, j - 1, i);
+                    u[0] = 2.0 - nb[0];
+                    u[1] = -nb[1];
+                    u[2] = -nb[2];
+                }
+                /* Couette walls j- and j+, rhs.c:608-624: the wall moves with w = 1 */
+                if (j == 0 && bc[2] == 12) {
+                    u[0] = -VX(L2, k, j + 1, i);
+                    u[1] = -VY(L2, k, j + 1, i);
+                    u[2] = 2.0 - VZ(L2, k, j + 1, i);
+                    if (solid_flag) u[0] = u[1] = u[2] = 0;
+                }
+                if (j == my - 1 && bc[3] == 12) {
+                    u[0] = -VX(L2, k, j - 1, i);
+                    u[1] = -VY(L2, k, j - 1, i);
+                    u[2] = 2.0 - VZ(L2, k, j - 1, i);
+                    if (solid_flag) u[0] = u[1] = u[2] = 0;
+                }
+                /* outflow j+, rhs.c:640-642 */
+                if (bc[3] == 4 && j == my - 1 && (i != 0 && i != mx - 1 && k != 0 && k != mz - 1))
+                    for (int c = 0; c < 3; c++) u[c] = VC(L2, c, k, j - 1, i);
                 /* outlet k+, rhs.c:641-660 */
                 if (bc[5] == 4 && k == mz - 1 && (i != 0 && i != mx - 1 && j != 0 && j != my - 1)) {
                     if (S(nvert, k - 1, j, i) > 0.1) u[0] = u[1] = u[2] = 0;
@@ -349,6 +372,95 @@ void orc_form_bcs(const orc_params *p, const orc_metrics *m, double *ucont, doub
     const double *nvert = m->nvert;
     const double threshold = 0.1; /* bcs.c:11 */
     orc_contra2cart(p, m, ucont, ucat, ucat_old);
+    const int *bc = p->bctype;
+
+    /* far field (code 6), bcs.c:1745-1993, no frame motion: the boundary
+     * fluxes from the adjacent cell's velocity; with a far-field k+ face the
+     * in/out flux difference is spread over the far-field faces */
+    double FarFluxIn = 0, FarFluxOut = 0, FarAreaIn = 0, FarAreaOut = 0;
+    if (bc[0] == 6)
+        for (int k = 1; k < mz - 1; k++)
+            for (int j = 1; j < my - 1; j++) {
+                VX(ucont, k, j, 0) = VX(ucat, k, j, 1) * VX(m->csi, k, j, 1);
+                FarFluxIn += VX(ucont, k, j, 0);
+                FarAreaIn += VX(m->csi, k, j, 0);
+            }
+    if (bc[1] == 6)
+        for (int k = 1; k < mz - 1; k++)
+            for (int j = 1; j < my - 1; j++) {
+                VX(ucont, k, j, mx - 2) = VX(ucat, k, j, mx - 2) * VX(m->csi, k, j, mx - 2);
+                FarFluxOut += VX(ucont, k, j, mx - 2);
+                FarAreaOut += VX(m->csi, k, j, mx - 2);
+            }
+    if (bc[2] == 6)
+        for (int k = 1; k < mz - 1; k++)
+            for (int i = 1; i < mx - 1; i++) {
+                VY(ucont, k, 0, i) = VY(ucat, k, 1, i) * VY(m->eta, k, 1, i);
+                FarFluxIn += VY(ucont, k, 0, i);
+                FarAreaIn += VY(m->eta, k, 0, i);
+            }
+    if (bc[3] == 6)
+        for (int k = 1; k < mz - 1; k++)
+            for (int i = 1; i < mx - 1; i++) {
+                VY(ucont, k, my - 2, i) = VY(ucat, k, my - 2, i) * VY(m->eta, k, my - 2, i);
+                FarFluxOut += VY(ucont, k, my - 2, i);
+                FarAreaOut += VY(m->eta, k, my - 2, i);
+            }
+    if (bc[4] == 6)
+        for (int j = 1; j < my - 1; j++)
+            for (int i = 1; i < mx - 1; i++) {
+                VZ(ucont, 0, j, i) = VZ(ucat, 1, j, i) * VZ(m->zet, 1, j, i);
+                FarFluxIn += VZ(ucont, 0, j, i);
+                FarAreaIn += VZ(m->zet, 0, j, i);
+            }
+    if (bc[5] == 6)
+        for (int j = 1; j < my - 1; j++)
+            for (int i = 1; i < mx - 1; i++) {
+                VZ(ucont, mz - 2, j, i) = VZ(ucat, mz - 2, j, i) * VZ(m->zet, mz - 2, j, i);
+                FarFluxOut += VZ(ucont, mz - 2, j, i);
+                FarAreaOut += VZ(m->zet, mz - 2, j, i);
+            }
+    if (bc[5] == 6) {
+        const double FluxDiff = 0.5 * (FarFluxIn - FarFluxOut);
+        double VelDiffIn = FluxDiff / FarAreaIn, VelDiffOut = FluxDiff / FarAreaOut;
+        if (fabs(FluxDiff) < 1.e-6 || fabs(FarAreaIn) < 1.e-6) VelDiffIn = 0.;
+        if (fabs(FluxDiff) < 1.e-6 || fabs(FarAreaOut) < 1.e-6) VelDiffOut = 0.;
+        for (int j = 1; j < my - 1; j++)
+            for (int i = 1; i < mx - 1; i++)
+                VZ(ucont, mz - 2, j, i) = (VZ(ucat, mz - 2, j, i) + VelDiffOut) * VZ(m->zet, mz - 2, j, i);
+        if (bc[3] == 6)
+            for (int k = 1; k < mz - 1; k++)
+                for (int i = 1; i < mx - 1; i++)
+                    VY(ucont, k, my - 2, i) = (VY(ucat, k, my - 2, i) + VelDiffOut) * VY(m->eta, k, my - 2, i);
+        if (bc[1] == 6)
+            for (int k = 1; k < mz - 1; k++)
+                for (int j = 1; j < my - 1; j++)
+                    VX(ucont, k, j, mx - 2) = (VX(ucat, k, j, mx - 2) + VelDiffOut) * VX(m->csi, k, j, mx - 2);
+        if (bc[0] == 6)
+            for (int k = 1; k < mz - 1; k++)
+                for (int j = 1; j < my - 1; j++) VX(ucont, k, j, 0) = (VX(ucat, k, j, 1) - VelDiffIn) * VX(m->csi, k, j, 1);
+        if (bc[2] == 6)
+            for (int k = 1; k < mz - 1; k++)
+                for (int i = 1; i < mx - 1; i++) VY(ucont, k, 0, i) = (VY(ucat, k, 1, i) - VelDiffIn) * VY(m->eta, k, 1, i);
+        if (bc[4] == 6)
+            for (int j = 1; j < my - 1; j++)
+                for (int i = 1; i < mx - 1; i++) VZ(ucont, 0, j, i) = (VZ(ucat, 1, j, i) - VelDiffIn) * VZ(m->zet, 1, j, i);
+    }
+
+    /* characteristic outlet (code 8) on k+, bcs.c:1997-2041: after the first
+     * two steps the outlet fluxes are scaled to carry the inflow */
+    if (bc[5] == 8) {
+        double FluxOut = 0;
+        for (int j = 1; j < my - 1; j++)
+            for (int i = 1; i < mx - 1; i++) FluxOut += VZ(ucont, mz - 2, j, i);
+        const double FluxIn = p->flux_in_sum + FarFluxIn;
+        double ratio = FluxIn / FluxOut;
+        if (fabs(FluxOut) < 1.e-6) ratio = 1.;
+        if (fabs(FluxIn) < 1.e-6) ratio = 0.;
+        if (!(p->ti == 0 || p->ti == 1))
+            for (int j = 1; j < my - 1; j++)
+                for (int i = 1; i < mx - 1; i++) VZ(ucont, mz - 2, j, i) = VZ(ucont, mz - 2, j, i) * ratio;
+    }
 
     /* outlet flux balance, bcs.c:2127-2260 */
     if (p->bctype[5] == 4) {
@@ -370,6 +482,21 @@ void orc_form_bcs(const orc_params *p, const orc_metrics *m, double *ucont, doub
                         VZ(ucont, k - 1, j, i) += (p->flux_in_sum - FluxOut) * Area / lArea;
                 }
         }
+    }
+    /* outlet (code 4) on k-, bcs.c:2294-2340 */
+    if (bc[4] == 4) {
+        double FluxOut = 0, lArea = 0;
+        for (int j = 1; j < my - 1; j++)
+            for (int i = 1; i < mx - 1; i++)
+                if (S(nvert, 0, j, i) < threshold) {
+                    FluxOut += (VX(ucat, 1, j, i) * (VX(m->zet, 0, j, i) + VX(m->zet, 1, j, i)) +
+                                VY(ucat, 1, j, i) * (VY(m->zet, 0, j, i) + VY(m->zet, 1, j, i)) +
+                                VZ(ucat, 1, j, i) * (VZ(m->zet, 0, j, i) + VZ(m->zet, 1, j, i))) * 0.5;
+                    lArea += VZ(m->zet, 0, j, i);
+                }
+        const double ratio = (p->flux_in_sum - FluxOut) / lArea;
+        for (int j = 1; j < my - 1; j++)
+            for (int i = 1; i < mx - 1; i++) VZ(ucont, 0, j, i) = (VZ(ucat, 1, j, i) + ratio) * VZ(m->zet, 0, j, i);
     }
     /* wall normal flux, bcs.c:2347-2386 */
     if (p->bctype[0] == 1 || p->bctype[0] == 10)
@@ -623,31 +750,122 @@ double orc_inlet_area(const orc_params *p, const orc_metrics *m)
     return a;
 }
 
-/* Sets the inlet fluxes ucont(0,j,i).z = uin |zet| (uin = 1 when no -flux is
- * given, inlet_flux / inletArea otherwise), the ghost velocity
- * ucat(0) = 2 u - ucat(1) with u from Contra2Cart_single, and returns FluxInSum. */
+/* InletInterpolation (bcs.c:636-681) returns the constant Flux_in of main.c:13
+ * before its table code is reached. */
+#define ORC_FLUX_IN 4.104388e-04
+
+/* uin of bcs.c:308-371 for the k- inlet cell whose first cell has face
+ * centre (xc, yc) and radius r; *ok = 0 for a profile the oracle does not
+ * restate (11 adds time-seeded noise; 13, 16, 20, 101 leave uin unset). */
+static double inlet_profile(const orc_params *p, double inlet_area, double xc, double yc, double r, int *ok)
+{
+    (void)xc;
+    *ok = 1;
+    switch (p->inletprofile) {
+    case 0:
+    case 3: return ORC_FLUX_IN;
+    case 1:
+    case 2: return p->inlet_flux < 0 ? 1. : p->inlet_flux / inlet_area;
+    case 10: {
+        const double delta = 0.45263, a = 5.99;
+        if (yc >= delta) return 1.0;
+        if (yc <= 0) return 0.0;
+        return pow(yc / delta, 1. / a);
+    }
+    case 12: return 2 * (1 - pow(r / 0.5, 2.0));
+    case 14: {
+        static const double R[11] = {0.000, 0.062, 0.125, 0.188, 0.250, 0.312, 0.354, 0.399, 0.438, 0.469, 0.500};
+        static const double W[11] = {1.120, 1.115, 1.110, 1.093, 1.050, 1.010, 0.963, 0.915, 0.825, 0.730, 0.000};
+        int ii;
+        for (ii = 1; ii < 11; ii++)
+            if (R[ii] >= r && R[ii - 1] < r) break;
+        if (ii == 11) ii = 10;  /* r = 0 or r > 0.5 (the value is replaced below for r > 0.5) */
+        double u = (W[ii] - W[ii - 1]) / (R[ii] - R[ii - 1]) * (r - R[ii - 1]) + W[ii - 1];
+        return r > 0.5 ? 0 : u;
+    }
+    case 15: return r > 0.5 ? 0 : 1;
+    case 21: return 1.5 * 1.0 * yc * (2 - yc);
+    case 100: return 0;  /* the plane data are used instead */
+    default: *ok = 0; return 0;
+    }
+}
+
+/* InflowFlux (bcs.c:247-626) for the k- inlet (bctype 5) with -inlet 0, 1, 2,
+ * 3, 10, 12, 14, 15, 21 and 100 (plane data p->inflow_plane, my x mx
+ * Cmpnts of Cartesian velocity) and the pseudo-periodic inlet.  Returns
+ * FluxInSum (the previous value when no k- inlet is configured). */
 double orc_inflow_flux(const orc_params *p, const orc_metrics *m, double *ucont, double *ucat)
 {
-    if (p->bctype[4] != 5) return p->flux_in_sum;
-    const double uin = p->inlet_flux < 0 ? 1. : p->inlet_flux / orc_inlet_area(p, m);
-    double flux = 0;
-    for (int j = 1; j < p->my - 1; j++)
-        for (int i = 1; i < p->mx - 1; i++) {
-            const int k = 0;
-            double area = sqrt(VX(m->zet, k, j, i) * VX(m->zet, k, j, i) + VY(m->zet, k, j, i) * VY(m->zet, k, j, i) +
-                               VZ(m->zet, k, j, i) * VZ(m->zet, k, j, i));
-            if (S(m->nvert, k + 1, j, i) < 0.1) {
-                VZ(ucont, k, j, i) = uin * area;
-                VX(ucont, k, j, i) = 0;
-                VY(ucont, k, j, i) = 0;
-                double q[3] = {0, 0, VZ(ucont, k, j, i)}, u[3];
-                cramer(&VX(m->csi, k, j, i), &VX(m->eta, k, j, i), &VX(m->zet, k, j, i), q, u);
-                for (int c = 0; c < 3; c++) VC(ucat, c, k, j, i) = -VC(ucat, c, k + 1, j, i) + 2 * u[c];
-                flux += VZ(ucont, k, j, i);
-            } else {
-                VZ(ucat, k, j, i) = 0;
-                VZ(ucont, k, j, i) = 0;
+    const int mx = p->mx, my = p->my;
+    const double *nv = m->nvert, *coor = m->coor;
+    const double th = 0.1;
+    double FluxIn = 0;
+    if (p->bctype[4] == 5) {
+        const int plane = p->pseudo_periodic || p->inletprofile == 100;
+        if (plane && !p->inflow_plane) { fprintf(stderr, "oracle: inlet plane data missing\n"); abort(); }
+        const double area_in = orc_inlet_area(p, m);
+        double flux0 = 0, flux1 = 0;
+        const int k = 0;
+        for (int j = 1; j < my - 1; j++)
+            for (int i = 1; i < mx - 1; i++) {
+                double xc = 0, yc = 0;
+                if (coor) {
+                    xc = (VX(coor, k + 1, j, i) + VX(coor, k + 1, j - 1, i) + VX(coor, k + 1, j, i - 1) + VX(coor, k + 1, j - 1, i - 1)) * 0.25;
+                    yc = (VY(coor, k + 1, j, i) + VY(coor, k + 1, j - 1, i) + VY(coor, k + 1, j, i - 1) + VY(coor, k + 1, j - 1, i - 1)) * 0.25;
+                }
+                const double r = sqrt(xc * xc + yc * yc);
+                const double area = sqrt(VX(m->zet, k, j, i) * VX(m->zet, k, j, i) + VY(m->zet, k, j, i) * VY(m->zet, k, j, i) +
+                                          VZ(m->zet, k, j, i) * VZ(m->zet, k, j, i));
+                int ok;
+                const double uin = inlet_profile(p, area_in, xc, yc, r, &ok);
+                if (!ok) { fprintf(stderr, "oracle: -inlet %d is not restated\n", p->inletprofile); abort(); }
+                if (S(nv, k + 1, j, i) < th) {
+                    if (plane) {
+                        const double *pl = p->inflow_plane + 3 * ((size_t)j * mx + i);
+                        for (int c = 0; c < 3; c++) VC(ucat, c, k, j, i) = pl[c];
+                        VZ(ucont, k, j, i) = 0.5 * (VX(ucat, k, j, i) + VX(ucat, k + 1, j, i)) * VX(m->zet, k, j, i) +
+                                             0.5 * (VY(ucat, k, j, i) + VY(ucat, k + 1, j, i)) * VY(m->zet, k, j, i) +
+                                             0.5 * (VZ(ucat, k, j, i) + VZ(ucat, k + 1, j, i)) * VZ(m->zet, k, j, i);
+                    } else {
+                        VZ(ucont, k, j, i) = uin * area;
+                        VX(ucont, k, j, i) = 0;
+                        VY(ucont, k, j, i) = 0;
+                        double q[3] = {0, 0, VZ(ucont, k, j, i)}, u[3];
+                        cramer(&VX(m->csi, k, j, i), &VX(m->eta, k, j, i), &VX(m->zet, k, j, i), q, u);
+                        for (int c = 0; c < 3; c++) VC(ucat, c, k, j, i) = -VC(ucat, c, k + 1, j, i) + 2 * u[c];
+                    }
+                } else {
+                    VZ(ucat, k, j, i) = 0;
+                    VZ(ucont, k, j, i) = 0;
+                }
+                flux0 += VZ(ucont, k, j, i);
+                flux1 += VZ(ucont, k, j, i);
             }
-        }
-    return flux;
+        /* bcs.c:478-496: a held flux for the plane / pseudo-periodic / pipe inlets */
+        const int held = p->pseudo_periodic || p->inletprofile == 100 || p->inletprofile == 14;
+        if (held) flux0 = p->inlet_flux;
+        if (held && p->inlet_flux > 0)
+            for (int j = 1; j < my - 1; j++)
+                for (int i = 1; i < mx - 1; i++) {
+                    const double A = sqrt(VX(m->zet, k, j, i) * VX(m->zet, k, j, i) + VY(m->zet, k, j, i) * VY(m->zet, k, j, i) +
+                                          VZ(m->zet, k, j, i) * VZ(m->zet, k, j, i));
+                    if (S(nv, k + 1, j, i) < th) VZ(ucont, k, j, i) += (flux0 - flux1) * A / area_in;
+                }
+        for (int j = 1; j < my - 1; j++)
+            for (int i = 1; i < mx - 1; i++)
+                if (S(nv, k + 1, j, i) < th) FluxIn += VZ(ucont, k, j, i);
+    } else {
+        return p->flux_in_sum;
+    }
+    return FluxIn;
+}
+
+/* pseudo_periodic_BC (bcs.c:29-95): the Cartesian velocity of plane
+ * kplane = (6 mz) / 10 as the next inlet plane (my x mx Cmpnts). */
+void orc_pseudo_periodic_plane(const orc_params *p, const double *ucat, double *plane)
+{
+    const int kplane = (p->mz * 6) / 10;
+    for (int j = 0; j < p->my; j++)
+        for (int i = 0; i < p->mx; i++)
+            for (int c = 0; c < 3; c++) plane[3 * ((size_t)j * p->mx + i) + c] = VC(ucat, c, kplane, j, i);
 }

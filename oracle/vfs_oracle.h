@@ -42,6 +42,13 @@ typedef struct {
     double flux_in_sum;             /* FluxInSum from InflowFlux (outlet scaling)*/
     double max_cs;                  /* -max_cs (main.c:308, 1728), les == 2      */
     int i_homo, j_homo, k_homo;     /* -i/j/k_homo_filter (main.c:324-326)       */
+    int inletprofile;               /* -inlet (main.c:24, 1383), default 1      */
+    int pseudo_periodic;            /* -pseudo (main.c:289, 1419)               */
+    const double *inflow_plane;     /* user->ucont_plane (my x mx Cmpnts of
+                                       Cartesian velocity) for -inlet 100 and
+                                       the pseudo-periodic inlet              */
+    int ti;                         /* time index (the characteristic outlet
+                                       scales its fluxes from ti = 2 on)       */
 } orc_params;
 
 /* Metric set as the momentum/Poisson kernels read it (lCsi ... lKAj after the
@@ -161,9 +168,13 @@ double orc_dfunc_s3h(double r);
 double orc_dfunc_s4h(double r);
 double orc_dfunc_exp(double r, double n);
 
-/* InflowFlux (bcs.c:247-626), uniform inlet on the k- face: returns FluxInSum */
+/* InflowFlux (bcs.c:247-626): the k- inlet with -inlet 0, 1, 2, 3, 10, 12, 14,
+ * 15, 21, 100 and the pseudo-periodic inlet; returns FluxInSum.
+ * Calc_Inlet_Area's inletArea (bcs.c:201-234). */
 double orc_inlet_area(const orc_params *p, const orc_metrics *m);
 double orc_inflow_flux(const orc_params *p, const orc_metrics *m, double *ucont, double *ucat);
+/* pseudo_periodic_BC (bcs.c:29-95): ucat of plane (6 mz) / 10 into plane. */
+void orc_pseudo_periodic_plane(const orc_params *p, const double *ucat, double *plane);
 
 /* a7b: CN residual, FormFunction_SNES (momentum.c:2237-2336), time_coeff() == 1 */
 void orc_formfunction_snes(const orc_params *p, const orc_metrics *m, const double *u, const double *ucont_o,

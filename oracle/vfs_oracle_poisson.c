@@ -916,6 +916,14 @@ int orc_fractional_step(orc_params *p, const orc_metrics *m, double *ucont, doub
                         double tol, double *maxdiv, double *relres)
 {
     size_t n = npts(p);
+    /* pseudo_periodic_BC, solvers.c:355-358: this step's inlet plane */
+    const double *plane_in = p->inflow_plane;
+    double *plane = NULL;
+    if (p->pseudo_periodic) {
+        plane = dalloc(3 * (size_t)p->mx * p->my);
+        orc_pseudo_periodic_plane(p, ucat, plane);
+        p->inflow_plane = plane;
+    }
     /* Calc_k_Flux, solvers.c:363 */
     p->mean_k_flux = orc_calc_k_flux(p, m, ucont);
     /* LES, solvers.c:365-371 */
@@ -945,5 +953,6 @@ int orc_fractional_step(orc_params *p, const orc_metrics *m, double *ucont, doub
     orc_projection(p, m, phi, ucont, ucat, ucat_old);
     /* solvers.c:684 */
     if (maxdiv) *maxdiv = orc_divergence(p, m, ucont);
+    if (plane) { p->inflow_plane = plane_in; free(plane); }
     return it;
 }

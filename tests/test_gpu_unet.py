@@ -101,3 +101,79 @@ def test_learned_poisson_solve_reaches_tolerance():
     res = r - o.poisson_apply(x)
     assert np.linalg.norm(res) <= 1.01e-8 * np.linalg.norm(r)
     net.close(); ctx.close()
+
+
+@pytest.mark.parametrize("fill", [np.nan, 0.0], ids=["nan-weights", "zero-weights"])
+def test_learned_solve_falls_back(fill):
+    """A network that cannot help (non-finite or all-zero weights) triggers the
+    fallback: after three iterations the residual has not halved, the solve
+    restarts from the original guess with the plain V-cycle, reports
+    fallback = 1 and still reaches the tolerance under the oracle's operator."""
+    c, o, ctx, net, orc, b, phi = _setup()
+    p = net.params(0)
+    p[:] = fill
+    net.set_params(p)
+    ctx.runge_kutta()
+    ctx.poisson_rhs()
+    ctx.upload("PHI", np.zeros(ctx.field_shape("PHI")))
+    for fp32 in (1, 0):
+        ctx.upload("PHI", np.zeros(ctx.field_shape("PHI")))
+        st = net.poisson_solve(vfs.default_solve_opts(tol=1e-8, mg_fp32=fp32, use_guess=0))
+        assert st.fallback == 1 and st.relres <= 1e-8
+        x, r = ctx.download("PHI"), ctx.download("B")
+        assert np.all(np.isfinite(x))
+        o.set("b", r)
+        assert np.linalg.norm(r - o.poisson_apply(x)) <= 1.01e-8 * np.linalg.norm(r)
+    net.close(); ctx.close()
+
+
+def test_learned_stage_on_held_out_seed():
+    """The shipped network on the 256x192x256 channel of a seed that neither
+    training nor validation used, three steps into the run: the warm start
+    x0 = g + alpha net(b - A g) from the previous step's correction g lowers
+    the residual below that of g alone, and the learned FGMRES reaches 1e-8
+    without falling back.  Prints the residual ratio and the iterations and
+    times of the solves with and without the network."""
+    import importlib
+
+    learned = importlib.import_module("vfs-wind_amd.learned")
+    ctx = learned.channel_context(256, 192, 256, learned.HELD_OUT_SEED)
+    with ctx:
+        opts = vfs.default_solve_opts(tol=1e-8)
+        for _ in range(3):
+            ctx.fractional_step(opts)
+        # the next step's right-hand side; PHI holds the previous correction (the guess)
+        ctx.calc_k_flux()
+        ctx.contra2cart()
+        ctx.eddy_viscosity()
+        ctx.pressure_gradient()
+        ctx.runge_kutta()
+        ctx.poisson_rhs()
+        n = int(np.prod(ctx.field_shape("PHI")))
+        b, g = ctx.download("B"), ctx.download("PHI")
+
+        def apply(x):
+            ctx.upload("PHI", x)
+            ctx.poisson_apply_dev(ctx.device_ptr("PHI"), ctx.device_ptr("RHS"))
+            return ctx.download("RHS").reshape(-1)[:n].reshape(x.shape)
+
+        r = b - apply(g)
+        net = learned.make_net(ctx)
+        ctx.upload("B", r)
+        net.forward(ctx.device_ptr("B"), ctx.device_ptr("PHI"))
+        y = ctx.download("PHI")
+        Ay = apply(y)
+        alpha = float(np.vdot(r, Ay) / np.vdot(Ay, Ay))
+        ratio = float(np.linalg.norm(r - alpha * Ay) / np.linalg.norm(r))
+        res = {}
+        for name, nn in (("fp32_mg", None), ("fp32_mg_net", net)):
+            ctx.upload("B", b)
+            ctx.upload("PHI", g)
+            ctx.sync()
+            o = vfs.default_solve_opts(tol=1e-8, mg_fp32=1, use_guess=1)
+            st = nn.poisson_solve(o) if nn is not None else ctx.poisson_solve(o)
+            res[name] = (st.iterations, round(st.ms, 2), st.fallback, st.relres)
+        print(f"[held-out] warm-start residual ratio {ratio:.3e} (alpha {alpha:.3e}); solves {res}", flush=True)
+        assert ratio < 1.0
+        assert res["fp32_mg_net"][2] == 0 and res["fp32_mg_net"][3] <= 1e-8
+        net.close()

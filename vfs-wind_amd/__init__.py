@@ -48,6 +48,7 @@ class Params(C.Structure):
         ("poisson_threshold", C.c_double), ("flux_in_sum", C.c_double),
         ("max_cs", C.c_double), ("dynamic_freq", C.c_int),
         ("i_homo_filter", C.c_int), ("j_homo_filter", C.c_int), ("k_homo_filter", C.c_int),
+        ("inletprofile", C.c_int), ("pseudo_periodic", C.c_int),
     ]
 
 
@@ -171,6 +172,15 @@ _SIGS = {
     "vfs_unet_get_params": ([C.c_void_p, C.c_int, C.c_void_p], C.c_int),
     "vfs_unet_set_params": ([C.c_void_p, C.c_void_p], C.c_int),
     "vfs_poisson_solve_learned": ([C.c_void_p, C.c_void_p, C.POINTER(SolveOpts), C.POINTER(SolveStats)], C.c_int),
+    "vfs_inflow_set_plane": ([C.c_void_p, C.c_void_p], C.c_int),
+    "vfs_inflow_get_plane": ([C.c_void_p, C.c_void_p], C.c_int),
+    "vfs_pseudo_periodic_plane": ([C.c_void_p], C.c_int),
+    "vfs_inflow_write_section": ([C.c_char_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_int, C.c_int,
+                                  C.c_void_p], C.c_int),
+    "vfs_inflow_reader_open": ([C.c_char_p, C.c_double, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                C.POINTER(C.c_void_p)], C.c_int),
+    "vfs_inflow_reader_next": ([C.c_void_p, C.c_int, C.c_void_p], C.c_int),
+    "vfs_inflow_reader_close": ([C.c_void_p], C.c_int),
     "vfs_vec_write": ([C.c_char_p, C.c_void_p, C.c_longlong], C.c_int),
     "vfs_vec_read": ([C.c_char_p, C.c_void_p, C.c_longlong], C.c_int),
     "vfs_vec_length": ([C.c_char_p, C.POINTER(C.c_longlong)], C.c_int),
@@ -255,6 +265,44 @@ def vec_read(path, shape=None) -> np.ndarray:
     return out if shape is None else out.reshape(shape)
 
 
+def inflow_write_section(directory, ti: int, tistart: int, period: int, minus: int, dt: float, plane: np.ndarray):
+    """save_inflow_section: appends a (my, mx, 3) plane to directory/inflow_%06d_dt=%g.dat."""
+    lib = load_library()
+    a = np.ascontiguousarray(plane, dtype=np.float64)
+    my, mx = a.shape[:2]
+    _raise(lib, lib.vfs_inflow_write_section(os.fsencode(directory), ti, tistart, period, minus, dt, mx, my,
+                                             a.ctypes.data))
+
+
+class InflowReader:
+    """read_inflow_section's reader: planes of path_inflow/inflow/inflow_%06d_dt=%g.dat."""
+
+    def __init__(self, path_inflow, dt_inflow: float, period: int, minus: int, recycle: int, tistart: int, mx: int,
+                 my: int):
+        self.lib = load_library()
+        self.shape = (my, mx, 3)
+        h = C.c_void_p()
+        _raise(self.lib, self.lib.vfs_inflow_reader_open(os.fsencode(path_inflow), dt_inflow, period, minus, recycle,
+                                                         tistart, mx, my, C.byref(h)))
+        self.h = h
+
+    def next(self, ti: int) -> np.ndarray:
+        out = np.empty(self.shape)
+        _raise(self.lib, self.lib.vfs_inflow_reader_next(self.h, ti, out.ctypes.data))
+        return out
+
+    def close(self):
+        if self.h:
+            self.lib.vfs_inflow_reader_close(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def grid_dims(path, fmt: str = "text", block: int = 0):
     """(IM, JM, KM) node counts of one block of a grid.dat / xyz.dat file."""
     lib = load_library()
@@ -281,7 +329,8 @@ def default_newton_opts(rtol=1e-4, restart=30, max_newton=50, max_linear=1000, u
 
 def make_params(mx, my, mz, *, per=(1, 0, 1), bctype=(100, 100, 1, 1, 100, 100), les=1, second_order=1,
                 laplacian=1, ren=2800.0, dt=1e-3, st=1.0, cs2=0.01, inlet_flux=-1.0,
-                poisson_threshold=0.1, flux_in_sum=0.0, max_cs=0.5, dynamic_freq=1, homo=(0, 0, 0)) -> Params:
+                poisson_threshold=0.1, flux_in_sum=0.0, max_cs=0.5, dynamic_freq=1, homo=(0, 0, 0), inletprofile=1,
+                pseudo=0) -> Params:
     p = Params()
     p.mx, p.my, p.mz = mx, my, mz
     p.per_i, p.per_j, p.per_k = per
@@ -292,6 +341,7 @@ def make_params(mx, my, mz, *, per=(1, 0, 1), bctype=(100, 100, 1, 1, 100, 100),
     p.inlet_flux, p.poisson_threshold, p.flux_in_sum = inlet_flux, poisson_threshold, flux_in_sum
     p.max_cs, p.dynamic_freq = max_cs, dynamic_freq
     p.i_homo_filter, p.j_homo_filter, p.k_homo_filter = (int(h) for h in homo)
+    p.inletprofile, p.pseudo_periodic = int(inletprofile), int(pseudo)
     return p
 
 
@@ -464,6 +514,20 @@ class Context:
         self._check(self.lib.vfs_fractional_step_cn(self.h, C.byref(nopts or default_newton_opts()),
                                                     C.byref(opts or default_solve_opts()), C.byref(st)))
         return st
+
+    # -- inflow planes ------------------------------------------------------------
+    def inflow_set_plane(self, plane: np.ndarray):
+        a = np.ascontiguousarray(plane, dtype=np.float64)
+        assert a.size == 3 * self.params.mx * self.params.my
+        self._check(self.lib.vfs_inflow_set_plane(self.h, a.ctypes.data))
+
+    def inflow_get_plane(self) -> np.ndarray:
+        out = np.empty((self.params.my, self.params.mx, 3))
+        self._check(self.lib.vfs_inflow_get_plane(self.h, out.ctypes.data))
+        return out
+
+    def pseudo_periodic_plane(self):
+        self._check(self.lib.vfs_pseudo_periodic_plane(self.h))
 
     # -- checkpoints (vfield/ufield/pfield/nvfield) -----------------------------
     def checkpoint_write(self, directory, ti: int, block: int = 0):

@@ -121,12 +121,14 @@ def channel(nx: int, ny: int, nz: int, seed: int = SEED, beta: float = 2.0, bump
 
 
 def params_for(mod, case: Case, dt: float, les: int = 1, second_order: int = 1, laplacian: int = 1,
-               cs2: float = 0.01, max_cs: float = 0.5, homo=(0, 0, 0)):
+               cs2: float = 0.01, max_cs: float = 0.5, homo=(0, 0, 0), inletprofile: int = 1, pseudo: int = 0,
+               inlet_flux: float | None = None):
     """vfs_params of a case (mod: the vfs-wind_amd module or the oracle binding)."""
     mx, my, mz = case.da
     return mod.make_params(mx, my, mz, per=case.per, bctype=case.bctype, les=les, second_order=second_order,
-                           laplacian=laplacian, ren=case.ren, dt=dt, st=1.0, cs2=cs2, inlet_flux=case.inlet_flux,
-                           max_cs=max_cs, homo=homo)
+                           laplacian=laplacian, ren=case.ren, dt=dt, st=1.0, cs2=cs2,
+                           inlet_flux=case.inlet_flux if inlet_flux is None else inlet_flux,
+                           max_cs=max_cs, homo=homo, inletprofile=inletprofile, pseudo=pseudo)
 
 
 def box(nx: int, ny: int, nz: int, h: float = 1.0, bump: float = 0.0, seed: int = SEED, noise: float = 0.05) -> dict:

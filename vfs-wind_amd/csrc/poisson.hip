@@ -1610,8 +1610,10 @@ static int fgmres(vfs_ctx *c, const vfs_solve_opts *o, vfs_unet *net, vfs_solve_
     const unsigned nb = blocks_for(n, PTPB);
     double *x = c->f[VFS_PHI], *b = c->f[VFS_B];
     double *V = ps->V, *Z = ps->Z, *w = ps->w;
-    // the FP32 V-cycle's outputs are kept as FP32 vectors in Z's memory (exact)
-    const bool z32 = o->mg_fp32 && !net;
+    // the FP32 V-cycle's outputs are kept as FP32 vectors in Z's memory (exact);
+    // the network's direction is rounded to FP32 as well, and A is applied to
+    // the rounded vector, so the FGMRES relation holds for the stored z exactly
+    const bool z32 = o->mg_fp32 != 0;
     float *Z32 = reinterpret_cast<float *>(Z);
     if (!o->use_guess) hipLaunchKernelGGL(k_zero<double>, dim3(nb), dim3(PTPB), 0, st, n, x);
     double d[MAXV + 1];
@@ -1665,8 +1667,12 @@ static int fgmres(vfs_ctx *c, const vfs_solve_opts *o, vfs_unet *net, vfs_solve_
         for (jj = 0; jj < m && it < maxit; jj++, it++) {
             double *vj = V + (size_t)jj * n, *zj = Z + (size_t)jj * n;
             float *zj32 = Z32 + (size_t)jj * n;
-            rc = net && it == 0 ? vfs_unet_forward(net, vj, zj)
-                                : precondition(c, o, vj, zj, z32 ? zj32 : nullptr, z32 && jj > 0);
+            if (net && it == 0) {
+                rc = vfs_unet_forward(net, vj, z32 ? w : zj);
+                if (!rc && z32) hipLaunchKernelGGL((k_convert<double, float>), dim3(nb), dim3(PTPB), 0, st, n, (const double *)w, zj32);
+            } else {
+                rc = precondition(c, o, vj, zj, z32 ? zj32 : nullptr, z32 && jj > 0);
+            }
             if (rc) return rc;
             prof_begin(c, K_APPLY64);
             if (z32)
@@ -1708,7 +1714,8 @@ static int fgmres(vfs_ctx *c, const vfs_solve_opts *o, vfs_unet *net, vfs_solve_
             if (trace) std::fprintf(stderr, "  it %d: est relres %.3e  h_{j+1,j} %.3e  ww0 %.3e ww %.3e\n", it,
                                     std::fabs(gv[jj + 1]) / bnorm, hn, ww0, ww);
             if (std::fabs(gv[jj + 1]) / bnorm <= tol) { jj++; it++; break; }
-            if (net && it == 2 && std::fabs(gv[jj + 1]) / bnorm > 0.5 * rel_start) {
+            // (written so that a NaN residual, from a network that returns non-finite values, also falls back)
+            if (net && it == 2 && !(std::fabs(gv[jj + 1]) / bnorm <= 0.5 * rel_start)) {
                 *fell_back = true;
                 return VFS_OK;
             }

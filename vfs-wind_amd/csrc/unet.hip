@@ -8,15 +8,20 @@
 // convolution gives the single output channel.  Zero padding at the edges.
 //
 // Precision: BF16 weights and activations, FP32 accumulation on the matrix
-// cores (v_mfma_f32_16x16x16_bf16), FP32 master weights, gradients and Adam
-// state.  Activations are channels-last ([k][j][i][c], i fastest among the
-// points) so that one lane's four consecutive channels are one 8-byte load.
+// cores, FP32 master weights, gradients and Adam state.  Activations are
+// channels-last ([k][j][i][c], i fastest among the points) so that one lane's
+// eight consecutive channels are one 16-byte load.
 //
-// Convolution as implicit GEMM: M = points (a 64-point run along i per
-// workgroup, 16 per wave), N = output channels (16 per MFMA tile), K = 27
-// taps x input channels (16 per step).  The data gradient is the same kernel
-// with the taps flipped and the weights transposed; the weight gradient is a
-// second MFMA kernel with K = points.
+// Convolution as implicit GEMM on v_mfma_f32_16x16x32_bf16: M = points (one
+// MFMA tile = 16 consecutive points along i), N = output channels (16 per
+// tile), K = taps x input channels.  A workgroup owns a 16 x 4*MT x 1 tile of
+// points and stages its 18 x (4*MT+2) x 3 halo in LDS, CC input channels at a
+// time; within a chunk K runs over (tap, channel) flattened, 32 per MFMA, so
+// a lane's eight K values are eight consecutive channels of one tap.  The data
+// gradient is the same kernel on the output gradient with the weights
+// transposed and the taps mirrored (both folded into the packing); the weight
+// gradient is a second MFMA kernel with K = points, reduced over fixed
+// point ranges in a fixed order (no atomics: training is deterministic).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -34,6 +39,8 @@ namespace vfs {
 
 typedef uint16_t bf16;
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef short s16x8 __attribute__((ext_vector_type(8)));     // eight BF16 as stored
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));   // MFMA operand
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 __host__ __device__ inline bf16 f2bf(float f)
@@ -58,64 +65,116 @@ struct Dims {
     __host__ __device__ long long np() const { return (long long)ni * nj * nk; }
 };
 
-constexpr int CV_TPB = 256;  // 4 waves, 64 points along i
+constexpr int CV_TPB = 256;  // 4 waves
 
-// Y[p][co] = act(sum_{tap, ci} X[p + tap][ci] W[tap][ci][co] + bias[co]).
-// Wf holds B fragments: Wf[((tap * CB + cb) * NT + t) * 64 + lane] = the four
-// weights (ci = 16 cb + 4 (lane/16) + 0..3, co = 16 t + lane%16).
-template <int NT, bool RELU, bool OUT_F32>
-__global__ void __launch_bounds__(CV_TPB) k_conv3(Dims d, int cin, const bf16 *__restrict__ X, const bf16x4 *__restrict__ Wf,
-                                                  const float *__restrict__ bias, bool flip, void *__restrict__ Yv)
+// Channel chunk staged in LDS per pass: 32 when the K-channel count allows, else 16.
+__host__ __device__ constexpr int conv_cc(int kc) { return kc % 32 == 0 ? 32 : 16; }
+// MFMA K steps per chunk: 27 taps x CC channels, 32 per step (the last one zero padded).
+__host__ __device__ constexpr int conv_ks(int cc) { return (27 * cc + 31) / 32; }
+// M tiles per wave: fewer for wide outputs (the accumulators are MT x NT x 4 VGPRs).
+__host__ __device__ constexpr int conv_mt(int nt) { return nt <= 2 ? 4 : nt <= 4 ? 2 : 1; }
+// LDS bytes of the halo tile: 18 x (4 MT + 2) x 3 points of CC channels, each
+// point padded by 16 bytes so that 16 lanes reading 16 consecutive points hit
+// distinct banks.
+__host__ __device__ constexpr int conv_lds(int mt, int cc) { return 18 * (4 * mt + 2) * 3 * (2 * cc + 16); }
+
+// Y[p][co] = act(sum_{tap, ci} X[p + tap][ci] W[tap][ci][co] + bias[co]) for
+// the points of one 16 x 4*MT x 1 tile.  Wf holds the B fragments in the
+// order the K loop consumes them: Wf[((chunk * KS + s) * NT + t) * 64 + lane]
+// = the eight weights of flattened K = 32 s + 8 (lane / 16) + 0..7 (tap = K /
+// CC, channel = chunk * CC + K % CC) and output channel 16 t + lane % 16.
+template <int NT, int MT, int CC, bool RELU, bool OUT_F32>
+__global__ void __launch_bounds__(CV_TPB) k_conv3t(Dims d, int cin, const bf16 *__restrict__ X, const bf16x8 *__restrict__ Wf,
+                                                   const float *__restrict__ bias, void *__restrict__ Yv)
 {
+    constexpr int TJ = 4 * MT, HI = 18, HJ = TJ + 2, HP = HI * HJ * 3;
+    constexpr int PS = 2 * CC + 16;  // bytes per halo point
+    constexpr int KS = conv_ks(CC);
+    constexpr int U = CC / 8;        // 16-byte units per halo point
+    extern __shared__ __align__(16) unsigned char lds[];
     const int cout = 16 * NT;
-    const int CB = cin / 16;
-    const int nbi = (d.ni + 63) / 64;
-    const int bi = blockIdx.x % nbi;
-    const int j = (blockIdx.x / nbi) % d.nj;
-    const int k = blockIdx.x / (nbi * d.nj);
+    const int nbi = (d.ni + 15) / 16, nbj = (d.nj + TJ - 1) / TJ;
+    const int bi = blockIdx.x % nbi, bj = (blockIdx.x / nbi) % nbj, k = blockIdx.x / (nbi * nbj);
+    const int i0 = bi * 16, j0 = bj * TJ;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int m = lane & 15, kq = lane >> 4;
-    const int i = bi * 64 + wave * 16 + m;  // this lane's A row (point)
-    f32x4 acc[NT];
+    const int m = lane & 15, g = lane >> 4;
+    f32x4 acc[MT][NT];
 #pragma unroll
-    for (int t = 0; t < NT; t++) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
-    for (int tap = 0; tap < 27; tap++) {
-        const int st = flip ? 26 - tap : tap;  // data gradient: mirrored stencil
-        const int dk = st / 9 - 1, dj = (st / 3) % 3 - 1, di = st % 3 - 1;
-        const int ii = i + di, jj = j + dj, kk = k + dk;
-        const bool in = ii >= 0 && ii < d.ni && jj >= 0 && jj < d.nj && kk >= 0 && kk < d.nk;
-        const long long p = in ? ((long long)kk * d.nj + jj) * d.ni + ii : 0;
-        const bf16x4 *xa = reinterpret_cast<const bf16x4 *>(X + p * cin) + kq;
-        const bf16x4 *wt = Wf + (long long)tap * CB * NT * 64 + lane;
-        for (int cb = 0; cb < CB; cb++) {
-            const bf16x4 a = in ? xa[cb * 4] : bf16x4{0, 0, 0, 0};
+    for (int r = 0; r < MT; r++)
 #pragma unroll
-            for (int t = 0; t < NT; t++) acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, wt[(cb * NT + t) * 64], acc[t], 0, 0, 0);
+        for (int t = 0; t < NT; t++) acc[r][t] = f32x4{0.f, 0.f, 0.f, 0.f};
+    // this lane's halo point for tap (0, 0, 0) in each of its rows
+    int base[MT];
+#pragma unroll
+    for (int r = 0; r < MT; r++) base[r] = (HJ + wave * MT + r + 1) * HI + m + 1;
+    const int nch = cin / CC;
+    for (int cc = 0; cc < nch; cc++) {
+        if (cc) __syncthreads();  // every wave is done with the previous chunk
+        for (int u = threadIdx.x; u < HP * U; u += CV_TPB) {
+            const int hp = u / U, c8 = u - hp * U;
+            const int hi = hp % HI, hj = (hp / HI) % HJ, hk = hp / (HI * HJ);
+            const int ii = i0 - 1 + hi, jj = j0 - 1 + hj, kk = k - 1 + hk;
+            s16x8 v = {0, 0, 0, 0, 0, 0, 0, 0};
+            if (ii >= 0 && ii < d.ni && jj >= 0 && jj < d.nj && kk >= 0 && kk < d.nk)
+                v = *reinterpret_cast<const s16x8 *>(X + (((long long)kk * d.nj + jj) * d.ni + ii) * cin + cc * CC + 8 * c8);
+            *reinterpret_cast<s16x8 *>(lds + hp * PS + 16 * c8) = v;
+        }
+        __syncthreads();
+        const bf16x8 *wc = Wf + (long long)cc * KS * NT * 64 + lane;
+#pragma unroll 2
+        for (int s = 0; s < KS; s++) {
+            const int kf = 32 * s + 8 * g;
+            const int tap = kf / CC, ch = kf - tap * CC;
+            bf16x8 b[NT];
+#pragma unroll
+            for (int t = 0; t < NT; t++) b[t] = wc[(s * NT + t) * 64];
+            bf16x8 a[MT];
+            if (tap < 27) {
+                const int dk = tap / 9 - 1, dj = (tap / 3) % 3 - 1, di = tap % 3 - 1;
+                const int off = (dk * HJ + dj) * HI + di;
+#pragma unroll
+                for (int r = 0; r < MT; r++)
+                    a[r] = *reinterpret_cast<const bf16x8 *>(lds + (base[r] + off) * PS + 2 * ch);
+            } else {
+#pragma unroll
+                for (int r = 0; r < MT; r++) a[r] = bf16x8{};
+            }
+#pragma unroll
+            for (int r = 0; r < MT; r++)
+#pragma unroll
+                for (int t = 0; t < NT; t++) acc[r][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[r], b[t], acc[r][t], 0, 0, 0);
         }
     }
-    // lane holds rows 4*kq + r (points) of column m (channel) of each tile
+    // lane holds rows 4 g + q (points along i) of column m (channel) of each tile
 #pragma unroll
-    for (int t = 0; t < NT; t++) {
-        const int co = 16 * t + m;
-        const float bb = bias ? bias[co] : 0.f;
+    for (int r = 0; r < MT; r++) {
+        const int jp = j0 + wave * MT + r;
+        if (jp >= d.nj) continue;
 #pragma unroll
-        for (int r = 0; r < 4; r++) {
-            const int ip = bi * 64 + wave * 16 + 4 * kq + r;
-            if (ip >= d.ni) continue;
-            float v = acc[t][r] + bb;
-            if (RELU) v = v > 0.f ? v : 0.f;
-            const long long q = (((long long)k * d.nj + j) * d.ni + ip) * cout + co;
-            if (OUT_F32) reinterpret_cast<float *>(Yv)[q] = v;
-            else reinterpret_cast<bf16 *>(Yv)[q] = f2bf(v);
+        for (int t = 0; t < NT; t++) {
+            const int co = 16 * t + m;
+            const float bb = bias ? bias[co] : 0.f;
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const int ip = i0 + 4 * g + q;
+                if (ip >= d.ni) continue;
+                float v = acc[r][t][q] + bb;
+                if (RELU) v = v > 0.f ? v : 0.f;
+                const long long o = (((long long)k * d.nj + jp) * d.ni + ip) * cout + co;
+                if (OUT_F32) reinterpret_cast<float *>(Yv)[o] = v;
+                else reinterpret_cast<bf16 *>(Yv)[o] = f2bf(v);
+            }
         }
     }
 }
 
-// dW[tap][ci][co] += sum_p X[p + tap][ci] G[p][co] over 16 rows of 64 points
-// per wave; one wave per (row group, tap, ci tile, co tile)
-constexpr int WG_ROWS = 16;
+// Partial weight gradients: block (x, y) sums X[p + tap][ci] G[p][co] over
+// the rows x, x + P, x + 2P, ... (P = gridDim.x; a row is a 64-point run along
+// i) for its (tap, 16-channel ci tile, 16-channel co tile) and writes the tile
+// into slice x of `part` (each slice 27 cin cout floats); k_sum_partials then
+// adds the slices in a fixed order, so the gradient does not depend on timing.
 __global__ void __launch_bounds__(64) k_conv3_wgrad(Dims d, int cin, int cout, const bf16 *__restrict__ X,
-                                                    const bf16 *__restrict__ G, float *__restrict__ dW)
+                                                    const bf16 *__restrict__ G, float *__restrict__ part)
 {
     const int CB = cin / 16, NT = cout / 16;
     int w = blockIdx.y;
@@ -127,7 +186,7 @@ __global__ void __launch_bounds__(64) k_conv3_wgrad(Dims d, int cin, int cout, c
     const int nbi = (d.ni + 63) / 64;
     const long long nrows = (long long)nbi * d.nj * d.nk;
     f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-    for (long long row = (long long)blockIdx.x * WG_ROWS; row < nrows && row < ((long long)blockIdx.x + 1) * WG_ROWS; row++) {
+    for (long long row = blockIdx.x; row < nrows; row += gridDim.x) {
         const int bi = row % nbi, j = (row / nbi) % d.nj, k = row / ((long long)nbi * d.nj);
         const int jj = j + dj, kk = k + dk;
         const bool rowin = jj >= 0 && jj < d.nj && kk >= 0 && kk < d.nk;
@@ -147,36 +206,57 @@ __global__ void __launch_bounds__(64) k_conv3_wgrad(Dims d, int cin, int cout, c
             acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
         }
     }
-    // acc[r] = dW[ci = 4 kq + r][co = m] for this tile
+    // acc[r] = dW[ci = 4 kq + r][co = m] of this tile
+    float *slice = part + (long long)blockIdx.x * 27 * cin * cout;
 #pragma unroll
     for (int r = 0; r < 4; r++) {
         const int ci = cb * 16 + 4 * kq + r, co = t * 16 + m;
-        atomicAdd(&dW[((long long)tap * cin + ci) * cout + co], acc[r]);
+        slice[((long long)tap * cin + ci) * cout + co] = acc[r];
     }
 }
 
-// B-fragment packing of FP32 master weights W[27][cin][cout] (forward) or of
-// the transposed weights W'[tap][co][ci] = W[tap][ci][co] (data gradient, the
-// kernel then walks the taps mirrored)
-__global__ void k_pack(int cin, int cout, int transpose, const float *__restrict__ W, bf16x4 *__restrict__ Wf)
+// out[i] = sum over p = 0 .. P-1, in order, of part[p * stride + i], i < n
+__global__ void k_sum_partials(const float *__restrict__ part, int P, long long stride, long long n, float *__restrict__ out)
 {
-    const int kin = transpose ? cout : cin, kout = transpose ? cin : cout;
-    const int CB = kin / 16, NT = kout / 16;
-    const long long total = 27LL * CB * NT * 64;
+    const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float s = 0.f;
+    for (int p = 0; p < P; p++) s += part[p * stride + i];
+    out[i] = s;
+}
+
+// B-fragment packing (see k_conv3t) of the FP32 master weights W[27][cin][cout]:
+// forward (K channels = cin, outputs = cout) or, for the data gradient,
+// transposed with the taps mirrored (K channels = cout, outputs = cin,
+// W'[tap][co][ci] = W[26 - tap][ci][co]).
+__global__ void k_pack(int cin, int cout, int transpose, const float *__restrict__ W, s16x8 *__restrict__ Wf)
+{
+    const int kc = transpose ? cout : cin, oc = transpose ? cin : cout;
+    const int CC = conv_cc(kc), KS = conv_ks(CC), NT = oc / 16, nch = kc / CC;
+    const long long total = (long long)nch * KS * NT * 64;
     long long q = blockIdx.x * (long long)blockDim.x + threadIdx.x;
     if (q >= total) return;
     const int lane = q % 64;
     long long r = q / 64;
     const int t = r % NT;
     r /= NT;
-    const int cb = r % CB, tap = r / CB;
-    bf16x4 v;
-    for (int e = 0; e < 4; e++) {
-        const int a = cb * 16 + 4 * (lane >> 4) + e, o = t * 16 + (lane & 15);
-        const float w = transpose ? W[((long long)tap * cin + o) * cout + a] : W[((long long)tap * cin + a) * cout + o];
+    const int s = r % KS, cc = r / KS;
+    const int o = 16 * t + (lane & 15);
+    s16x8 v;
+    for (int e = 0; e < 8; e++) {
+        const int kf = 32 * s + 8 * (lane >> 4) + e;
+        const int tap = kf / CC, ch = cc * CC + kf % CC;
+        float w = 0.f;
+        if (tap < 27) w = transpose ? W[((long long)(26 - tap) * cin + o) * cout + ch] : W[((long long)tap * cin + ch) * cout + o];
         v[e] = (short)f2bf(w);
     }
     Wf[q] = v;
+}
+
+static long long packed_count(int kc, int oc)
+{
+    const int CC = conv_cc(kc);
+    return (long long)(kc / CC) * conv_ks(CC) * (oc / 16) * 64;
 }
 
 // ---- elementwise layers ------------------------------------------------------
@@ -276,8 +356,9 @@ __global__ void k_relu_mask(long long n, const bf16 *__restrict__ y, const float
     gm[q] = bf2f(y[q]) > 0.f ? f2bf(g[q]) : (bf16)0;
 }
 
-// bias gradient: db[c] += sum_p gm[p][c]
-__global__ void k_bias_grad(long long np, int C, const bf16 *__restrict__ gm, float *__restrict__ db)
+// bias gradient partials: part[x * C + c] = sum of gm[p][c] over block x's grid-stride points
+constexpr int BG_BLOCKS = 64;
+__global__ void k_bias_grad(long long np, int C, const bf16 *__restrict__ gm, float *__restrict__ part)
 {
     const int c = blockIdx.y;
     float s = 0.f;
@@ -290,7 +371,7 @@ __global__ void k_bias_grad(long long np, int C, const bf16 *__restrict__ gm, fl
         if ((int)threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
         __syncthreads();
     }
-    if (threadIdx.x == 0) atomicAdd(&db[c], red[0]);
+    if (threadIdx.x == 0) part[blockIdx.x * C + c] = red[0];
 }
 
 // output 1x1x1 convolution (C -> 1), FP32; also writes y * out_scale into the
@@ -327,9 +408,11 @@ __global__ void k_loss(Dims d, Grid g, const float *__restrict__ out, const doub
     if (threadIdx.x == 0) part[blockIdx.x] = s;
 }
 
-// backward of the output convolution: g_y[p][c] = g_out[p] w[c]; dw, db by partials
+// backward of the output convolution: g_y[p][c] = g_out[p] w[c]; the
+// partial sums of dw (C values) and db (1) of block x go to part[x (C + 1) ..]
+constexpr int OB_BLOCKS = 256;
 __global__ void k_out_back(Dims d, int C, const bf16 *__restrict__ Y, const float *__restrict__ gout, const float *__restrict__ w,
-                           float *__restrict__ gy, float *__restrict__ dw, float *__restrict__ db)
+                           float *__restrict__ gy, float *__restrict__ part)
 {
     float sw[64];
     for (int c = 0; c < C; c++) sw[c] = 0.f;
@@ -350,7 +433,7 @@ __global__ void k_out_back(Dims d, int C, const bf16 *__restrict__ Y, const floa
             if ((int)threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
             __syncthreads();
         }
-        if (threadIdx.x == 0) atomicAdd(c < C ? &dw[c] : &db[0], red[0]);
+        if (threadIdx.x == 0) part[(long long)blockIdx.x * (C + 1) + c] = red[0];
         __syncthreads();
     }
 }
@@ -402,7 +485,8 @@ struct Conv {
     long long nw = 0;                 // 27 * cin * cout
     float *w = nullptr, *b = nullptr;  // FP32 masters
     float *gw = nullptr, *gb = nullptr, *mw = nullptr, *vw = nullptr, *mb = nullptr, *vb = nullptr;
-    bf16x4 *wf = nullptr, *wt = nullptr;  // packed forward / transposed fragments
+    bf16x8 *wf = nullptr, *wt = nullptr;  // packed forward / transposed B fragments
+    long long nwf = 0, nwt = 0;           // their counts
 };
 
 }  // namespace vfs
@@ -422,6 +506,7 @@ struct vfs_unet {
     float *out = nullptr, *gout = nullptr;
     double *sc = nullptr;      // [0] input scale, [1] output scale, [2..3] target scales
     double *part = nullptr;
+    float *wpart = nullptr;    // partial weight / bias gradient slices
     double out_scale = 1.0;    // phi / rms(b) of the training data
     int step = 0;
     std::vector<void *> allocs;
@@ -438,24 +523,40 @@ static int ualloc(vfs_unet *u, void **p, size_t bytes)
     return e == hipSuccess ? VFS_OK : check_hip(e, "hipMemset (unet)");
 }
 
-template <int NT, bool RELU, bool OUT_F32>
-static void conv_launch(hipStream_t st, const Dims &d, int cin, const bf16 *X, const bf16x4 *Wf, const float *bias, bool flip,
-                        void *Y)
+template <int NT, int CC, bool RELU, bool OUT_F32>
+static int conv_launch(hipStream_t st, const Dims &d, int kc, const bf16 *X, const bf16x8 *Wf, const float *bias, void *Y)
 {
-    const unsigned nb = (unsigned)(((d.ni + 63) / 64) * (long long)d.nj * d.nk);
-    hipLaunchKernelGGL((k_conv3<NT, RELU, OUT_F32>), dim3(nb), dim3(CV_TPB), 0, st, d, cin, X, Wf, bias, flip, Y);
+    constexpr int MT = conv_mt(NT), TJ = 4 * MT, LDS = conv_lds(MT, CC);
+    auto kern = k_conv3t<NT, MT, CC, RELU, OUT_F32>;
+    static bool attr = false;  // the halo tile may exceed the 64 KiB default of dynamic LDS
+    if (!attr) {
+        VFS_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(kern), hipFuncAttributeMaxDynamicSharedMemorySize, LDS));
+        attr = true;
+    }
+    const unsigned nb = (unsigned)((long long)((d.ni + 15) / 16) * ((d.nj + TJ - 1) / TJ) * d.nk);
+    hipLaunchKernelGGL(kern, dim3(nb), dim3(CV_TPB), LDS, st, d, kc, X, Wf, bias, Y);
+    return VFS_OK;
+}
+
+template <int NT, bool RELU, bool OUT_F32>
+static int conv_launch_cc(hipStream_t st, const Dims &d, int kc, const bf16 *X, const bf16x8 *Wf, const float *bias, void *Y)
+{
+    return conv_cc(kc) == 32 ? conv_launch<NT, 32, RELU, OUT_F32>(st, d, kc, X, Wf, bias, Y)
+                             : conv_launch<NT, 16, RELU, OUT_F32>(st, d, kc, X, Wf, bias, Y);
 }
 
 // forward convolution with ReLU (BF16 out)
 static int conv_fwd(hipStream_t st, const Dims &d, const Conv &c, const bf16 *X, bf16 *Y)
 {
+    int rc;
     switch (c.cout / 16) {
-    case 1: conv_launch<1, true, false>(st, d, c.cin, X, c.wf, c.b, false, Y); break;
-    case 2: conv_launch<2, true, false>(st, d, c.cin, X, c.wf, c.b, false, Y); break;
-    case 4: conv_launch<4, true, false>(st, d, c.cin, X, c.wf, c.b, false, Y); break;
-    case 8: conv_launch<8, true, false>(st, d, c.cin, X, c.wf, c.b, false, Y); break;
+    case 1: rc = conv_launch_cc<1, true, false>(st, d, c.cin, X, c.wf, c.b, Y); break;
+    case 2: rc = conv_launch_cc<2, true, false>(st, d, c.cin, X, c.wf, c.b, Y); break;
+    case 4: rc = conv_launch_cc<4, true, false>(st, d, c.cin, X, c.wf, c.b, Y); break;
+    case 8: rc = conv_launch_cc<8, true, false>(st, d, c.cin, X, c.wf, c.b, Y); break;
     default: return set_error(VFS_EINVAL, "unet: unsupported output channel count");
     }
+    if (rc) return rc;
     VFS_LAUNCH_CHECK("unet conv");
     return VFS_OK;
 }
@@ -463,35 +564,41 @@ static int conv_fwd(hipStream_t st, const Dims &d, const Conv &c, const bf16 *X,
 // data gradient dX (FP32) from the masked output gradient G (BF16)
 static int conv_bwd_data(hipStream_t st, const Dims &d, const Conv &c, const bf16 *G, float *dX)
 {
+    int rc;
     switch (c.cin / 16) {
-    case 1: conv_launch<1, false, true>(st, d, c.cout, G, c.wt, nullptr, true, dX); break;
-    case 2: conv_launch<2, false, true>(st, d, c.cout, G, c.wt, nullptr, true, dX); break;
-    case 3: conv_launch<3, false, true>(st, d, c.cout, G, c.wt, nullptr, true, dX); break;
-    case 4: conv_launch<4, false, true>(st, d, c.cout, G, c.wt, nullptr, true, dX); break;
-    case 6: conv_launch<6, false, true>(st, d, c.cout, G, c.wt, nullptr, true, dX); break;
-    case 8: conv_launch<8, false, true>(st, d, c.cout, G, c.wt, nullptr, true, dX); break;
-    case 12: conv_launch<12, false, true>(st, d, c.cout, G, c.wt, nullptr, true, dX); break;
+    case 1: rc = conv_launch_cc<1, false, true>(st, d, c.cout, G, c.wt, nullptr, dX); break;
+    case 2: rc = conv_launch_cc<2, false, true>(st, d, c.cout, G, c.wt, nullptr, dX); break;
+    case 3: rc = conv_launch_cc<3, false, true>(st, d, c.cout, G, c.wt, nullptr, dX); break;
+    case 4: rc = conv_launch_cc<4, false, true>(st, d, c.cout, G, c.wt, nullptr, dX); break;
+    case 6: rc = conv_launch_cc<6, false, true>(st, d, c.cout, G, c.wt, nullptr, dX); break;
+    case 8: rc = conv_launch_cc<8, false, true>(st, d, c.cout, G, c.wt, nullptr, dX); break;
+    case 12: rc = conv_launch_cc<12, false, true>(st, d, c.cout, G, c.wt, nullptr, dX); break;
     default: return set_error(VFS_EINVAL, "unet: unsupported input channel count");
     }
+    if (rc) return rc;
     VFS_LAUNCH_CHECK("unet conv data gradient");
     return VFS_OK;
 }
 
-static int conv_bwd_weight(hipStream_t st, const Dims &d, const Conv &c, const bf16 *X, const bf16 *G)
+constexpr int WG_SLICES = 64;  // partial weight-gradient slices (fixed: the summation order is too)
+
+static int conv_bwd_weight(hipStream_t st, const Dims &d, const Conv &c, const bf16 *X, const bf16 *G, float *wpart)
 {
-    const long long rows = (long long)((d.ni + 63) / 64) * d.nj * d.nk;
-    dim3 grid((unsigned)((rows + WG_ROWS - 1) / WG_ROWS), (unsigned)(27 * (c.cin / 16) * (c.cout / 16)));
-    hipLaunchKernelGGL(k_conv3_wgrad, grid, dim3(64), 0, st, d, c.cin, c.cout, X, G, c.gw);
-    hipLaunchKernelGGL(k_bias_grad, dim3(64, c.cout), dim3(256), 0, st, d.np(), c.cout, G, c.gb);
+    dim3 grid(WG_SLICES, (unsigned)(27 * (c.cin / 16) * (c.cout / 16)));
+    hipLaunchKernelGGL(k_conv3_wgrad, grid, dim3(64), 0, st, d, c.cin, c.cout, X, G, wpart);
+    hipLaunchKernelGGL(k_sum_partials, dim3(blocks_for(c.nw, 256)), dim3(256), 0, st, wpart, WG_SLICES, c.nw, c.nw, c.gw);
+    hipLaunchKernelGGL(k_bias_grad, dim3(BG_BLOCKS, c.cout), dim3(256), 0, st, d.np(), c.cout, G, wpart);
+    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, st, wpart, BG_BLOCKS, (long long)c.cout, (long long)c.cout, c.gb);
     VFS_LAUNCH_CHECK("unet conv weight gradient");
     return VFS_OK;
 }
 
 static int pack(hipStream_t st, Conv &c)
 {
-    const long long n = 27LL * (c.cin / 16) * (c.cout / 16) * 64;
-    hipLaunchKernelGGL(k_pack, dim3(blocks_for(n, 256)), dim3(256), 0, st, c.cin, c.cout, 0, c.w, c.wf);
-    hipLaunchKernelGGL(k_pack, dim3(blocks_for(n, 256)), dim3(256), 0, st, c.cin, c.cout, 1, c.w, c.wt);
+    hipLaunchKernelGGL(k_pack, dim3(blocks_for(c.nwf, 256)), dim3(256), 0, st, c.cin, c.cout, 0, c.w,
+                       reinterpret_cast<s16x8 *>(c.wf));
+    hipLaunchKernelGGL(k_pack, dim3(blocks_for(c.nwt, 256)), dim3(256), 0, st, c.cin, c.cout, 1, c.w,
+                       reinterpret_cast<s16x8 *>(c.wt));
     VFS_LAUNCH_CHECK("unet pack");
     return VFS_OK;
 }
@@ -507,9 +614,10 @@ static int conv_alloc(vfs_unet *u, Conv &c, int cin, int cout, int level, int fa
     for (float **f : fs) rc = rc ? rc : ualloc(u, (void **)f, c.nw * 4);
     float **bs[] = {&c.b, &c.gb, &c.mb, &c.vb};
     for (float **f : bs) rc = rc ? rc : ualloc(u, (void **)f, cout * 4);
-    const long long nf = 27LL * (cin / 16) * (cout / 16) * 64;
-    rc = rc ? rc : ualloc(u, (void **)&c.wf, nf * sizeof(bf16x4));
-    rc = rc ? rc : ualloc(u, (void **)&c.wt, nf * sizeof(bf16x4));
+    c.nwf = packed_count(cin, cout);
+    c.nwt = packed_count(cout, cin);
+    rc = rc ? rc : ualloc(u, (void **)&c.wf, c.nwf * sizeof(bf16x8));
+    rc = rc ? rc : ualloc(u, (void **)&c.wt, c.nwt * sizeof(bf16x8));
     if (rc) return rc;
     // He initialisation over the real fan-in (the first layer has one live input channel)
     const int fan = 27 * fan_cin;
@@ -560,20 +668,18 @@ static int backward(vfs_unet *u)
     const int L = u->levels;
     const Dims &d0 = u->dims[0];
     auto zero = [&](float *p, long long n) { return hipMemsetAsync(p, 0, n * 4, st); };
-    for (auto &cv : u->enc) { zero(cv.gw, cv.nw); zero(cv.gb, cv.cout); }
-    for (auto &cv : u->dec) { zero(cv.gw, cv.nw); zero(cv.gb, cv.cout); }
-    zero(u->gwo, u->dec[0].cout);
-    zero(u->gbo, 1);
     for (int l = 0; l < L; l++) zero(u->ge[l], u->dims[l].np() * u->enc[l].cout);
-    hipLaunchKernelGGL(k_out_back, dim3(256), dim3(256), 0, st, d0, u->dec[0].cout, u->d[0], u->gout, u->wo, u->gd[0], u->gwo,
-                       u->gbo);
+    const int C0 = u->dec[0].cout;
+    hipLaunchKernelGGL(k_out_back, dim3(OB_BLOCKS), dim3(256), 0, st, d0, C0, u->d[0], u->gout, u->wo, u->gd[0], u->wpart);
+    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, st, u->wpart, OB_BLOCKS, (long long)C0 + 1, (long long)C0, u->gwo);
+    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, st, u->wpart + C0, OB_BLOCKS, (long long)C0 + 1, 1LL, u->gbo);
     int rc = 0;
     for (int l = 0; l <= L - 2 && !rc; l++) {
         Conv &cv = u->dec[l];
         const Dims &d = u->dims[l];
         const long long n = d.np() * cv.cout;
         hipLaunchKernelGGL(k_relu_mask, dim3(blocks_for(n, 256)), dim3(256), 0, st, n, u->d[l], u->gd[l], u->gm);
-        rc = conv_bwd_weight(st, d, cv, u->cat[l], u->gm);
+        rc = conv_bwd_weight(st, d, cv, u->cat[l], u->gm, u->wpart);
         if (!rc) rc = conv_bwd_data(st, d, cv, u->gm, u->gcat[l]);
         if (rc) break;
         const int Cu = cv.cin - u->enc[l].cout, Cs = u->enc[l].cout;
@@ -589,7 +695,7 @@ static int backward(vfs_unet *u)
         const long long n = d.np() * cv.cout;
         hipLaunchKernelGGL(k_relu_mask, dim3(blocks_for(n, 256)), dim3(256), 0, st, n, u->e[l], u->ge[l], u->gm);
         const bf16 *X = l == 0 ? u->x0 : u->pool[l];
-        rc = conv_bwd_weight(st, d, cv, X, u->gm);
+        rc = conv_bwd_weight(st, d, cv, X, u->gm, u->wpart);
         if (rc || l == 0) break;
         rc = conv_bwd_data(st, d, cv, u->gm, u->gpool[l]);
         if (rc) break;
@@ -703,6 +809,10 @@ int vfs_unet_create(vfs_ctx *c, int base, int levels, unsigned long long seed, v
     rc = rc ? rc : ualloc(u, (void **)&u->gout, np0 * 4);
     rc = rc ? rc : ualloc(u, (void **)&u->sc, 8 * sizeof(double));
     rc = rc ? rc : ualloc(u, (void **)&u->part, (RED_BLOCKS + 8) * sizeof(double));
+    long long wmax = (long long)OB_BLOCKS * (base + 1);
+    for (auto *v : {&u->enc, &u->dec})
+        for (auto &cv : *v) wmax = std::max({wmax, (long long)WG_SLICES * cv.nw, (long long)BG_BLOCKS * cv.cout});
+    rc = rc ? rc : ualloc(u, (void **)&u->wpart, wmax * sizeof(float));
     if (!rc) rc = repack(u);
     if (rc) { vfs_unet_destroy(u); return rc; }
     *out = u;

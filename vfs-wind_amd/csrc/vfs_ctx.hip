@@ -46,9 +46,35 @@ int validate(const vfs_params *p)
         return vfs::set_error(VFS_EINVAL, "ren and dt must be positive");
     if (p->per_j)
         return vfs::set_error(VFS_EINVAL, "a periodic j direction is not supported (wall-normal line smoother)");
+    // boundary codes of the non-periodic faces (bcs.dat): 1 no-slip, 10 slip
+    // (-10 on j+), 6 far field, 0 interface and 3 symmetry (they only set the
+    // reference's Ubcs, which nothing on this path reads), 2 moving lid (j+;
+    // on k+ it only sets Ubcs), 12 Couette wall (j-, j+), 4 outflow (k+, k-,
+    // j+), 5 inlet (k-), 8 characteristic outlet (k+)
+    const int per[3] = {p->per_i, p->per_j, p->per_k};
+    for (int f = 0; f < 6; f++) {
+        if (per[f / 2]) continue;
+        const int b = p->bctype[f];
+        const bool ok = b == 1 || b == 10 || b == 6 || b == 0 || b == 3 || (b == -10 && f == 3) ||
+                        (b == 2 && (f == 3 || f == 5)) || (b == 12 && (f == 2 || f == 3)) ||
+                        (b == 4 && (f == 5 || f == 4 || f == 3)) || (b == 5 && f == 4) || (b == 8 && f == 5);
+        if (!ok)
+            return vfs::set_error(VFS_EINVAL, "bcs code " + std::to_string(b) + " on face " + std::to_string(f) +
+                                                  " is not supported (wall functions -1/-2, the body-fitted inflow 11 "
+                                                  "and the level-set codes are outside this library)");
+    }
     for (int f = 0; f < 6; f++)
         if (p->bctype[f] == 5 && (f != 4 || p->per_k))
             return vfs::set_error(VFS_EINVAL, "an inlet (bcs code 5) is supported on the k- face of a non-periodic k direction");
+    if (p->bctype[4] == 5) {
+        const int ip = p->inletprofile;
+        if (!(ip == 0 || ip == 1 || ip == 2 || ip == 3 || ip == 10 || ip == 12 || ip == 14 || ip == 15 || ip == 21 || ip == 100))
+            return vfs::set_error(VFS_EINVAL, "-inlet " + std::to_string(ip) +
+                                                  " is not supported (11 adds time-seeded noise; 13, 16, 20 and 101 leave "
+                                                  "the inlet velocity unset)");
+    }
+    if (p->pseudo_periodic && p->bctype[4] != 5)
+        return vfs::set_error(VFS_EINVAL, "-pseudo needs the inlet (bcs code 5) on the k- face");
     return VFS_OK;
 }
 }  // namespace
@@ -126,7 +152,7 @@ int vfs_ctx_create(const vfs_params *params, vfs_ctx **out)
     std::memset(c->f, 0, sizeof(c->f));
     c->p = *params;
     c->g = vfs::make_grid(*params);
-    c->face = c->pucont = c->snap = c->fpc = c->red = c->red_host = nullptr;
+    c->face = c->pucont = c->snap = c->fpc = c->red = c->red_host = c->inflow_plane = nullptr;
     c->ps = nullptr;
     c->cs_mode = vfs::CS_PARAM;
     c->dyn = nullptr;
@@ -161,6 +187,7 @@ int vfs_ctx_create(const vfs_params *params, vfs_ctx **out)
     if (!rc) rc = alloc(&c->snap, 3 * n);
     if (!rc) rc = alloc(&c->fpc, 3 * n);
     if (!rc) rc = alloc(&c->red, 1 << 16);
+    if (!rc) rc = alloc(&c->inflow_plane, 3 * (size_t)c->g.plane);
     if (!rc) {
         hipError_t ee = hipHostMalloc((void **)&c->red_host, 4096 * sizeof(double), hipHostMallocDefault);
         if (ee != hipSuccess) rc = vfs::check_hip(ee, "hipHostMalloc");
@@ -185,6 +212,7 @@ int vfs_ctx_destroy(vfs_ctx *c)
     if (c->snap) hipFree(c->snap);
     if (c->fpc) hipFree(c->fpc);
     if (c->red) hipFree(c->red);
+    if (c->inflow_plane) hipFree(c->inflow_plane);
     if (c->red_host) hipHostFree(c->red_host);
     if (c->imp_ws) hipFree(c->imp_ws);
     if (c->dyn) hipFree(c->dyn);
@@ -400,6 +428,29 @@ int vfs_runge_kutta(vfs_ctx *c)
     return vfs::form_bcs(c, 1);
 }
 
+int vfs_inflow_set_plane(vfs_ctx *c, const double *host)
+{
+    if (!c || !host) return vfs::set_error(VFS_EINVAL, "null argument");
+    VFS_HIP(hipMemcpyAsync(c->inflow_plane, host, 3 * (size_t)c->g.plane * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    VFS_HIP(hipStreamSynchronize(c->stream));
+    return VFS_OK;
+}
+
+int vfs_inflow_get_plane(vfs_ctx *c, double *host)
+{
+    if (!c || !host) return vfs::set_error(VFS_EINVAL, "null argument");
+    VFS_HIP(hipMemcpyAsync(host, c->inflow_plane, 3 * (size_t)c->g.plane * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    VFS_HIP(hipStreamSynchronize(c->stream));
+    return VFS_OK;
+}
+
+int vfs_pseudo_periodic_plane(vfs_ctx *c)
+{
+    if (!c) return vfs::set_error(VFS_EINVAL, "null context");
+    int rc = vfs::c2c(c);  // Ucat must be current
+    return rc ? rc : vfs::pseudo_periodic_plane(c);
+}
+
 int vfs_inflow_flux(vfs_ctx *c, double *flux_in)
 {
     if (!c) return vfs::set_error(VFS_EINVAL, "null context");
@@ -498,7 +549,8 @@ static int step_common(vfs_ctx *c, const vfs_solve_opts *o, const vfs_newton_opt
 {
     using clk = std::chrono::steady_clock;
     auto t0 = clk::now();
-    int rc = vfs_calc_k_flux(c, nullptr);                   // solvers.c:363
+    int rc = c->p.pseudo_periodic ? vfs_pseudo_periodic_plane(c) : VFS_OK;  // solvers.c:355-358
+    if (!rc) rc = vfs_calc_k_flux(c, nullptr);              // solvers.c:363
     if (!rc && c->p.les) rc = vfs::c2c(c);                  // solvers.c:368
     const int freq = c->p.dynamic_freq < 1 ? 1 : c->p.dynamic_freq;  // main.c:1439
     if (!rc && c->p.les && (c->ti % freq == 0 || c->ti == c->tistart))

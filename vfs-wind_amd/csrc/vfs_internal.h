@@ -189,6 +189,7 @@ struct vfs_ctx {
     double *snap;     // 3 doubles per point: snapshot for BC passes
     double *fpc;      // 3 planes of n doubles: Fp of every cell (Formfunction_2)
     double *red;      // reduction scratch (device)
+    double *inflow_plane;  // my x mx Cmpnts: the inlet plane of -inlet 100 / -pseudo (user->ucont_plane)
     double *red_host; // pinned host mirror of reduction results
     double mean_k_area;
     double mean_k_flux;
@@ -272,6 +273,7 @@ int smagorinsky_constant(vfs_ctx *c, int ti, int tistart, int rstart);
 int materialize_cs(vfs_ctx *c);
 int form_bcs(vfs_ctx *c, int outflow_scale);
 int inflow_flux(vfs_ctx *c, double *flux_in);
+int pseudo_periodic_plane(vfs_ctx *c);
 int eddy_viscosity(vfs_ctx *c);
 int pressure_gradient(vfs_ctx *c);
 int calc_k_flux(vfs_ctx *c, double *out);

@@ -176,6 +176,95 @@ int vfs_checkpoint_read(vfs_ctx *c, const char *dir, int ti, int block, int imme
     return VFS_OK;
 }
 
+// ---- inflow planes (rhs2.c:716-862) ---------------------------------------
+
+int vfs_inflow_write_section(const char *dir, int ti, int tistart, int save_inflow_period, int save_inflow_minus,
+                             double dt, int mx, int my, const double *plane)
+{
+    if (!dir || !plane || save_inflow_period <= 0 || mx <= 0 || my <= 0)
+        return vfs::set_error(VFS_EINVAL, "vfs_inflow_write_section: bad argument");
+    const int ti0 = ti - save_inflow_minus;
+    const int ti_name = ((ti0 - 1) / save_inflow_period) * save_inflow_period + 1;  // rhs2.c:728
+    char name[96];
+    std::snprintf(name, sizeof(name), "inflow_%06d_dt=%g.dat", ti_name, dt);
+    const std::string path = std::string(dir) + "/" + name;
+    if (ti0 == tistart || ti0 % save_inflow_period == 1) std::remove(path.c_str());  // a new file (rhs2.c:731)
+    errno = 0;
+    File f(path.c_str(), "ab");
+    if (!f.f) return io_error("cannot open for appending", path.c_str());
+    // my rows of mx Cmpnts (rhs2.c:736)
+    if (std::fwrite(plane, 3 * sizeof(double), (size_t)mx * my, f.f) != (size_t)mx * my)
+        return io_error("write failed", path.c_str());
+    return VFS_OK;
+}
+
+}  // extern "C"
+
+struct vfs_inflow_reader {
+    std::string path_inflow;
+    double dt_inflow;
+    int period, minus, recycle, tistart, mx, my;
+    FILE *fp;
+};
+
+extern "C" {
+
+int vfs_inflow_reader_open(const char *path_inflow, double dt_inflow, int save_inflow_period, int save_inflow_minus,
+                           int recycle_period, int tistart, int mx, int my, vfs_inflow_reader **out)
+{
+    if (!path_inflow || !out || save_inflow_period <= 0 || recycle_period <= 0 || mx <= 0 || my <= 0)
+        return vfs::set_error(VFS_EINVAL, "vfs_inflow_reader_open: bad argument");
+    *out = new vfs_inflow_reader{path_inflow, dt_inflow, save_inflow_period, save_inflow_minus, recycle_period,
+                                 tistart, mx, my, nullptr};
+    return VFS_OK;
+}
+
+int vfs_inflow_reader_next(vfs_inflow_reader *r, int ti, double *plane)
+{
+    if (!r || !plane) return vfs::set_error(VFS_EINVAL, "vfs_inflow_reader_next: bad argument");
+    const size_t np = (size_t)r->mx * r->my;
+    // rhs2.c:797-805: the time index inside the recycled record and its file
+    int ti2 = ti - r->minus;
+    if (ti2 == 0) ti2 = 1;
+    if (ti2 > r->recycle) ti2 -= (ti2 / r->recycle) * r->recycle;
+    const int ti_name = ((ti2 - 1) / r->period) * r->period + 1;
+    char name[96];
+    std::snprintf(name, sizeof(name), "/inflow/inflow_%06d_dt=%g.dat", ti_name, r->dt_inflow);
+    const std::string path = r->path_inflow + name;
+    // rhs2.c:807-820: (re)open at the first step and at the first step of each file
+    if (ti == r->tistart || (ti > r->tistart + 90 && ti2 % r->period == 1)) {
+        if (ti != r->tistart && r->fp) std::fclose(r->fp);
+        errno = 0;
+        r->fp = std::fopen(path.c_str(), "rb");
+        if (!r->fp) return io_error("cannot open the inflow file", path.c_str());
+        if (ti == r->tistart)
+            for (int it = 0; it < (ti2 - 1) % r->period; it++)
+                if (std::fseek(r->fp, (long)(np * 3 * sizeof(double)), SEEK_CUR) != 0)
+                    return io_error("cannot position in the inflow file", path.c_str());
+    }
+    // rhs2.c:821-823: nothing is read at the first step of a fresh start, and
+    // the plane is then zero (the reference sums zeroed per-rank buffers)
+    if (r->tistart == 0 && ti == 1) {
+        std::memset(plane, 0, np * 3 * sizeof(double));
+        return VFS_OK;
+    }
+    if (!r->fp) return vfs::set_error(VFS_EINVAL, "vfs_inflow_reader_next: the first call must be for ti = tistart");
+    // past the end of a file (within 90 steps of a restart the reference does
+    // not switch files) the values not read stay zero, as in the reference's
+    // zeroed read buffer
+    std::memset(plane, 0, np * 3 * sizeof(double));
+    if (std::fread(plane, 3 * sizeof(double), np, r->fp) != np) std::clearerr(r->fp);
+    return VFS_OK;
+}
+
+int vfs_inflow_reader_close(vfs_inflow_reader *r)
+{
+    if (!r) return VFS_OK;
+    if (r->fp) std::fclose(r->fp);
+    delete r;
+    return VFS_OK;
+}
+
 // ---- grid readers (init.c:258-394) --------------------------------------
 
 namespace {

@@ -403,6 +403,31 @@ __global__ void k_c2c_bc(Grid g, int axis, const double *__restrict__ L2, const 
         for (int c = 0; c < 3; c++) u[c] = -L2[3 * s + c];
         solid_flag = 1; touched = true;
     }
+    // lid-driven cavity j+ (rhs.c:593-605): the wall moves with u = 1
+    if (j == my - 1 && bc[3] == 2) {
+        const double *nb = (solid_flag ? uold : L2) + 3 * gidx(g, k, j - 1, i);
+        u[0] = 2.0 - nb[0]; u[1] = -nb[1]; u[2] = -nb[2];
+        touched = true;
+    }
+    // Couette walls j- / j+ (rhs.c:608-624): the wall moves with w = 1
+    if (j == 0 && bc[2] == 12) {
+        const double *nb = L2 + 3 * gidx(g, k, j + 1, i);
+        u[0] = -nb[0]; u[1] = -nb[1]; u[2] = 2.0 - nb[2];
+        if (solid_flag) u[0] = u[1] = u[2] = 0;
+        touched = true;
+    }
+    if (j == my - 1 && bc[3] == 12) {
+        const double *nb = L2 + 3 * gidx(g, k, j - 1, i);
+        u[0] = -nb[0]; u[1] = -nb[1]; u[2] = 2.0 - nb[2];
+        if (solid_flag) u[0] = u[1] = u[2] = 0;
+        touched = true;
+    }
+    // outflow j+ (rhs.c:640-642)
+    if (bc[3] == 4 && j == my - 1 && (i != 0 && i != mx - 1 && k != 0 && k != mz - 1)) {
+        const long long s = gidx(g, k, j - 1, i);
+        for (int c = 0; c < 3; c++) u[c] = L2[3 * s + c];
+        touched = true;
+    }
     if (bc[5] == 4 && k == mz - 1 && (i != 0 && i != mx - 1 && j != 0 && j != my - 1)) {
         if (nvert[gidx(g, k - 1, j, i)] > 0.1) { u[0] = u[1] = u[2] = 0; touched = true; }
         if (solid_flag) { u[0] = u[1] = u[2] = 0; touched = true; }
@@ -422,6 +447,9 @@ __global__ void k_c2c_corner(Grid g, double *__restrict__ ucat)
 
 static bool bc_needs_snapshot(const Grid &g)
 {
+    // the cavity, Couette and j+ outflow rules read the pre-pass values of cells
+    // another axis' rule may write
+    if (g.bc[3] == 2 || g.bc[2] == 12 || g.bc[3] == 12 || g.bc[3] == 4) return true;
     // a no-slip rule reads the neighbouring plane, which another axis' rule
     // may write in the same pass only if walls sit on two or more axes
     const int per[3] = {g.pi, g.pj, g.pk};
@@ -572,14 +600,52 @@ __global__ void k_zero_edges(Grid g, double *__restrict__ ucat)
 }
 
 // ---------------------------------------------------------------------------
-// InflowFlux (bcs.c:247-626): uniform inlet (-inlet 1) on the k- face
+// InflowFlux (bcs.c:247-626): the inlet on the k- face
 // ---------------------------------------------------------------------------
 
-// inletArea (bcs.c:201-234) and, with mode 1, the inlet fluxes themselves:
-// ucont(0).z = uin |zet|, ucat(0) = 2 u - ucat(1) with u = Contra2Cart_single
-// (rhs.c:42-63) of (0, 0, ucont.z); per-block partial sums of the area (mode
-// 0) or of the fluid inlet flux (mode 1)
-__global__ void k_inflow(Grid g, int mode, double uin, const double *__restrict__ csi, const double *__restrict__ eta,
+// InletInterpolation (bcs.c:636-681) returns main.c:13's constant Flux_in
+// before reaching its table code
+constexpr double FLUX_IN_CONST = 4.104388e-04;
+
+// uin of bcs.c:308-371 at an inlet face with centre (xc, yc), radius r;
+// uniform = 1 or -flux / inletArea (profiles 1 and 2)
+__device__ inline double inlet_uin(int profile, double uniform, double yc, double r)
+{
+    switch (profile) {
+    case 0:
+    case 3: return FLUX_IN_CONST;
+    case 10: {
+        const double delta = 0.45263, a = 5.99;
+        if (yc >= delta) return 1.0;
+        if (yc <= 0) return 0.0;
+        return pow(yc / delta, 1. / a);
+    }
+    case 12: return 2 * (1 - pow(r / 0.5, 2.0));
+    case 14: {
+        const double R[11] = {0.000, 0.062, 0.125, 0.188, 0.250, 0.312, 0.354, 0.399, 0.438, 0.469, 0.500};
+        const double W[11] = {1.120, 1.115, 1.110, 1.093, 1.050, 1.010, 0.963, 0.915, 0.825, 0.730, 0.000};
+        int ii;
+        for (ii = 1; ii < 11; ii++)
+            if (R[ii] >= r && R[ii - 1] < r) break;
+        if (ii == 11) ii = 10;  // r = 0 or r > 0.5 (replaced by 0 below); the reference reads past its table
+        const double u = (W[ii] - W[ii - 1]) / (R[ii] - R[ii - 1]) * (r - R[ii - 1]) + W[ii - 1];
+        return r > 0.5 ? 0 : u;
+    }
+    case 15: return r > 0.5 ? 0 : 1;
+    case 21: return 1.5 * 1.0 * yc * (2 - yc);
+    default: return uniform;  // 1, 2 (100 uses the plane)
+    }
+}
+
+// mode 0: partial sums of inletArea (bcs.c:201-234);
+// mode 1: the inlet fluxes and ghost velocities (bcs.c:379-449) and partial
+//         sums of the inlet flux;
+// mode 2: the held-flux correction ucont.z += (flux0 - flux1) |zet| / inletArea
+//         (bcs.c:487-496, sums[0] = flux1, sums[1] = inletArea) and partial
+//         sums of the corrected fluid inlet flux
+__global__ void k_inflow(Grid g, int mode, int profile, int plane, double inlet_flux, double flux0,
+                         const double *__restrict__ sums, const double *__restrict__ coor,
+                         const double *__restrict__ pl, const double *__restrict__ csi, const double *__restrict__ eta,
                          const double *__restrict__ zet, const double *__restrict__ nvert, double *__restrict__ ucont,
                          double *__restrict__ ucat, double *__restrict__ part)
 {
@@ -595,22 +661,44 @@ __global__ void k_inflow(Grid g, int mode, double uin, const double *__restrict_
             if (fluid) s += area;
             continue;
         }
+        if (mode == 2) {
+            if (fluid) {
+                ucont[3 * q + 2] += (flux0 - sums[0]) * area / sums[1];
+                s += ucont[3 * q + 2];
+            }
+            continue;
+        }
         if (fluid) {
-            const double q2 = uin * area;
-            ucont[3 * q + 2] = q2;
-            ucont[3 * q] = 0;
-            ucont[3 * q + 1] = 0;
-            const double m00 = csi[3 * q], m01 = csi[3 * q + 1], m02 = csi[3 * q + 2];
-            const double m10 = eta[3 * q], m11 = eta[3 * q + 1], m12 = eta[3 * q + 2];
-            const double m20 = z[0], m21 = z[1], m22 = z[2];
-            const double q0 = 0, qq1 = 0;
-            const double det = m00 * (m11 * m22 - m12 * m21) - m01 * (m10 * m22 - m12 * m20) + m02 * (m10 * m21 - m11 * m20);
-            const double det0 = q0 * (m11 * m22 - m12 * m21) - qq1 * (m01 * m22 - m02 * m21) + q2 * (m01 * m12 - m02 * m11);
-            const double det1 = -q0 * (m10 * m22 - m12 * m20) + qq1 * (m00 * m22 - m02 * m20) - q2 * (m00 * m12 - m02 * m10);
-            const double det2 = q0 * (m10 * m21 - m11 * m20) - qq1 * (m00 * m21 - m01 * m20) + q2 * (m00 * m11 - m01 * m10);
-            const double u[3] = {det0 / det, det1 / det, det2 / det};
-            for (int c = 0; c < 3; c++) ucat[3 * q + c] = -ucat[3 * q1 + c] + 2 * u[c];
-            s += q2;
+            if (plane) {
+                const double *u = pl + 3 * ((long long)j * g.mx + i);
+                for (int c = 0; c < 3; c++) ucat[3 * q + c] = u[c];
+                ucont[3 * q + 2] = 0.5 * (ucat[3 * q] + ucat[3 * q1]) * z[0] + 0.5 * (ucat[3 * q + 1] + ucat[3 * q1 + 1]) * z[1] +
+                                   0.5 * (ucat[3 * q + 2] + ucat[3 * q1 + 2]) * z[2];
+            } else {
+                // face centre of the inlet cell from the k = 1 node plane (bcs.c:307-308)
+                const long long a0 = gidx(g, 1, j, i), a1 = gidx(g, 1, j - 1, i), a2 = gidx(g, 1, j, i - 1),
+                                a3 = gidx(g, 1, j - 1, i - 1);
+                const double xc = (coor[3 * a0] + coor[3 * a1] + coor[3 * a2] + coor[3 * a3]) * 0.25;
+                const double yc = (coor[3 * a0 + 1] + coor[3 * a1 + 1] + coor[3 * a2 + 1] + coor[3 * a3 + 1]) * 0.25;
+                const double r = sqrt(xc * xc + yc * yc);
+                const double uniform = inlet_flux < 0 ? 1. : inlet_flux / sums[1];
+                const double q2 = inlet_uin(profile, uniform, yc, r) * area;
+                ucont[3 * q + 2] = q2;
+                ucont[3 * q] = 0;
+                ucont[3 * q + 1] = 0;
+                // Contra2Cart_single (rhs.c:42-63) of (0, 0, q2)
+                const double m00 = csi[3 * q], m01 = csi[3 * q + 1], m02 = csi[3 * q + 2];
+                const double m10 = eta[3 * q], m11 = eta[3 * q + 1], m12 = eta[3 * q + 2];
+                const double m20 = z[0], m21 = z[1], m22 = z[2];
+                const double q0 = 0, qq1 = 0;
+                const double det = m00 * (m11 * m22 - m12 * m21) - m01 * (m10 * m22 - m12 * m20) + m02 * (m10 * m21 - m11 * m20);
+                const double det0 = q0 * (m11 * m22 - m12 * m21) - qq1 * (m01 * m22 - m02 * m21) + q2 * (m01 * m12 - m02 * m11);
+                const double det1 = -q0 * (m10 * m22 - m12 * m20) + qq1 * (m00 * m22 - m02 * m20) - q2 * (m00 * m12 - m02 * m10);
+                const double det2 = q0 * (m10 * m21 - m11 * m20) - qq1 * (m00 * m21 - m01 * m20) + q2 * (m00 * m11 - m01 * m10);
+                const double u[3] = {det0 / det, det1 / det, det2 / det};
+                for (int c = 0; c < 3; c++) ucat[3 * q + c] = -ucat[3 * q1 + c] + 2 * u[c];
+            }
+            s += ucont[3 * q + 2];
         } else {
             ucat[3 * q + 2] = 0;
             ucont[3 * q + 2] = 0;
@@ -618,6 +706,23 @@ __global__ void k_inflow(Grid g, int mode, double uin, const double *__restrict_
     }
     s = block_reduce<OpSum>(s);
     if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+// pseudo_periodic_BC (bcs.c:29-95): plane := ucat(kplane), kplane = (6 mz) / 10
+__global__ void k_pseudo_plane(Grid g, const double *__restrict__ ucat, double *__restrict__ pl)
+{
+    const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (t >= 3 * g.plane) return;
+    const long long kplane = (g.mz * 6) / 10;
+    pl[t] = ucat[3 * kplane * g.plane + t];
+}
+
+int pseudo_periodic_plane(vfs_ctx *c)
+{
+    const long long n = 3 * c->g.plane;
+    hipLaunchKernelGGL(k_pseudo_plane, dim3(blocks_for(n, TPB)), dim3(TPB), 0, c->stream, c->g, c->f[VFS_UCAT], c->inflow_plane);
+    VFS_LAUNCH_CHECK("pseudo-periodic plane");
+    return VFS_OK;
 }
 
 int inflow_flux(vfs_ctx *c, double *flux_in)
@@ -628,25 +733,213 @@ int inflow_flux(vfs_ctx *c, double *flux_in)
         return VFS_OK;
     }
     hipStream_t st = c->stream;
-    double uin = 1.0;
-    if (c->p.inlet_flux >= 0) {
-        hipLaunchKernelGGL(k_inflow, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, st, g, 0, 0.0, c->f[VFS_CSI], c->f[VFS_ETA],
-                           c->f[VFS_ZET], c->f[VFS_NVERT], c->f[VFS_UCONT], c->f[VFS_UCAT], c->red);
-        hipLaunchKernelGGL(finish_reduce<OpSum>, dim3(1), dim3(RED_THREADS), 0, st, c->red, RED_BLOCKS, c->red + RED_BLOCKS);
-        VFS_LAUNCH_CHECK("inlet area");
-        VFS_HIP(hipMemcpyAsync(c->red_host, c->red + RED_BLOCKS, sizeof(double), hipMemcpyDeviceToHost, st));
-        VFS_HIP(hipStreamSynchronize(st));
-        uin = c->p.inlet_flux / c->red_host[0];
-    }
-    hipLaunchKernelGGL(k_inflow, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, st, g, 1, uin, c->f[VFS_CSI], c->f[VFS_ETA],
+    const int profile = c->p.inletprofile;
+    const int plane = c->p.pseudo_periodic || profile == 100;
+    const bool held = c->p.pseudo_periodic || profile == 100 || profile == 14;
+    const double inlet_flux = c->p.inlet_flux;
+    // device scalars: sums[0] = flux of the set inlet (flux1), sums[1] = inletArea, sums[2] = FluxIn
+    double *sums = c->red + 2 * RED_BLOCKS;
+    hipLaunchKernelGGL(k_inflow, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, st, g, 0, profile, plane, inlet_flux, 0.0,
+                       (const double *)nullptr, c->f[VFS_COOR], c->inflow_plane, c->f[VFS_CSI], c->f[VFS_ETA],
                        c->f[VFS_ZET], c->f[VFS_NVERT], c->f[VFS_UCONT], c->f[VFS_UCAT], c->red);
+    hipLaunchKernelGGL(finish_reduce<OpSum>, dim3(1), dim3(RED_THREADS), 0, st, c->red, RED_BLOCKS, sums + 1);
+    hipLaunchKernelGGL(k_inflow, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, st, g, 1, profile, plane, inlet_flux, 0.0,
+                       (const double *)sums, c->f[VFS_COOR], c->inflow_plane, c->f[VFS_CSI], c->f[VFS_ETA], c->f[VFS_ZET],
+                       c->f[VFS_NVERT], c->f[VFS_UCONT], c->f[VFS_UCAT], c->red);
+    hipLaunchKernelGGL(finish_reduce<OpSum>, dim3(1), dim3(RED_THREADS), 0, st, c->red, RED_BLOCKS, sums);
+    double *result = sums;
+    if (held && inlet_flux > 0) {
+        hipLaunchKernelGGL(k_inflow, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, st, g, 2, profile, plane, inlet_flux, inlet_flux,
+                           (const double *)sums, c->f[VFS_COOR], c->inflow_plane, c->f[VFS_CSI], c->f[VFS_ETA],
+                           c->f[VFS_ZET], c->f[VFS_NVERT], c->f[VFS_UCONT], c->f[VFS_UCAT], c->red);
+        hipLaunchKernelGGL(finish_reduce<OpSum>, dim3(1), dim3(RED_THREADS), 0, st, c->red, RED_BLOCKS, sums + 2);
+        result = sums + 2;
+    }
     ucat_stale(c);
-    hipLaunchKernelGGL(finish_reduce<OpSum>, dim3(1), dim3(RED_THREADS), 0, st, c->red, RED_BLOCKS, c->red + RED_BLOCKS);
     VFS_LAUNCH_CHECK("inflow flux");
-    VFS_HIP(hipMemcpyAsync(c->red_host, c->red + RED_BLOCKS, sizeof(double), hipMemcpyDeviceToHost, st));
+    VFS_HIP(hipMemcpyAsync(c->red_host, result, sizeof(double), hipMemcpyDeviceToHost, st));
     VFS_HIP(hipStreamSynchronize(st));
     c->p.flux_in_sum = c->red_host[0];  // FluxInSum, read by the outlet balance of FormBCS
     if (flux_in) *flux_in = c->red_host[0];
+    return VFS_OK;
+}
+
+// Boundary-face loops of FormBCS.  Face f (0..5 = i-, i+, j-, j+, k-, k+) is
+// iterated over the interior of the other two directions; (k, j, i) of its
+// t-th point on the ghost layer.
+__device__ inline bool face_point(const Grid &g, int f, long long t, int &k, int &j, int &i)
+{
+    const int a = f / 2;
+    const int n0 = a == 0 ? g.my - 2 : g.mx - 2, n1 = a == 2 ? g.my - 2 : g.mz - 2;
+    if (t >= (long long)n0 * n1) return false;
+    const int u = 1 + (int)(t % n0), v = 1 + (int)(t / n0);
+    const int w = (f & 1) ? (a == 0 ? g.mx - 1 : a == 1 ? g.my - 1 : g.mz - 1) : 0;
+    if (a == 0) { i = w; j = u; k = v; }
+    else if (a == 1) { j = w; i = u; k = v; }
+    else { k = w; i = u; j = v; }
+    return true;
+}
+
+// far field (code 6), bcs.c:1785-1888: the boundary flux of face f from the
+// adjacent cell's velocity component along the face's own axis times the
+// cell's matching metric component; partial sums of the flux and of the
+// ghost-face area (in: faces i-, j-, k-; out: i+, j+, k+).  With mode 1 the
+// flux-difference correction of bcs.c:1890-1993 (dv[0] = VelDiffIn,
+// dv[1] = VelDiffOut) is applied instead.
+__global__ void k_farfield(Grid g, int f, int mode, const double *__restrict__ dv, const double *__restrict__ ucat,
+                           const double *__restrict__ m_axis, double *__restrict__ ucont, double *__restrict__ part)
+{
+    const int a = f / 2, hi = f & 1;
+    double flux = 0, area = 0;
+    for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;; t += (long long)gridDim.x * blockDim.x) {
+        int k, j, i;
+        if (!face_point(g, f, t, k, j, i)) break;
+        // the face stored at the ghost index (low side) or at the last interior index (high side)
+        int kf = k, jf = j, if_ = i, kc = k, jc = j, ic = i;
+        if (hi) { if (a == 0) if_ = ic = i - 1; else if (a == 1) jf = jc = j - 1; else kf = kc = k - 1; }
+        else { if (a == 0) ic = i + 1; else if (a == 1) jc = j + 1; else kc = k + 1; }
+        const long long qf = gidx(g, kf, jf, if_), qc = gidx(g, kc, jc, ic), qg = gidx(g, k, j, i);
+        double v = ucat[3 * qc + a];
+        if (mode) v = hi ? v + dv[1] : v - dv[0];
+        ucont[3 * qf + a] = v * m_axis[3 * qc + a];
+        flux += ucont[3 * qf + a];
+        area += m_axis[3 * (hi ? qf : qg) + a];
+    }
+    if (mode) return;
+    flux = block_reduce<OpSum>(flux);
+    area = block_reduce<OpSum>(area);
+    if (threadIdx.x == 0) { part[blockIdx.x] = flux; part[RED_BLOCKS + blockIdx.x] = area; }
+}
+
+// VelDiffIn / VelDiffOut (bcs.c:1890-1902) from s = {FluxIn, AreaIn, FluxOut, AreaOut}
+__global__ void k_farfield_diff(const double *__restrict__ s, double *__restrict__ dv)
+{
+    const double FluxDiff = 0.5 * (s[0] - s[2]);
+    double in = FluxDiff / s[1], out = FluxDiff / s[3];
+    if (fabs(FluxDiff) < 1.e-6 || fabs(s[1]) < 1.e-6) in = 0.;
+    if (fabs(FluxDiff) < 1.e-6 || fabs(s[3]) < 1.e-6) out = 0.;
+    dv[0] = in;
+    dv[1] = out;
+}
+
+// partial sums over the interior of the k- ghost plane (mode 0: k- outlet,
+// bcs.c:2294-2320, the face flux of the first cell's velocity and zet.z of
+// fluid ghost cells) or over the faces k = mz-2 (mode 1: the characteristic
+// outlet's FluxOut, bcs.c:1997-2012)
+__global__ void k_kplane_sums(Grid g, int mode, const double *__restrict__ ucat, const double *__restrict__ ucont,
+                              const double *__restrict__ zet, const double *__restrict__ nvert, double *__restrict__ part)
+{
+    const long long n = (long long)(g.mx - 2) * (g.my - 2);
+    double s0 = 0, s1 = 0;
+    for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < n; t += (long long)gridDim.x * blockDim.x) {
+        const int i = 1 + t % (g.mx - 2), j = 1 + t / (g.mx - 2);
+        if (mode == 1) {
+            s0 += ucont[3 * gidx(g, g.mz - 2, j, i) + 2];
+            continue;
+        }
+        const long long q0 = gidx(g, 0, j, i), q1 = gidx(g, 1, j, i);
+        if (nvert[q0] < 0.1) {
+            s0 += (ucat[3 * q1] * (zet[3 * q0] + zet[3 * q1]) + ucat[3 * q1 + 1] * (zet[3 * q0 + 1] + zet[3 * q1 + 1]) +
+                   ucat[3 * q1 + 2] * (zet[3 * q0 + 2] + zet[3 * q1 + 2])) * 0.5;
+            s1 += zet[3 * q0 + 2];
+        }
+    }
+    s0 = block_reduce<OpSum>(s0);
+    s1 = block_reduce<OpSum>(s1);
+    if (threadIdx.x == 0) { part[blockIdx.x] = s0; part[RED_BLOCKS + blockIdx.x] = s1; }
+}
+
+// mode 0: the k- outlet fluxes ucont(0).z = (ucat(1).z + ratio) zet(0).z with
+// ratio = (FluxInSum - s[0]) / s[1] (bcs.c:2322-2338); mode 1: the
+// characteristic outlet's scaling ucont(mz-2).z *= ratio with ratio =
+// (FluxInSum + FarFluxIn) / FluxOut (bcs.c:2013-2037; s[0] = FluxOut, s[2] = FarFluxIn)
+__global__ void k_kplane_set(Grid g, int mode, double flux_in_sum, const double *__restrict__ s,
+                             const double *__restrict__ ucat, const double *__restrict__ zet, double *__restrict__ ucont)
+{
+    const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (t >= (long long)(g.mx - 2) * (g.my - 2)) return;
+    const int i = 1 + t % (g.mx - 2), j = 1 + t / (g.mx - 2);
+    if (mode == 0) {
+        const double ratio = (flux_in_sum - s[0]) / s[1];
+        const long long q0 = gidx(g, 0, j, i), q1 = gidx(g, 1, j, i);
+        ucont[3 * q0 + 2] = (ucat[3 * q1 + 2] + ratio) * zet[3 * q0 + 2];
+    } else {
+        const double FluxIn = flux_in_sum + s[2];
+        double ratio = FluxIn / s[0];
+        if (fabs(s[0]) < 1.e-6) ratio = 1.;
+        if (fabs(FluxIn) < 1.e-6) ratio = 0.;
+        const long long q = gidx(g, g.mz - 2, j, i);
+        ucont[3 * q + 2] = ucont[3 * q + 2] * ratio;
+    }
+}
+
+// far field and characteristic outlet of FormBCS (bcs.c:1745-2041);
+// scal[0..3] = far-field sums, scal[4..5] = VelDiffIn/Out, scal[6] = the
+// characteristic outlet's FluxOut, scal[8] = FarFluxIn again
+static int form_bcs_open(vfs_ctx *c)
+{
+    const Grid &g = c->g;
+    hipStream_t st = c->stream;
+    double *part = c->red, *scal = c->red + 3 * RED_BLOCKS;
+    const double *m_axis[3] = {c->f[VFS_CSI], c->f[VFS_ETA], c->f[VFS_ZET]};
+    bool far = false, changed = false;
+    for (int f = 0; f < 6; f++) far = far || g.bc[f] == 6;
+    VFS_HIP(hipMemsetAsync(scal, 0, 9 * sizeof(double), st));
+    if (far) {
+        for (int f = 0; f < 6; f++) {
+            if (g.bc[f] != 6) continue;
+            hipLaunchKernelGGL(k_farfield, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, st, g, f, 0, (const double *)nullptr,
+                               c->f[VFS_UCAT], m_axis[f / 2], c->f[VFS_UCONT], part);
+            // accumulate into scal[0..1] (in) or scal[2..3] (out) in face order
+            const int o = (f & 1) ? 2 : 0;
+            hipLaunchKernelGGL(finish_reduce_add<OpSum>, dim3(1), dim3(RED_THREADS), 0, st, part, RED_BLOCKS, scal + o);
+            hipLaunchKernelGGL(finish_reduce_add<OpSum>, dim3(1), dim3(RED_THREADS), 0, st, part + RED_BLOCKS, RED_BLOCKS,
+                               scal + o + 1);
+        }
+        if (g.bc[5] == 6) {
+            hipLaunchKernelGGL(k_farfield_diff, dim3(1), dim3(1), 0, st, scal, scal + 4);
+            const int order[6] = {5, 3, 1, 0, 2, 4};  // bcs.c:1922-1993
+            for (int f : order) {
+                if (g.bc[f] != 6) continue;
+                hipLaunchKernelGGL(k_farfield, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, st, g, f, 1, scal + 4,
+                                   c->f[VFS_UCAT], m_axis[f / 2], c->f[VFS_UCONT], part);
+            }
+        }
+        changed = true;
+    }
+    const long long np = (long long)(g.mx - 2) * (g.my - 2);
+    if (g.bc[5] == 8) {
+        hipLaunchKernelGGL(k_kplane_sums, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, st, g, 1, c->f[VFS_UCAT], c->f[VFS_UCONT],
+                           c->f[VFS_ZET], c->f[VFS_NVERT], part);
+        hipLaunchKernelGGL(finish_reduce<OpSum>, dim3(1), dim3(RED_THREADS), 0, st, part, RED_BLOCKS, scal + 6);
+        // k_kplane_set mode 1 reads FluxOut at s[0] and FarFluxIn at s[2]
+        VFS_HIP(hipMemcpyAsync(scal + 8, scal, sizeof(double), hipMemcpyDeviceToDevice, st));
+        // the scaling starts at the third step (bcs.c:2026-2036)
+        if (!(c->ti == 0 || c->ti == 1)) {
+            hipLaunchKernelGGL(k_kplane_set, dim3(blocks_for(np, TPB)), dim3(TPB), 0, st, g, 1, c->p.flux_in_sum, scal + 6,
+                               c->f[VFS_UCAT], c->f[VFS_ZET], c->f[VFS_UCONT]);
+            changed = true;
+        }
+    }
+    VFS_LAUNCH_CHECK("far field / characteristic outlet");
+    if (changed) ucat_stale(c);
+    return VFS_OK;
+}
+
+static int form_bcs_kminus_outlet(vfs_ctx *c)
+{
+    const Grid &g = c->g;
+    hipStream_t st = c->stream;
+    double *part = c->red, *scal = c->red + 3 * RED_BLOCKS + 16;
+    hipLaunchKernelGGL(k_kplane_sums, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, st, g, 0, c->f[VFS_UCAT], c->f[VFS_UCONT],
+                       c->f[VFS_ZET], c->f[VFS_NVERT], part);
+    hipLaunchKernelGGL(finish_reduce<OpSum>, dim3(1), dim3(RED_THREADS), 0, st, part, RED_BLOCKS, scal);
+    hipLaunchKernelGGL(finish_reduce<OpSum>, dim3(1), dim3(RED_THREADS), 0, st, part + RED_BLOCKS, RED_BLOCKS, scal + 1);
+    const long long np = (long long)(g.mx - 2) * (g.my - 2);
+    hipLaunchKernelGGL(k_kplane_set, dim3(blocks_for(np, TPB)), dim3(TPB), 0, st, g, 0, c->p.flux_in_sum, scal,
+                       c->f[VFS_UCAT], c->f[VFS_ZET], c->f[VFS_UCONT]);
+    VFS_LAUNCH_CHECK("k- outlet");
+    ucat_stale(c);
     return VFS_OK;
 }
 
@@ -655,6 +948,8 @@ int form_bcs(vfs_ctx *c, int outflow_scale)
     const Grid &g = c->g;
     hipStream_t st = c->stream;
     int rc = c2c(c);
+    if (rc) return rc;
+    rc = form_bcs_open(c);  // bcs.c:1745-2041
     if (rc) return rc;
     if (g.bc[5] == 4 && !g.pk) {
         hipLaunchKernelGGL(k_outlet_sums, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, st, g, c->f[VFS_UCONT],
@@ -670,6 +965,10 @@ int form_bcs(vfs_ctx *c, int outflow_scale)
             ucat_stale(c);
             VFS_LAUNCH_CHECK("outlet scale");
         }
+    }
+    if (g.bc[4] == 4) {  // bcs.c:2294-2340
+        rc = form_bcs_kminus_outlet(c);
+        if (rc) return rc;
     }
     long long nw = (long long)(g.my > g.mx ? g.my : g.mx) * g.mz;
     hipLaunchKernelGGL(k_wall_flux, dim3(blocks_for(nw, TPB)), dim3(TPB), 0, st, g, c->f[VFS_UCONT]);

@@ -60,4 +60,16 @@ __global__ void __launch_bounds__(RED_THREADS) finish_reduce(const double *__res
     if (threadIdx.x == 0) out[0] = v;
 }
 
+// The same, adding the result to out[0] (partial results of several passes
+// summed in launch order).
+template <class Op>
+__global__ void __launch_bounds__(RED_THREADS) finish_reduce_add(const double *__restrict__ part, int n,
+                                                                 double *__restrict__ out)
+{
+    double v = Op::init();
+    for (int q = threadIdx.x; q < n; q += blockDim.x) v = Op::op(v, part[q]);
+    v = block_reduce<Op>(v);
+    if (threadIdx.x == 0) out[0] = Op::op(out[0], v);
+}
+
 }  // namespace vfs
