@@ -175,6 +175,16 @@ int vfs_update_pressure(vfs_ctx *ctx);
 int vfs_projection(vfs_ctx *ctx);
 /* Divergence (main.c:936-1074): max |div U| * aj over fluid cells. */
 int vfs_divergence(vfs_ctx *ctx, double *maxdiv);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, serving loops): vfs_divergence on
+ * device arrays of the DA layout described by params (ucont: 3 doubles per
+ * point, aj and nvert: 1), enqueued on `stream` (a hipStream_t; NULL = the
+ * null stream) with the result written to the device double *maxdiv and no
+ * host synchronisation.  The per-block partials live in a fixed 8 KiB device
+ * array per GPU (1024 doubles), which calls on one GPU share: calls that
+ * overlap on one GPU race.  Grids of 2^31 points or more are refused. */
+int vfs_divergence_noalloc(const vfs_params *params, const double *ucont, const double *aj, const double *nvert,
+                           double *maxdiv, void *stream);
 
 /* ---- implicit momentum, -imp 4 (implicitsolver.c, momentum.c) ---------- */
 typedef struct {

@@ -144,6 +144,7 @@ _SIGS = {
     "vfs_update_pressure": ([C.c_void_p], C.c_int),
     "vfs_projection": ([C.c_void_p], C.c_int),
     "vfs_divergence": ([C.c_void_p, C.POINTER(C.c_double)], C.c_int),
+    "vfs_divergence_noalloc": ([C.POINTER(Params), C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p], C.c_int),
     "vfs_fractional_step": ([C.c_void_p, C.POINTER(SolveOpts), C.POINTER(StepStats)], C.c_int),
     "vfs_alm_create": ([C.c_void_p, C.POINTER(Turbine), C.c_int, C.c_int, C.c_void_p, C.c_int, C.c_void_p,
                         C.POINTER(C.c_void_p)], C.c_int),

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <mutex>
 
 #include "vfs_internal.h"
 #include "vfs_reduce.h"
@@ -408,4 +409,70 @@ int divergence(vfs_ctx *c, double *out)
     return VFS_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Divergence without device allocation, for callers that cannot allocate
+// (code under HIP graph capture, serving loops).  The same kernels as
+// divergence() write their per-block partials into `scratch`, a file-scope
+// device array of NOALLOC_PARTS doubles (8 KiB) that every GPU the library
+// runs on holds until the process ends.  Its device address is looked up
+// once per device (hipGetSymbolAddress, on the first call with that device
+// current) and only that address is handed to kernels.  Calls that overlap
+// on one GPU share the array and race.
+// ---------------------------------------------------------------------------
+
+constexpr int NOALLOC_PARTS = RED_BLOCKS;
+constexpr int NOALLOC_MAX_DEVICES = 64;
+static __device__ __align__(256) double scratch[NOALLOC_PARTS];
+
+static std::mutex g_scratch_mu;
+static double *g_scratch_ptr[NOALLOC_MAX_DEVICES] = {};
+
+static int scratch_address(double **out)
+{
+    int dev = -1;
+    VFS_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= NOALLOC_MAX_DEVICES) return set_error(VFS_EINVAL, "divergence_noalloc: device index out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) return check_hip(e != hipSuccess ? e : hipErrorInvalidSymbol, "hipGetSymbolAddress(scratch)");
+        g_scratch_ptr[dev] = static_cast<double *>(p);
+    }
+    *out = g_scratch_ptr[dev];
+    return VFS_OK;
+}
+
+int divergence_noalloc(const vfs_params &p, const double *ucont, const double *aj, const double *nvert, double *maxdiv,
+                       hipStream_t st)
+{
+    if (p.mx < 3 || p.my < 3 || p.mz < 3) return set_error(VFS_EINVAL, "divergence_noalloc: each DA size must be at least 3");
+    if ((long long)p.mx * p.my * p.mz >= (1LL << 31))
+        return set_error(VFS_EINVAL, "divergence_noalloc: grid too large: the DA must hold fewer than 2^31 points");
+    const Grid g = make_grid(p);
+    const int kc = km_reductions() ? km_chunk(g) : 0;
+    const long long nparts = kc > 0 ? (long long)km_blocks(g, kc) : RED_BLOCKS;
+    if (nparts > NOALLOC_PARTS)
+        return set_error(VFS_EINVAL, "divergence_noalloc: " + std::to_string(nparts) + " partial results exceed the " +
+                                         std::to_string(NOALLOC_PARTS) + " of the scratch array");
+    double *part = nullptr;
+    int rc = scratch_address(&part);
+    if (rc) return rc;
+    // every block writes its own partial, so nothing needs clearing between calls
+    if (kc > 0)
+        hipLaunchKernelGGL(k_divergence_km, dim3((unsigned)nparts), dim3(KM_I * KM_J), 0, st, g, kc, ucont, aj, nvert, part);
+    else
+        hipLaunchKernelGGL(k_divergence, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, st, g, ucont, aj, nvert, part);
+    hipLaunchKernelGGL(finish_reduce<OpMax>, dim3(1), dim3(RED_THREADS), 0, st, part, (int)nparts, maxdiv);
+    VFS_LAUNCH_CHECK("divergence_noalloc");
+    return VFS_OK;
+}
+
 }  // namespace vfs
+
+extern "C" int vfs_divergence_noalloc(const vfs_params *params, const double *ucont, const double *aj,
+                                      const double *nvert, double *maxdiv, void *stream)
+{
+    if (!params || !ucont || !aj || !nvert || !maxdiv) return vfs::set_error(VFS_EINVAL, "divergence_noalloc: null argument");
+    return vfs::divergence_noalloc(*params, ucont, aj, nvert, maxdiv, static_cast<hipStream_t>(stream));
+}
