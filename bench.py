@@ -68,6 +68,8 @@ ALGO = {
 
 # HBM bytes per launch of the finest-level launch, from rocprofv3 FETCH_SIZE (x2, calibrated on
 # copy kernels of known size) + WRITE_SIZE on this workload at 256x192x256
+TRAFFIC_SOURCE = ("not measured by this run: rocprofv3 --pmc FETCH_SIZE (x2.0) + WRITE_SIZE of the same kernel on "
+                  "this workload in a separate run, profiles/hbm_traffic_channel_256x192x256_pmc.md")
 MEASURED_TRAFFIC = {
     "formfunction2": 5.307e9,
     "face_fluxes": 3.119e9,
@@ -295,6 +297,7 @@ def main():
         roofline = {"bound": "hbm", "kernel": dom, "achieved": round(a, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                     "frac": round(a / HBM_PEAK_GBS, 4),
                     "traffic": MEASURED_TRAFFIC.get(dom) if (args.nx, args.ny, args.nz) == (256, 192, 256) else None,
+                    "traffic_source": TRAFFIC_SOURCE if (args.nx, args.ny, args.nz) == (256, 192, 256) else None,
                     "bytes_per_unit": bpu, "unit_of_work": unit, "units_per_launch": units[unit],
                     "avg_launch_ms": round(kern[dom]["avg_ms"], 4)}
     log("kernels (2 untimed steps, every kernel timed): " +

@@ -153,6 +153,10 @@ typedef struct {
     int coarse_sweeps;   /* coarsest-level sweeps when it is too big for a dense solve */
     int mg_fp32;         /* 1: FP32 V-cycle inside the FP64 FGMRES           */
     int use_guess;       /* 1: start from VFS_PHI; 0: from zero              */
+    int amg;             /* 1: algebraic multigrid preconditioner (the role of
+                            HYPRE BoomerAMG, poisson_hypre.c:34-97) in place
+                            of the geometric V-cycle; FP64, mg_fp32 ignored   */
+    double amg_thresh;   /* -amg_thresh: strength threshold (0.75)           */
 } vfs_solve_opts;
 
 typedef struct {
@@ -388,6 +392,31 @@ int vfs_grid_dims(const char *path, int format, int block, int *IM, int *JM, int
  * each value divided by cl and multiplied by L_dim as the reference does. */
 int vfs_grid_read(const char *path, int format, int block, double cl, double L_dim, int mx, int my, int mz,
                   double *coor);
+
+/* ---- algebraic multigrid, host side (diagnostics) ---------------------- */
+/* The setup of the AMG preconditioner (the role of HYPRE BoomerAMG,
+ * poisson_hypre.c:34-97: PMIS coarsening, FF1 interpolation, strength
+ * threshold theta = -amg_thresh, Galerkin coarse operators) on a CSR matrix
+ * given by the host; the solver builds the same hierarchy from its own
+ * operator (vfs_solve_opts.amg).  Levels are added while the last has more
+ * than max_coarse rows (512 when max_coarse <= 0) and fewer than max_levels
+ * exist. */
+typedef struct vfs_amg_host vfs_amg_host;
+int vfs_amg_host_build(int n, const int *ptr, const int *col, const double *val, double theta, int max_levels,
+                       int max_coarse, vfs_amg_host **out);
+int vfs_amg_host_levels(const vfs_amg_host *h, int *nlevels);
+/* which = 0: level's operator, 1: its interpolation to the next level; with
+ * null arrays only the sizes are returned. */
+int vfs_amg_host_matrix(const vfs_amg_host *h, int level, int which, int *rows, int *cols, long long *nnz, int *ptr,
+                        int *col, double *val);
+/* C/F splitting of a level: 1 = C-point, -1 = F-point. */
+int vfs_amg_host_splitting(const vfs_amg_host *h, int level, int *cf);
+/* The multicolour Gauss-Seidel smoother's colouring of a level (the GPU form
+ * of HYPRE's default hybrid Gauss-Seidel relaxation): rows grouped by colour
+ * (n entries) and colour offsets (ncolours + 1 entries); with null arrays
+ * only *ncolours is returned. */
+int vfs_amg_host_colouring(const vfs_amg_host *h, int level, int *ncolours, int *rows, int *offsets);
+int vfs_amg_host_free(vfs_amg_host *h);
 
 /* ---- measurement (no counterpart in the reference) -------------------- */
 /* Enable (1) or disable (0) HIP-event timing of the hot kernels' launches on

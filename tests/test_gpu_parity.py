@@ -429,3 +429,29 @@ def test_write_through_kept_pointer():
         o.set("ucont", u2)
         o.contra2cart()
         assert relerr(ctx.download("UCAT"), o["ucat"]) <= RHS_TOL
+
+
+def test_matrix_free_operator_bitwise(monkeypatch):
+    """With VFS_POISSON_MF=1 and no blanked point the finest FP64 operator
+    is applied matrix-free from nine face products per point; its products
+    and a whole FGMRES solve have the bits of the stored-coefficient path
+    (the default)."""
+    out = {}
+    for mf in ("1", "0"):
+        monkeypatch.setenv("VFS_POISSON_MF", mf)
+        c, o, ctx = channel_pair(24, 16, 20)
+        with ctx:
+            rng = np.random.default_rng(3)
+            x = rng.standard_normal(o.shape)
+            ctx.upload("PHI", x)
+            ctx.poisson_apply_dev(ctx.device_ptr("PHI"), ctx.device_ptr("RHS"))
+            y = ctx.download("RHS").reshape(-1)[: x.size].copy()
+            ctx.runge_kutta()
+            ctx.poisson_rhs()
+            ctx.upload("PHI", np.zeros(o.shape))
+            st = ctx.poisson_solve(vfs.default_solve_opts(tol=1e-10, use_guess=0))
+            out[mf] = (y, ctx.download("PHI"), st.iterations)
+    monkeypatch.delenv("VFS_POISSON_MF")
+    assert np.array_equal(out["1"][0], out["0"][0])
+    assert np.array_equal(out["1"][1], out["0"][1]) and out["1"][2] == out["0"][2]
+    assert relerr(out["1"][0].reshape(o.shape), o.poisson_apply(x)) <= RHS_TOL

@@ -55,6 +55,36 @@ def test_unet_forward_matches_oracle():
     net.close(); ctx.close()
 
 
+def test_k_marching_convolution_matches_tiled(monkeypatch):
+    """The k-marching convolution (halo ring and weights in LDS) against the
+    plane-by-plane one on a 4-level network over a grid whose i and j counts
+    are not multiples of the tile: the same layer outputs up to FP32
+    accumulation order (layers with more than one 16-channel chunk sum the
+    K terms in another order), and the forward output matches the oracle."""
+    c, o, ctx = channel_pair(40, 24, 24, les=1)
+    with ctx:
+        rng = np.random.default_rng(9)
+        shape = ctx.field_shape("B")
+        b = np.zeros(shape)
+        b[1:-1, 1:-1, 1:-1] = rng.standard_normal((shape[0] - 2, shape[1] - 2, shape[2] - 2))
+        ctx.upload("B", b)
+        outs = []
+        for km in ("1", "0"):
+            monkeypatch.setenv("VFS_UNET_KMARCH", km)
+            net = vfs.UNet(ctx, base=16, levels=4, seed=11)
+            net.forward(ctx.device_ptr("B"), ctx.device_ptr("P"))
+            outs.append(ctx.download("P"))
+            params = net.params(0).astype(np.float64)
+            net.close()
+        monkeypatch.delenv("VFS_UNET_KMARCH")
+        err = _nrel(outs[0], outs[1])
+        print(f"[k-marching vs tiled] {err:.3e}", flush=True)
+        assert err <= 1e-2
+        orc = uo.UNetOracle(16, 4)
+        out, cache = orc.forward(params, b[1:-1, 1:-1, 1:-1], bf16=True)
+        assert _nrel(outs[0][1:-1, 1:-1, 1:-1], out * cache["rms"]) <= 2e-2
+
+
 def test_unet_gradients_match_oracle():
     c, o, ctx, net, orc, b, phi = _setup()
     params = net.params(0).astype(np.float64)

@@ -281,3 +281,24 @@ def test_snes_residual_time_term_and_consistency():
     lin = (r2 - r) * (-dt)
     sel = np.abs(d) > 0
     assert np.allclose(lin[sel], d[sel], rtol=0, atol=1e-3 * np.abs(d).max())
+
+
+def test_oracle_thread_count_invariance():
+    """The oracle's OpenMP loops write disjoint cells and its reductions run
+    in a fixed order: one fractional step (les 2, Poisson to 1e-10) gives the
+    same bits on 1 thread and on 4."""
+    out = []
+    before = po.set_threads()
+    try:
+        for n in (1, 4):
+            po.set_threads(n)
+            c, o, _ = channel_pair(20, 14, 18, les=2, with_gpu=False)
+            o.p.ti = 5  # past the first steps, so the dynamic coefficient is computed
+            it, md, rel = o.fractional_step(tol=1e-10)
+            out.append((it, md, rel, o["ucont"].copy(), o["p"].copy(), o["nu_t"].copy()))
+    finally:
+        po.set_threads(before)
+    a, b = out
+    assert a[:3] == b[:3]
+    for x, y in zip(a[3:], b[3:]):
+        assert np.array_equal(x, y)

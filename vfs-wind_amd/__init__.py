@@ -56,7 +56,7 @@ class SolveOpts(C.Structure):
     _fields_ = [
         ("tol", C.c_double), ("restart", C.c_int), ("maxit", C.c_int), ("levels", C.c_int),
         ("pre", C.c_int), ("post", C.c_int), ("coarse_sweeps", C.c_int),
-        ("mg_fp32", C.c_int), ("use_guess", C.c_int),
+        ("mg_fp32", C.c_int), ("use_guess", C.c_int), ("amg", C.c_int), ("amg_thresh", C.c_double),
     ]
 
 
@@ -182,6 +182,14 @@ _SIGS = {
                                 C.POINTER(C.c_void_p)], C.c_int),
     "vfs_inflow_reader_next": ([C.c_void_p, C.c_int, C.c_void_p], C.c_int),
     "vfs_inflow_reader_close": ([C.c_void_p], C.c_int),
+    "vfs_amg_host_build": ([C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_double, C.c_int, C.c_int,
+                            C.POINTER(C.c_void_p)], C.c_int),
+    "vfs_amg_host_levels": ([C.c_void_p, C.POINTER(C.c_int)], C.c_int),
+    "vfs_amg_host_matrix": ([C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                             C.POINTER(C.c_longlong), C.c_void_p, C.c_void_p, C.c_void_p], C.c_int),
+    "vfs_amg_host_splitting": ([C.c_void_p, C.c_int, C.c_void_p], C.c_int),
+    "vfs_amg_host_colouring": ([C.c_void_p, C.c_int, C.POINTER(C.c_int), C.c_void_p, C.c_void_p], C.c_int),
+    "vfs_amg_host_free": ([C.c_void_p], C.c_int),
     "vfs_vec_write": ([C.c_char_p, C.c_void_p, C.c_longlong], C.c_int),
     "vfs_vec_read": ([C.c_char_p, C.c_void_p, C.c_longlong], C.c_int),
     "vfs_vec_length": ([C.c_char_p, C.POINTER(C.c_longlong)], C.c_int),
@@ -304,6 +312,63 @@ class InflowReader:
             pass
 
 
+class AmgHost:
+    """The AMG setup (PMIS, FF1, Galerkin) run on the host on a CSR matrix
+    (scipy.sparse-like: indptr, indices, data); levels' operators, splittings
+    and interpolations are returned as (indptr, indices, data, shape)."""
+
+    def __init__(self, indptr, indices, data, theta=0.75, max_levels=25, max_coarse=0):
+        self.lib = load_library()
+        ip = np.ascontiguousarray(indptr, dtype=np.int32)
+        ix = np.ascontiguousarray(indices, dtype=np.int32)
+        dv = np.ascontiguousarray(data, dtype=np.float64)
+        h = C.c_void_p()
+        _raise(self.lib, self.lib.vfs_amg_host_build(len(ip) - 1, ip.ctypes.data, ix.ctypes.data, dv.ctypes.data, theta,
+                                                     max_levels, max_coarse, C.byref(h)))
+        self.h = h
+        nl = C.c_int()
+        _raise(self.lib, self.lib.vfs_amg_host_levels(self.h, C.byref(nl)))
+        self.levels = nl.value
+
+    def matrix(self, level: int, which: int = 0):
+        r, c, z = C.c_int(), C.c_int(), C.c_longlong()
+        _raise(self.lib, self.lib.vfs_amg_host_matrix(self.h, level, which, C.byref(r), C.byref(c), C.byref(z), None, None,
+                                                      None))
+        ptr = np.empty(r.value + 1, np.int32)
+        col = np.empty(z.value, np.int32)
+        val = np.empty(z.value)
+        _raise(self.lib, self.lib.vfs_amg_host_matrix(self.h, level, which, None, None, None, ptr.ctypes.data,
+                                                      col.ctypes.data, val.ctypes.data))
+        return ptr, col, val, (r.value, c.value)
+
+    def splitting(self, level: int) -> np.ndarray:
+        n = self.matrix(level, 0)[3][0]
+        out = np.empty(n, np.int32)
+        _raise(self.lib, self.lib.vfs_amg_host_splitting(self.h, level, out.ctypes.data))
+        return out
+
+    def colouring(self, level: int):
+        """(rows grouped by colour, colour offsets) of the smoother on a level."""
+        n = self.matrix(level, 0)[3][0]
+        nc = C.c_int()
+        _raise(self.lib, self.lib.vfs_amg_host_colouring(self.h, level, C.byref(nc), None, None))
+        rows = np.empty(n, np.int32)
+        off = np.empty(nc.value + 1, np.int32)
+        _raise(self.lib, self.lib.vfs_amg_host_colouring(self.h, level, C.byref(nc), rows.ctypes.data, off.ctypes.data))
+        return rows, off
+
+    def close(self):
+        if self.h:
+            self.lib.vfs_amg_host_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def grid_dims(path, fmt: str = "text", block: int = 0):
     """(IM, JM, KM) node counts of one block of a grid.dat / xyz.dat file."""
     lib = load_library()
@@ -347,10 +412,11 @@ def make_params(mx, my, mz, *, per=(1, 0, 1), bctype=(100, 100, 1, 1, 100, 100),
 
 
 def default_solve_opts(tol=1e-8, restart=20, maxit=200, levels=6, pre=2, post=3, coarse_sweeps=20, mg_fp32=1,
-                       use_guess=1) -> SolveOpts:
+                       use_guess=1, amg=0, amg_thresh=0.75) -> SolveOpts:
     o = SolveOpts()
     o.tol, o.restart, o.maxit, o.levels = tol, restart, maxit, levels
     o.pre, o.post, o.coarse_sweeps, o.mg_fp32, o.use_guess = pre, post, coarse_sweeps, mg_fp32, use_guess
+    o.amg, o.amg_thresh = amg, amg_thresh
     return o
 
 

@@ -814,12 +814,23 @@ __global__ void __launch_bounds__(FT_I * FT_J) k_cell_rhs(CellArgs a, const doub
             have = false;
         }
         if (a.pucont) {
-            // Rhs = 0 + F; Rhs += -1*dP; Ucont = alpha*Rhs + pUcont  (rhs.c:1647-1658)
+            // Rhs = 0 + F; Rhs += -1*dP; Ucont = alpha*Rhs + pUcont  (rhs.c:1647-1658).
+            // Every input is loaded before the three stores, which then leave
+            // together (a store between loads wrote partial lines to HBM).
+            const double *__restrict__ dp = a.dp + 3 * q;
+            const double *__restrict__ pu = a.pucont + 3 * q;
+            double d[3], u[3], o[3];
+#pragma unroll
+            for (int c = 0; c < 3; c++) { d[c] = dp[c]; u[c] = pu[c]; }
+#pragma unroll
             for (int c = 0; c < 3; c++) {
                 double rv = zero[c] ? 0.0 : (0.0 + r[c]);
-                rv += -1 * a.dp[3 * q + c];
-                a.ucont_out[3 * q + c] = a.alpha * rv + a.pucont[3 * q + c];
+                rv += -1 * d[c];
+                o[c] = a.alpha * rv + u[c];
             }
+            double *__restrict__ out = a.ucont_out + 3 * q;
+#pragma unroll
+            for (int c = 0; c < 3; c++) out[c] = o[c];
         } else {
             for (int c = 0; c < 3; c++) a.rhs[3 * q + c] = zero[c] ? 0.0 : a.rhs[3 * q + c] + r[c];
         }
@@ -876,12 +887,23 @@ __global__ void __launch_bounds__(CR_I * CR_J) k_cell_rhs_pr(CellArgs a, const d
             have = false;
         }
         if (a.pucont) {
-            // Rhs = 0 + F; Rhs += -1*dP; Ucont = alpha*Rhs + pUcont  (rhs.c:1647-1658)
+            // Rhs = 0 + F; Rhs += -1*dP; Ucont = alpha*Rhs + pUcont  (rhs.c:1647-1658).
+            // Every input is loaded before the three stores, which then leave
+            // together (a store between loads wrote partial lines to HBM).
+            const double *__restrict__ dp = a.dp + 3 * q;
+            const double *__restrict__ pu = a.pucont + 3 * q;
+            double d[3], u[3], o[3];
+#pragma unroll
+            for (int c = 0; c < 3; c++) { d[c] = dp[c]; u[c] = pu[c]; }
+#pragma unroll
             for (int c = 0; c < 3; c++) {
                 double rv = zero[c] ? 0.0 : (0.0 + r[c]);
-                rv += -1 * a.dp[3 * q + c];
-                a.ucont_out[3 * q + c] = a.alpha * rv + a.pucont[3 * q + c];
+                rv += -1 * d[c];
+                o[c] = a.alpha * rv + u[c];
             }
+            double *__restrict__ out = a.ucont_out + 3 * q;
+#pragma unroll
+            for (int c = 0; c < 3; c++) out[c] = o[c];
         } else {
             for (int c = 0; c < 3; c++) a.rhs[3 * q + c] = zero[c] ? 0.0 : a.rhs[3 * q + c] + r[c];
         }

@@ -89,14 +89,38 @@ __device__ inline void face_g(const Grid &g, const FaceSet &F, int k, int j, int
     gg[8] = (kz[0] * kz[0] + kz[1] * kz[1] + kz[2] * kz[2]) * kaj;
 }
 
-// one row of PoissonLHSNew (poisson.c:1159-1561), poisson==1 form
-__device__ void assemble_row(const Grid &g, const FaceSet &F, const double *nvert, double th, int k, int j, int i,
+// the nine face metric products of point (k, j, i), precomputed planes
+struct FaceG {
+    const double *gp;  // 9 planes of g.n values
+    long long n;
+};
+__device__ inline void face_g(const Grid &g, const FaceG &F, int k, int j, int i, double gg[9])
+{
+    const long long q = gidx(g, k, j, i);
+#pragma unroll
+    for (int c = 0; c < 9; c++) gg[c] = F.gp[c * F.n + q];
+}
+
+__global__ void k_face_g(Grid g, FaceSet F, double *__restrict__ gp)
+{
+    const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (t >= g.n) return;
+    const int t32 = (int)t; const int i = t32 % g.mx; const int r = t32 / g.mx; const int j = r % g.my, k = r / g.my;
+    double gg[9];
+    face_g(g, F, k, j, i, gg);
+    for (int c = 0; c < 9; c++) gp[c * g.n + t] = gg[c];
+}
+
+// one row of PoissonLHSNew (poisson.c:1159-1561), poisson==1 form; with
+// NB (no point blanked) every nvert read is the constant 0
+template <bool NB = false, class FS = FaceSet>
+__device__ void assemble_row(const Grid &g, const FS &F, const double *nvert, double th, int k, int j, int i,
                              double vol[NST])
 {
     const int mx = g.mx, my = g.my, mz = g.mz, pi_ = g.pi, pj = g.pj, pk = g.pk;
     const double r = 1.0;
     for (int s = 0; s < NST; s++) vol[s] = 0.;
-#define NV(kk, jj, ii) nvert[gidx(g, (kk), (jj), (ii))]
+#define NV(kk, jj, ii) (NB ? 0.0 : nvert[gidx(g, (kk), (jj), (ii))])
     double gg[9];
     if (NV(k, j, i + 1) < th && (i != mx - 2 || pi_)) {
         face_g(g, F, k, j, i, gg);
@@ -308,6 +332,40 @@ __global__ void __launch_bounds__(PTPB) k_apply(Grid g, const T *__restrict__ co
     T s = 0;
 #pragma unroll
     for (int q = 0; q < NQ; q++) s += coef[q * g.n + t] * (T)x[nbr_idx(g, k, j, i, q)];
+    y[t] = b ? b[t] - s : s;
+}
+
+// The same product without stored coefficients, for grids with no blanked
+// point: each row's 19 coefficients are rebuilt from the nine precomputed
+// face metric products of its own point and of its i-1, j-1 and k-1
+// neighbours, in the assembly's operation order, and summed in the stored
+// path's slot order, so the result has the same bits as k_apply on the
+// assembled coefficients (the row mask drops the ghost columns as there).
+template <typename T, typename X = T>
+__global__ void __launch_bounds__(PTPB) k_apply_mf(Grid g, FaceG F, double th, const unsigned char *__restrict__ row,
+                                                   const X *__restrict__ x, const T *__restrict__ b, T *__restrict__ y)
+{
+    long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (t >= g.n) return;
+    if (!row[t]) { y[t] = 0; return; }
+    const int t32 = (int)t; int i = t32 % g.mx; int r = t32 / g.mx; int j = r % g.my, k = r / g.my;
+    double vol[NST];
+    assemble_row<true>(g, F, nullptr, th, k, j, i, vol);
+    T s = 0;
+    if (i >= 2 && i <= g.mx - 3 && j >= 2 && j <= g.my - 3 && k >= 2 && k <= g.mz - 3) {
+        // every neighbour is an interior point, hence a row (no point is
+        // blanked on this path): no wrap and no mask reads
+#pragma unroll
+        for (int q = 0; q < NST; q++)
+            s += (T)vol[q] * (T)x[t + ((long long)soff(q, 2) * g.my + soff(q, 1)) * g.mx + soff(q, 0)];
+    } else {
+#pragma unroll
+        for (int q = 0; q < NST; q++) {
+            const long long nb = nbr_idx(g, k, j, i, q);
+            const T c = (q == 0 || row[nb]) ? (T)vol[q] : (T)0.;
+            s += c * (T)x[nb];
+        }
+    }
     y[t] = b ? b[t] - s : s;
 }
 
@@ -1026,6 +1084,9 @@ struct PLevel {
     unsigned char *row = nullptr;
     double *c64 = nullptr;
     float *c32 = nullptr;
+    // nine face metric products per point (finest level, no point blanked):
+    // the FP64 operator is then applied matrix-free (k_apply_mf)
+    double *fg = nullptr;
     // colour-split smoother coefficients + Thomas factors, one-colour line rhs
     // (FP32: sc32 = factors, sh16 = FP16 coefficient ratios, sd32 = diagonal)
     float *sc32 = nullptr, *rz32 = nullptr, *sd32 = nullptr;
@@ -1052,8 +1113,19 @@ struct SubGraph {
     int pre = -1, post = -1, coarse = -1;
 };
 
+namespace vfs {
+struct AmgDev;
+int amg_build(const double *c64_host, const unsigned char *row_host, long long n_pts,
+              long long (*nbr)(const void *g, long long p, int s), const void *grid, double theta, AmgDev **out);
+int amg_apply(hipStream_t st, AmgDev *a, long long n_pts, const double *v, double *z);
+void amg_free(AmgDev *a);
+double amg_theta(const AmgDev *a);
+}  // namespace vfs
+
 struct vfs_poisson {
-    int nl = 0;
+    vfs::AmgDev *amg = nullptr;  // algebraic multigrid hierarchy of level 0 (when requested)
+    int nl = 0;          // levels built
+    int req_levels = 0;  // levels requested when it was built
     int restart = 0;
     PLevel L[8];
     double *V = nullptr, *Z = nullptr, *w = nullptr, *dev_scalars = nullptr;
@@ -1087,6 +1159,7 @@ void poisson_free(vfs_ctx *c)
     for (SubGraph &s : c->ps->sub)
         if (s.exec) hipGraphExecDestroy(s.exec);
     for (void *p : c->ps->allocs) hipFree(p);
+    amg_free(c->ps->amg);
     delete c->ps;
     c->ps = nullptr;
 }
@@ -1255,7 +1328,9 @@ static int level_dense_inverse(vfs_ctx *c, vfs_poisson *ps, PLevel &L)
 
 static int poisson_setup(vfs_ctx *c, int levels)
 {
-    if (c->ps && c->ps->nl == levels) return VFS_OK;
+    // the hierarchy is kept while the requested level count is the same (the
+    // built count can be smaller: coarsening stops at an odd cell count)
+    if (c->ps && c->ps->req_levels == levels) return VFS_OK;
     // the solver's point kernels decode (i, j, k) in 32-bit arithmetic
     if (c->g.n >= (1LL << 31)) return set_error(VFS_EINVAL, "Poisson solver: grid of 2^31 points or more");
     poisson_free(c);
@@ -1281,6 +1356,29 @@ static int poisson_setup(vfs_ctx *c, int levels)
                             c->f[VFS_KCSI], c->f[VFS_KETA], c->f[VFS_KZET], c->f[VFS_KAJ], c->f[VFS_CENT]};
     rc = level_assemble(c, ps->L[0], m0, c->f[VFS_NVERT]);
     if (rc) return rc;
+    {
+        // VFS_POISSON_MF=1 (no blanked point): the finest FP64 operator runs
+        // matrix-free from nine face products per point.  Same bits as the
+        // stored coefficients, and at 256x192x256 the same time with an FP32
+        // x (395 against 400 us) and slower with an FP64 one (506 against
+        // 479 us): the stored apply streams its 152 B of coefficients at about
+        // 6.3 TB/s, while the matrix-free one is bound by its 37 loads per point
+        // (18 face products of four points, 19 neighbours), not by bytes.  The
+        // stored operator is kept by default (the smoother set-up and the AMG
+        // read it anyway).
+        bool nb = false;
+        rc = nvert_all_zero(c, &nb);
+        if (rc) return rc;
+        const char *e = std::getenv("VFS_POISSON_MF");
+        if (nb && e && e[0] == '1') {
+            PLevel &L = ps->L[0];
+            rc = palloc(ps, (void **)&L.fg, 9 * (size_t)L.g.n * sizeof(double));
+            if (rc) return rc;
+            FaceSet F = {m0[4], m0[5], m0[6], m0[7], m0[8], m0[9], m0[10], m0[11], m0[12], m0[13], m0[14], m0[15]};
+            hipLaunchKernelGGL(k_face_g, dim3(blocks_for(L.g.n, PTPB)), dim3(PTPB), 0, c->stream, L.g, F, L.fg);
+            VFS_LAUNCH_CHECK("face products");
+        }
+    }
     for (int l = 1; l < nl; l++) {
         PLevel &Lf = ps->L[l - 1], &Lc = ps->L[l];
         vfs_params pc = c->p;
@@ -1307,6 +1405,7 @@ static int poisson_setup(vfs_ctx *c, int levels)
         if (rc) return rc;
     }
     if (nl > 1) rc = level_dense_inverse(c, ps, ps->L[nl - 1]);
+    if (!rc) ps->req_levels = levels;
     return rc;
 }
 
@@ -1474,6 +1573,49 @@ static void vcycle(vfs_ctx *c, vfs_poisson *ps, int l, int pre, int post, int co
 // With z32 the FP32 V-cycle's finest iterate is z32 itself (no FP64 copy);
 // the FP64 V-cycle iterates in z directly.  The finest level's kernels take
 // the iterate pointer at launch (the captured coarse levels never see it).
+// y = A x or r = b - A x with the finest-level FP64 operator: matrix-free
+// when the face products are there, else from the stored coefficients
+template <typename X>
+static void apply0(vfs_ctx *c, const X *x, const double *b, double *y)
+{
+    PLevel &L = c->ps->L[0];
+    const unsigned nb = blocks_for(L.g.n, PTPB);
+    if (L.fg) {
+        const FaceG F{L.fg, L.g.n};
+        hipLaunchKernelGGL((k_apply_mf<double, X>), dim3(nb), dim3(PTPB), 0, c->stream, L.g, F, c->p.poisson_threshold,
+                           L.row, x, b, y);
+    } else {
+        hipLaunchKernelGGL((k_apply<double, X>), dim3(nb), dim3(PTPB), 0, c->stream, L.g, L.c64, L.row, x, b, y);
+    }
+}
+
+// neighbour point of point p in stencil slot s on the level-0 grid
+static long long amg_nbr(const void *grid, long long p, int s)
+{
+    const Grid &g = *static_cast<const Grid *>(grid);
+    const int i = (int)(p % g.mx), j = (int)((p / g.mx) % g.my), k = (int)(p / g.plane);
+    return nbr_idx(g, k, j, i, s);
+}
+
+// The AMG hierarchy of the level-0 operator for the requested threshold
+// (built on the host from the assembled FP64 coefficients).
+static int amg_ensure(vfs_ctx *c, const vfs_solve_opts *o)
+{
+    vfs_poisson *ps = c->ps;
+    const double theta = o->amg_thresh > 0 && o->amg_thresh < 1 ? o->amg_thresh : 0.75;
+    if (ps->amg && amg_theta(ps->amg) == theta) return VFS_OK;
+    amg_free(ps->amg);
+    ps->amg = nullptr;
+    PLevel &L = ps->L[0];
+    const long long n = L.g.n;
+    std::vector<double> c64((size_t)NST * n);
+    std::vector<unsigned char> row(n);
+    VFS_HIP(hipStreamSynchronize(c->stream));
+    VFS_HIP(hipMemcpy(c64.data(), L.c64, c64.size() * sizeof(double), hipMemcpyDeviceToHost));
+    VFS_HIP(hipMemcpy(row.data(), L.row, n, hipMemcpyDeviceToHost));
+    return amg_build(c64.data(), row.data(), n, amg_nbr, &L.g, theta, &ps->amg);
+}
+
 static int precondition(vfs_ctx *c, const vfs_solve_opts *o, const double *v, double *z, float *z32 = nullptr,
                         bool b32_ready = false)
 {
@@ -1482,6 +1624,7 @@ static int precondition(vfs_ctx *c, const vfs_solve_opts *o, const double *v, do
     PLevel &L = ps->L[0];
     const long long n = L.g.n;
     const unsigned nb = blocks_for(n, PTPB);
+    if (o->amg) return amg_apply(st, ps->amg, n, v, z);
     if (o->mg_fp32) {
         if (!b32_ready) hipLaunchKernelGGL((k_convert<double, float>), dim3(nb), dim3(PTPB), 0, st, n, v, L.b32);
         float *x0 = L.x32;
@@ -1584,9 +1727,7 @@ int poisson_apply(vfs_ctx *c, const double *x, double *y)
 {
     int rc = poisson_setup(c, c->ps ? c->ps->nl : 4);
     if (rc) return rc;
-    PLevel &L = c->ps->L[0];
-    hipLaunchKernelGGL(k_apply<double>, dim3(blocks_for(L.g.n, PTPB)), dim3(PTPB), 0, c->stream, L.g, L.c64, L.row, x,
-                       (const double *)nullptr, y);
+    apply0(c, x, (const double *)nullptr, y);
     VFS_LAUNCH_CHECK("poisson apply");
     VFS_HIP(hipStreamSynchronize(c->stream));
     return VFS_OK;
@@ -1613,17 +1754,20 @@ static int fgmres(vfs_ctx *c, const vfs_solve_opts *o, vfs_unet *net, vfs_solve_
     // the FP32 V-cycle's outputs are kept as FP32 vectors in Z's memory (exact);
     // the network's direction is rounded to FP32 as well, and A is applied to
     // the rounded vector, so the FGMRES relation holds for the stored z exactly
-    const bool z32 = o->mg_fp32 != 0;
+    const bool z32 = o->mg_fp32 != 0 && !o->amg;
+    if (o->amg) {
+        rc = amg_ensure(c, o);
+        if (rc) return rc;
+    }
     float *Z32 = reinterpret_cast<float *>(Z);
     if (!o->use_guess) hipLaunchKernelGGL(k_zero<double>, dim3(nb), dim3(PTPB), 0, st, n, x);
     double d[MAXV + 1];
     if (net) {
         // x += alpha y, y = net(b - A x), alpha = <r, A y> / <A y, A y>
-        hipLaunchKernelGGL(k_apply<double>, dim3(nb), dim3(PTPB), 0, st, L.g, L.c64, L.row, x, b, w);
+        apply0(c, (const double *)x, b, w);
         rc = vfs_unet_forward(net, w, Z);
         if (rc) return rc;
-        hipLaunchKernelGGL(k_apply<double>, dim3(nb), dim3(PTPB), 0, st, L.g, L.c64, L.row, (const double *)Z,
-                           (const double *)nullptr, V + n);
+        apply0(c, (const double *)Z, (const double *)nullptr, V + n);
         rc = multidot(c, 1, V + n, w, d);
         if (rc) return rc;
         const double rAy = d[0];
@@ -1650,7 +1794,7 @@ static int fgmres(vfs_ctx *c, const vfs_solve_opts *o, vfs_unet *net, vfs_solve_
     const bool trace = tr && tr[0] == '1';
     while (it < maxit) {
         // r = b - A x into V_0
-        hipLaunchKernelGGL(k_apply<double>, dim3(nb), dim3(PTPB), 0, st, L.g, L.c64, L.row, x, b, V);
+        apply0(c, (const double *)x, b, V);
         rc = multidot(c, 0, V, V, d);
         if (rc) return rc;
         double beta = std::sqrt(d[0]);
@@ -1676,10 +1820,9 @@ static int fgmres(vfs_ctx *c, const vfs_solve_opts *o, vfs_unet *net, vfs_solve_
             if (rc) return rc;
             prof_begin(c, K_APPLY64);
             if (z32)
-                hipLaunchKernelGGL((k_apply<double, float>), dim3(nb), dim3(PTPB), 0, st, L.g, L.c64, L.row, (const float *)zj32,
-                                   (const double *)nullptr, w);
+                apply0(c, (const float *)zj32, (const double *)nullptr, w);
             else
-                hipLaunchKernelGGL(k_apply<double>, dim3(nb), dim3(PTPB), 0, st, L.g, L.c64, L.row, zj, (const double *)nullptr, w);
+                apply0(c, (const double *)zj, (const double *)nullptr, w);
             prof_end(c);
             // classical Gram-Schmidt, re-orthogonalised when the norm drops;
             // the first pass also forms the dot products of the second
@@ -1734,7 +1877,7 @@ static int fgmres(vfs_ctx *c, const vfs_solve_opts *o, vfs_unet *net, vfs_solve_
         VFS_LAUNCH_CHECK("fgmres update");
     }
     if (!converged) {
-        hipLaunchKernelGGL(k_apply<double>, dim3(nb), dim3(PTPB), 0, st, L.g, L.c64, L.row, x, b, w);
+        apply0(c, (const double *)x, b, w);
         rc = multidot(c, 0, V, w, d);
         if (rc) return rc;
         rel = std::sqrt(d[0]) / bnorm;

@@ -27,6 +27,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <cstdlib>
 #include <cstring>
 #include <functional>
 #include <random>
@@ -225,14 +226,174 @@ __global__ void k_sum_partials(const float *__restrict__ part, int P, long long 
     out[i] = s;
 }
 
+// The same convolution marching along k, for layers whose three halo planes
+// of all KC input channels and whole packed weight set fit in LDS together
+// (conv_m_lds): a block owns a 16 x 4*MT column of points over a run of kr
+// planes.  Three halo planes sit in an LDS ring; while the matrix cores work
+// on plane k, each thread's share of plane k + 2 is already in flight into
+// registers, and it replaces plane k - 1 in the ring afterwards.  Each input
+// plane is read from memory once per run instead of three times, and the
+// weights once per block.  Wf is packed as one chunk of KC channels
+// (k_pack with cc_all): Wf[(s * NT + t) * 64 + lane].
+__host__ __device__ constexpr int conv_m_ring(int mt, int kc) { return 3 * 18 * (4 * mt + 2) * (2 * kc + 16); }
+__host__ __device__ constexpr int conv_m_lds(int nt, int kc)
+{
+    return conv_m_ring(conv_mt(nt), kc) + conv_ks(kc) * nt * 64 * 16;
+}
+
+// UPCAT: the decoder's concatenation is read in place: input channels
+// [0, cu) are the half-resolution tensor Xc upsampled (nearest), channels
+// [cu, KC) the skip tensor X (KC - cu channels), so the concatenated tensor
+// is never written.
+template <int NT, int MT, int KC, bool RELU, bool OUT_F32, bool UPCAT = false>
+__global__ void __launch_bounds__(CV_TPB) k_conv3m(Dims d, int kr, const bf16 *__restrict__ X, const bf16 *__restrict__ Xc,
+                                                   int cu, const s16x8 *__restrict__ Wf, const float *__restrict__ bias,
+                                                   void *__restrict__ Yv)
+{
+    constexpr int TJ = 4 * MT, HI = 18, HJ = TJ + 2, PP = HI * HJ;
+    constexpr int PS = 2 * KC + 16;          // bytes per halo point (16 lanes, 16 points: distinct banks)
+    constexpr int U = KC / 8;                // 16-byte units per halo point
+    constexpr int KS = conv_ks(KC);
+    constexpr int NPF = (PP * U + CV_TPB - 1) / CV_TPB;  // halo units per thread and plane
+    constexpr int PLANE = PP * PS, RING = 3 * PLANE;
+    extern __shared__ __align__(16) unsigned char lds[];
+    const int cout = 16 * NT;
+    const int nbi = (d.ni + 15) / 16, nbj = (d.nj + TJ - 1) / TJ;
+    const int bi = blockIdx.x % nbi, bj = (blockIdx.x / nbi) % nbj, bk = blockIdx.x / (nbi * nbj);
+    const int i0 = bi * 16, j0 = bj * TJ, k0 = bk * kr;
+    const int k1 = k0 + kr < d.nk ? k0 + kr : d.nk;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int m = lane & 15, g = lane >> 4;
+    const int cs = UPCAT ? KC - cu : KC;                // channels read from X
+    const long long xplane = (long long)d.ni * d.nj * cs;
+    const long long cplane = UPCAT ? (long long)(d.ni / 2) * (d.nj / 2) * cu : 0;
+    // the layer's weights, once per block
+    s16x8 *Bs = reinterpret_cast<s16x8 *>(lds + RING);
+    for (int q = threadIdx.x; q < KS * NT * 64; q += CV_TPB) Bs[q] = Wf[q];
+    // this thread's halo units: the same (i, j, channel) offsets on every plane
+    int goff[NPF], loff[NPF];
+    unsigned coarse = 0;  // bit n: unit n reads the half-resolution tensor
+#pragma unroll
+    for (int n = 0; n < NPF; n++) {
+        const int u = threadIdx.x + n * CV_TPB;
+        goff[n] = -1;
+        loff[n] = -1;
+        if (u < PP * U) {
+            const int hp = u / U, c8 = u - hp * U;
+            const int hi = hp % HI, hj = hp / HI;
+            const int ii = i0 - 1 + hi, jj = j0 - 1 + hj;
+            loff[n] = hp * PS + 16 * c8;
+            if (ii >= 0 && ii < d.ni && jj >= 0 && jj < d.nj) {
+                if (UPCAT && 8 * c8 < cu) {
+                    goff[n] = ((jj >> 1) * (d.ni >> 1) + (ii >> 1)) * cu + 8 * c8;
+                    coarse |= 1u << n;
+                } else {
+                    goff[n] = (jj * d.ni + ii) * cs + 8 * c8 - (UPCAT ? cu : 0);
+                }
+            }
+        }
+    }
+    s16x8 v[NPF];
+    auto fetch = [&](int kk) {
+        const bool in = kk >= 0 && kk < d.nk;
+        const bf16 *xp = X + (long long)kk * xplane;
+        const bf16 *cp = UPCAT ? Xc + (long long)(kk >> 1) * cplane : nullptr;
+#pragma unroll
+        for (int n = 0; n < NPF; n++) {
+            v[n] = s16x8{0, 0, 0, 0, 0, 0, 0, 0};
+            if (in && goff[n] >= 0)
+                v[n] = *reinterpret_cast<const s16x8 *>((UPCAT && (coarse >> n & 1u) ? cp : xp) + goff[n]);
+        }
+    };
+    auto put = [&](int slot) {
+#pragma unroll
+        for (int n = 0; n < NPF; n++)
+            if (loff[n] >= 0) *reinterpret_cast<s16x8 *>(lds + slot * PLANE + loff[n]) = v[n];
+    };
+    // ring slot of plane kk: (kk - k0 + 1) mod 3
+    fetch(k0 - 1);
+    put(0);
+    fetch(k0);
+    put(1);
+    fetch(k0 + 1);
+    put(2);
+    __syncthreads();
+    int base[MT];
+#pragma unroll
+    for (int r = 0; r < MT; r++) base[r] = (wave * MT + r + 1) * HI + m + 1;
+    for (int k = k0; k < k1; k++) {
+        const bool next = k + 1 < k1;
+        if (next) fetch(k + 2);  // in flight during the matrix work
+        const int s0 = (k - k0) % 3;  // slot of plane k - 1
+        const int sm = s0 * PLANE, sc = (s0 == 2 ? 0 : s0 + 1) * PLANE, sp = (s0 == 0 ? 2 : s0 - 1) * PLANE;
+        f32x4 acc[MT][NT];
+#pragma unroll
+        for (int r = 0; r < MT; r++)
+#pragma unroll
+            for (int t = 0; t < NT; t++) acc[r][t] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int s = 0; s < KS; s++) {
+            const int kf = 32 * s + 8 * g;
+            const int tap = kf / KC, ch = kf - tap * KC;
+            bf16x8 b[NT];
+#pragma unroll
+            for (int t = 0; t < NT; t++) b[t] = __builtin_bit_cast(bf16x8, Bs[(s * NT + t) * 64 + lane]);
+            bf16x8 a[MT];
+            if (tap < 27) {
+                const int dk = tap / 9, dj = (tap / 3) % 3 - 1, di = tap % 3 - 1;
+                const int pl = dk == 0 ? sm : dk == 1 ? sc : sp;
+                const int off = dj * HI + di;
+#pragma unroll
+                for (int r = 0; r < MT; r++)
+                    a[r] = *reinterpret_cast<const bf16x8 *>(lds + pl + (base[r] + off) * PS + 2 * ch);
+            } else {
+#pragma unroll
+                for (int r = 0; r < MT; r++) a[r] = bf16x8{};
+            }
+#pragma unroll
+            for (int r = 0; r < MT; r++)
+#pragma unroll
+                for (int t = 0; t < NT; t++) acc[r][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[r], b[t], acc[r][t], 0, 0, 0);
+        }
+        // lane holds rows 4 g + q (points along i) of column m (channel) of each tile
+#pragma unroll
+        for (int r = 0; r < MT; r++) {
+            const int jp = j0 + wave * MT + r;
+            if (jp >= d.nj) continue;
+#pragma unroll
+            for (int t = 0; t < NT; t++) {
+                const int co = 16 * t + m;
+                const float bb = bias ? bias[co] : 0.f;
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    const int ip = i0 + 4 * g + q;
+                    if (ip >= d.ni) continue;
+                    float val = acc[r][t][q] + bb;
+                    if (RELU) val = val > 0.f ? val : 0.f;
+                    const long long o = (((long long)k * d.nj + jp) * d.ni + ip) * cout + co;
+                    if (OUT_F32) reinterpret_cast<float *>(Yv)[o] = val;
+                    else reinterpret_cast<bf16 *>(Yv)[o] = f2bf(val);
+                }
+            }
+        }
+        if (next) {
+            __syncthreads();  // every wave is done with plane k - 1
+            put(s0);          // plane k + 2 takes its slot
+            __syncthreads();
+        }
+    }
+}
+
 // B-fragment packing (see k_conv3t) of the FP32 master weights W[27][cin][cout]:
 // forward (K channels = cin, outputs = cout) or, for the data gradient,
 // transposed with the taps mirrored (K channels = cout, outputs = cin,
 // W'[tap][co][ci] = W[26 - tap][ci][co]).
-__global__ void k_pack(int cin, int cout, int transpose, const float *__restrict__ W, s16x8 *__restrict__ Wf)
+// cc = 0 packs in conv_cc(K channels) chunks; cc = K channels packs one chunk
+// (the order of k_conv3m).
+__global__ void k_pack(int cin, int cout, int transpose, int cc_all, const float *__restrict__ W, s16x8 *__restrict__ Wf)
 {
     const int kc = transpose ? cout : cin, oc = transpose ? cin : cout;
-    const int CC = conv_cc(kc), KS = conv_ks(CC), NT = oc / 16, nch = kc / CC;
+    const int CC = cc_all ? kc : conv_cc(kc), KS = conv_ks(CC), NT = oc / 16, nch = kc / CC;
     const long long total = (long long)nch * KS * NT * 64;
     long long q = blockIdx.x * (long long)blockDim.x + threadIdx.x;
     if (q >= total) return;
@@ -253,40 +414,49 @@ __global__ void k_pack(int cin, int cout, int transpose, const float *__restrict
     Wf[q] = v;
 }
 
-static long long packed_count(int kc, int oc)
+static long long packed_count(int kc, int oc, bool cc_all = false)
 {
-    const int CC = conv_cc(kc);
+    const int CC = cc_all ? kc : conv_cc(kc);
     return (long long)(kc / CC) * conv_ks(CC) * (oc / 16) * 64;
 }
 
 // ---- elementwise layers ------------------------------------------------------
-// input channel 0 = x * scale (double -> bf16), channels 1..15 = 0
+// input channel 0 = x * scale (double -> bf16), channels 1..15 = 0; one
+// thread per point, two 16-byte stores
 __global__ void k_in(Dims d, Grid g, const double *__restrict__ x, const double *__restrict__ scale, bf16 *__restrict__ X)
 {
-    long long p = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-    if (p >= d.np()) return;
-    const int i = p % d.ni, j = (p / d.ni) % d.nj, k = p / ((long long)d.ni * d.nj);
+    const unsigned p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= (unsigned)d.np()) return;
+    const unsigned i = p % d.ni, r = p / d.ni, j = r % d.nj, k = r / d.nj;
     const double v = x[((long long)(k + 1) * g.my + j + 1) * g.mx + i + 1] * scale[0];
-    bf16 *o = X + p * 16;
-    o[0] = f2bf((float)v);
-    for (int c = 1; c < 16; c++) o[c] = 0;
+    s16x8 lo = {0, 0, 0, 0, 0, 0, 0, 0}, hi = lo;
+    lo[0] = (short)f2bf((float)v);
+    s16x8 *o = reinterpret_cast<s16x8 *>(X + (long long)p * 16);
+    o[0] = lo;
+    o[1] = hi;
 }
 
-// 2x2x2 average pooling, channels-last
+// 2x2x2 average pooling, channels-last: one thread per (coarse point, eight
+// channels), 16-byte loads of the eight children, summed in child order
 __global__ void k_pool(Dims dc, int C, const bf16 *__restrict__ Xf, bf16 *__restrict__ Xc)
 {
-    long long q = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-    if (q >= dc.np() * C) return;
-    const int c = q % C;
-    const long long P = q / C;
-    const int I = P % dc.ni, J = (P / dc.ni) % dc.nj, K = P / ((long long)dc.ni * dc.nj);
-    const int nif = 2 * dc.ni, njf = 2 * dc.nj;
-    float s = 0.f;
+    const unsigned U = C / 8;
+    const unsigned q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= (unsigned)dc.np() * U) return;
+    const unsigned u = q % U, P = q / U;
+    const unsigned I = P % dc.ni, R = P / dc.ni, J = R % dc.nj, K = R / dc.nj;
+    const unsigned nif = 2 * dc.ni, njf = 2 * dc.nj;
+    float s[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     for (int e = 0; e < 8; e++) {
         const long long pf = ((long long)(2 * K + (e >> 2)) * njf + 2 * J + ((e >> 1) & 1)) * nif + 2 * I + (e & 1);
-        s += bf2f(Xf[pf * C + c]);
+        const s16x8 v = *reinterpret_cast<const s16x8 *>(Xf + pf * C + 8 * u);
+#pragma unroll
+        for (int c = 0; c < 8; c++) s[c] += bf2f((bf16)v[c]);
     }
-    Xc[q] = f2bf(s * 0.125f);
+    s16x8 o;
+#pragma unroll
+    for (int c = 0; c < 8; c++) o[c] = (short)f2bf(s[c] * 0.125f);
+    *reinterpret_cast<s16x8 *>(Xc + (long long)q * 8) = o;
 }
 
 // gf[fine] += gc[parent] / 8
@@ -301,22 +471,24 @@ __global__ void k_pool_back(Dims df, int C, const float *__restrict__ gc, float 
     gf[q] += gc[P * C + c] * 0.125f;
 }
 
-// concat buffer of the decoder: [upsampled coarse (Cu) | skip (Cs)]
+// concat buffer of the decoder: [upsampled coarse (Cu) | skip (Cs)]; one
+// thread per (point, eight channels), 16-byte copies
 __global__ void k_up_cat(Dims df, int Cu, int Cs, const bf16 *__restrict__ Yc, const bf16 *__restrict__ S,
                          bf16 *__restrict__ Z)
 {
-    const int C = Cu + Cs;
-    long long q = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-    if (q >= df.np() * C) return;
-    const int c = q % C;
-    const long long p = q / C;
-    const int i = p % df.ni, j = (p / df.ni) % df.nj, k = p / ((long long)df.ni * df.nj);
-    if (c < Cu) {
+    const unsigned Uu = Cu / 8, U = (Cu + Cs) / 8;
+    const unsigned q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= (unsigned)df.np() * U) return;
+    const unsigned u = q % U, p = q / U;
+    s16x8 v;
+    if (u < Uu) {
+        const unsigned i = p % df.ni, r = p / df.ni, j = r % df.nj, k = r / df.nj;
         const long long P = ((long long)(k / 2) * (df.nj / 2) + j / 2) * (df.ni / 2) + i / 2;
-        Z[q] = Yc[P * Cu + c];
+        v = *reinterpret_cast<const s16x8 *>(Yc + P * Cu + 8 * u);
     } else {
-        Z[q] = S[p * Cs + c - Cu];
+        v = *reinterpret_cast<const s16x8 *>(S + (long long)p * Cs + 8 * (u - Uu));
     }
+    *reinterpret_cast<s16x8 *>(Z + (long long)q * 8) = v;
 }
 
 // gradient of the concat: coarse part summed over the 8 children, skip part
@@ -379,13 +551,18 @@ __global__ void k_bias_grad(long long np, int C, const bf16 *__restrict__ gm, fl
 __global__ void k_out(Dims d, Grid g, int C, const bf16 *__restrict__ Y, const float *__restrict__ w, const float *__restrict__ b,
                       float *__restrict__ out, const double *__restrict__ oscale, double *__restrict__ yd)
 {
-    long long p = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-    if (p >= d.np()) return;
+    const unsigned p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= (unsigned)d.np()) return;
     float s = b[0];
-    for (int c = 0; c < C; c++) s += bf2f(Y[p * C + c]) * w[c];
+    const s16x8 *y = reinterpret_cast<const s16x8 *>(Y + (long long)p * C);
+    for (int u = 0; u < C / 8; u++) {
+        const s16x8 v = y[u];
+#pragma unroll
+        for (int c = 0; c < 8; c++) s += bf2f((bf16)v[c]) * w[8 * u + c];
+    }
     out[p] = s;
     if (yd) {
-        const int i = p % d.ni, j = (p / d.ni) % d.nj, k = p / ((long long)d.ni * d.nj);
+        const unsigned i = p % d.ni, r = p / d.ni, j = r % d.nj, k = r / d.nj;
         yd[((long long)(k + 1) * g.my + j + 1) * g.mx + i + 1] = (double)s * oscale[0];
     }
 }
@@ -487,6 +664,7 @@ struct Conv {
     float *gw = nullptr, *gb = nullptr, *mw = nullptr, *vw = nullptr, *mb = nullptr, *vb = nullptr;
     bf16x8 *wf = nullptr, *wt = nullptr;  // packed forward / transposed B fragments
     long long nwf = 0, nwt = 0;           // their counts
+    bool march = false;                   // forward by k_conv3m (wf packed as one chunk)
 };
 
 }  // namespace vfs
@@ -545,10 +723,56 @@ static int conv_launch_cc(hipStream_t st, const Dims &d, int kc, const bf16 *X, 
                              : conv_launch<NT, 16, RELU, OUT_F32>(st, d, kc, X, Wf, bias, Y);
 }
 
+// k_conv3m launch: runs of kr planes, enough blocks to fill the GPU
+template <int NT, int KC, bool UPCAT = false>
+static int conv_m_launch(hipStream_t st, const Dims &d, const bf16 *X, const bf16x8 *Wf, const float *bias, bf16 *Y,
+                         const bf16 *Xc = nullptr, int cu = 0)
+{
+    constexpr int MT = conv_mt(NT), TJ = 4 * MT, LDS = conv_m_lds(NT, KC);
+    static_assert(LDS <= 160 * 1024, "halo ring and weights exceed the LDS");
+    auto kern = k_conv3m<NT, MT, KC, true, false, UPCAT>;
+    static bool attr = false;
+    if (!attr) {
+        VFS_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(kern), hipFuncAttributeMaxDynamicSharedMemorySize, LDS));
+        attr = true;
+    }
+    const long long nbp = (long long)((d.ni + 15) / 16) * ((d.nj + TJ - 1) / TJ);
+    long long kr = d.nk * nbp / 1024;
+    kr = kr < 2 ? 2 : kr > 16 ? 16 : kr;
+    if (kr > d.nk) kr = d.nk;
+    const unsigned nb = (unsigned)(nbp * ((d.nk + kr - 1) / kr));
+    hipLaunchKernelGGL(kern, dim3(nb), dim3(CV_TPB), LDS, st, d, (int)kr, X, Xc, cu, reinterpret_cast<const s16x8 *>(Wf),
+                       bias, Y);
+    return VFS_OK;
+}
+
+// layers k_conv3m takes: (output tiles, input channels) with an instance
+static bool conv_m_ok(int cin, int cout)
+{
+    const int nt = cout / 16;
+    return (nt == 1 && (cin == 16 || cin == 32 || cin == 48)) || (nt == 2 && (cin == 16 || cin == 32)) ||
+           (nt == 4 && cin == 32);
+}
+
 // forward convolution with ReLU (BF16 out)
 static int conv_fwd(hipStream_t st, const Dims &d, const Conv &c, const bf16 *X, bf16 *Y)
 {
     int rc;
+    if (c.march) {
+        const int key = (c.cout / 16) * 1000 + c.cin;
+        switch (key) {
+        case 1016: rc = conv_m_launch<1, 16>(st, d, X, c.wf, c.b, Y); break;
+        case 1032: rc = conv_m_launch<1, 32>(st, d, X, c.wf, c.b, Y); break;
+        case 1048: rc = conv_m_launch<1, 48>(st, d, X, c.wf, c.b, Y); break;
+        case 2016: rc = conv_m_launch<2, 16>(st, d, X, c.wf, c.b, Y); break;
+        case 2032: rc = conv_m_launch<2, 32>(st, d, X, c.wf, c.b, Y); break;
+        case 4032: rc = conv_m_launch<4, 32>(st, d, X, c.wf, c.b, Y); break;
+        default: return set_error(VFS_EINVAL, "unet: no k-marching instance for this layer");
+        }
+        if (rc) return rc;
+        VFS_LAUNCH_CHECK("unet conv (k-marching)");
+        return VFS_OK;
+    }
     switch (c.cout / 16) {
     case 1: rc = conv_launch_cc<1, true, false>(st, d, c.cin, X, c.wf, c.b, Y); break;
     case 2: rc = conv_launch_cc<2, true, false>(st, d, c.cin, X, c.wf, c.b, Y); break;
@@ -595,9 +819,9 @@ static int conv_bwd_weight(hipStream_t st, const Dims &d, const Conv &c, const b
 
 static int pack(hipStream_t st, Conv &c)
 {
-    hipLaunchKernelGGL(k_pack, dim3(blocks_for(c.nwf, 256)), dim3(256), 0, st, c.cin, c.cout, 0, c.w,
+    hipLaunchKernelGGL(k_pack, dim3(blocks_for(c.nwf, 256)), dim3(256), 0, st, c.cin, c.cout, 0, c.march ? 1 : 0, c.w,
                        reinterpret_cast<s16x8 *>(c.wf));
-    hipLaunchKernelGGL(k_pack, dim3(blocks_for(c.nwt, 256)), dim3(256), 0, st, c.cin, c.cout, 1, c.w,
+    hipLaunchKernelGGL(k_pack, dim3(blocks_for(c.nwt, 256)), dim3(256), 0, st, c.cin, c.cout, 1, 0, c.w,
                        reinterpret_cast<s16x8 *>(c.wt));
     VFS_LAUNCH_CHECK("unet pack");
     return VFS_OK;
@@ -614,7 +838,9 @@ static int conv_alloc(vfs_unet *u, Conv &c, int cin, int cout, int level, int fa
     for (float **f : fs) rc = rc ? rc : ualloc(u, (void **)f, c.nw * 4);
     float **bs[] = {&c.b, &c.gb, &c.mb, &c.vb};
     for (float **f : bs) rc = rc ? rc : ualloc(u, (void **)f, cout * 4);
-    c.nwf = packed_count(cin, cout);
+    const char *pe = std::getenv("VFS_UNET_KMARCH");  // 0: every layer on k_conv3t
+    c.march = !(pe && pe[0] == '0') && conv_m_ok(cin, cout);
+    c.nwf = packed_count(cin, cout, c.march);
     c.nwt = packed_count(cout, cin);
     rc = rc ? rc : ualloc(u, (void **)&c.wf, c.nwf * sizeof(bf16x8));
     rc = rc ? rc : ualloc(u, (void **)&c.wt, c.nwt * sizeof(bf16x8));
@@ -628,7 +854,15 @@ static int conv_alloc(vfs_unet *u, Conv &c, int cin, int cout, int level, int fa
     return VFS_OK;
 }
 
-static int forward(vfs_unet *u, const double *b_dev, double *y_dev)
+// the decoder layers whose concatenation k_conv3m reads in place
+static bool upcat_fused(const Conv &c, int cu)
+{
+    return c.march && c.cout == 16 && c.cin == 48 && cu == 32;
+}
+
+// keep_cat: the concatenated decoder inputs are written (the weight
+// gradients read them); inference reads them in place where it can
+static int forward(vfs_unet *u, const double *b_dev, double *y_dev, bool keep_cat)
 {
     vfs_ctx *c = u->ctx;
     hipStream_t st = c->stream;
@@ -643,14 +877,19 @@ static int forward(vfs_unet *u, const double *b_dev, double *y_dev)
     int rc = conv_fwd(st, d0, u->enc[0], u->x0, u->e[0]);
     for (int l = 1; l < L && !rc; l++) {
         const int C = u->enc[l - 1].cout;
-        hipLaunchKernelGGL(k_pool, dim3(blocks_for(u->dims[l].np() * C, 256)), dim3(256), 0, st, u->dims[l], C, u->e[l - 1],
+        hipLaunchKernelGGL(k_pool, dim3(blocks_for(u->dims[l].np() * C / 8, 256)), dim3(256), 0, st, u->dims[l], C, u->e[l - 1],
                            u->pool[l]);
         rc = conv_fwd(st, u->dims[l], u->enc[l], u->pool[l], u->e[l]);
     }
     for (int l = L - 2; l >= 0 && !rc; l--) {
         const bf16 *coarse = l == L - 2 ? u->e[L - 1] : u->d[l + 1];
         const int Cu = l == L - 2 ? u->enc[L - 1].cout : u->dec[l + 1].cout, Cs = u->enc[l].cout;
-        hipLaunchKernelGGL(k_up_cat, dim3(blocks_for(u->dims[l].np() * (Cu + Cs), 256)), dim3(256), 0, st, u->dims[l], Cu, Cs,
+        if (!keep_cat && upcat_fused(u->dec[l], Cu)) {
+            rc = conv_m_launch<1, 48, true>(st, u->dims[l], u->e[l], u->dec[l].wf, u->dec[l].b, u->d[l], coarse, Cu);
+            if (!rc) VFS_LAUNCH_CHECK("unet conv (k-marching, concatenation in place)");
+            continue;
+        }
+        hipLaunchKernelGGL(k_up_cat, dim3(blocks_for(u->dims[l].np() * (Cu + Cs) / 8, 256)), dim3(256), 0, st, u->dims[l], Cu, Cs,
                            coarse, u->e[l], u->cat[l]);
         rc = conv_fwd(st, u->dims[l], u->dec[l], u->cat[l], u->d[l]);
     }
@@ -756,6 +995,9 @@ int vfs_unet_create(vfs_ctx *c, int base, int levels, unsigned long long seed, v
     const int ni = c->g.mx - 2, nj = c->g.my - 2, nk = c->g.mz - 2, f = 1 << (levels - 1);
     if (ni % f || nj % f || nk % f)
         return set_error(VFS_EINVAL, "vfs_unet_create: interior cell counts must be divisible by 2^(levels-1)");
+    // the elementwise layers index (point, eight channels) in 32-bit arithmetic
+    if ((long long)ni * nj * nk * (2 * base / 8) >= (1LL << 32))
+        return set_error(VFS_EINVAL, "vfs_unet_create: grid too large for the network's 32-bit channel indexing");
     vfs_unet *u = new vfs_unet();
     u->ctx = c;
     u->levels = levels;
@@ -839,13 +1081,13 @@ int vfs_unet_forward(vfs_unet *u, const double *b_dev, double *y_dev)
 {
     if (!u || !b_dev || !y_dev) return set_error(VFS_EINVAL, "vfs_unet_forward: null argument");
     VFS_HIP(hipMemsetAsync(y_dev, 0, (size_t)u->ctx->g.n * sizeof(double), u->ctx->stream));
-    return forward(u, b_dev, y_dev);
+    return forward(u, b_dev, y_dev, false);
 }
 
 int vfs_unet_gradients(vfs_unet *u, const double *b_dev, const double *phi_dev, double *loss)
 {
     if (!u || !b_dev || !phi_dev) return set_error(VFS_EINVAL, "vfs_unet_gradients: null argument");
-    int rc = forward(u, b_dev, nullptr);
+    int rc = forward(u, b_dev, nullptr, true);
     if (!rc) rc = loss_of(u, phi_dev, loss);
     if (!rc) rc = backward(u);
     return rc;

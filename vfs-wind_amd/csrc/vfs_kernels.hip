@@ -435,11 +435,13 @@ __global__ void k_c2c_bc(Grid g, int axis, const double *__restrict__ L2, const 
     if (touched) { ucat[3 * q] = u[0]; ucat[3 * q + 1] = u[1]; ucat[3 * q + 2] = u[2]; }
 }
 
-// interior cells next to two walls (rhs.c:664-669)
+// cells next to two walls (rhs.c:674-680): every k but 0 and mz-2, the k+
+// ghost plane included (the reference's loop spans the whole DA)
 __global__ void k_c2c_corner(Grid g, double *__restrict__ ucat)
 {
     int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k < 1 || k >= g.mz - 2) return;
+    if (k < 1 || k >= g.mz || k == g.mz - 2) return;
+    if (g.my - 2 == 1 || g.mx - 1 == 1) return;
     const int *bc = g.bc;
     if (bc[0] <= 1 && bc[2] <= 1) { long long q = gidx(g, k, 1, 1); ucat[3 * q] = ucat[3 * q + 1] = ucat[3 * q + 2] = 0; }
     if (bc[1] <= 1 && bc[2] <= 1) { long long q = gidx(g, k, 1, g.mx - 2); ucat[3 * q] = ucat[3 * q + 1] = ucat[3 * q + 2] = 0; }
@@ -467,7 +469,7 @@ static bool bc_needs_snapshot(const Grid &g)
 
 static bool corner_rule_active(const Grid &g)
 {
-    return !g.pi && !g.pj && ((g.bc[0] <= 1 && g.bc[2] <= 1) || (g.bc[1] <= 1 && g.bc[2] <= 1));
+    return (g.bc[0] <= 1 && g.bc[2] <= 1) || (g.bc[1] <= 1 && g.bc[2] <= 1);
 }
 
 int c2c(vfs_ctx *c)
