@@ -71,9 +71,9 @@ ALGO = {
 TRAFFIC_SOURCE = ("not measured by this run: rocprofv3 --pmc FETCH_SIZE (x2.0) + WRITE_SIZE of the same kernel on "
                   "this workload in a separate run, profiles/hbm_traffic_channel_256x192x256_pmc.md")
 MEASURED_TRAFFIC = {
-    "formfunction2": 5.307e9,
+    "formfunction2": 5.011e9,
     "face_fluxes": 3.119e9,
-    "cell_rhs": 2.132e9,
+    "cell_rhs": 1.838e9,
     "line_rhs_fp32": 0.382e9,
     "line_rhs_compact_fp32": 0.208e9,
     "poisson_apply_fp64": 2.501e9,

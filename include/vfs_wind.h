@@ -325,10 +325,11 @@ long long vfs_unet_param_count(vfs_unet *net);
 int vfs_unet_get_params(vfs_unet *net, int which, float *host);
 int vfs_unet_set_params(vfs_unet *net, const float *host);
 /* vfs_poisson_solve with the network: x0 = guess + alpha * net(b - A guess)
- * (alpha by least squares on the residual), the first FGMRES direction
- * preconditioned by the network, the rest by the V-cycle; when after three
- * iterations the residual has not dropped below half of its starting value
- * the solve restarts from the guess with the plain V-cycle (stats->fallback = 1). */
+ * (alpha by least squares on the residual), then FGMRES with the V-cycle
+ * (environment VFS_LEARNED_DIRECTION=1: the first direction preconditioned by
+ * the network); when after three iterations the residual has not dropped
+ * below half of its starting value the solve restarts from the guess with the
+ * plain V-cycle (stats->fallback = 1). */
 int vfs_poisson_solve_learned(vfs_ctx *ctx, vfs_unet *net, const vfs_solve_opts *opts, vfs_solve_stats *stats);
 
 /* ---- inflow planes (-inlet 100, -pseudo 1) ----------------------------- */

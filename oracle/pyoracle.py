@@ -41,7 +41,10 @@ class OrcMetrics(C.Structure):
 class OrcTurbine(C.Structure):
     _fields_ = [("nx", C.c_double), ("ny", C.c_double), ("nz", C.c_double), ("xc", C.c_double), ("yc", C.c_double),
                 ("zc", C.c_double), ("angvel", C.c_double), ("pitch", C.c_double), ("r_nacelle", C.c_double),
-                ("refangle", C.c_double)]
+                ("refangle", C.c_double), ("shen", C.c_int), ("prandtl", C.c_int), ("shen1", C.c_int),
+                ("r_rotor", C.c_double), ("tsr", C.c_double), ("num_blade", C.c_int), ("a_shen", C.c_double),
+                ("b_shen", C.c_double), ("c_shen", C.c_double), ("correction_shen1", C.c_double),
+                ("correction3d_cl", C.c_int), ("c0_cl", C.c_double), ("smoothforce", C.c_int)]
 
 
 class OrcFoil(C.Structure):
@@ -121,6 +124,7 @@ def load(path: str | os.PathLike | None = None) -> C.CDLL:
         "orc_alm_flagr": ([C.POINTER(OrcTurbine), C.c_int, C.c_int, D, D, D, D, D, C.POINTER(OrcFoil), C.c_int,
                            D, D, D, D], None),
         "orc_alm_feul": ([P, M, C.c_int, D, C.POINTER(C.c_int), D, D, C.c_int, C.c_double, D], None),
+        "orc_alm_loads": ([C.POINTER(OrcTurbine), C.c_int, D, D, D, D, D], None),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)
@@ -293,9 +297,23 @@ def _iptr(a: np.ndarray):
     return a.ctypes.data_as(C.POINTER(C.c_int))
 
 
-def make_turbine(nx, ny, nz, xc, yc, zc, angvel, pitch=0.0, r_nacelle=0.0, refangle=20.0) -> OrcTurbine:
+def make_turbine(nx, ny, nz, xc, yc, zc, angvel, pitch=0.0, r_nacelle=0.0, refangle=20.0, shen=0, prandtl=0, shen1=0,
+                 r_rotor=0.0, tsr=0.0, num_blade=3, a_shen=0.125, b_shen=21.0, c_shen=0.1, correction_shen1=1.0,
+                 correction3d_cl=0, c0_cl=1.0, smoothforce=0) -> OrcTurbine:
+    """Turbine parameters; the correction options default to main.c:161, 224-238."""
     rr = float(np.sqrt(nx * nx + ny * ny + nz * nz))
-    return OrcTurbine(nx / rr, ny / rr, nz / rr, xc, yc, zc, angvel, pitch, r_nacelle, refangle)
+    return OrcTurbine(nx / rr, ny / rr, nz / rr, xc, yc, zc, angvel, pitch, r_nacelle, refangle, shen, prandtl, shen1,
+                      r_rotor, tsr, num_blade, a_shen, b_shen, c_shen, correction_shen1, correction3d_cl, c0_cl,
+                      smoothforce)
+
+
+def alm_loads(t: OrcTurbine, cent, flagr, ulagr, dA):
+    """Calc_forces_ACL sums: (A_Sum, F_Sum, P_Sum, U_mean, Fx, Fy, Fz, M_Sum)."""
+    lib = load()
+    cent, flagr, ulagr, dA = (np.ascontiguousarray(a, dtype=np.float64) for a in (cent, flagr, ulagr, dA))
+    out = np.zeros(8)
+    lib.orc_alm_loads(C.byref(t), int(dA.size), _ptr(cent), _ptr(flagr), _ptr(ulagr), _ptr(dA), _ptr(out))
+    return out
 
 
 class Foils:

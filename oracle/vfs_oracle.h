@@ -189,6 +189,16 @@ typedef struct {
     double pitch;            /* pitch[0] (degrees)                                */
     double r_nacelle;        /* r_nacelle / reflength_wt                          */
     double refangle;         /* refangle_AL (degrees), rotor_model 6               */
+    /* Calc_F_lagr_ACL's optional corrections (rotor_model.c:2488-2584; main.c:224-238) */
+    int shen, prandtl, shen1;        /* -Shen_AL, -Prandtl_AL, -Shen1_AL tip-loss models */
+    double r_rotor;                  /* fsi.r_rotor / reflength_wt                        */
+    double tsr;                      /* ibm.Tipspeedratio                                  */
+    int num_blade;                   /* -num_blade                                         */
+    double a_shen, b_shen, c_shen;   /* -a_shen -b_shen -c_shen                             */
+    double correction_shen1;         /* -correction_ALShen1                                 */
+    int correction3d_cl;             /* -correction3D_CL                                    */
+    double c0_cl;                    /* -c0_CL                                              */
+    int smoothforce;                 /* -smoothforce_AL                                     */
 } orc_turbine;
 
 typedef struct {
@@ -212,6 +222,10 @@ void orc_alm_flagr(const orc_turbine *t, int nb, int nvb, const double *cent, co
                    double *cd, double *flagr);
 void orc_alm_feul(const orc_params *p, const orc_metrics *m, int n, const double *cent, const int *box,
                   const double *flagr, const double *dA, int df, double halfwidth, double *feul);
+/* Calc_forces_ACL (rotor_model.c:4148-4180): out = {A_Sum, F_Sum, P_Sum, U_Sum / A_Sum, Fx_Sum, Fy_Sum,
+ * Fz_Sum, M_Sum}; the reference's Torque_aero is -M_Sum and Force_axis -F_Sum. */
+void orc_alm_loads(const orc_turbine *t, int n, const double *cent, const double *flagr, const double *ulagr,
+                   const double *dA, double out[8]);
 
 #ifdef __cplusplus
 }

@@ -346,9 +346,63 @@ void orc_alm_flagr(const orc_turbine *t, int nb, int nvb, const double *cent, co
             const double factor = chord[l];
             for (int c = 0; c < 3; c++) flagr[3 * l + c] = -0.5 * uref * uref * fabs(cd[l]) * nrel[c] * factor;
             for (int c = 0; c < 3; c++) flagr[3 * l + c] += -0.5 * uref * uref * fabs(cl[l]) * nL[c] * factor;
-            /* the tip-loss factor F1 = 1.0 multiplies in place (rotor_model.c:2545-2547) */
-            for (int c = 0; c < 3; c++) flagr[3 * l + c] *= 1.0;
+            /* tip-loss factor F1, rotor_model.c:2488-2544 (the later model wins; its pi is 3.1415926) */
+            double F1 = 1.0;
+            if (t->shen) {
+                const double R = t->r_rotor, pi1 = 3.1415926;
+                const double phi1 = (aoa[l] + twist[l] + t->pitch) * pi1 / 180.0;
+                const double g1 = exp(-t->a_shen * (t->num_blade * fabs(t->tsr) - t->b_shen)) + t->c_shen;
+                F1 = 2.0 * acos(exp(-g1 * t->num_blade * (fmax(R - r, 0.0)) / 2.0 / r / fabs(sin(phi1)))) / pi1;
+            }
+            if (t->prandtl) {
+                const double R = t->r_rotor, pi1 = 3.1415926;
+                const double g1 = sqrt(1 + pow(t->tsr, 2));
+                F1 = 2.0 * acos(exp(-g1 * t->num_blade * (fmax(R - r, 0.0)) / 2.0 / R)) / pi1;
+            }
+            if (t->shen1) {
+                const double R = t->r_rotor, pi1 = 3.1415926;
+                const double phi1 = (aoa[l] + twist[l] + t->pitch) * pi1 / 180.0;
+                const double Lambda = fabs(t->tsr);
+                double g1;
+                if (Lambda < 7.0636) g1 = -1.7636 * Lambda + 11.6112;
+                else if (Lambda > 7.0636 && Lambda < 9.8146) g1 = -0.5805 * Lambda + 3.2542;
+                else g1 = -0.5077 * Lambda + 2.5397;
+                g1 *= t->correction_shen1;
+                g1 = exp(g1);
+                F1 = 2.0 * acos(exp(-g1 * t->num_blade * (fmax(R - r, 0.0)) / 2.0 / r / fabs(sin(phi1)))) / pi1;
+            }
+            for (int c = 0; c < 3; c++) flagr[3 * l + c] *= F1;
+            /* 3D lift correction inboard of 0.75 R along the rotor axis, rotor_model.c:2548-2558 */
+            if (t->correction3d_cl && r < 0.75 * t->r_rotor) {
+                const double inflow = twist[l] * pi / 180 + aoa[l] * pi / 180;
+                const double Fc = -0.5 * t->c0_cl * pow(chord[l] / r, 1) * pow(cos(inflow), 2) * uref * uref * factor;
+                flagr[3 * l] += Fc * t->nx;
+                flagr[3 * l + 1] += Fc * t->ny;
+                flagr[3 * l + 2] += Fc * t->nz;
+            }
         }
+    /* force smoothing along each blade, rotor_model.c:2562-2598: smoothforce + 1
+     * passes (do-while), each element j in [4, ne - 4) of a blade takes
+     * sum_{i = j-4}^{j+3} F_i dfunc_s4h(i - j) of the previous pass */
+    if (t->smoothforce) {
+        const int n = nb * ne;
+        double *tmp = (double *)malloc(3 * (size_t)n * sizeof(double));
+        int pass = 0;
+        do {
+            pass++;
+            memcpy(tmp, flagr, 3 * (size_t)n * sizeof(double));
+            for (int b = 0; b < nb; b++)
+                for (int j = b * ne + 4; j < (b + 1) * ne - 4; j++) {
+                    double s[3] = {0, 0, 0};
+                    for (int i = j - 4; i < j + 4; i++) {
+                        const double w = orc_dfunc_s4h((double)(i - j));
+                        for (int c = 0; c < 3; c++) s[c] += tmp[3 * i + c] * w;
+                    }
+                    for (int c = 0; c < 3; c++) flagr[3 * j + c] = s[c];
+                }
+        } while (pass <= t->smoothforce);
+        free(tmp);
+    }
     for (int l = 0; l < nb * ne; l++) {
         double r = sqrt(pow((cent[3 * l] - t->xc), 2) + pow((cent[3 * l + 1] - t->yc), 2) + pow((cent[3 * l + 2] - t->zc), 2));
         if (r < t->r_nacelle) flagr[3 * l] = flagr[3 * l + 1] = flagr[3 * l + 2] = 0.0;
@@ -434,3 +488,30 @@ void orc_alm_feul(const orc_params *p, const orc_metrics *m, int n, const double
             }
 }
 #undef CO
+
+/* Calc_forces_ACL, rotor_model.c:4148-4180: rotor sums over the elements */
+void orc_alm_loads(const orc_turbine *t, int n, const double *cent, const double *flagr, const double *ulagr,
+                   const double *dA, double out[8])
+{
+    double A_Sum = 0, F_Sum = 0, P_Sum = 0, U_Sum = 0, Fx = 0, Fy = 0, Fz = 0, M_Sum = 0;
+    const double nx = t->nx, ny = t->ny, nz = t->nz;
+    for (int l = 0; l < n; l++) {
+        const double *F = flagr + 3 * l, *U = ulagr + 3 * l;
+        A_Sum += dA[l];
+        const double F_axis = F[0] * nx + F[1] * ny + F[2] * nz;
+        const double U_axis = U[0] * nx + U[1] * ny + U[2] * nz;
+        F_Sum += F_axis * dA[l];
+        P_Sum += F_axis * dA[l] * U_axis;
+        U_Sum += U_axis * dA[l];
+        Fx += F[0] * dA[l];
+        Fy += F[1] * dA[l];
+        Fz += F[2] * dA[l];
+        const double rx = cent[3 * l] - t->xc, ry = cent[3 * l + 1] - t->yc, rz = cent[3 * l + 2] - t->zc;
+        const double M_x = ry * F[2] * dA[l] - rz * F[1] * dA[l];
+        const double M_y = rz * F[0] * dA[l] - rx * F[2] * dA[l];
+        const double M_z = rx * F[1] * dA[l] - ry * F[0] * dA[l];
+        M_Sum += M_x * nx + M_y * ny + M_z * nz;
+    }
+    out[0] = A_Sum; out[1] = F_Sum; out[2] = P_Sum; out[3] = U_Sum / A_Sum;
+    out[4] = Fx; out[5] = Fy; out[6] = Fz; out[7] = M_Sum;
+}

@@ -197,3 +197,153 @@ def test_clipper_deck_readers():
     # the disk spans radius R = 0.5 and lies 2R upstream of the hub
     assert np.allclose(x[:, 2], -1.0) and np.isclose(np.abs(x[:, :2]).max(), 0.5)
     assert dA.sum() == pytest.approx(math.pi * 0.25, rel=0.02)
+
+
+def _blade(n_el=24, r0=1.0, r1=10.0, omega=0.4, U=1.3, chord=0.8, **kw):
+    """One straight blade along +x rotating about +z in a uniform axial wind,
+    n_el elements between r0 and r1, a linear lift polar and constant drag."""
+    t = po.make_turbine(0, 0, 1, 0.0, 0.0, 0.0, angvel=omega, **kw)
+    xs = np.linspace(r0, r1, n_el + 1)
+    nodes = np.zeros((1, n_el + 1, 3))
+    nodes[0, :, 0] = xs
+    unode = np.zeros((n_el + 1, 3))
+    unode[:, 1] = omega * xs
+    cent, dA = po.alm_elements(nodes)
+    ang = np.arange(-90.0, 91.0, 1.0)
+    foil = {"r": np.array([0.0, 2 * r1]), "twist": np.zeros(2), "chord": np.full(2, chord), "ang_cl": ang,
+            "cl": 2 * np.pi * np.radians(ang), "ang_cd": np.array([-90.0, 90.0]), "cd": np.array([0.01, 0.01])}
+    foils = po.Foils([foil])
+    tw, ch = po.alm_twist_chord(t, cent, foils)
+    ulagr = np.tile([0.0, 0.0, U], (n_el, 1))
+    return t, cent, dA, ulagr, unode, tw, ch, foils
+
+
+def test_prandtl_tip_loss_closed_form():
+    """-Prandtl_AL, rotor_model.c:2502-2508: F1 = (2/pi1) acos(exp(-g1 B (R - r) / (2R))),
+    g1 = sqrt(1 + TSR^2), pi1 = 3.1415926.  Pinned where it is fixed without the
+    formula: F1 = 0 at and beyond the tip, F1 -> 1 far inboard, and at the radius
+    where the exponential is 1/2, F1 = 2 acos(1/2) / 3.1415926 = (2 pi / 3) / 3.1415926."""
+    R, tsr, B = 10.0, 6.0, 3
+    g1 = math.sqrt(1 + tsr * tsr)
+    r_half = R - 2 * R * math.log(2.0) / (g1 * B)
+    for r1, n_el in ((12.0, 24), (2 * r_half - 1.0, 1)):
+        args = _blade(n_el=n_el, r0=1.0, r1=r1)
+        t0, cent = args[0], args[1]
+        f0 = po.alm_flagr(t0, 1, n_el + 1, *args[1:2], args[3], args[4], args[5], args[6], args[7])[0]
+        tp = po.make_turbine(0, 0, 1, 0.0, 0.0, 0.0, angvel=0.4, prandtl=1, r_rotor=R, tsr=tsr, num_blade=B)
+        fp = po.alm_flagr(tp, 1, n_el + 1, *args[1:2], args[3], args[4], args[5], args[6], args[7])[0]
+        r = np.linalg.norm(cent, axis=1)
+        ratio = np.linalg.norm(fp, axis=1) / np.linalg.norm(f0, axis=1)
+        if n_el == 1:
+            assert r[0] == pytest.approx(r_half, rel=1e-12)
+            assert ratio[0] == pytest.approx((2 * math.pi / 3) / 3.1415926, rel=1e-12)
+        else:
+            assert np.all(ratio[r >= R] == 0.0)
+            # pi1 < pi: F1 can exceed 1 by pi / 3.1415926 - 1 = 1.7e-8 far inboard
+            assert np.all((ratio >= 0) & (ratio <= 1 + 2e-8))
+            assert np.all(np.diff(ratio) <= 1e-15)  # monotone towards the tip
+            assert ratio[0] > 0.9997  # exp(-8) from the tip's influence at r = 1.25
+
+
+def test_shen_tip_loss_reduces_to_its_exponential():
+    """-Shen_AL (rotor_model.c:2490-2501) and -Shen1_AL (2509-2530):
+    F1 = (2/pi1) acos(exp(-g1 B (R - r) / (2 r |sin phi|))), phi = alpha + twist
+    + pitch, g1 = exp(-a (B TSR - b)) + c for Shen and the piecewise fit for
+    Shen1.  With a = 0 the Shen g1 is 1 + c; both give F1 = 0 at the tip and
+    the same F1 when the Shen1 exponent is chosen to make the two g1 equal."""
+    R, tsr, B = 10.0, 7.5, 3
+    args = _blade(n_el=16, r0=2.0, r1=12.0)
+    t0 = args[0]
+    f0, aoa = po.alm_flagr(t0, 1, 17, args[1], args[3], args[4], args[5], args[6], args[7])[:2]
+    c = 0.3
+    ts = po.make_turbine(0, 0, 1, 0.0, 0.0, 0.0, angvel=0.4, shen=1, r_rotor=R, tsr=tsr, num_blade=B, a_shen=0.0, c_shen=c)
+    fs = po.alm_flagr(ts, 1, 17, args[1], args[3], args[4], args[5], args[6], args[7])[0]
+    # Shen1 at TSR 7.5 (middle branch): g1 = exp(k (-0.5805 TSR + 3.2542)); pick k so g1 = 1 + c
+    k = math.log(1 + c) / (-0.5805 * tsr + 3.2542)
+    t1 = po.make_turbine(0, 0, 1, 0.0, 0.0, 0.0, angvel=0.4, shen1=1, r_rotor=R, tsr=tsr, num_blade=B, correction_shen1=k)
+    f1 = po.alm_flagr(t1, 1, 17, args[1], args[3], args[4], args[5], args[6], args[7])[0]
+    r = np.linalg.norm(args[1], axis=1)
+    assert np.allclose(fs, f1, rtol=1e-12, atol=0)
+    rs = np.linalg.norm(fs, axis=1) / np.linalg.norm(f0, axis=1)
+    assert np.all(rs[r >= R] == 0.0) and np.all(rs[r < R] > 0) and np.all(rs <= 1 + 2e-8)
+    # a larger g1 (c_shen) steepens the exponential: less loss at every inboard element
+    ts2 = po.make_turbine(0, 0, 1, 0.0, 0.0, 0.0, angvel=0.4, pitch=0.0, shen=1, r_rotor=R, tsr=tsr, num_blade=B,
+                          a_shen=0.0, c_shen=2 * c)
+    fs2 = po.alm_flagr(ts2, 1, 17, args[1], args[3], args[4], args[5], args[6], args[7])[0]
+    inb = r < R
+    assert np.all(np.linalg.norm(fs2, axis=1)[inb] > np.linalg.norm(fs, axis=1)[inb])
+
+
+def test_correction_3d_lift():
+    """-correction3D_CL (rotor_model.c:2548-2558): inboard of 0.75 R every
+    element gains a force along the rotor axis, proportional to c0_CL (linear)
+    and to chord^2 / r at a fixed inflow; outboard elements are untouched."""
+    R = 10.0
+    args = _blade(n_el=20, r0=1.0, r1=11.0)
+    f0 = po.alm_flagr(args[0], 1, 21, args[1], args[3], args[4], args[5], args[6], args[7])[0]
+    out = []
+    for c0 in (1.0, 2.0):
+        t = po.make_turbine(0, 0, 1, 0.0, 0.0, 0.0, angvel=0.4, correction3d_cl=1, c0_cl=c0, r_rotor=R)
+        out.append(po.alm_flagr(t, 1, 21, args[1], args[3], args[4], args[5], args[6], args[7])[0] - f0)
+    r = np.linalg.norm(args[1], axis=1)
+    inb = r < 0.75 * R
+    assert np.all(out[0][~inb] == 0.0)
+    assert np.all(out[0][inb, :2] == 0.0) and np.all(out[0][inb, 2] < 0)
+    assert np.allclose(out[1], 2 * out[0], rtol=1e-14, atol=0)
+
+
+def test_force_smoothing_keeps_linear_profiles():
+    """-smoothforce_AL (rotor_model.c:2562-2598): each pass replaces the force
+    of the elements [4, n - 4) of a blade by sum_{i=-4..3} F_{j+i} dfunc_s4h(i).
+    dfunc_s4h has unit sum and zero first moment over the integers, so a force
+    linear along the blade is kept exactly, while the kink of a |x| profile is
+    smoothed; the four elements at each end are never touched."""
+    import ctypes
+    lib = po.load()
+    lib.orc_dfunc_s4h.restype = ctypes.c_double
+    lib.orc_dfunc_s4h.argtypes = [ctypes.c_double]
+    n_el = 30
+    args = _blade(n_el=n_el)
+    t0 = po.make_turbine(0, 0, 1, 0.0, 0.0, 0.0, angvel=0.4, smoothforce=2)
+    f_plain = po.alm_flagr(args[0], 1, n_el + 1, args[1], args[3], args[4], args[5], args[6], args[7])[0]
+    f_s = po.alm_flagr(t0, 1, n_el + 1, args[1], args[3], args[4], args[5], args[6], args[7])[0]
+    assert np.array_equal(f_s[:4], f_plain[:4]) and np.array_equal(f_s[-4:], f_plain[-4:])
+    # three passes (smoothforce + 1, a do-while) of the 8-point stencil on the element forces
+    wts = np.array([lib.orc_dfunc_s4h(float(i)) for i in range(-4, 4)])
+    assert wts.sum() == pytest.approx(1.0, abs=1e-15)
+    assert (np.arange(-4, 4) * wts).sum() == pytest.approx(0.0, abs=1e-15)
+    g = f_plain.copy()
+    for _ in range(3):
+        prev = g.copy()
+        for j in range(4, n_el - 4):
+            g[j] = sum(prev[j + i] * wts[i + 4] for i in range(-4, 4))
+    assert np.allclose(f_s, g, rtol=1e-13, atol=1e-15)
+    # a linear profile passes unchanged
+    lin = np.outer(np.arange(n_el, dtype=float), [1.0, -2.0, 0.5])
+    h = lin.copy()
+    for j in range(4, n_el - 4):
+        h[j] = sum(lin[j + i] * wts[i + 4] for i in range(-4, 4))
+    assert np.allclose(h, lin, rtol=0, atol=1e-12)
+
+
+def test_rotor_loads_of_a_known_force():
+    """Calc_forces_ACL (rotor_model.c:4148-4180) on forces set by hand: a
+    tangential force f e_theta on every element of a blade along +x gives the
+    axial moment sum r f dA (Torque_aero = -M_Sum), an axial force g e_z gives
+    F_Sum = g sum dA, and a uniform axial velocity U gives the mean U."""
+    n_el = 12
+    t = po.make_turbine(0, 0, 1, 0.0, 0.0, 0.0, angvel=0.4)
+    nodes = np.zeros((1, n_el + 1, 3))
+    nodes[0, :, 0] = np.linspace(1.0, 7.0, n_el + 1)
+    cent, dA = po.alm_elements(nodes)
+    f, g, U = 0.7, -1.3, 2.5
+    F = np.tile([0.0, f, g], (n_el, 1))
+    Ul = np.tile([0.1, -0.2, U], (n_el, 1))
+    A, Fs, Ps, Um, Fx, Fy, Fz, M = po.alm_loads(t, cent, F, Ul, dA)
+    r = cent[:, 0]
+    assert A == pytest.approx(dA.sum(), rel=1e-14)
+    assert M == pytest.approx(np.sum(r * f * dA), rel=1e-13)
+    assert Fs == pytest.approx(g * dA.sum(), rel=1e-14)
+    assert Ps == pytest.approx(g * U * dA.sum(), rel=1e-14)
+    assert Um == pytest.approx(U, rel=1e-14)
+    assert (Fx, Fy, Fz) == pytest.approx((0.0, f * dA.sum(), g * dA.sum()), rel=1e-14)

@@ -117,7 +117,11 @@ def test_unet_training_reduces_loss():
     net.close(); ctx.close()
 
 
-def test_learned_poisson_solve_reaches_tolerance():
+@pytest.mark.parametrize("direction", ["0", "1"], ids=["warm-start", "warm-start+direction"])
+def test_learned_poisson_solve_reaches_tolerance(direction, monkeypatch):
+    """The learned solve (warm start; optionally also the network as the
+    first FGMRES direction) reaches 1e-8 under the oracle's operator."""
+    monkeypatch.setenv("VFS_LEARNED_DIRECTION", direction)
     c, o, ctx, net, orc, b, phi = _setup()
     ctx.runge_kutta()
     ctx.poisson_rhs()
@@ -133,12 +137,18 @@ def test_learned_poisson_solve_reaches_tolerance():
     net.close(); ctx.close()
 
 
-@pytest.mark.parametrize("fill", [np.nan, 0.0], ids=["nan-weights", "zero-weights"])
-def test_learned_solve_falls_back(fill):
-    """A network that cannot help (non-finite or all-zero weights) triggers the
-    fallback: after three iterations the residual has not halved, the solve
-    restarts from the original guess with the plain V-cycle, reports
-    fallback = 1 and still reaches the tolerance under the oracle's operator."""
+@pytest.mark.parametrize("fill,direction,fallback", [(np.nan, "0", 1), (np.nan, "1", 1), (0.0, "1", 1), (0.0, "0", 0)],
+                         ids=["nan-weights", "nan-weights+direction", "zero-weights+direction", "zero-weights"])
+def test_learned_solve_falls_back(fill, direction, fallback, monkeypatch):
+    """A network that cannot help triggers the fallback: after three
+    iterations the residual has not halved, the solve restarts from the
+    original guess with the plain V-cycle, reports fallback = 1 and still
+    reaches the tolerance under the oracle's operator.  Non-finite weights
+    poison the warm start (fallback either way); all-zero weights give a zero
+    warm start (alpha = 0: the guess is kept and the V-cycle FGMRES converges
+    on its own, no fallback) and, as the first direction, a zero direction
+    (fallback)."""
+    monkeypatch.setenv("VFS_LEARNED_DIRECTION", direction)
     c, o, ctx, net, orc, b, phi = _setup()
     p = net.params(0)
     p[:] = fill
@@ -149,7 +159,7 @@ def test_learned_solve_falls_back(fill):
     for fp32 in (1, 0):
         ctx.upload("PHI", np.zeros(ctx.field_shape("PHI")))
         st = net.poisson_solve(vfs.default_solve_opts(tol=1e-8, mg_fp32=fp32, use_guess=0))
-        assert st.fallback == 1 and st.relres <= 1e-8
+        assert st.fallback == fallback and st.relres <= 1e-8
         x, r = ctx.download("PHI"), ctx.download("B")
         assert np.all(np.isfinite(x))
         o.set("b", r)

@@ -1792,6 +1792,11 @@ static int fgmres(vfs_ctx *c, const vfs_solve_opts *o, vfs_unet *net, vfs_solve_
     bool converged = false;
     const char *tr = std::getenv("VFS_SOLVER_TRACE");
     const bool trace = tr && tr[0] == '1';
+    // the network gives the warm start; with VFS_LEARNED_DIRECTION=1 also the
+    // first Krylov direction (on a held-out 256x192x256 channel that took one
+    // FGMRES iteration more than the V-cycle's direction: 7 against 6)
+    const char *ld = std::getenv("VFS_LEARNED_DIRECTION");
+    const bool net_dir = net && ld && ld[0] == '1';
     while (it < maxit) {
         // r = b - A x into V_0
         apply0(c, (const double *)x, b, V);
@@ -1811,7 +1816,7 @@ static int fgmres(vfs_ctx *c, const vfs_solve_opts *o, vfs_unet *net, vfs_solve_
         for (jj = 0; jj < m && it < maxit; jj++, it++) {
             double *vj = V + (size_t)jj * n, *zj = Z + (size_t)jj * n;
             float *zj32 = Z32 + (size_t)jj * n;
-            if (net && it == 0) {
+            if (net_dir && it == 0) {
                 rc = vfs_unet_forward(net, vj, z32 ? w : zj);
                 if (!rc && z32) hipLaunchKernelGGL((k_convert<double, float>), dim3(nb), dim3(PTPB), 0, st, n, (const double *)w, zj32);
             } else {

@@ -235,26 +235,33 @@ __global__ void k_sum_partials(const float *__restrict__ part, int P, long long 
 // plane is read from memory once per run instead of three times, and the
 // weights once per block.  Wf is packed as one chunk of KC channels
 // (k_pack with cc_all): Wf[(s * NT + t) * 64 + lane].
-__host__ __device__ constexpr int conv_m_ring(int mt, int kc) { return 3 * 18 * (4 * mt + 2) * (2 * kc + 16); }
+// 16-output layers run eight waves of two M tiles (the same 16 x 16 tile as
+// four waves of four: twice the waves per SIMD to hide the LDS latency of
+// their one-MFMA-per-A-fragment inner loop); wider layers four waves.
+__host__ __device__ constexpr int conv_m_waves(int nt) { return nt == 1 ? 8 : 4; }
+__host__ __device__ constexpr int conv_m_mt(int nt) { return nt == 1 ? 2 : conv_mt(nt); }
+__host__ __device__ constexpr int conv_m_tj(int nt) { return conv_m_waves(nt) * conv_m_mt(nt); }
+__host__ __device__ constexpr int conv_m_ring(int tj, int kc) { return 3 * 18 * (tj + 2) * (2 * kc + 16); }
 __host__ __device__ constexpr int conv_m_lds(int nt, int kc)
 {
-    return conv_m_ring(conv_mt(nt), kc) + conv_ks(kc) * nt * 64 * 16;
+    return conv_m_ring(conv_m_tj(nt), kc) + conv_ks(kc) * nt * 64 * 16;
 }
 
 // UPCAT: the decoder's concatenation is read in place: input channels
 // [0, cu) are the half-resolution tensor Xc upsampled (nearest), channels
 // [cu, KC) the skip tensor X (KC - cu channels), so the concatenated tensor
 // is never written.
-template <int NT, int MT, int KC, bool RELU, bool OUT_F32, bool UPCAT = false>
-__global__ void __launch_bounds__(CV_TPB) k_conv3m(Dims d, int kr, const bf16 *__restrict__ X, const bf16 *__restrict__ Xc,
-                                                   int cu, const s16x8 *__restrict__ Wf, const float *__restrict__ bias,
-                                                   void *__restrict__ Yv)
+template <int NT, int MT, int KC, bool RELU, bool OUT_F32, bool UPCAT, int WAVES>
+__global__ void __launch_bounds__(64 * WAVES) k_conv3m(Dims d, int kr, const bf16 *__restrict__ X, const bf16 *__restrict__ Xc,
+                                                       int cu, const s16x8 *__restrict__ Wf, const float *__restrict__ bias,
+                                                       void *__restrict__ Yv)
 {
-    constexpr int TJ = 4 * MT, HI = 18, HJ = TJ + 2, PP = HI * HJ;
+    constexpr int TPB = 64 * WAVES;
+    constexpr int TJ = WAVES * MT, HI = 18, HJ = TJ + 2, PP = HI * HJ;
     constexpr int PS = 2 * KC + 16;          // bytes per halo point (16 lanes, 16 points: distinct banks)
     constexpr int U = KC / 8;                // 16-byte units per halo point
     constexpr int KS = conv_ks(KC);
-    constexpr int NPF = (PP * U + CV_TPB - 1) / CV_TPB;  // halo units per thread and plane
+    constexpr int NPF = (PP * U + TPB - 1) / TPB;  // halo units per thread and plane
     constexpr int PLANE = PP * PS, RING = 3 * PLANE;
     extern __shared__ __align__(16) unsigned char lds[];
     const int cout = 16 * NT;
@@ -269,13 +276,13 @@ __global__ void __launch_bounds__(CV_TPB) k_conv3m(Dims d, int kr, const bf16 *_
     const long long cplane = UPCAT ? (long long)(d.ni / 2) * (d.nj / 2) * cu : 0;
     // the layer's weights, once per block
     s16x8 *Bs = reinterpret_cast<s16x8 *>(lds + RING);
-    for (int q = threadIdx.x; q < KS * NT * 64; q += CV_TPB) Bs[q] = Wf[q];
+    for (int q = threadIdx.x; q < KS * NT * 64; q += TPB) Bs[q] = Wf[q];
     // this thread's halo units: the same (i, j, channel) offsets on every plane
     int goff[NPF], loff[NPF];
     unsigned coarse = 0;  // bit n: unit n reads the half-resolution tensor
 #pragma unroll
     for (int n = 0; n < NPF; n++) {
-        const int u = threadIdx.x + n * CV_TPB;
+        const int u = threadIdx.x + n * TPB;
         goff[n] = -1;
         loff[n] = -1;
         if (u < PP * U) {
@@ -728,9 +735,9 @@ template <int NT, int KC, bool UPCAT = false>
 static int conv_m_launch(hipStream_t st, const Dims &d, const bf16 *X, const bf16x8 *Wf, const float *bias, bf16 *Y,
                          const bf16 *Xc = nullptr, int cu = 0)
 {
-    constexpr int MT = conv_mt(NT), TJ = 4 * MT, LDS = conv_m_lds(NT, KC);
+    constexpr int MT = conv_m_mt(NT), W = conv_m_waves(NT), TJ = conv_m_tj(NT), LDS = conv_m_lds(NT, KC);
     static_assert(LDS <= 160 * 1024, "halo ring and weights exceed the LDS");
-    auto kern = k_conv3m<NT, MT, KC, true, false, UPCAT>;
+    auto kern = k_conv3m<NT, MT, KC, true, false, UPCAT, W>;
     static bool attr = false;
     if (!attr) {
         VFS_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(kern), hipFuncAttributeMaxDynamicSharedMemorySize, LDS));
@@ -741,7 +748,7 @@ static int conv_m_launch(hipStream_t st, const Dims &d, const bf16 *X, const bf1
     kr = kr < 2 ? 2 : kr > 16 ? 16 : kr;
     if (kr > d.nk) kr = d.nk;
     const unsigned nb = (unsigned)(nbp * ((d.nk + kr - 1) / kr));
-    hipLaunchKernelGGL(kern, dim3(nb), dim3(CV_TPB), LDS, st, d, (int)kr, X, Xc, cu, reinterpret_cast<const s16x8 *>(Wf),
+    hipLaunchKernelGGL(kern, dim3(nb), dim3(64 * W), LDS, st, d, (int)kr, X, Xc, cu, reinterpret_cast<const s16x8 *>(Wf),
                        bias, Y);
     return VFS_OK;
 }
