@@ -255,7 +255,32 @@ typedef struct {
     int rotor_model;        /* 3 or 6                                           */
     int deltafunc;          /* -deltafunc: 0 dfunc_2h, 7 dfunc_exp, else s4h    */
     double halfwidth;       /* -halfwidth_dfunc (deltafunc 7)                   */
+    /* Calc_F_lagr_ACL's optional corrections (rotor_model.c:2488-2598), all off
+     * when zero; the reference's defaults (main.c:161, 224-238) are num_blade 3,
+     * a_shen 0.125, b_shen 21, c_shen 0.1, correction_shen1 1, c0_cl 1. */
+    int shen, prandtl, shen1;       /* -Shen_AL, -Prandtl_AL, -Shen1_AL tip-loss models */
+    double r_rotor;                 /* fsi.r_rotor / reflength_wt (R of the models)      */
+    double tsr;                     /* ibm.Tipspeedratio (vfs_alm_turbine_angvel sets it) */
+    int num_blade;                  /* -num_blade (B of the models)                      */
+    double a_shen, b_shen, c_shen;  /* -a_shen -b_shen -c_shen                            */
+    double correction_shen1;        /* -correction_ALShen1                                */
+    int correction3d_cl;            /* -correction3D_CL (inboard of 0.75 R)               */
+    double c0_cl;                   /* -c0_CL                                             */
+    int smoothforce;                /* -smoothforce_AL: smoothforce + 1 passes            */
 } vfs_turbine;
+
+/* Calc_turbineangvel's inputs (rotor_model.c:5519-5601; main.c globals and the
+ * Turbine.inp control columns). */
+typedef struct {
+    int mode;                       /* 0 -fixturbineangvel, 1 -FixTipSpeedRatio, 2 -turbinetorquecontrol */
+    double angvel_fixed;            /* mode 0: fsi.angvel_fixed                          */
+    double tsr;                     /* mode 1: ibm.Tipspeedratio                         */
+    double r_rotor;                 /* fsi.r_rotor (dimensional)                         */
+    double reflength_wt, refvel_wt, refvel_cfd, rho_air;
+    double J_rotation, CP_max, TSR_max, X_control;          /* mode 2                 */
+    int fixturbinegeneratortorque;  /* mode 2: keep torque_generator                    */
+    double torque_generator;        /* mode 2 with fixturbinegeneratortorque            */
+} vfs_turbine_control;
 
 typedef struct {
     int n_ac; const double *r_ac, *twist, *chord;   /* FOILnn                      */
@@ -290,11 +315,24 @@ int vfs_alm_rotate(vfs_alm *alm, double dt);
 int vfs_alm_preprocess(vfs_alm *alm);
 /* Calc_U_lagr (rotor_model.c:2937-3146) from VFS_UCAT. */
 int vfs_alm_u_lagr(vfs_alm *alm);
-/* Calc_F_lagr_ACL (rotor_model.c:2370-2681) without the optional tip/3D corrections. */
+/* Calc_F_lagr_ACL (rotor_model.c:2370-2681) with the corrections vfs_turbine selects
+ * (tip-loss models, 3D lift correction, force smoothing) and the nacelle zeroing. */
 int vfs_alm_f_lagr(vfs_alm *alm);
+/* Calc_forces_ACL (rotor_model.c:4148-4180): out[8] = {A_Sum, F_Sum, P_Sum, U_Sum / A_Sum,
+ * Fx_Sum, Fy_Sum, Fz_Sum, M_Sum} of the current F_lagr and U_lagr; keeps Torque_aero =
+ * -M_Sum for vfs_alm_turbine_angvel. */
+int vfs_alm_loads(vfs_alm *alm, double *out);
+/* Calc_turbineangvel (rotor_model.c:5519-5601) for time step ti with the reference
+ * velocity uref (vfs_alm_uref): updates the angular velocity the next rotation uses and
+ * the tip speed ratio of the tip-loss models; returns both (either pointer may be null).
+ * A restart from the turbine-rotation file is not read here (set vfs_turbine.angvel). */
+int vfs_alm_turbine_angvel(vfs_alm *alm, const vfs_turbine_control *tc, double dt, double uref, int ti, int tistart,
+                           double *angvel, double *tsr);
 /* VecSet(F_eul, 0) and Calc_F_eul (rotor_model.c:3668-3960) into VFS_FEUL. */
 int vfs_alm_f_eul(vfs_alm *alm);
-/* The rotor_model 6 block of Flow_Solver (solvers.c:535-563) with a fixed angular velocity. */
+/* The rotor_model 6 block of Flow_Solver (solvers.c:535-563) at the current angular
+ * velocity: rotate, Pre_process, U_lagr, F_lagr, F_eul (Uref_ACL, Calc_turbineangvel and
+ * Calc_forces_ACL are the separate calls above, made where the host needs them). */
 int vfs_alm_step(vfs_alm *alm, double dt);
 /* Copy an element or node array to the host. */
 int vfs_alm_get(vfs_alm *alm, vfs_alm_array what, double *host);

@@ -125,6 +125,7 @@ def load(path: str | os.PathLike | None = None) -> C.CDLL:
                            D, D, D, D], None),
         "orc_alm_feul": ([P, M, C.c_int, D, C.POINTER(C.c_int), D, D, C.c_int, C.c_double, D], None),
         "orc_alm_loads": ([C.POINTER(OrcTurbine), C.c_int, D, D, D, D, D], None),
+        "orc_turbine_angvel": ([C.c_int, D, C.c_double, C.c_double, C.c_int, C.c_int, D], None),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)
@@ -430,3 +431,13 @@ def _alm_methods():
 
 
 _alm_methods()
+
+
+def turbine_angvel(mode, prm, dt, uref, ti, tistart, state):
+    """Calc_turbineangvel: state = [angvel, Tipspeedratio, Torque_aero, Torque_generator,
+    ang_axis] updated and returned; prm as orc_turbine_angvel documents."""
+    lib = load()
+    prm = np.ascontiguousarray(prm, dtype=np.float64)
+    st = np.ascontiguousarray(state, dtype=np.float64).copy()
+    lib.orc_turbine_angvel(int(mode), _ptr(prm), float(dt), float(uref), int(ti), int(tistart), _ptr(st))
+    return st

@@ -226,6 +226,12 @@ void orc_alm_feul(const orc_params *p, const orc_metrics *m, int n, const double
  * Fz_Sum, M_Sum}; the reference's Torque_aero is -M_Sum and Force_axis -F_Sum. */
 void orc_alm_loads(const orc_turbine *t, int n, const double *cent, const double *flagr, const double *ulagr,
                    const double *dA, double out[8]);
+/* Calc_turbineangvel (rotor_model.c:5519-5601), one turbine, no restart file: mode 0
+ * -fixturbineangvel, 1 -FixTipSpeedRatio, 2 -turbinetorquecontrol.  st = {angvel,
+ * Tipspeedratio, Torque_aero, Torque_generator, ang_axis} is read and updated; prm =
+ * {angvel_fixed, tsr, r_rotor, reflength_wt, refvel_wt, refvel_cfd, rho_air, J_rotation,
+ * CP_max, TSR_max, X_control, fixturbinegeneratortorque, torque_generator}. */
+void orc_turbine_angvel(int mode, const double prm[13], double dt, double uref, int ti, int tistart, double st[5]);
 
 #ifdef __cplusplus
 }

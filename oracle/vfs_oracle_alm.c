@@ -515,3 +515,58 @@ void orc_alm_loads(const orc_turbine *t, int n, const double *cent, const double
     out[0] = A_Sum; out[1] = F_Sum; out[2] = P_Sum; out[3] = U_Sum / A_Sum;
     out[4] = Fx; out[5] = Fy; out[6] = Fz; out[7] = M_Sum;
 }
+
+/* Calc_turbineangvel, rotor_model.c:5519-5601 */
+void orc_turbine_angvel(int mode, const double prm[13], double dt, double uref, int ti, int tistart, double st[5])
+{
+    const double angvel_fixed = prm[0], tsr_in = prm[1], r_rotor = prm[2], reflength_wt = prm[3];
+    const double refvel_wt = prm[4], refvel_cfd = prm[5], rho_air = prm[6], J = prm[7], CP_max = prm[8];
+    const double TSR_max = prm[9], X_control = prm[10], Tgen_fixed = prm[12];
+    const int fixgen = prm[11] != 0;
+    double R0 = r_rotor / reflength_wt;
+    double rho_cfd = 1.0;
+    double T_wt = r_rotor / refvel_wt;
+    double T_cfd = R0 / refvel_cfd;
+    double ratio_rho = rho_air / rho_cfd;
+    double ratio_L = reflength_wt;
+    double ratio_T = T_wt / T_cfd;
+    double ratio_V = refvel_wt / refvel_cfd;
+    double R = r_rotor / reflength_wt;
+    if (ti == tistart) {
+        st[4] = 0.0;
+        if (mode == 1) {
+            double TSR = tsr_in;
+            st[0] = TSR * uref / R;
+            st[1] = tsr_in;
+        } else if (mode == 2) {
+            double TSR = TSR_max;
+            st[0] = TSR * uref / R;
+        } else {
+            st[0] = angvel_fixed;
+            st[1] = st[0] * R / uref;
+        }
+    } else {
+        if (mode == 1) {
+            st[1] = tsr_in;
+            st[0] = st[1] * uref / R;
+        } else if (mode == 0) {
+            st[0] = angvel_fixed;
+            st[1] = st[0] * R / uref;
+        } else {
+            double K = 0.5 * M_PI * pow(r_rotor, 2 + X_control) * CP_max / pow(TSR_max, X_control);
+            double angvel_wt = st[0] / ratio_T;
+            double UU = uref * ratio_V;
+            if (!fixgen) st[3] = K * pow(angvel_wt, X_control - 1) * pow(UU, 3 - X_control);
+            else st[3] = Tgen_fixed;
+            double Torque_c = st[3];
+            double Torque_a = st[2] * ratio_rho * pow(ratio_V, 2) * pow(ratio_L, 3);
+            double Torque = Torque_a - Torque_c;
+            double dt_wt = dt * ratio_T;
+            double angvel_0 = st[0] / ratio_T;
+            double angvel_1 = angvel_0 + dt_wt * Torque / J;
+            st[0] = angvel_1 * ratio_T;
+            st[1] = st[0] * R / uref;
+        }
+    }
+    st[4] += st[0] * dt;
+}

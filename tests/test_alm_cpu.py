@@ -347,3 +347,33 @@ def test_rotor_loads_of_a_known_force():
     assert Ps == pytest.approx(g * U * dA.sum(), rel=1e-14)
     assert Um == pytest.approx(U, rel=1e-14)
     assert (Fx, Fy, Fz) == pytest.approx((0.0, f * dA.sum(), g * dA.sum()), rel=1e-14)
+
+
+def test_turbine_angvel_modes_and_torque_balance():
+    """Calc_turbineangvel (rotor_model.c:5519-5601).  Fixed TSR: omega = TSR U / R;
+    fixed omega: TSR = omega R / U; torque control: the generator torque
+    K omega^(X-1) U^(3-X), K = pi R^(2+X) CP_max / (2 TSR_max^X), absorbs the power
+    1/2 pi R^2 CP_max U^3 at the design tip speed ratio whatever X is; with the
+    aerodynamic torque equal to it omega stays put, and a torque surplus dT
+    accelerates the rotor by dt dT / J.  ang_axis integrates omega dt."""
+    R, U, J, CP, TSRm = 40.0, 9.0, 3.2e6, 0.45, 7.5
+    for X in (2.0, 3.0):
+        # unit reference scales: ratio_T = ratio_V = ratio_L = 1, rho_air 1
+        prm = [0.7, 6.0, R, 1.0, 1.0, 1.0, 1.0, J, CP, TSRm, X, 0, 0.0]
+        w0 = TSRm * U / R
+        st = po.turbine_angvel(2, prm, 0.1, U, 0, 0, [0.0, 0.0, 0.0, 0.0, 5.0])
+        assert st[0] == pytest.approx(w0, rel=1e-15) and st[4] == pytest.approx(w0 * 0.1, rel=1e-15)
+        K = 0.5 * math.pi * R ** (2 + X) * CP / TSRm ** X
+        Tc = K * w0 ** (X - 1) * U ** (3 - X)
+        assert Tc * w0 == pytest.approx(0.5 * math.pi * R * R * CP * U ** 3, rel=1e-13)
+        st = po.turbine_angvel(2, prm, 0.1, U, 1, 0, [w0, 0.0, Tc, 0.0, 0.0])
+        assert st[0] == pytest.approx(w0, rel=1e-13) and st[3] == pytest.approx(Tc, rel=1e-13)
+        dT = 0.02 * Tc
+        st = po.turbine_angvel(2, prm, 0.1, U, 1, 0, [w0, 0.0, Tc + dT, 0.0, 0.0])
+        assert st[0] == pytest.approx(w0 + 0.1 * dT / J, rel=1e-12)
+        assert st[1] == pytest.approx(st[0] * R / U, rel=1e-15)
+    prm = [0.7, 6.0, R, 1.0, 1.0, 1.0, 1.0, J, CP, TSRm, 2.0, 0, 0.0]
+    st = po.turbine_angvel(1, prm, 0.1, U, 3, 0, [0.0, 0.0, 0.0, 0.0, 0.0])
+    assert st[0] == pytest.approx(6.0 * U / R, rel=1e-15) and st[1] == 6.0
+    st = po.turbine_angvel(0, prm, 0.1, U, 3, 0, [0.0, 0.0, 0.0, 0.0, 0.0])
+    assert st[0] == 0.7 and st[1] == pytest.approx(0.7 * R / U, rel=1e-15)

@@ -135,3 +135,65 @@ def test_alm_uref_disk():
     assert 0.9 < uref < 1.1  # the inflow is 1 + 5% noise
     alm.close()
     ctx.close()
+
+
+CORRECTIONS = {
+    "prandtl": dict(prandtl=1),
+    "shen": dict(shen=1),
+    "shen1": dict(shen1=1, correction_shen1=0.9),
+    "lift3d": dict(correction3d_cl=1, c0_cl=1.3),
+    "smooth": dict(smoothforce=2),
+    "all": dict(shen=1, correction3d_cl=1, smoothforce=1),
+}
+
+
+@pytest.mark.parametrize("name", list(CORRECTIONS))
+def test_f_lagr_corrections_and_loads(name):
+    """Calc_F_lagr_ACL with each optional correction (tip-loss models, 3D lift
+    correction, force smoothing; rotor_model.c:2488-2608) and Calc_forces_ACL's
+    rotor sums (rotor_model.c:4148-4180) against the oracle."""
+    vfs, ctx, o, t, ts, alm = _setup(6, 7, 2.0)
+    alm.close()
+    opts = dict(r_rotor=8.0, tsr=6.5, num_blade=ts.nodes.shape[0], **CORRECTIONS[name])
+    tv = ts.turbine_struct(vfs)
+    for k, v in opts.items():
+        setattr(tv, k, v)
+    alm = vfs.ALM(ctx, tv, ts.nodes, ts.foils)
+    to = po.make_turbine(ts.nx, ts.ny, ts.nz, ts.xc, ts.yc, ts.zc, ts.angvel, ts.pitch, ts.r_nacelle, ts.refangle,
+                         **opts)
+    nb, nvb = ts.nodes.shape[:2]
+    foils = po.Foils(ts.foils)
+    alm.rotate(0.05); alm.preprocess(); alm.u_lagr(); alm.f_lagr()
+    ul = alm.get("U_LAGR")
+    f, aoa, cl, cd = po.alm_flagr(to, nb, nvb, alm.get("CENT"), ul, alm.get("NODE_VEL"), alm.get("TWIST"),
+                                  alm.get("CHORD"), foils)
+    f0 = po.alm_flagr(t, nb, nvb, alm.get("CENT"), ul, alm.get("NODE_VEL"), alm.get("TWIST"), alm.get("CHORD"), foils)[0]
+    assert relerr(f, f0) > 1e-6  # the correction changes the forces
+    assert relerr(alm.get("F_LAGR"), f) <= TOL
+    loads = alm.loads()
+    want = po.alm_loads(to, alm.get("CENT"), alm.get("F_LAGR"), ul, alm.get("DA"))
+    assert np.allclose(loads, want, rtol=1e-13, atol=0)
+    alm.close()
+    ctx.close()
+
+
+@pytest.mark.parametrize("mode", [0, 1, 2], ids=["fixed-omega", "fixed-tsr", "torque-control"])
+def test_turbine_angvel_against_oracle(mode):
+    """Calc_turbineangvel over three steps (the first one initialises), with
+    the aerodynamic torque of vfs_alm_loads feeding the torque controller."""
+    vfs, ctx, o, t, ts, alm = _setup(6, 7, 2.0)
+    tc = vfs.TurbineControl(mode=mode, angvel_fixed=0.9, tsr=6.0, r_rotor=40.0, reflength_wt=5.0, refvel_wt=9.0,
+                            refvel_cfd=1.0, rho_air=1.2, J_rotation=4e6, CP_max=0.45, TSR_max=7.5, X_control=2.0,
+                            fixturbinegeneratortorque=0, torque_generator=0.0)
+    prm = [tc.angvel_fixed, tc.tsr, tc.r_rotor, tc.reflength_wt, tc.refvel_wt, tc.refvel_cfd, tc.rho_air, tc.J_rotation,
+           tc.CP_max, tc.TSR_max, tc.X_control, tc.fixturbinegeneratortorque, tc.torque_generator]
+    st = np.array([ts.angvel, 0.0, 0.0, 0.0, 0.0])
+    uref, dt = 0.95, 0.05
+    for ti in range(3):
+        w, tsr = alm.turbine_angvel(tc, dt, uref, ti, 0)
+        st = po.turbine_angvel(mode, prm, dt, uref, ti, 0, st)
+        assert w == pytest.approx(st[0], rel=1e-14, abs=0) and tsr == pytest.approx(st[1], rel=1e-14, abs=1e-300)
+        alm.rotate(dt); alm.preprocess(); alm.u_lagr(); alm.f_lagr()
+        st[2] = -alm.loads()[7]  # Torque_aero = -M_Sum
+    alm.close()
+    ctx.close()

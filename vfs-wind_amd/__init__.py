@@ -90,6 +90,26 @@ class Turbine(C.Structure):
         ("xc", C.c_double), ("yc", C.c_double), ("zc", C.c_double),
         ("angvel", C.c_double), ("pitch", C.c_double), ("r_nacelle", C.c_double), ("refangle", C.c_double),
         ("rotor_model", C.c_int), ("deltafunc", C.c_int), ("halfwidth", C.c_double),
+        ("shen", C.c_int), ("prandtl", C.c_int), ("shen1", C.c_int), ("r_rotor", C.c_double), ("tsr", C.c_double),
+        ("num_blade", C.c_int), ("a_shen", C.c_double), ("b_shen", C.c_double), ("c_shen", C.c_double),
+        ("correction_shen1", C.c_double), ("correction3d_cl", C.c_int), ("c0_cl", C.c_double), ("smoothforce", C.c_int),
+    ]
+
+    def __init__(self, *args, **kw):
+        # the correction options' defaults of main.c:161, 224-238
+        base = {"num_blade": 3, "a_shen": 0.125, "b_shen": 21.0, "c_shen": 0.1, "correction_shen1": 1.0, "c0_cl": 1.0}
+        base.update(kw)
+        super().__init__(*args, **base)
+
+
+class TurbineControl(C.Structure):
+    """vfs_turbine_control: Calc_turbineangvel's mode and parameters."""
+
+    _fields_ = [
+        ("mode", C.c_int), ("angvel_fixed", C.c_double), ("tsr", C.c_double), ("r_rotor", C.c_double),
+        ("reflength_wt", C.c_double), ("refvel_wt", C.c_double), ("refvel_cfd", C.c_double), ("rho_air", C.c_double),
+        ("J_rotation", C.c_double), ("CP_max", C.c_double), ("TSR_max", C.c_double), ("X_control", C.c_double),
+        ("fixturbinegeneratortorque", C.c_int), ("torque_generator", C.c_double),
     ]
 
 
@@ -159,6 +179,9 @@ _SIGS = {
     "vfs_alm_get": ([C.c_void_p, C.c_int, C.c_void_p], C.c_int),
     "vfs_alm_set_disk": ([C.c_void_p, C.c_int, C.c_void_p, C.c_void_p], C.c_int),
     "vfs_alm_uref": ([C.c_void_p, C.POINTER(C.c_double)], C.c_int),
+    "vfs_alm_loads": ([C.c_void_p, C.c_void_p], C.c_int),
+    "vfs_alm_turbine_angvel": ([C.c_void_p, C.POINTER(TurbineControl), C.c_double, C.c_double, C.c_int, C.c_int,
+                                C.POINTER(C.c_double), C.POINTER(C.c_double)], C.c_int),
     "vfs_formfunction_snes": ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_int], C.c_int),
     "vfs_store_old_state": ([C.c_void_p], C.c_int),
     "vfs_implicit_matrixfree": ([C.c_void_p, C.POINTER(NewtonOpts), C.POINTER(NewtonStats)], C.c_int),
@@ -688,6 +711,19 @@ class ALM:
         v = C.c_double()
         self.ctx._check(self.lib.vfs_alm_uref(self.h, C.byref(v)))
         return v.value
+
+    def loads(self) -> np.ndarray:
+        """Calc_forces_ACL: (A_Sum, F_Sum, P_Sum, U_mean, Fx, Fy, Fz, M_Sum)."""
+        out = np.zeros(8)
+        self.ctx._check(self.lib.vfs_alm_loads(self.h, out.ctypes.data))
+        return out
+
+    def turbine_angvel(self, control: "TurbineControl", dt: float, uref: float, ti: int, tistart: int = 0):
+        """Calc_turbineangvel: (angular velocity, tip speed ratio) after the update."""
+        w, t = C.c_double(), C.c_double()
+        self.ctx._check(self.lib.vfs_alm_turbine_angvel(self.h, C.byref(control), float(dt), float(uref), int(ti),
+                                                        int(tistart), C.byref(w), C.byref(t)))
+        return w.value, t.value
 
 
 class UNet:

@@ -203,9 +203,45 @@ __device__ inline double polar(double aoa, int nq, const double *ang, const doub
 struct TurbineDev {
     double nx, ny, nz, xc, yc, zc, pitch, r_nacelle;
     int nb, nvb;
+    // optional corrections (rotor_model.c:2488-2558)
+    int shen, prandtl, shen1, c3d, num_blade, nacelle_here;
+    double r_rotor, tsr, a_shen, b_shen, c_shen, k_shen1, c0_cl;
 };
 
-// Calc_F_lagr_ACL, rotor_model.c:2384-2597 (rotor_model 3/6, no optional corrections)
+// tip-loss factor F1 of the selected models, rotor_model.c:2488-2544: the later
+// model wins; these branches use pi = 3.1415926
+__device__ inline double tip_loss(const TurbineDev &t, double r, double aoa, double twist)
+{
+    double F1 = 1.0;
+    if (t.shen) {
+        const double R = t.r_rotor, pi1 = 3.1415926;
+        const double phi1 = (aoa + twist + t.pitch) * pi1 / 180.0;
+        const double g1 = exp(-t.a_shen * (t.num_blade * fabs(t.tsr) - t.b_shen)) + t.c_shen;
+        F1 = 2.0 * acos(exp(-g1 * t.num_blade * (fmax(R - r, 0.0)) / 2.0 / r / fabs(sin(phi1)))) / pi1;
+    }
+    if (t.prandtl) {
+        const double R = t.r_rotor, pi1 = 3.1415926;
+        const double g1 = sqrt(1 + pow(t.tsr, 2));
+        F1 = 2.0 * acos(exp(-g1 * t.num_blade * (fmax(R - r, 0.0)) / 2.0 / R)) / pi1;
+    }
+    if (t.shen1) {
+        const double R = t.r_rotor, pi1 = 3.1415926;
+        const double phi1 = (aoa + twist + t.pitch) * pi1 / 180.0;
+        const double lam = fabs(t.tsr);
+        double g1;
+        if (lam < 7.0636) g1 = -1.7636 * lam + 11.6112;
+        else if (lam > 7.0636 && lam < 9.8146) g1 = -0.5805 * lam + 3.2542;
+        else g1 = -0.5077 * lam + 2.5397;
+        g1 *= t.k_shen1;
+        g1 = exp(g1);
+        F1 = 2.0 * acos(exp(-g1 * t.num_blade * (fmax(R - r, 0.0)) / 2.0 / r / fabs(sin(phi1)))) / pi1;
+    }
+    return F1;
+}
+
+// Calc_F_lagr_ACL, rotor_model.c:2384-2597 (rotor_model 3/6): blade-element
+// forces with the selected tip-loss factor and 3D lift correction; the
+// nacelle zeroing here unless the force smoothing runs first
 __global__ void k_alm_flagr(TurbineDev t, Polars P, const double *__restrict__ cent, const double *__restrict__ ulagr,
                             const double *__restrict__ unode, const double *__restrict__ twist,
                             const double *__restrict__ chord, double *__restrict__ aoa, double *__restrict__ cl,
@@ -253,8 +289,44 @@ __global__ void k_alm_flagr(TurbineDev t, Polars P, const double *__restrict__ c
     double F[3];
     for (int c = 0; c < 3; c++) F[c] = -0.5 * uref * uref * fabs(vcd) * nrel[c] * factor;
     for (int c = 0; c < 3; c++) F[c] += -0.5 * uref * uref * fabs(vcl) * nL[c] * factor;
+    const double F1 = tip_loss(t, r, a, twist[l]);
+    for (int c = 0; c < 3; c++) F[c] *= F1;
+    // 3D lift correction inboard of 0.75 R along the rotor axis, rotor_model.c:2548-2558
+    if (t.c3d && r < 0.75 * t.r_rotor) {
+        const double inflow = twist[l] * pi / 180 + a * pi / 180;
+        const double Fc = -0.5 * t.c0_cl * pow(chord[l] / r, 1) * pow(cos(inflow), 2) * uref * uref * factor;
+        F[0] += Fc * t.nx;
+        F[1] += Fc * t.ny;
+        F[2] += Fc * t.nz;
+    }
     const double rn = sqrt(rx * rx + ry * ry + rz * rz);
-    for (int c = 0; c < 3; c++) flagr[3 * l + c] = rn < t.r_nacelle ? 0.0 : F[c];
+    for (int c = 0; c < 3; c++) flagr[3 * l + c] = t.nacelle_here && rn < t.r_nacelle ? 0.0 : F[c];
+}
+
+// one pass of the force smoothing along each blade, rotor_model.c:2562-2598:
+// element j in [4, ne - 4) of its blade takes sum_{i=j-4}^{j+3} F_i dfunc_s4h(i - j)
+// of the previous pass (prev); the other elements keep their force
+__global__ void k_alm_smooth(int nb, int ne, const double *__restrict__ prev, double *__restrict__ flagr)
+{
+    const int l = blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= nb * ne) return;
+    const int b = l / ne, jj = l - b * ne;
+    if (jj < 4 || jj >= ne - 4) return;
+    double s[3] = {0, 0, 0};
+    for (int i = l - 4; i < l + 4; i++) {
+        const double w = dfunc_s4h((double)(i - l));
+        for (int c = 0; c < 3; c++) s[c] += prev[3 * i + c] * w;
+    }
+    for (int c = 0; c < 3; c++) flagr[3 * l + c] = s[c];
+}
+
+// elements inside the nacelle radius carry no force, rotor_model.c:2599-2608
+__global__ void k_alm_nacelle(TurbineDev t, const double *__restrict__ cent, double *__restrict__ flagr)
+{
+    const int l = blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= t.nb * (t.nvb - 1)) return;
+    const double rx = cent[3 * l] - t.xc, ry = cent[3 * l + 1] - t.yc, rz = cent[3 * l + 2] - t.zc;
+    if (sqrt(rx * rx + ry * ry + rz * rz) < t.r_nacelle) flagr[3 * l] = flagr[3 * l + 1] = flagr[3 * l + 2] = 0.0;
 }
 
 // Calc_F_eul, rotor_model.c:3738-3830, gathered per cell over the union box
@@ -355,6 +427,9 @@ struct vfs_alm {
     int n_disk = 0;
     std::vector<double> disk_cent, disk_dA;
     std::vector<int> box_host;  // box of the elements spread by Calc_F_eul
+    double *d_ftmp = nullptr;   // previous pass of the force smoothing
+    // Calc_forces_ACL / Calc_turbineangvel state (fsi.Torque_aero, Torque_generator, ang_axis)
+    double torque_aero = 0, torque_generator = 0, ang_axis = 0;
 };
 
 namespace vfs {
@@ -522,6 +597,7 @@ int vfs_alm_create(vfs_ctx *c, const vfs_turbine *t, int num_blade, int nv_blade
     rc = rc ? rc : dal((void **)&a->d_chord, ne * 8);
     rc = rc ? rc : dal((void **)&a->d_ulagr, 3 * ne * 8);
     rc = rc ? rc : dal((void **)&a->d_flagr, 3 * ne * 8);
+    rc = rc ? rc : dal((void **)&a->d_ftmp, 3 * ne * 8);
     rc = rc ? rc : dal((void **)&a->d_aoa, ne * 8);
     rc = rc ? rc : dal((void **)&a->d_cl, ne * 8);
     rc = rc ? rc : dal((void **)&a->d_cd, ne * 8);
@@ -548,7 +624,7 @@ int vfs_alm_destroy(vfs_alm *a)
     hipStreamSynchronize(a->ctx->stream);
     void *ps[] = {a->d_cent, a->d_refcent, a->d_dA, a->d_unode, a->d_twist, a->d_chord, a->d_ulagr, a->d_flagr, a->d_aoa,
                   a->d_cl, a->d_cd, a->d_tab, a->d_box, a->d_refbox, a->d_off, a->d_dcent, a->d_dA_disk, a->d_ulagr_disk,
-                  a->d_dbox};
+                  a->d_dbox, a->d_ftmp};
     for (void *p : ps)
         if (p) hipFree(p);
     delete a;
@@ -619,11 +695,110 @@ int vfs_alm_f_lagr(vfs_alm *a)
     if (!a) return set_error(VFS_EINVAL, "vfs_alm_f_lagr: null handle");
     vfs_ctx *c = a->ctx;
     const int ne = a->nb * a->ne;
-    TurbineDev td{a->t.nx, a->t.ny, a->t.nz, a->t.xc, a->t.yc, a->t.zc, a->t.pitch, a->t.r_nacelle, a->nb, a->nvb};
+    const vfs_turbine &T = a->t;
+    const int smooth = T.smoothforce;
+    TurbineDev td{T.nx, T.ny, T.nz, T.xc, T.yc, T.zc, T.pitch, T.r_nacelle, a->nb, a->nvb, T.shen, T.prandtl, T.shen1,
+                  T.correction3d_cl, T.num_blade, smooth ? 0 : 1, T.r_rotor, T.tsr, T.a_shen, T.b_shen, T.c_shen,
+                  T.correction_shen1, T.c0_cl};
     Polars P{(int)a->foil_rbeg.size(), a->d_off, a->d_tab};
     hipLaunchKernelGGL(k_alm_flagr, dim3(blocks_for(ne, 64)), dim3(64), 0, c->stream, td, P, a->d_cent, a->d_ulagr, a->d_unode,
                        a->d_twist, a->d_chord, a->d_aoa, a->d_cl, a->d_cd, a->d_flagr);
+    if (smooth) {
+        // smoothforce_AL + 1 passes (the reference's do-while), then the nacelle zeroing
+        for (int pass = 0; pass <= smooth; pass++) {
+            VFS_HIP(hipMemcpyAsync(a->d_ftmp, a->d_flagr, 3 * (size_t)ne * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+            hipLaunchKernelGGL(k_alm_smooth, dim3(blocks_for(ne, 64)), dim3(64), 0, c->stream, a->nb, a->ne, a->d_ftmp,
+                               a->d_flagr);
+        }
+        hipLaunchKernelGGL(k_alm_nacelle, dim3(blocks_for(ne, 64)), dim3(64), 0, c->stream, td, a->d_cent, a->d_flagr);
+    }
     VFS_LAUNCH_CHECK("alm f_lagr");
+    return VFS_OK;
+}
+
+int vfs_alm_loads(vfs_alm *a, double *out)
+{
+    // Calc_forces_ACL, rotor_model.c:4148-4180, summed on the host in element order
+    if (!a || !out) return set_error(VFS_EINVAL, "vfs_alm_loads: null argument");
+    vfs_ctx *c = a->ctx;
+    const int ne = a->nb * a->ne;
+    std::vector<double> F(3 * (size_t)ne), U(3 * (size_t)ne);
+    VFS_HIP(hipMemcpyAsync(F.data(), a->d_flagr, F.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    VFS_HIP(hipMemcpyAsync(U.data(), a->d_ulagr, U.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    VFS_HIP(hipStreamSynchronize(c->stream));
+    double A_Sum = 0, F_Sum = 0, P_Sum = 0, U_Sum = 0, Fx = 0, Fy = 0, Fz = 0, M_Sum = 0;
+    const double nx = a->t.nx, ny = a->t.ny, nz = a->t.nz;
+    for (int l = 0; l < ne; l++) {
+        const double *f = &F[3 * l], *u = &U[3 * l], dA = a->dA[l];
+        A_Sum += dA;
+        const double F_axis = f[0] * nx + f[1] * ny + f[2] * nz;
+        const double U_axis = u[0] * nx + u[1] * ny + u[2] * nz;
+        F_Sum += F_axis * dA;
+        P_Sum += F_axis * dA * U_axis;
+        U_Sum += U_axis * dA;
+        Fx += f[0] * dA;
+        Fy += f[1] * dA;
+        Fz += f[2] * dA;
+        const double rx = a->cent[3 * l] - a->t.xc, ry = a->cent[3 * l + 1] - a->t.yc, rz = a->cent[3 * l + 2] - a->t.zc;
+        const double M_x = ry * f[2] * dA - rz * f[1] * dA;
+        const double M_y = rz * f[0] * dA - rx * f[2] * dA;
+        const double M_z = rx * f[1] * dA - ry * f[0] * dA;
+        M_Sum += M_x * nx + M_y * ny + M_z * nz;
+    }
+    const double v[8] = {A_Sum, F_Sum, P_Sum, U_Sum / A_Sum, Fx, Fy, Fz, M_Sum};
+    std::memcpy(out, v, sizeof(v));
+    a->torque_aero = -M_Sum;
+    return VFS_OK;
+}
+
+int vfs_alm_turbine_angvel(vfs_alm *a, const vfs_turbine_control *tc, double dt, double uref, int ti, int tistart,
+                           double *angvel, double *tsr)
+{
+    // Calc_turbineangvel, rotor_model.c:5519-5601 (one turbine; no restart file)
+#pragma clang fp contract(off)
+    if (!a || !tc) return set_error(VFS_EINVAL, "vfs_alm_turbine_angvel: null argument");
+    if (tc->mode < 0 || tc->mode > 2) return set_error(VFS_EINVAL, "vfs_alm_turbine_angvel: mode must be 0, 1 or 2");
+    if (!(tc->reflength_wt > 0) || !(uref != 0)) return set_error(VFS_EINVAL, "vfs_alm_turbine_angvel: reflength_wt and uref");
+    const double R0 = tc->r_rotor / tc->reflength_wt;
+    const double rho_cfd = 1.0;
+    const double T_wt = tc->r_rotor / tc->refvel_wt;
+    const double T_cfd = R0 / tc->refvel_cfd;
+    const double ratio_rho = tc->rho_air / rho_cfd, ratio_L = tc->reflength_wt, ratio_T = T_wt / T_cfd;
+    const double ratio_V = tc->refvel_wt / tc->refvel_cfd;
+    const double R = tc->r_rotor / tc->reflength_wt;
+    vfs_turbine &T = a->t;
+    if (ti == tistart) {
+        a->ang_axis = 0.0;
+        if (tc->mode == 1) T.angvel = tc->tsr * uref / R, T.tsr = tc->tsr;
+        else if (tc->mode == 2) T.angvel = tc->TSR_max * uref / R;
+        else T.angvel = tc->angvel_fixed, T.tsr = T.angvel * R / uref;
+    } else {
+        if (tc->mode == 1) {
+            T.tsr = tc->tsr;
+            T.angvel = T.tsr * uref / R;
+        } else if (tc->mode == 0) {
+            T.angvel = tc->angvel_fixed;
+            T.tsr = T.angvel * R / uref;
+        } else {
+            const double J = tc->J_rotation;
+            const double K = 0.5 * M_PI * pow(tc->r_rotor, 2 + tc->X_control) * tc->CP_max / pow(tc->TSR_max, tc->X_control);
+            const double angvel_wt = T.angvel / ratio_T;
+            const double UU = uref * ratio_V;
+            if (!tc->fixturbinegeneratortorque) a->torque_generator = K * pow(angvel_wt, tc->X_control - 1) * pow(UU, 3 - tc->X_control);
+            else a->torque_generator = tc->torque_generator;
+            const double Torque_c = a->torque_generator;
+            const double Torque_a = a->torque_aero * ratio_rho * pow(ratio_V, 2) * pow(ratio_L, 3);
+            const double Torque = Torque_a - Torque_c;
+            const double dt_wt = dt * ratio_T;
+            const double angvel_0 = T.angvel / ratio_T;
+            const double angvel_1 = angvel_0 + dt_wt * Torque / J;
+            T.angvel = angvel_1 * ratio_T;
+            T.tsr = T.angvel * R / uref;
+        }
+    }
+    a->ang_axis += T.angvel * dt;
+    if (angvel) *angvel = T.angvel;
+    if (tsr) *tsr = T.tsr;
     return VFS_OK;
 }
 
