@@ -61,6 +61,9 @@ ALGO = {
     # compact FP32 smoother (weak cross couplings): four axis ratios as FP16 8 + diagonal 4 + b 4 + x 4;
     # line rhs out 4
     "line_rhs_compact_fp32": (8 + 4 + 4 + 4 + 4, "line point"),
+    # the same smoother in one pass (right-hand side formed inside the line solve): four axis ratios
+    # 8 + diagonal 4 + b 4 + x 4 + Thomas factors 12; x out 4
+    "line_relax_compact_fp32": (8 + 4 + 4 + 4 + 12 + 4, "line point"),
     # FP64: 19 coefficients 152 + x 8 + row mask 1; y out 8
     "poisson_apply_fp64": (152 + 8 + 1 + 8, "point"),
 }

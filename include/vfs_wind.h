@@ -465,13 +465,15 @@ int vfs_profile(vfs_ctx *ctx, int enable);
  * kernel_mask (0 disables); nested regions of unselected ids record nothing.
  * Fewer event pairs on the stream perturb the timed run less. */
 int vfs_profile_select(vfs_ctx *ctx, unsigned int kernel_mask);
-/* Total time and launch count recorded for kernel id 0..12 (finest level for
+/* Total time and launch count recorded for kernel id 0..13 (finest level for
  * the smoother): face_fluxes, cell_rhs, line_rhs_fp32, line_rhs_fp64,
  * poisson_apply_fp64, contra2cart, pressure_gradient, eddy_viscosity,
  * line_solve_fp32, line_solve_fp64, cell_fp, formfunction2 (one whole
- * Formfunction_2 call: face pass, its periodic fix and both cell passes), and
+ * Formfunction_2 call: face pass, its periodic fix and both cell passes),
  * line_rhs_compact_fp32 (the line right-hand side of the compact FP32
- * smoother, which a level takes when its cross couplings are weak). */
+ * smoother, which a level takes when its cross couplings are weak) and
+ * line_relax_compact_fp32 (the compact smoother's one-pass form: right-hand
+ * side and line solve of one colour in one launch). */
 int vfs_profile_read(vfs_ctx *ctx, int kernel, double *total_ms, long long *count, const char **name);
 
 #ifdef __cplusplus

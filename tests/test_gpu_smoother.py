@@ -71,6 +71,21 @@ def test_line_solve_paths(monkeypatch, path, fp32):
             monkeypatch.delenv("VFS_LINE_PATH")
 
 
+@pytest.mark.parametrize("cells", [(24, 16, 20), (18, 13, 22), (40, 70, 36)], ids=["24x16x20", "18x13x22", "40x70x36"])
+@pytest.mark.parametrize("path", ["short", "long"])
+def test_fused_compact_relaxation_bitwise(monkeypatch, path, cells):
+    """The compact smoother's one-pass relaxation (each line segment forms its
+    right-hand sides inside the line solve) reads only the other colour, as
+    the two-pass form does, so the whole FGMRES solve gives the same bits and
+    the same iteration count with either form."""
+    env = {"VFS_SMOOTH_COMPACT": "1", "VFS_LINE_PATH": path}
+    st1, phi1, res1 = _solve(monkeypatch, {**env, "VFS_ZEBRA_FUSED": "1"}, cells, tol=1e-10)
+    st0, phi0, res0 = _solve(monkeypatch, {**env, "VFS_ZEBRA_FUSED": "0"}, cells, tol=1e-10)
+    assert st0.relres <= 1e-10 and res0 <= 1.01e-10
+    assert st1.iterations == st0.iterations
+    assert np.array_equal(phi1, phi0)
+
+
 def test_compact_smoother_odd_periodic(monkeypatch):
     """An odd periodic interior count (19 cells in x) puts lines of one colour
     next to each other across the wrap; the right-hand side pass reads the

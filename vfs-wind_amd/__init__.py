@@ -231,7 +231,7 @@ _SIGS = {
 KERNEL_IDS = {"face_fluxes": 0, "cell_rhs": 1, "line_rhs_fp32": 2, "line_rhs_fp64": 3,
               "poisson_apply_fp64": 4, "contra2cart": 5, "pressure_gradient": 6, "eddy_viscosity": 7,
               "line_solve_fp32": 8, "line_solve_fp64": 9, "cell_fp": 10, "formfunction2": 11,
-              "line_rhs_compact_fp32": 12}
+              "line_rhs_compact_fp32": 12, "line_relax_compact_fp32": 13}
 
 
 class VFSError(RuntimeError):

@@ -747,6 +747,111 @@ __global__ void __launch_bounds__(LINE_LINES * LINE_MAXSEG) k_zebra_scan(Grid g,
     }
 }
 
+// Compact line relaxation of colour c in one pass: each (line, segment) thread
+// forms its points' line right-hand sides as k_zebra_rhs_c4 does, in
+// registers, and solves them as k_zebra_scan does, so rz is neither written
+// nor read back and a half-sweep is one launch instead of two.  The compact
+// form's off-line neighbours (i+-1 and k+-1 in the line's own row) all belong
+// to the other colour when every periodic direction has an even interior
+// count (zebra_fused_ok), and this pass writes only colour c, so the values it
+// reads are the ones the two-pass form reads: the same bits.  Blocks take
+// their 32 lines in XCD order, so the k+-1 rows a block reads were read by the
+// blocks just before it on the same XCD.
+__host__ __device__ inline bool zebra_fused_ok(const Grid &g)
+{
+    return (!g.pi || ((g.mx - 2) & 1) == 0) && (!g.pk || ((g.mz - 2) & 1) == 0);
+}
+
+template <int SLM>
+__global__ void __launch_bounds__(LINE_LINES * LINE_MAXSEG) k_zebra_line_c4(Grid g, ZGrid z, int c, int nseg, int sl,
+                                                                           const h4_t *__restrict__ sh,
+                                                                           const float *__restrict__ sd,
+                                                                           const float *__restrict__ b,
+                                                                           const float *__restrict__ sf, float *x)
+{
+    __shared__ float sA[LINE_MAXSEG * LINE_LINES], sB[LINE_MAXSEG * LINE_LINES];
+    const int ln = threadIdx.x % LINE_LINES, seg = threadIdx.x / LINE_LINES;
+    const long long nlines = (long long)z.nkI * z.nh;
+    const long long lid = (long long)xcd_block1() * LINE_LINES + ln;
+    int kk = 0, h = 0, k = 1, i = 1;
+    bool act = lid < nlines;
+    if (act) {
+        kk = (int)(lid / z.nh);
+        h = (int)(lid % z.nh);
+        k = kk + 1;
+        i = zebra_i0(k, c) + 2 * h;
+        act = i <= g.mx - 2;
+    }
+    const int j0 = seg * sl;
+    const int j1 = j0 + sl < z.njI ? j0 + sl : z.njI;
+    const long long sstride = 2 * z.cplane;
+    const long long base = (long long)kk * z.njI * z.nh + h;  // + jj * nh
+    const long long cb = c * z.cplane + base;
+    const float *lo = sf + cb, *mm = sf + sstride + cb, *cp = sf + 2 * sstride + cb;
+    const long long nh = z.nh;
+    const int I[3] = {canon(i - 1, g.mx, g.pi), i, canon(i + 1, g.mx, g.pi)};
+    const int K[3] = {canon(k - 1, g.mz, g.pk), k, canon(k + 1, g.mz, g.pk)};
+    float r[SLM], l[SLM], m[SLM], u[SLM];
+#pragma unroll
+    for (int v = 0; v < SLM; v++) {
+        const int jj = j0 + v;
+        if (act && jj < j1) {
+            const long long o = jj * nh;
+            r[v] = c4_rhs(g, I, K, jj + 1, sh[cb + o], sd[cb + o], x, b[((long long)k * g.my + jj + 1) * g.mx + i]);
+            l[v] = lo[o];
+            m[v] = mm[o];
+            u[v] = cp[o];
+        } else {
+            r[v] = 0; l[v] = 0; m[v] = 0; u[v] = 0;
+        }
+    }
+    // forward map of the segment: d_out = A d_in + B
+    float A = 1, B = 0;
+#pragma unroll
+    for (int v = 0; v < SLM; v++)
+        if (j0 + v < j1) {
+            B = (r[v] - l[v] * B) * m[v];
+            A = (-l[v] * A) * m[v];
+        }
+    const int me = seg * LINE_LINES + ln;
+    sA[me] = A;
+    sB[me] = B;
+    __syncthreads();
+    float d = 0;
+    for (int s = 0; s < seg; s++) d = sA[s * LINE_LINES + ln] * d + sB[s * LINE_LINES + ln];
+#pragma unroll
+    for (int v = 0; v < SLM; v++)
+        if (j0 + v < j1) {
+            d = (r[v] - l[v] * d) * m[v];
+            r[v] = d;
+        }
+    __syncthreads();
+    // backward map of the segment: x_first = A x_above + B
+    A = 1;
+    B = 0;
+#pragma unroll
+    for (int v = SLM - 1; v >= 0; v--)
+        if (j0 + v < j1) {
+            B = r[v] - u[v] * B;
+            A = -u[v] * A;
+        }
+    sA[me] = A;
+    sB[me] = B;
+    __syncthreads();
+    float xl = 0;
+    for (int s = nseg - 1; s > seg; s--) xl = sA[s * LINE_LINES + ln] * xl + sB[s * LINE_LINES + ln];
+    if (act) {
+#pragma unroll
+        for (int v = SLM - 1; v >= 0; v--) {
+            const int jj = j0 + v;
+            if (jj < j1) {
+                xl = r[v] - u[v] * xl;
+                x[((long long)k * g.my + jj + 1) * g.mx + i] = xl;
+            }
+        }
+    }
+}
+
 // coarsest level: x = B^{-1} b over the rows, B^{-1} stored column-major.
 // Each block gathers b into LDS once.
 constexpr int DENSE_ROWS = 64, DENSE_SPLIT = 16, DENSE_LDS = 1024;
@@ -1484,8 +1589,23 @@ static void smooth(vfs_ctx *c, PLevel &L, int sweeps)
     const bool c4 = sizeof(T) == 4 && L.compact;
     const h4_t *sh4 = reinterpret_cast<const h4_t *>(L.sh16);
     const unsigned scan_blocks = (unsigned)((lines + LINE_LINES - 1) / LINE_LINES);
+    // the compact form's right-hand sides are formed inside the line solve
+    // (k_zebra_line_c4) unless VFS_ZEBRA_FUSED=0 asks for the two-pass form
+    const char *fz = std::getenv("VFS_ZEBRA_FUSED");
+    const bool fused = c4 && scan && zebra_fused_ok(L.g) && !(fz && fz[0] == '0');
     for (int s = 0; s < sweeps; s++)
         for (int col = 0; col < 2; col++) {
+            if (fused) {
+                if (fine) prof_begin(c, K_LRELAXC32);
+                if (short_seg)
+                    hipLaunchKernelGGL((k_zebra_line_c4<LINE_SLM_SHORT>), dim3(scan_blocks), dim3(LINE_LINES * nseg), 0, st, L.g,
+                                       L.z, col, nseg, sl, sh4, L.sd32, L.b32, L.sc32, L.x32);
+                else
+                    hipLaunchKernelGGL((k_zebra_line_c4<line_slm<float>()>), dim3(scan_blocks), dim3(LINE_LINES * nseg), 0, st,
+                                       L.g, L.z, col, nseg, sl, sh4, L.sd32, L.b32, L.sc32, L.x32);
+                if (fine) prof_end(c);
+                continue;
+            }
             if (fine) prof_begin(c, sizeof(T) == 4 ? (c4 ? K_LINEC32 : K_LINE32) : K_LINE64);
             if constexpr (sizeof(T) == 4) {
                 if (c4)

@@ -508,7 +508,7 @@ int vfs_divergence(vfs_ctx *c, double *maxdiv)
 static const char *k_names[vfs::K_NUMK] = {"face_fluxes",     "cell_rhs",          "line_rhs_fp32",  "line_rhs_fp64",
                                            "poisson_apply_fp64", "contra2cart",     "pressure_gradient", "eddy_viscosity",
                                            "line_solve_fp32",  "line_solve_fp64",   "cell_fp",           "formfunction2",
-                                           "line_rhs_compact_fp32"};
+                                           "line_rhs_compact_fp32", "line_relax_compact_fp32"};
 
 int vfs_profile(vfs_ctx *c, int enable)
 {

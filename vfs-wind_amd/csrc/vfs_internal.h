@@ -174,6 +174,7 @@ enum KernelId {
     K_FACE = 0, K_CELL, K_LINE32, K_LINE64, K_APPLY64, K_C2C, K_PGRAD, K_EDDY, K_LSOLVE32, K_LSOLVE64, K_CELLFP,
     K_FF,       // a whole Formfunction_2 (face pass, periodic fix, both cell passes)
     K_LINEC32,  // the compact FP32 smoother's line right-hand side
+    K_LRELAXC32,  // the compact FP32 smoother's one-pass line relaxation (right-hand side and solve)
     K_NUMK
 };
 }
