@@ -15,7 +15,7 @@ from helpers import cases, channel_pair, po, relerr, vfs
 pytestmark = pytest.mark.gpu
 
 
-def _solve(monkeypatch, env, shape_cells, tol=1e-8):
+def _solve(monkeypatch, env, shape_cells, tol=1e-8, mg_fp32=1):
     for k, v in env.items():
         monkeypatch.setenv(k, v)
     c, o, ctx = channel_pair(*shape_cells, les=0)
@@ -23,7 +23,7 @@ def _solve(monkeypatch, env, shape_cells, tol=1e-8):
         o.poisson_rhs()
         ctx.poisson_rhs()
         ctx.upload("PHI", np.zeros(o.shape))
-        opts = vfs.default_solve_opts(tol=tol, mg_fp32=1, use_guess=0, maxit=400)
+        opts = vfs.default_solve_opts(tol=tol, mg_fp32=mg_fp32, use_guess=0, maxit=400)
         st = ctx.poisson_solve(opts)
         phi = ctx.download("PHI")
         b = o["b"]
@@ -72,15 +72,21 @@ def test_line_solve_paths(monkeypatch, path, fp32):
 
 
 @pytest.mark.parametrize("cells", [(24, 16, 20), (18, 13, 22), (40, 70, 36)], ids=["24x16x20", "18x13x22", "40x70x36"])
-@pytest.mark.parametrize("path", ["short", "long"])
-def test_fused_compact_relaxation_bitwise(monkeypatch, path, cells):
-    """The compact smoother's one-pass relaxation (each line segment forms its
-    right-hand sides inside the line solve) reads only the other colour, as
-    the two-pass form does, so the whole FGMRES solve gives the same bits and
-    the same iteration count with either form."""
-    env = {"VFS_SMOOTH_COMPACT": "1", "VFS_LINE_PATH": path}
-    st1, phi1, res1 = _solve(monkeypatch, {**env, "VFS_ZEBRA_FUSED": "1"}, cells, tol=1e-10)
-    st0, phi0, res0 = _solve(monkeypatch, {**env, "VFS_ZEBRA_FUSED": "0"}, cells, tol=1e-10)
+@pytest.mark.parametrize("form", ["compact-short", "compact-long", "full", "fp64"])
+def test_one_pass_forms_bitwise(monkeypatch, form, cells):
+    """The V-cycle's one-pass forms against their two-pass forms, on whole
+    FGMRES solves: the compact smoother's relaxation that forms each line
+    segment's right-hand sides inside the line solve (it reads only the other
+    colour, as the two-pass form does; on coarse levels its first half-sweep
+    takes the iterate as zero instead of reading a zero-filled one), and the
+    residual formed inside the restriction instead of stored (compact, full
+    and FP64 levels).  Same bits, same iteration count."""
+    env = {"VFS_LINE_PATH": "short" if form == "compact-short" else "long"}
+    env["VFS_SMOOTH_COMPACT"] = "0" if form == "full" else "1"
+    fp32 = 0 if form == "fp64" else 1
+    st1, phi1, res1 = _solve(monkeypatch, env, cells, tol=1e-10, mg_fp32=fp32)
+    st0, phi0, res0 = _solve(monkeypatch, {**env, "VFS_ZEBRA_FUSED": "0", "VFS_MG_FUSED_RESTRICT": "0"}, cells, tol=1e-10,
+                             mg_fp32=fp32)
     assert st0.relres <= 1e-10 and res0 <= 1.01e-10
     assert st1.iterations == st0.iterations
     assert np.array_equal(phi1, phi0)

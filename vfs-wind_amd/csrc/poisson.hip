@@ -371,12 +371,14 @@ __global__ void __launch_bounds__(PTPB) k_apply_mf(Grid g, FaceG F, double th, c
 
 // ---- zebra line Gauss-Seidel over wall-normal (j) lines --------------------
 // Lines (i,k) are coloured by (i+k) & 1.  A sweep relaxes colour 0 then colour
-// 1; each half-sweep is two kernels: a fully parallel pass that folds the
-// off-line part of every row into a line right-hand side (reading the other
-// colour's current values), then one thread per line runs the forward and
-// backward Thomas recurrences with factors precomputed at setup.  The
-// smoother's coefficients live in a colour-split copy so that both passes
-// read contiguous memory: plane [slot][colour][k][j][h] with i = i0 + 2h.
+// 1; a half-sweep folds the off-line part of every row into a line
+// right-hand side (reading the other colour's current values), then runs the
+// forward and backward Thomas recurrences of each line with factors
+// precomputed at setup.  The full and FP64 forms do this in two kernels (a
+// fully parallel right-hand-side pass, then the line solve); the compact form
+// in one (k_zebra_line_c4).  The smoother's coefficients live in a
+// colour-split copy so that the passes read contiguous memory: plane
+// [slot][colour][k][j][h] with i = i0 + 2h.
 constexpr int NOFF = 16;
 __host__ __device__ constexpr int offline_slot(int u)
 {
@@ -464,6 +466,15 @@ __global__ void __launch_bounds__(PTPB) k_zebra_rhs_h2(Grid g, ZGrid z, int c, c
 
 // Compact line right-hand side of one point: b - d (r_E x_E + r_W x_W +
 // r_T x_T + r_B x_B), the neighbours in the point's own row j
+__device__ inline float c4_sum(h4_t w, float d, float xe, float xw, float xt, float xb, float bq)
+{
+    float acc = 0;
+    acc += (float)w.x * xe;
+    acc += (float)w.y * xw;
+    acc += (float)w.z * xt;
+    acc += (float)w.w * xb;
+    return bq - d * acc;
+}
 __device__ inline float c4_rhs(const Grid &g, const int I[3], const int K[3], int j, h4_t w, float d, const float *__restrict__ x,
                                float bq)
 {
@@ -472,12 +483,7 @@ __device__ inline float c4_rhs(const Grid &g, const int I[3], const int K[3], in
     const float xw = x[(long long)K[1] * g.plane + row0 + I[0]];
     const float xt = x[(long long)K[2] * g.plane + row0 + I[1]];
     const float xb = x[(long long)K[0] * g.plane + row0 + I[1]];
-    float acc = 0;
-    acc += (float)w.x * xe;
-    acc += (float)w.y * xw;
-    acc += (float)w.z * xt;
-    acc += (float)w.w * xb;
-    return bq - d * acc;
+    return c4_sum(w, d, xe, xw, xt, xb, bq);
 }
 
 // compact line right-hand sides of colour c, one thread per line point
@@ -762,7 +768,7 @@ __host__ __device__ inline bool zebra_fused_ok(const Grid &g)
     return (!g.pi || ((g.mx - 2) & 1) == 0) && (!g.pk || ((g.mz - 2) & 1) == 0);
 }
 
-template <int SLM>
+template <int SLM, bool XZ>
 __global__ void __launch_bounds__(LINE_LINES * LINE_MAXSEG) k_zebra_line_c4(Grid g, ZGrid z, int c, int nseg, int sl,
                                                                            const h4_t *__restrict__ sh,
                                                                            const float *__restrict__ sd,
@@ -797,7 +803,13 @@ __global__ void __launch_bounds__(LINE_LINES * LINE_MAXSEG) k_zebra_line_c4(Grid
         const int jj = j0 + v;
         if (act && jj < j1) {
             const long long o = jj * nh;
-            r[v] = c4_rhs(g, I, K, jj + 1, sh[cb + o], sd[cb + o], x, b[((long long)k * g.my + jj + 1) * g.mx + i]);
+            const float bq = b[((long long)k * g.my + jj + 1) * g.mx + i];
+            // XZ: the iterate is zero on entry (the first half-sweep of a
+            // V-cycle level): the same sum over zero neighbours, without reading them
+            if constexpr (XZ)
+                r[v] = c4_sum(sh[cb + o], sd[cb + o], 0.f, 0.f, 0.f, 0.f, bq);
+            else
+                r[v] = c4_rhs(g, I, K, jj + 1, sh[cb + o], sd[cb + o], x, bq);
             l[v] = lo[o];
             m[v] = mm[o];
             u[v] = cp[o];
@@ -900,6 +912,50 @@ __global__ void k_restrict(Grid gc, Grid gf, const unsigned char *__restrict__ r
             for (int di = 0; di < 2; di++) {
                 long long qf = gidx(gf, 2 * k - 1 + dk, 2 * j - 1 + dj, 2 * i - 1 + di);
                 if (rowf[qf]) s += rf[qf];
+            }
+    bc[t] = s;
+}
+
+// k_apply's residual r = b - A x (its first NQ stencil slots) at the eight
+// children of each coarse point, summed into the coarse b as k_restrict sums
+// them: the fine residual is never stored.  Each child's residual is formed
+// with k_apply's expression in its slot order and the children are added in
+// k_restrict's order, so bc has the same bits as the two-kernel form.  The
+// x values of the 4x4x4 fine block around the children come through the
+// per-axis canonical positions nbr_idx uses.
+template <typename T, int NQ>
+__global__ void __launch_bounds__(PTPB) k_residual_restrict(Grid gc, Grid gf, const T *__restrict__ coef,
+                                                            const unsigned char *__restrict__ rowc,
+                                                            const unsigned char *__restrict__ rowf, const T *__restrict__ x,
+                                                            const T *__restrict__ b, T *__restrict__ bc)
+{
+    long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (t >= gc.n) return;
+    if (!rowc[t]) { bc[t] = 0; return; }
+    const int t32 = (int)t; int i = t32 % gc.mx; int r = t32 / gc.mx; int j = r % gc.my, k = r / gc.my;
+    // fine positions 2c-2 .. 2c+1 per axis (children at 2c-1 and 2c)
+    int I[4], J[4], K[4];
+#pragma unroll
+    for (int d = 0; d < 4; d++) {
+        I[d] = canon(2 * i - 2 + d, gf.mx, gf.pi);
+        J[d] = canon(2 * j - 2 + d, gf.my, gf.pj);
+        K[d] = canon(2 * k - 2 + d, gf.mz, gf.pk);
+    }
+    T s = 0;
+#pragma unroll
+    for (int dk = 0; dk < 2; dk++)
+#pragma unroll
+        for (int dj = 0; dj < 2; dj++)
+#pragma unroll
+            for (int di = 0; di < 2; di++) {
+                const long long qf = gidx(gf, 2 * k - 1 + dk, 2 * j - 1 + dj, 2 * i - 1 + di);
+                if (!rowf[qf]) continue;
+                T a = 0;
+#pragma unroll
+                for (int q = 0; q < NQ; q++)
+                    a += coef[q * gf.n + qf] *
+                         x[((long long)K[dk + 1 + soff(q, 2)] * gf.my + J[dj + 1 + soff(q, 1)]) * gf.mx + I[di + 1 + soff(q, 0)]];
+                s += b[qf] - a;
             }
     bc[t] = s;
 }
@@ -1567,42 +1623,60 @@ struct LV<double> {
 
 constexpr int LINE_PREFETCH = 16;
 
+// Which line-relaxation form a level takes.
+struct LinePlan {
+    bool short_seg, scan, c4, fused;
+    int nseg, sl;
+};
 template <typename T>
-static void smooth(vfs_ctx *c, PLevel &L, int sweeps)
+static LinePlan line_plan(const PLevel &L)
 {
-    hipStream_t st = c->stream;
-    const long long lines = (long long)L.z.nkI * L.z.nh;
-    const bool fine = L.g.n == c->g.n;
     constexpr int SLM = line_slm<T>();
+    LinePlan p;
     // VFS_LINE_PATH forces one line-solve form (tests reach every form on
     // small grids): "short" the 8-point FP32 scan, "long" the SLM-point scan,
     // "solve" one thread per line; otherwise the line length chooses
     const char *lp = std::getenv("VFS_LINE_PATH");
     const int force = !lp ? 0 : lp[0] == 's' && lp[1] == 'h' ? 1 : lp[0] == 'l' ? 2 : lp[0] == 's' ? 3 : 0;
-    bool short_seg = sizeof(T) == 4 && L.z.njI <= LINE_SLM_SHORT * LINE_MAXSEG;
-    if (force == 1) short_seg = sizeof(T) == 4;
-    if (force == 2) short_seg = false;
-    const int slm = short_seg ? LINE_SLM_SHORT : SLM;
-    const int nseg = (L.z.njI + slm - 1) / slm;
-    const bool scan = nseg <= LINE_MAXSEG && force != 3;  // lines up to 32 * slm points
-    const int sl = (L.z.njI + nseg - 1) / nseg;
-    const bool c4 = sizeof(T) == 4 && L.compact;
-    const h4_t *sh4 = reinterpret_cast<const h4_t *>(L.sh16);
-    const unsigned scan_blocks = (unsigned)((lines + LINE_LINES - 1) / LINE_LINES);
+    p.short_seg = sizeof(T) == 4 && L.z.njI <= LINE_SLM_SHORT * LINE_MAXSEG;
+    if (force == 1) p.short_seg = sizeof(T) == 4;
+    if (force == 2) p.short_seg = false;
+    const int slm = p.short_seg ? LINE_SLM_SHORT : SLM;
+    p.nseg = (L.z.njI + slm - 1) / slm;
+    p.scan = p.nseg <= LINE_MAXSEG && force != 3;  // lines up to 32 * slm points
+    p.sl = (L.z.njI + p.nseg - 1) / p.nseg;
+    p.c4 = sizeof(T) == 4 && L.compact;
     // the compact form's right-hand sides are formed inside the line solve
     // (k_zebra_line_c4) unless VFS_ZEBRA_FUSED=0 asks for the two-pass form
     const char *fz = std::getenv("VFS_ZEBRA_FUSED");
-    const bool fused = c4 && scan && zebra_fused_ok(L.g) && !(fz && fz[0] == '0');
+    p.fused = p.c4 && p.scan && zebra_fused_ok(L.g) && !(fz && fz[0] == '0');
+    return p;
+}
+
+// sweeps of zebra line Gauss-Seidel on level L.  x_zero: the iterate is zero
+// on entry and need not be read by the first half-sweep (one-pass form only;
+// the caller then skips the zero fill of the rows, the ghosts being zero).
+template <typename T>
+static void smooth(vfs_ctx *c, PLevel &L, int sweeps, bool x_zero = false)
+{
+    hipStream_t st = c->stream;
+    const long long lines = (long long)L.z.nkI * L.z.nh;
+    const bool fine = L.g.n == c->g.n;
+    constexpr int SLM = line_slm<T>();
+    const LinePlan P = line_plan<T>(L);
+    const bool short_seg = P.short_seg, scan = P.scan, c4 = P.c4, fused = P.fused;
+    const int nseg = P.nseg, sl = P.sl;
+    const h4_t *sh4 = reinterpret_cast<const h4_t *>(L.sh16);
+    const unsigned scan_blocks = (unsigned)((lines + LINE_LINES - 1) / LINE_LINES);
     for (int s = 0; s < sweeps; s++)
         for (int col = 0; col < 2; col++) {
             if (fused) {
+                const bool xz = x_zero && s == 0 && col == 0;
                 if (fine) prof_begin(c, K_LRELAXC32);
-                if (short_seg)
-                    hipLaunchKernelGGL((k_zebra_line_c4<LINE_SLM_SHORT>), dim3(scan_blocks), dim3(LINE_LINES * nseg), 0, st, L.g,
-                                       L.z, col, nseg, sl, sh4, L.sd32, L.b32, L.sc32, L.x32);
-                else
-                    hipLaunchKernelGGL((k_zebra_line_c4<line_slm<float>()>), dim3(scan_blocks), dim3(LINE_LINES * nseg), 0, st,
-                                       L.g, L.z, col, nseg, sl, sh4, L.sd32, L.b32, L.sc32, L.x32);
+                auto k = short_seg ? (xz ? k_zebra_line_c4<LINE_SLM_SHORT, true> : k_zebra_line_c4<LINE_SLM_SHORT, false>)
+                                   : (xz ? k_zebra_line_c4<line_slm<float>(), true> : k_zebra_line_c4<line_slm<float>(), false>);
+                hipLaunchKernelGGL(k, dim3(scan_blocks), dim3(LINE_LINES * nseg), 0, st, L.g, L.z, col, nseg, sl, sh4, L.sd32,
+                                   L.b32, L.sc32, L.x32);
                 if (fine) prof_end(c);
                 continue;
             }
@@ -1637,6 +1711,22 @@ static void smooth(vfs_ctx *c, PLevel &L, int sweeps)
 template <typename T>
 static void vcycle(vfs_ctx *c, vfs_poisson *ps, int l, int pre, int post, int coarse);
 
+// A coarse level's iterate starts each V-cycle at zero.  When its first
+// smoothing half-sweep is the one-pass compact form it reads no x (x_zero)
+// and rewrites every interior point, and the coarsest dense solve rewrites
+// every row; the other points (ghosts, blanked points) are zero since
+// allocation and no kernel writes them, so the zero fill is then not needed.
+// Levels on other paths keep it.
+template <typename T>
+static bool coarse_zero_free(const vfs_poisson *ps, int l, int pre, int coarse)
+{
+    if (l < 1) return false;
+    const PLevel &L = ps->L[l];
+    const bool last = l == ps->nl - 1;
+    if (last && L.nr > 0) return true;  // dense coarsest level
+    return (last ? coarse : pre) >= 1 && line_plan<T>(L).fused;
+}
+
 // The coarse part of a V-cycle (zero x on level 1, then levels 1..nl-1) is a
 // fixed sequence of small kernels on fixed buffers: it is captured once into
 // a HIP graph per hierarchy and smoothing setting and replayed, which cuts the
@@ -1648,7 +1738,8 @@ static void coarse_subcycle(vfs_ctx *c, vfs_poisson *ps, int pre, int post, int 
     hipStream_t st = c->stream;
     PLevel &C = ps->L[1];
     auto direct = [&]() {
-        hipLaunchKernelGGL(k_zero<T>, dim3(blocks_for(C.g.n, PTPB)), dim3(PTPB), 0, st, C.g.n, LV<T>::x(C));
+        if (!coarse_zero_free<T>(ps, 1, pre, coarse))
+            hipLaunchKernelGGL(k_zero<T>, dim3(blocks_for(C.g.n, PTPB)), dim3(PTPB), 0, st, C.g.n, LV<T>::x(C));
         vcycle<T>(c, ps, 1, pre, post, coarse);
     };
     const char *env = std::getenv("VFS_MG_GRAPH");
@@ -1679,24 +1770,39 @@ static void vcycle(vfs_ctx *c, vfs_poisson *ps, int l, int pre, int post, int co
             hipLaunchKernelGGL(k_dense_solve<T>, dim3(blocks_for(L.nr, DENSE_ROWS)), dim3(DENSE_ROWS * DENSE_SPLIT), 0, st, L.nr, L.ridx,
                                LV<T>::inv(L), LV<T>::b(L), LV<T>::x(L));
         else
-            smooth<T>(c, L, coarse);
+            smooth<T>(c, L, coarse, coarse_zero_free<T>(ps, l, pre, coarse));
         return;
     }
-    smooth<T>(c, L, pre);
+    smooth<T>(c, L, pre, coarse_zero_free<T>(ps, l, pre, coarse));
     // a compact level's V-cycle works on the axis part of its operator: the
-    // residual it restricts takes the centre and the six axis neighbours
-    if (sizeof(T) == 4 && L.compact)
-        hipLaunchKernelGGL((k_apply<T, T, 7>), dim3(nb), dim3(PTPB), 0, st, L.g, LV<T>::coef(L), L.row, LV<T>::x(L), LV<T>::b(L),
-                           LV<T>::r(L));
-    else
-        hipLaunchKernelGGL(k_apply<T>, dim3(nb), dim3(PTPB), 0, st, L.g, LV<T>::coef(L), L.row, LV<T>::x(L), LV<T>::b(L),
-                           LV<T>::r(L));
+    // residual it restricts takes the centre and the six axis neighbours.
+    // The residual is formed inside the restriction (k_residual_restrict)
+    // unless VFS_MG_FUSED_RESTRICT=0 asks for the stored-residual form.
     PLevel &C = ps->L[l + 1];
-    hipLaunchKernelGGL(k_restrict<T>, dim3(blocks_for(C.g.n, PTPB)), dim3(PTPB), 0, st, C.g, L.g, C.row, L.row, LV<T>::r(L), LV<T>::b(C));
+    const unsigned nbc = blocks_for(C.g.n, PTPB);
+    const char *fr = std::getenv("VFS_MG_FUSED_RESTRICT");
+    const bool axis = sizeof(T) == 4 && L.compact;
+    if (!(fr && fr[0] == '0')) {
+        if (axis)
+            hipLaunchKernelGGL((k_residual_restrict<T, 7>), dim3(nbc), dim3(PTPB), 0, st, C.g, L.g, LV<T>::coef(L), C.row, L.row,
+                               (const T *)LV<T>::x(L), (const T *)LV<T>::b(L), LV<T>::b(C));
+        else
+            hipLaunchKernelGGL((k_residual_restrict<T, NST>), dim3(nbc), dim3(PTPB), 0, st, C.g, L.g, LV<T>::coef(L), C.row,
+                               L.row, (const T *)LV<T>::x(L), (const T *)LV<T>::b(L), LV<T>::b(C));
+    } else {
+        if (axis)
+            hipLaunchKernelGGL((k_apply<T, T, 7>), dim3(nb), dim3(PTPB), 0, st, L.g, LV<T>::coef(L), L.row, LV<T>::x(L),
+                               LV<T>::b(L), LV<T>::r(L));
+        else
+            hipLaunchKernelGGL(k_apply<T>, dim3(nb), dim3(PTPB), 0, st, L.g, LV<T>::coef(L), L.row, LV<T>::x(L), LV<T>::b(L),
+                               LV<T>::r(L));
+        hipLaunchKernelGGL(k_restrict<T>, dim3(nbc), dim3(PTPB), 0, st, C.g, L.g, C.row, L.row, LV<T>::r(L), LV<T>::b(C));
+    }
     if (l == 0) {
         coarse_subcycle<T>(c, ps, pre, post, coarse);
     } else {
-        hipLaunchKernelGGL(k_zero<T>, dim3(blocks_for(C.g.n, PTPB)), dim3(PTPB), 0, st, C.g.n, LV<T>::x(C));
+        if (!coarse_zero_free<T>(ps, l + 1, pre, coarse))
+            hipLaunchKernelGGL(k_zero<T>, dim3(blocks_for(C.g.n, PTPB)), dim3(PTPB), 0, st, C.g.n, LV<T>::x(C));
         vcycle<T>(c, ps, l + 1, pre, post, coarse);
     }
     hipLaunchKernelGGL(k_prolong<T>, dim3(nb), dim3(PTPB), 0, st, C.g, L.g, L.row, LV<T>::x(C), LV<T>::x(L));
