@@ -157,6 +157,10 @@ typedef struct {
                             HYPRE BoomerAMG, poisson_hypre.c:34-97) in place
                             of the geometric V-cycle; FP64, mg_fp32 ignored   */
     double amg_thresh;   /* -amg_thresh: strength threshold (0.75)           */
+    int deep_level;      /* first V-cycle level that smooths deep_pre and
+                            deep_post sweeps instead of pre and post;
+                            0 (the default): every level takes pre and post */
+    int deep_pre, deep_post;
 } vfs_solve_opts;
 
 typedef struct {

@@ -186,3 +186,30 @@ def test_cross_strength_and_choice(monkeypatch, capfd, amp, form):
     assert abs(float(xs) - ref) <= 1e-6 + 1e-5 * ref
     assert got == form
     assert (ref <= 0.01) == (form == "compact")
+
+
+def test_deep_level_sweeps(monkeypatch):
+    """deep_level / deep_pre / deep_post: the levels from deep_level on smooth
+    their own sweep counts.  In one context a default solve, a solve with
+    1+2 sweeps from level 1 (the captured coarse levels are recaptured for
+    it) and the default again: every solve meets the tolerance with the
+    oracle's operator, the deep variant gives a different iterate, and the
+    third solve repeats the first bit for bit."""
+    c, o, ctx = channel_pair(40, 32, 40, les=0)
+    try:
+        o.poisson_rhs()
+        ctx.poisson_rhs()
+        b = o["b"]
+        out = []
+        for deep in (0, 1, 0):
+            ctx.upload("PHI", np.zeros(o.shape))
+            st = ctx.poisson_solve(vfs.default_solve_opts(tol=1e-10, use_guess=0, maxit=400, deep_level=deep,
+                                                          deep_pre=1, deep_post=2))
+            phi = ctx.download("PHI")
+            assert st.relres <= 1e-10
+            assert np.linalg.norm(b - o.poisson_apply(phi)) / np.linalg.norm(b) <= 1.01e-10
+            out.append((st.iterations, phi))
+        assert out[0][0] == out[2][0] and np.array_equal(out[0][1], out[2][1])
+        assert not np.array_equal(out[0][1], out[1][1])
+    finally:
+        ctx.close()

@@ -57,6 +57,7 @@ class SolveOpts(C.Structure):
         ("tol", C.c_double), ("restart", C.c_int), ("maxit", C.c_int), ("levels", C.c_int),
         ("pre", C.c_int), ("post", C.c_int), ("coarse_sweeps", C.c_int),
         ("mg_fp32", C.c_int), ("use_guess", C.c_int), ("amg", C.c_int), ("amg_thresh", C.c_double),
+        ("deep_level", C.c_int), ("deep_pre", C.c_int), ("deep_post", C.c_int),
     ]
 
 
@@ -435,11 +436,12 @@ def make_params(mx, my, mz, *, per=(1, 0, 1), bctype=(100, 100, 1, 1, 100, 100),
 
 
 def default_solve_opts(tol=1e-8, restart=20, maxit=200, levels=6, pre=2, post=3, coarse_sweeps=20, mg_fp32=1,
-                       use_guess=1, amg=0, amg_thresh=0.75) -> SolveOpts:
+                       use_guess=1, amg=0, amg_thresh=0.75, deep_level=0, deep_pre=1, deep_post=2) -> SolveOpts:
     o = SolveOpts()
     o.tol, o.restart, o.maxit, o.levels = tol, restart, maxit, levels
     o.pre, o.post, o.coarse_sweeps, o.mg_fp32, o.use_guess = pre, post, coarse_sweeps, mg_fp32, use_guess
     o.amg, o.amg_thresh = amg, amg_thresh
+    o.deep_level, o.deep_pre, o.deep_post = deep_level, deep_pre, deep_post
     return o
 
 

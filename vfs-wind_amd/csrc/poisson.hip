@@ -1283,9 +1283,23 @@ struct PLevel {
 
 // a captured coarse sub-cycle (levels 1..nl-1 of one V-cycle) and the
 // smoothing counts it was captured with
+// Smoothing sweeps of a V-cycle: pre and post on the levels below
+// deep_level, deep_pre and deep_post from deep_level on (deep_level 0: every
+// level takes pre and post), coarse on a coarsest level without a dense solve.
+struct Sweeps {
+    int pre, post, coarse, deep_level, deep_pre, deep_post;
+    int pre_at(int l) const { return deep_level > 0 && l >= deep_level ? deep_pre : pre; }
+    int post_at(int l) const { return deep_level > 0 && l >= deep_level ? deep_post : post; }
+    bool operator==(const Sweeps &o) const
+    {
+        return pre == o.pre && post == o.post && coarse == o.coarse && deep_level == o.deep_level && deep_pre == o.deep_pre &&
+               deep_post == o.deep_post;
+    }
+};
+
 struct SubGraph {
     hipGraphExec_t exec = nullptr;
-    int pre = -1, post = -1, coarse = -1;
+    Sweeps key{-1, -1, -1, -1, -1, -1};
 };
 
 namespace vfs {
@@ -1709,7 +1723,7 @@ static void smooth(vfs_ctx *c, PLevel &L, int sweeps, bool x_zero = false)
 }
 
 template <typename T>
-static void vcycle(vfs_ctx *c, vfs_poisson *ps, int l, int pre, int post, int coarse);
+static void vcycle(vfs_ctx *c, vfs_poisson *ps, int l, const Sweeps &S);
 
 // A coarse level's iterate starts each V-cycle at zero.  When its first
 // smoothing half-sweep is the one-pass compact form it reads no x (x_zero)
@@ -1718,13 +1732,13 @@ static void vcycle(vfs_ctx *c, vfs_poisson *ps, int l, int pre, int post, int co
 // allocation and no kernel writes them, so the zero fill is then not needed.
 // Levels on other paths keep it.
 template <typename T>
-static bool coarse_zero_free(const vfs_poisson *ps, int l, int pre, int coarse)
+static bool coarse_zero_free(const vfs_poisson *ps, int l, const Sweeps &S)
 {
     if (l < 1) return false;
     const PLevel &L = ps->L[l];
     const bool last = l == ps->nl - 1;
     if (last && L.nr > 0) return true;  // dense coarsest level
-    return (last ? coarse : pre) >= 1 && line_plan<T>(L).fused;
+    return (last ? S.coarse : S.pre_at(l)) >= 1 && line_plan<T>(L).fused;
 }
 
 // The coarse part of a V-cycle (zero x on level 1, then levels 1..nl-1) is a
@@ -1733,19 +1747,19 @@ static bool coarse_zero_free(const vfs_poisson *ps, int l, int pre, int coarse)
 // per-launch gaps of the ~4 us coarse-level kernels.  It holds kernel nodes
 // only.  VFS_MG_GRAPH=0 launches the same kernels directly.
 template <typename T>
-static void coarse_subcycle(vfs_ctx *c, vfs_poisson *ps, int pre, int post, int coarse)
+static void coarse_subcycle(vfs_ctx *c, vfs_poisson *ps, const Sweeps &S)
 {
     hipStream_t st = c->stream;
     PLevel &C = ps->L[1];
     auto direct = [&]() {
-        if (!coarse_zero_free<T>(ps, 1, pre, coarse))
+        if (!coarse_zero_free<T>(ps, 1, S))
             hipLaunchKernelGGL(k_zero<T>, dim3(blocks_for(C.g.n, PTPB)), dim3(PTPB), 0, st, C.g.n, LV<T>::x(C));
-        vcycle<T>(c, ps, 1, pre, post, coarse);
+        vcycle<T>(c, ps, 1, S);
     };
     const char *env = std::getenv("VFS_MG_GRAPH");
     if (env && env[0] == '0') { direct(); return; }
     SubGraph &sg = ps->sub[sizeof(T) == 4 ? 0 : 1];
-    if (!sg.exec || sg.pre != pre || sg.post != post || sg.coarse != coarse) {
+    if (!sg.exec || !(sg.key == S)) {
         if (sg.exec) { hipGraphExecDestroy(sg.exec); sg.exec = nullptr; }
         hipGraph_t graph = nullptr;
         if (hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) != hipSuccess) { direct(); return; }
@@ -1754,26 +1768,27 @@ static void coarse_subcycle(vfs_ctx *c, vfs_poisson *ps, int pre, int post, int 
         hipError_t e = hipGraphInstantiate(&sg.exec, graph, nullptr, nullptr, 0);
         hipGraphDestroy(graph);
         if (e != hipSuccess) { sg.exec = nullptr; direct(); return; }
-        sg.pre = pre; sg.post = post; sg.coarse = coarse;
+        sg.key = S;
     }
     hipGraphLaunch(sg.exec, st);
 }
 
 template <typename T>
-static void vcycle(vfs_ctx *c, vfs_poisson *ps, int l, int pre, int post, int coarse)
+static void vcycle(vfs_ctx *c, vfs_poisson *ps, int l, const Sweeps &S)
 {
     hipStream_t st = c->stream;
     PLevel &L = ps->L[l];
+    const int pre = S.pre_at(l), post = S.post_at(l);
     const unsigned nb = blocks_for(L.g.n, PTPB);
     if (l == ps->nl - 1) {
         if (L.nr > 0)
             hipLaunchKernelGGL(k_dense_solve<T>, dim3(blocks_for(L.nr, DENSE_ROWS)), dim3(DENSE_ROWS * DENSE_SPLIT), 0, st, L.nr, L.ridx,
                                LV<T>::inv(L), LV<T>::b(L), LV<T>::x(L));
         else
-            smooth<T>(c, L, coarse, coarse_zero_free<T>(ps, l, pre, coarse));
+            smooth<T>(c, L, S.coarse, coarse_zero_free<T>(ps, l, S));
         return;
     }
-    smooth<T>(c, L, pre, coarse_zero_free<T>(ps, l, pre, coarse));
+    smooth<T>(c, L, pre, coarse_zero_free<T>(ps, l, S));
     // a compact level's V-cycle works on the axis part of its operator: the
     // residual it restricts takes the centre and the six axis neighbours.
     // The residual is formed inside the restriction (k_residual_restrict)
@@ -1799,11 +1814,11 @@ static void vcycle(vfs_ctx *c, vfs_poisson *ps, int l, int pre, int post, int co
         hipLaunchKernelGGL(k_restrict<T>, dim3(nbc), dim3(PTPB), 0, st, C.g, L.g, C.row, L.row, LV<T>::r(L), LV<T>::b(C));
     }
     if (l == 0) {
-        coarse_subcycle<T>(c, ps, pre, post, coarse);
+        coarse_subcycle<T>(c, ps, S);
     } else {
-        if (!coarse_zero_free<T>(ps, l + 1, pre, coarse))
+        if (!coarse_zero_free<T>(ps, l + 1, S))
             hipLaunchKernelGGL(k_zero<T>, dim3(blocks_for(C.g.n, PTPB)), dim3(PTPB), 0, st, C.g.n, LV<T>::x(C));
-        vcycle<T>(c, ps, l + 1, pre, post, coarse);
+        vcycle<T>(c, ps, l + 1, S);
     }
     hipLaunchKernelGGL(k_prolong<T>, dim3(nb), dim3(PTPB), 0, st, C.g, L.g, L.row, LV<T>::x(C), LV<T>::x(L));
     smooth<T>(c, L, post);
@@ -1856,6 +1871,11 @@ static int amg_ensure(vfs_ctx *c, const vfs_solve_opts *o)
     return amg_build(c64.data(), row.data(), n, amg_nbr, &L.g, theta, &ps->amg);
 }
 
+static Sweeps sweeps_of(const vfs_solve_opts *o)
+{
+    return Sweeps{o->pre, o->post, o->coarse_sweeps, o->deep_level, o->deep_pre, o->deep_post};
+}
+
 static int precondition(vfs_ctx *c, const vfs_solve_opts *o, const double *v, double *z, float *z32 = nullptr,
                         bool b32_ready = false)
 {
@@ -1870,7 +1890,7 @@ static int precondition(vfs_ctx *c, const vfs_solve_opts *o, const double *v, do
         float *x0 = L.x32;
         if (z32) L.x32 = z32;
         hipLaunchKernelGGL(k_zero<float>, dim3(nb), dim3(PTPB), 0, st, n, L.x32);
-        vcycle<float>(c, ps, 0, o->pre, o->post, o->coarse_sweeps);
+        vcycle<float>(c, ps, 0, sweeps_of(o));
         if (!z32) hipLaunchKernelGGL((k_convert<float, double>), dim3(nb), dim3(PTPB), 0, st, n, L.x32, z);
         L.x32 = x0;
     } else {
@@ -1878,7 +1898,7 @@ static int precondition(vfs_ctx *c, const vfs_solve_opts *o, const double *v, do
         double *x0 = L.x64;
         L.x64 = z;
         hipLaunchKernelGGL(k_zero<double>, dim3(nb), dim3(PTPB), 0, st, n, L.x64);
-        vcycle<double>(c, ps, 0, o->pre, o->post, o->coarse_sweeps);
+        vcycle<double>(c, ps, 0, sweeps_of(o));
         L.x64 = x0;
     }
     VFS_LAUNCH_CHECK("v-cycle");
