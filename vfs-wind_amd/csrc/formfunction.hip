@@ -434,6 +434,10 @@ constexpr int FX_J = FT_J + 1;
 static_assert(FT_I == 32, "a row of the tile is half a wavefront (the i-face exchange shuffles within it)");
 constexpr int HX_T0 = 64;  // first thread of the second wavefront
 static_assert(FT_I * FT_J >= HX_T0 + FT_J && FT_J * 2 <= 64, "halo lanes of the i faces");
+inline unsigned face_fp_blocks(const Grid &g)
+{
+    return (unsigned)(((g.mx - 2 + FT_I - 1) / FT_I) * ((g.my - 2 + FT_J - 1) / FT_J) * ((g.mz - 2 + FT_K - 1) / FT_K));
+}
 template <bool NB>
 __global__ void __launch_bounds__(FT_I * FT_J) __attribute__((amdgpu_waves_per_eu(2))) k_face_fp(FaceArgs a, double *__restrict__ fp)
 {
@@ -443,13 +447,19 @@ __global__ void __launch_bounds__(FT_I * FT_J) __attribute__((amdgpu_waves_per_e
     __shared__ double fxj[6][FX_J][FT_I];  // j-face div xyz, visc xyz; row 0 = face j0-1
     __shared__ double hx[6][FT_J];         // i-face div xyz, visc xyz at i0-1 of each row
     const Grid &g = a.g;
-    const int nbi = (g.mx + FT_I - 1) / FT_I, nbj = (g.my + FT_J - 1) / FT_J;
-    int b = blockIdx.x;
+    // The tiles cover the interior points (1 .. m-2 on each axis; face_fp_blocks
+    // blocks): the faces at index 0 are the halo faces of the first tiles, and
+    // the ghost points' projections are not written (zero at the non-periodic
+    // ghosts since allocation, copies at the periodic ones by the periodic fix
+    // that follows).  Tiles are taken in XCD order, so the tiles that share a
+    // halo run on one XCD's L2.
+    const int nbi = (g.mx - 2 + FT_I - 1) / FT_I, nbj = (g.my - 2 + FT_J - 1) / FT_J;
+    int b = (int)xcd_block1();
     const int bi = b % nbi;
     b /= nbi;
     const int bj = b % nbj, bk = b / nbj;
-    const int i0 = bi * FT_I, j0 = bj * FT_J, k0 = bk * FT_K;
-    const int k1 = k0 + FT_K < g.mz ? k0 + FT_K : g.mz;
+    const int i0 = 1 + bi * FT_I, j0 = 1 + bj * FT_J, k0 = 1 + bk * FT_K;
+    const int k1 = k0 + FT_K < g.mz - 1 ? k0 + FT_K : g.mz - 1;
     const int tx = threadIdx.x % FT_I, ty = threadIdx.x / FT_I;
     const int i = i0 + tx, j = j0 + ty;
     const bool inb = i < g.mx && j < g.my;
@@ -492,7 +502,7 @@ __global__ void __launch_bounds__(FT_I * FT_J) __attribute__((amdgpu_waves_per_e
             }
         }
     };
-    const int kstart = k0 > 0 ? k0 - 1 : k0;
+    const int kstart = k0 - 1;
     fetch_plane(kstart - 1);
     store_plane(0);
     fetch_plane(kstart);
@@ -841,20 +851,22 @@ __global__ void __launch_bounds__(FT_I * FT_J) k_cell_rhs(CellArgs a, const doub
 // k_cell_rhs on the projections written by k_face_fp: pr[0..2] planes hold
 // csi.Fp, eta.Fp, zet.Fp of every cell, so the contravariant projection
 // reads no cell metrics (the same products, hence the same bits).
-// The cell pass needs no LDS tile; rows of 64 points (1.5 KB of each AoS
-// array) leave fewer partly written cache lines at block edges than the
-// face pass's 32-point rows (measured: 0.372 -> 0.344 ms at 256x192x256).
-constexpr int CR_I = 64, CR_J = 4;
-__global__ void __launch_bounds__(CR_I * CR_J) k_cell_rhs_pr(CellArgs a, const double *__restrict__ pr)
+// The cell pass needs no LDS tile: a block takes CR_T consecutive points of
+// a k plane (the plane flattened, so 6 KB of each AoS array per block and no
+// block covers only the last few points of a row) and marches FT_K planes.
+constexpr int CR_T = 256;
+inline unsigned cell_rhs_blocks(const Grid &g)
+{
+    return (unsigned)(((g.plane + CR_T - 1) / CR_T) * ((g.mz + FT_K - 1) / FT_K));
+}
+__global__ void __launch_bounds__(CR_T) k_cell_rhs_pr(CellArgs a, const double *__restrict__ pr)
 {
     const Grid &g = a.g;
-    const int nbi = (g.mx + CR_I - 1) / CR_I, nbj = (g.my + CR_J - 1) / CR_J;
-    int b = blockIdx.x;
-    const int bi = b % nbi;
-    b /= nbi;
-    const int bj = b % nbj, bk = b / nbj;
-    const int i = bi * CR_I + threadIdx.x % CR_I, j = bj * CR_J + threadIdx.x / CR_I, k0 = bk * FT_K;
-    if (i >= g.mx || j >= g.my) return;
+    const long long nbp = (g.plane + CR_T - 1) / CR_T;
+    const long long bp = blockIdx.x % nbp, bk = blockIdx.x / nbp;
+    const long long pp = bp * CR_T + threadIdx.x;
+    if (pp >= g.plane) return;
+    const int i = (int)(pp % g.mx), j = (int)(pp / g.mx), k0 = (int)bk * FT_K;
     const int k1 = k0 + FT_K < g.mz ? k0 + FT_K : g.mz;
     const long long n = g.n;
     const int mx = g.mx, my = g.my, mz = g.mz;
@@ -973,9 +985,9 @@ int formfunction2(vfs_ctx *c, double scale, const double *pucont, double rk_alph
     }
     prof_begin(c, K_FACE);
     if (fused && nb)
-        hipLaunchKernelGGL(k_face_fp<true>, dim3(nbt), dim3(FT_I * FT_J), 0, st, fa, c->fpc);
+        hipLaunchKernelGGL(k_face_fp<true>, dim3(face_fp_blocks(g)), dim3(FT_I * FT_J), 0, st, fa, c->fpc);
     else if (fused)
-        hipLaunchKernelGGL(k_face_fp<false>, dim3(nbt), dim3(FT_I * FT_J), 0, st, fa, c->fpc);
+        hipLaunchKernelGGL(k_face_fp<false>, dim3(face_fp_blocks(g)), dim3(FT_I * FT_J), 0, st, fa, c->fpc);
     else if (fa.second_order && c->face_derived)
         hipLaunchKernelGGL((k_face_fluxes<true, true>), dim3(nbt), dim3(FT_I * FT_J), 0, st, fa);
     else if (fa.second_order)
@@ -1021,9 +1033,7 @@ int formfunction2(vfs_ctx *c, double scale, const double *pucont, double rk_alph
     prof_begin(c, K_CELL);
     if (fused && nb) ca.nvert = nullptr;  // the cell pass then skips its blanking tests
     if (fused)
-        hipLaunchKernelGGL(k_cell_rhs_pr,
-                           dim3((unsigned)(((g.mx + CR_I - 1) / CR_I) * ((g.my + CR_J - 1) / CR_J) * ((g.mz + FT_K - 1) / FT_K))),
-                           dim3(CR_I * CR_J), 0, st, ca, (const double *)c->fpc);
+        hipLaunchKernelGGL(k_cell_rhs_pr, dim3(cell_rhs_blocks(g)), dim3(CR_T), 0, st, ca, (const double *)c->fpc);
     else
         hipLaunchKernelGGL(k_cell_rhs, dim3(nbt), dim3(FT_I * FT_J), 0, st, ca, (const double *)c->fpc);
     prof_end(c);
