@@ -213,7 +213,8 @@ typedef struct {
 
 /* FormFunction_SNES (momentum.c:2237-2336), time_coeff() == 1:
  * r = -(u - Ucont_o)/dt + 0.5 F(u) + 0.5 RHS_o - dP (+ F_eul); device arrays of
- * 3 doubles per point.  Leaves VFS_UCONT/VFS_UCAT at the wall-zeroed u. */
+ * 3 doubles per point.  Leaves VFS_UCONT/VFS_UCAT at the wall-zeroed u; VFS_RHS
+ * is scratch (unspecified afterwards). */
 int vfs_formfunction_snes(vfs_ctx *ctx, const double *u_dev, double *r_dev, int use_feul);
 /* Ucont_o = Ucont and RHS_o = Formfunction_2(Ucont) (solvers.c:627-628). */
 int vfs_store_old_state(vfs_ctx *ctx);

@@ -61,6 +61,33 @@ def test_snes_residual(les):
     ctx.close()
 
 
+@pytest.mark.parametrize("feul", [0, 1], ids=["no-feul", "feul"])
+def test_snes_residual_cell_pass_bitwise(monkeypatch, feul):
+    """The residual formed inside Formfunction_2's cell pass (the default on
+    the fused path) against the three separate passes (VFS_FUSED_CN=0): the
+    same operations in the same order, so the same bits, with and without a
+    body force."""
+    c, o, ctx, uo, rhso = _state(1)
+    rng = np.random.default_rng(9)
+    if feul:
+        ctx.upload("FEUL", 1e-2 * rng.standard_normal(uo.shape))
+    u = uo + 1e-3 * rng.standard_normal(uo.shape) * (np.abs(uo) + 1e-3)
+    out = []
+    for env in ("1", "0"):
+        monkeypatch.setenv("VFS_FUSED_CN", env)
+        u_d = _dev(u)
+        r_d = torch.zeros_like(u_d)
+        ctx.formfunction_snes(u_d.data_ptr(), r_d.data_ptr(), feul)
+        ctx.sync()
+        out.append((r_d.cpu().numpy(), ctx.download("UCONT"), ctx.download("UCAT")))
+    monkeypatch.delenv("VFS_FUSED_CN")
+    for a, b in zip(out[0], out[1]):
+        assert np.array_equal(a, b)
+    if not feul:
+        assert relerr(out[0][0], o.formfunction_snes(u, uo, rhso)) <= 1e-12
+    ctx.close()
+
+
 def test_implicit_solve_root():
     c, o, ctx, uo, rhso = _state(1)
     tol = 1e-6

@@ -647,6 +647,11 @@ struct CellArgs {
     const double *pucont; // RK mode: pUcont
     double alpha;         // RK mode: alfa * dt * st
     double *ucont_out;    // RK mode: Ucont
+    // CN mode (cn_r != nullptr; k_cell_rhs_pr only): FormFunction_SNES's
+    // residual r = (0 - u/dt + u_o/dt) + F + 0.5 RHS_o - dP (+ F_eul)
+    const double *cn_u, *cn_uo, *cn_rhso, *cn_feul;
+    double cn_rdt;
+    double *cn_r;
 };
 
 __device__ inline double Fv(const CellArgs &a, int plane, int k, int j, int i)
@@ -841,6 +846,33 @@ __global__ void __launch_bounds__(FT_I * FT_J) k_cell_rhs(CellArgs a, const doub
             double *__restrict__ out = a.ucont_out + 3 * q;
 #pragma unroll
             for (int c = 0; c < 3; c++) out[c] = o[c];
+        } else if (a.cn_r) {
+            // Rhs = 0; Rhs += (-1/dt) u; Rhs += (1/dt) u_o; Rhs += 0.5 F (the
+            // cell's r); R = Rhs + 0.5 RHS_o - dP (+ F_eul), in the separate
+            // passes' operation order (momentum.c:2237-2336)
+            const double *__restrict__ u = a.cn_u + 3 * q;
+            const double *__restrict__ uo = a.cn_uo + 3 * q;
+            const double *__restrict__ ro = a.cn_rhso + 3 * q;
+            const double *__restrict__ dp = a.dp + 3 * q;
+            double uu[3], uoo[3], roo[3], d[3], fe[3] = {0, 0, 0}, o[3];
+#pragma unroll
+            for (int c = 0; c < 3; c++) { uu[c] = u[c]; uoo[c] = uo[c]; roo[c] = ro[c]; d[c] = dp[c]; }
+            if (a.cn_feul)
+#pragma unroll
+                for (int c = 0; c < 3; c++) fe[c] = a.cn_feul[3 * q + c];
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                const double s0 = 0.0 + (-a.cn_rdt) * uu[c];
+                const double base = s0 + a.cn_rdt * uoo[c];
+                const double rv = zero[c] ? 0.0 : base + r[c];
+                double s = rv + 0.5 * roo[c];
+                s = s + (-1.0) * d[c];
+                if (a.cn_feul) s = s + fe[c];
+                o[c] = s;
+            }
+            double *__restrict__ out = a.cn_r + 3 * q;
+#pragma unroll
+            for (int c = 0; c < 3; c++) out[c] = o[c];
         } else {
             for (int c = 0; c < 3; c++) a.rhs[3 * q + c] = zero[c] ? 0.0 : a.rhs[3 * q + c] + r[c];
         }
@@ -916,6 +948,33 @@ __global__ void __launch_bounds__(CR_T) k_cell_rhs_pr(CellArgs a, const double *
             double *__restrict__ out = a.ucont_out + 3 * q;
 #pragma unroll
             for (int c = 0; c < 3; c++) out[c] = o[c];
+        } else if (a.cn_r) {
+            // Rhs = 0; Rhs += (-1/dt) u; Rhs += (1/dt) u_o; Rhs += 0.5 F (the
+            // cell's r); R = Rhs + 0.5 RHS_o - dP (+ F_eul), in the separate
+            // passes' operation order (momentum.c:2237-2336)
+            const double *__restrict__ u = a.cn_u + 3 * q;
+            const double *__restrict__ uo = a.cn_uo + 3 * q;
+            const double *__restrict__ ro = a.cn_rhso + 3 * q;
+            const double *__restrict__ dp = a.dp + 3 * q;
+            double uu[3], uoo[3], roo[3], d[3], fe[3] = {0, 0, 0}, o[3];
+#pragma unroll
+            for (int c = 0; c < 3; c++) { uu[c] = u[c]; uoo[c] = uo[c]; roo[c] = ro[c]; d[c] = dp[c]; }
+            if (a.cn_feul)
+#pragma unroll
+                for (int c = 0; c < 3; c++) fe[c] = a.cn_feul[3 * q + c];
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                const double s0 = 0.0 + (-a.cn_rdt) * uu[c];
+                const double base = s0 + a.cn_rdt * uoo[c];
+                const double rv = zero[c] ? 0.0 : base + r[c];
+                double s = rv + 0.5 * roo[c];
+                s = s + (-1.0) * d[c];
+                if (a.cn_feul) s = s + fe[c];
+                o[c] = s;
+            }
+            double *__restrict__ out = a.cn_r + 3 * q;
+#pragma unroll
+            for (int c = 0; c < 3; c++) out[c] = o[c];
         } else {
             for (int c = 0; c < 3; c++) a.rhs[3 * q + c] = zero[c] ? 0.0 : a.rhs[3 * q + c] + r[c];
         }
@@ -949,7 +1008,14 @@ int nvert_all_zero(vfs_ctx *c, bool *zero)
     return VFS_OK;
 }
 
-int formfunction2(vfs_ctx *c, double scale, const double *pucont, double rk_alpha, double *ucont_out)
+bool formfunction2_fused(const vfs_ctx *c)
+{
+    // VFS_FUSED_FP=0 keeps the flux planes (the face pass, then k_cell_fp)
+    const char *env = std::getenv("VFS_FUSED_FP");
+    return c->p.second_order && c->face_derived && !(env && env[0] == '0');
+}
+
+int formfunction2(vfs_ctx *c, double scale, const double *pucont, double rk_alpha, double *ucont_out, const CnResidual *cn)
 {
     const Grid &g = c->g;
     hipStream_t st = c->stream;
@@ -974,9 +1040,8 @@ int formfunction2(vfs_ctx *c, double scale, const double *pucont, double rk_alph
     const unsigned nbt = (unsigned)(((g.mx + FT_I - 1) / FT_I) * ((g.my + FT_J - 1) / FT_J) * ((g.mz + FT_K - 1) / FT_K));
     const long long np[3] = {2LL * g.my * g.mz, 2LL * g.mx * g.mz, 2LL * g.mx * g.my};
     const int per[3] = {g.pi, g.pj, g.pk};
-    // VFS_FUSED_FP=0 keeps the flux planes (the face pass, then k_cell_fp)
-    const char *env = std::getenv("VFS_FUSED_FP");
-    const bool fused = fa.second_order && c->face_derived && !(env && env[0] == '0');
+    const bool fused = formfunction2_fused(c);
+    if (cn && !fused) return set_error(VFS_EINVAL, "formfunction2: the CN residual needs the fused passes");
     // a grid with no blanked point takes the face pass without blanking tests
     bool nb = false;
     if (fused) {
@@ -1022,6 +1087,12 @@ int formfunction2(vfs_ctx *c, double scale, const double *pucont, double rk_alph
     ca.pucont = pucont;
     ca.alpha = rk_alpha;
     ca.ucont_out = ucont_out;
+    ca.cn_u = c->f[VFS_UCONT];
+    ca.cn_uo = cn ? cn->uo : nullptr;
+    ca.cn_rhso = cn ? cn->rhso : nullptr;
+    ca.cn_feul = cn ? cn->feul : nullptr;
+    ca.cn_rdt = cn ? cn->rdt : 0.0;
+    ca.cn_r = cn ? cn->r : nullptr;
     if (!fused) {
         prof_begin(c, K_CELLFP);
         if (ca.second_order)

@@ -20,6 +20,7 @@
 
 #include "vfs_internal.h"
 #include "vfs_reduce.h"
+#include "vfs_krylov.h"
 
 #pragma clang fp contract(off)
 
@@ -123,53 +124,8 @@ __global__ void k_imp_post(Grid g, const double *__restrict__ eta, const double 
     }
 }
 
-// ---- device vector algebra (deterministic reductions) ----------------------
-// dots[v] = <w, V_v> (v < nv), dots[nv] = <w, w>
-__global__ void __launch_bounds__(RED_THREADS) k_im_multidot(long long n, int nv, const double *__restrict__ V, long long ld,
-                                                             const double *__restrict__ w, double *__restrict__ part)
-{
-    double acc[IM_MAXV - 1];
-#pragma unroll
-    for (int v = 0; v < IM_MAXV - 1; v++) acc[v] = 0;
-    double ww = 0;
-    for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < n; t += (long long)gridDim.x * blockDim.x) {
-        const double wt = w[t];
-#pragma unroll
-        for (int v = 0; v < IM_MAXV - 1; v++)
-            if (v < nv) acc[v] += V[v * ld + t] * wt;
-        ww += wt * wt;
-    }
-#pragma unroll
-    for (int v = 0; v < IM_MAXV - 1; v++)
-        if (v < nv) {
-            const double s = block_reduce<OpSum>(acc[v]);
-            if (threadIdx.x == 0) part[v * RED_BLOCKS + blockIdx.x] = s;
-        }
-    const double s = block_reduce<OpSum>(ww);
-    if (threadIdx.x == 0) part[nv * RED_BLOCKS + blockIdx.x] = s;
-}
-
-__global__ void k_im_finish(int nv, const double *__restrict__ part, double *__restrict__ out)
-{
-    const int v = blockIdx.x;
-    if (v > nv) return;
-    double s = 0;
-    for (int q = threadIdx.x; q < RED_BLOCKS; q += blockDim.x) s += part[v * RED_BLOCKS + q];
-    s = block_reduce<OpSum>(s);
-    if (threadIdx.x == 0) out[v] = s;
-}
-
-// w = w - sum_v h[v] V_v
-__global__ void k_im_multiaxpy(long long n, int nv, const double *__restrict__ h, const double *__restrict__ V, long long ld,
-                               double *__restrict__ w)
-{
-    long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-    if (t >= n) return;
-    double s = w[t];
-    for (int v = 0; v < nv; v++) s -= h[v] * V[v * ld + t];
-    w[t] = s;
-}
-
+// ---- device vector algebra (deterministic reductions; the Gram-Schmidt
+// sweeps are vfs_krylov.h's) --------------------------------------------------
 // y = a x + b y, coefficients from device memory (a = s[ia], b = s[ib] or 0/1)
 __global__ void k_im_axpby(long long n, double a, const double *__restrict__ x, double b, double *__restrict__ y)
 {
@@ -185,6 +141,17 @@ __global__ void k_im_waxpy(long long n, const double *__restrict__ x, double a, 
     long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
     if (t >= n) return;
     z[t] = x[t] + a * y[t];
+}
+
+// z = z2 = x + a y (the line search's trial state and the residual's input)
+__global__ void k_im_waxpy2(long long n, const double *__restrict__ x, double a, const double *__restrict__ y,
+                            double *__restrict__ z, double *__restrict__ z2)
+{
+    long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const double v = x[t] + a * y[t];
+    z[t] = v;
+    z2[t] = v;
 }
 
 // sum of |x| (for the differencing parameter)
@@ -259,6 +226,13 @@ int snes_function(vfs_ctx *c, const double *u, double *r, int use_feul)
     VFS_LAUNCH_CHECK("snes bc");
     int rc = c2c(c);
     if (rc) return rc;
+    // on the fused path the residual is formed in the cell pass (the same
+    // operations in the same order); VFS_FUSED_CN=0 keeps the three passes
+    const char *fe = std::getenv("VFS_FUSED_CN");
+    if (formfunction2_fused(c) && !(fe && fe[0] == '0')) {
+        const CnResidual cn{c->f[VFS_UCONT_O], c->f[VFS_RHS_O], use_feul ? c->f[VFS_FEUL] : nullptr, 1. / c->p.dt, r};
+        return formfunction2(c, 0.5, nullptr, 0.0, nullptr, &cn);
+    }
     hipLaunchKernelGGL(k_cn_base, dim3(blocks_for(n3, IM_TPB)), dim3(IM_TPB), 0, st, n3, 1. / c->p.dt, c->f[VFS_UCONT],
                        c->f[VFS_UCONT_O], c->f[VFS_RHS]);
     rc = formfunction2(c, 0.5, nullptr, 0.0, nullptr);
@@ -269,17 +243,44 @@ int snes_function(vfs_ctx *c, const double *u, double *r, int use_feul)
     return VFS_OK;
 }
 
-// <w, V_v> for v < nv and <w, w> on the host
-static int im_dots(vfs_ctx *c, long long n, int nv, const double *V, const double *w, double *out)
+// the nv + 1 sums of the partials in c->red, on the host in out
+static int im_finish(vfs_ctx *c, int nv, double *out)
 {
     hipStream_t st = c->stream;
-    hipLaunchKernelGGL(k_im_multidot, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, st, n, nv, V, n, w, c->red);
-    hipLaunchKernelGGL(k_im_finish, dim3(nv + 1), dim3(RED_THREADS), 0, st, nv, c->red, c->red + IM_MAXV * RED_BLOCKS);
+    hipLaunchKernelGGL(k_finish_many, dim3(nv + 1), dim3(RED_THREADS), 0, st, nv, c->red, c->red + IM_MAXV * RED_BLOCKS);
     VFS_LAUNCH_CHECK("implicit dots");
     VFS_HIP(hipMemcpyAsync(c->red_host, c->red + IM_MAXV * RED_BLOCKS, (nv + 1) * sizeof(double), hipMemcpyDeviceToHost, st));
     VFS_HIP(hipStreamSynchronize(st));
     std::memcpy(out, c->red_host, (nv + 1) * sizeof(double));
     return VFS_OK;
+}
+
+// <w, V_v> for v < nv and <w, w> on the host
+static int im_dots(vfs_ctx *c, long long n, int nv, const double *V, const double *w, double *out)
+{
+    hipStream_t st = c->stream;
+    const dim3 gb(RED_BLOCKS), tb(RED_THREADS);
+    if (nv <= 4) hipLaunchKernelGGL((k_multidot<4, 4>), gb, tb, 0, st, n, nv, V, n, w, c->red);
+    else if (nv <= 8) hipLaunchKernelGGL((k_multidot<8, 4>), gb, tb, 0, st, n, nv, V, n, w, c->red);
+    else if (nv <= 16) hipLaunchKernelGGL((k_multidot<16, 2>), gb, tb, 0, st, n, nv, V, n, w, c->red);
+    else hipLaunchKernelGGL((k_multidot<IM_MAXV - 1, 1>), gb, tb, 0, st, n, nv, V, n, w, c->red);
+    return im_finish(c, nv, out);
+}
+
+// w -= sum_v h_v V_v (h on the device in s.h); with DOTS the partials of
+// <V_v, w_new> (v < nv) and <w_new, w_new>, else of <w_new, w_new> alone;
+// the finished sums on the host in out
+template <bool DOTS>
+static int im_multiaxpy(vfs_ctx *c, long long n, int nv, const double *h, const double *V, double *w, double *out)
+{
+    hipStream_t st = c->stream;
+    const dim3 gb(RED_BLOCKS), tb(RED_THREADS);
+    const double *g0 = nullptr;
+    if (nv <= 4) hipLaunchKernelGGL((k_multiaxpy<4, 4, DOTS>), gb, tb, 0, st, n, nv, h, V, n, w, c->red, g0, g0);
+    else if (nv <= 8) hipLaunchKernelGGL((k_multiaxpy<8, 4, DOTS>), gb, tb, 0, st, n, nv, h, V, n, w, c->red, g0, g0);
+    else if (nv <= 16) hipLaunchKernelGGL((k_multiaxpy<16, 2, DOTS>), gb, tb, 0, st, n, nv, h, V, n, w, c->red, g0, g0);
+    else hipLaunchKernelGGL((k_multiaxpy<IM_MAXV - 1, 1, DOTS>), gb, tb, 0, st, n, nv, h, V, n, w, c->red, g0, g0);
+    return im_finish(c, DOTS ? nv : 0, out);
 }
 
 static int im_norm(vfs_ctx *c, long long n, const double *w, double *out)
@@ -312,8 +313,9 @@ static int mffd(vfs_ctx *c, ImpState &s, const double *v, double *jv, int use_fe
     double hh = dot;
     if (std::fabs(hh) < umin * asum) hh = (hh >= 0 ? 1.0 : -1.0) * umin * asum;
     hh = err * hh / vv;
-    hipLaunchKernelGGL(k_im_waxpy, dim3(blocks_for(n, IM_TPB)), dim3(IM_TPB), 0, st, n, s.U, hh, v, s.W);
-    int rc = snes_function(c, s.W, jv, use_feul);
+    // U + h v goes straight into UCONT, where the residual evaluation reads it
+    hipLaunchKernelGGL(k_im_waxpy, dim3(blocks_for(n, IM_TPB)), dim3(IM_TPB), 0, st, n, s.U, hh, v, c->f[VFS_UCONT]);
+    int rc = snes_function(c, c->f[VFS_UCONT], jv, use_feul);
     if (rc) return rc;
     (*nfunc)++;
     // jv = (F(U + h v) - F(U)) / h
@@ -363,6 +365,8 @@ static int gmres(vfs_ctx *c, ImpState &s, double rtol, int maxit, int use_feul, 
             double *vj = s.V + (size_t)jj * n, *w = s.V + (size_t)(jj + 1) * n;
             rc = mffd(c, s, vj, w, use_feul, nfunc);
             if (rc) return rc;
+            // classical Gram-Schmidt; the first sweep w -= V h also forms the
+            // dot products a second (re-orthogonalisation) pass needs
             double h[IM_MAXV];
             rc = im_dots(c, n, jj + 1, s.V, w, d);
             if (rc) return rc;
@@ -370,17 +374,15 @@ static int gmres(vfs_ctx *c, ImpState &s, double rtol, int maxit, int use_feul, 
             const double ww0 = d[jj + 1];
             std::memcpy(c->red_host, h, (jj + 1) * sizeof(double));
             VFS_HIP(hipMemcpyAsync(s.h, c->red_host, (jj + 1) * sizeof(double), hipMemcpyHostToDevice, st));
-            hipLaunchKernelGGL(k_im_multiaxpy, dim3(nb), dim3(IM_TPB), 0, st, n, jj + 1, s.h, s.V, n, w);
-            rc = im_dots(c, n, jj + 1, s.V, w, d);
+            rc = im_multiaxpy<true>(c, n, jj + 1, s.h, s.V, w, d);
             if (rc) return rc;
             double ww = d[jj + 1];
             if (ww < 0.25 * ww0) {  // second classical Gram-Schmidt pass
                 for (int v = 0; v <= jj; v++) h[v] += d[v];
                 std::memcpy(c->red_host, d, (jj + 1) * sizeof(double));
                 VFS_HIP(hipMemcpyAsync(s.h, c->red_host, (jj + 1) * sizeof(double), hipMemcpyHostToDevice, st));
-                hipLaunchKernelGGL(k_im_multiaxpy, dim3(nb), dim3(IM_TPB), 0, st, n, jj + 1, s.h, s.V, n, w);
                 double d2[IM_MAXV + 1];
-                rc = im_dots(c, n, 0, nullptr, w, d2);
+                rc = im_multiaxpy<false>(c, n, jj + 1, s.h, s.V, w, d2);
                 if (rc) return rc;
                 ww = d2[0];
             }
@@ -493,8 +495,8 @@ int vfs_implicit_matrixfree(vfs_ctx *c, const vfs_newton_opts *o, vfs_newton_sta
         double lam = 1.0, fnew = 0;
         bool accepted = false;
         for (int half = 0; half < 8; half++) {
-            hipLaunchKernelGGL(k_im_waxpy, dim3(nb), dim3(IM_TPB), 0, st, n, s.U, lam, s.dx, s.W);
-            rc = snes_function(c, s.W, s.FW, o->use_feul);
+            hipLaunchKernelGGL(k_im_waxpy2, dim3(nb), dim3(IM_TPB), 0, st, n, s.U, lam, s.dx, s.W, c->f[VFS_UCONT]);
+            rc = snes_function(c, c->f[VFS_UCONT], s.FW, o->use_feul);
             if (rc) return fail(rc);
             nfunc++;
             rc = im_norm(c, n, s.FW, &fnew);

@@ -279,7 +279,19 @@ int eddy_viscosity(vfs_ctx *c);
 int pressure_gradient(vfs_ctx *c);
 int calc_k_flux(vfs_ctx *c, double *out);
 int mean_k_area(vfs_ctx *c, double *out);
-int formfunction2(vfs_ctx *c, double scale, const double *pucont, double rk_alpha, double *ucont_out);
+// FormFunction_SNES's residual (momentum.c:2237-2336) formed in Formfunction_2's
+// cell pass: r = (0 - u/dt + u_o/dt) + scale F(u) + 0.5 RHS_o - dP (+ F_eul),
+// u = VFS_UCONT; VFS_RHS is not written.  Only on the fused path
+// (formfunction2_fused).
+struct CnResidual {
+    const double *uo, *rhso, *feul;  // feul may be null
+    double rdt;                      // 1 / dt
+    double *r;
+};
+int formfunction2(vfs_ctx *c, double scale, const double *pucont, double rk_alpha, double *ucont_out,
+                  const CnResidual *cn = nullptr);
+// whether Formfunction_2 takes the fused face pass and k_cell_rhs_pr
+bool formfunction2_fused(const vfs_ctx *c);
 int min_dt(vfs_ctx *c, double *out);
 int poisson_rhs(vfs_ctx *c);
 int poisson_apply(vfs_ctx *c, const double *x, double *y);
