@@ -143,6 +143,18 @@ __global__ void k_im_waxpy(long long n, const double *__restrict__ x, double a, 
     z[t] = x[t] + a * y[t];
 }
 
+// x = sum_v y[v] V_v (first) or x + sum_v y[v] V_v, the terms added in v
+// order with the fused multiply-add of a chain of k_im_axpby updates
+__global__ void k_im_update(long long n, int nv, const double *__restrict__ y, const double *__restrict__ V, long long ld,
+                            double *__restrict__ x, int first)
+{
+    long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    double s = first ? 0.0 : x[t];
+    for (int v = 0; v < nv; v++) s = y[v] * V[v * ld + t] + s;
+    x[t] = s;
+}
+
 // z = z2 = x + a y (the line search's trial state and the residual's input)
 __global__ void k_im_waxpy2(long long n, const double *__restrict__ x, double a, const double *__restrict__ y,
                             double *__restrict__ z, double *__restrict__ z2)
@@ -331,17 +343,19 @@ static int gmres(vfs_ctx *c, ImpState &s, double rtol, int maxit, int use_feul, 
     const long long n = s.n3;
     const int m = s.m;
     const unsigned nb = blocks_for(n, IM_TPB);
-    VFS_HIP(hipMemsetAsync(s.dx, 0, n * sizeof(double), st));
     double d[IM_MAXV + 1];
     std::vector<double> H((size_t)(m + 1) * m), cs(m), sn(m), gv(m + 1), y(m);
-    // r0 = -R (dx = 0)
+    // r0 = -R (dx = 0); dx is first written by the first restart's update
     int rc = im_dots(c, n, 0, nullptr, s.R, d);
     if (rc) return rc;
     const double bnorm = std::sqrt(d[0]);
     *its = 0;
-    if (bnorm == 0) return VFS_OK;
+    if (bnorm == 0) {
+        VFS_HIP(hipMemsetAsync(s.dx, 0, n * sizeof(double), st));
+        return VFS_OK;
+    }
     double beta = bnorm;
-    bool first = true;
+    bool first = true, restart0 = true;
     while (*its < maxit) {
         double *V0 = s.V;
         if (first) {
@@ -411,8 +425,11 @@ static int gmres(vfs_ctx *c, ImpState &s, double rtol, int maxit, int use_feul, 
             for (int q = i + 1; q < jj; q++) sacc -= H[(size_t)i * m + q] * y[q];
             y[i] = H[(size_t)i * m + i] != 0 ? sacc / H[(size_t)i * m + i] : 0.0;
         }
-        for (int q = 0; q < jj; q++)
-            hipLaunchKernelGGL(k_im_axpby, dim3(nb), dim3(IM_TPB), 0, st, n, y[q], s.V + (size_t)q * n, 1.0, s.dx);
+        // dx += sum_q y_q V_q in one sweep (dx = 0 before the first restart)
+        std::memcpy(c->red_host, y.data(), jj * sizeof(double));
+        VFS_HIP(hipMemcpyAsync(s.h, c->red_host, jj * sizeof(double), hipMemcpyHostToDevice, st));
+        hipLaunchKernelGGL(k_im_update, dim3(nb), dim3(IM_TPB), 0, st, n, jj, s.h, s.V, n, s.dx, restart0 ? 1 : 0);
+        restart0 = false;
         VFS_LAUNCH_CHECK("gmres update");
         if (std::fabs(gv[jj]) <= rtol * bnorm) break;
     }
