@@ -88,6 +88,23 @@ def test_snes_residual_cell_pass_bitwise(monkeypatch, feul):
     ctx.close()
 
 
+def test_implicit_solve_cell_pass_bitwise(monkeypatch):
+    """The Newton-Krylov solve with the residual and the Jacobian products'
+    difference quotients formed in the cell pass, against the separate passes
+    (VFS_FUSED_CN=0): the same iterates, hence the same velocity bits and
+    counts."""
+    out = []
+    for env in ("1", "0"):
+        monkeypatch.setenv("VFS_FUSED_CN", env)
+        c, o, ctx, uo, rhso = _state(1)
+        st = ctx.implicit_matrixfree(vfs.default_newton_opts(rtol=1e-6, restart=30))
+        out.append((st.newton_iterations, st.linear_iterations, ctx.download("UCONT"), ctx.download("UCAT")))
+        ctx.close()
+    monkeypatch.delenv("VFS_FUSED_CN")
+    assert out[0][:2] == out[1][:2]
+    assert np.array_equal(out[0][2], out[1][2]) and np.array_equal(out[0][3], out[1][3])
+
+
 def test_implicit_solve_root():
     c, o, ctx, uo, rhso = _state(1)
     tol = 1e-6

@@ -652,6 +652,8 @@ struct CellArgs {
     const double *cn_u, *cn_uo, *cn_rhso, *cn_feul;
     double cn_rdt;
     double *cn_r;
+    const double *cn_rbase;  // set: cn_r = cn_da cn_rbase + cn_db R
+    double cn_da, cn_db;
 };
 
 __device__ inline double Fv(const CellArgs &a, int plane, int k, int j, int i)
@@ -870,6 +872,12 @@ __global__ void __launch_bounds__(FT_I * FT_J) k_cell_rhs(CellArgs a, const doub
                 if (a.cn_feul) s = s + fe[c];
                 o[c] = s;
             }
+            if (a.cn_rbase)
+#pragma unroll
+                for (int c = 0; c < 3; c++) {
+                    const double rb = a.cn_rbase[3 * q + c];
+                    o[c] = a.cn_da * rb + a.cn_db * o[c];
+                }
             double *__restrict__ out = a.cn_r + 3 * q;
 #pragma unroll
             for (int c = 0; c < 3; c++) out[c] = o[c];
@@ -972,6 +980,12 @@ __global__ void __launch_bounds__(CR_T) k_cell_rhs_pr(CellArgs a, const double *
                 if (a.cn_feul) s = s + fe[c];
                 o[c] = s;
             }
+            if (a.cn_rbase)
+#pragma unroll
+                for (int c = 0; c < 3; c++) {
+                    const double rb = a.cn_rbase[3 * q + c];
+                    o[c] = a.cn_da * rb + a.cn_db * o[c];
+                }
             double *__restrict__ out = a.cn_r + 3 * q;
 #pragma unroll
             for (int c = 0; c < 3; c++) out[c] = o[c];
@@ -1093,6 +1107,9 @@ int formfunction2(vfs_ctx *c, double scale, const double *pucont, double rk_alph
     ca.cn_feul = cn ? cn->feul : nullptr;
     ca.cn_rdt = cn ? cn->rdt : 0.0;
     ca.cn_r = cn ? cn->r : nullptr;
+    ca.cn_rbase = cn ? cn->rbase : nullptr;
+    ca.cn_da = cn ? cn->da : 0.0;
+    ca.cn_db = cn ? cn->db : 0.0;
     if (!fused) {
         prof_begin(c, K_CELLFP);
         if (ca.second_order)

@@ -225,8 +225,11 @@ static int im_workspace(vfs_ctx *c, ImpState &s)
     return VFS_OK;
 }
 
-// R(u) into r (FormFunction_SNES); leaves UCONT/UCAT at the BC-zeroed u
-int snes_function(vfs_ctx *c, const double *u, double *r, int use_feul)
+// R(u) into r (FormFunction_SNES); leaves UCONT/UCAT at the BC-zeroed u.
+// With rbase, r = da rbase + db R(u) (one k_im_axpby's arithmetic), formed in
+// the cell pass on the fused path.
+int snes_function(vfs_ctx *c, const double *u, double *r, int use_feul, const double *rbase = nullptr, double da = 0,
+                  double db = 0)
 {
     const Grid &g = c->g;
     hipStream_t st = c->stream;
@@ -242,7 +245,8 @@ int snes_function(vfs_ctx *c, const double *u, double *r, int use_feul)
     // operations in the same order); VFS_FUSED_CN=0 keeps the three passes
     const char *fe = std::getenv("VFS_FUSED_CN");
     if (formfunction2_fused(c) && !(fe && fe[0] == '0')) {
-        const CnResidual cn{c->f[VFS_UCONT_O], c->f[VFS_RHS_O], use_feul ? c->f[VFS_FEUL] : nullptr, 1. / c->p.dt, r};
+        const CnResidual cn{c->f[VFS_UCONT_O], c->f[VFS_RHS_O], use_feul ? c->f[VFS_FEUL] : nullptr, 1. / c->p.dt, r,
+                            rbase, da, db};
         return formfunction2(c, 0.5, nullptr, 0.0, nullptr, &cn);
     }
     hipLaunchKernelGGL(k_cn_base, dim3(blocks_for(n3, IM_TPB)), dim3(IM_TPB), 0, st, n3, 1. / c->p.dt, c->f[VFS_UCONT],
@@ -251,6 +255,7 @@ int snes_function(vfs_ctx *c, const double *u, double *r, int use_feul)
     if (rc) return rc;
     hipLaunchKernelGGL(k_cn_finish, dim3(blocks_for(n3, IM_TPB)), dim3(IM_TPB), 0, st, n3, c->f[VFS_RHS], c->f[VFS_RHS_O],
                        c->f[VFS_DP], use_feul ? c->f[VFS_FEUL] : (const double *)nullptr, r);
+    if (rbase) hipLaunchKernelGGL(k_im_axpby, dim3(blocks_for(n3, IM_TPB)), dim3(IM_TPB), 0, st, n3, da, rbase, db, r);
     VFS_LAUNCH_CHECK("snes residual");
     return VFS_OK;
 }
@@ -327,11 +332,10 @@ static int mffd(vfs_ctx *c, ImpState &s, const double *v, double *jv, int use_fe
     hh = err * hh / vv;
     // U + h v goes straight into UCONT, where the residual evaluation reads it
     hipLaunchKernelGGL(k_im_waxpy, dim3(blocks_for(n, IM_TPB)), dim3(IM_TPB), 0, st, n, s.U, hh, v, c->f[VFS_UCONT]);
-    int rc = snes_function(c, c->f[VFS_UCONT], jv, use_feul);
+    // jv = (F(U + h v) - F(U)) / h, the quotient formed with the residual
+    int rc = snes_function(c, c->f[VFS_UCONT], jv, use_feul, s.R, -1.0 / hh, 1.0 / hh);
     if (rc) return rc;
     (*nfunc)++;
-    // jv = (F(U + h v) - F(U)) / h
-    hipLaunchKernelGGL(k_im_axpby, dim3(blocks_for(n, IM_TPB)), dim3(IM_TPB), 0, st, n, -1.0 / hh, s.R, 1.0 / hh, jv);
     VFS_LAUNCH_CHECK("mffd");
     return VFS_OK;
 }

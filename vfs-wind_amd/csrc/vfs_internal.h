@@ -287,6 +287,10 @@ struct CnResidual {
     const double *uo, *rhso, *feul;  // feul may be null
     double rdt;                      // 1 / dt
     double *r;
+    // when rbase is set, r receives da rbase + db R(u) instead (the
+    // matrix-free Jacobian product's difference quotient, da = -1/h, db = 1/h)
+    const double *rbase = nullptr;
+    double da = 0, db = 0;
 };
 int formfunction2(vfs_ctx *c, double scale, const double *pucont, double rk_alpha, double *ucont_out,
                   const CnResidual *cn = nullptr);
