@@ -72,7 +72,7 @@ def test_line_solve_paths(monkeypatch, path, fp32):
 
 
 @pytest.mark.parametrize("cells", [(24, 16, 20), (18, 13, 22), (40, 70, 36)], ids=["24x16x20", "18x13x22", "40x70x36"])
-@pytest.mark.parametrize("form", ["compact-short", "compact-long", "full", "fp64"])
+@pytest.mark.parametrize("form", ["compact-short", "compact-long", "compact-natural", "full", "fp64"])
 def test_one_pass_forms_bitwise(monkeypatch, form, cells):
     """The V-cycle's one-pass forms against their two-pass forms, on whole
     FGMRES solves: the compact smoother's relaxation that forms each line
@@ -80,11 +80,14 @@ def test_one_pass_forms_bitwise(monkeypatch, form, cells):
     colour, as the two-pass form does; on coarse levels its first half-sweep
     takes the iterate as zero instead of reading a zero-filled one), and the
     residual formed inside the restriction instead of stored (compact, full
-    and FP64 levels).  Same bits, same iteration count."""
+    and FP64 levels), and on the compact path the finest level's vectors in
+    the colour-split layout (compact-natural: in the natural one).  Same
+    bits, same iteration count."""
     env = {"VFS_LINE_PATH": "short" if form == "compact-short" else "long"}
     env["VFS_SMOOTH_COMPACT"] = "0" if form == "full" else "1"
     fp32 = 0 if form == "fp64" else 1
-    st1, phi1, res1 = _solve(monkeypatch, env, cells, tol=1e-10, mg_fp32=fp32)
+    env1 = {**env, "VFS_MG_SPLIT": "0"} if form == "compact-natural" else env
+    st1, phi1, res1 = _solve(monkeypatch, env1, cells, tol=1e-10, mg_fp32=fp32)
     st0, phi0, res0 = _solve(monkeypatch, {**env, "VFS_ZEBRA_FUSED": "0", "VFS_MG_FUSED_RESTRICT": "0"}, cells, tol=1e-10,
                              mg_fp32=fp32)
     assert st0.relres <= 1e-10 and res0 <= 1.01e-10

@@ -395,6 +395,39 @@ struct ZGrid {
 
 __host__ __device__ inline int zebra_i0(int k, int c) { return 1 + ((1 + k + c) & 1); }
 
+// Colour-split ("S") layout of a level vector: the interior points only, in
+// the smoother's order [colour][k-1][j-1][h] with i = zebra_i0(k, colour) + 2h
+// (the colour of (i, k) is (i + k) & 1).  The finest level's iterate and
+// right-hand side take it on the compact one-pass path, so that a half-sweep
+// reads and writes whole cache lines of one colour.
+__host__ __device__ inline bool zs_interior(const Grid &g, int k, int j, int i)
+{
+    return i >= 1 && i <= g.mx - 2 && j >= 1 && j <= g.my - 2 && k >= 1 && k <= g.mz - 2;
+}
+__host__ __device__ inline long long zs_index(const ZGrid &z, int k, int j, int i)
+{
+    const int c = (i + k) & 1;
+    const int h = (i - zebra_i0(k, c)) >> 1;
+    return c * z.cplane + ((long long)(k - 1) * z.njI + (j - 1)) * z.nh + h;
+}
+// the same in 32-bit arithmetic (grids hold fewer than 2^31 points, so every
+// S position does too), for the per-point kernels
+__device__ inline unsigned zs_index32(const ZGrid &z, int k, int j, int i)
+{
+    const unsigned c = (unsigned)(i + k) & 1u;
+    const unsigned h = (unsigned)(i - zebra_i0(k, (int)c)) >> 1;
+    return c * (unsigned)z.cplane + ((unsigned)(k - 1) * (unsigned)z.njI + (unsigned)(j - 1)) * (unsigned)z.nh + h;
+}
+// (k, j, i) of natural point t of g, in 32-bit arithmetic
+__device__ inline void nat_decode(const Grid &g, unsigned t, int &k, int &j, int &i)
+{
+    const unsigned mx = (unsigned)g.mx, my = (unsigned)g.my;
+    const unsigned r = t / mx;
+    i = (int)(t - r * mx);
+    k = (int)(r / my);
+    j = (int)(r - (unsigned)k * my);
+}
+
 // The FP32 smoother keeps the 16 off-line coefficients of a row divided by
 // its diagonal, as FP16 pairs (8 planes of 32-bit words, slots 2p and 2p+1
 // of offline_slot), with the diagonal in an FP32 plane: 36 bytes per point
@@ -769,7 +802,7 @@ __host__ __device__ inline bool zebra_fused_ok(const Grid &g)
     return (!g.pi || ((g.mx - 2) & 1) == 0) && (!g.pk || ((g.mz - 2) & 1) == 0);
 }
 
-template <int SLM, bool XZ>
+template <int SLM, bool XZ, bool SPL>
 __global__ void __launch_bounds__(LINE_LINES * LINE_MAXSEG) k_zebra_line_c4(Grid g, ZGrid z, int c, int nseg, int sl,
                                                                            const h4_t *__restrict__ sh,
                                                                            const float *__restrict__ sd,
@@ -798,17 +831,31 @@ __global__ void __launch_bounds__(LINE_LINES * LINE_MAXSEG) k_zebra_line_c4(Grid
     const long long nh = z.nh;
     const int I[3] = {canon(i - 1, g.mx, g.pi), i, canon(i + 1, g.mx, g.pi)};
     const int K[3] = {canon(k - 1, g.mz, g.pk), k, canon(k + 1, g.mz, g.pk)};
+    // SPL: b and x in the S layout.  The neighbours are of the other colour
+    // (q): rows kk (i +- 1), K[0] - 1 and K[2] - 1 (i, at the line's own h);
+    // a non-periodic ghost neighbour reads as the zero it holds in the
+    // natural layout
+    const int q = 1 - c;
+    const bool okw = I[0] >= 1 && I[0] <= g.mx - 2, oke = I[2] >= 1 && I[2] <= g.mx - 2;
+    const bool okb = K[0] >= 1 && K[0] <= g.mz - 2, okt = K[2] >= 1 && K[2] <= g.mz - 2;
+    const long long hw = okw ? (I[0] - zebra_i0(k, q)) >> 1 : 0, he = oke ? (I[2] - zebra_i0(k, q)) >> 1 : 0;
+    const long long rowq = q * z.cplane, nrow = (long long)z.njI * z.nh;
+    const long long sw = rowq + kk * nrow + hw, se = rowq + kk * nrow + he;
+    const long long sbt = rowq + (okb ? K[0] - 1 : 0) * nrow + h, stp = rowq + (okt ? K[2] - 1 : 0) * nrow + h;
     float r[SLM], l[SLM], m[SLM], u[SLM];
 #pragma unroll
     for (int v = 0; v < SLM; v++) {
         const int jj = j0 + v;
         if (act && jj < j1) {
             const long long o = jj * nh;
-            const float bq = b[((long long)k * g.my + jj + 1) * g.mx + i];
+            const float bq = SPL ? b[cb + o] : b[((long long)k * g.my + jj + 1) * g.mx + i];
             // XZ: the iterate is zero on entry (the first half-sweep of a
             // V-cycle level): the same sum over zero neighbours, without reading them
             if constexpr (XZ)
                 r[v] = c4_sum(sh[cb + o], sd[cb + o], 0.f, 0.f, 0.f, 0.f, bq);
+            else if constexpr (SPL)
+                r[v] = c4_sum(sh[cb + o], sd[cb + o], oke ? x[se + o] : 0.f, okw ? x[sw + o] : 0.f, okt ? x[stp + o] : 0.f,
+                              okb ? x[sbt + o] : 0.f, bq);
             else
                 r[v] = c4_rhs(g, I, K, jj + 1, sh[cb + o], sd[cb + o], x, bq);
             l[v] = lo[o];
@@ -859,7 +906,8 @@ __global__ void __launch_bounds__(LINE_LINES * LINE_MAXSEG) k_zebra_line_c4(Grid
             const int jj = j0 + v;
             if (jj < j1) {
                 xl = r[v] - u[v] * xl;
-                x[((long long)k * g.my + jj + 1) * g.mx + i] = xl;
+                if (SPL) x[cb + jj * nh] = xl;
+                else x[((long long)k * g.my + jj + 1) * g.mx + i] = xl;
             }
         }
     }
@@ -924,11 +972,12 @@ __global__ void k_restrict(Grid gc, Grid gf, const unsigned char *__restrict__ r
 // k_restrict's order, so bc has the same bits as the two-kernel form.  The
 // x values of the 4x4x4 fine block around the children come through the
 // per-axis canonical positions nbr_idx uses.
-template <typename T, int NQ>
+// SPL: the fine x and b in the S layout (zf), a ghost x reading as zero.
+template <typename T, int NQ, bool SPL = false>
 __global__ void __launch_bounds__(PTPB) k_residual_restrict(Grid gc, Grid gf, const T *__restrict__ coef,
                                                             const unsigned char *__restrict__ rowc,
                                                             const unsigned char *__restrict__ rowf, const T *__restrict__ x,
-                                                            const T *__restrict__ b, T *__restrict__ bc)
+                                                            const T *__restrict__ b, T *__restrict__ bc, ZGrid zf = {})
 {
     long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
     if (t >= gc.n) return;
@@ -953,10 +1002,15 @@ __global__ void __launch_bounds__(PTPB) k_residual_restrict(Grid gc, Grid gf, co
                 if (!rowf[qf]) continue;
                 T a = 0;
 #pragma unroll
-                for (int q = 0; q < NQ; q++)
-                    a += coef[q * gf.n + qf] *
-                         x[((long long)K[dk + 1 + soff(q, 2)] * gf.my + J[dj + 1 + soff(q, 1)]) * gf.mx + I[di + 1 + soff(q, 0)]];
-                s += b[qf] - a;
+                for (int q = 0; q < NQ; q++) {
+                    const int kq = K[dk + 1 + soff(q, 2)], jq = J[dj + 1 + soff(q, 1)], iq = I[di + 1 + soff(q, 0)];
+                    T xv;
+                    if constexpr (SPL) xv = zs_interior(gf, kq, jq, iq) ? x[zs_index32(zf, kq, jq, iq)] : T(0);
+                    else xv = x[((long long)kq * gf.my + jq) * gf.mx + iq];
+                    a += coef[q * gf.n + qf] * xv;
+                }
+                const T bq = SPL ? b[zs_index32(zf, 2 * k - 1 + dk, 2 * j - 1 + dj, 2 * i - 1 + di)] : b[qf];
+                s += bq - a;
             }
     bc[t] = s;
 }
@@ -987,6 +1041,55 @@ __global__ void k_prolong(Grid gc, Grid gf, const unsigned char *__restrict__ ro
           T(3) * (XC(K, J2, I2) + XC(K2, J, I2) + XC(K2, J2, I)) + XC(K2, J2, I2);
 #undef XC
     xf[t] += v / T(64);
+}
+
+// k_prolong on a fine iterate in the S layout: one thread per S position,
+// the same interpolation expression
+template <typename T>
+__global__ void k_prolong_split(Grid gc, Grid gf, ZGrid zf, const unsigned char *__restrict__ rowf, const T *__restrict__ xc,
+                                T *__restrict__ xs)
+{
+    const unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
+    const unsigned cp = (unsigned)zf.cplane, nh = (unsigned)zf.nh, nj = (unsigned)zf.njI;
+    if (t >= 2 * cp) return;
+    const unsigned col = t >= cp ? 1u : 0u;
+    const unsigned rem = t - col * cp;
+    const unsigned r = rem / nh, h = rem - r * nh;
+    const unsigned kk = r / nj, jj = r - kk * nj;
+    const int j = (int)jj + 1, k = (int)kk + 1;
+    const int i = zebra_i0(k, (int)col) + 2 * (int)h;
+    if (i > gf.mx - 2) return;
+    if (!rowf[((long long)k * gf.my + j) * gf.mx + i]) return;
+    int I = (i + 1) / 2, J = (j + 1) / 2, K = (k + 1) / 2;
+    int I2 = parent_nb(I, (i & 1) ? -1 : 1, gc.mx, gc.pi);
+    int J2 = parent_nb(J, (j & 1) ? -1 : 1, gc.my, gc.pj);
+    int K2 = parent_nb(K, (k & 1) ? -1 : 1, gc.mz, gc.pk);
+#define XC(kk, jj, ii) xc[((long long)(kk) * gc.my + (jj)) * gc.mx + (ii)]
+    T v = T(27) * XC(K, J, I) + T(9) * (XC(K, J, I2) + XC(K, J2, I) + XC(K2, J, I)) +
+          T(3) * (XC(K, J2, I2) + XC(K2, J, I2) + XC(K2, J2, I)) + XC(K2, J2, I2);
+#undef XC
+    xs[t] += v / T(64);
+}
+
+// natural FP64 v -> S-layout FP32 (the finest right-hand side)
+__global__ void k_convert_split(Grid g, ZGrid z, const double *__restrict__ v, float *__restrict__ bs)
+{
+    const unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (unsigned)g.n) return;
+    int k, j, i;
+    nat_decode(g, t, k, j, i);
+    if (zs_interior(g, k, j, i)) bs[zs_index32(z, k, j, i)] = (float)v[t];
+}
+
+// S-layout FP32 iterate -> natural FP32 vector, zero at the ghosts (as the
+// natural finest iterate holds them)
+__global__ void k_split_to_natural(Grid g, ZGrid z, const float *__restrict__ xs, float *__restrict__ x)
+{
+    const unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (unsigned)g.n) return;
+    int k, j, i;
+    nat_decode(g, t, k, j, i);
+    x[t] = zs_interior(g, k, j, i) ? xs[zs_index32(z, k, j, i)] : 0.f;
 }
 
 // coarse node coordinates: every other fine node (init.c:402-455)
@@ -1060,8 +1163,9 @@ __global__ void __launch_bounds__(RED_THREADS) k_finish_gated(const double *__re
 // y = x / sqrt(ww) (0 when ww <= 0), ww read from device memory
 // y = x / sqrt(ww[0]) (0 when ww[0] <= 0); y32, when given, receives y as
 // FP32 (the next FP32 V-cycle's right-hand side)
+// (natural layout, or the S layout of zs when zs.cplane > 0: interior points only)
 __global__ void k_scale_rnorm(long long n, const double *__restrict__ ww, const double *__restrict__ x,
-                              double *__restrict__ y, float *__restrict__ y32)
+                              double *__restrict__ y, float *__restrict__ y32, Grid g, ZGrid zs)
 {
     long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
     const double hn = sqrt(ww[0] > 0 ? ww[0] : 0);
@@ -1069,7 +1173,15 @@ __global__ void k_scale_rnorm(long long n, const double *__restrict__ ww, const 
     if (t < n) {
         const double v = s * x[t];
         y[t] = v;
-        if (y32) y32[t] = (float)v;
+        if (y32) {
+            if (zs.cplane > 0) {
+                int k, j, i;
+                nat_decode(g, (unsigned)t, k, j, i);
+                if (zs_interior(g, k, j, i)) y32[zs_index32(zs, k, j, i)] = (float)v;
+            } else {
+                y32[t] = (float)v;
+            }
+        }
     }
 }
 
@@ -1137,6 +1249,10 @@ struct PLevel {
     double *sc64 = nullptr, *rz64 = nullptr;
     // work vectors
     float *x32 = nullptr, *b32 = nullptr, *r32 = nullptr;
+    // the finest level's FP32 iterate and right-hand side in the S layout
+    // (the compact one-pass path; finest set on level 0 only)
+    float *xs32 = nullptr, *bs32 = nullptr;
+    bool finest = false;
     double *x64 = nullptr, *b64 = nullptr, *r64 = nullptr;
     // dense inverse on the coarsest level (nr rows), column-major
     int nr = 0;
@@ -1402,7 +1518,10 @@ static int poisson_setup(vfs_ctx *c, int levels)
     }
     ps->nl = nl;
     ps->L[0].g = c->g;
+    ps->L[0].finest = true;
     rc = level_alloc_work(ps, ps->L[0]);
+    if (!rc) rc = palloc(ps, (void **)&ps->L[0].xs32, 2 * (size_t)ps->L[0].z.cplane * 4);
+    if (!rc) rc = palloc(ps, (void **)&ps->L[0].bs32, 2 * (size_t)ps->L[0].z.cplane * 4);
     if (rc) return rc;
     double *const m0[17] = {c->f[VFS_CSI], c->f[VFS_ETA], c->f[VFS_ZET], c->f[VFS_AJ], c->f[VFS_ICSI], c->f[VFS_IETA],
                             c->f[VFS_IZET], c->f[VFS_IAJ], c->f[VFS_JCSI], c->f[VFS_JETA], c->f[VFS_JZET], c->f[VFS_JAJ],
@@ -1504,6 +1623,7 @@ constexpr int LINE_PREFETCH = 16;
 // Which line-relaxation form a level takes.
 struct LinePlan {
     bool short_seg, scan, c4, fused;
+    bool split;  // the finest level on the one-pass path: x and b in the S layout
     int nseg, sl;
 };
 template <typename T>
@@ -1528,6 +1648,11 @@ static LinePlan line_plan(const PLevel &L)
     // (k_zebra_line_c4) unless VFS_ZEBRA_FUSED=0 asks for the two-pass form
     const char *fz = std::getenv("VFS_ZEBRA_FUSED");
     p.fused = p.c4 && p.scan && zebra_fused_ok(L.g) && !(fz && fz[0] == '0');
+    // VFS_MG_SPLIT=0 keeps the finest level's vectors in the natural layout
+    // (as does the stored-residual restriction, VFS_MG_FUSED_RESTRICT=0)
+    const char *fs = std::getenv("VFS_MG_SPLIT"), *fr = std::getenv("VFS_MG_FUSED_RESTRICT");
+    // (and a finest level that is also a dense coarsest level keeps it)
+    p.split = p.fused && L.finest && L.xs32 && L.nr == 0 && !(fs && fs[0] == '0') && !(fr && fr[0] == '0');
     return p;
 }
 
@@ -1551,10 +1676,13 @@ static void smooth(vfs_ctx *c, PLevel &L, int sweeps, bool x_zero = false)
             if (fused) {
                 const bool xz = x_zero && s == 0 && col == 0;
                 if (fine) prof_begin(c, K_LRELAXC32);
-                auto k = short_seg ? (xz ? k_zebra_line_c4<LINE_SLM_SHORT, true> : k_zebra_line_c4<LINE_SLM_SHORT, false>)
-                                   : (xz ? k_zebra_line_c4<line_slm<float>(), true> : k_zebra_line_c4<line_slm<float>(), false>);
+                constexpr int SL = LINE_SLM_SHORT, LG = line_slm<float>();
+                auto k = P.split ? (short_seg ? (xz ? k_zebra_line_c4<SL, true, true> : k_zebra_line_c4<SL, false, true>)
+                                              : (xz ? k_zebra_line_c4<LG, true, true> : k_zebra_line_c4<LG, false, true>))
+                                 : (short_seg ? (xz ? k_zebra_line_c4<SL, true, false> : k_zebra_line_c4<SL, false, false>)
+                                              : (xz ? k_zebra_line_c4<LG, true, false> : k_zebra_line_c4<LG, false, false>));
                 hipLaunchKernelGGL(k, dim3(scan_blocks), dim3(LINE_LINES * nseg), 0, st, L.g, L.z, col, nseg, sl, sh4, L.sd32,
-                                   L.b32, L.sc32, L.x32);
+                                   P.split ? L.bs32 : L.b32, L.sc32, P.split ? L.xs32 : L.x32);
                 if (fine) prof_end(c);
                 continue;
             }
@@ -1644,15 +1772,22 @@ static void vcycle(vfs_ctx *c, vfs_poisson *ps, int l, const Sweeps &S)
     PLevel &L = ps->L[l];
     const int pre = S.pre_at(l), post = S.post_at(l);
     const unsigned nb = blocks_for(L.g.n, PTPB);
-    if (l == ps->nl - 1) {
+    // the finest level in the S layout (FP32 one-pass path): its iterate starts
+    // unfilled; the first half-sweep takes it as zero (a level with no sweep
+    // before its iterate is read gets a zero fill)
+    const bool spl = sizeof(T) == 4 && line_plan<T>(L).split;
+    const bool last = l == ps->nl - 1;
+    if (spl && (last ? S.coarse : pre) < 1)
+        hipLaunchKernelGGL(k_zero<float>, dim3(blocks_for(2 * L.z.cplane, PTPB)), dim3(PTPB), 0, st, 2 * L.z.cplane, L.xs32);
+    if (last) {
         if (L.nr > 0)
             hipLaunchKernelGGL(k_dense_solve<T>, dim3(blocks_for(L.nr, DENSE_ROWS)), dim3(DENSE_ROWS * DENSE_SPLIT), 0, st, L.nr, L.ridx,
                                LV<T>::inv(L), LV<T>::b(L), LV<T>::x(L));
         else
-            smooth<T>(c, L, S.coarse, coarse_zero_free<T>(ps, l, S));
+            smooth<T>(c, L, S.coarse, spl || coarse_zero_free<T>(ps, l, S));
         return;
     }
-    smooth<T>(c, L, pre, coarse_zero_free<T>(ps, l, S));
+    smooth<T>(c, L, pre, spl || coarse_zero_free<T>(ps, l, S));
     // a compact level's V-cycle works on the axis part of its operator: the
     // residual it restricts takes the centre and the six axis neighbours.
     // The residual is formed inside the restriction (k_residual_restrict)
@@ -1661,13 +1796,17 @@ static void vcycle(vfs_ctx *c, vfs_poisson *ps, int l, const Sweeps &S)
     const unsigned nbc = blocks_for(C.g.n, PTPB);
     const char *fr = std::getenv("VFS_MG_FUSED_RESTRICT");
     const bool axis = sizeof(T) == 4 && L.compact;
-    if (!(fr && fr[0] == '0')) {
+    if (spl) {
+        if constexpr (sizeof(T) == 4)
+            hipLaunchKernelGGL((k_residual_restrict<float, 7, true>), dim3(nbc), dim3(PTPB), 0, st, C.g, L.g, L.c32, C.row, L.row,
+                               (const float *)L.xs32, (const float *)L.bs32, C.b32, L.z);
+    } else if (!(fr && fr[0] == '0')) {
         if (axis)
             hipLaunchKernelGGL((k_residual_restrict<T, 7>), dim3(nbc), dim3(PTPB), 0, st, C.g, L.g, LV<T>::coef(L), C.row, L.row,
-                               (const T *)LV<T>::x(L), (const T *)LV<T>::b(L), LV<T>::b(C));
+                               (const T *)LV<T>::x(L), (const T *)LV<T>::b(L), LV<T>::b(C), L.z);
         else
             hipLaunchKernelGGL((k_residual_restrict<T, NST>), dim3(nbc), dim3(PTPB), 0, st, C.g, L.g, LV<T>::coef(L), C.row,
-                               L.row, (const T *)LV<T>::x(L), (const T *)LV<T>::b(L), LV<T>::b(C));
+                               L.row, (const T *)LV<T>::x(L), (const T *)LV<T>::b(L), LV<T>::b(C), L.z);
     } else {
         if (axis)
             hipLaunchKernelGGL((k_apply<T, T, 7>), dim3(nb), dim3(PTPB), 0, st, L.g, LV<T>::coef(L), L.row, LV<T>::x(L),
@@ -1684,7 +1823,13 @@ static void vcycle(vfs_ctx *c, vfs_poisson *ps, int l, const Sweeps &S)
             hipLaunchKernelGGL(k_zero<T>, dim3(blocks_for(C.g.n, PTPB)), dim3(PTPB), 0, st, C.g.n, LV<T>::x(C));
         vcycle<T>(c, ps, l + 1, S);
     }
-    hipLaunchKernelGGL(k_prolong<T>, dim3(nb), dim3(PTPB), 0, st, C.g, L.g, L.row, LV<T>::x(C), LV<T>::x(L));
+    if (spl) {
+        if constexpr (sizeof(T) == 4)
+            hipLaunchKernelGGL(k_prolong_split<float>, dim3(blocks_for(2 * L.z.cplane, PTPB)), dim3(PTPB), 0, st, C.g, L.g, L.z,
+                               L.row, (const float *)C.x32, L.xs32);
+    } else {
+        hipLaunchKernelGGL(k_prolong<T>, dim3(nb), dim3(PTPB), 0, st, C.g, L.g, L.row, LV<T>::x(C), LV<T>::x(L));
+    }
     smooth<T>(c, L, post);
 }
 
@@ -1750,11 +1895,19 @@ static int precondition(vfs_ctx *c, const vfs_solve_opts *o, const double *v, do
     const unsigned nb = blocks_for(n, PTPB);
     if (o->amg) return amg_apply(st, ps->amg, n, v, z);
     if (o->mg_fp32) {
-        if (!b32_ready) hipLaunchKernelGGL((k_convert<double, float>), dim3(nb), dim3(PTPB), 0, st, n, v, L.b32);
+        // on the one-pass path the V-cycle's finest vectors are in the S
+        // layout: b converted into it, the iterate converted back into the
+        // natural x (zero at the ghosts) at the end
+        const bool spl = line_plan<float>(L).split;
+        if (!b32_ready) {
+            if (spl) hipLaunchKernelGGL(k_convert_split, dim3(nb), dim3(PTPB), 0, st, L.g, L.z, v, L.bs32);
+            else hipLaunchKernelGGL((k_convert<double, float>), dim3(nb), dim3(PTPB), 0, st, n, v, L.b32);
+        }
         float *x0 = L.x32;
         if (z32) L.x32 = z32;
-        hipLaunchKernelGGL(k_zero<float>, dim3(nb), dim3(PTPB), 0, st, n, L.x32);
+        if (!spl) hipLaunchKernelGGL(k_zero<float>, dim3(nb), dim3(PTPB), 0, st, n, L.x32);
         vcycle<float>(c, ps, 0, sweeps_of(o));
+        if (spl) hipLaunchKernelGGL(k_split_to_natural, dim3(nb), dim3(PTPB), 0, st, L.g, L.z, (const float *)L.xs32, L.x32);
         if (!z32) hipLaunchKernelGGL((k_convert<float, double>), dim3(nb), dim3(PTPB), 0, st, n, L.x32, z);
         L.x32 = x0;
     } else {
@@ -1810,7 +1963,8 @@ static void launch_multiaxpy(vfs_ctx *c, int nv, const double *V, double *w, con
 // at [0..nv], [40..40+nv], [80].
 constexpr int GS_D2 = 40, GS_D3 = 80;
 static_assert(GS_D2 >= MAXV && GS_D3 >= GS_D2 + MAXV, "Gram-Schmidt result slots");
-static int gs_step(vfs_ctx *c, int nv, const double *V, double *w, double *vnext, double *out, float *vnext32)
+static int gs_step(vfs_ctx *c, int nv, const double *V, double *w, double *vnext, double *out, float *vnext32,
+                   ZGrid zs = {})
 {
     hipStream_t st = c->stream;
     const long long n = c->g.n;
@@ -1826,7 +1980,7 @@ static int gs_step(vfs_ctx *c, int nv, const double *V, double *w, double *vnext
     launch_multiaxpy<false>(c, nv, V, w, R + GS_D2, R + GS_D2 + nv, R + nv);
     hipLaunchKernelGGL(k_finish_gated, dim3(1), dim3(RED_THREADS), 0, st, c->red, RED_BLOCKS, R + GS_D2 + nv, R + nv,
                        R + GS_D3);
-    hipLaunchKernelGGL(k_scale_rnorm, dim3(blocks_for(n, PTPB)), dim3(PTPB), 0, st, n, R + GS_D3, w, vnext, vnext32);
+    hipLaunchKernelGGL(k_scale_rnorm, dim3(blocks_for(n, PTPB)), dim3(PTPB), 0, st, n, R + GS_D3, w, vnext, vnext32, c->g, zs);
     VFS_LAUNCH_CHECK("gram-schmidt");
     VFS_HIP(hipMemcpyAsync(c->red_host, R, (GS_D3 + 1) * sizeof(double), hipMemcpyDeviceToHost, st));
     VFS_HIP(hipStreamSynchronize(st));
@@ -1958,7 +2112,10 @@ static int fgmres(vfs_ctx *c, const vfs_solve_opts *o, vfs_unet *net, vfs_solve_
             double gs[GS_D3 + 1];
             // with FP32 V-cycles the new basis vector is also written as the
             // next V-cycle's FP32 right-hand side
-            rc = gs_step(c, jj + 1, V, w, V + (size_t)(jj + 1) * n, gs, z32 ? L.b32 : nullptr);
+            // (in the S layout when the V-cycle's finest level takes it)
+            const bool spl = z32 && line_plan<float>(L).split;
+            rc = gs_step(c, jj + 1, V, w, V + (size_t)(jj + 1) * n, gs, z32 ? (spl ? L.bs32 : L.b32) : nullptr,
+                         spl ? L.z : ZGrid{});
             if (rc) return rc;
             double h[MAXV];
             for (int v = 0; v <= jj; v++) h[v] = gs[v];
