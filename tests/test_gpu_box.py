@@ -92,3 +92,30 @@ def test_box_cn_with_rotor():
     assert np.linalg.norm(r[keep]) <= 2 * tol * np.linalg.norm(r0[keep])
     alm.close()
     ctx.close()
+
+
+def test_box_poisson_one_pass_forms_bitwise(monkeypatch):
+    """On the box (no periodic direction, so the compact smoother's i and k
+    neighbours at the walls are ghosts): the FP32 V-cycle with the finest
+    level in the colour-split layout, in the natural layout
+    (VFS_MG_SPLIT=0), and the two-pass natural form (VFS_ZEBRA_FUSED=0,
+    VFS_MG_FUSED_RESTRICT=0) give the same solve, bit for bit."""
+    out = []
+    variants = ({}, {"VFS_MG_SPLIT": "0"}, {"VFS_ZEBRA_FUSED": "0", "VFS_MG_FUSED_RESTRICT": "0"})
+    for env in variants:
+        env = {"VFS_SMOOTH_COMPACT": "1", **env}
+        for k, v in env.items():
+            monkeypatch.setenv(k, v)
+        c, o, ctx = _pair()
+        try:
+            ctx.poisson_rhs()
+            ctx.upload("PHI", np.zeros(o.shape))
+            st = ctx.poisson_solve(vfs.default_solve_opts(tol=1e-10, use_guess=0, maxit=400))
+            assert st.relres <= 1e-10
+            out.append((st.iterations, ctx.download("PHI")))
+        finally:
+            ctx.close()
+            for k in env:
+                monkeypatch.delenv(k)
+    for its, phi in out[1:]:
+        assert its == out[0][0] and np.array_equal(phi, out[0][1])
