@@ -44,3 +44,22 @@ def test_partial_contra2cart_bitwise(kind, monkeypatch):
         ctx.close()
     for a, b in zip(*runs):
         assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("kind", ["channel", "box"])
+def test_lazy_ucat_after_rk_bitwise(kind, monkeypatch):
+    """The fractional step leaves the last RK stage's Contra2Cart (and
+    FormBCS's) to the projection's, which recomputes Ucat from the same
+    Ucont: whole steps give the bits of the eager sequence (VFS_RK_LAZY=0),
+    Ucat's ghosts and edges included."""
+    runs = []
+    for lazy in ("1", "0"):
+        monkeypatch.setenv("VFS_RK_LAZY", lazy)
+        ctx = _context(kind)
+        for _ in range(2):
+            st = ctx.fractional_step(vfs.default_solve_opts(tol=1e-10))
+            assert st.poisson_relres <= 1e-10
+        runs.append([ctx.download(f) for f in ("UCONT", "UCAT", "P")])
+        ctx.close()
+    for a, b in zip(*runs):
+        assert np.array_equal(a, b)

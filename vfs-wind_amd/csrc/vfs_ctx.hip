@@ -406,7 +406,17 @@ int vfs_formfunction2(vfs_ctx *c, double scale)
 int vfs_runge_kutta(vfs_ctx *c)
 {
     if (!c) return vfs::set_error(VFS_EINVAL, "null context");
+    return vfs::runge_kutta(c, false);
+}
+
+}  // extern "C"
+
+int vfs::runge_kutta(vfs_ctx *c, bool lazy_ucat)
+{
     static const double alfa[4] = {0.25, 1 / 3., 0.5, 1.};
+    // VFS_RK_LAZY=0 computes Ucat at the end of the stages in any case
+    const char *lz = std::getenv("VFS_RK_LAZY");
+    const bool lazy = lazy_ucat && !(lz && lz[0] == '0') && !vfs::form_bcs_reads_ucat(c);
     int rc = vfs::inflow_flux(c, nullptr);  // rhs.c:1612
     if (!rc) rc = vfs::form_bcs(c, 0);
     if (rc) return rc;
@@ -422,11 +432,13 @@ int vfs_runge_kutta(vfs_ctx *c)
         // Rhs = 0; Formfunction_2(Rhs, 1); Rhs -= dP; Ucont = a*Rhs + pUcont  (rhs.c:1647-1658)
         rc = vfs::formfunction2(c, 1.0, c->pucont, alfa[s] * c->p.dt * c->p.st, c->f[VFS_UCONT]);
         if (rc) return rc;
-        rc = vfs::c2c(c);
+        if (s < 3 || !lazy) rc = vfs::c2c(c);
         if (rc) return rc;
     }
-    return vfs::form_bcs(c, 1);
+    return vfs::form_bcs(c, 1, lazy);
 }
+
+extern "C" {
 
 int vfs_inflow_set_plane(vfs_ctx *c, const double *host)
 {
@@ -564,7 +576,9 @@ static int step_common(vfs_ctx *c, const vfs_solve_opts *o, const vfs_newton_opt
             rc = vfs_store_old_state(c);
             if (!rc) rc = vfs_implicit_matrixfree(c, no, &ns);
         } else {
-            rc = vfs_runge_kutta(c);                        // solvers.c:635
+            // Ucat after the stages is recomputed by the projection's
+            // Contra2Cart before anything reads it
+            rc = vfs::runge_kutta(c, true);                 // solvers.c:635
         }
     }
     if (rc) return rc;

@@ -272,7 +272,13 @@ enum { CS_PARAM = 0, CS_ZERO = 1, CS_FIELD = 2 };
 // current lCs into VFS_CS when it is a constant
 int smagorinsky_constant(vfs_ctx *c, int ti, int tistart, int rstart);
 int materialize_cs(vfs_ctx *c);
-int form_bcs(vfs_ctx *c, int outflow_scale);
+// FormBCS; lazy_ucat leaves Ucat to the next Contra2Cart where no boundary
+// code reads it (form_bcs_reads_ucat)
+int form_bcs(vfs_ctx *c, int outflow_scale, bool lazy_ucat = false);
+bool form_bcs_reads_ucat(const vfs_ctx *c);
+// RungeKutta; lazy_ucat: the last stage's Contra2Cart and FormBCS's are left
+// to the next Contra2Cart (the fractional step's projection recomputes Ucat)
+int runge_kutta(vfs_ctx *c, bool lazy_ucat);
 int inflow_flux(vfs_ctx *c, double *flux_in);
 int pseudo_periodic_plane(vfs_ctx *c);
 int eddy_viscosity(vfs_ctx *c);

@@ -945,11 +945,23 @@ static int form_bcs_kminus_outlet(vfs_ctx *c)
     return VFS_OK;
 }
 
-int form_bcs(vfs_ctx *c, int outflow_scale)
+bool form_bcs_reads_ucat(const vfs_ctx *c)
+{
+    const Grid &g = c->g;
+    bool r = g.bc[5] == 8 || g.bc[4] == 4;  // characteristic / k- outlets (k_kplane_sums, kminus)
+    for (int f = 0; f < 6; f++) r = r || g.bc[f] == 6;  // far field
+    return r;
+}
+
+int form_bcs(vfs_ctx *c, int outflow_scale, bool lazy_ucat)
 {
     const Grid &g = c->g;
     hipStream_t st = c->stream;
-    int rc = c2c(c);
+    // lazy_ucat: Ucat is left to the next Contra2Cart (the caller's next use
+    // recomputes it), so the two Contra2Cart calls are skipped; only where no
+    // boundary code here reads Ucat
+    const bool lazy = lazy_ucat && !form_bcs_reads_ucat(c);
+    int rc = lazy ? VFS_OK : c2c(c);
     if (rc) return rc;
     rc = form_bcs_open(c);  // bcs.c:1745-2041
     if (rc) return rc;
@@ -976,7 +988,7 @@ int form_bcs(vfs_ctx *c, int outflow_scale)
     hipLaunchKernelGGL(k_wall_flux, dim3(blocks_for(nw, TPB)), dim3(TPB), 0, st, g, c->f[VFS_UCONT]);
     ucat_walls_written(c);
     VFS_LAUNCH_CHECK("wall flux");
-    rc = c2c(c);
+    rc = lazy ? VFS_OK : c2c(c);
     if (rc) return rc;
     int ne = g.mx > g.my ? g.mx : g.my;
     if (g.mz > ne) ne = g.mz;
