@@ -437,6 +437,7 @@ static int gmres(vfs_ctx *c, ImpState &s, double rtol, int maxit, int use_feul, 
         VFS_LAUNCH_CHECK("gmres update");
         if (std::fabs(gv[jj]) <= rtol * bnorm) break;
     }
+    if (restart0) VFS_HIP(hipMemsetAsync(s.dx, 0, n * sizeof(double), st));  // no update was made
     return VFS_OK;
 }
 
