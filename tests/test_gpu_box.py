@@ -94,18 +94,19 @@ def test_box_cn_with_rotor():
     ctx.close()
 
 
+@pytest.mark.parametrize("path", ["short", "long"])
 @pytest.mark.parametrize("cells", [(24, 20, 32), (23, 20, 31)], ids=["24x20x32", "odd-23x20x31"])
-def test_box_poisson_one_pass_forms_bitwise(monkeypatch, cells):
+def test_box_poisson_one_pass_forms_bitwise(monkeypatch, cells, path):
     """On the box (no periodic direction, so the compact smoother's i and k
     neighbours at the walls are ghosts; with an odd i count one colour has a
     line fewer per row): the FP32 V-cycle with the finest level in the
     colour-split layout, in the natural layout (VFS_MG_SPLIT=0), and the
     two-pass natural form (VFS_ZEBRA_FUSED=0, VFS_MG_FUSED_RESTRICT=0) give
-    the same solve, bit for bit."""
+    the same solve, bit for bit (each with the same line segments)."""
     out = []
     variants = ({}, {"VFS_MG_SPLIT": "0"}, {"VFS_ZEBRA_FUSED": "0", "VFS_MG_FUSED_RESTRICT": "0"})
     for env in variants:
-        env = {"VFS_SMOOTH_COMPACT": "1", **env}
+        env = {"VFS_SMOOTH_COMPACT": "1", "VFS_LINE_PATH": path, **env}
         for k, v in env.items():
             monkeypatch.setenv(k, v)
         c, o, ctx = _pair(*cells)

@@ -802,17 +802,19 @@ __host__ __device__ inline bool zebra_fused_ok(const Grid &g)
     return (!g.pi || ((g.mx - 2) & 1) == 0) && (!g.pk || ((g.mz - 2) & 1) == 0);
 }
 
-template <int SLM, bool XZ, bool SPL>
-__global__ void __launch_bounds__(LINE_LINES * LINE_MAXSEG) k_zebra_line_c4(Grid g, ZGrid z, int c, int nseg, int sl,
+// LL lines per block (32, or 64 with at most 16 segments per line)
+template <int SLM, bool XZ, bool SPL, int LL>
+__global__ void __launch_bounds__(1024) k_zebra_line_c4(Grid g, ZGrid z, int c, int nseg, int sl,
                                                                            const h4_t *__restrict__ sh,
                                                                            const float *__restrict__ sd,
                                                                            const float *__restrict__ b,
                                                                            const float *__restrict__ sf, float *x)
 {
-    __shared__ float sA[LINE_MAXSEG * LINE_LINES], sB[LINE_MAXSEG * LINE_LINES];
-    const int ln = threadIdx.x % LINE_LINES, seg = threadIdx.x / LINE_LINES;
+    static_assert(LL * (LL == 64 ? 16 : LINE_MAXSEG) <= 1024, "threads per block");
+    __shared__ float sA[1024], sB[1024];
+    const int ln = threadIdx.x % LL, seg = threadIdx.x / LL;
     const long long nlines = (long long)z.nkI * z.nh;
-    const long long lid = (long long)xcd_block1() * LINE_LINES + ln;
+    const long long lid = (long long)xcd_block1() * LL + ln;
     int kk = 0, h = 0, k = 1, i = 1;
     bool act = lid < nlines;
     if (act) {
@@ -873,12 +875,12 @@ __global__ void __launch_bounds__(LINE_LINES * LINE_MAXSEG) k_zebra_line_c4(Grid
             B = (r[v] - l[v] * B) * m[v];
             A = (-l[v] * A) * m[v];
         }
-    const int me = seg * LINE_LINES + ln;
+    const int me = seg * LL + ln;
     sA[me] = A;
     sB[me] = B;
     __syncthreads();
     float d = 0;
-    for (int s = 0; s < seg; s++) d = sA[s * LINE_LINES + ln] * d + sB[s * LINE_LINES + ln];
+    for (int s = 0; s < seg; s++) d = sA[s * LL + ln] * d + sB[s * LL + ln];
 #pragma unroll
     for (int v = 0; v < SLM; v++)
         if (j0 + v < j1) {
@@ -899,7 +901,7 @@ __global__ void __launch_bounds__(LINE_LINES * LINE_MAXSEG) k_zebra_line_c4(Grid
     sB[me] = B;
     __syncthreads();
     float xl = 0;
-    for (int s = nseg - 1; s > seg; s--) xl = sA[s * LINE_LINES + ln] * xl + sB[s * LINE_LINES + ln];
+    for (int s = nseg - 1; s > seg; s--) xl = sA[s * LL + ln] * xl + sB[s * LL + ln];
     if (act) {
 #pragma unroll
         for (int v = SLM - 1; v >= 0; v--) {
@@ -1625,6 +1627,7 @@ struct LinePlan {
     bool short_seg, scan, c4, fused;
     bool split;  // the finest level on the one-pass path: x and b in the S layout
     int nseg, sl;
+    int lines;   // lines per block of the one-pass kernel (32 or 64)
 };
 template <typename T>
 static LinePlan line_plan(const PLevel &L)
@@ -1637,6 +1640,12 @@ static LinePlan line_plan(const PLevel &L)
     const char *lp = std::getenv("VFS_LINE_PATH");
     const int force = !lp ? 0 : lp[0] == 's' && lp[1] == 'h' ? 1 : lp[0] == 'l' ? 2 : lp[0] == 's' ? 3 : 0;
     p.short_seg = sizeof(T) == 4 && L.z.njI <= LINE_SLM_SHORT * LINE_MAXSEG;
+    const char *fz = std::getenv("VFS_ZEBRA_FUSED");
+    const bool fusable = sizeof(T) == 4 && L.compact && zebra_fused_ok(L.g) && !(fz && fz[0] == '0');
+    // on the finest level the one-pass kernel takes 16-point segments and 64
+    // lines per block up to 256-point lines (a segment row is a whole
+    // wavefront; the coarse levels, which fill fewer blocks, keep 8 and 32)
+    if (fusable && L.finest && L.z.njI <= line_slm<float>() * 16) p.short_seg = false;
     if (force == 1) p.short_seg = sizeof(T) == 4;
     if (force == 2) p.short_seg = false;
     const int slm = p.short_seg ? LINE_SLM_SHORT : SLM;
@@ -1646,8 +1655,8 @@ static LinePlan line_plan(const PLevel &L)
     p.c4 = sizeof(T) == 4 && L.compact;
     // the compact form's right-hand sides are formed inside the line solve
     // (k_zebra_line_c4) unless VFS_ZEBRA_FUSED=0 asks for the two-pass form
-    const char *fz = std::getenv("VFS_ZEBRA_FUSED");
-    p.fused = p.c4 && p.scan && zebra_fused_ok(L.g) && !(fz && fz[0] == '0');
+    p.fused = fusable && p.scan;
+    p.lines = p.fused && !p.short_seg && p.nseg <= 16 ? 64 : 32;
     // VFS_MG_SPLIT=0 keeps the finest level's vectors in the natural layout
     // (as does the stored-residual restriction, VFS_MG_FUSED_RESTRICT=0)
     const char *fs = std::getenv("VFS_MG_SPLIT"), *fr = std::getenv("VFS_MG_FUSED_RESTRICT");
@@ -1677,11 +1686,16 @@ static void smooth(vfs_ctx *c, PLevel &L, int sweeps, bool x_zero = false)
                 const bool xz = x_zero && s == 0 && col == 0;
                 if (fine) prof_begin(c, K_LRELAXC32);
                 constexpr int SL = LINE_SLM_SHORT, LG = line_slm<float>();
-                auto k = P.split ? (short_seg ? (xz ? k_zebra_line_c4<SL, true, true> : k_zebra_line_c4<SL, false, true>)
-                                              : (xz ? k_zebra_line_c4<LG, true, true> : k_zebra_line_c4<LG, false, true>))
-                                 : (short_seg ? (xz ? k_zebra_line_c4<SL, true, false> : k_zebra_line_c4<SL, false, false>)
-                                              : (xz ? k_zebra_line_c4<LG, true, false> : k_zebra_line_c4<LG, false, false>));
-                hipLaunchKernelGGL(k, dim3(scan_blocks), dim3(LINE_LINES * nseg), 0, st, L.g, L.z, col, nseg, sl, sh4, L.sd32,
+                const bool w64 = P.lines == 64;  // (long segments only)
+                auto k = P.split
+                             ? (short_seg ? (xz ? k_zebra_line_c4<SL, true, true, 32> : k_zebra_line_c4<SL, false, true, 32>)
+                                : w64     ? (xz ? k_zebra_line_c4<LG, true, true, 64> : k_zebra_line_c4<LG, false, true, 64>)
+                                          : (xz ? k_zebra_line_c4<LG, true, true, 32> : k_zebra_line_c4<LG, false, true, 32>))
+                             : (short_seg ? (xz ? k_zebra_line_c4<SL, true, false, 32> : k_zebra_line_c4<SL, false, false, 32>)
+                                : w64     ? (xz ? k_zebra_line_c4<LG, true, false, 64> : k_zebra_line_c4<LG, false, false, 64>)
+                                          : (xz ? k_zebra_line_c4<LG, true, false, 32> : k_zebra_line_c4<LG, false, false, 32>));
+                const unsigned blocks = (unsigned)((lines + P.lines - 1) / P.lines);
+                hipLaunchKernelGGL(k, dim3(blocks), dim3(P.lines * nseg), 0, st, L.g, L.z, col, nseg, sl, sh4, L.sd32,
                                    P.split ? L.bs32 : L.b32, L.sc32, P.split ? L.xs32 : L.x32);
                 if (fine) prof_end(c);
                 continue;
