@@ -74,8 +74,8 @@ ALGO = {
 TRAFFIC_SOURCE = ("not measured by this run: rocprofv3 --pmc FETCH_SIZE (x2.0) + WRITE_SIZE of the same kernel on "
                   "this workload in a separate run, profiles/hbm_traffic_channel_256x192x256_pmc.md")
 MEASURED_TRAFFIC = {
-    "formfunction2": 4.148e9,
-    "face_fluxes": 2.371e9,
+    "formfunction2": 4.200e9,
+    "face_fluxes": 2.423e9,
     "cell_rhs": 1.723e9,
     "line_rhs_fp32": 0.382e9,
     "line_rhs_compact_fp32": 0.208e9,
