@@ -1776,7 +1776,11 @@ static void coarse_subcycle(vfs_ctx *c, vfs_poisson *ps, const Sweeps &S)
         if (e != hipSuccess) { sg.exec = nullptr; direct(); return; }
         sg.key = S;
     }
-    hipGraphLaunch(sg.exec, st);
+    // a replay that cannot be enqueued runs the same kernels directly
+    if (hipGraphLaunch(sg.exec, st) != hipSuccess) {
+        (void)hipGetLastError();
+        direct();
+    }
 }
 
 template <typename T>
